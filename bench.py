@@ -1,0 +1,230 @@
+#!/usr/bin/env python
+"""Benchmark: ~1M-DOF HEX8 elasticity assemble + Jacobi-PCG on one MI355X.
+
+Workload (default): the 200x40x40 cantilever, spacing 1, 320,000 HEX8
+elements, 1,013,643 DOFs; E = 70e3 (node var), nu = 0.3; x = 0 clamped;
+traction 1e-3 e_z on x = 200; Jacobi-PCG to relative residual 1e-8.
+One step = assemble the BC-eliminated CSR Jacobian + right-hand side +
+Jacobi-PCG solve, structure and parameters already resident in HBM
+(the timing scope of examples/benchmark/bench_linear_elasticity.py:240-250
+after TracedStructure.from_problem).  Forms timed: "fp64" (FP64 assembly,
+SpMV, CG) and "mixed" (FP32 difference-form matrix, FP64 vectors, FP64
+defect correction).
+
+Prints one JSON line (see the fields below); --gpus N > 1 is refused: this
+build runs on one GPU.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+
+CONFIGS = {
+    "c2": dict(dims=(200, 40, 40), spacing=1.0),
+    "c1": dict(dims=(50, 5, 5), spacing=2.0),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build(cfg):
+    import numpy as np
+    import feax_amd as fe
+
+    class LinearElasticity(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.IsotropicLinearElastic(nu=0.3)
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.0, 0.0, 1.0))]
+
+    nx, ny, nz = cfg["dims"]
+    h = cfg["spacing"]
+    L = nx * h
+    mesh = fe.StructuredGrid((nx, ny, nz), spacing=(h, h, h)).to_mesh()
+    problem = LinearElasticity(mesh, vec=3, dim=3, location_fns=[lambda p: np.isclose(p[0], L)])
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[0], 0.0), "all", 0.0)]).create_bc(problem)
+    E = fe.TracedParams.create_node_var(problem, 70e3)
+    t = fe.TracedParams.create_uniform_surface_var(problem, 1e-3)
+    import torch
+    tp = fe.TracedParams(volume_vars=(torch.as_tensor(E, device="cuda"),),
+                         surface_vars=[(torch.as_tensor(t, device="cuda"),)])
+    ts = fe.TracedStructure.from_problem(problem)
+    return problem, bc, tp, ts
+
+
+def event_time(fn, reps):
+    """Average seconds per call, HIP events on torch's current stream (the
+    stream the kernels are launched on)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) * 1e-3 / reps
+
+
+def cpu_baseline(problem, bc, A64, pcg_iters, budget_s=20.0):
+    """Oracle (numpy/scipy FP64 port) on the host: element-matrix assembly on
+    a sample of elements and Jacobi-PCG iterations on the same CSR, scaled
+    to the full workload (assembly x E/sample, PCG x the GPU iteration
+    count)."""
+    import numpy as np
+    from oracle import hex8, krylov
+    mat = hex8.Material("linear_elastic", nu=0.3)
+    E = problem.num_cells
+    ns = min(E, 20000)
+    pts, cells = problem.points, problem.cells[:ns].astype(np.int64)
+    grads, JxW = hex8.geometry(pts, cells)
+    pq = hex8.param_at_quads(np.full(len(pts), 70e3), cells, len(pts))
+    ta = time.perf_counter()
+    hex8.element_stiffness(np.zeros((ns, 8, 3)), grads, JxW, pq, mat)
+    t_asm_sample = time.perf_counter() - ta  # element matrices only (geometry excluded)
+    K = A64.to_scipy()
+    n = K.shape[0]
+    rng = np.random.default_rng(20261203)
+    b = rng.standard_normal(n)
+    dinv = krylov.jacobi_dinv(K.diagonal())
+    it = 3
+    while True:
+        tp0 = time.perf_counter()
+        krylov.pcg(K, b, None, 0.0, 0.0, it, dinv)
+        dt = time.perf_counter() - tp0
+        if dt > 0.25 * budget_s or it >= 400:
+            break
+        it *= 2
+    per_it = dt / it
+    total = t_asm_sample * E / ns + per_it * pcg_iters
+    return dict(value=total, unit="s", cores=1, kind="port",
+                sample=(f"oracle element matrices on {ns} of {E} elements (x{E / ns:.1f}) + {it} scipy "
+                        f"Jacobi-PCG iterations on the {n}-DOF CSR (x{pcg_iters}/{it}); "
+                        f"host affinity {len(os.sched_getaffinity(0))} cpus, OMP_NUM_THREADS="
+                        f"{os.environ.get('OMP_NUM_THREADS')}"),
+                asm_s=t_asm_sample * E / ns, pcg_s_per_iter=per_it)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is not built.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import assemble_csr, residual
+    from feax_amd.solvers.common import device_norm
+
+    cfg = CONFIGS[args.config]
+    t0 = time.perf_counter()
+    problem, bc, tp, ts = build(cfg)
+    torch.cuda.synchronize()
+    log(f"setup {time.perf_counter() - t0:.2f} s: n={ts.n} nse={ts.nse} E={ts.num_cells}")
+    n, nse, Ecount, N = ts.n, ts.nse, ts.num_cells, ts.num_nodes
+    u0 = fe.zero_like_initial_guess(problem, bc, device="cuda")
+
+    def solver_for(precision):
+        opts = fe.KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=20000,
+                                      use_jacobi_preconditioner=True, precision=precision)
+        return fe.create_solver(problem, bc, opts, linear=True, traced_params=tp, traced_structure=ts,
+                                return_solution=False)
+
+    results = {}
+    b0 = residual(problem, ts, tp, u0, bc, negate=True)
+    bnorm = device_norm(b0)
+    for form in ("fp64", "mixed"):
+        solver = solver_for(form)
+        for _ in range(args.warmup):
+            solver(tp, u0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            u = solver(tp, u0)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t1) / args.steps
+        r = residual(problem, ts, tp, u, bc, negate=True)
+        results[form] = dict(s=el, iters=solver.stats.iters, outer=solver.stats.outer,
+                             inner=solver.stats.inner_iters, relres=device_norm(r) / bnorm, u=u)
+        log(f"{form}: {el:.4f} s/step  iters={solver.stats.iters} outer={solver.stats.outer} "
+            f"true relres={results[form]['relres']:.3e}")
+    du = results["mixed"]["u"] - results["fp64"]["u"]
+    mixed_vs_fp64 = float(du.norm() / results["fp64"]["u"].norm())
+
+    # Dominant kernel: the SpMV inside the PCG iterations.  Standalone launches
+    # of the same kernels, timed with HIP events.
+    A64 = assemble_csr(problem, ts, tp, None, bc, True, torch.float64)
+    Ad, _ = assemble_csr(problem, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12)
+    x64 = torch.randn(n, dtype=torch.float64, device="cuda")
+    y64 = torch.empty_like(x64)
+    t_spmv64 = event_time(lambda: A64.matvec(x64, out=y64), 50)
+    t_diff = event_time(lambda: Ad.matvec(x64, out=y64), 50)
+    # Algorithmic bytes per launch: nse (s_v + 4) + 4 (n + 1) + 2 n s_x.
+    B64 = nse * (8 + 4) + 4 * (n + 1) + 2 * n * 8
+    Bd = nse * (4 + 4) + 4 * (n + 1) + 2 * n * 8
+    t_asm64 = event_time(lambda: assemble_csr(problem, ts, tp, None, bc, True, torch.float64), 5)
+
+    headline = "mixed"
+    dom = dict(kernel="pcg SpMV, FP32 difference form (fx_csr_spmv_diff3_f32f64)", t=t_diff, B=Bd)
+    if results["fp64"]["s"] < results["mixed"]["s"]:
+        headline = "fp64"
+        dom = dict(kernel="pcg SpMV, FP64 block-3 (fx_csr_spmv_b3_f64)", t=t_spmv64, B=B64)
+    achieved = dom["B"] / dom["t"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(problem, bc, A64, results["fp64"]["iters"])
+        log(f"cpu baseline: {cpu['value']:.1f} s ({cpu['sample']})")
+
+    line = {
+        "metric": "1M-DOF hex8 assemble+CG s (mixed vs FP64), SpMV HBM GB/s, grad/solve ratio",
+        "value": round(results[headline]["s"], 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(results[headline]["s"] * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64" if headline == "fp64" else "f32-matrix/f64-vectors",
+        "data": "synthetic",
+        "config": {"workload": f"hex8 cantilever {cfg['dims'][0]}x{cfg['dims'][1]}x{cfg['dims'][2]} "
+                               f"h={cfg['spacing']}, {n} DOF, nse={nse}, Jacobi-PCG rel 1e-8",
+                   "form": headline},
+        "forms": {k: {"s": round(v["s"], 6), "iters": v["iters"], "outer": v["outer"],
+                      "true_relres": v["relres"]} for k, v in results.items()},
+        "mixed_vs_fp64_rel_diff": mixed_vs_fp64,
+        "spmv": {"fp64_b3_us": round(t_spmv64 * 1e6, 2), "fp64_alg_GBps": round(B64 / t_spmv64 / 1e9, 1),
+                 "diff3_us": round(t_diff * 1e6, 2), "diff3_alg_GBps": round(Bd / t_diff / 1e9, 1)},
+        "assemble_fp64_ms": round(t_asm64 * 1e3, 3),
+        "grad_solve_ratio": None,
+        "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(achieved / 1e9, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": None},
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

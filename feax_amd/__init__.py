@@ -1,0 +1,23 @@
+"""feax_amd: MI355X-native HEX8 assembly + Krylov path with the feax API.
+
+Drop-in for the KrylovSolverOptions route of feax.create_solver: Problem,
+DirichletBCSpec / DirichletBCConfig, TracedParams, TracedStructure,
+KrylovSolverOptions, create_solver, Solution, zero_like_initial_guess,
+CSRMatrix.  All compute runs in libfeaxhip.so (hand-written HIP for
+gfx950); the package itself holds only host-side structure and dispatch.
+"""
+
+from . import materials
+from .assembler import (TracedStructure, create_J_bc_csr_parametric, create_res_bc_parametric,
+                        create_res_J_bc_csr_parametric, get_res)
+from .csr import CSRMatrix
+from .DCboundary import DirichletBC, DirichletBCConfig, DirichletBCSpec, dirichlet_bc_config
+from .mesh import Mesh, StructuredGrid
+from .problem import MatrixView, Problem
+from .solution import Solution
+from .solver import create_solver
+from .solvers.options import KrylovSolverOptions, MatrixProperty, NewtonOptions
+from .traced_params import TracedParams
+from .utils import zero_like_initial_guess
+
+__version__ = "0.1.0"

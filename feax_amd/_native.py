@@ -1,0 +1,130 @@
+"""ctypes binding of libfeaxhip.so (the C ABI declared in include/feaxhip.h).
+
+The library is the only compute backend.  If it is missing, importing the
+compute entry points raises immediately: there is no CPU fallback.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libfeaxhip.so")
+
+FX_MAT_LINEAR_ELASTIC = 0
+FX_MAT_SIMP = 1
+FX_MAT_NEO_HOOKEAN = 2
+FX_PARAM_NODE = 0
+FX_PARAM_CELL = 1
+FX_PCG_BLOCK3 = 1
+FX_PCG_DIFF3 = 2
+FX_ASM_DIFF3 = 1
+FX_ASM_COMPUTE32 = 2
+
+
+class FxMaterial(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("param_loc", ctypes.c_int32), ("nu", ctypes.c_double),
+                ("E0", ctypes.c_double), ("Emin", ctypes.c_double), ("penal", ctypes.c_double),
+                ("kappa", ctypes.c_double)]
+
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_F64 = ctypes.c_double
+_SZ = ctypes.c_size_t
+
+# name -> (restype, argtypes); every symbol include/feaxhip.h declares.
+SIGNATURES = {
+    "fx_last_error": (_I32, [ctypes.c_char_p, _SZ]),
+    "fx_version": (_I32, [ctypes.c_char_p, _SZ]),
+    "fx_hex8_pattern": (_I32, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "fx_hex8_face_incidence": (_I32, [_I32, _I32, _P, _I32, _P, _P, _P]),
+    "fx_csr_is_block3": (_I32, [_I32, _P, _P, _P]),
+    "fx_csr_spmv_f64": (_I32, [_P, _I32, _I64, _P, _P, _P, _P, _P]),
+    "fx_csr_spmv_f32": (_I32, [_P, _I32, _I64, _P, _P, _P, _P, _P]),
+    "fx_csr_block3_index": (_I32, [_P, _I32, _P, _P, _P]),
+    "fx_csr_spmv_b3_f64": (_I32, [_P, _I32, _I64, _P, _P, _P, _P, _P]),
+    "fx_csr_spmv_b3_f32": (_I32, [_P, _I32, _I64, _P, _P, _P, _P, _P]),
+    "fx_csr_spmv_diff3_f32f64": (_I32, [_P, _I32, _I64, _P, _P, _P, _P, _P]),
+    "fx_csr_jacobi_f64": (_I32, [_P, _I32, _P, _P, _P, _F64, _P]),
+    "fx_csr_jacobi_f32": (_I32, [_P, _I32, _P, _P, _P, _F64, _P]),
+    "fx_csr_diagonal_f64": (_I32, [_P, _I32, _P, _P, _P, _P]),
+    "fx_csr_diagonal_f32": (_I32, [_P, _I32, _P, _P, _P, _P]),
+    "fx_pcg_workspace_bytes": (_SZ, [_I32, _I32]),
+    "fx_pcg_f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P,
+                          _P, _P, _SZ]),
+    "fx_pcg_f32": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P,
+                          _P, _P, _SZ]),
+    "fx_pcg_f32f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P,
+                             _P, _P, _P, _SZ]),
+    "fx_dot_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _SZ]),
+    "fx_axpy_f32_into_f64": (_I32, [_P, _I32, _F64, _P, _P]),
+    "fx_convert_f64_to_f32": (_I32, [_P, _I32, _F64, _P, _P]),
+    "fx_axpby_f64": (_I32, [_P, _I32, _F64, _P, _F64, _P, _P]),
+    "fx_bc_lift_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P]),
+    "fx_hex8_assemble_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                    _I32, _I32, _F64, _P, _P]),
+    "fx_hex8_assemble_f32": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                    _I32, _I32, _F64, _P, _P]),
+    "fx_hex8_residual_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P,
+                                    _P, _P, _I32, _P, _P, _I32, _P]),
+}
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libfeaxhip.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeError(
+                f"{LIB_PATH} is missing: build it with `python -m feax_amd._build` "
+                "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(512)
+    lib().fx_last_error(buf, 512)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc, what=""):
+    if rc != 0:
+        raise NativeError(f"{what} failed with code {rc}: {last_error()}")
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args), name)
+
+
+def ptr(t):
+    """Raw pointer of a torch tensor / numpy array / None."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data
+    return t.data_ptr()
+
+
+def stream():
+    """hipStream_t of torch's current stream on the current device."""
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def material_struct(kind, param_loc, nu=0.3, E0=1.0, Emin=1e-9, penal=3.0, kappa=0.0):
+    return FxMaterial(int(kind), int(param_loc), float(nu), float(E0), float(Emin), float(penal), float(kappa))

@@ -1,0 +1,225 @@
+"""Device assembly: TracedStructure, CSR Jacobian and residual (feax.assembler API).
+
+feax entry points mirrored (feax/assembler.py):
+  get_res                         1618-1667  -> residual kernel, list of (N, 3)
+  create_J_bc_csr_parametric      1695-1713  -> fn(sol, tp, bc, ts) -> CSRMatrix
+  create_res_bc_parametric        1833-1858  -> fn(sol, tp, bc, ts) -> res_bc
+  create_res_J_bc_csr_parametric  1716-1737  -> fn(sol, tp, bc, ts) -> (res_bc, J)
+TracedStructure mirrors feax/traced_structure.py:45-164: the mesh-sized
+structure arrays, resident on the device, passed to every call.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _native as nat
+from .csr import CSRMatrix
+from .materials import SurfaceTraction, TensorMap
+
+
+class TracedStructure:
+    """Device-resident structure of a Problem (SoA coordinates, connectivity,
+    node->element incidence, CSR pattern, loaded faces)."""
+
+    def __init__(self, problem, device="cuda"):
+        import torch
+        dev = torch.device(device)
+        f64 = dict(dtype=torch.float64, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        P = problem.points
+        self.device = dev
+        self.num_nodes = problem.num_total_nodes
+        self.num_cells = problem.num_cells
+        self.n = problem.num_total_dofs_all_vars
+        self.nse = problem.csr_nse
+        self.x = torch.as_tensor(np.ascontiguousarray(P[:, 0]), **f64)
+        self.y = torch.as_tensor(np.ascontiguousarray(P[:, 1]), **f64)
+        self.z = torch.as_tensor(np.ascontiguousarray(P[:, 2]), **f64)
+        self.cells = torch.as_tensor(problem.cells.reshape(-1), **i32)
+        self.inc_ptr = torch.as_tensor(problem.node_inc_ptr, **i32)
+        self.inc = torch.as_tensor(problem.node_inc, **i32)
+        self.rank = torch.as_tensor(problem.elem_rank, dtype=torch.uint8, device=dev)
+        self.indptr = torch.as_tensor(problem.csr_indptr, **i32)
+        self.indices = torch.as_tensor(problem.csr_indices, **i32)
+        self.num_faces = len(problem.all_faces)
+        self.faces = torch.as_tensor(problem.all_faces.reshape(-1), **i32) if self.num_faces else None
+        self.face_inc_ptr = torch.as_tensor(problem.face_inc_ptr, **i32)
+        self.face_inc = torch.as_tensor(problem.face_inc, **i32)
+        self.face_offsets = problem.face_offsets
+        # Per-face traction direction from the problem's surface maps.
+        maps = problem.get_surface_maps() or []
+        fdir = np.zeros((self.num_faces, 3))
+        for i in range(len(problem.boundary_inds_list)):
+            if i < len(maps):
+                m = maps[i]
+                if not isinstance(m, SurfaceTraction):
+                    raise NotImplementedError(
+                        "surface maps on the HIP path are feax_amd.materials.SurfaceTraction (dead loads)")
+                fdir[self.face_offsets[i]:self.face_offsets[i + 1]] = m.direction
+        self.fdir = torch.as_tensor(fdir.reshape(-1), **f64)
+        self._cache = {}
+        self._bcol = None
+
+    def bcol(self):
+        """Column node of each 3x3 block of the CSR pattern (nse/9 int32)."""
+        import torch
+        if self._bcol is None:
+            b = torch.empty(self.nse // 9, dtype=torch.int32, device=self.device)
+            nat.call("fx_csr_block3_index", nat.stream(), self.n, nat.ptr(self.indptr), nat.ptr(self.indices),
+                     nat.ptr(b))
+            self._bcol = b
+        return self._bcol
+
+    @classmethod
+    def from_problem(cls, problem, device="cuda", free_scratch=False):
+        return cls(problem, device)
+
+
+def _as_dev(a, ts, dtype=None):
+    import torch
+    dtype = torch.float64 if dtype is None else dtype
+    if isinstance(a, torch.Tensor):
+        return a.to(device=ts.device, dtype=dtype).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=ts.device)
+
+
+def resolve_material(problem, traced_params, ts):
+    """(fx_material, param tensor) for the element kernels.
+
+    The parameter is volume_vars[0], classified by its global shape like
+    feax/assembler.py:69-103 (node var (N,) or cell var (E,)); a tensor map
+    with a fixed modulus and no volume vars gets a constant cell var.
+    """
+    import torch
+    tm = problem.get_tensor_map()
+    if tm is None:
+        tm = problem.get_energy_density()
+    if not isinstance(tm, TensorMap):
+        raise NotImplementedError(
+            "the HIP element kernels need get_tensor_map()/get_energy_density() to return a "
+            "feax_amd.materials law (IsotropicLinearElastic, SIMPLinearElastic, CompressibleNeoHookean)")
+    if problem.get_mass_map() is not None:
+        raise NotImplementedError("mass maps are not on the HIP path")
+    vv = traced_params.volume_vars if traced_params is not None else ()
+    if len(vv):
+        p = _as_dev(vv[0], ts)
+        if p.ndim == 1 and p.shape[0] == ts.num_nodes:
+            loc = nat.FX_PARAM_NODE
+        elif p.ndim == 1 and p.shape[0] == ts.num_cells:
+            loc = nat.FX_PARAM_CELL
+        else:
+            raise NotImplementedError(f"volume var of shape {tuple(p.shape)} (node (N,) or cell (E,) only)")
+    else:
+        E = getattr(tm, "E", None)
+        if E is None:
+            raise ValueError("tensor map needs its modulus from TracedParams.volume_vars[0]")
+        key = ("constE", E)
+        if key not in ts._cache:
+            ts._cache[key] = torch.full((ts.num_cells,), float(E), dtype=torch.float64, device=ts.device)
+        p, loc = ts._cache[key], nat.FX_PARAM_CELL
+    return tm.material(loc), p
+
+
+def surface_values(problem, traced_params, ts):
+    """(F*4,) float64: surface_vars[i][0] of every loaded boundary, concatenated."""
+    import torch
+    if ts.num_faces == 0:
+        return None
+    sv = traced_params.surface_vars if traced_params is not None else []
+    parts = []
+    for i in range(len(problem.boundary_inds_list)):
+        nf = int(ts.face_offsets[i + 1] - ts.face_offsets[i])
+        if i < len(sv) and len(sv[i]):
+            t = _as_dev(sv[i][0], ts)
+            if t.numel() == nf:
+                t = t.reshape(nf, 1).expand(nf, 4)
+            parts.append(t.reshape(nf, 4))
+        else:
+            parts.append(torch.zeros((nf, 4), dtype=torch.float64, device=ts.device))
+    return torch.cat(parts).reshape(-1).contiguous()
+
+
+def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, dtype=None, diff_form=False,
+                 jacobi_shift=None):
+    """BC-eliminated CSR Jacobian on the device (fx_hex8_assemble_*).
+
+    diff_form: FP32 difference form for the mixed solver.  jacobi_shift:
+    also return dinv = 1 / (diag + shift) (FP64) computed in the same pass.
+    Returns the CSRMatrix, or (CSRMatrix, dinv) when jacobi_shift is given.
+    """
+    import torch
+    dtype = torch.float64 if dtype is None else dtype
+    mat, p = resolve_material(problem, traced_params, ts)
+    data = torch.empty(ts.nse, dtype=dtype, device=ts.device)
+    mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
+    u = _as_dev(sol, ts) if sol is not None else None
+    dinv = torch.empty(ts.n, dtype=torch.float64, device=ts.device) if jacobi_shift is not None else None
+    name = "fx_hex8_assemble_f64" if dtype == torch.float64 else "fx_hex8_assemble_f32"
+    nat.call(name, nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
+             nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), nat.ptr(ts.rank), nat.ptr(ts.indptr),
+             nat.ptr(ts.indices), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u), nat.ptr(mask), int(bool(symmetric)),
+             nat.FX_ASM_DIFF3 if diff_form else 0, float(jacobi_shift or 0.0), nat.ptr(dinv), nat.ptr(data))
+    A = CSRMatrix(data, ts.indptr, ts.indices, (ts.n, ts.n), block3=True, form="diff3" if diff_form else "csr",
+                  bcol=ts.bcol())
+    return A if jacobi_shift is None else (A, dinv)
+
+
+def residual(problem, ts, traced_params, sol, bc=None, negate=False, out=None):
+    """R(u) (or -R_bc(u) with negate) as a flat float64 device vector."""
+    import torch
+    mat, p = resolve_material(problem, traced_params, ts)
+    u = _as_dev(sol, ts)
+    res = torch.empty(ts.n, dtype=torch.float64, device=ts.device) if out is None else out
+    sv = surface_values(problem, traced_params, ts)
+    if bc is not None and len(bc.bc_rows):
+        rows, _, vals = bc.device(ts.device)
+        nbc = len(bc.bc_rows)
+    else:
+        rows = vals = None
+        nbc = 0
+    nf = ts.num_faces
+    nat.call("fx_hex8_residual_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
+             nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u),
+             nf, nat.ptr(ts.faces) if nf else None, nat.ptr(ts.face_inc_ptr) if nf else None,
+             nat.ptr(ts.face_inc) if nf else None, nat.ptr(sv), nat.ptr(ts.fdir) if nf else None, nbc,
+             nat.ptr(rows), nat.ptr(vals), int(bool(negate)), nat.ptr(res))
+    return res
+
+
+def _ts_or_build(problem, ts):
+    if ts is not None:
+        return ts
+    cached = getattr(problem, "_default_ts", None)
+    if cached is None:
+        cached = TracedStructure.from_problem(problem)
+        problem._default_ts = cached
+    return cached
+
+
+def get_res(problem, sol_list, traced_params, ts=None):
+    """Residual as a list of (N, 3) arrays (feax/assembler.py:1618-1667)."""
+    ts = _ts_or_build(problem, ts)
+    flat = sol_list[0].reshape(-1) if isinstance(sol_list, (list, tuple)) else sol_list
+    return problem.unflatten_fn_sol_list(residual(problem, ts, traced_params, flat))
+
+
+def create_J_bc_csr_parametric(problem, symmetric=True, dtype=None):
+    def fn(sol_flat, traced_params, bc, ts=None):
+        return assemble_csr(problem, _ts_or_build(problem, ts), traced_params, sol_flat, bc, symmetric, dtype)
+    return fn
+
+
+def create_res_bc_parametric(problem):
+    def fn(sol_flat, traced_params, bc, ts=None):
+        return residual(problem, _ts_or_build(problem, ts), traced_params, sol_flat, bc)
+    return fn
+
+
+def create_res_J_bc_csr_parametric(problem, symmetric=True):
+    jf = create_J_bc_csr_parametric(problem, symmetric)
+    rf = create_res_bc_parametric(problem)
+
+    def fn(sol_flat, traced_params, bc, ts=None):
+        return rf(sol_flat, traced_params, bc, ts), jf(sol_flat, traced_params, bc, ts)
+    return fn

@@ -1,0 +1,498 @@
+// HEX8 element kernels for gfx950: CSR Jacobian assembly and residual.
+//
+// feax evaluates element Jacobians with 24 forward-mode JVPs per element
+// inside a 256-cell lax.scan and accumulates them with a sorted
+// segment_sum into the CSR value array (feax/assembler.py:808-818,
+// 1253-1318); geometry (shape gradients, JxW) is precomputed and stored per
+// element (feax/fe.py:96-129), ~3.2 KB/element.
+//
+// Here one 64-lane wavefront owns one mesh node n and produces its three
+// CSR rows 3n..3n+2 (contiguous in the CSR value array, 9*deg(n) values):
+//   for each element e of n, in ascending element order (= feax's
+//   summation order for a slot):
+//     lane (q, b) = (lane >> 3, lane & 7) recomputes the geometry at Gauss
+//     point q from the element's SoA coordinates (nothing per element is
+//     stored), forms the 3x3 block K_ab^q of node pair (a, b) with a the
+//     local index of n, and the 8 quadrature lanes of each b are summed
+//     with a fixed butterfly;
+//     lanes q == 0 add the block into an LDS accumulator at the slot given
+//     by elem_rank[e][a][b] (position of node b in n's neighbour list);
+//   the accumulator is written once, coalesced, with the Dirichlet
+//   elimination of feax/DCboundary.py:147-199 fused into the store.
+// No float atomics; bitwise repeatable; CSR values written exactly once.
+#include <cstdint>
+
+#include "common.h"
+
+namespace fx {
+
+constexpr int kNodesPerBlock = 4;  // one wave per node, 4 waves per block
+constexpr int kMaxDeg = 64;        // LDS accumulator: 9 * kMaxDeg values per wave
+
+__device__ __forceinline__ double gauss_pt(int i) {
+  // (1 -+ 1/sqrt(3)) / 2: the 2-point Gauss-Legendre rule on [0, 1].
+  return i ? 0.78867513459481288225 : 0.21132486540518711775;
+}
+
+// Reference-cell corner of feax local node a (basix vertex order re-ordered
+// by [0,1,3,2,4,5,7,6], feax/basis.py:65-66).
+__device__ __forceinline__ void corner(int a, int& c0, int& c1, int& c2) {
+  c0 = (a == 1 || a == 2 || a == 5 || a == 6);
+  c1 = (a == 2 || a == 3 || a == 6 || a == 7);
+  c2 = (a >= 4);
+}
+
+template <typename T>
+__device__ __forceinline__ void shape(int a, T x0, T x1, T x2, T& N, T& d0, T& d1, T& d2) {
+  int c0, c1, c2;
+  corner(a, c0, c1, c2);
+  const T f0 = c0 ? x0 : T(1) - x0, f1 = c1 ? x1 : T(1) - x1, f2 = c2 ? x2 : T(1) - x2;
+  const T s0 = c0 ? T(1) : T(-1), s1 = c1 ? T(1) : T(-1), s2 = c2 ? T(1) : T(-1);
+  N = f0 * f1 * f2;
+  d0 = s0 * f1 * f2;
+  d1 = f0 * s1 * f2;
+  d2 = f0 * f1 * s2;
+}
+
+// J_ij = sum_c X_c,i dN_c/dxi_j; returns det J and writes J^-1 (feax/fe.py:96-129).
+template <typename T>
+__device__ __forceinline__ T jac_inv(const T (&X)[8][3], T x0, T x1, T x2, T (&Ji)[3][3]) {
+  T J[3][3] = {{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    T N, d0, d1, d2;
+    shape<T>(c, x0, x1, x2, N, d0, d1, d2);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      J[i][0] += X[c][i] * d0;
+      J[i][1] += X[c][i] * d1;
+      J[i][2] += X[c][i] * d2;
+    }
+  }
+  const T c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const T c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const T c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const T det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const T id = T(1) / det;
+  Ji[0][0] = c00 * id;
+  Ji[1][0] = c01 * id;
+  Ji[2][0] = c02 * id;
+  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+  return det;
+}
+
+// Physical gradient g_i = sum_j dN/dxi_j (J^-1)_ji.
+template <typename T>
+__device__ __forceinline__ void phys_grad(const T (&Ji)[3][3], T d0, T d1, T d2, T (&g)[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) g[i] = d0 * Ji[0][i] + d1 * Ji[1][i] + d2 * Ji[2][i];
+}
+
+struct MatParams {
+  int kind, loc;
+  double nu, E0, Emin, penal, kappa;
+};
+
+// Element coordinates relative to local node 0 (differences taken in FP64,
+// so FP32 geometry does not lose digits to large absolute coordinates).
+template <typename T>
+__device__ __forceinline__ void load_coords(const double* __restrict__ x, const double* __restrict__ y,
+                                            const double* __restrict__ z, const int* __restrict__ cn,
+                                            T (&X)[8][3]) {
+  const int n0 = cn[0];
+  const double ox = x[n0], oy = y[n0], oz = z[n0];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int nd = cn[c];
+    X[c][0] = static_cast<T>(x[nd] - ox);
+    X[c][1] = static_cast<T>(y[nd] - oy);
+    X[c][2] = static_cast<T>(z[nd] - oz);
+  }
+}
+
+// Material parameter at the quadrature point (feax/assembler.py:227-269:
+// node vars interpolated with N, cell vars broadcast), then the modulus.
+template <typename T>
+__device__ __forceinline__ T modulus_at(const MatParams& m, const double* __restrict__ param, int e,
+                                        const int* __restrict__ cn, T x0, T x1, T x2) {
+  T p;
+  if (m.loc == FX_PARAM_NODE) {
+    p = T(0);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      T N, d0, d1, d2;
+      shape<T>(c, x0, x1, x2, N, d0, d1, d2);
+      p += N * static_cast<T>(param[cn[c]]);
+    }
+  } else {
+    p = static_cast<T>(param[e]);
+  }
+  if (m.kind == FX_MAT_SIMP) p = static_cast<T>(m.Emin) + pow(p, static_cast<T>(m.penal)) * static_cast<T>(m.E0 - m.Emin);
+  return p;
+}
+
+template <typename T>
+__device__ __forceinline__ T xor_sum(T v, int mask) {
+  return v + __shfl_xor(v, mask, 64);
+}
+
+// ------------------------------------------------------------------ assembly
+template <typename T, typename C>
+__global__ __launch_bounds__(64 * kNodesPerBlock) void assemble_kernel(
+    int num_nodes, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const int* __restrict__ cells, const int* __restrict__ inc_ptr, const int* __restrict__ inc,
+    const uint8_t* __restrict__ rank, const int* __restrict__ indptr, const int* __restrict__ indices,
+    MatParams mat, const double* __restrict__ param, const uint8_t* __restrict__ bc_mask, int symmetric,
+    int diff_form, double jacobi_shift, double* __restrict__ dinv, T* __restrict__ data) {
+  __shared__ C acc_all[kNodesPerBlock][9 * kMaxDeg];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nd = blockIdx.x * kNodesPerBlock + wid;
+  if (nd >= num_nodes) return;
+  C* acc = acc_all[wid];
+  const int row0 = indptr[3 * nd];
+  const int len = indptr[3 * nd + 1] - row0;  // 3 * deg
+  const int nval = 3 * len;
+  if (len > 3 * kMaxDeg) return;  // host guarantees deg <= kMaxDeg
+  for (int i = lane; i < nval; i += 64) acc[i] = C(0);
+  __builtin_amdgcn_wave_barrier();
+
+  const int q = lane >> 3, b = lane & 7;
+  const C x0 = static_cast<C>(gauss_pt(q >> 2)), x1 = static_cast<C>(gauss_pt((q >> 1) & 1)),
+          x2 = static_cast<C>(gauss_pt(q & 1));
+  const C nu = static_cast<C>(mat.nu);
+  const C lam0 = nu / ((C(1) + nu) * (C(1) - C(2) * nu)), mu0 = C(1) / (C(2) * (C(1) + nu));
+
+  for (int k = inc_ptr[nd]; k < inc_ptr[nd + 1]; ++k) {
+    const int pk = inc[k];
+    const int e = pk >> 3, a = pk & 7;
+    const int* cn = cells + 8 * static_cast<long>(e);
+    C X[8][3];
+    load_coords<C>(x, y, z, cn, X);
+    C Ji[3][3];
+    const C det = jac_inv<C>(X, x0, x1, x2, Ji);
+    C N, d0, d1, d2, ga[3], gb[3];
+    shape<C>(a, x0, x1, x2, N, d0, d1, d2);
+    phys_grad<C>(Ji, d0, d1, d2, ga);
+    shape<C>(b, x0, x1, x2, N, d0, d1, d2);
+    phys_grad<C>(Ji, d0, d1, d2, gb);
+    const C E = modulus_at<C>(mat, param, e, cn, x0, x1, x2);
+    const C w = det * C(0.125);  // JxW, Gauss weights 1/8 on [0,1]^3
+    const C wl = w * E * lam0, wm = w * E * mu0;
+    const C gg = wm * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+    C K[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        K[3 * i + j] = wl * ga[i] * gb[j] + wm * ga[j] * gb[i] + (i == j ? gg : C(0));
+    // sum over the 8 Gauss points (lanes b, b+8, ..., b+56)
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      C v = K[t];
+      v = xor_sum(v, 8);
+      v = xor_sum(v, 16);
+      v = xor_sum(v, 32);
+      K[t] = v;
+    }
+    if (q == 0) {
+      const int kb = rank[64 * static_cast<long>(e) + 8 * a + b];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) acc[i * len + 3 * kb + j] += K[3 * i + j];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  __builtin_amdgcn_wave_barrier();
+  // Position of node nd in its own neighbour list (its diagonal 3x3 block).
+  int kself = 0;
+  {
+    const int pk = inc[inc_ptr[nd]];
+    const int a0 = pk & 7;
+    kself = rank[64 * static_cast<long>(pk >> 3) + 9 * a0];
+  }
+  if (dinv && lane < 3) {
+    // Jacobi: 1 / (K_bc[r, r] + shift) (feax/solvers/common.py:86-94); BC rows have diagonal 1.
+    const int r = 3 * nd + lane;
+    const double d = (bc_mask && bc_mask[r]) ? 1.0 : static_cast<double>(acc[lane * len + 3 * kself + lane]);
+    dinv[r] = 1.0 / (d + jacobi_shift);
+  }
+  if (diff_form) {
+    // Difference form: the diagonal block slot holds S = block row sums of
+    // the BC-eliminated matrix.  Unconstrained element rows sum to zero
+    // (sum_b grad N_b = 0), so for a free row S[c][d] = -(sum of its entries
+    // in eliminated columns (m, d)); a BC row has S[c][d] = delta_cd.
+    C S = C(0);
+    const int c = lane / 3, d = lane - 3 * (lane / 3);
+    if (lane < 9) {
+      if (bc_mask && bc_mask[3 * nd + c]) {
+        S = (c == d) ? C(1) : C(0);
+      } else if (bc_mask) {
+        for (int kb = 0; kb < len / 3; ++kb) {
+          const int m = indices[row0 + 3 * kb] / 3;
+          if (bc_mask[3 * m + d]) S -= acc[c * len + 3 * kb + d];
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < 9) acc[c * len + 3 * kself + d] = S;
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int idx = lane; idx < nval; idx += 64) {
+    C v = acc[idx];
+    const int slot = row0 + idx;
+    const bool own = ((idx % len) / 3) == kself;
+    if (bc_mask && !(diff_form && own)) {
+      const int r = 3 * nd + idx / len;
+      const int c = indices[slot];
+      const bool zr = bc_mask[r] != 0;
+      if (zr || (symmetric && bc_mask[c] != 0)) v = C(0);
+      if (zr && r == c) v = C(1);
+    }
+    data[slot] = static_cast<T>(v);
+  }
+}
+
+// ------------------------------------------------------------------ residual
+// Face quadrature point (lf, q) in the reference cell: the 2x2 Gauss points
+// of the unit square mapped onto basix facet lf (feax/basis.py:286-305).
+__device__ __forceinline__ void face_point(int lf, int q, double& p0, double& p1, double& p2) {
+  const double s = gauss_pt(q >> 1), t = gauss_pt(q & 1);
+  switch (lf) {
+    case 0: p0 = s; p1 = t; p2 = 0.0; break;
+    case 1: p0 = s; p1 = 0.0; p2 = t; break;
+    case 2: p0 = 0.0; p1 = s; p2 = t; break;
+    case 3: p0 = 1.0; p1 = s; p2 = t; break;
+    case 4: p0 = s; p1 = 1.0; p2 = t; break;
+    default: p0 = s; p1 = t; p2 = 1.0; break;
+  }
+}
+
+__device__ __forceinline__ void face_normal(int lf, double (&nrm)[3]) {
+  nrm[0] = nrm[1] = nrm[2] = 0.0;
+  switch (lf) {
+    case 0: nrm[2] = -1.0; break;
+    case 1: nrm[1] = -1.0; break;
+    case 2: nrm[0] = -1.0; break;
+    case 3: nrm[0] = 1.0; break;
+    case 4: nrm[1] = 1.0; break;
+    default: nrm[2] = 1.0; break;
+  }
+}
+
+__global__ __launch_bounds__(64 * kNodesPerBlock) void residual_kernel(
+    int num_nodes, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const int* __restrict__ cells, const int* __restrict__ inc_ptr, const int* __restrict__ inc, MatParams mat,
+    const double* __restrict__ param, const double* __restrict__ u, const int* __restrict__ faces,
+    const int* __restrict__ finc_ptr, const int* __restrict__ finc, const double* __restrict__ svar,
+    const double* __restrict__ fdir, int negate, double* __restrict__ res) {
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nd = blockIdx.x * kNodesPerBlock + wid;
+  if (nd >= num_nodes) return;
+  const int q = lane >> 3, c = lane & 7;
+  const double x0 = gauss_pt(q >> 2), x1 = gauss_pt((q >> 1) & 1), x2 = gauss_pt(q & 1);
+  const double nu = mat.nu;
+  const double lam0 = nu / ((1 + nu) * (1 - 2 * nu)), mu0 = 1 / (2 * (1 + nu));
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+  for (int k = inc_ptr[nd]; k < inc_ptr[nd + 1]; ++k) {
+    const int pk = inc[k];
+    const int e = pk >> 3, a = pk & 7;
+    const int* cn = cells + 8 * static_cast<long>(e);
+    double X[8][3];
+    load_coords<double>(x, y, z, cn, X);
+    double Ji[3][3];
+    const double det = jac_inv<double>(X, x0, x1, x2, Ji);
+    double N, d0, d1, d2, gc[3], ga[3];
+    shape<double>(c, x0, x1, x2, N, d0, d1, d2);
+    phys_grad<double>(Ji, d0, d1, d2, gc);
+    shape<double>(a, x0, x1, x2, N, d0, d1, d2);
+    phys_grad<double>(Ji, d0, d1, d2, ga);
+    const int nc = cn[c];
+    double ug[9];  // du_i/dx_d, summed over the 8 nodes of this Gauss point
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) ug[3 * i + d] = u[3 * nc + i] * gc[d];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      double v = ug[t];
+      v = xor_sum(v, 1);
+      v = xor_sum(v, 2);
+      v = xor_sum(v, 4);
+      ug[t] = v;
+    }
+    const double E = modulus_at<double>(mat, param, e, cn, x0, x1, x2);
+    const double lam = E * lam0, mu = E * mu0;
+    const double tr = ug[0] + ug[4] + ug[8];
+    double sig[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        sig[3 * i + d] = mu * (ug[3 * i + d] + ug[3 * d + i]) + (i == d ? lam * tr : 0.0);
+    const double w = det * 0.125;
+    double ri[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ri[i] = (sig[3 * i] * ga[0] + sig[3 * i + 1] * ga[1] + sig[3 * i + 2] * ga[2]) * w;
+    // every lane of a Gauss-point group holds the same value: sum over groups
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double v = ri[i];
+      v = xor_sum(v, 8);
+      v = xor_sum(v, 16);
+      v = xor_sum(v, 32);
+      ri[i] = v;
+    }
+    r0 += ri[0];
+    r1 += ri[1];
+    r2 += ri[2];
+  }
+  // Dead surface traction t = dir * svar[f, q] on the loaded faces of this
+  // node, ascending face order (feax/assembler.py:494-535, + sign).
+  if (finc_ptr) {
+    for (int k = finc_ptr[nd]; k < finc_ptr[nd + 1]; ++k) {
+      const int pk = finc[k];
+      const int f = pk >> 3, a = pk & 7;
+      const int e = faces[2 * f], lf = faces[2 * f + 1];
+      double v = 0.0;
+      if (lane < 4) {
+        double p0, p1, p2;
+        face_point(lf, lane, p0, p1, p2);
+        double X[8][3];
+        load_coords<double>(x, y, z, cells + 8 * static_cast<long>(e), X);
+        double Ji[3][3];
+        const double det = jac_inv<double>(X, p0, p1, p2, Ji);
+        double nrm[3];
+        face_normal(lf, nrm);
+        double m[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) m[i] = nrm[0] * Ji[0][i] + nrm[1] * Ji[1][i] + nrm[2] * Ji[2][i];
+        const double nanson = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) * det * 0.25;
+        double N, d0, d1, d2;
+        shape<double>(a, p0, p1, p2, N, d0, d1, d2);
+        v = svar[4 * f + lane] * N * nanson;
+      }
+      v = xor_sum(v, 1);
+      v = xor_sum(v, 2);
+      r0 += fdir[3 * f] * v;
+      r1 += fdir[3 * f + 1] * v;
+      r2 += fdir[3 * f + 2] * v;
+    }
+  }
+  if (lane == 0) {
+    const double s = negate ? -1.0 : 1.0;
+    res[3 * nd] = s * r0;
+    res[3 * nd + 1] = s * r1;
+    res[3 * nd + 2] = s * r2;
+  }
+}
+
+__global__ void bc_res_kernel(int nbc, const int* __restrict__ rows, const double* __restrict__ vals,
+                              const double* __restrict__ u, int negate, double* __restrict__ res) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbc) return;
+  const int r = rows[i];
+  const double v = (u ? u[r] : 0.0) - vals[i];
+  res[r] = negate ? -v : v;
+}
+
+inline MatParams to_params(const fx_material* m) {
+  MatParams p;
+  p.kind = m->kind;
+  p.loc = m->param_loc;
+  p.nu = m->nu;
+  p.E0 = m->E0;
+  p.Emin = m->Emin;
+  p.penal = m->penal;
+  p.kappa = m->kappa;
+  return p;
+}
+
+template <typename T>
+int assemble(void* stream, int num_nodes, int num_cells, const double* x, const double* y, const double* z,
+             const int* cells, const int* inc_ptr, const int* inc, const uint8_t* rank, const int* indptr,
+             const int* indices, const fx_material* mat, const double* param, const double* u,
+             const uint8_t* bc_mask, int symmetric, int flags, double jacobi_shift, double* dinv, T* data) {
+  if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !z || !cells || !inc_ptr || !inc || !rank || !indptr ||
+      !indices || !mat || !param || !data)
+    return fail(FX_EINVAL, "fx_hex8_assemble: bad arguments");
+  if (mat->kind == FX_MAT_NEO_HOOKEAN) return fail(FX_EINVAL, "fx_hex8_assemble: Neo-Hookean tangent not built");
+  if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP)
+    return fail(FX_EINVAL, "fx_hex8_assemble: unknown material %d", mat->kind);
+  (void)u;
+  const int diff = (flags & FX_ASM_DIFF3) != 0;
+  const int grid = (num_nodes + kNodesPerBlock - 1) / kNodesPerBlock;
+  // Element matrices are formed in FP64 and rounded once at the store, unless
+  // FX_ASM_COMPUTE32 asks for FP32 arithmetic as well.
+  if (sizeof(T) == 4 && (flags & FX_ASM_COMPUTE32)) {
+    hipLaunchKernelGGL((assemble_kernel<T, float>), dim3(grid), dim3(64 * kNodesPerBlock), 0, as_stream(stream),
+                       num_nodes, x, y, z, cells, inc_ptr, inc, rank, indptr, indices, to_params(mat), param, bc_mask,
+                       symmetric, diff, jacobi_shift, dinv, data);
+    FX_LAUNCHED("assemble_kernel");
+    return FX_OK;
+  }
+  hipLaunchKernelGGL((assemble_kernel<T, double>), dim3(grid), dim3(64 * kNodesPerBlock), 0, as_stream(stream), num_nodes,
+                     x, y, z, cells, inc_ptr, inc, rank, indptr, indices, to_params(mat), param, bc_mask,
+                     symmetric, diff, jacobi_shift, dinv, data);
+  FX_LAUNCHED("assemble_kernel");
+  return FX_OK;
+}
+
+}  // namespace fx
+
+using namespace fx;
+
+extern "C" {
+
+int fx_hex8_assemble_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                         const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                         const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                         const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                         int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, double* data) {
+  return assemble<double>(stream, num_nodes, num_cells, x, y, z, cells, inc_ptr, inc, elem_rank, indptr, indices,
+                          mat, param, u, bc_mask, symmetric, flags, jacobi_shift, dinv, data);
+}
+
+int fx_hex8_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                         const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                         const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                         const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                         int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, float* data) {
+  return assemble<float>(stream, num_nodes, num_cells, x, y, z, cells, inc_ptr, inc, elem_rank, indptr, indices,
+                         mat, param, u, bc_mask, symmetric, flags, jacobi_shift, dinv, data);
+}
+
+int fx_hex8_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                         const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                         const fx_material* mat, const double* param, const double* u, int32_t num_faces,
+                         const int32_t* faces, const int32_t* finc_ptr, const int32_t* finc, const double* svar,
+                         const double* fdir, int32_t num_bc, const int32_t* bc_rows, const double* bc_vals,
+                         int32_t negate, double* res) {
+  if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !z || !cells || !inc_ptr || !inc || !mat || !param || !u ||
+      !res)
+    return fail(FX_EINVAL, "fx_hex8_residual_f64: bad arguments");
+  if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP)
+    return fail(FX_EINVAL, "fx_hex8_residual_f64: material %d not built", mat->kind);
+  if (num_faces > 0 && (!faces || !finc_ptr || !finc || !svar || !fdir))
+    return fail(FX_EINVAL, "fx_hex8_residual_f64: face arrays missing");
+  hipStream_t s = as_stream(stream);
+  const int grid = (num_nodes + kNodesPerBlock - 1) / kNodesPerBlock;
+  hipLaunchKernelGGL(residual_kernel, dim3(grid), dim3(64 * kNodesPerBlock), 0, s, num_nodes, x, y, z, cells,
+                     inc_ptr, inc, to_params(mat), param, u, num_faces > 0 ? faces : nullptr,
+                     num_faces > 0 ? finc_ptr : nullptr, num_faces > 0 ? finc : nullptr, svar, fdir, negate, res);
+  FX_LAUNCHED("residual_kernel");
+  if (num_bc > 0 && bc_rows) {
+    if (!bc_vals) return fail(FX_EINVAL, "fx_hex8_residual_f64: bc_vals missing");
+    hipLaunchKernelGGL(bc_res_kernel, dim3((num_bc + 255) / 256), dim3(256), 0, s, num_bc, bc_rows, bc_vals, u,
+                       negate, res);
+    FX_LAUNCHED("bc_res_kernel");
+  }
+  return FX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,655 @@
+// CSR SpMV, Jacobi diagonal and Jacobi-preconditioned CG for gfx950.
+//
+// SpMV replaces CSRMatrix.__matmul__ (feax/csr.py:81-90): y = A x with the
+// CSR triple (data, indptr, indices) in feax's exact layout (int32 indptr /
+// indices, row-major, sorted columns).
+//
+// Two kernels:
+//  * scalar CSR: a group of L lanes of a 64-wide wavefront owns one row and
+//    walks it with coalesced loads of data/indices.
+//  * block-3 (HEX8, vec = 3): rows 3m..3m+2 of a node share one column list
+//    made of whole 3-blocks.  A group of L lanes owns a node; lane t reads
+//    value position k of all three rows (coalesced), one column per 3x3
+//    block from `bcol` (nse/9 int32 built once by fx_csr_block3_index), and
+//    each gathered x entry is used three times.  Bytes per nonzero:
+//    s_v + 4/9 instead of s_v + 4.
+// A fixed grid of at most kMaxGridBlocks workgroups walks contiguous row
+// ranges; the block id is remapped so each XCD owns one contiguous range
+// (x entries gathered by neighbouring rows hit that XCD's L2).  Matrix
+// values and indices are streamed with non-temporal loads so they do not
+// evict the gathered vector.
+//
+// PCG follows jax.scipy.sparse.linalg.cg (feax/solvers/common.py:229-239)
+// with scalars kept on the device.  Per iteration: SpMV fused with the p.q
+// block partials, a 1-block finalize (alpha), the vector update fused with
+// r.z / r.r partials, a 1-block finalize (beta, stop test), the direction
+// update.  FP64 partial sums, fixed order: bitwise repeatable.
+#include <algorithm>
+#include <cstdint>
+
+#include "common.h"
+#include "reduce.h"
+
+namespace fx {
+
+template <typename T>
+__device__ __forceinline__ T ldnt(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
+template <typename T, int L>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, L);
+  return v;
+}
+
+// One row, L lanes, value type V, arithmetic type T.
+template <typename V, typename T, int L>
+__device__ __forceinline__ T csr_row(const int* __restrict__ indptr, const int* __restrict__ indices,
+                                     const V* __restrict__ data, const T* __restrict__ x, int row, int gl) {
+  const int s = indptr[row], e = indptr[row + 1];
+  T acc = T(0);
+  for (int k = s + gl; k < e; k += L) acc += static_cast<T>(ldnt(data + k)) * x[ldnt(indices + k)];
+  return group_sum<T, L>(acc);
+}
+
+// Rows 3m..3m+2 of a block-3 pattern, L lanes.  DIFF: difference form --
+// the node's own 3x3 block slot holds S (block row sums) and the other
+// blocks act on x_m - x_n:  y = sum_{m != n} K_nm (x_m - x_n) + S x_n.
+template <typename V, typename T, int L, bool DIFF>
+__device__ __forceinline__ void b3_rows(const int* __restrict__ indptr, const int* __restrict__ bcol,
+                                        const V* __restrict__ data, const T* __restrict__ x, int m, int gl,
+                                        T& y0, T& y1, T& y2) {
+  const int r0 = indptr[3 * m];
+  const int len = indptr[3 * m + 1] - r0;  // 3 * deg
+  const int* bc = bcol + r0 / 9;
+  const V* d0 = data + r0;
+  T a0 = T(0), a1 = T(0), a2 = T(0);
+  T xs0 = T(0), xs1 = T(0), xs2 = T(0);
+  if constexpr (DIFF) {
+    xs0 = x[3 * m];
+    xs1 = x[3 * m + 1];
+    xs2 = x[3 * m + 2];
+  }
+  for (int k = gl; k < len; k += L) {
+    const int kb = k / 3, j = k - 3 * kb;
+    const int c = ldnt(bc + kb);
+    T xv = x[3 * c + j];
+    if constexpr (DIFF) {
+      if (c != m) xv -= (j == 0 ? xs0 : (j == 1 ? xs1 : xs2));
+    }
+    a0 += static_cast<T>(ldnt(d0 + k)) * xv;
+    a1 += static_cast<T>(ldnt(d0 + len + k)) * xv;
+    a2 += static_cast<T>(ldnt(d0 + 2 * len + k)) * xv;
+  }
+  y0 = group_sum<T, L>(a0);
+  y1 = group_sum<T, L>(a1);
+  y2 = group_sum<T, L>(a2);
+}
+
+struct Range {
+  int begin, end;
+};
+
+// Contiguous range of groups owned by this (XCD-remapped) block.
+__device__ __forceinline__ Range block_range(int ngroups, int& lblock) {
+  lblock = xcd_block(blockIdx.x, gridDim.x);
+  const int chunk = (ngroups + gridDim.x - 1) / gridDim.x;
+  Range r;
+  r.begin = min(ngroups, lblock * chunk);
+  r.end = min(ngroups, r.begin + chunk);
+  return r;
+}
+
+// Row / node loop shared by the plain and the PCG SpMV.  When DOT, the
+// block partial of sum_i p_i (A p)_i goes to partials[lblock].
+template <typename V, typename T, int L, bool B3, bool DIFF, bool DOT>
+__device__ __forceinline__ void spmv_body(int n, const int* __restrict__ indptr, const int* __restrict__ idx,
+                                          const V* __restrict__ data, const T* __restrict__ x, T* __restrict__ y,
+                                          double* partials) {
+  int lb;
+  const int ngroups = B3 ? n / 3 : n;
+  const Range rg = block_range(ngroups, lb);
+  const int gl = threadIdx.x % L;
+  DVec<1> acc;
+  acc.v[0] = 0.0;
+  for (int g = rg.begin + threadIdx.x / L; g < rg.end; g += kBlock / L) {
+    if constexpr (B3) {
+      T y0, y1, y2;
+      b3_rows<V, T, L, DIFF>(indptr, idx, data, x, g, gl, y0, y1, y2);
+      if (gl == 0) {
+        y[3 * g] = y0;
+        y[3 * g + 1] = y1;
+        y[3 * g + 2] = y2;
+        if constexpr (DOT)
+          acc.v[0] += static_cast<double>(x[3 * g]) * static_cast<double>(y0) +
+                      static_cast<double>(x[3 * g + 1]) * static_cast<double>(y1) +
+                      static_cast<double>(x[3 * g + 2]) * static_cast<double>(y2);
+      }
+    } else {
+      const T v = csr_row<V, T, L>(indptr, idx, data, x, g, gl);
+      if (gl == 0) {
+        y[g] = v;
+        if constexpr (DOT) acc.v[0] += static_cast<double>(x[g]) * static_cast<double>(v);
+      }
+    }
+  }
+  if constexpr (DOT) store_partials<1>(block_sum<1, kBlock>(acc), lb, partials);
+}
+
+template <typename V, typename T, int L, bool B3, bool DIFF>
+__global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const int* __restrict__ indptr,
+                                                      const int* __restrict__ idx, const V* __restrict__ data,
+                                                      const T* __restrict__ x, T* __restrict__ y) {
+  spmv_body<V, T, L, B3, DIFF, false>(n, indptr, idx, data, x, y, nullptr);
+}
+
+// Lanes per group from the mean row length (scalar) or mean 3*deg (block-3).
+inline int pick_lanes(double mean_len, bool b3) {
+  if (b3) {
+    if (mean_len > 48) return 32;
+    if (mean_len > 20) return 16;
+    return 8;
+  }
+  if (mean_len > 96) return 64;
+  if (mean_len > 40) return 32;
+  if (mean_len > 14) return 16;
+  if (mean_len > 6) return 8;
+  return 4;
+}
+
+inline int lanes_for(int n, long long nse, bool b3) {
+  // Tuning override (FEAXHIP_SPMV_LANES / FEAXHIP_SPMV_B3_LANES), read once.
+  static const int ov_csr = getenv("FEAXHIP_SPMV_LANES") ? atoi(getenv("FEAXHIP_SPMV_LANES")) : 0;
+  static const int ov_b3 = getenv("FEAXHIP_SPMV_B3_LANES") ? atoi(getenv("FEAXHIP_SPMV_B3_LANES")) : 0;
+  const int ov = b3 ? ov_b3 : ov_csr;
+  if (ov == 4 || ov == 8 || ov == 16 || ov == 32 || ov == 64) return ov;
+  const double mean = n ? static_cast<double>(nse) / n : 0.0;  // nnz per row = 3 * deg
+  return pick_lanes(mean, b3);
+}
+
+inline int grid_for(int ngroups, int lanes) {
+  const int gpb = kBlock / lanes;
+  const long need = (static_cast<long>(ngroups) + gpb - 1) / gpb;
+  return static_cast<int>(std::max<long>(1, std::min<long>(need, kMaxGridBlocks)));
+}
+
+inline int vec_grid(int n) { return std::max(1, std::min((n + kBlock - 1) / kBlock, kMaxGridBlocks)); }
+
+#define FX_LANE_SWITCH(lanes, ...)                                            \
+  switch (lanes) {                                                            \
+    case 4: { constexpr int LL = 4; __VA_ARGS__; break; }                     \
+    case 8: { constexpr int LL = 8; __VA_ARGS__; break; }                     \
+    case 16: { constexpr int LL = 16; __VA_ARGS__; break; }                   \
+    case 32: { constexpr int LL = 32; __VA_ARGS__; break; }                   \
+    case 64: { constexpr int LL = 64; __VA_ARGS__; break; }                   \
+    default: return fail(FX_EINVAL, "unsupported lane count %d", lanes);     \
+  }
+
+template <typename V, typename T, bool B3, bool DIFF>
+int launch_spmv(hipStream_t st, int n, const int* indptr, const int* idx, const V* data, const T* x, T* y,
+                int lanes) {
+  const int ngroups = B3 ? n / 3 : n;
+  if (ngroups == 0) return FX_OK;
+  const int grid = grid_for(ngroups, lanes);
+  FX_LANE_SWITCH(lanes, hipLaunchKernelGGL((spmv_kernel<V, T, LL, B3, DIFF>), dim3(grid), dim3(kBlock), 0, st, n,
+                                            indptr, idx, data, x, y));
+  FX_LAUNCHED("spmv_kernel");
+  return FX_OK;
+}
+
+// bcol[indptr[3m]/9 + k] = column node of the k-th 3x3 block of node m.
+__global__ __launch_bounds__(kBlock) void block3_index_kernel(int nnodes, const int* __restrict__ indptr,
+                                                              const int* __restrict__ indices,
+                                                              int* __restrict__ bcol) {
+  const int m = blockIdx.x * (kBlock / 32) + threadIdx.x / 32;
+  if (m >= nnodes) return;
+  const int r0 = indptr[3 * m];
+  const int deg = (indptr[3 * m + 1] - r0) / 3;
+  for (int k = threadIdx.x % 32; k < deg; k += 32) bcol[r0 / 9 + k] = indices[r0 + 3 * k] / 3;
+}
+
+// ---------------------------------------------------------------- Jacobi
+template <typename T>
+__global__ __launch_bounds__(kBlock) void jacobi_kernel(int n, const int* __restrict__ indptr,
+                                                        const int* __restrict__ indices, const T* __restrict__ data,
+                                                        double shift, T* __restrict__ dinv) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  // feax CSRMatrix.diagonal (csr.py:115-121): sum of the row's (i, i) slots.
+  double d = 0.0;
+  for (int k = indptr[i]; k < indptr[i + 1]; ++k)
+    if (indices[k] == i) d += static_cast<double>(data[k]);
+  dinv[i] = static_cast<T>(1.0 / (d + shift));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void diag_kernel(int n, const int* __restrict__ indptr,
+                                                      const int* __restrict__ indices, const T* __restrict__ data,
+                                                      T* __restrict__ diag) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  T d = T(0);
+  for (int k = indptr[i]; k < indptr[i + 1]; ++k)
+    if (indices[k] == i) d += data[k];
+  diag[i] = d;
+}
+
+// ---------------------------------------------------------------- PCG
+struct PcgState {
+  double alpha, beta, gamma, rs, atol2, bb;
+  int k, done, maxiter, nparts;
+};
+
+constexpr size_t kStateBytes = 256;
+// Partial slots (x kMaxGridBlocks doubles): p.q [0], r.z r.r [1, 3), init sums [3, 6).
+constexpr size_t kPartialBytes = 6 * kMaxGridBlocks * sizeof(double);
+
+inline size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+template <typename V, typename T, int L, bool B3, bool DIFF>
+__global__ __launch_bounds__(kBlock) void pcg_spmv_kernel(int n, const int* __restrict__ indptr,
+                                                          const int* __restrict__ idx, const V* __restrict__ data,
+                                                          const T* __restrict__ p, T* __restrict__ q,
+                                                          const PcgState* __restrict__ st, double* partials) {
+  if (st->done) return;
+  spmv_body<V, T, L, B3, DIFF, true>(n, indptr, idx, data, p, q, partials);
+}
+
+__global__ __launch_bounds__(kFinBlock) void pcg_fin_alpha(PcgState* st, const double* __restrict__ partials,
+                                                           int nparts) {
+  if (st->done) return;
+  DVec<1> t = sum_partials<1>(partials, nparts);
+  if (threadIdx.x == 0) st->alpha = st->gamma / t.v[0];
+}
+
+// x += alpha p; r -= alpha q; z = M r; partials of r.z and r.r.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, T* __restrict__ x, T* __restrict__ r,
+                                                            T* __restrict__ z, const T* __restrict__ p,
+                                                            const T* __restrict__ q, const T* __restrict__ dinv,
+                                                            const PcgState* __restrict__ st, double* partials) {
+  if (st->done) return;
+  int lb;
+  const Range rg = block_range(n, lb);
+  const T alpha = static_cast<T>(st->alpha);
+  DVec<2> acc;
+  acc.v[0] = 0.0;
+  acc.v[1] = 0.0;
+  for (int i = rg.begin + threadIdx.x; i < rg.end; i += kBlock) {
+    x[i] += alpha * p[i];
+    const T ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const T zi = dinv ? dinv[i] * ri : ri;
+    z[i] = zi;
+    acc.v[0] += static_cast<double>(ri) * static_cast<double>(zi);
+    acc.v[1] += static_cast<double>(ri) * static_cast<double>(ri);
+  }
+  store_partials<2>(block_sum<2, kBlock>(acc), lb, partials);
+}
+
+__global__ __launch_bounds__(kFinBlock) void pcg_fin_beta(PcgState* st, const double* __restrict__ partials,
+                                                          int nparts) {
+  if (st->done) return;
+  DVec<2> t = sum_partials<2>(partials, nparts);
+  if (threadIdx.x == 0) {
+    const double g_new = t.v[0];
+    st->beta = g_new / st->gamma;
+    st->gamma = g_new;
+    st->rs = t.v[1];
+    st->k += 1;
+    st->done = !(st->rs > st->atol2 && st->k < st->maxiter);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void pcg_dir_kernel(int n, T* __restrict__ p, const T* __restrict__ z,
+                                                         const PcgState* __restrict__ st) {
+  if (st->done) return;
+  const T beta = static_cast<T>(st->beta);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = z[i] + beta * p[i];
+}
+
+// r = b - q (q = A x0), z = M r, p = z; partials of r.z, r.r, b.b.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void pcg_init_kernel(int n, const T* __restrict__ b, const T* __restrict__ q,
+                                                          T* __restrict__ r, T* __restrict__ z, T* __restrict__ p,
+                                                          const T* __restrict__ dinv, double* partials) {
+  int lb;
+  const Range rg = block_range(n, lb);
+  DVec<3> acc;
+  acc.v[0] = acc.v[1] = acc.v[2] = 0.0;
+  for (int i = rg.begin + threadIdx.x; i < rg.end; i += kBlock) {
+    const T bi = b[i];
+    const T ri = bi - q[i];
+    r[i] = ri;
+    const T zi = dinv ? dinv[i] * ri : ri;
+    z[i] = zi;
+    p[i] = zi;
+    acc.v[0] += static_cast<double>(ri) * static_cast<double>(zi);
+    acc.v[1] += static_cast<double>(ri) * static_cast<double>(ri);
+    acc.v[2] += static_cast<double>(bi) * static_cast<double>(bi);
+  }
+  store_partials<3>(block_sum<3, kBlock>(acc), lb, partials);
+}
+
+__global__ __launch_bounds__(kFinBlock) void pcg_fin_init(PcgState* st, const double* __restrict__ partials,
+                                                          int nparts, double tol, double atol, int maxiter) {
+  DVec<3> t = sum_partials<3>(partials, nparts);
+  if (threadIdx.x == 0) {
+    st->alpha = st->beta = 0.0;
+    st->gamma = t.v[0];
+    st->rs = t.v[1];
+    st->bb = t.v[2];
+    st->atol2 = fmax(tol * tol * t.v[2], atol * atol);
+    st->k = 0;
+    st->maxiter = maxiter;
+    st->done = !(st->rs > st->atol2 && 0 < maxiter);
+  }
+}
+
+struct PcgArgs {
+  int n;
+  long long nse;
+  int flags;
+  const int* indptr;
+  const int* idx;  // indices (scalar kernel) or bcol (block-3 kernels)
+};
+
+template <typename V, typename T, int L, bool B3, bool DIFF>
+void launch_pcg_iter(hipStream_t s, const PcgArgs& a, int grid_mv, int grid_v, const V* data, const T* dinv, T* x,
+                     T* r, T* z, T* p, T* q, PcgState* st, double* partials) {
+  hipLaunchKernelGGL((pcg_spmv_kernel<V, T, L, B3, DIFF>), dim3(grid_mv), dim3(kBlock), 0, s, a.n, a.indptr, a.idx,
+                     data, p, q, st, partials);
+  hipLaunchKernelGGL(pcg_fin_alpha, dim3(1), dim3(kFinBlock), 0, s, st, partials, grid_mv);
+  hipLaunchKernelGGL((pcg_update_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, a.n, x, r, z, p, q, dinv, st,
+                     partials + kMaxGridBlocks);
+  hipLaunchKernelGGL(pcg_fin_beta, dim3(1), dim3(kFinBlock), 0, s, st, partials + kMaxGridBlocks, grid_v);
+  hipLaunchKernelGGL((pcg_dir_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, a.n, p, z, st);
+}
+
+template <typename V, typename T, bool B3, bool DIFF>
+int pcg_loop(hipStream_t s, const PcgArgs& a, int lanes, const V* data, const T* dinv, const T* b, const T* x0,
+             double tol, double atol, int maxiter, int check_every, T* x, int* iters, double* rs, void* workspace) {
+  const int n = a.n;
+  char* w = static_cast<char*>(workspace);
+  PcgState* st = reinterpret_cast<PcgState*>(w);
+  double* partials = reinterpret_cast<double*>(w + kStateBytes);
+  const size_t vb = align256(static_cast<size_t>(n) * sizeof(T));
+  T* r = reinterpret_cast<T*>(w + kStateBytes + kPartialBytes);
+  T* z = reinterpret_cast<T*>(reinterpret_cast<char*>(r) + vb);
+  T* p = reinterpret_cast<T*>(reinterpret_cast<char*>(z) + vb);
+  T* q = reinterpret_cast<T*>(reinterpret_cast<char*>(p) + vb);
+  const int grid_mv = grid_for(B3 ? n / 3 : n, lanes);
+  const int grid_v = vec_grid(n);
+  if (x0) {
+    if (x0 != x) FX_HIP(hipMemcpyAsync(x, x0, static_cast<size_t>(n) * sizeof(T), hipMemcpyDeviceToDevice, s));
+  } else {
+    FX_HIP(hipMemsetAsync(x, 0, static_cast<size_t>(n) * sizeof(T), s));
+  }
+  int rc = launch_spmv<V, T, B3, DIFF>(s, n, a.indptr, a.idx, data, x, q, lanes);
+  if (rc) return rc;
+  hipLaunchKernelGGL((pcg_init_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, n, b, q, r, z, p, dinv,
+                     partials + 3 * kMaxGridBlocks);
+  hipLaunchKernelGGL(pcg_fin_init, dim3(1), dim3(kFinBlock), 0, s, st, partials + 3 * kMaxGridBlocks, grid_v, tol,
+                     atol, maxiter);
+  FX_LAUNCHED("pcg_init");
+  if (check_every <= 0) check_every = 32;
+  struct {
+    int k, done;
+  } h = {0, 0};
+  FX_HIP(hipMemcpyAsync(&h, &st->k, sizeof(h), hipMemcpyDeviceToHost, s));
+  FX_HIP(hipStreamSynchronize(s));
+  int launched = 0;
+  while (!h.done && launched < maxiter) {
+    const int chunk = std::min(check_every, maxiter - launched);
+    for (int c = 0; c < chunk; ++c) {
+      FX_LANE_SWITCH(lanes, launch_pcg_iter<V, T, LL, B3, DIFF>(s, a, grid_mv, grid_v, data, dinv, x, r, z, p, q,
+                                                                 st, partials));
+    }
+    FX_LAUNCHED("pcg_iter");
+    launched += chunk;
+    FX_HIP(hipMemcpyAsync(&h, &st->k, sizeof(h), hipMemcpyDeviceToHost, s));
+    FX_HIP(hipStreamSynchronize(s));
+  }
+  double hrs = 0.0;
+  FX_HIP(hipMemcpyAsync(&hrs, &st->rs, sizeof(double), hipMemcpyDeviceToHost, s));
+  FX_HIP(hipStreamSynchronize(s));
+  *iters = h.k;
+  *rs = hrs;
+  return FX_OK;
+}
+
+template <typename V, typename T>
+int run_pcg(void* stream, int n, long long nse, int flags, const int* indptr, const int* indices, const int* bcol,
+            const V* data, const T* dinv, const T* b, const T* x0, double tol, double atol, int maxiter,
+            int check_every, T* x, int* iters, double* rs, void* workspace, size_t ws_bytes) {
+  hipStream_t s = as_stream(stream);
+  if (n < 0 || !indptr || !data || !b || !x || !iters || !rs) return fail(FX_EINVAL, "pcg: bad arguments");
+  if (maxiter < 0) return fail(FX_EINVAL, "pcg: maxiter < 0");
+  const bool b3 = (flags & (FX_PCG_BLOCK3 | FX_PCG_DIFF3)) != 0;
+  const bool diff = (flags & FX_PCG_DIFF3) != 0;
+  if (b3 && (n % 3 || !bcol)) return fail(FX_EINVAL, "pcg: block-3 kernels need n %% 3 == 0 and bcol");
+  if (!b3 && !indices) return fail(FX_EINVAL, "pcg: indices missing");
+  const size_t need = kStateBytes + kPartialBytes + 4 * align256(static_cast<size_t>(n) * sizeof(T));
+  if (!workspace || ws_bytes < need) return fail(FX_EWORKSPACE, "pcg: workspace %zu < %zu bytes", ws_bytes, need);
+  if (n == 0) {
+    *iters = 0;
+    *rs = 0.0;
+    return FX_OK;
+  }
+  PcgArgs a{n, nse, flags, indptr, b3 ? bcol : indices};
+  const int lanes = lanes_for(n, nse, b3);
+  if (diff) return pcg_loop<V, T, true, true>(s, a, lanes, data, dinv, b, x0, tol, atol, maxiter, check_every, x,
+                                               iters, rs, workspace);
+  if (b3) return pcg_loop<V, T, true, false>(s, a, lanes, data, dinv, b, x0, tol, atol, maxiter, check_every, x,
+                                             iters, rs, workspace);
+  return pcg_loop<V, T, false, false>(s, a, lanes, data, dinv, b, x0, tol, atol, maxiter, check_every, x, iters, rs,
+                                      workspace);
+}
+
+// ---------------------------------------------------------------- vectors
+__global__ __launch_bounds__(kBlock) void dot_kernel(int n, const double* __restrict__ a,
+                                                     const double* __restrict__ b, double* partials) {
+  int lb;
+  const Range rg = block_range(n, lb);
+  DVec<1> acc;
+  acc.v[0] = 0.0;
+  for (int i = rg.begin + threadIdx.x; i < rg.end; i += kBlock) acc.v[0] += a[i] * b[i];
+  store_partials<1>(block_sum<1, kBlock>(acc), lb, partials);
+}
+
+__global__ __launch_bounds__(kFinBlock) void dot_fin(const double* __restrict__ partials, int nparts, double* out) {
+  DVec<1> t = sum_partials<1>(partials, nparts);
+  if (threadIdx.x == 0) *out = t.v[0];
+}
+
+__global__ __launch_bounds__(kBlock) void axpy_f32_f64_kernel(int n, double scale, const float* __restrict__ x,
+                                                              double* __restrict__ y) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    y[i] += scale * static_cast<double>(x[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void convert_kernel(int n, double scale, const double* __restrict__ x,
+                                                         float* __restrict__ y) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    y[i] = static_cast<float>(scale * x[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void axpby_kernel(int n, double a, const double* __restrict__ x, double b,
+                                                       const double* __restrict__ y, double* __restrict__ out) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) out[i] = a * x[i] + b * y[i];
+}
+
+// x0 = 0 except x0[bc_rows] = bc_vals - u0[bc_rows] (feax/solvers/common.py:23-53).
+__global__ __launch_bounds__(kBlock) void bc_lift_kernel(int nbc, const int* __restrict__ rows,
+                                                         const double* __restrict__ vals,
+                                                         const double* __restrict__ u0, double* __restrict__ x0) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < nbc) x0[rows[i]] = vals[i] - (u0 ? u0[rows[i]] : 0.0);
+}
+
+}  // namespace fx
+
+using namespace fx;
+
+extern "C" {
+
+int fx_csr_spmv_f64(void* stream, int32_t n, int64_t nse, const int32_t* indptr, const int32_t* indices,
+                    const double* data, const double* x, double* y) {
+  return launch_spmv<double, double, false, false>(as_stream(stream), n, indptr, indices, data, x, y,
+                                                   lanes_for(n, nse, false));
+}
+
+int fx_csr_spmv_f32(void* stream, int32_t n, int64_t nse, const int32_t* indptr, const int32_t* indices,
+                    const float* data, const float* x, float* y) {
+  return launch_spmv<float, float, false, false>(as_stream(stream), n, indptr, indices, data, x, y,
+                                                 lanes_for(n, nse, false));
+}
+
+int fx_csr_block3_index(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, int32_t* bcol) {
+  if (n % 3) return fail(FX_EINVAL, "fx_csr_block3_index: n %% 3 != 0");
+  const int nn = n / 3;
+  if (nn == 0) return FX_OK;
+  hipLaunchKernelGGL(block3_index_kernel, dim3((nn + kBlock / 32 - 1) / (kBlock / 32)), dim3(kBlock), 0,
+                     as_stream(stream), nn, indptr, indices, bcol);
+  FX_LAUNCHED("block3_index_kernel");
+  return FX_OK;
+}
+
+int fx_csr_spmv_b3_f64(void* stream, int32_t n, int64_t nse, const int32_t* indptr, const int32_t* bcol,
+                       const double* data, const double* x, double* y) {
+  if (n % 3) return fail(FX_EINVAL, "spmv_b3: n %% 3 != 0");
+  return launch_spmv<double, double, true, false>(as_stream(stream), n, indptr, bcol, data, x, y,
+                                                  lanes_for(n, nse, true));
+}
+
+int fx_csr_spmv_b3_f32(void* stream, int32_t n, int64_t nse, const int32_t* indptr, const int32_t* bcol,
+                       const float* data, const float* x, float* y) {
+  if (n % 3) return fail(FX_EINVAL, "spmv_b3: n %% 3 != 0");
+  return launch_spmv<float, float, true, false>(as_stream(stream), n, indptr, bcol, data, x, y,
+                                                lanes_for(n, nse, true));
+}
+
+int fx_csr_spmv_diff3_f32f64(void* stream, int32_t n, int64_t nse, const int32_t* indptr, const int32_t* bcol,
+                             const float* data, const double* x, double* y) {
+  if (n % 3) return fail(FX_EINVAL, "spmv_diff3: n %% 3 != 0");
+  return launch_spmv<float, double, true, true>(as_stream(stream), n, indptr, bcol, data, x, y,
+                                                lanes_for(n, nse, true));
+}
+
+int fx_csr_jacobi_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, const double* data,
+                      double shift, double* dinv) {
+  if (n <= 0) return FX_OK;
+  hipLaunchKernelGGL((jacobi_kernel<double>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, as_stream(stream),
+                     n, indptr, indices, data, shift, dinv);
+  FX_LAUNCHED("jacobi_kernel");
+  return FX_OK;
+}
+
+int fx_csr_jacobi_f32(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, const float* data,
+                      double shift, float* dinv) {
+  if (n <= 0) return FX_OK;
+  hipLaunchKernelGGL((jacobi_kernel<float>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, as_stream(stream),
+                     n, indptr, indices, data, shift, dinv);
+  FX_LAUNCHED("jacobi_kernel");
+  return FX_OK;
+}
+
+int fx_csr_diagonal_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, const double* data,
+                        double* diag) {
+  if (n <= 0) return FX_OK;
+  hipLaunchKernelGGL((diag_kernel<double>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, as_stream(stream), n,
+                     indptr, indices, data, diag);
+  FX_LAUNCHED("diag_kernel");
+  return FX_OK;
+}
+
+int fx_csr_diagonal_f32(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, const float* data,
+                        float* diag) {
+  if (n <= 0) return FX_OK;
+  hipLaunchKernelGGL((diag_kernel<float>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, as_stream(stream), n,
+                     indptr, indices, data, diag);
+  FX_LAUNCHED("diag_kernel");
+  return FX_OK;
+}
+
+size_t fx_pcg_workspace_bytes(int32_t n, int32_t elem_bytes) {
+  return kStateBytes + kPartialBytes + 4 * align256(static_cast<size_t>(std::max(n, 0)) * elem_bytes);
+}
+
+int fx_pcg_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr, const int32_t* indices,
+               const int32_t* bcol, const double* data, const double* dinv, const double* b, const double* x0,
+               double tol, double atol, int32_t maxiter, int32_t check_every, double* x, int32_t* iters, double* rs,
+               void* workspace, size_t ws_bytes) {
+  if (flags & FX_PCG_DIFF3) return fail(FX_EINVAL, "fx_pcg_f64: FX_PCG_DIFF3 is for FP32 matrices");
+  return run_pcg<double, double>(stream, n, nse, flags, indptr, indices, bcol, data, dinv, b, x0, tol, atol, maxiter,
+                                 check_every, x, iters, rs, workspace, ws_bytes);
+}
+
+int fx_pcg_f32(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr, const int32_t* indices,
+               const int32_t* bcol, const float* data, const float* dinv, const float* b, const float* x0, double tol,
+               double atol, int32_t maxiter, int32_t check_every, float* x, int32_t* iters, double* rs,
+               void* workspace, size_t ws_bytes) {
+  if (flags & FX_PCG_DIFF3) return fail(FX_EINVAL, "fx_pcg_f32: use fx_pcg_f32f64 for the difference form");
+  return run_pcg<float, float>(stream, n, nse, flags, indptr, indices, bcol, data, dinv, b, x0, tol, atol, maxiter,
+                               check_every, x, iters, rs, workspace, ws_bytes);
+}
+
+int fx_pcg_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr, const int32_t* indices,
+                  const int32_t* bcol, const float* data, const double* dinv, const double* b, const double* x0,
+                  double tol, double atol, int32_t maxiter, int32_t check_every, double* x, int32_t* iters,
+                  double* rs, void* workspace, size_t ws_bytes) {
+  return run_pcg<float, double>(stream, n, nse, flags, indptr, indices, bcol, data, dinv, b, x0, tol, atol, maxiter,
+                                check_every, x, iters, rs, workspace, ws_bytes);
+}
+
+int fx_dot_f64(void* stream, int32_t n, const double* a, const double* b, double* out, void* workspace,
+               size_t ws_bytes) {
+  if (!workspace || ws_bytes < sizeof(double) * kMaxGridBlocks) return fail(FX_EWORKSPACE, "fx_dot_f64: workspace");
+  hipStream_t s = as_stream(stream);
+  const int grid = vec_grid(n);
+  double* partials = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(dot_kernel, dim3(grid), dim3(kBlock), 0, s, n, a, b, partials);
+  hipLaunchKernelGGL(dot_fin, dim3(1), dim3(kFinBlock), 0, s, partials, grid, out);
+  FX_LAUNCHED("dot_kernel");
+  return FX_OK;
+}
+
+int fx_axpy_f32_into_f64(void* stream, int32_t n, double scale, const float* x, double* y) {
+  if (n <= 0) return FX_OK;
+  hipLaunchKernelGGL(axpy_f32_f64_kernel, dim3(vec_grid(n)), dim3(kBlock), 0, as_stream(stream), n, scale, x, y);
+  FX_LAUNCHED("axpy_f32_f64_kernel");
+  return FX_OK;
+}
+
+int fx_convert_f64_to_f32(void* stream, int32_t n, double scale, const double* x, float* y) {
+  if (n <= 0) return FX_OK;
+  hipLaunchKernelGGL(convert_kernel, dim3(vec_grid(n)), dim3(kBlock), 0, as_stream(stream), n, scale, x, y);
+  FX_LAUNCHED("convert_kernel");
+  return FX_OK;
+}
+
+int fx_axpby_f64(void* stream, int32_t n, double a, const double* x, double b, const double* y, double* out) {
+  if (n <= 0) return FX_OK;
+  hipLaunchKernelGGL(axpby_kernel, dim3(vec_grid(n)), dim3(kBlock), 0, as_stream(stream), n, a, x, b, y, out);
+  FX_LAUNCHED("axpby_kernel");
+  return FX_OK;
+}
+
+int fx_bc_lift_f64(void* stream, int32_t n, int32_t nbc, const int32_t* bc_rows, const double* bc_vals,
+                   const double* u0, double* x0) {
+  if (n <= 0) return FX_OK;
+  hipStream_t s = as_stream(stream);
+  FX_HIP(hipMemsetAsync(x0, 0, static_cast<size_t>(n) * sizeof(double), s));
+  if (nbc > 0) {
+    if (!bc_rows || !bc_vals) return fail(FX_EINVAL, "fx_bc_lift_f64: bc arrays missing");
+    hipLaunchKernelGGL(bc_lift_kernel, dim3((nbc + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nbc, bc_rows, bc_vals,
+                       u0, x0);
+    FX_LAUNCHED("bc_lift_kernel");
+  }
+  return FX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+// Host-side structure builders for HEX8, vec = 3 meshes.
+//
+// fx_hex8_pattern produces the CSR pattern that feax builds in
+// Problem._build_csr_assembly_structure (feax/problem.py:316-484): the
+// unique (row, col) pairs of the element DOF blocks, row-major, columns
+// ascending.  For a vec = 3 problem row 3n+i holds columns 3m+j for every
+// node m sharing an element with node n (boundary faces re-use their cell's
+// DOF block, feax/problem.py:195-205, so they add no new pairs).  Instead of
+// feax's argsort over 576(E+F) int64 keys we build it from node adjacency,
+// O(E) memory; the result is the same integer arrays.
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+
+namespace {
+
+// feax local face vertex lists (basix hexahedron facets through the
+// re-order [0,1,3,2,4,5,7,6], feax/basis.py:65-66, 299-322).
+const int kFaceVerts[6][4] = {{0, 1, 3, 2}, {0, 1, 4, 5}, {0, 3, 4, 7},
+                              {1, 2, 5, 6}, {3, 2, 7, 6}, {4, 5, 7, 6}};
+
+}  // namespace
+
+extern "C" int fx_hex8_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells,
+                               int64_t* nse_out, int32_t* max_degree, int32_t* indptr,
+                               int32_t* indices, uint8_t* elem_rank, int32_t* inc_ptr,
+                               int32_t* inc) {
+  if (num_nodes <= 0 || num_cells < 0 || !cells || !nse_out || !indptr)
+    return fx::fail(FX_EINVAL, "fx_hex8_pattern: bad arguments");
+  if (static_cast<int64_t>(num_nodes) * 3 >= INT32_MAX || static_cast<int64_t>(num_cells) * 8 >= INT32_MAX)
+    return fx::fail(FX_ELIMIT, "fx_hex8_pattern: mesh too large for int32 indices");
+  const int64_t E = num_cells, N = num_nodes;
+  // node -> (element, local) incidence, ascending element id per node.
+  std::vector<int32_t> iptr(N + 1, 0);
+  for (int64_t k = 0; k < E * 8; ++k) {
+    int32_t nd = cells[k];
+    if (nd < 0 || nd >= num_nodes) return fx::fail(FX_EINVAL, "fx_hex8_pattern: node id %d out of range", nd);
+    iptr[nd + 1]++;
+  }
+  for (int64_t i = 0; i < N; ++i) iptr[i + 1] += iptr[i];
+  std::vector<int32_t> ilist(E * 8);
+  {
+    std::vector<int32_t> pos(iptr.begin(), iptr.end() - 1);
+    for (int64_t e = 0; e < E; ++e)
+      for (int a = 0; a < 8; ++a) ilist[pos[cells[e * 8 + a]]++] = static_cast<int32_t>(e * 8 + a);
+  }
+  // neighbour lists
+  std::vector<int32_t> nptr(N + 1, 0);
+  std::vector<int32_t> nbr;
+  nbr.reserve(N * 27);
+  std::vector<int32_t> tmp;
+  int32_t maxdeg = 0;
+  for (int64_t nd = 0; nd < N; ++nd) {
+    tmp.clear();
+    for (int32_t k = iptr[nd]; k < iptr[nd + 1]; ++k) {
+      int32_t e = ilist[k] >> 3;
+      for (int b = 0; b < 8; ++b) tmp.push_back(cells[static_cast<int64_t>(e) * 8 + b]);
+    }
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    nbr.insert(nbr.end(), tmp.begin(), tmp.end());
+    nptr[nd + 1] = static_cast<int32_t>(nbr.size());
+    maxdeg = std::max<int32_t>(maxdeg, static_cast<int32_t>(tmp.size()));
+  }
+  if (maxdeg > 255) return fx::fail(FX_ELIMIT, "fx_hex8_pattern: node degree %d > 255", maxdeg);
+  int64_t nse = 0;
+  indptr[0] = 0;
+  for (int64_t nd = 0; nd < N; ++nd) {
+    int64_t deg = nptr[nd + 1] - nptr[nd];
+    for (int i = 0; i < 3; ++i) {
+      nse += 3 * deg;
+      if (nse >= INT32_MAX) return fx::fail(FX_ELIMIT, "fx_hex8_pattern: nse exceeds int32");
+      indptr[nd * 3 + i + 1] = static_cast<int32_t>(nse);
+    }
+  }
+  *nse_out = nse;
+  if (max_degree) *max_degree = maxdeg;
+  if (!indices) return FX_OK;
+  for (int64_t nd = 0; nd < N; ++nd) {
+    for (int i = 0; i < 3; ++i) {
+      int64_t s = indptr[nd * 3 + i];
+      for (int32_t k = nptr[nd]; k < nptr[nd + 1]; ++k)
+        for (int j = 0; j < 3; ++j) indices[s++] = nbr[k] * 3 + j;
+    }
+  }
+  if (elem_rank) {
+    for (int64_t e = 0; e < E; ++e)
+      for (int a = 0; a < 8; ++a) {
+        int32_t nd = cells[e * 8 + a];
+        const int32_t* lo = nbr.data() + nptr[nd];
+        const int32_t* hi = nbr.data() + nptr[nd + 1];
+        for (int b = 0; b < 8; ++b) {
+          const int32_t* it = std::lower_bound(lo, hi, cells[e * 8 + b]);
+          elem_rank[e * 64 + a * 8 + b] = static_cast<uint8_t>(it - lo);
+        }
+      }
+  }
+  if (inc_ptr) std::copy(iptr.begin(), iptr.end(), inc_ptr);
+  if (inc) std::copy(ilist.begin(), ilist.end(), inc);
+  return FX_OK;
+}
+
+extern "C" int fx_hex8_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells,
+                                      int32_t num_faces, const int32_t* faces, int32_t* finc_ptr,
+                                      int32_t* finc) {
+  if (num_nodes <= 0 || num_faces < 0 || !cells || !finc_ptr || (num_faces > 0 && (!faces || !finc)))
+    return fx::fail(FX_EINVAL, "fx_hex8_face_incidence: bad arguments");
+  std::vector<int32_t> cnt(static_cast<size_t>(num_nodes) + 1, 0);
+  for (int32_t f = 0; f < num_faces; ++f) {
+    int32_t e = faces[2 * f], lf = faces[2 * f + 1];
+    if (e < 0 || e >= num_cells || lf < 0 || lf > 5)
+      return fx::fail(FX_EINVAL, "fx_hex8_face_incidence: bad face (%d, %d)", e, lf);
+    for (int v = 0; v < 4; ++v) cnt[cells[static_cast<int64_t>(e) * 8 + kFaceVerts[lf][v]] + 1]++;
+  }
+  for (int32_t i = 0; i < num_nodes; ++i) cnt[i + 1] += cnt[i];
+  std::copy(cnt.begin(), cnt.end(), finc_ptr);
+  std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
+  for (int32_t f = 0; f < num_faces; ++f) {
+    int32_t e = faces[2 * f], lf = faces[2 * f + 1];
+    for (int v = 0; v < 4; ++v) {
+      int a = kFaceVerts[lf][v];
+      finc[pos[cells[static_cast<int64_t>(e) * 8 + a]]++] = f * 8 + a;
+    }
+  }
+  return FX_OK;
+}
+
+extern "C" int fx_csr_is_block3(int32_t n, const int32_t* indptr, const int32_t* indices,
+                                int32_t* is_block3) {
+  if (n < 0 || !indptr || !is_block3) return fx::fail(FX_EINVAL, "fx_csr_is_block3: bad arguments");
+  *is_block3 = 0;
+  if (n % 3) return FX_OK;
+  for (int32_t m = 0; m < n / 3; ++m) {
+    int32_t r0 = indptr[3 * m], len = indptr[3 * m + 1] - r0;
+    if (len % 3 || indptr[3 * m + 2] - indptr[3 * m + 1] != len || indptr[3 * m + 3] - indptr[3 * m + 2] != len)
+      return FX_OK;
+    for (int32_t k = 0; k < len; ++k) {
+      int32_t c = indices[r0 + k];
+      if (indices[r0 + len + k] != c || indices[r0 + 2 * len + k] != c) return FX_OK;
+      if (k % 3 == 0 ? (c % 3 != 0) : (c != indices[r0 + k - 1] + 1)) return FX_OK;
+    }
+  }
+  *is_block3 = 1;
+  return FX_OK;
+}
+
+extern "C" int fx_last_error(char* buf, size_t len) {
+  if (!buf || len == 0) return FX_EINVAL;
+  const std::string& m = fx::last_error();
+  size_t k = std::min(len - 1, m.size());
+  std::memcpy(buf, m.data(), k);
+  buf[k] = 0;
+  return FX_OK;
+}
+
+extern "C" int fx_version(char* buf, size_t len) {
+  if (!buf || len == 0) return FX_EINVAL;
+  snprintf(buf, len, "feaxhip 0.1 gfx950");
+  return FX_OK;
+}

@@ -1,0 +1,100 @@
+"""Constitutive laws and surface loads the HIP element kernels implement.
+
+In feax a Problem subclass returns a JAX function from get_tensor_map()
+(feax/problem.py:595-627) or get_energy_density() (629-661) and the
+element kernel differentiates it with jacfwd.  Here the kernels are
+hand-written, so those hooks return one of the descriptors below.  Each is
+still a plain callable with the same signature as the feax function
+(``stress(u_grad, *params)`` / ``surface_map(u, x, *params)``), evaluated
+with numpy, so user code reads the same; the device path reads its
+parameters.
+"""
+
+import numpy as np
+
+from . import _native as nat
+
+
+class TensorMap:
+    """Base class: a stress law sigma(u_grad, param) known to the kernels."""
+    kind = None
+
+    def material(self, param_loc):
+        raise NotImplementedError
+
+
+class IsotropicLinearElastic(TensorMap):
+    """sigma = lam tr(eps) I + 2 mu eps with lam, mu from (E, nu).
+
+    ``E`` fixed here, or (E=None) taken from TracedParams.volume_vars[0] as a
+    node var (N,) or cell var (E,) -- the tensor map of
+    examples/benchmark/bench_linear_elasticity.py:327-339 and
+    tests/conftest.py:70-76.
+    """
+    kind = nat.FX_MAT_LINEAR_ELASTIC
+
+    def __init__(self, nu=0.3, E=None):
+        self.nu = float(nu)
+        self.E = None if E is None else float(E)
+
+    def __call__(self, u_grad, E=None):
+        E = self.E if E is None else E
+        lam = E * self.nu / ((1 + self.nu) * (1 - 2 * self.nu))
+        mu = E / (2 * (1 + self.nu))
+        eps = 0.5 * (u_grad + np.swapaxes(u_grad, -1, -2))
+        return lam * np.trace(eps, axis1=-2, axis2=-1)[..., None, None] * np.eye(3) + 2 * mu * eps
+
+    def material(self, param_loc):
+        return nat.material_struct(self.kind, param_loc, nu=self.nu)
+
+
+class SIMPLinearElastic(IsotropicLinearElastic):
+    """Linear elastic with E(rho) = Emin + rho^penal (E0 - Emin), rho a cell
+    var: the SIMP law of feax's topology-optimisation examples
+    (feax/solvers/cmg.py:312-323 uses the same law)."""
+    kind = nat.FX_MAT_SIMP
+
+    def __init__(self, nu=0.3, E0=1.0, Emin=1e-9, penal=3.0):
+        super().__init__(nu)
+        self.E0, self.Emin, self.penal = float(E0), float(Emin), float(penal)
+
+    def modulus(self, rho):
+        return self.Emin + np.asarray(rho) ** self.penal * (self.E0 - self.Emin)
+
+    def __call__(self, u_grad, rho):
+        return IsotropicLinearElastic.__call__(self, u_grad, self.modulus(rho))
+
+    def material(self, param_loc):
+        return nat.material_struct(self.kind, param_loc, nu=self.nu, E0=self.E0, Emin=self.Emin,
+                                   penal=self.penal)
+
+
+class CompressibleNeoHookean(TensorMap):
+    """P = mu (F - F^-T) + kappa (J - 1) J F^-T, psi = mu/2 (I1 - 3) - mu ln J +
+    kappa/2 (J - 1)^2, mu a cell var (tests/test_newton_solver.py:65-89)."""
+    kind = nat.FX_MAT_NEO_HOOKEAN
+
+    def __init__(self, kappa=10.0):
+        self.kappa = float(kappa)
+
+    def __call__(self, u_grad, mu):
+        F = np.eye(3) + u_grad
+        J = np.linalg.det(F)
+        FiT = np.swapaxes(np.linalg.inv(F), -1, -2)
+        return mu * (F - FiT) + self.kappa * (J - 1) * J * FiT
+
+    def material(self, param_loc):
+        return nat.material_struct(self.kind, param_loc, kappa=self.kappa)
+
+
+class SurfaceTraction:
+    """Dead traction t = direction * s with s the surface var at each face
+    Gauss point: ``lambda u, x, t: np.array([0., 0., t])`` of
+    tests/conftest.py:78 and the benchmark.  Added to the residual with a +
+    sign (feax/assembler.py:529-533)."""
+
+    def __init__(self, direction=(0.0, 0.0, 1.0)):
+        self.direction = np.asarray(direction, float).reshape(3)
+
+    def __call__(self, u, x, t):
+        return self.direction * t

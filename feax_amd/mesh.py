@@ -1,0 +1,94 @@
+"""Mesh container and the implicit structured HEX8 grid.
+
+Mirrors feax.Mesh (feax/mesh.py:20-184, the container only) and
+feax.StructuredGrid (feax/spgrid.py:28-186): same node / cell numbering and
+the same HEX8 local corner order, so a structured box gives the same
+(points, cells) arrays as in feax.
+"""
+
+import numpy as np
+
+# feax HEX8 local corner offsets (dx, dy, dz) from a cell's base corner
+# (feax/spgrid.py:21-25).
+HEX8_OFFSETS = np.array([(0, 1, 1), (0, 0, 1), (0, 0, 0), (0, 1, 0),
+                         (1, 1, 1), (1, 0, 1), (1, 0, 0), (1, 1, 0)], dtype=np.int64)
+
+
+class Mesh:
+    """points (N, dim) float64, cells (E, nodes_per_cell) int, ele_type."""
+
+    def __init__(self, points, cells, ele_type="HEX8"):
+        self.points = np.asarray(points, dtype=np.float64)
+        self.cells = np.asarray(cells)
+        self.ele_type = ele_type
+
+    @property
+    def num_nodes(self):
+        return len(self.points)
+
+    @property
+    def num_cells(self):
+        return len(self.cells)
+
+
+class StructuredGrid:
+    """Uniform HEX8 grid: cell (cx*ny + cy)*nz + cz, node (i*(ny+1) + j)*(nz+1) + k
+    (feax/spgrid.py:28-41)."""
+
+    ele_type = "HEX8"
+
+    def __init__(self, dims, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0)):
+        self.nx, self.ny, self.nz = (int(d) for d in dims)
+        self.spacing = np.asarray(spacing, float)
+        self.origin = np.asarray(origin, float)
+
+    @property
+    def num_cells(self):
+        return self.nx * self.ny * self.nz
+
+    @property
+    def num_nodes(self):
+        return (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
+
+    def cell_ijk(self, cell_ids):
+        e = np.asarray(cell_ids, np.int64)
+        nynz = self.ny * self.nz
+        cx, r = e // nynz, e % nynz
+        return cx, r // self.nz, r % self.nz
+
+    def cell_to_nodes(self, cell_ids):
+        """(n, 8) global node ids in feax HEX8 local order (spgrid.py:115-123)."""
+        cx, cy, cz = self.cell_ijk(cell_ids)
+        ny1, nz1 = self.ny + 1, self.nz + 1
+        I = cx[:, None] + HEX8_OFFSETS[None, :, 0]
+        J = cy[:, None] + HEX8_OFFSETS[None, :, 1]
+        K = cz[:, None] + HEX8_OFFSETS[None, :, 2]
+        return (I * ny1 + J) * nz1 + K
+
+    def node_coords(self, node_ids):
+        n = np.asarray(node_ids, np.int64)
+        ny1, nz1 = self.ny + 1, self.nz + 1
+        i, r = n // (ny1 * nz1), n % (ny1 * nz1)
+        j, k = r // nz1, r % nz1
+        return self.origin + np.stack([i, j, k], axis=-1) * self.spacing
+
+    def node_id(self, i, j, k):
+        i, j, k = (np.asarray(v, np.int64) for v in (i, j, k))
+        return (i * (self.ny + 1) + j) * (self.nz + 1) + k
+
+    def cell_id(self, ex, ey, ez):
+        ex, ey, ez = (np.asarray(v, np.int64) for v in (ex, ey, ez))
+        return (ex * self.ny + ey) * self.nz + ez
+
+    def cell_centroids(self, cell_ids):
+        cx, cy, cz = self.cell_ijk(cell_ids)
+        return self.origin + (np.stack([cx, cy, cz], axis=-1) + 0.5) * self.spacing
+
+    def to_mesh(self, cell_ids=None):
+        """Explicit Mesh of ``cell_ids`` (default all), nodes renumbered by
+        unique + searchsorted (feax/spgrid.py:156-170)."""
+        ids = (np.arange(self.num_cells, dtype=np.int64) if cell_ids is None
+               else np.asarray(cell_ids, np.int64).ravel())
+        cells = self.cell_to_nodes(ids)
+        used = np.unique(cells)
+        return Mesh(self.node_coords(used), np.searchsorted(used, cells), ele_type=self.ele_type)

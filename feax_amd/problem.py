@@ -1,0 +1,238 @@
+"""Problem: FE structure for HEX8, vec = 3 problems (the feax.Problem API).
+
+Mirrors feax.Problem (feax/problem.py:37-846): constructor arguments,
+``location_fns`` / ``boundary_inds_list``, ``matrix_view``, the physics
+hooks (get_tensor_map, get_energy_density, get_surface_maps, get_mass_map,
+custom_init) and the CSR structure attributes (csr_nse, csr_indptr,
+csr_indices, csr_row_of_slot, csr_diag_slots, I / J).
+
+Difference by design: the CSR pattern is built by the native host routine
+fx_hex8_pattern from node adjacency (O(E) memory) instead of an argsort
+over 576(E+F) int64 keys (feax/problem.py:386-423, ~10 GB at 1M DOF); the
+integer arrays are the same.  The nnz-sized raw COO arrays (I, J) are only
+materialised if asked for.
+"""
+
+import ctypes
+from dataclasses import dataclass
+from enum import Enum
+from typing import Any, Callable, List, Optional, Tuple, Union
+
+import numpy as np
+
+from . import _native as nat
+from .fe import NUM_FACE_QUADS, NUM_QUADS, boundary_conditions_inds
+from .mesh import Mesh
+
+
+class MatrixView(Enum):
+    """Which entries the assembled matrix keeps (feax/problem.py:22-35)."""
+    FULL = 0
+    UPPER = 1
+    LOWER = 2
+
+
+class FiniteElement:
+    """The per-variable facts feax.FiniteElement exposes that callers read
+    (feax/fe.py:20-94): counts only; geometry lives in the kernels."""
+
+    def __init__(self, mesh, vec, dim):
+        self.mesh = mesh
+        self.points = mesh.points
+        self.cells = mesh.cells
+        self.vec = vec
+        self.dim = dim
+        self.num_cells = len(mesh.cells)
+        self.num_total_nodes = len(mesh.points)
+        self.num_total_dofs = self.num_total_nodes * vec
+        self.num_nodes = 8
+        self.num_quads = NUM_QUADS
+        self.num_face_quads = NUM_FACE_QUADS
+        self.num_faces = 6
+
+
+@dataclass
+class Problem:
+    """HEX8 vector problem (vec = 3, dim = 3) on an explicit mesh."""
+    mesh: Union[Mesh, List[Mesh]]
+    vec: Union[int, List[int]]
+    dim: int
+    ele_type: Union[str, List[str]] = "HEX8"
+    gauss_order: Optional[Union[int, List[int]]] = None
+    location_fns: Optional[List[Callable]] = None
+    matrix_view: Union[MatrixView, str] = MatrixView.FULL
+    additional_info: Tuple[Any, ...] = ()
+
+    def __post_init__(self):
+        if isinstance(self.matrix_view, str):
+            try:
+                self.matrix_view = MatrixView[self.matrix_view.upper()]
+            except KeyError:
+                raise ValueError(f"Invalid matrix_view: {self.matrix_view}. Must be 'FULL', 'UPPER', or 'LOWER'")
+        if isinstance(self.mesh, list):
+            if len(self.mesh) != 1:
+                raise NotImplementedError("multi-variable problems are not on the HIP path")
+            self.mesh, self.vec = self.mesh[0], self.vec[0]
+            self.ele_type = self.ele_type[0] if isinstance(self.ele_type, list) else self.ele_type
+            self.gauss_order = self.gauss_order[0] if isinstance(self.gauss_order, list) else self.gauss_order
+        if self.ele_type != "HEX8" or self.vec != 3 or self.dim != 3:
+            raise NotImplementedError(
+                f"the HIP path covers HEX8 with vec=3, dim=3 (got {self.ele_type}, vec={self.vec}, dim={self.dim})")
+        if self.gauss_order not in (None, 2):
+            raise NotImplementedError("HEX8 uses the 2x2x2 Gauss rule (gauss_order=2)")
+        self.num_vars = 1
+        self.fes = [FiniteElement(self.mesh, self.vec, self.dim)]
+        self.points = np.ascontiguousarray(self.mesh.points, dtype=np.float64)
+        self.cells = np.ascontiguousarray(self.mesh.cells, dtype=np.int32)
+        if self.cells.ndim != 2 or self.cells.shape[1] != 8:
+            raise ValueError(f"HEX8 cells must be (E, 8), got {self.cells.shape}")
+        self.cells_list = [self.cells]
+        self.num_cells = len(self.cells)
+        self.num_total_nodes = len(self.points)
+        self.num_total_dofs_all_vars = 3 * self.num_total_nodes
+        self.offset = [0]
+        self.boundary_inds_list = boundary_conditions_inds(self.points, self.cells, self.location_fns)
+        self._build_csr_assembly_structure()
+        self.custom_init(*self.additional_info)
+
+    # ------------------------------------------------------------------ structure
+    def _build_csr_assembly_structure(self):
+        """CSR pattern + assembly maps (feax/problem.py:316-484) via fx_hex8_pattern."""
+        N, E = self.num_total_nodes, self.num_cells
+        n = 3 * N
+        L = nat.lib()
+        nse = ctypes.c_int64(0)
+        maxdeg = ctypes.c_int32(0)
+        indptr = np.zeros(n + 1, dtype=np.int32)
+        nat.check(L.fx_hex8_pattern(N, E, nat.ptr(self.cells), ctypes.addressof(nse), ctypes.addressof(maxdeg),
+                                    nat.ptr(indptr), None, None, None, None), "fx_hex8_pattern")
+        indices = np.empty(nse.value, dtype=np.int32)
+        rank = np.empty(E * 64, dtype=np.uint8)
+        inc_ptr = np.empty(N + 1, dtype=np.int32)
+        inc = np.empty(E * 8, dtype=np.int32)
+        nat.check(L.fx_hex8_pattern(N, E, nat.ptr(self.cells), ctypes.addressof(nse), ctypes.addressof(maxdeg),
+                                    nat.ptr(indptr), nat.ptr(indices), nat.ptr(rank), nat.ptr(inc_ptr),
+                                    nat.ptr(inc)), "fx_hex8_pattern")
+        self.csr_nse = int(nse.value)
+        self.csr_indptr = indptr
+        self.csr_indices = indices
+        self.csr_max_degree = int(maxdeg.value)
+        self.elem_rank = rank
+        self.node_inc_ptr = inc_ptr
+        self.node_inc = inc
+        self.csr_structurally_symmetric = True   # node adjacency is symmetric
+        # Loaded faces of all boundaries, concatenated in location_fns order
+        # (the volume-then-boundaries order of feax/problem.py:195-209).
+        faces = [b for b in self.boundary_inds_list]
+        self.face_offsets = np.cumsum([0] + [len(b) for b in faces])
+        allf = (np.concatenate(faces) if faces else np.zeros((0, 2), np.int64)).astype(np.int32)
+        self.all_faces = np.ascontiguousarray(allf)
+        finc_ptr = np.zeros(N + 1, dtype=np.int32)
+        finc = np.zeros(max(4 * len(allf), 1), dtype=np.int32)
+        nat.check(L.fx_hex8_face_incidence(N, E, nat.ptr(self.cells), len(allf), nat.ptr(self.all_faces),
+                                           nat.ptr(finc_ptr), nat.ptr(finc)), "fx_hex8_face_incidence")
+        self.face_inc_ptr, self.face_inc = finc_ptr, finc
+        if self.csr_max_degree > 64:
+            raise NotImplementedError(f"node degree {self.csr_max_degree} > 64 not supported by the assembly kernel")
+
+    @property
+    def csr_row_of_slot(self):
+        """Row of each CSR slot (nse,) int32 (feax/problem.py:436)."""
+        if getattr(self, "_row_of_slot", None) is None:
+            self._row_of_slot = np.repeat(np.arange(self.num_total_dofs_all_vars, dtype=np.int32),
+                                          np.diff(self.csr_indptr))
+        return self._row_of_slot
+
+    @property
+    def csr_diag_slots(self):
+        """Slot of (d, d) for every row, -1 if absent (feax/problem.py:372-378)."""
+        if getattr(self, "_diag_slots", None) is None:
+            rows = self.csr_row_of_slot
+            pos = np.nonzero(rows == self.csr_indices)[0]
+            d = np.full(self.num_total_dofs_all_vars, -1, dtype=np.int32)
+            d[rows[pos]] = pos
+            self._diag_slots = d
+        return self._diag_slots
+
+    def _element_inds(self, cells):
+        c = np.asarray(cells, np.int64)
+        return (3 * c[:, :, None] + np.arange(3)[None, None, :]).reshape(len(c), -1).astype(np.int32)
+
+    def _coo(self):
+        inds = [self._element_inds(self.cells)] + [self._element_inds(self.cells[b[:, 0]])
+                                                   for b in self.boundary_inds_list]
+        I = np.concatenate([np.repeat(i[:, :, None], 24, axis=2).reshape(-1) for i in inds])
+        J = np.concatenate([np.repeat(i[:, None, :], 24, axis=1).reshape(-1) for i in inds])
+        return I, J
+
+    @property
+    def I(self):
+        """Raw COO rows, volume then faces (feax/problem.py:183-212); built on demand."""
+        if getattr(self, "_I", None) is None:
+            self._I, self._J = self._coo()
+        return self._I
+
+    @property
+    def J(self):
+        if getattr(self, "_J", None) is None:
+            self._I, self._J = self._coo()
+        return self._J
+
+    def _filter_mask(self):
+        if self.matrix_view == MatrixView.UPPER:
+            return self.J >= self.I
+        if self.matrix_view == MatrixView.LOWER:
+            return self.J <= self.I
+        return None
+
+    @property
+    def I_filtered(self):
+        m = self._filter_mask()
+        return self.I if m is None else self.I[m]
+
+    @property
+    def J_filtered(self):
+        m = self._filter_mask()
+        return self.J if m is None else self.J[m]
+
+    @property
+    def filter_indices(self):
+        m = self._filter_mask()
+        return None if m is None else np.where(m)[0].astype(np.int32)
+
+    def view_pattern(self):
+        """(indptr, indices, full_slot_of_view_slot) of the matrix_view
+        pattern: the FULL pattern filtered to j >= i (UPPER) or j <= i
+        (LOWER), which is what feax's dedupe of the filtered (I, J) gives."""
+        if self.matrix_view == MatrixView.FULL:
+            return self.csr_indptr, self.csr_indices, None
+        rows = self.csr_row_of_slot
+        keep = self.csr_indices >= rows if self.matrix_view == MatrixView.UPPER else self.csr_indices <= rows
+        slots = np.nonzero(keep)[0].astype(np.int32)
+        counts = np.bincount(rows[slots], minlength=self.num_total_dofs_all_vars)
+        indptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+        return indptr, self.csr_indices[slots], slots
+
+    # ------------------------------------------------------------------ layout
+    def unflatten_fn_sol_list(self, flat):
+        flat = getattr(flat, "dofs", flat)
+        return [flat.reshape(self.num_total_nodes, 3)]
+
+    # ------------------------------------------------------------------ hooks
+    def custom_init(self, *args):
+        pass
+
+    def get_tensor_map(self):
+        return None
+
+    def get_energy_density(self):
+        return None
+
+    def get_surface_maps(self):
+        return []
+
+    def get_mass_map(self):
+        return None
+
+    def get_weak_form(self):
+        return None

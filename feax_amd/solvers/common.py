@@ -1,0 +1,164 @@
+"""Krylov building blocks on the device (feax/solvers/common.py API).
+
+create_iterative_solve_fn(options) -> solve(A, b, x0) runs the HIP
+Jacobi-PCG (fx_pcg_f64 / fx_pcg_f32) with jax.scipy.sparse.linalg.cg
+stopping semantics (feax/solvers/common.py:213-239).  Unlike feax's
+matrix-free default, ``use_jacobi_preconditioner`` is honoured (the
+operator here is the assembled CSR).
+"""
+
+import ctypes
+
+import numpy as np
+
+from .. import _native as nat
+from ..csr import CSRMatrix
+
+_WS = {}
+
+
+def _workspace(n, dtype, device):
+    """Cached device scratch for the PCG driver (sized by fx_pcg_workspace_bytes)."""
+    import torch
+    esz = 8 if dtype == torch.float64 else 4
+    need = int(nat.lib().fx_pcg_workspace_bytes(n, esz))
+    key = (str(device), esz)
+    ws = _WS.get(key)
+    if ws is None or ws.numel() < need:
+        ws = torch.empty(need, dtype=torch.uint8, device=device)
+        _WS[key] = ws
+    return ws
+
+
+class PCGInfo:
+    def __init__(self, iters, rs):
+        self.iters = int(iters)
+        self.rs = float(rs)
+
+    def __repr__(self):
+        return f"PCGInfo(iters={self.iters}, rs={self.rs:.3e})"
+
+
+def pcg(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=None, block3=None,
+        check_every=32, out=None, vec_dtype=None):
+    """Jacobi-PCG on the device; returns (x, PCGInfo).
+
+    Vectors are in the matrix precision, except for a difference-form FP32
+    matrix (A.form == "diff3") or vec_dtype=float64 with an FP32 matrix,
+    which run FP64 vectors (fx_pcg_f32f64).
+    """
+    import torch
+    n = A.shape[0]
+    mdt = A.data.dtype
+    if A.form == "diff3":
+        vec_dtype = torch.float64
+    vdt = mdt if vec_dtype is None else vec_dtype
+    b = b.to(vdt).contiguous()
+    x = torch.empty(n, dtype=vdt, device=A.data.device) if out is None else out
+    x0 = None if x0 is None else x0.to(vdt).contiguous()
+    dinv = None if dinv is None else dinv.to(vdt).contiguous()
+    if A.form == "diff3":
+        flags = nat.FX_PCG_DIFF3
+    else:
+        use_b3 = A.block3 if block3 is None else (bool(block3) and A.block3)
+        flags = nat.FX_PCG_BLOCK3 if use_b3 else 0
+    bcol = A.bcol if flags else None
+    ws = _workspace(n, vdt, A.data.device)
+    iters = ctypes.c_int32(0)
+    rs = ctypes.c_double(0.0)
+    if mdt == torch.float64:
+        name = "fx_pcg_f64"
+    elif vdt == torch.float64:
+        name = "fx_pcg_f32f64"
+    else:
+        name = "fx_pcg_f32"
+    nat.call(name, nat.stream(), n, A.nse, flags, nat.ptr(A.indptr), nat.ptr(A.indices), nat.ptr(bcol),
+             nat.ptr(A.data), nat.ptr(dinv), nat.ptr(b), nat.ptr(x0), float(tol), float(atol), int(maxiter),
+             int(check_every), nat.ptr(x), ctypes.addressof(iters), ctypes.addressof(rs), nat.ptr(ws), ws.numel())
+    return x, PCGInfo(iters.value, rs.value)
+
+
+def create_jacobi_preconditioner(A: CSRMatrix, shift=1e-12):
+    """dinv = 1 / (diag + shift) (feax/solvers/common.py:86-94), as a device vector."""
+    import torch
+    d = torch.empty(A.shape[0], dtype=A.data.dtype, device=A.data.device)
+    name = "fx_csr_jacobi_f64" if A.data.dtype == torch.float64 else "fx_csr_jacobi_f32"
+    nat.call(name, nat.stream(), A.shape[0], nat.ptr(A.indptr), nat.ptr(A.indices), nat.ptr(A.data),
+             float(shift), nat.ptr(d))
+    return d
+
+
+def create_x0(bc_rows=None, bc_vals=None):
+    """x0_fn(current_sol) lifting BC increments (feax/solvers/common.py:23-53)."""
+    import torch
+
+    def x0_fn(current_sol):
+        x0 = torch.empty_like(current_sol, dtype=torch.float64)
+        if bc_rows is None or bc_vals is None or len(bc_rows) == 0:
+            nat.call("fx_bc_lift_f64", nat.stream(), x0.numel(), 0, None, None, None, nat.ptr(x0))
+            return x0
+        rows = torch.as_tensor(np.asarray(bc_rows, np.int32), device=x0.device) \
+            if not isinstance(bc_rows, torch.Tensor) else bc_rows.to(torch.int32)
+        vals = torch.as_tensor(np.asarray(bc_vals, np.float64), device=x0.device) \
+            if not isinstance(bc_vals, torch.Tensor) else bc_vals.to(torch.float64)
+        u0 = current_sol.to(torch.float64).contiguous()
+        nat.call("fx_bc_lift_f64", nat.stream(), x0.numel(), len(rows), nat.ptr(rows.contiguous()),
+                 nat.ptr(vals.contiguous()), nat.ptr(u0), nat.ptr(x0))
+        return x0
+    return x0_fn
+
+
+def device_norm(v):
+    """||v||_2 in FP64 with the fixed-order device reduction."""
+    import torch
+    v = v.to(torch.float64).contiguous()
+    out = torch.empty(1, dtype=torch.float64, device=v.device)
+    ws = _dot_ws(v.device)
+    nat.call("fx_dot_f64", nat.stream(), v.numel(), nat.ptr(v), nat.ptr(v), nat.ptr(out), nat.ptr(ws), ws.numel())
+    return float(out.cpu()[0]) ** 0.5
+
+
+_DOT_WS = {}
+
+
+def _dot_ws(device):
+    import torch
+    k = str(device)
+    if k not in _DOT_WS:
+        _DOT_WS[k] = torch.empty(8 * 2048 + 256, dtype=torch.uint8, device=device)
+    return _DOT_WS[k]
+
+
+def check_convergence(A, x, b, solver_options, matrix_view=None, solver_label="Linear solver"):
+    """NaN-fill unless ||Ax - b|| / (||b|| + 1e-12) < threshold (common.py:376-395)."""
+    import torch
+    Ax = A(x) if callable(A) and not isinstance(A, CSRMatrix) else A @ x
+    r = torch.empty_like(b, dtype=torch.float64)
+    nat.call("fx_axpby_f64", nat.stream(), r.numel(), 1.0, nat.ptr(Ax.to(torch.float64).contiguous()), -1.0,
+             nat.ptr(b.to(torch.float64).contiguous()), nat.ptr(r))
+    rel = device_norm(r) / (device_norm(b) + 1e-12)
+    if getattr(solver_options, "verbose", False):
+        print(f"{solver_label} residual: rel={rel:.6e}")
+    if rel < getattr(solver_options, "convergence_threshold", 0.1):
+        return x
+    return torch.full_like(x, float("nan"))
+
+
+def create_iterative_solve_fn(options):
+    """solve(A, b, x0) -> x for KrylovSolverOptions (common.py:213-270)."""
+    if options.solver == "auto":
+        raise ValueError("KrylovSolverOptions.solver is 'auto' (unresolved). "
+                         "Call resolve_iterative_solver() before create_iterative_solve_fn().")
+    if options.solver != "cg":
+        raise NotImplementedError(f"{options.solver} is not on the HIP path (cg only)")
+    if options.preconditioner is not None:
+        raise NotImplementedError("custom preconditioners are not on the HIP path (Jacobi only)")
+
+    def solve(A, b, x0):
+        dinv = create_jacobi_preconditioner(A, options.jacobi_shift) if options.use_jacobi_preconditioner else None
+        b3 = {"auto": None, "csr": False, "block3": True}[options.spmv]
+        x, info = pcg(A, b, x0, options.tol, options.atol, options.maxiter, dinv, b3, options.check_every)
+        solve.last_info = info
+        return x
+    solve.last_info = None
+    return solve

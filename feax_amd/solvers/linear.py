@@ -1,0 +1,171 @@
+"""Differentiable linear solve (feax/solvers/linear.py:166-499 API).
+
+create_linear_solver(problem, bc, solver_options, ...) returns
+``solver(traced_params, initial_guess=None, bc=None, traced_structure=None)``
+computing u = u0 + du with J_bc du = -R_bc(u0) (linear.py:387-417):
+
+* precision "fp64": FP64 CSR assembly with the Dirichlet elimination fused
+  into the store, FP64 right-hand side from the residual kernel, HIP
+  Jacobi-PCG.
+* precision "mixed": FP32 CSR assembly and FP32 Jacobi-PCG (FP64 dot
+  products) as the inner solver of an FP64 defect correction
+  u <- u + K32^-1 (b - K u), where b - K u is the FP64 matrix-free residual
+  kernel (never the FP32 matrix), until ||b - K u|| <= max(tol ||b||, atol)
+  (Goeddeke, Strzodka, Turek 2007).
+
+Gradients (torch.autograd): the adjoint of feax's f_bwd (linear.py:426-481):
+lambda = J_bc^-1 v (J_bc symmetric under symmetric elimination), then
+-(dR_bc/dparams)^T lambda from the hand-written VJP kernels.
+"""
+
+import numpy as np
+
+from .. import _native as nat
+from ..assembler import TracedStructure, assemble_csr, residual
+from ..DCboundary import DirichletBC
+from .common import (create_iterative_solve_fn, create_jacobi_preconditioner, create_x0, device_norm, pcg,
+                     check_convergence)
+from .options import KrylovSolverOptions, MatrixProperty, resolve_iterative_solver
+
+
+class SolveStats:
+    """What the last solve did (iterations, residuals, outer steps)."""
+
+    def __init__(self):
+        self.iters = 0
+        self.outer = 0
+        self.rs = 0.0
+        self.relres = float("nan")
+        self.inner_iters = []
+
+    def __repr__(self):
+        return (f"SolveStats(iters={self.iters}, outer={self.outer}, relres={self.relres:.3e}, "
+                f"inner={self.inner_iters})")
+
+
+def _b3(options):
+    return {"auto": None, "csr": False, "block3": True}[options.spmv]
+
+
+def solve_fp64(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None):
+    """One FP64 solve on the assembled CSR.  Returns u (FP64 device vector)."""
+    import torch
+    if opts.use_jacobi_preconditioner:
+        J, dinv = assemble_csr(problem, ts, tp, u0, bc, symmetric, torch.float64, jacobi_shift=opts.jacobi_shift)
+    else:
+        J, dinv = assemble_csr(problem, ts, tp, u0, bc, symmetric, torch.float64), None
+    b = residual(problem, ts, tp, u0, bc, negate=True)
+    x0_fn = opts.x0_fn if opts.x0_fn is not None else create_x0(bc.bc_rows, bc.bc_vals)
+    x0 = x0_fn(u0)
+    du, info = pcg(J, b, x0, opts.tol, opts.atol, opts.maxiter, dinv, _b3(opts), opts.check_every)
+    if opts.check_convergence:
+        du = check_convergence(J, du, b, opts)
+    if stats is not None:
+        stats.iters, stats.rs, stats.outer = info.iters, info.rs, 1
+        stats.inner_iters = [info.iters]
+    u = torch.empty_like(u0)
+    nat.call("fx_axpby_f64", nat.stream(), u.numel(), 1.0, nat.ptr(u0), 1.0, nat.ptr(du), nat.ptr(u))
+    return u
+
+
+def solve_mixed(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None):
+    """FP32 matrix inside an FP64 defect correction.  Returns u (FP64).
+
+    precision "mixed": FP32 difference-form CSR (FX_ASM_DIFF3) applied to
+    FP64 CG vectors.  precision "mixed-fp32vec": FP32 CSR and FP32 CG
+    vectors (FP64 dot products) on the residual scaled to unit norm.
+    The outer residual b - K u is always the FP64 matrix-free kernel.
+    """
+    import torch
+    n = ts.n
+    fp32vec = opts.precision == "mixed-fp32vec"
+    shift = opts.jacobi_shift if opts.use_jacobi_preconditioner else None
+    out = assemble_csr(problem, ts, tp, u0, bc, symmetric, torch.float32, diff_form=not fp32vec, jacobi_shift=shift)
+    A, dinv = out if shift is not None else (out, None)
+    if fp32vec and dinv is not None:
+        dinv = dinv.to(torch.float32)
+    u = u0.clone()
+    r = residual(problem, ts, tp, u, bc, negate=True)             # b - K u0 (FP64)
+    bnorm = device_norm(r)
+    target = max(opts.tol * bnorm, opts.atol)
+    vdt = torch.float32 if fp32vec else torch.float64
+    rv = torch.empty(n, dtype=vdt, device=u.device) if fp32vec else r
+    d = torch.empty(n, dtype=vdt, device=u.device)
+    rn = bnorm
+    inner, total, outer = [], 0, 0
+    budget = opts.maxiter
+    while rn > target and outer < opts.max_outer and budget > 0:
+        if fp32vec:
+            nat.call("fx_convert_f64_to_f32", nat.stream(), n, 1.0 / rn, nat.ptr(r), nat.ptr(rv))
+        _, info = pcg(A, rv, None, opts.inner_tol, 0.0, budget, dinv, True, opts.check_every, out=d)
+        if fp32vec:
+            nat.call("fx_axpy_f32_into_f64", nat.stream(), n, rn, nat.ptr(d), nat.ptr(u))
+        else:
+            nat.call("fx_axpby_f64", nat.stream(), n, 1.0, nat.ptr(u), 1.0, nat.ptr(d), nat.ptr(u))
+        residual(problem, ts, tp, u, bc, negate=True, out=r)
+        rn = device_norm(r)
+        inner.append(info.iters)
+        total += info.iters
+        budget -= info.iters
+        outer += 1
+    if stats is not None:
+        stats.iters, stats.outer, stats.inner_iters = total, outer, inner
+        stats.rs = rn * rn
+        stats.relres = rn / bnorm if bnorm > 0 else 0.0
+    if opts.check_convergence and not (rn / (bnorm + 1e-12) < opts.convergence_threshold):
+        u = torch.full_like(u, float("nan"))
+    return u
+
+
+def _dev_vec(a, ts):
+    import torch
+    a = getattr(a, "dofs", a)
+    if isinstance(a, torch.Tensor):
+        return a.to(device=ts.device, dtype=torch.float64).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a, np.float64), device=ts.device)
+
+
+def create_linear_solver(problem, bc, solver_options=None, adjoint_solver_options=None, traced_params=None,
+                         symmetric_elimination=True, traced_structure=None):
+    if solver_options is None:
+        solver_options = KrylovSolverOptions()
+    if not isinstance(solver_options, KrylovSolverOptions):
+        raise NotImplementedError("only KrylovSolverOptions (cg) are on the HIP path; direct solvers are not")
+    solver_options = resolve_iterative_solver(solver_options, MatrixProperty.SPD)
+    if adjoint_solver_options is None:
+        adjoint_solver_options = solver_options
+    else:
+        adjoint_solver_options = resolve_iterative_solver(adjoint_solver_options, MatrixProperty.SPD)
+    create_iterative_solve_fn(solver_options)          # validates the options
+    default_bc = bc
+    ts_default = traced_structure
+    stats = SolveStats()
+
+    def forward(tp, ig, eff_bc, ts):
+        if ts is None:
+            ts = getattr(problem, "_default_ts", None) or TracedStructure.from_problem(problem)
+            problem._default_ts = ts
+        u0 = _dev_vec(ig, ts) if ig is not None else _zero_like(problem, eff_bc, ts)
+        if solver_options.precision != "fp64":
+            return solve_mixed(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats)
+        return solve_fp64(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats)
+
+    def solver_wrapper(traced_params, initial_guess=None, bc=None, traced_structure=None):
+        eff_bc = bc if isinstance(bc, DirichletBC) else default_bc
+        ts = traced_structure if traced_structure is not None else ts_default
+        from .adjoint import differentiable_linear_solve
+        return differentiable_linear_solve(problem, forward, traced_params, initial_guess, eff_bc, ts,
+                                           adjoint_solver_options, symmetric_elimination)
+
+    solver_wrapper.stats = stats
+    return solver_wrapper
+
+
+def _zero_like(problem, bc, ts):
+    """zeros with u[bc_rows] = bc_vals (feax/utils.py:464-499), on the device."""
+    import torch
+    u = torch.empty(ts.n, dtype=torch.float64, device=ts.device)
+    nbc = len(bc.bc_rows)
+    rows, _, vals = bc.device(ts.device) if nbc else (None, None, None)
+    nat.call("fx_bc_lift_f64", nat.stream(), ts.n, nbc, nat.ptr(rows), nat.ptr(vals), None, nat.ptr(u))
+    return u

@@ -1,0 +1,92 @@
+"""Solver option dataclasses (the feax.solvers.options API).
+
+Field names and defaults follow feax/solvers/options.py:444-512 and
+783-889.  Added fields (all defaulted, so feax call sites work unchanged):
+
+KrylovSolverOptions
+  precision  "fp64" (default; assembly, SpMV and CG in FP64 like feax's
+             FEAX_X64 default); "mixed" (FP32 assembly and FP32 matrix in
+             difference form, FP64 CG vectors and dot products, inside an
+             FP64 defect correction whose residual b - K u is the FP64
+             matrix-free residual kernel); "mixed-fp32vec" (same with FP32
+             CG vectors).
+  spmv       "auto" | "csr" | "block3": SpMV kernel for the CG iterations.
+  check_every  iterations between host reads of the convergence flag.
+  inner_tol, max_outer  mixed-precision inner CG tolerance and outer cap.
+"""
+
+from dataclasses import dataclass
+from enum import Enum
+from typing import Callable, Optional
+import logging
+
+logger = logging.getLogger(__name__)
+
+
+class MatrixProperty(Enum):
+    SPD = "spd"
+    SYMMETRIC = "symmetric"
+    GENERAL = "general"
+
+
+@dataclass(frozen=True)
+class AbstractSolverOptions:
+    verbose: bool = False
+    check_convergence: bool = False
+    convergence_threshold: float = 0.1
+
+    def uses_x0(self) -> bool:
+        raise NotImplementedError
+
+
+@dataclass(frozen=True)
+class NewtonOptions:
+    tol: float = 1e-6
+    rel_tol: float = 1e-8
+    max_iter: int = 100
+    line_search_max_backtracks: int = 30
+    line_search_c1: float = 1e-4
+    line_search_rho: float = 0.5
+    raise_on_line_search_failure: bool = True
+    verbose: bool = False
+
+
+@dataclass(frozen=True)
+class KrylovSolverOptions(AbstractSolverOptions):
+    solver: str = "auto"
+    tol: float = 1e-10
+    atol: float = 1e-10
+    maxiter: int = 10000
+    preconditioner: Optional[Callable] = None
+    use_jacobi_preconditioner: bool = False
+    jacobi_shift: float = 1e-12
+    restart: Optional[int] = None
+    x0_fn: Optional[Callable] = None
+    precision: str = "fp64"
+    spmv: str = "auto"
+    check_every: int = 32
+    inner_tol: float = 1e-5
+    max_outer: int = 30
+
+    def __post_init__(self):
+        valid = ("auto", "cg", "bicgstab", "gmres")
+        if self.solver not in valid:
+            raise ValueError(f"Invalid iterative solver: {self.solver}. Choose from {valid}")
+        if self.precision not in ("fp64", "mixed", "mixed-fp32vec"):
+            raise ValueError(f"Invalid precision: {self.precision}. Choose from ('fp64', 'mixed', 'mixed-fp32vec')")
+        if self.spmv not in ("auto", "csr", "block3"):
+            raise ValueError(f"Invalid spmv: {self.spmv}. Choose from ('auto', 'csr', 'block3')")
+
+    def uses_x0(self) -> bool:
+        return True
+
+
+def resolve_iterative_solver(options, matrix_property):
+    """'auto' -> cg / bicgstab / gmres by matrix property (options.py:837-889)."""
+    if options.solver != "auto":
+        return options
+    name = {MatrixProperty.SPD: "cg", MatrixProperty.SYMMETRIC: "bicgstab",
+            MatrixProperty.GENERAL: "gmres"}[matrix_property]
+    import dataclasses
+    logger.info(f"IterativeSolver auto: matrix_property={matrix_property.name} -> {name}")
+    return dataclasses.replace(options, solver=name)

@@ -1,0 +1,222 @@
+/*
+ * feaxhip.h -- C ABI of libfeaxhip.so, the MI355X (gfx950) implementation of
+ * feax's HEX8 assembly + Krylov hot path.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success, else a hipError_t value or one
+ *     of the FX_E* codes below; fx_last_error() returns the message of the
+ *     last failure on the calling thread.
+ *   - Device entry points are stream-ordered on the hipStream_t passed as
+ *     `stream` (void* here so the header needs no HIP include), own no
+ *     buffers and allocate nothing: scratch comes in through `workspace`
+ *     whose size is given by the matching *_workspace_bytes() query.
+ *   - Only the PCG drivers synchronise the stream (to read the convergence
+ *     flag every `check_every` iterations).  Everything else is async.
+ *   - Index arrays are int32 (feax keeps all structure in int32,
+ *     feax/problem.py:236-244, 441-451).  Meshes are HEX8 with vec = 3.
+ *   - (d) marks device pointers, (h) host pointers.
+ */
+#ifndef FEAXHIP_H
+#define FEAXHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FX_OK 0
+#define FX_EINVAL 1001     /* bad argument / shape */
+#define FX_ELIMIT 1002     /* a size limit of a kernel exceeded */
+#define FX_EWORKSPACE 1003 /* workspace too small */
+
+/* Material kinds understood by the element kernels. */
+#define FX_MAT_LINEAR_ELASTIC 0 /* sigma = lam tr(eps) I + 2 mu eps, E = param        */
+#define FX_MAT_SIMP 1           /* as above with E = Emin + rho^p (E0 - Emin)         */
+#define FX_MAT_NEO_HOOKEAN 2    /* P = mu (F - F^-T) + kappa (J-1) J F^-T, mu = param */
+
+/* Where the per-material parameter lives. */
+#define FX_PARAM_NODE 0 /* (N,) interpolated with the shape functions at quads */
+#define FX_PARAM_CELL 1 /* (E,) constant over the element                       */
+
+typedef struct fx_material {
+  int32_t kind;           /* FX_MAT_*                               */
+  int32_t param_loc;      /* FX_PARAM_*                             */
+  double nu;              /* Poisson ratio (linear elastic / SIMP)  */
+  double E0, Emin, penal; /* SIMP law                               */
+  double kappa;           /* Neo-Hookean bulk modulus               */
+} fx_material;
+
+/* Last error message of this thread (NUL-terminated, truncated to len). */
+int fx_last_error(char* buf, size_t len);
+/* Library identification string. */
+int fx_version(char* buf, size_t len);
+
+/* ------------------------------------------------------------------------
+ * Structure (host).  Replaces Problem._build_csr_assembly_structure
+ * (feax/problem.py:316-484) and the per-batch slot sorts
+ * (feax/assembler.py:1164-1250) for HEX8, vec = 3, MatrixView.FULL: the
+ * deduplicated CSR pattern of all (I, J) = element DOF pairs, row-major with
+ * sorted columns, identical to feax's argsort + dedupe construction.
+ *
+ * Two calls: first with indices == NULL to get *nse, *max_degree and indptr
+ * (3N+1); then with indices (nse), elem_rank (E*64: rank[e][a][b] = position
+ * of node cells[e][b] in the sorted neighbour list of node cells[e][a]),
+ * inc_ptr (N+1) and inc (8E, packed e*8 + a, ascending e for each node).
+ * ---------------------------------------------------------------------- */
+int fx_hex8_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells /*(h) E*8*/,
+                    int64_t* nse, int32_t* max_degree, int32_t* indptr /*(h) 3N+1*/,
+                    int32_t* indices /*(h) nse or NULL*/, uint8_t* elem_rank /*(h) E*64 or NULL*/,
+                    int32_t* inc_ptr /*(h) N+1 or NULL*/, int32_t* inc /*(h) 8E or NULL*/);
+
+/* Node -> loaded-face incidence over a face list (F,2) of (cell, local face)
+ * as built by feax/fe.py:267-318.  finc_ptr (N+1), finc (4F, packed f*8 + a
+ * with a the cell-local node index, ascending f per node). */
+int fx_hex8_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells /*(h)*/,
+                           int32_t num_faces, const int32_t* faces /*(h) F*2*/,
+                           int32_t* finc_ptr /*(h) N+1*/, int32_t* finc /*(h) 4F*/);
+
+/* Host check: rows 3m..3m+2 share one sorted column list of whole 3-blocks. */
+int fx_csr_is_block3(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t* is_block3);
+
+/* ------------------------------------------------------------------------
+ * CSR kernels.  Replace CSRMatrix.__matmul__ (feax/csr.py:81-90) and
+ * CSRMatrix.diagonal + the Jacobi preconditioner (feax/csr.py:115-121,
+ * feax/solvers/common.py:86-94).  nse (= indptr[n]) only sizes the launch.
+ * ---------------------------------------------------------------------- */
+int fx_csr_spmv_f64(void* stream, int32_t n, int64_t nse, const int32_t* indptr, const int32_t* indices,
+                    const double* data, const double* x, double* y);
+int fx_csr_spmv_f32(void* stream, int32_t n, int64_t nse, const int32_t* indptr, const int32_t* indices,
+                    const float* data, const float* x, float* y);
+
+/* Block-3 column index of a block-3 pattern: bcol (nse/9 int32, (d)),
+ * bcol[indptr[3m]/9 + k] = column node of the k-th 3x3 block of node m. */
+int fx_csr_block3_index(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices,
+                        int32_t* bcol);
+/* Same product as fx_csr_spmv_* reading one column index per 3x3 block. */
+int fx_csr_spmv_b3_f64(void* stream, int32_t n, int64_t nse, const int32_t* indptr, const int32_t* bcol,
+                       const double* data, const double* x, double* y);
+int fx_csr_spmv_b3_f32(void* stream, int32_t n, int64_t nse, const int32_t* indptr, const int32_t* bcol,
+                       const float* data, const float* x, float* y);
+/* Difference-form operator (FX_ASM_DIFF3 values, FP32) on FP64 vectors:
+ * y = sum_{m != n} K_nm (x_m - x_n) + S_n x_n, S_n stored in the diagonal
+ * block slot.  Equals K x; FP32 rounding acts on differences only. */
+int fx_csr_spmv_diff3_f32f64(void* stream, int32_t n, int64_t nse, const int32_t* indptr,
+                             const int32_t* bcol, const float* data, const double* x, double* y);
+
+/* dinv[i] = 1 / (A[i,i] + shift);  diag[i] = A[i,i]. */
+int fx_csr_jacobi_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices,
+                      const double* data, double shift, double* dinv);
+int fx_csr_jacobi_f32(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices,
+                      const float* data, double shift, float* dinv);
+int fx_csr_diagonal_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices,
+                        const double* data, double* diag);
+int fx_csr_diagonal_f32(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices,
+                        const float* data, float* diag);
+
+/* ------------------------------------------------------------------------
+ * Jacobi-preconditioned CG with jax.scipy.sparse.linalg.cg semantics
+ * (called at feax/solvers/common.py:229-239): stop when
+ * ||r||^2 <= max(tol^2 ||b||^2, atol^2) or k == maxiter; returns the last x.
+ * dinv == NULL: no preconditioner.  x0 == NULL: x0 = 0.  Dot products are
+ * reduced in FP64 in a fixed order (bitwise repeatable).  *iters = k,
+ * *rs = ||r_k||^2 of the recursive residual.  flags select the SpMV:
+ * 0 scalar CSR (indices), FX_PCG_BLOCK3 block-3 (bcol), FX_PCG_DIFF3
+ * difference form (bcol, fx_pcg_f32f64 only).
+ * ---------------------------------------------------------------------- */
+#define FX_PCG_BLOCK3 1
+#define FX_PCG_DIFF3 2
+size_t fx_pcg_workspace_bytes(int32_t n, int32_t vec_elem_bytes);
+int fx_pcg_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
+               const int32_t* indices, const int32_t* bcol, const double* data, const double* dinv,
+               const double* b, const double* x0, double tol, double atol, int32_t maxiter,
+               int32_t check_every, double* x, int32_t* iters, double* rs, void* workspace,
+               size_t ws_bytes);
+/* FP32 matrix and FP32 vectors (FP64 dot products). */
+int fx_pcg_f32(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
+               const int32_t* indices, const int32_t* bcol, const float* data, const float* dinv,
+               const float* b, const float* x0, double tol, double atol, int32_t maxiter,
+               int32_t check_every, float* x, int32_t* iters, double* rs, void* workspace,
+               size_t ws_bytes);
+/* FP32 matrix (plain or difference form), FP64 vectors. */
+int fx_pcg_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
+                  const int32_t* indices, const int32_t* bcol, const float* data, const double* dinv,
+                  const double* b, const double* x0, double tol, double atol, int32_t maxiter,
+                  int32_t check_every, double* x, int32_t* iters, double* rs, void* workspace,
+                  size_t ws_bytes);
+
+/* ------------------------------------------------------------------------
+ * Vector helpers.
+ * ---------------------------------------------------------------------- */
+/* out(d) = sum_i a_i b_i in FP64, fixed order; workspace >= 8*2048 bytes. */
+int fx_dot_f64(void* stream, int32_t n, const double* a, const double* b, double* out, void* workspace,
+               size_t ws_bytes);
+/* y64 += scale * (double) x32 */
+int fx_axpy_f32_into_f64(void* stream, int32_t n, double scale, const float* x, double* y);
+/* y32 = (float) (scale * x64) */
+int fx_convert_f64_to_f32(void* stream, int32_t n, double scale, const double* x, float* y);
+/* out = a x + b y (FP64); out may alias x or y.  u = u0 + du of
+ * feax/solvers/linear.py:417. */
+int fx_axpby_f64(void* stream, int32_t n, double a, const double* x, double b, const double* y,
+                 double* out);
+/* x0 = 0 except x0[bc_rows] = bc_vals - u0[bc_rows] (u0 may be NULL):
+ * create_x0 of feax/solvers/common.py:23-53 (u0 NULL: zero_like_initial_guess,
+ * feax/utils.py:464-499). */
+int fx_bc_lift_f64(void* stream, int32_t n, int32_t nbc, const int32_t* bc_rows, const double* bc_vals,
+                   const double* u0, double* x0);
+
+/* ------------------------------------------------------------------------
+ * HEX8 element kernels.  One 64-lane wavefront per mesh node gathers the
+ * contributions of the node's elements in ascending element order (the
+ * summation order of feax/assembler.py:1253-1318): no float atomics,
+ * bitwise repeatable.
+ *
+ * x, y, z: (d) node coordinates (SoA, FP64).  cells: (d) E*8 int32.
+ * param: (d) N or E values (mat->param_loc).  u: (d) 3N FP64 or NULL.
+ * bc_mask: (d) 3N uint8 or NULL.
+ * ---------------------------------------------------------------------- */
+
+/* CSR values of the Jacobian, BC-eliminated when bc_mask != NULL (symmetric:
+ * zero BC rows and columns, BC diagonal 1; else rows only;
+ * feax/DCboundary.py:147-199).  Replaces _get_J_csr + apply_boundary_to_J_csr
+ * (feax/assembler.py:1362-1404, 1695-1713).  flags FX_ASM_DIFF3: store the
+ * difference form of fx_csr_spmv_diff3_f32f64.  dinv (d, 3N, nullable):
+ * 1 / (K_bc[r, r] + jacobi_shift). */
+#define FX_ASM_DIFF3 1
+/* fx_hex8_assemble_f32: element matrices are formed in FP64 and rounded to
+ * FP32 once at the store; FX_ASM_COMPUTE32 forms them in FP32 arithmetic. */
+#define FX_ASM_COMPUTE32 2
+int fx_hex8_assemble_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x,
+                         const double* y, const double* z, const int32_t* cells, const int32_t* inc_ptr,
+                         const int32_t* inc, const uint8_t* elem_rank, const int32_t* indptr,
+                         const int32_t* indices, const fx_material* mat, const double* param,
+                         const double* u, const uint8_t* bc_mask, int32_t symmetric, int32_t flags,
+                         double jacobi_shift, double* dinv, double* data);
+int fx_hex8_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, const double* x,
+                         const double* y, const double* z, const int32_t* cells, const int32_t* inc_ptr,
+                         const int32_t* inc, const uint8_t* elem_rank, const int32_t* indptr,
+                         const int32_t* indices, const fx_material* mat, const double* param,
+                         const double* u, const uint8_t* bc_mask, int32_t symmetric, int32_t flags,
+                         double jacobi_shift, double* dinv, float* data);
+
+/* Global residual R(u) = volume + traction (FP64): feax get_res
+ * (feax/assembler.py:1618-1667).  Surface part: the loaded faces of all
+ * boundaries concatenated in location_fns order, faces (F,2) (cell, local
+ * face), node->face incidence finc_ptr/finc over that list, dead traction
+ * t(f, q) = fdir[f] * svar[f, q] (svar (d) F*4, fdir (d) F*3): the surface
+ * map `lambda u, x, t: dir * t` of the benchmark.  num_faces = 0 for none.
+ * If bc_rows != NULL, applies res[bc] = u[bc] - bc_vals
+ * (feax/DCboundary.py:202-251); negate != 0 returns -R_bc, the right-hand
+ * side b of feax/solvers/linear.py:400. */
+int fx_hex8_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x,
+                         const double* y, const double* z, const int32_t* cells, const int32_t* inc_ptr,
+                         const int32_t* inc, const fx_material* mat, const double* param, const double* u,
+                         int32_t num_faces, const int32_t* faces, const int32_t* finc_ptr,
+                         const int32_t* finc, const double* svar, const double* fdir, int32_t num_bc,
+                         const int32_t* bc_rows, const double* bc_vals, int32_t negate, double* res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEAXHIP_H */

@@ -1,0 +1,101 @@
+"""Oracle: Jacobi-preconditioned CG and the linear solve wrapper (FP64).
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+Restates:
+  jax.scipy.sparse.linalg.cg (third-party; called at feax/solvers/common.py:232).
+    Not vendored in the reference and JAX is absent here, so the published
+    open-source algorithm is restated: r0 = b - A x0, z0 = M r0, p0 = z0,
+    gamma0 = r0.z0; loop while ||r||^2 > max(tol^2 ||b||^2, atol^2) and
+    k < maxiter: q = A p, alpha = gamma / (p.q), x += alpha p, r -= alpha q,
+    z = M r, gamma' = r.z, beta = gamma'/gamma, p = z + beta p.  The
+    recursively updated r is the one tested; no error on maxiter.
+  feax/solvers/common.py:86-94   M(x) = x / (diag + jacobi_shift)
+  feax/solvers/common.py:23-53   x0 = lift of BC increments
+  feax/solvers/common.py:376-395 check_convergence: NaN unless
+                                 ||Ax - b|| / (||b|| + 1e-12) < threshold
+  feax/solvers/linear.py:387-417 u = u0 + CG(J_bc, -R_bc(u0), x0)
+
+Pinned by: a scipy.sparse.linalg.spsolve direct solve of the same system
+(tests/test_oracle_krylov.py), the stopping-rule invariants (returned
+residual below the threshold, iteration cap honoured, maxiter=0 returns x0),
+and exact convergence in <= n steps on small SPD systems.  The iterate
+trajectory of JAX's own cg is parity unpinned (needs an import).
+"""
+
+import numpy as np
+
+
+def pcg(A, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=None):
+    """Return (x, iters, ||r||^2) following jax.scipy.sparse.linalg.cg."""
+    matvec = A if callable(A) else (lambda v: A @ v)
+    M = (lambda v: v) if dinv is None else (lambda v: dinv * v)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=float)
+    atol2 = max(tol * tol * float(b @ b), atol * atol)
+    r = b - matvec(x)
+    z = M(r)
+    p = z.copy()
+    gamma = float(r @ z)
+    k = 0
+    rs = float(r @ r)
+    while rs > atol2 and k < maxiter:
+        q = matvec(p)
+        alpha = gamma / float(p @ q)
+        x = x + alpha * p
+        r = r - alpha * q
+        z = M(r)
+        gamma_new = float(r @ z)
+        beta = gamma_new / gamma
+        p = z + beta * p
+        gamma = gamma_new
+        rs = float(r @ r)
+        k += 1
+    return x, k, rs
+
+
+def jacobi_dinv(diag, shift=1e-12):
+    """1 / (diag + shift) -- feax/solvers/common.py:86-94."""
+    return 1.0 / (diag + shift)
+
+
+def x0_lift(sol, bc_rows, bc_vals):
+    """x0[bc] = bc_vals - sol[bc], 0 elsewhere -- feax/solvers/common.py:23-53."""
+    x0 = np.zeros_like(sol)
+    x0[bc_rows] = bc_vals - sol[bc_rows]
+    return x0
+
+
+def check_convergence(A, x, b, threshold=0.1):
+    """NaN-fill unless the relative residual < threshold -- common.py:376-395."""
+    matvec = A if callable(A) else (lambda v: A @ v)
+    rel = np.linalg.norm(matvec(x) - b) / (np.linalg.norm(b) + 1e-12)
+    return x if rel < threshold else np.full_like(x, np.nan)
+
+
+def linear_solve(model, mat, param, tractions, bc_rows, bc_vals, u0=None,
+                 tol=1e-10, atol=1e-10, maxiter=10000, jacobi=True, shift=1e-12,
+                 symmetric=True):
+    """One linear solve u = u0 + CG(J_bc, -R_bc(u0)) on the assembled CSR.
+
+    Returns (u, info) with info = dict(iters, rs, K (scipy, BC-applied), b, x0).
+    feax/solvers/linear.py:387-417 with the CSR operator of
+    feax/assembler.py:1695-1713 (identical to the matrix-free one,
+    assembler.py:1753-1754).
+    """
+    from . import hex8
+    n = model.n
+    bc_mask = np.zeros(n, bool)
+    bc_mask[bc_rows] = True
+    if u0 is None:
+        u0 = np.zeros(n)
+        u0[bc_rows] = bc_vals
+    data = model.csr_values(u0, param, mat)
+    data_bc = hex8.apply_bc_csr(data, model.csr, bc_mask, bc_rows, symmetric)
+    K = model.scipy_csr(data_bc)
+    res = model.residual(u0, param, mat, tractions)
+    res_bc = hex8.apply_bc_res(res, u0, bc_rows, bc_vals)
+    b = -res_bc
+    x0 = x0_lift(u0, bc_rows, bc_vals)
+    dinv = jacobi_dinv(K.diagonal(), shift) if jacobi else None
+    du, iters, rs = pcg(K, b, x0, tol, atol, maxiter, dinv)
+    return u0 + du, dict(iters=iters, rs=rs, K=K, b=b, x0=x0, data=data, data_bc=data_bc)

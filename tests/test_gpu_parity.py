@@ -1,0 +1,225 @@
+"""GPU parity of the HIP path against the FP64 oracle (oracle/)."""
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spl
+
+from conftest import E_MOD, NU, TRACTION, cantilever, oracle_model
+
+pytestmark = pytest.mark.gpu
+
+EPS64 = np.finfo(np.float64).eps
+EPS32 = np.finfo(np.float32).eps
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def c1():
+    """Config c1: 50x5x5 HEX8, spacing 2, clamped x=0, traction on x=100."""
+    from oracle import hex8, krylov
+    problem, bc, tp = cantilever((50, 5, 5), (2.0, 2.0, 2.0), 100.0)
+    m = oracle_model(problem)
+    mat = hex8.Material("linear_elastic", nu=NU)
+    E = np.full(m.N, E_MOD)
+    tr = [hex8.constant_traction(problem.boundary_inds_list[0], TRACTION)]
+    u_ref, info = krylov.linear_solve(m, mat, E, tr, bc.bc_rows, bc.bc_vals, tol=1e-12, atol=0.0)
+    u_dir = spl.spsolve(info["K"].tocsc(), info["b"])
+    return dict(problem=problem, bc=bc, tp=tp, m=m, mat=mat, E=E, tr=tr, info=info, u_ref=u_ref, u_dir=u_dir)
+
+
+def _row_scale(K):
+    """max |K_ij| over each row, broadcast to the row's slots."""
+    absK = abs(K)
+    rmax = absK.max(axis=1).toarray().ravel()
+    return np.repeat(rmax, np.diff(K.indptr))
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_assembly_matches_oracle(c1, dtype):
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import assemble_csr
+    p, bc, tp = c1["problem"], c1["bc"], c1["tp"]
+    ts = fe.TracedStructure.from_problem(p)
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    A = assemble_csr(p, ts, tp, None, bc, True, dt)
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(A.indptr), c1["m"].csr["indptr"])
+    assert np.array_equal(_np(A.indices), c1["m"].csr["indices"])
+    ref = c1["info"]["data_bc"]
+    got = _np(A.data).astype(np.float64)
+    scale = _row_scale(c1["info"]["K"])
+    tol = (64 * EPS64) if dtype == "f64" else (64 * EPS32)
+    err = np.abs(got - ref) / scale
+    print(f"assembly {dtype}: max err / row max = {err.max():.3e} (tol {tol:.1e})")
+    assert err.max() <= tol
+
+
+def test_assembly_unbc_and_simp_general_geometry():
+    """Distorted (non-affine) mesh, SIMP cell var, no BC: vs oracle."""
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import assemble_csr
+    from oracle import hex8, structure as st
+    rng = np.random.default_rng(7)
+    pts, cells = st.structured_grid((6, 4, 3), (1.0, 1.5, 0.7))
+    pts = pts + rng.uniform(-0.15, 0.15, pts.shape)
+
+    class Simp(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.SIMPLinearElastic(nu=0.3, E0=1.0, Emin=1e-9, penal=3.0)
+
+    p = Simp(fe.Mesh(pts, cells), vec=3, dim=3)
+    rho = rng.uniform(0.3, 1.0, p.num_cells)
+    tp = fe.TracedParams(volume_vars=(rho,))
+    ts = fe.TracedStructure.from_problem(p)
+    A = assemble_csr(p, ts, tp, None, None, True, torch.float64)
+    m = hex8.Model(pts, cells)
+    ref = m.csr_values(np.zeros(m.n), rho, hex8.Material("simp", nu=0.3))
+    K = m.scipy_csr(ref)
+    err = np.abs(_np(A.data) - ref) / _row_scale(K)
+    assert err.max() <= 64 * EPS64, err.max()
+
+
+def test_assembly_bitwise_repeatable(c1):
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import assemble_csr
+    p, bc, tp = c1["problem"], c1["bc"], c1["tp"]
+    ts = fe.TracedStructure.from_problem(p)
+    a = _np(assemble_csr(p, ts, tp, None, bc).data)
+    b = _np(assemble_csr(p, ts, tp, None, bc).data)
+    assert a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("kernel", ["csr", "block3"])
+def test_spmv_matches_scipy(c1, dtype, kernel):
+    import torch
+    from feax_amd.csr import CSRMatrix
+    K = c1["info"]["K"]
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    npdt = np.float64 if dtype == "f64" else np.float32
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(K.shape[0]).astype(npdt)
+    data = K.data.astype(npdt)
+    A = CSRMatrix(torch.as_tensor(data, device="cuda"), torch.as_tensor(K.indptr.astype(np.int32), device="cuda"),
+                  torch.as_tensor(K.indices.astype(np.int32), device="cuda"), K.shape, block3=(kernel == "block3"))
+    y = _np(A.matvec(torch.as_tensor(x, device="cuda"))).astype(np.float64)
+    Kd = K.copy()
+    Kd.data = data.astype(np.float64)
+    yref = Kd @ x.astype(np.float64)
+    absy = abs(Kd) @ np.abs(x.astype(np.float64))
+    nnz = np.diff(K.indptr)
+    u = EPS64 / 2 if dtype == "f64" else EPS32 / 2
+    bound = 2 * nnz * u * absy + 1e-300
+    assert np.all(np.abs(y - yref) <= bound), np.max(np.abs(y - yref) / bound)
+
+
+def test_residual_matches_oracle(c1):
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import residual
+    p, bc, tp, m = c1["problem"], c1["bc"], c1["tp"], c1["m"]
+    ts = fe.TracedStructure.from_problem(p)
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(m.n) * 1e-3
+    r = _np(residual(p, ts, tp, torch.as_tensor(u, device="cuda")))
+    rref = m.residual(u, c1["E"], c1["mat"], c1["tr"])
+    K0 = m.scipy_csr(m.csr_values(u, c1["E"], c1["mat"]))
+    scale = abs(K0) @ np.abs(u) + np.abs(m.residual(np.zeros(m.n), c1["E"], c1["mat"], c1["tr"]))
+    assert np.all(np.abs(r - rref) <= 1e-12 * scale + 1e-300)
+    # BC-applied and negated: b = -R_bc (feax/solvers/linear.py:400)
+    b = _np(residual(p, ts, tp, torch.as_tensor(u, device="cuda"), bc, negate=True))
+    bref = -rref.copy()
+    bref[bc.bc_rows] = -(u[bc.bc_rows] - bc.bc_vals)
+    assert np.all(np.abs(b - bref) <= 1e-12 * scale + 1e-300)
+
+
+@pytest.mark.parametrize("spmv", ["csr", "block3"])
+def test_fp64_solve_matches_direct(c1, spmv):
+    import feax_amd as fe
+    p, bc, tp = c1["problem"], c1["bc"], c1["tp"]
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=0.0, maxiter=10000,
+                                  use_jacobi_preconditioner=True, spmv=spmv)
+    solver = fe.create_solver(p, bc, opts, linear=True, traced_params=tp)
+    u = np.asarray(solver(tp, fe.zero_like_initial_guess(p, bc)))
+    err = np.linalg.norm(u - c1["u_dir"]) / np.linalg.norm(c1["u_dir"])
+    from oracle import krylov
+    K, b = c1["info"]["K"], c1["info"]["b"]
+    _, it_ref, _ = krylov.pcg(K, b, c1["info"]["x0"], 1e-10, 0.0, 10000, krylov.jacobi_dinv(K.diagonal()))
+    print("fp64", spmv, solver.stats, "err", err, "oracle iters", it_ref)
+    assert err < 1e-8
+    # same algorithm and stopping rule; FP rounding may move the count slightly
+    assert abs(solver.stats.iters - it_ref) <= max(3, 0.02 * it_ref)
+
+
+def test_diff_form_operator_matches_oracle(c1):
+    """FP32 difference-form CSR applied to FP64 vectors equals K x up to FP32
+    rounding of the entries acting on differences x_m - x_n."""
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import assemble_csr
+    p, bc, tp = c1["problem"], c1["bc"], c1["tp"]
+    ts = fe.TracedStructure.from_problem(p)
+    A, dinv = assemble_csr(p, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12)
+    K = c1["info"]["K"]
+    rng = np.random.default_rng(5)
+    x = 1.0 + 1e-3 * rng.standard_normal(K.shape[0])      # large common translation + small variation
+    y = _np(A.matvec(torch.as_tensor(x, device="cuda")))
+    yref = K @ x
+    rows = np.repeat(np.arange(K.shape[0]), np.diff(K.indptr))
+    own = (K.indices // 3) == (rows // 3)
+    xn = x.reshape(-1, 3)[rows // 3, K.indices % 3]
+    absdiff = np.abs(np.where(own, 0.0, K.data)) * np.abs(x[K.indices] - xn)
+    # S (block row sums) of rows coupled to eliminated columns: from the
+    # unconstrained oracle values, |S| <= sum of |entries| in BC columns.
+    full = c1["info"]["data"]
+    bcm = np.zeros(K.shape[0], bool)
+    bcm[bc.bc_rows] = True
+    srow = np.bincount(rows, np.where(bcm[K.indices] & ~bcm[rows], np.abs(full), 0.0) * np.abs(xn),
+                       minlength=K.shape[0])
+    bound = (np.bincount(rows, absdiff, minlength=K.shape[0]) + srow) * 8 * EPS32
+    err = np.abs(y - yref)
+    print("diff3 max err/bound", (err / (bound + 1e-300)).max())
+    assert np.all(err <= bound + 64 * EPS64 * (abs(K) @ np.abs(x)))
+    dref = 1.0 / (K.diagonal() + 1e-12)
+    assert np.allclose(_np(dinv), dref, rtol=1e-5)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "mixed-fp32vec"])
+def test_mixed_solve_matches_direct(c1, precision):
+    import feax_amd as fe
+    p, bc, tp = c1["problem"], c1["bc"], c1["tp"]
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=20000,
+                                  use_jacobi_preconditioner=True, precision=precision)
+    solver = fe.create_solver(p, bc, opts, linear=True, traced_params=tp)
+    u = np.asarray(solver(tp, fe.zero_like_initial_guess(p, bc)))
+    err = np.linalg.norm(u - c1["u_dir"]) / np.linalg.norm(c1["u_dir"])
+    st = solver.stats
+    print(precision, st, "err", err)
+    assert st.relres <= 1e-8
+    assert err < 1e-6
+
+
+def test_pcg_repeatable_and_maxiter(c1):
+    import torch
+    from feax_amd.csr import CSRMatrix
+    from feax_amd.solvers.common import pcg
+    K, b = c1["info"]["K"], c1["info"]["b"]
+    A = CSRMatrix(torch.as_tensor(K.data, device="cuda"), torch.as_tensor(K.indptr.astype(np.int32), device="cuda"),
+                  torch.as_tensor(K.indices.astype(np.int32), device="cuda"), K.shape, block3=True)
+    bt = torch.as_tensor(b, device="cuda")
+    xs, is_ = pcg(A, bt, None, 1e-10, 0.0, 10000, block3=False)
+    x1, i1 = pcg(A, bt, None, 1e-10, 0.0, 10000)
+    x2, i2 = pcg(A, bt, None, 1e-10, 0.0, 10000)
+    assert _np(x1).tobytes() == _np(x2).tobytes() and i1.iters == i2.iters
+    assert abs(is_.iters - i1.iters) <= 2
+    x3, i3 = pcg(A, bt, None, 1e-10, 0.0, 7, check_every=3)
+    assert i3.iters == 7
+    x0 = torch.zeros_like(bt)
+    x4, i4 = pcg(A, bt, x0, 1e-10, 0.0, 0)
+    assert i4.iters == 0 and float(x4.abs().max()) == 0.0
