@@ -54,11 +54,13 @@ SIGNATURES = {
     "fx_csr_diagonal_f32": (_I32, [_P, _I32, _P, _P, _P, _P]),
     "fx_pcg_workspace_bytes": (_SZ, [_I32, _I32]),
     "fx_pcg_f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P,
-                          _P, _P, _SZ]),
+                          _P, _P, _P, _I32, _P, _SZ]),
     "fx_pcg_f32": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P,
-                          _P, _P, _SZ]),
+                          _P, _P, _P, _I32, _P, _SZ]),
     "fx_pcg_f32f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P,
-                             _P, _P, _P, _SZ]),
+                             _P, _P, _P, _P, _I32, _P, _SZ]),
+    "fx_krylov_project_workspace_bytes": (_SZ, [_I32]),
+    "fx_krylov_project_f32": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _SZ]),
     "fx_dot_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _SZ]),
     "fx_axpy_f32_into_f64": (_I32, [_P, _I32, _F64, _P, _P]),
     "fx_convert_f64_to_f32": (_I32, [_P, _I32, _F64, _P, _P]),

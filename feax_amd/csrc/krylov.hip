@@ -50,7 +50,21 @@ __device__ __forceinline__ T csr_row(const int* __restrict__ indptr, const int* 
                                      const V* __restrict__ data, const T* __restrict__ x, int row, int gl) {
   const int s = indptr[row], e = indptr[row + 1];
   T acc = T(0);
-  for (int k = s + gl; k < e; k += L) acc += static_cast<T>(ldnt(data + k)) * x[ldnt(indices + k)];
+  constexpr int J = (96 + L - 1) / L;  // positions per lane issued together
+  for (int k0 = s; k0 < e; k0 += J * L) {
+    int col[J];
+    V v[J];
+#pragma unroll
+    for (int t = 0; t < J; ++t) {
+      const int k = k0 + gl + t * L;
+      const bool ok = k < e;
+      col[t] = ok ? ldnt(indices + k) : -1;
+      v[t] = ok ? ldnt(data + k) : V(0);
+    }
+#pragma unroll
+    for (int t = 0; t < J; ++t)
+      if (col[t] >= 0) acc += static_cast<T>(v[t]) * x[col[t]];
+  }
   return group_sum<T, L>(acc);
 }
 
@@ -258,10 +272,13 @@ __global__ __launch_bounds__(kBlock) void pcg_spmv_kernel(int n, const int* __re
 }
 
 __global__ __launch_bounds__(kFinBlock) void pcg_fin_alpha(PcgState* st, const double* __restrict__ partials,
-                                                           int nparts) {
+                                                           int nparts, double* __restrict__ rec_pq, int rec_max) {
   if (st->done) return;
   DVec<1> t = sum_partials<1>(partials, nparts);
-  if (threadIdx.x == 0) st->alpha = st->gamma / t.v[0];
+  if (threadIdx.x == 0) {
+    st->alpha = st->gamma / t.v[0];
+    if (rec_pq && st->k < rec_max) rec_pq[st->k] = t.v[0];
+  }
 }
 
 // x += alpha p; r -= alpha q; z = M r; partials of r.z and r.r.
@@ -269,16 +286,21 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, T* __restrict__ x, T* __restrict__ r,
                                                             T* __restrict__ z, const T* __restrict__ p,
                                                             const T* __restrict__ q, const T* __restrict__ dinv,
-                                                            const PcgState* __restrict__ st, double* partials) {
+                                                            const PcgState* __restrict__ st, double* partials,
+                                                            float* __restrict__ rec_p, int rec_max) {
   if (st->done) return;
   int lb;
   const Range rg = block_range(n, lb);
   const T alpha = static_cast<T>(st->alpha);
+  // Recording: direction p_k of this iteration, FP32, slot k.
+  float* rec = (rec_p && st->k < rec_max) ? rec_p + static_cast<size_t>(st->k) * n : nullptr;
   DVec<2> acc;
   acc.v[0] = 0.0;
   acc.v[1] = 0.0;
   for (int i = rg.begin + threadIdx.x; i < rg.end; i += kBlock) {
-    x[i] += alpha * p[i];
+    const T pi = p[i];
+    if (rec) rec[i] = static_cast<float>(pi);
+    x[i] += alpha * pi;
     const T ri = r[i] - alpha * q[i];
     r[i] = ri;
     const T zi = dinv ? dinv[i] * ri : ri;
@@ -355,6 +377,9 @@ struct PcgArgs {
   int flags;
   const int* indptr;
   const int* idx;  // indices (scalar kernel) or bcol (block-3 kernels)
+  float* rec_p;    // optional: directions p_k (rec_max x n FP32)
+  double* rec_pq;  // optional: p_k . A p_k
+  int rec_max;
 };
 
 template <typename V, typename T, int L, bool B3, bool DIFF>
@@ -362,9 +387,9 @@ void launch_pcg_iter(hipStream_t s, const PcgArgs& a, int grid_mv, int grid_v, c
                      T* r, T* z, T* p, T* q, PcgState* st, double* partials) {
   hipLaunchKernelGGL((pcg_spmv_kernel<V, T, L, B3, DIFF>), dim3(grid_mv), dim3(kBlock), 0, s, a.n, a.indptr, a.idx,
                      data, p, q, st, partials);
-  hipLaunchKernelGGL(pcg_fin_alpha, dim3(1), dim3(kFinBlock), 0, s, st, partials, grid_mv);
+  hipLaunchKernelGGL(pcg_fin_alpha, dim3(1), dim3(kFinBlock), 0, s, st, partials, grid_mv, a.rec_pq, a.rec_max);
   hipLaunchKernelGGL((pcg_update_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, a.n, x, r, z, p, q, dinv, st,
-                     partials + kMaxGridBlocks);
+                     partials + kMaxGridBlocks, a.rec_p, a.rec_max);
   hipLaunchKernelGGL(pcg_fin_beta, dim3(1), dim3(kFinBlock), 0, s, st, partials + kMaxGridBlocks, grid_v);
   hipLaunchKernelGGL((pcg_dir_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, a.n, p, z, st);
 }
@@ -424,8 +449,11 @@ int pcg_loop(hipStream_t s, const PcgArgs& a, int lanes, const V* data, const T*
 template <typename V, typename T>
 int run_pcg(void* stream, int n, long long nse, int flags, const int* indptr, const int* indices, const int* bcol,
             const V* data, const T* dinv, const T* b, const T* x0, double tol, double atol, int maxiter,
-            int check_every, T* x, int* iters, double* rs, void* workspace, size_t ws_bytes) {
+            int check_every, T* x, int* iters, double* rs, float* rec_p, double* rec_pq, int rec_max,
+            void* workspace, size_t ws_bytes) {
   hipStream_t s = as_stream(stream);
+  if ((rec_p == nullptr) != (rec_pq == nullptr) || (rec_p && rec_max < 0))
+    return fail(FX_EINVAL, "pcg: recording needs rec_p, rec_pq and rec_max >= 0");
   if (n < 0 || !indptr || !data || !b || !x || !iters || !rs) return fail(FX_EINVAL, "pcg: bad arguments");
   if (maxiter < 0) return fail(FX_EINVAL, "pcg: maxiter < 0");
   const bool b3 = (flags & (FX_PCG_BLOCK3 | FX_PCG_DIFF3)) != 0;
@@ -439,7 +467,7 @@ int run_pcg(void* stream, int n, long long nse, int flags, const int* indptr, co
     *rs = 0.0;
     return FX_OK;
   }
-  PcgArgs a{n, nse, flags, indptr, b3 ? bcol : indices};
+  PcgArgs a{n, nse, flags, indptr, b3 ? bcol : indices, rec_p, rec_pq, rec_p ? rec_max : 0};
   const int lanes = lanes_for(n, nse, b3);
   if (diff) return pcg_loop<V, T, true, true>(s, a, lanes, data, dinv, b, x0, tol, atol, maxiter, check_every, x,
                                                iters, rs, workspace);
@@ -480,6 +508,42 @@ __global__ __launch_bounds__(kBlock) void convert_kernel(int n, double scale, co
 __global__ __launch_bounds__(kBlock) void axpby_kernel(int n, double a, const double* __restrict__ x, double b,
                                                        const double* __restrict__ y, double* __restrict__ out) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) out[i] = a * x[i] + b * y[i];
+}
+
+// ---------------------------------------------------------------- recycling
+// Galerkin projection onto recorded A-conjugate directions:
+//   x0 = sum_k p_k (p_k . r) / (p_k . A p_k).
+constexpr int kProjSplit = 64;  // blocks per direction for the dot products
+
+__global__ __launch_bounds__(kBlock) void proj_dot_kernel(int n, const float* __restrict__ P,
+                                                          const double* __restrict__ r, double* partials) {
+  const int k = blockIdx.x, b = blockIdx.y;
+  const float* pk = P + static_cast<size_t>(k) * n;
+  const int chunk = (n + kProjSplit - 1) / kProjSplit;
+  const int beg = b * chunk, end = min(n, beg + chunk);
+  DVec<1> acc;
+  acc.v[0] = 0.0;
+  for (int i = beg + threadIdx.x; i < end; i += kBlock) acc.v[0] += static_cast<double>(pk[i]) * r[i];
+  DVec<1> t = block_sum<1, kBlock>(acc);
+  if (threadIdx.x == 0) partials[static_cast<size_t>(k) * kProjSplit + b] = t.v[0];
+}
+
+__global__ __launch_bounds__(kBlock) void proj_coef_kernel(int nvec, const double* __restrict__ partials,
+                                                           const double* __restrict__ pq, double* __restrict__ c) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= nvec) return;
+  double s = 0.0;
+  for (int b = 0; b < kProjSplit; ++b) s += partials[static_cast<size_t>(k) * kProjSplit + b];
+  c[k] = pq[k] != 0.0 ? s / pq[k] : 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void proj_combine_kernel(int n, int nvec, const float* __restrict__ P,
+                                                              const double* __restrict__ c, double* __restrict__ x0) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < nvec; ++k) s += c[k] * static_cast<double>(ldnt(P + static_cast<size_t>(k) * n + i));
+  x0[i] = s;
 }
 
 // x0 = 0 except x0[bc_rows] = bc_vals - u0[bc_rows] (feax/solvers/common.py:23-53).
@@ -582,27 +646,52 @@ size_t fx_pcg_workspace_bytes(int32_t n, int32_t elem_bytes) {
 int fx_pcg_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr, const int32_t* indices,
                const int32_t* bcol, const double* data, const double* dinv, const double* b, const double* x0,
                double tol, double atol, int32_t maxiter, int32_t check_every, double* x, int32_t* iters, double* rs,
-               void* workspace, size_t ws_bytes) {
+               float* rec_p, double* rec_pq, int32_t rec_max, void* workspace, size_t ws_bytes) {
   if (flags & FX_PCG_DIFF3) return fail(FX_EINVAL, "fx_pcg_f64: FX_PCG_DIFF3 is for FP32 matrices");
   return run_pcg<double, double>(stream, n, nse, flags, indptr, indices, bcol, data, dinv, b, x0, tol, atol, maxiter,
-                                 check_every, x, iters, rs, workspace, ws_bytes);
+                                 check_every, x, iters, rs, rec_p, rec_pq, rec_max, workspace, ws_bytes);
 }
 
 int fx_pcg_f32(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr, const int32_t* indices,
                const int32_t* bcol, const float* data, const float* dinv, const float* b, const float* x0, double tol,
                double atol, int32_t maxiter, int32_t check_every, float* x, int32_t* iters, double* rs,
-               void* workspace, size_t ws_bytes) {
+               float* rec_p, double* rec_pq, int32_t rec_max, void* workspace, size_t ws_bytes) {
   if (flags & FX_PCG_DIFF3) return fail(FX_EINVAL, "fx_pcg_f32: use fx_pcg_f32f64 for the difference form");
   return run_pcg<float, float>(stream, n, nse, flags, indptr, indices, bcol, data, dinv, b, x0, tol, atol, maxiter,
-                               check_every, x, iters, rs, workspace, ws_bytes);
+                               check_every, x, iters, rs, rec_p, rec_pq, rec_max, workspace, ws_bytes);
 }
 
 int fx_pcg_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr, const int32_t* indices,
                   const int32_t* bcol, const float* data, const double* dinv, const double* b, const double* x0,
                   double tol, double atol, int32_t maxiter, int32_t check_every, double* x, int32_t* iters,
-                  double* rs, void* workspace, size_t ws_bytes) {
+                  double* rs, float* rec_p, double* rec_pq, int32_t rec_max, void* workspace, size_t ws_bytes) {
   return run_pcg<float, double>(stream, n, nse, flags, indptr, indices, bcol, data, dinv, b, x0, tol, atol, maxiter,
-                                check_every, x, iters, rs, workspace, ws_bytes);
+                                check_every, x, iters, rs, rec_p, rec_pq, rec_max, workspace, ws_bytes);
+}
+
+size_t fx_krylov_project_workspace_bytes(int32_t nvec) {
+  return (static_cast<size_t>(std::max(nvec, 0)) * (kProjSplit + 1) + 32) * sizeof(double);
+}
+
+int fx_krylov_project_f32(void* stream, int32_t n, int32_t nvec, const float* P, const double* pq, const double* r,
+                          double* x0, void* workspace, size_t ws_bytes) {
+  if (n < 0 || nvec < 0 || (nvec > 0 && (!P || !pq)) || !r || !x0)
+    return fail(FX_EINVAL, "fx_krylov_project_f32: bad arguments");
+  if (!workspace || ws_bytes < fx_krylov_project_workspace_bytes(nvec))
+    return fail(FX_EWORKSPACE, "fx_krylov_project_f32: workspace too small");
+  hipStream_t s = as_stream(stream);
+  if (n == 0) return FX_OK;
+  if (nvec == 0) {
+    FX_HIP(hipMemsetAsync(x0, 0, static_cast<size_t>(n) * sizeof(double), s));
+    return FX_OK;
+  }
+  double* partials = static_cast<double*>(workspace);
+  double* c = partials + static_cast<size_t>(nvec) * kProjSplit;
+  hipLaunchKernelGGL(proj_dot_kernel, dim3(nvec, kProjSplit), dim3(kBlock), 0, s, n, P, r, partials);
+  hipLaunchKernelGGL(proj_coef_kernel, dim3((nvec + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nvec, partials, pq, c);
+  hipLaunchKernelGGL(proj_combine_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, nvec, P, c, x0);
+  FX_LAUNCHED("proj kernels");
+  return FX_OK;
 }
 
 int fx_dot_f64(void* stream, int32_t n, const double* a, const double* b, double* out, void* workspace,

@@ -40,12 +40,13 @@ class PCGInfo:
 
 
 def pcg(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=None, block3=None,
-        check_every=32, out=None, vec_dtype=None):
+        check_every=32, out=None, vec_dtype=None, record=None):
     """Jacobi-PCG on the device; returns (x, PCGInfo).
 
     Vectors are in the matrix precision, except for a difference-form FP32
     matrix (A.form == "diff3") or vec_dtype=float64 with an FP32 matrix,
-    which run FP64 vectors (fx_pcg_f32f64).
+    which run FP64 vectors (fx_pcg_f32f64).  record: a KrylovRecord that
+    receives the search directions (for recycle()).
     """
     import torch
     n = A.shape[0]
@@ -74,8 +75,36 @@ def pcg(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=Non
         name = "fx_pcg_f32"
     nat.call(name, nat.stream(), n, A.nse, flags, nat.ptr(A.indptr), nat.ptr(A.indices), nat.ptr(bcol),
              nat.ptr(A.data), nat.ptr(dinv), nat.ptr(b), nat.ptr(x0), float(tol), float(atol), int(maxiter),
-             int(check_every), nat.ptr(x), ctypes.addressof(iters), ctypes.addressof(rs), nat.ptr(ws), ws.numel())
+             int(check_every), nat.ptr(x), ctypes.addressof(iters), ctypes.addressof(rs),
+             nat.ptr(record.P) if record is not None else None, nat.ptr(record.pq) if record is not None else None,
+             record.capacity if record is not None else 0, nat.ptr(ws), ws.numel())
+    if record is not None:
+        record.count = min(iters.value, record.capacity)
     return x, PCGInfo(iters.value, rs.value)
+
+
+class KrylovRecord:
+    """Device storage for the search directions of one CG solve (FP32,
+    capacity x n) and their curvatures p.Ap, used to start a later solve
+    with the same matrix from the Galerkin projection of its right-hand
+    side (fx_krylov_project_f32)."""
+
+    def __init__(self, n, capacity, device):
+        import torch
+        self.n, self.capacity, self.count = int(n), int(capacity), 0
+        self.P = torch.empty((self.capacity, self.n), dtype=torch.float32, device=device)
+        self.pq = torch.empty(self.capacity, dtype=torch.float64, device=device)
+        self._ws = torch.empty(int(nat.lib().fx_krylov_project_workspace_bytes(self.capacity)), dtype=torch.uint8,
+                               device=device)
+
+    def project(self, r, out=None):
+        """x0 = sum_k p_k (p_k . r) / (p_k . A p_k) over the recorded directions."""
+        import torch
+        r = r.to(torch.float64).contiguous()
+        x0 = torch.empty(self.n, dtype=torch.float64, device=r.device) if out is None else out
+        nat.call("fx_krylov_project_f32", nat.stream(), self.n, self.count, nat.ptr(self.P), nat.ptr(self.pq),
+                 nat.ptr(r), nat.ptr(x0), nat.ptr(self._ws), self._ws.numel())
+        return x0
 
 
 def create_jacobi_preconditioner(A: CSRMatrix, shift=1e-12):

@@ -23,8 +23,8 @@ import numpy as np
 from .. import _native as nat
 from ..assembler import TracedStructure, assemble_csr, residual
 from ..DCboundary import DirichletBC
-from .common import (create_iterative_solve_fn, create_jacobi_preconditioner, create_x0, device_norm, pcg,
-                     check_convergence)
+from .common import (KrylovRecord, check_convergence, create_iterative_solve_fn, create_jacobi_preconditioner,
+                     create_x0, device_norm, pcg)
 from .options import KrylovSolverOptions, MatrixProperty, resolve_iterative_solver
 
 
@@ -94,10 +94,19 @@ def solve_mixed(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None):
     rn = bnorm
     inner, total, outer = [], 0, 0
     budget = opts.maxiter
+    # Krylov recycling: the first inner solve records its search directions;
+    # later inner solves (same matrix) start from the Galerkin projection of
+    # their right-hand side onto them.
+    record = None
+    if opts.recycle > 0 and not fp32vec:
+        record = _record_for(n, min(opts.recycle, opts.maxiter), u.device)
+        record.count = 0
     while rn > target and outer < opts.max_outer and budget > 0:
         if fp32vec:
             nat.call("fx_convert_f64_to_f32", nat.stream(), n, 1.0 / rn, nat.ptr(r), nat.ptr(rv))
-        _, info = pcg(A, rv, None, opts.inner_tol, 0.0, budget, dinv, True, opts.check_every, out=d)
+        x0 = record.project(rv) if (record is not None and outer > 0) else None
+        rec = record if (record is not None and outer == 0) else None
+        _, info = pcg(A, rv, x0, opts.inner_tol, 0.0, budget, dinv, True, opts.check_every, out=d, record=rec)
         if fp32vec:
             nat.call("fx_axpy_f32_into_f64", nat.stream(), n, rn, nat.ptr(d), nat.ptr(u))
         else:
@@ -115,6 +124,20 @@ def solve_mixed(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None):
     if opts.check_convergence and not (rn / (bnorm + 1e-12) < opts.convergence_threshold):
         u = torch.full_like(u, float("nan"))
     return u
+
+
+_RECORDS = {}
+
+
+def _record_for(n, capacity, device):
+    """Reusable direction store (capacity x n FP32) for Krylov recycling."""
+    key = (str(device), n)
+    rec = _RECORDS.get(key)
+    if rec is None or rec.capacity < capacity:
+        _RECORDS.pop(key, None)
+        rec = KrylovRecord(n, capacity, device)
+        _RECORDS[key] = rec
+    return rec
 
 
 def _dev_vec(a, ts):

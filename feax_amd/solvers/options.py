@@ -13,6 +13,10 @@ KrylovSolverOptions
   spmv       "auto" | "csr" | "block3": SpMV kernel for the CG iterations.
   check_every  iterations between host reads of the convergence flag.
   inner_tol, max_outer  mixed-precision inner CG tolerance and outer cap.
+  recycle    mixed precision: number of search directions of the first
+             inner solve kept (FP32, recycle x n in HBM) to start the later
+             inner solves from the Galerkin projection of their residual
+             (0 disables).
 """
 
 from dataclasses import dataclass
@@ -67,6 +71,7 @@ class KrylovSolverOptions(AbstractSolverOptions):
     check_every: int = 32
     inner_tol: float = 1e-5
     max_outer: int = 30
+    recycle: int = 3000
 
     def __post_init__(self):
         valid = ("auto", "cg", "bicgstab", "gmres")

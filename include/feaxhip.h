@@ -127,24 +127,35 @@ int fx_csr_diagonal_f32(void* stream, int32_t n, const int32_t* indptr, const in
  * ---------------------------------------------------------------------- */
 #define FX_PCG_BLOCK3 1
 #define FX_PCG_DIFF3 2
+/* Optional recording (all three drivers): when rec_p != NULL, iteration k <
+ * rec_max stores its search direction p_k as FP32 in rec_p + k*n (d) and
+ * p_k . A p_k in rec_pq[k] (d), for fx_krylov_project_f32. */
 size_t fx_pcg_workspace_bytes(int32_t n, int32_t vec_elem_bytes);
 int fx_pcg_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
                const int32_t* indices, const int32_t* bcol, const double* data, const double* dinv,
                const double* b, const double* x0, double tol, double atol, int32_t maxiter,
-               int32_t check_every, double* x, int32_t* iters, double* rs, void* workspace,
-               size_t ws_bytes);
+               int32_t check_every, double* x, int32_t* iters, double* rs, float* rec_p, double* rec_pq,
+               int32_t rec_max, void* workspace, size_t ws_bytes);
 /* FP32 matrix and FP32 vectors (FP64 dot products). */
 int fx_pcg_f32(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
                const int32_t* indices, const int32_t* bcol, const float* data, const float* dinv,
                const float* b, const float* x0, double tol, double atol, int32_t maxiter,
-               int32_t check_every, float* x, int32_t* iters, double* rs, void* workspace,
-               size_t ws_bytes);
+               int32_t check_every, float* x, int32_t* iters, double* rs, float* rec_p, double* rec_pq,
+               int32_t rec_max, void* workspace, size_t ws_bytes);
 /* FP32 matrix (plain or difference form), FP64 vectors. */
 int fx_pcg_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
                   const int32_t* indices, const int32_t* bcol, const float* data, const double* dinv,
                   const double* b, const double* x0, double tol, double atol, int32_t maxiter,
-                  int32_t check_every, double* x, int32_t* iters, double* rs, void* workspace,
-                  size_t ws_bytes);
+                  int32_t check_every, double* x, int32_t* iters, double* rs, float* rec_p,
+                  double* rec_pq, int32_t rec_max, void* workspace, size_t ws_bytes);
+
+/* Krylov recycling: Galerkin projection of r onto nvec recorded A-conjugate
+ * directions, x0 = sum_k p_k (p_k . r) / (p_k . A p_k) (init-CG, Erhel and
+ * Guyomarc'h 2000).  P: (d) nvec*n FP32, pq: (d) nvec, r, x0: (d) n FP64.
+ * Fixed-order FP64 reductions. */
+size_t fx_krylov_project_workspace_bytes(int32_t nvec);
+int fx_krylov_project_f32(void* stream, int32_t n, int32_t nvec, const float* P, const double* pq,
+                          const double* r, double* x0, void* workspace, size_t ws_bytes);
 
 /* ------------------------------------------------------------------------
  * Vector helpers.

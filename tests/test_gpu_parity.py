@@ -223,3 +223,24 @@ def test_pcg_repeatable_and_maxiter(c1):
     x0 = torch.zeros_like(bt)
     x4, i4 = pcg(A, bt, x0, 1e-10, 0.0, 0)
     assert i4.iters == 0 and float(x4.abs().max()) == 0.0
+
+
+def test_krylov_projection_matches_numpy():
+    """fx_krylov_project_f32: x0 = sum_k p_k (p_k . r) / pq_k."""
+    import torch
+    from feax_amd.solvers.common import KrylovRecord
+    rng = np.random.default_rng(9)
+    n, kvec = 1000, 7
+    rec = KrylovRecord(n, 10, "cuda")
+    P = rng.standard_normal((kvec, n)).astype(np.float32)
+    pq = rng.uniform(0.5, 2.0, kvec)
+    rec.P[:kvec] = torch.as_tensor(P, device="cuda")
+    rec.pq[:kvec] = torch.as_tensor(pq, device="cuda")
+    rec.count = kvec
+    r = rng.standard_normal(n)
+    x0 = _np(rec.project(torch.as_tensor(r, device="cuda")))
+    Pd = P.astype(np.float64)
+    ref = Pd.T @ ((Pd @ r) / pq)
+    assert np.allclose(x0, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    rec.count = 0
+    assert float(rec.project(torch.as_tensor(r, device="cuda")).abs().max()) == 0.0
