@@ -26,6 +26,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 
+# HBM bytes per pcg_spmv_kernel dispatch on the 200x40x40 workload, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/spmv_c2_pmc.md), each
+# scaled by the same counter's reading for a copy of known size.
+TRAFFIC = {}
+
 CONFIGS = {
     "c2": dict(dims=(200, 40, 40), spacing=1.0),
     "c1": dict(dims=(50, 5, 5), spacing=2.0),
@@ -117,6 +122,34 @@ def cpu_baseline(problem, bc, A64, pcg_iters, budget_s=20.0):
                 asm_s=t_asm_sample * E / ns, pcg_s_per_iter=per_it)
 
 
+def kernels_only(cfg, iters=8, copy_bytes=1 << 30):
+    """A short run for hardware-counter passes: `iters` PCG iterations in
+    each matrix form (so pcg_spmv_kernel is dispatched `iters` times per
+    form) and three plain copies of `copy_bytes` (torch's copy kernel), whose
+    FETCH_SIZE / WRITE_SIZE calibrate the counters against known bytes."""
+    import torch
+    from feax_amd.assembler import assemble_csr
+    from feax_amd.solvers.common import create_jacobi_preconditioner, pcg
+
+    problem, bc, tp, ts = build(cfg)
+    A64 = assemble_csr(problem, ts, tp, None, bc, True, torch.float64)
+    d64 = create_jacobi_preconditioner(A64)
+    Ad, dd = assemble_csr(problem, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12)
+    g = torch.Generator(device="cuda").manual_seed(20261203)
+    b = torch.randn(ts.n, dtype=torch.float64, device="cuda", generator=g)
+    torch.cuda.synchronize()
+    log(f"kernels-only: n={ts.n} nse={ts.nse}; {iters} PCG iterations per form")
+    pcg(A64, b, None, 0.0, 0.0, iters, d64, True, check_every=iters)
+    pcg(Ad, b, None, 0.0, 0.0, iters, dd, True, check_every=iters)
+    src = torch.empty(copy_bytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernels_only": True, "n": ts.n, "nse": ts.nse, "pcg_iters": iters,
+                      "copy_bytes": copy_bytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,17 +157,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="launch only the PCG SpMV kernels and a 1 GiB copy (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is not built.",
               file=sys.stderr)
         sys.exit(2)
+    if args.kernels_only:
+        kernels_only(CONFIGS[args.config])
+        return
 
     import numpy as np
     import torch
     import feax_amd as fe
     from feax_amd.assembler import assemble_csr, residual
-    from feax_amd.solvers.common import device_norm
+    from feax_amd.solvers.common import SpmvTimer, device_norm
 
     cfg = CONFIGS[args.config]
     t0 = time.perf_counter()
@@ -153,6 +191,7 @@ def main():
     results = {}
     b0 = residual(problem, ts, tp, u0, bc, negate=True)
     bnorm = device_norm(b0)
+    spmv_live = {}
     for form in ("fp64", "mixed"):
         solver = solver_for(form)
         for _ in range(args.warmup):
@@ -163,6 +202,12 @@ def main():
             u = solver(tp, u0)
         torch.cuda.synchronize()
         el = (time.perf_counter() - t1) / args.steps
+        # One more step with HIP events around every SpMV launch inside the
+        # PCG loop (kept out of the wall-clock region above).
+        with SpmvTimer() as timer:
+            solver(tp, u0)
+            torch.cuda.synchronize()
+        spmv_live.update({k: (v[0] / v[1], v[1]) for k, v in timer.kernels.items() if v[1]})
         r = residual(problem, ts, tp, u, bc, negate=True)
         results[form] = dict(s=el, iters=solver.stats.iters, outer=solver.stats.outer,
                              inner=solver.stats.inner_iters, relres=device_norm(r) / bnorm, u=u)
@@ -182,14 +227,22 @@ def main():
     # Algorithmic bytes per launch: nse (s_v + 4) + 4 (n + 1) + 2 n s_x.
     B64 = nse * (8 + 4) + 4 * (n + 1) + 2 * n * 8
     Bd = nse * (4 + 4) + 4 * (n + 1) + 2 * n * 8
+    # What the block-3 kernels must move: one int32 column per 3x3 block,
+    # one row pointer per node.
+    Bmin64 = nse * 8 + (nse // 9) * 4 + 4 * (n // 3 + 1) + 2 * n * 8
+    Bmind = nse * 4 + (nse // 9) * 4 + 4 * (n // 3 + 1) + 2 * n * 8
     t_asm64 = event_time(lambda: assemble_csr(problem, ts, tp, None, bc, True, torch.float64), 5)
 
     headline = "mixed"
-    dom = dict(kernel="pcg SpMV, FP32 difference form (fx_csr_spmv_diff3_f32f64)", t=t_diff, B=Bd)
+    dom = dict(kernel="pcg_spmv_kernel, FP32 difference form, FP64 vectors (f32f64/diff3)", label="f32f64/diff3",
+               B=Bd)
     if results["fp64"]["s"] < results["mixed"]["s"]:
         headline = "fp64"
-        dom = dict(kernel="pcg SpMV, FP64 block-3 (fx_csr_spmv_b3_f64)", t=t_spmv64, B=B64)
+        dom = dict(kernel="pcg_spmv_kernel, FP64 block-3 (f64/block3)", label="f64/block3", B=B64)
+    t_live, n_live = spmv_live[dom["label"]]
+    dom["t"] = t_live * 1e-6
     achieved = dom["B"] / dom["t"]
+    log(f"in-solve SpMV {dom['label']}: {t_live:.2f} us avg over {n_live} launches -> {achieved / 1e9:.0f} GB/s")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(problem, bc, A64, results["fp64"]["iters"])
@@ -215,12 +268,16 @@ def main():
                       "true_relres": v["relres"]} for k, v in results.items()},
         "mixed_vs_fp64_rel_diff": mixed_vs_fp64,
         "spmv": {"fp64_b3_us": round(t_spmv64 * 1e6, 2), "fp64_alg_GBps": round(B64 / t_spmv64 / 1e9, 1),
-                 "diff3_us": round(t_diff * 1e6, 2), "diff3_alg_GBps": round(Bd / t_diff / 1e9, 1)},
+                 "diff3_us": round(t_diff * 1e6, 2), "diff3_alg_GBps": round(Bd / t_diff / 1e9, 1),
+                 "in_solve_us": {k: round(v[0], 2) for k, v in spmv_live.items()},
+                 "in_solve_launches": {k: v[1] for k, v in spmv_live.items()},
+                 "B_alg_fp64": B64, "B_alg_diff3": Bd, "B_min_fp64": Bmin64, "B_min_diff3": Bmind},
         "assemble_fp64_ms": round(t_asm64 * 1e3, 3),
         "grad_solve_ratio": None,
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": None},
+                     "traffic": TRAFFIC.get(dom["label"]), "kernel_us": round(dom["t"] * 1e6, 2),
+                     "alg_bytes": dom["B"]},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }
     print(json.dumps(line), flush=True)

@@ -19,6 +19,7 @@ FX_PARAM_NODE = 0
 FX_PARAM_CELL = 1
 FX_PCG_BLOCK3 = 1
 FX_PCG_DIFF3 = 2
+FX_PCG_TIME = 4
 FX_ASM_DIFF3 = 1
 FX_ASM_COMPUTE32 = 2
 

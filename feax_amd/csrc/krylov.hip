@@ -26,6 +26,7 @@
 // update.  FP64 partial sums, fixed order: bitwise repeatable.
 #include <algorithm>
 #include <cstdint>
+#include <vector>
 
 #include "common.h"
 #include "reduce.h"
@@ -50,35 +51,26 @@ __device__ __forceinline__ T csr_row(const int* __restrict__ indptr, const int* 
                                      const V* __restrict__ data, const T* __restrict__ x, int row, int gl) {
   const int s = indptr[row], e = indptr[row + 1];
   T acc = T(0);
-  constexpr int J = (96 + L - 1) / L;  // positions per lane issued together
-  for (int k0 = s; k0 < e; k0 += J * L) {
-    int col[J];
-    V v[J];
-#pragma unroll
-    for (int t = 0; t < J; ++t) {
-      const int k = k0 + gl + t * L;
-      const bool ok = k < e;
-      col[t] = ok ? ldnt(indices + k) : -1;
-      v[t] = ok ? ldnt(data + k) : V(0);
-    }
-#pragma unroll
-    for (int t = 0; t < J; ++t)
-      if (col[t] >= 0) acc += static_cast<T>(v[t]) * x[col[t]];
-  }
+  for (int k = s + gl; k < e; k += L) acc += static_cast<T>(data[k]) * x[indices[k]];
   return group_sum<T, L>(acc);
 }
 
-// Rows 3m..3m+2 of a block-3 pattern, L lanes.  DIFF: difference form --
-// the node's own 3x3 block slot holds S (block row sums) and the other
-// blocks act on x_m - x_n:  y = sum_{m != n} K_nm (x_m - x_n) + S x_n.
+// Rows 3m..3m+2 of a block-3 pattern; one lane per 3x3 block (L lanes per
+// node): one column index, three consecutive x entries and three runs of
+// three consecutive values.  DIFF: difference form -- the node's own block
+// slot holds S (block row sums), the others act on x_m - x_n:
+//   y = sum_{m != n} K_nm (x_m - x_n) + S x_n.
 template <typename V, typename T, int L, bool DIFF>
 __device__ __forceinline__ void b3_rows(const int* __restrict__ indptr, const int* __restrict__ bcol,
                                         const V* __restrict__ data, const T* __restrict__ x, int m, int gl,
                                         T& y0, T& y1, T& y2) {
   const int r0 = indptr[3 * m];
   const int len = indptr[3 * m + 1] - r0;  // 3 * deg
+  const int nb = len / 3;
   const int* bc = bcol + r0 / 9;
   const V* d0 = data + r0;
+  const V* d1 = d0 + len;
+  const V* d2 = d1 + len;
   T a0 = T(0), a1 = T(0), a2 = T(0);
   T xs0 = T(0), xs1 = T(0), xs2 = T(0);
   if constexpr (DIFF) {
@@ -86,16 +78,20 @@ __device__ __forceinline__ void b3_rows(const int* __restrict__ indptr, const in
     xs1 = x[3 * m + 1];
     xs2 = x[3 * m + 2];
   }
-  for (int k = gl; k < len; k += L) {
-    const int kb = k / 3, j = k - 3 * kb;
-    const int c = ldnt(bc + kb);
-    T xv = x[3 * c + j];
+  for (int kb = gl; kb < nb; kb += L) {
+    const int c = bc[kb];
+    T x0 = x[3 * c], x1 = x[3 * c + 1], x2 = x[3 * c + 2];
     if constexpr (DIFF) {
-      if (c != m) xv -= (j == 0 ? xs0 : (j == 1 ? xs1 : xs2));
+      if (c != m) {
+        x0 -= xs0;
+        x1 -= xs1;
+        x2 -= xs2;
+      }
     }
-    a0 += static_cast<T>(ldnt(d0 + k)) * xv;
-    a1 += static_cast<T>(ldnt(d0 + len + k)) * xv;
-    a2 += static_cast<T>(ldnt(d0 + 2 * len + k)) * xv;
+    const int o = 3 * kb;
+    a0 += static_cast<T>(d0[o]) * x0 + static_cast<T>(d0[o + 1]) * x1 + static_cast<T>(d0[o + 2]) * x2;
+    a1 += static_cast<T>(d1[o]) * x0 + static_cast<T>(d1[o + 1]) * x1 + static_cast<T>(d1[o + 2]) * x2;
+    a2 += static_cast<T>(d2[o]) * x0 + static_cast<T>(d2[o + 1]) * x1 + static_cast<T>(d2[o + 2]) * x2;
   }
   y0 = group_sum<T, L>(a0);
   y1 = group_sum<T, L>(a1);
@@ -106,7 +102,7 @@ struct Range {
   int begin, end;
 };
 
-// Contiguous range of groups owned by this (XCD-remapped) block.
+// Contiguous range of items owned by this (XCD-remapped) block (vector kernels).
 __device__ __forceinline__ Range block_range(int ngroups, int& lblock) {
   lblock = xcd_block(blockIdx.x, gridDim.x);
   const int chunk = (ngroups + gridDim.x - 1) / gridDim.x;
@@ -116,19 +112,20 @@ __device__ __forceinline__ Range block_range(int ngroups, int& lblock) {
   return r;
 }
 
-// Row / node loop shared by the plain and the PCG SpMV.  When DOT, the
-// block partial of sum_i p_i (A p)_i goes to partials[lblock].
+// SpMV body: one group of L lanes per row (scalar) or node (block-3), one
+// row / node per group; kBlock / L groups per block and as many blocks as
+// needed (a grid-stride or persistent loop measured slower here).  When
+// DOT, the block partial of sum_i x_i (A x)_i goes to partials[blockIdx.x].
 template <typename V, typename T, int L, bool B3, bool DIFF, bool DOT>
 __device__ __forceinline__ void spmv_body(int n, const int* __restrict__ indptr, const int* __restrict__ idx,
                                           const V* __restrict__ data, const T* __restrict__ x, T* __restrict__ y,
                                           double* partials) {
-  int lb;
   const int ngroups = B3 ? n / 3 : n;
-  const Range rg = block_range(ngroups, lb);
   const int gl = threadIdx.x % L;
+  const int g = blockIdx.x * (kBlock / L) + threadIdx.x / L;
   DVec<1> acc;
   acc.v[0] = 0.0;
-  for (int g = rg.begin + threadIdx.x / L; g < rg.end; g += kBlock / L) {
+  if (g < ngroups) {
     if constexpr (B3) {
       T y0, y1, y2;
       b3_rows<V, T, L, DIFF>(indptr, idx, data, x, g, gl, y0, y1, y2);
@@ -137,19 +134,19 @@ __device__ __forceinline__ void spmv_body(int n, const int* __restrict__ indptr,
         y[3 * g + 1] = y1;
         y[3 * g + 2] = y2;
         if constexpr (DOT)
-          acc.v[0] += static_cast<double>(x[3 * g]) * static_cast<double>(y0) +
-                      static_cast<double>(x[3 * g + 1]) * static_cast<double>(y1) +
-                      static_cast<double>(x[3 * g + 2]) * static_cast<double>(y2);
+          acc.v[0] = static_cast<double>(x[3 * g]) * static_cast<double>(y0) +
+                     static_cast<double>(x[3 * g + 1]) * static_cast<double>(y1) +
+                     static_cast<double>(x[3 * g + 2]) * static_cast<double>(y2);
       }
     } else {
       const T v = csr_row<V, T, L>(indptr, idx, data, x, g, gl);
       if (gl == 0) {
         y[g] = v;
-        if constexpr (DOT) acc.v[0] += static_cast<double>(x[g]) * static_cast<double>(v);
+        if constexpr (DOT) acc.v[0] = static_cast<double>(x[g]) * static_cast<double>(v);
       }
     }
   }
-  if constexpr (DOT) store_partials<1>(block_sum<1, kBlock>(acc), lb, partials);
+  if constexpr (DOT) store_partials<1>(block_sum<1, kBlock>(acc), blockIdx.x, partials);
 }
 
 template <typename V, typename T, int L, bool B3, bool DIFF>
@@ -159,35 +156,41 @@ __global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const int* __restri
   spmv_body<V, T, L, B3, DIFF, false>(n, indptr, idx, data, x, y, nullptr);
 }
 
-// Lanes per group from the mean row length (scalar) or mean 3*deg (block-3).
-inline int pick_lanes(double mean_len, bool b3) {
+// Lanes per group.  Scalar CSR: from the mean row length.  Block-3: from the
+// mean number of 3x3 blocks per node (27 for interior HEX8 nodes): 32 lanes
+// (one pass) for FP64 values, 16 lanes (two passes) for FP32 values -- the
+// faster mappings measured on MI355X for the 1M-DOF cantilever.
+inline int pick_lanes(double mean_row, bool b3, bool v64) {
   if (b3) {
-    if (mean_len > 48) return 32;
-    if (mean_len > 20) return 16;
-    return 8;
+    const double nb = mean_row / 3.0;
+    if (v64) return nb > 16 ? 32 : (nb > 8 ? 16 : 8);
+    return nb > 8 ? 16 : 8;
   }
-  if (mean_len > 96) return 64;
-  if (mean_len > 40) return 32;
-  if (mean_len > 14) return 16;
-  if (mean_len > 6) return 8;
+  if (mean_row > 96) return 64;
+  if (mean_row > 40) return 32;
+  if (mean_row > 14) return 16;
+  if (mean_row > 6) return 8;
   return 4;
 }
 
-inline int lanes_for(int n, long long nse, bool b3) {
+inline int lanes_for(int n, long long nse, bool b3, bool v64) {
   // Tuning override (FEAXHIP_SPMV_LANES / FEAXHIP_SPMV_B3_LANES), read once.
   static const int ov_csr = getenv("FEAXHIP_SPMV_LANES") ? atoi(getenv("FEAXHIP_SPMV_LANES")) : 0;
   static const int ov_b3 = getenv("FEAXHIP_SPMV_B3_LANES") ? atoi(getenv("FEAXHIP_SPMV_B3_LANES")) : 0;
   const int ov = b3 ? ov_b3 : ov_csr;
   if (ov == 4 || ov == 8 || ov == 16 || ov == 32 || ov == 64) return ov;
-  const double mean = n ? static_cast<double>(nse) / n : 0.0;  // nnz per row = 3 * deg
-  return pick_lanes(mean, b3);
+  const double mean = n ? static_cast<double>(nse) / n : 0.0;
+  return pick_lanes(mean, b3, v64);
 }
 
+// Blocks of an SpMV launch: one group per row / node.
 inline int grid_for(int ngroups, int lanes) {
   const int gpb = kBlock / lanes;
-  const long need = (static_cast<long>(ngroups) + gpb - 1) / gpb;
-  return static_cast<int>(std::max<long>(1, std::min<long>(need, kMaxGridBlocks)));
+  return std::max(1, (ngroups + gpb - 1) / gpb);
 }
+
+// Upper bound of grid_for over all lane counts (>= 4): SpMV partial slots.
+inline size_t spmv_partial_cap(int n) { return static_cast<size_t>(n) / 4 + 64; }
 
 inline int vec_grid(int n) { return std::max(1, std::min((n + kBlock - 1) / kBlock, kMaxGridBlocks)); }
 
@@ -257,10 +260,16 @@ struct PcgState {
 };
 
 constexpr size_t kStateBytes = 256;
-// Partial slots (x kMaxGridBlocks doubles): p.q [0], r.z r.r [1, 3), init sums [3, 6).
-constexpr size_t kPartialBytes = 6 * kMaxGridBlocks * sizeof(double);
+// Vector-kernel partial slots (x kMaxGridBlocks doubles): r.z r.r [0, 2), init sums [2, 5).
+constexpr size_t kPartialBytes = 5 * kMaxGridBlocks * sizeof(double);
 
 inline size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+// Workspace: state | vector partials | SpMV partials (one per SpMV block) | r z p q.
+inline size_t spmv_partial_bytes(int n) { return align256(spmv_partial_cap(n) * sizeof(double)); }
+inline size_t pcg_ws_bytes(int n, size_t elem) {
+  return kStateBytes + kPartialBytes + spmv_partial_bytes(n) + 4 * align256(static_cast<size_t>(n) * elem);
+}
 
 template <typename V, typename T, int L, bool B3, bool DIFF>
 __global__ __launch_bounds__(kBlock) void pcg_spmv_kernel(int n, const int* __restrict__ indptr,
@@ -384,13 +393,17 @@ struct PcgArgs {
 
 template <typename V, typename T, int L, bool B3, bool DIFF>
 void launch_pcg_iter(hipStream_t s, const PcgArgs& a, int grid_mv, int grid_v, const V* data, const T* dinv, T* x,
-                     T* r, T* z, T* p, T* q, PcgState* st, double* partials) {
+                     T* r, T* z, T* p, T* q, PcgState* st, double* partials, double* spmv_partials,
+                     hipEvent_t ev0, hipEvent_t ev1) {
+  if (ev0) (void)hipEventRecord(ev0, s);
   hipLaunchKernelGGL((pcg_spmv_kernel<V, T, L, B3, DIFF>), dim3(grid_mv), dim3(kBlock), 0, s, a.n, a.indptr, a.idx,
-                     data, p, q, st, partials);
-  hipLaunchKernelGGL(pcg_fin_alpha, dim3(1), dim3(kFinBlock), 0, s, st, partials, grid_mv, a.rec_pq, a.rec_max);
+                     data, p, q, st, spmv_partials);
+  if (ev1) (void)hipEventRecord(ev1, s);
+  hipLaunchKernelGGL(pcg_fin_alpha, dim3(1), dim3(kFinBlock), 0, s, st, spmv_partials, grid_mv, a.rec_pq,
+                     a.rec_max);
   hipLaunchKernelGGL((pcg_update_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, a.n, x, r, z, p, q, dinv, st,
-                     partials + kMaxGridBlocks, a.rec_p, a.rec_max);
-  hipLaunchKernelGGL(pcg_fin_beta, dim3(1), dim3(kFinBlock), 0, s, st, partials + kMaxGridBlocks, grid_v);
+                     partials, a.rec_p, a.rec_max);
+  hipLaunchKernelGGL(pcg_fin_beta, dim3(1), dim3(kFinBlock), 0, s, st, partials, grid_v);
   hipLaunchKernelGGL((pcg_dir_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, a.n, p, z, st);
 }
 
@@ -401,8 +414,9 @@ int pcg_loop(hipStream_t s, const PcgArgs& a, int lanes, const V* data, const T*
   char* w = static_cast<char*>(workspace);
   PcgState* st = reinterpret_cast<PcgState*>(w);
   double* partials = reinterpret_cast<double*>(w + kStateBytes);
+  double* spmv_partials = reinterpret_cast<double*>(w + kStateBytes + kPartialBytes);
   const size_t vb = align256(static_cast<size_t>(n) * sizeof(T));
-  T* r = reinterpret_cast<T*>(w + kStateBytes + kPartialBytes);
+  T* r = reinterpret_cast<T*>(w + kStateBytes + kPartialBytes + spmv_partial_bytes(n));
   T* z = reinterpret_cast<T*>(reinterpret_cast<char*>(r) + vb);
   T* p = reinterpret_cast<T*>(reinterpret_cast<char*>(z) + vb);
   T* q = reinterpret_cast<T*>(reinterpret_cast<char*>(p) + vb);
@@ -416,8 +430,8 @@ int pcg_loop(hipStream_t s, const PcgArgs& a, int lanes, const V* data, const T*
   int rc = launch_spmv<V, T, B3, DIFF>(s, n, a.indptr, a.idx, data, x, q, lanes);
   if (rc) return rc;
   hipLaunchKernelGGL((pcg_init_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, n, b, q, r, z, p, dinv,
-                     partials + 3 * kMaxGridBlocks);
-  hipLaunchKernelGGL(pcg_fin_init, dim3(1), dim3(kFinBlock), 0, s, st, partials + 3 * kMaxGridBlocks, grid_v, tol,
+                     partials + 2 * kMaxGridBlocks);
+  hipLaunchKernelGGL(pcg_fin_init, dim3(1), dim3(kFinBlock), 0, s, st, partials + 2 * kMaxGridBlocks, grid_v, tol,
                      atol, maxiter);
   FX_LAUNCHED("pcg_init");
   if (check_every <= 0) check_every = 32;
@@ -426,23 +440,55 @@ int pcg_loop(hipStream_t s, const PcgArgs& a, int lanes, const V* data, const T*
   } h = {0, 0};
   FX_HIP(hipMemcpyAsync(&h, &st->k, sizeof(h), hipMemcpyDeviceToHost, s));
   FX_HIP(hipStreamSynchronize(s));
+  // FX_PCG_TIME: HIP events around every SpMV launch of the executed
+  // iterations; average in rs[1] (microseconds), count in rs[2].
+  const bool timing = (a.flags & FX_PCG_TIME) != 0;
+  std::vector<hipEvent_t> ev;
+  if (timing) {
+    ev.resize(2 * static_cast<size_t>(check_every));
+    for (auto& e : ev) FX_HIP(hipEventCreate(&e));
+  }
+  double t_sum_ms = 0.0;
+  long t_count = 0;
   int launched = 0;
+  int rc_loop = FX_OK;
   while (!h.done && launched < maxiter) {
     const int chunk = std::min(check_every, maxiter - launched);
+    const int k_before = h.k;
     for (int c = 0; c < chunk; ++c) {
+      hipEvent_t e0 = timing ? ev[2 * c] : nullptr, e1 = timing ? ev[2 * c + 1] : nullptr;
       FX_LANE_SWITCH(lanes, launch_pcg_iter<V, T, LL, B3, DIFF>(s, a, grid_mv, grid_v, data, dinv, x, r, z, p, q,
-                                                                 st, partials));
+                                                                 st, partials, spmv_partials, e0, e1));
     }
-    FX_LAUNCHED("pcg_iter");
     launched += chunk;
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+      rc_loop = hip_fail(le, "pcg_iter");
+      break;
+    }
     FX_HIP(hipMemcpyAsync(&h, &st->k, sizeof(h), hipMemcpyDeviceToHost, s));
     FX_HIP(hipStreamSynchronize(s));
+    if (timing) {
+      for (int c = 0; c < chunk && k_before + c < h.k; ++c) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[2 * c], ev[2 * c + 1]) == hipSuccess) {
+          t_sum_ms += ms;
+          ++t_count;
+        }
+      }
+    }
   }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc_loop) return rc_loop;
   double hrs = 0.0;
   FX_HIP(hipMemcpyAsync(&hrs, &st->rs, sizeof(double), hipMemcpyDeviceToHost, s));
   FX_HIP(hipStreamSynchronize(s));
   *iters = h.k;
-  *rs = hrs;
+  rs[0] = hrs;
+  if (timing) {
+    rs[1] = t_count ? 1e3 * t_sum_ms / t_count : 0.0;
+    rs[2] = static_cast<double>(t_count);
+  }
   return FX_OK;
 }
 
@@ -460,15 +506,16 @@ int run_pcg(void* stream, int n, long long nse, int flags, const int* indptr, co
   const bool diff = (flags & FX_PCG_DIFF3) != 0;
   if (b3 && (n % 3 || !bcol)) return fail(FX_EINVAL, "pcg: block-3 kernels need n %% 3 == 0 and bcol");
   if (!b3 && !indices) return fail(FX_EINVAL, "pcg: indices missing");
-  const size_t need = kStateBytes + kPartialBytes + 4 * align256(static_cast<size_t>(n) * sizeof(T));
+  const size_t need = pcg_ws_bytes(n, sizeof(T));
   if (!workspace || ws_bytes < need) return fail(FX_EWORKSPACE, "pcg: workspace %zu < %zu bytes", ws_bytes, need);
   if (n == 0) {
     *iters = 0;
-    *rs = 0.0;
+    rs[0] = 0.0;
+    if (flags & FX_PCG_TIME) rs[1] = rs[2] = 0.0;
     return FX_OK;
   }
   PcgArgs a{n, nse, flags, indptr, b3 ? bcol : indices, rec_p, rec_pq, rec_p ? rec_max : 0};
-  const int lanes = lanes_for(n, nse, b3);
+  const int lanes = lanes_for(n, nse, b3, sizeof(V) == 8);
   if (diff) return pcg_loop<V, T, true, true>(s, a, lanes, data, dinv, b, x0, tol, atol, maxiter, check_every, x,
                                                iters, rs, workspace);
   if (b3) return pcg_loop<V, T, true, false>(s, a, lanes, data, dinv, b, x0, tol, atol, maxiter, check_every, x,
@@ -563,13 +610,13 @@ extern "C" {
 int fx_csr_spmv_f64(void* stream, int32_t n, int64_t nse, const int32_t* indptr, const int32_t* indices,
                     const double* data, const double* x, double* y) {
   return launch_spmv<double, double, false, false>(as_stream(stream), n, indptr, indices, data, x, y,
-                                                   lanes_for(n, nse, false));
+                                                   lanes_for(n, nse, false, true));
 }
 
 int fx_csr_spmv_f32(void* stream, int32_t n, int64_t nse, const int32_t* indptr, const int32_t* indices,
                     const float* data, const float* x, float* y) {
   return launch_spmv<float, float, false, false>(as_stream(stream), n, indptr, indices, data, x, y,
-                                                 lanes_for(n, nse, false));
+                                                 lanes_for(n, nse, false, false));
 }
 
 int fx_csr_block3_index(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, int32_t* bcol) {
@@ -586,21 +633,21 @@ int fx_csr_spmv_b3_f64(void* stream, int32_t n, int64_t nse, const int32_t* indp
                        const double* data, const double* x, double* y) {
   if (n % 3) return fail(FX_EINVAL, "spmv_b3: n %% 3 != 0");
   return launch_spmv<double, double, true, false>(as_stream(stream), n, indptr, bcol, data, x, y,
-                                                  lanes_for(n, nse, true));
+                                                  lanes_for(n, nse, true, true));
 }
 
 int fx_csr_spmv_b3_f32(void* stream, int32_t n, int64_t nse, const int32_t* indptr, const int32_t* bcol,
                        const float* data, const float* x, float* y) {
   if (n % 3) return fail(FX_EINVAL, "spmv_b3: n %% 3 != 0");
   return launch_spmv<float, float, true, false>(as_stream(stream), n, indptr, bcol, data, x, y,
-                                                lanes_for(n, nse, true));
+                                                lanes_for(n, nse, true, false));
 }
 
 int fx_csr_spmv_diff3_f32f64(void* stream, int32_t n, int64_t nse, const int32_t* indptr, const int32_t* bcol,
                              const float* data, const double* x, double* y) {
   if (n % 3) return fail(FX_EINVAL, "spmv_diff3: n %% 3 != 0");
   return launch_spmv<float, double, true, true>(as_stream(stream), n, indptr, bcol, data, x, y,
-                                                lanes_for(n, nse, true));
+                                                lanes_for(n, nse, true, false));
 }
 
 int fx_csr_jacobi_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, const double* data,
@@ -640,7 +687,7 @@ int fx_csr_diagonal_f32(void* stream, int32_t n, const int32_t* indptr, const in
 }
 
 size_t fx_pcg_workspace_bytes(int32_t n, int32_t elem_bytes) {
-  return kStateBytes + kPartialBytes + 4 * align256(static_cast<size_t>(std::max(n, 0)) * elem_bytes);
+  return pcg_ws_bytes(std::max(n, 0), static_cast<size_t>(elem_bytes));
 }
 
 int fx_pcg_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr, const int32_t* indices,

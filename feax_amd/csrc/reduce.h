@@ -65,12 +65,23 @@ __device__ __forceinline__ void store_partials(DVec<NV> part, int lblock, double
 // the nblocks partials of each quantity; valid in thread 0.
 template <int NV>
 __device__ __forceinline__ DVec<NV> sum_partials(const double* __restrict__ partials, int nblocks) {
+  // Eight independent loads in flight per thread: a long partial array
+  // (one entry per SpMV block) is latency-bound, not bandwidth-bound.
+  constexpr int kU = 8;
   DVec<NV> acc;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nblocks; i += kFinBlock) s += partials[k * kMaxGridBlocks + i];
-    acc.v[k] = s;
+    const double* src = partials + k * kMaxGridBlocks;
+    double s[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) s[u] = 0.0;
+    int i = threadIdx.x;
+    for (; i + (kU - 1) * kFinBlock < nblocks; i += kU * kFinBlock) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u) s[u] += src[i + u * kFinBlock];
+    }
+    for (; i < nblocks; i += kFinBlock) s[0] += src[i];
+    acc.v[k] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
   }
   return block_sum<NV, kFinBlock>(acc);
 }

@@ -39,6 +39,35 @@ class PCGInfo:
         return f"PCGInfo(iters={self.iters}, rs={self.rs:.3e})"
 
 
+class SpmvTimer:
+    """Collects the SpMV kernel times of every PCG solve run while active
+    (FX_PCG_TIME: HIP events around each SpMV launch on the solver's
+    stream).  ``kernels`` maps a kernel label ("f64/block3", "f32f64/diff3",
+    ...) to [total microseconds, launches]."""
+
+    _active = None
+
+    def __init__(self):
+        self.kernels = {}
+
+    def __enter__(self):
+        SpmvTimer._active = self
+        return self
+
+    def __exit__(self, *exc):
+        SpmvTimer._active = None
+        return False
+
+    def add(self, label, mean_us, count):
+        tot = self.kernels.setdefault(label, [0.0, 0])
+        tot[0] += mean_us * count
+        tot[1] += int(count)
+
+    def mean_us(self, label):
+        tot, cnt = self.kernels.get(label, (0.0, 0))
+        return tot / cnt if cnt else None
+
+
 def pcg(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=None, block3=None,
         check_every=32, out=None, vec_dtype=None, record=None):
     """Jacobi-PCG on the device; returns (x, PCGInfo).
@@ -64,9 +93,12 @@ def pcg(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=Non
         use_b3 = A.block3 if block3 is None else (bool(block3) and A.block3)
         flags = nat.FX_PCG_BLOCK3 if use_b3 else 0
     bcol = A.bcol if flags else None
+    timer = SpmvTimer._active
+    if timer is not None:
+        flags |= nat.FX_PCG_TIME
     ws = _workspace(n, vdt, A.data.device)
     iters = ctypes.c_int32(0)
-    rs = ctypes.c_double(0.0)
+    rs = (ctypes.c_double * 3)()
     if mdt == torch.float64:
         name = "fx_pcg_f64"
     elif vdt == torch.float64:
@@ -80,7 +112,10 @@ def pcg(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=Non
              record.capacity if record is not None else 0, nat.ptr(ws), ws.numel())
     if record is not None:
         record.count = min(iters.value, record.capacity)
-    return x, PCGInfo(iters.value, rs.value)
+    if timer is not None:
+        form = "diff3" if flags & nat.FX_PCG_DIFF3 else ("block3" if flags & nat.FX_PCG_BLOCK3 else "csr")
+        timer.add(f"{name[len('fx_pcg_'):]}/{form}", rs[1], rs[2])
+    return x, PCGInfo(iters.value, rs[0])
 
 
 class KrylovRecord:

@@ -127,6 +127,10 @@ int fx_csr_diagonal_f32(void* stream, int32_t n, const int32_t* indptr, const in
  * ---------------------------------------------------------------------- */
 #define FX_PCG_BLOCK3 1
 #define FX_PCG_DIFF3 2
+/* FX_PCG_TIME: time every SpMV launch of the executed iterations with HIP
+ * events on `stream`; rs must then point to 3 doubles: rs[0] = ||r||^2,
+ * rs[1] = mean SpMV kernel time in microseconds, rs[2] = launches timed. */
+#define FX_PCG_TIME 4
 /* Optional recording (all three drivers): when rec_p != NULL, iteration k <
  * rec_max stores its search direction p_k as FP32 in rec_p + k*n (d) and
  * p_k . A p_k in rec_pq[k] (d), for fx_krylov_project_f32. */

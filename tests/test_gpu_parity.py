@@ -223,6 +223,12 @@ def test_pcg_repeatable_and_maxiter(c1):
     x0 = torch.zeros_like(bt)
     x4, i4 = pcg(A, bt, x0, 1e-10, 0.0, 0)
     assert i4.iters == 0 and float(x4.abs().max()) == 0.0
+    # FX_PCG_TIME: one timed SpMV per executed iteration, same iterates.
+    from feax_amd.solvers.common import SpmvTimer
+    with SpmvTimer() as t:
+        x5, i5 = pcg(A, bt, None, 1e-10, 0.0, 10000)
+    assert _np(x5).tobytes() == _np(x1).tobytes() and i5.iters == i1.iters
+    assert t.kernels["f64/block3"][1] == i1.iters and t.mean_us("f64/block3") > 0.0
 
 
 def test_krylov_projection_matches_numpy():
