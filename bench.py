@@ -122,6 +122,114 @@ def cpu_baseline(problem, bc, A64, pcg_iters, budget_s=20.0):
                 asm_s=t_asm_sample * E / ns, pcg_s_per_iter=per_it)
 
 
+def simp_cantilever(dims=(120, 40, 20), load=0.25, seed=20261203):
+    """SIMP compliance workload: unit spacing, rho ~ U(0.3, 1.0) per cell
+    (cell-id order), E = 1e-9 + rho^3 (1 - 1e-9), nu = 0.3, x = 0 clamped,
+    traction t e_z on the faces of x = nx with |y - ny/2| <= 1 and
+    |z - nz/2| <= 1 (4 faces, area 4; t = 0.25: unit total load)."""
+    import numpy as np
+    import feax_amd as fe
+    nx, ny, nz = dims
+
+    class Simp(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.SIMPLinearElastic(nu=0.3, E0=1.0, Emin=1e-9, penal=3.0)
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.0, 0.0, 1.0))]
+
+    mesh = fe.StructuredGrid(dims, spacing=(1.0, 1.0, 1.0)).to_mesh()
+    patch = [lambda p: np.isclose(p[0], nx) & (np.abs(p[1] - ny / 2) <= 1 + 1e-9) & (np.abs(p[2] - nz / 2) <= 1 + 1e-9)]
+    prob = Simp(mesh, vec=3, dim=3, location_fns=patch)
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[0], 0.0), "all", 0.0)]).create_bc(prob)
+    rho = np.random.default_rng(seed).uniform(0.3, 1.0, prob.num_cells)
+    t = fe.TracedParams.create_uniform_surface_var(prob, load)
+    return prob, bc, rho, t
+
+
+def grad_bench(steps, warmup, n_fd=16, eps=1e-4):
+    """Gradient / solve ratio on the 120x40x20 SIMP cantilever (96,000 HEX8,
+    312,543 DOF): forward = assemble + solve to rel 1e-8; gradient of the
+    compliance C = f.u by (a) torch.autograd through the solver (adjoint
+    solve on the forward's operator + fx_hex8_param_vjp_f64) and (b) the
+    fused self-adjoint kernel fx_simp_compliance_sens_f64.  Checks: (a) vs
+    (b), and central differences of tight FP64 solves on 16 seeded cells."""
+    import numpy as np
+    import torch
+    import feax_amd as fe
+    from feax_amd.gene import compliance_and_sensitivity, create_compliance_fn
+    prob, bc, rho0, t = simp_cantilever()
+    ts = fe.TracedStructure.from_problem(prob)
+    assert len(prob.boundary_inds_list[0]) == 4
+    comp = create_compliance_fn(prob, surface_load_params=0.25, traced_structure=ts)
+    out = {"workload": f"SIMP cantilever 120x40x20, {ts.n} DOF, patch load 1, rel 1e-8"}
+    for form in ("fp64", "mixed"):
+        opts = fe.KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=50000,
+                                      use_jacobi_preconditioner=True, precision=form)
+        solver = fe.create_solver(prob, bc, opts, linear=True, return_solution=False, traced_structure=ts)
+        rho_t = torch.as_tensor(rho0, device="cuda")
+        tp_ng = fe.TracedParams(volume_vars=(rho_t,), surface_vars=[(t,)])
+        for _ in range(warmup):
+            solver(tp_ng)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            u = solver(tp_ng)
+        torch.cuda.synchronize()
+        t_fwd = (time.perf_counter() - t0) / steps
+        iters = solver.stats.iters
+        t_bwd = 0.0
+        for k in range(warmup + steps):
+            rho = rho_t.clone().requires_grad_(True)
+            ug = solver(fe.TracedParams(volume_vars=(rho,), surface_vars=[(t,)]))
+            C = comp(ug)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            C.backward()
+            torch.cuda.synchronize()
+            if k >= warmup:
+                t_bwd += (time.perf_counter() - t1) / steps
+        g = rho.grad
+        tpd = fe.TracedParams(volume_vars=(rho_t,))
+        t_sens = event_time(lambda: compliance_and_sensitivity(prob, tpd, u, 0.25, ts), 20)
+        _, dc = compliance_and_sensitivity(prob, tpd, u, 0.25, ts)
+        diff = float((g - dc).abs().max() / dc.abs().max())
+        out[form] = {"forward_s": round(t_fwd, 6), "iters": iters, "adjoint_s": round(t_bwd, 6),
+                     "fused_sens_s": round(t_sens, 6), "ratio_adjoint": round(t_bwd / t_fwd, 4),
+                     "ratio_fused": round(t_sens / t_fwd, 5), "autograd_vs_fused_rel": diff}
+        log(f"grad {form}: forward {t_fwd:.4f} s ({iters} it), adjoint+VJP {t_bwd:.4f} s, fused {t_sens * 1e3:.3f} ms,"
+            f" autograd vs fused {diff:.2e}")
+    # central differences of tight FP64 solves (compliance error is quadratic
+    # in the energy-norm error for a Galerkin CG iterate)
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-11, atol=0.0, maxiter=50000, use_jacobi_preconditioner=True)
+    fsolve = fe.create_solver(prob, bc, opts, linear=True, return_solution=False, traced_structure=ts)
+    rho_t = torch.as_tensor(rho0, device="cuda")
+    _, dc = compliance_and_sensitivity(prob, fe.TracedParams(volume_vars=(rho_t,)),
+                                       fsolve(fe.TracedParams(volume_vars=(rho_t,), surface_vars=[(t,)])), 0.25, ts)
+    idx = np.random.default_rng(20261203).choice(prob.num_cells, n_fd, replace=False)
+    errs = []
+    for i in idx:
+        vals = []
+        for sgn in (1.0, -1.0):
+            r = rho0.copy()
+            r[i] += sgn * eps
+            rt = torch.as_tensor(r, device="cuda")
+            vals.append(float(comp(fsolve(fe.TracedParams(volume_vars=(rt,), surface_vars=[(t,)])))))
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        errs.append(abs(fd - float(dc[i])))
+    # Relative to the largest sensitivity: cells with tiny sensitivities see
+    # the rounding of C itself in their differences.
+    scale = float(dc.abs().max())
+    out["fd_max_err_16"] = float(max(errs)) / scale
+    out["fd_cells"] = [int(i) for i in idx]
+    best = min(("fp64", "mixed"), key=lambda f: out[f]["forward_s"])
+    out["form"] = best
+    out["ratio_adjoint"] = out[best]["ratio_adjoint"]
+    log(f"grad: FD (16 cells, eps {eps}) max err / max|dC| {out['fd_max_err_16']:.2e}; ratio ({best}) "
+        f"{out['ratio_adjoint']:.3f}")
+    return out
+
+
 def kernels_only(cfg, iters=8, copy_bytes=1 << 30):
     """A short run for hardware-counter passes: `iters` PCG iterations in
     each matrix form (so pcg_spmv_kernel is dispatched `iters` times per
@@ -157,6 +265,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-grad", action="store_true", help="skip the SIMP compliance-gradient workload")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch only the PCG SpMV kernels and a 1 GiB copy (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -243,6 +352,7 @@ def main():
     dom["t"] = t_live * 1e-6
     achieved = dom["B"] / dom["t"]
     log(f"in-solve SpMV {dom['label']}: {t_live:.2f} us avg over {n_live} launches -> {achieved / 1e9:.0f} GB/s")
+    grad = None if args.no_grad else grad_bench(args.steps, args.warmup)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(problem, bc, A64, results["fp64"]["iters"])
@@ -273,7 +383,8 @@ def main():
                  "in_solve_launches": {k: v[1] for k, v in spmv_live.items()},
                  "B_alg_fp64": B64, "B_alg_diff3": Bd, "B_min_fp64": Bmin64, "B_min_diff3": Bmind},
         "assemble_fp64_ms": round(t_asm64 * 1e3, 3),
-        "grad_solve_ratio": None,
+        "grad_solve_ratio": None if grad is None else grad["ratio_adjoint"],
+        "grad": grad,
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                      "traffic": TRAFFIC.get(dom["label"]), "kernel_us": round(dom["t"] * 1e6, 2),

@@ -7,7 +7,7 @@ CSRMatrix.  All compute runs in libfeaxhip.so (hand-written HIP for
 gfx950); the package itself holds only host-side structure and dispatch.
 """
 
-from . import materials
+from . import gene, materials
 from .assembler import (TracedStructure, create_J_bc_csr_parametric, create_res_bc_parametric,
                         create_res_J_bc_csr_parametric, get_res)
 from .csr import CSRMatrix

@@ -187,6 +187,93 @@ def residual(problem, ts, traced_params, sol, bc=None, negate=False, out=None):
     return res
 
 
+def volume_residual(problem, ts, traced_params, sol, bc=None, negate=False, out=None):
+    """The volume part of R_bc: no traction, BC rows u[bc] - 0 when bc is given
+    (K_bc u for a linear law with u[bc] = 0; the adjoint operator)."""
+    import torch
+    mat, p = resolve_material(problem, traced_params, ts)
+    u = _as_dev(sol, ts)
+    res = torch.empty(ts.n, dtype=torch.float64, device=ts.device) if out is None else out
+    if bc is not None and len(bc.bc_rows):
+        rows = bc.device(ts.device)[0]
+        key = ("zero_bc_vals", len(bc.bc_rows))
+        if key not in ts._cache:
+            ts._cache[key] = torch.zeros(len(bc.bc_rows), dtype=torch.float64, device=ts.device)
+        vals, nbc = ts._cache[key], len(bc.bc_rows)
+    else:
+        rows = vals = None
+        nbc = 0
+    nat.call("fx_hex8_residual_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
+             nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat),
+             nat.ptr(p), nat.ptr(u), 0, None, None, None, None, None, nbc, nat.ptr(rows), nat.ptr(vals),
+             int(bool(negate)), nat.ptr(res))
+    return res
+
+
+def param_vjp(problem, ts, traced_params, sol, lam, bc=None, scale=1.0):
+    """scale * (dR/dp)^T lam for p = volume_vars[0] (fx_hex8_param_vjp_f64);
+    lam is read as 0 on the Dirichlet rows of bc.  Returns a float64 device
+    tensor shaped like the parameter (cell (E,) or node (N,))."""
+    import torch
+    mat, p = resolve_material(problem, traced_params, ts)
+    u = _as_dev(sol, ts)
+    lam = _as_dev(lam, ts)
+    mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
+    grad = torch.empty_like(p, dtype=torch.float64)
+    wsb = int(nat.lib().fx_hex8_param_vjp_workspace_bytes(ts.num_cells, mat.param_loc))
+    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=ts.device)
+    nat.call("fx_hex8_param_vjp_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
+             nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat),
+             nat.ptr(p), nat.ptr(u), nat.ptr(lam), nat.ptr(mask), float(scale), nat.ptr(grad), nat.ptr(ws), wsb)
+    return grad
+
+
+def traction_vjp(problem, ts, lam, bc=None, scale=1.0):
+    """scale * d(lam . R)/d svar per loaded boundary: a list of (F_i, 4)
+    float64 device tensors (fx_hex8_traction_vjp_f64)."""
+    import torch
+    if ts.num_faces == 0:
+        return []
+    lam = _as_dev(lam, ts)
+    mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
+    g = torch.empty(ts.num_faces * 4, dtype=torch.float64, device=ts.device)
+    nat.call("fx_hex8_traction_vjp_f64", nat.stream(), ts.num_faces, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
+             nat.ptr(ts.cells), nat.ptr(ts.faces), nat.ptr(ts.fdir), nat.ptr(lam), nat.ptr(mask), float(scale),
+             nat.ptr(g))
+    g = g.reshape(-1, 4)
+    off = ts.face_offsets
+    return [g[off[i]:off[i + 1]] for i in range(len(off) - 1)]
+
+
+def compliance_sensitivity(problem, ts, traced_params, sol):
+    """Fused dC/drho_e = -u_e^T dK_e/drho_e u_e for a linear law with a cell
+    variable (fx_simp_compliance_sens_f64): the self-adjoint gradient of the
+    compliance when the Dirichlet values are zero."""
+    import torch
+    mat, p = resolve_material(problem, traced_params, ts)
+    u = _as_dev(sol, ts)
+    dc = torch.empty(ts.num_cells, dtype=torch.float64, device=ts.device)
+    nat.call("fx_simp_compliance_sens_f64", nat.stream(), ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
+             nat.ptr(ts.cells), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u), nat.ptr(dc))
+    return dc
+
+
+def strain_energy(problem, ts, traced_params, sol, with_partial=False):
+    """Neo-Hookean strain energy Pi(u) (device scalar) and, with_partial,
+    dPi/dmu_e per element (fx_hex8_energy_f64)."""
+    import torch
+    mat, p = resolve_material(problem, traced_params, ts)
+    u = _as_dev(sol, ts)
+    out = torch.empty(1, dtype=torch.float64, device=ts.device)
+    dmu = torch.empty(ts.num_cells, dtype=torch.float64, device=ts.device) if with_partial else None
+    wsb = int(nat.lib().fx_hex8_energy_workspace_bytes(ts.num_cells))
+    ws = torch.empty(wsb, dtype=torch.uint8, device=ts.device)
+    nat.call("fx_hex8_energy_f64", nat.stream(), ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
+             nat.ptr(ts.cells), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u), nat.ptr(out), nat.ptr(dmu),
+             nat.ptr(ws), wsb)
+    return (out[0], dmu) if with_partial else out[0]
+
+
 def _ts_or_build(problem, ts):
     if ts is not None:
         return ts

@@ -22,133 +22,25 @@
 // No float atomics; bitwise repeatable; CSR values written exactly once.
 #include <cstdint>
 
-#include "common.h"
+#include "hex8_common.h"
 
 namespace fx {
 
 constexpr int kNodesPerBlock = 4;  // one wave per node, 4 waves per block
 constexpr int kMaxDeg = 64;        // LDS accumulator: 9 * kMaxDeg values per wave
 
-__device__ __forceinline__ double gauss_pt(int i) {
-  // (1 -+ 1/sqrt(3)) / 2: the 2-point Gauss-Legendre rule on [0, 1].
-  return i ? 0.78867513459481288225 : 0.21132486540518711775;
-}
-
-// Reference-cell corner of feax local node a (basix vertex order re-ordered
-// by [0,1,3,2,4,5,7,6], feax/basis.py:65-66).
-__device__ __forceinline__ void corner(int a, int& c0, int& c1, int& c2) {
-  c0 = (a == 1 || a == 2 || a == 5 || a == 6);
-  c1 = (a == 2 || a == 3 || a == 6 || a == 7);
-  c2 = (a >= 4);
-}
-
-template <typename T>
-__device__ __forceinline__ void shape(int a, T x0, T x1, T x2, T& N, T& d0, T& d1, T& d2) {
-  int c0, c1, c2;
-  corner(a, c0, c1, c2);
-  const T f0 = c0 ? x0 : T(1) - x0, f1 = c1 ? x1 : T(1) - x1, f2 = c2 ? x2 : T(1) - x2;
-  const T s0 = c0 ? T(1) : T(-1), s1 = c1 ? T(1) : T(-1), s2 = c2 ? T(1) : T(-1);
-  N = f0 * f1 * f2;
-  d0 = s0 * f1 * f2;
-  d1 = f0 * s1 * f2;
-  d2 = f0 * f1 * s2;
-}
-
-// J_ij = sum_c X_c,i dN_c/dxi_j; returns det J and writes J^-1 (feax/fe.py:96-129).
-template <typename T>
-__device__ __forceinline__ T jac_inv(const T (&X)[8][3], T x0, T x1, T x2, T (&Ji)[3][3]) {
-  T J[3][3] = {{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    T N, d0, d1, d2;
-    shape<T>(c, x0, x1, x2, N, d0, d1, d2);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      J[i][0] += X[c][i] * d0;
-      J[i][1] += X[c][i] * d1;
-      J[i][2] += X[c][i] * d2;
-    }
-  }
-  const T c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-  const T c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-  const T c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-  const T det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-  const T id = T(1) / det;
-  Ji[0][0] = c00 * id;
-  Ji[1][0] = c01 * id;
-  Ji[2][0] = c02 * id;
-  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-  return det;
-}
-
-// Physical gradient g_i = sum_j dN/dxi_j (J^-1)_ji.
-template <typename T>
-__device__ __forceinline__ void phys_grad(const T (&Ji)[3][3], T d0, T d1, T d2, T (&g)[3]) {
-#pragma unroll
-  for (int i = 0; i < 3; ++i) g[i] = d0 * Ji[0][i] + d1 * Ji[1][i] + d2 * Ji[2][i];
-}
-
-struct MatParams {
-  int kind, loc;
-  double nu, E0, Emin, penal, kappa;
-};
-
-// Element coordinates relative to local node 0 (differences taken in FP64,
-// so FP32 geometry does not lose digits to large absolute coordinates).
-template <typename T>
-__device__ __forceinline__ void load_coords(const double* __restrict__ x, const double* __restrict__ y,
-                                            const double* __restrict__ z, const int* __restrict__ cn,
-                                            T (&X)[8][3]) {
-  const int n0 = cn[0];
-  const double ox = x[n0], oy = y[n0], oz = z[n0];
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int nd = cn[c];
-    X[c][0] = static_cast<T>(x[nd] - ox);
-    X[c][1] = static_cast<T>(y[nd] - oy);
-    X[c][2] = static_cast<T>(z[nd] - oz);
-  }
-}
-
-// Material parameter at the quadrature point (feax/assembler.py:227-269:
-// node vars interpolated with N, cell vars broadcast), then the modulus.
-template <typename T>
-__device__ __forceinline__ T modulus_at(const MatParams& m, const double* __restrict__ param, int e,
-                                        const int* __restrict__ cn, T x0, T x1, T x2) {
-  T p;
-  if (m.loc == FX_PARAM_NODE) {
-    p = T(0);
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      T N, d0, d1, d2;
-      shape<T>(c, x0, x1, x2, N, d0, d1, d2);
-      p += N * static_cast<T>(param[cn[c]]);
-    }
-  } else {
-    p = static_cast<T>(param[e]);
-  }
-  if (m.kind == FX_MAT_SIMP) p = static_cast<T>(m.Emin) + pow(p, static_cast<T>(m.penal)) * static_cast<T>(m.E0 - m.Emin);
-  return p;
-}
-
-template <typename T>
-__device__ __forceinline__ T xor_sum(T v, int mask) {
-  return v + __shfl_xor(v, mask, 64);
-}
-
 // ------------------------------------------------------------------ assembly
-template <typename T, typename C>
+// NH: Neo-Hookean tangent at the state u (the Gauss-point deformation
+// gradient is formed from the element's 8 nodal displacements across the
+// 8 lanes of each Gauss-point group).
+template <typename T, typename C, bool NH>
 __global__ __launch_bounds__(64 * kNodesPerBlock) void assemble_kernel(
     int num_nodes, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     const int* __restrict__ cells, const int* __restrict__ inc_ptr, const int* __restrict__ inc,
     const uint8_t* __restrict__ rank, const int* __restrict__ indptr, const int* __restrict__ indices,
-    MatParams mat, const double* __restrict__ param, const uint8_t* __restrict__ bc_mask, int symmetric,
-    int diff_form, double jacobi_shift, double* __restrict__ dinv, T* __restrict__ data) {
+    MatParams mat, const double* __restrict__ param, const double* __restrict__ u,
+    const uint8_t* __restrict__ bc_mask, int symmetric, int diff_form, double jacobi_shift,
+    double* __restrict__ dinv, T* __restrict__ data) {
   __shared__ C acc_all[kNodesPerBlock][9 * kMaxDeg];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nd = blockIdx.x * kNodesPerBlock + wid;
@@ -180,16 +72,52 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void assemble_kernel(
     phys_grad<C>(Ji, d0, d1, d2, ga);
     shape<C>(b, x0, x1, x2, N, d0, d1, d2);
     phys_grad<C>(Ji, d0, d1, d2, gb);
-    const C E = modulus_at<C>(mat, param, e, cn, x0, x1, x2);
     const C w = det * C(0.125);  // JxW, Gauss weights 1/8 on [0,1]^3
-    const C wl = w * E * lam0, wm = w * E * mu0;
-    const C gg = wm * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
     C K[9];
+    if constexpr (NH) {
+      // grad u at this Gauss point: lane b holds u_b (x) grad N_b, summed
+      // over the 8 lanes of the point's group.
+      const int nb = cn[b];
+      C ug[3][3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
-        K[3 * i + j] = wl * ga[i] * gb[j] + wm * ga[j] * gb[i] + (i == j ? gg : C(0));
+        for (int d = 0; d < 3; ++d) {
+          C v = static_cast<C>(u[3 * nb + i]) * gb[d];
+          v = xor_sum(v, 1);
+          v = xor_sum(v, 2);
+          v = xor_sum(v, 4);
+          ug[i][d] = v;
+        }
+      C FiT[3][3];
+      const C J = inv_transpose_F<C>(ug, FiT);
+      const C mu = param_at<C>(mat, param, e, cn, x0, x1, x2);
+      const C kap = static_cast<C>(mat.kappa);
+      // dP/dF = mu d_ik d_JL + kappa (2J-1) J F^-T_iJ F^-T_kL - c F^-T_iL F^-T_kJ,
+      // c = -mu + kappa J (J-1); contracted with grad N_a (J) and grad N_b (L).
+      const C c1 = kap * (C(2) * J - C(1)) * J, c2 = -mu + kap * J * (J - C(1));
+      C fa[3], fb[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        fa[i] = FiT[i][0] * ga[0] + FiT[i][1] * ga[1] + FiT[i][2] * ga[2];
+        fb[i] = FiT[i][0] * gb[0] + FiT[i][1] * gb[1] + FiT[i][2] * gb[2];
+      }
+      const C gg = mu * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          K[3 * i + j] = w * ((i == j ? gg : C(0)) + c1 * fa[i] * fb[j] - c2 * fb[i] * fa[j]);
+    } else {
+      const C E = modulus_at<C>(mat, param, e, cn, x0, x1, x2);
+      const C wl = w * E * lam0, wm = w * E * mu0;
+      const C gg = wm * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          K[3 * i + j] = wl * ga[i] * gb[j] + wm * ga[j] * gb[i] + (i == j ? gg : C(0));
+    }
     // sum over the 8 Gauss points (lanes b, b+8, ..., b+56)
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
@@ -259,32 +187,6 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void assemble_kernel(
 }
 
 // ------------------------------------------------------------------ residual
-// Face quadrature point (lf, q) in the reference cell: the 2x2 Gauss points
-// of the unit square mapped onto basix facet lf (feax/basis.py:286-305).
-__device__ __forceinline__ void face_point(int lf, int q, double& p0, double& p1, double& p2) {
-  const double s = gauss_pt(q >> 1), t = gauss_pt(q & 1);
-  switch (lf) {
-    case 0: p0 = s; p1 = t; p2 = 0.0; break;
-    case 1: p0 = s; p1 = 0.0; p2 = t; break;
-    case 2: p0 = 0.0; p1 = s; p2 = t; break;
-    case 3: p0 = 1.0; p1 = s; p2 = t; break;
-    case 4: p0 = s; p1 = 1.0; p2 = t; break;
-    default: p0 = s; p1 = t; p2 = 1.0; break;
-  }
-}
-
-__device__ __forceinline__ void face_normal(int lf, double (&nrm)[3]) {
-  nrm[0] = nrm[1] = nrm[2] = 0.0;
-  switch (lf) {
-    case 0: nrm[2] = -1.0; break;
-    case 1: nrm[1] = -1.0; break;
-    case 2: nrm[0] = -1.0; break;
-    case 3: nrm[0] = 1.0; break;
-    case 4: nrm[1] = 1.0; break;
-    default: nrm[2] = 1.0; break;
-  }
-}
-
 __global__ __launch_bounds__(64 * kNodesPerBlock) void residual_kernel(
     int num_nodes, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     const int* __restrict__ cells, const int* __restrict__ inc_ptr, const int* __restrict__ inc, MatParams mat,
@@ -296,8 +198,6 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void residual_kernel(
   if (nd >= num_nodes) return;
   const int q = lane >> 3, c = lane & 7;
   const double x0 = gauss_pt(q >> 2), x1 = gauss_pt((q >> 1) & 1), x2 = gauss_pt(q & 1);
-  const double nu = mat.nu;
-  const double lam0 = nu / ((1 + nu) * (1 - 2 * nu)), mu0 = 1 / (2 * (1 + nu));
   double r0 = 0.0, r1 = 0.0, r2 = 0.0;
   for (int k = inc_ptr[nd]; k < inc_ptr[nd + 1]; ++k) {
     const int pk = inc[k];
@@ -313,32 +213,23 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void residual_kernel(
     shape<double>(a, x0, x1, x2, N, d0, d1, d2);
     phys_grad<double>(Ji, d0, d1, d2, ga);
     const int nc = cn[c];
-    double ug[9];  // du_i/dx_d, summed over the 8 nodes of this Gauss point
+    double ug[3][3];  // du_i/dx_d, summed over the 8 nodes of this Gauss point
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) ug[3 * i + d] = u[3 * nc + i] * gc[d];
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-      double v = ug[t];
-      v = xor_sum(v, 1);
-      v = xor_sum(v, 2);
-      v = xor_sum(v, 4);
-      ug[t] = v;
-    }
-    const double E = modulus_at<double>(mat, param, e, cn, x0, x1, x2);
-    const double lam = E * lam0, mu = E * mu0;
-    const double tr = ug[0] + ug[4] + ug[8];
-    double sig[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-        sig[3 * i + d] = mu * (ug[3 * i + d] + ug[3 * d + i]) + (i == d ? lam * tr : 0.0);
+      for (int d = 0; d < 3; ++d) {
+        double v = u[3 * nc + i] * gc[d];
+        v = xor_sum(v, 1);
+        v = xor_sum(v, 2);
+        v = xor_sum(v, 4);
+        ug[i][d] = v;
+      }
+    double sig[3][3];
+    material_stress<double>(mat, param_at<double>(mat, param, e, cn, x0, x1, x2), ug, sig);
     const double w = det * 0.125;
     double ri[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) ri[i] = (sig[3 * i] * ga[0] + sig[3 * i + 1] * ga[1] + sig[3 * i + 2] * ga[2]) * w;
+    for (int i = 0; i < 3; ++i) ri[i] = (sig[i][0] * ga[0] + sig[i][1] * ga[1] + sig[i][2] * ga[2]) * w;
     // every lane of a Gauss-point group holds the same value: sum over groups
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -365,14 +256,7 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void residual_kernel(
         face_point(lf, lane, p0, p1, p2);
         double X[8][3];
         load_coords<double>(x, y, z, cells + 8 * static_cast<long>(e), X);
-        double Ji[3][3];
-        const double det = jac_inv<double>(X, p0, p1, p2, Ji);
-        double nrm[3];
-        face_normal(lf, nrm);
-        double m[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) m[i] = nrm[0] * Ji[0][i] + nrm[1] * Ji[1][i] + nrm[2] * Ji[2][i];
-        const double nanson = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) * det * 0.25;
+        const double nanson = face_scale(X, lf, p0, p1, p2);
         double N, d0, d1, d2;
         shape<double>(a, p0, p1, p2, N, d0, d1, d2);
         v = svar[4 * f + lane] * N * nanson;
@@ -401,18 +285,6 @@ __global__ void bc_res_kernel(int nbc, const int* __restrict__ rows, const doubl
   res[r] = negate ? -v : v;
 }
 
-inline MatParams to_params(const fx_material* m) {
-  MatParams p;
-  p.kind = m->kind;
-  p.loc = m->param_loc;
-  p.nu = m->nu;
-  p.E0 = m->E0;
-  p.Emin = m->Emin;
-  p.penal = m->penal;
-  p.kappa = m->kappa;
-  return p;
-}
-
 template <typename T>
 int assemble(void* stream, int num_nodes, int num_cells, const double* x, const double* y, const double* z,
              const int* cells, const int* inc_ptr, const int* inc, const uint8_t* rank, const int* indptr,
@@ -421,24 +293,29 @@ int assemble(void* stream, int num_nodes, int num_cells, const double* x, const 
   if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !z || !cells || !inc_ptr || !inc || !rank || !indptr ||
       !indices || !mat || !param || !data)
     return fail(FX_EINVAL, "fx_hex8_assemble: bad arguments");
-  if (mat->kind == FX_MAT_NEO_HOOKEAN) return fail(FX_EINVAL, "fx_hex8_assemble: Neo-Hookean tangent not built");
-  if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP)
+  if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
     return fail(FX_EINVAL, "fx_hex8_assemble: unknown material %d", mat->kind);
-  (void)u;
+  const bool nh = mat->kind == FX_MAT_NEO_HOOKEAN;
+  if (nh && !u) return fail(FX_EINVAL, "fx_hex8_assemble: the Neo-Hookean tangent needs u");
   const int diff = (flags & FX_ASM_DIFF3) != 0;
   const int grid = (num_nodes + kNodesPerBlock - 1) / kNodesPerBlock;
+  const MatParams mp = to_params(mat);
+  hipStream_t s = as_stream(stream);
   // Element matrices are formed in FP64 and rounded once at the store, unless
   // FX_ASM_COMPUTE32 asks for FP32 arithmetic as well.
-  if (sizeof(T) == 4 && (flags & FX_ASM_COMPUTE32)) {
-    hipLaunchKernelGGL((assemble_kernel<T, float>), dim3(grid), dim3(64 * kNodesPerBlock), 0, as_stream(stream),
-                       num_nodes, x, y, z, cells, inc_ptr, inc, rank, indptr, indices, to_params(mat), param, bc_mask,
-                       symmetric, diff, jacobi_shift, dinv, data);
-    FX_LAUNCHED("assemble_kernel");
-    return FX_OK;
+  const bool c32 = sizeof(T) == 4 && (flags & FX_ASM_COMPUTE32);
+#define FX_ASM_LAUNCH(C, NHV)                                                                                    \
+  hipLaunchKernelGGL((assemble_kernel<T, C, NHV>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s, num_nodes, x, y, z, \
+                     cells, inc_ptr, inc, rank, indptr, indices, mp, param, u, bc_mask, symmetric, diff,          \
+                     jacobi_shift, dinv, data)
+  if (nh) {
+    if (c32) FX_ASM_LAUNCH(float, true);
+    else FX_ASM_LAUNCH(double, true);
+  } else {
+    if (c32) FX_ASM_LAUNCH(float, false);
+    else FX_ASM_LAUNCH(double, false);
   }
-  hipLaunchKernelGGL((assemble_kernel<T, double>), dim3(grid), dim3(64 * kNodesPerBlock), 0, as_stream(stream), num_nodes,
-                     x, y, z, cells, inc_ptr, inc, rank, indptr, indices, to_params(mat), param, bc_mask,
-                     symmetric, diff, jacobi_shift, dinv, data);
+#undef FX_ASM_LAUNCH
   FX_LAUNCHED("assemble_kernel");
   return FX_OK;
 }
@@ -476,8 +353,8 @@ int fx_hex8_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, con
   if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !z || !cells || !inc_ptr || !inc || !mat || !param || !u ||
       !res)
     return fail(FX_EINVAL, "fx_hex8_residual_f64: bad arguments");
-  if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP)
-    return fail(FX_EINVAL, "fx_hex8_residual_f64: material %d not built", mat->kind);
+  if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
+    return fail(FX_EINVAL, "fx_hex8_residual_f64: unknown material %d", mat->kind);
   if (num_faces > 0 && (!faces || !finc_ptr || !finc || !svar || !fdir))
     return fail(FX_EINVAL, "fx_hex8_residual_f64: face arrays missing");
   hipStream_t s = as_stream(stream);

@@ -1,9 +1,22 @@
 """Gradients of the linear solve through torch.autograd.
 
-torch.autograd.Function is the analogue of feax's jax.custom_vjp
-(feax/solvers/linear.py:387-483).  When no traced parameter needs a
-gradient the forward runs alone.
+torch.autograd.Function is the analogue of feax's jax.custom_vjp around
+the linear solve (feax/solvers/linear.py:387-483):
+
+  forward   u = u0 + J_bc^-1 (-R_bc(u0))        (the HIP solve, unchanged)
+  backward  lambda = J_bc^-T v                   J_bc^T = J_bc under symmetric
+                                                 elimination; lambda[bc] = 0
+            grad p    = -(dR/dp)^T lambda        fx_hex8_param_vjp_f64
+            grad svar = -(dR/dsvar)^T lambda     fx_hex8_traction_vjp_f64
+
+The adjoint system reuses the forward's operator: the FP64 CSR (precision
+"fp64"), or the FP32 matrix inside an FP64 defect correction whose outer
+residual is the matrix-free volume residual (precision "mixed"), started
+from the Galerkin projection onto the forward's recorded Krylov directions
+when they are still held.  Initial guesses get no gradient, as in feax.
 """
+
+import numpy as np
 
 
 def _needs_grad(tp):
@@ -16,7 +29,145 @@ def _needs_grad(tp):
     return False
 
 
+def _leaves(tp):
+    """[(path, tensor)] of the torch tensors in a TracedParams; path is
+    ("v", i) or ("s", i, j)."""
+    import torch
+    out = []
+    for i, v in enumerate(tp.volume_vars):
+        if isinstance(v, torch.Tensor):
+            out.append((("v", i), v))
+    for i, s in enumerate(tp.surface_vars):
+        for j, v in enumerate(s):
+            if isinstance(v, torch.Tensor):
+                out.append((("s", i, j), v))
+    return out
+
+
+def _rebuild(tp, paths, tensors):
+    from ..traced_params import TracedParams
+    vv = list(tp.volume_vars)
+    sv = [list(s) for s in tp.surface_vars]
+    for path, t in zip(paths, tensors):
+        if path[0] == "v":
+            vv[path[1]] = t
+        else:
+            sv[path[1]][path[2]] = t
+    return TracedParams(volume_vars=tuple(vv), surface_vars=[tuple(s) for s in sv])
+
+
+def adjoint_solve(problem, ts, tp, bc, v, keep, opts):
+    """lambda = J_bc^-1 v with lambda[bc] = 0, on the forward's operator."""
+    import torch
+    from ..assembler import volume_residual
+    from .common import device_norm, pcg
+    rhs = v.detach().to(device=ts.device, dtype=torch.float64).reshape(-1).clone()
+    if bc is not None and len(bc.bc_rows):
+        rhs.index_fill_(0, bc.device(ts.device)[0].long(), 0.0)
+    A, dinv = keep["A"], keep["dinv"]
+    if keep["form"] == "fp64":
+        lam, info = pcg(A, rhs, None, opts.tol, opts.atol, opts.maxiter, dinv, None, opts.check_every)
+        keep["adjoint_iters"] = [info.iters]
+        return lam
+    # FP32 operator in an FP64 defect correction.
+    rec = keep.get("record")
+    if rec is not None and (getattr(rec, "owner", None) is not keep.get("token") or rec.count == 0):
+        rec = None
+    lam = torch.zeros_like(rhs)
+    r = rhs.clone()
+    d = torch.empty_like(rhs)
+    bnorm = device_norm(r)
+    target = max(opts.tol * bnorm, opts.atol)
+    rn, outer, budget, inner = bnorm, 0, opts.maxiter, []
+    while rn > target and outer < opts.max_outer and budget > 0:
+        x0 = rec.project(r) if rec is not None else None
+        _, info = pcg(A, r, x0, opts.inner_tol, 0.0, budget, dinv, True, opts.check_every, out=d,
+                      vec_dtype=torch.float64)
+        lam += d
+        res = volume_residual(problem, ts, tp, lam, bc, negate=True)   # -K_bc lam
+        torch.add(rhs, res, out=r)
+        rn = device_norm(r)
+        inner.append(info.iters)
+        budget -= info.iters
+        outer += 1
+    keep["adjoint_iters"] = inner
+    return lam
+
+
+def parameter_gradients(problem, ts, tp, bc, u, lam, paths, shapes, needs):
+    """-(dR_bc/dp)^T lambda for every traced tensor (None where not needed)."""
+    from ..assembler import param_vjp, traction_vjp
+    grads = []
+    tv = None
+    for path, shape, need in zip(paths, shapes, needs):
+        if not need:
+            grads.append(None)
+            continue
+        if path == ("v", 0):
+            g = param_vjp(problem, ts, tp, u, lam, bc, scale=-1.0)
+            grads.append(g.reshape(shape))
+        elif path[0] == "s" and path[2] == 0:
+            if tv is None:
+                tv = traction_vjp(problem, ts, lam, bc, scale=-1.0)
+            g = tv[path[1]]
+            nel = int(np.prod(shape)) if len(shape) else 1
+            if nel == g.numel():
+                grads.append(g.reshape(shape))
+            elif nel == g.shape[0]:
+                grads.append(g.sum(dim=1).reshape(shape))
+            elif nel == 1:
+                grads.append(g.sum().reshape(shape))
+            else:
+                raise NotImplementedError(f"gradient for a surface var of shape {tuple(shape)}")
+        else:
+            # Further volume/surface vars do not enter the HIP residual.
+            import torch
+            grads.append(torch.zeros(shape, dtype=torch.float64, device=ts.device))
+    return grads
+
+
+def _function():
+    import torch
+
+    class LinearSolveFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, call, *tensors):
+            tp = _rebuild(call["tp"], call["paths"], [t.detach() for t in tensors])
+            keep = {}
+            u = call["forward"](tp, call["ig"], call["bc"], call["ts"], keep=keep)
+            ctx.call, ctx.keep, ctx.tp = call, keep, tp
+            ctx.shapes = [tuple(t.shape) for t in tensors]
+            ctx.dtypes = [t.dtype for t in tensors]
+            ctx.save_for_backward(u)
+            return u
+
+        @staticmethod
+        def backward(ctx, v):
+            (u,) = ctx.saved_tensors
+            call, keep = ctx.call, ctx.keep
+            ts = keep["ts"]
+            lam = adjoint_solve(call["problem"], ts, ctx.tp, call["bc"], v, keep, call["adj_opts"])
+            grads = parameter_gradients(call["problem"], ts, ctx.tp, call["bc"], u, lam, call["paths"],
+                                        ctx.shapes, ctx.needs_input_grad[1:])
+            grads = [g if g is None else g.to(dt) for g, dt in zip(grads, ctx.dtypes)]
+            return (None, *grads)
+
+    return LinearSolveFunction
+
+
+_FN = None
+
+
 def differentiable_linear_solve(problem, forward, tp, ig, bc, ts, adj_opts, symmetric):
     if not _needs_grad(tp):
         return forward(tp, ig, bc, ts)
-    raise NotImplementedError("adjoint kernels are not built yet")
+    global _FN
+    if _FN is None:
+        _FN = _function()
+    leaves = _leaves(tp)
+    call = dict(problem=problem, forward=forward, tp=tp, ig=ig, bc=bc, ts=ts, adj_opts=adj_opts,
+                paths=[p for p, _ in leaves])
+    return _FN.apply(call, *[t for _, t in leaves])
+
+
+__all__ = ["differentiable_linear_solve", "adjoint_solve", "parameter_gradients"]

@@ -47,8 +47,9 @@ def _b3(options):
     return {"auto": None, "csr": False, "block3": True}[options.spmv]
 
 
-def solve_fp64(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None):
-    """One FP64 solve on the assembled CSR.  Returns u (FP64 device vector)."""
+def solve_fp64(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None, keep=None):
+    """One FP64 solve on the assembled CSR.  Returns u (FP64 device vector).
+    keep (dict): receives the operator for an adjoint solve."""
     import torch
     if opts.use_jacobi_preconditioner:
         J, dinv = assemble_csr(problem, ts, tp, u0, bc, symmetric, torch.float64, jacobi_shift=opts.jacobi_shift)
@@ -58,6 +59,8 @@ def solve_fp64(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None):
     x0_fn = opts.x0_fn if opts.x0_fn is not None else create_x0(bc.bc_rows, bc.bc_vals)
     x0 = x0_fn(u0)
     du, info = pcg(J, b, x0, opts.tol, opts.atol, opts.maxiter, dinv, _b3(opts), opts.check_every)
+    if keep is not None:
+        keep.update(form="fp64", A=J, dinv=dinv)
     if opts.check_convergence:
         du = check_convergence(J, du, b, opts)
     if stats is not None:
@@ -68,7 +71,7 @@ def solve_fp64(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None):
     return u
 
 
-def solve_mixed(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None):
+def solve_mixed(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None, keep=None):
     """FP32 matrix inside an FP64 defect correction.  Returns u (FP64).
 
     precision "mixed": FP32 difference-form CSR (FX_ASM_DIFF3) applied to
@@ -101,6 +104,10 @@ def solve_mixed(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None):
     if opts.recycle > 0 and not fp32vec:
         record = _record_for(n, min(opts.recycle, opts.maxiter), u.device)
         record.count = 0
+        record.owner = object()
+    if keep is not None:
+        keep.update(form="mixed", A=A, dinv=dinv, record=record,
+                    token=record.owner if record is not None else None)
     while rn > target and outer < opts.max_outer and budget > 0:
         if fp32vec:
             nat.call("fx_convert_f64_to_f32", nat.stream(), n, 1.0 / rn, nat.ptr(r), nat.ptr(rv))
@@ -164,14 +171,16 @@ def create_linear_solver(problem, bc, solver_options=None, adjoint_solver_option
     ts_default = traced_structure
     stats = SolveStats()
 
-    def forward(tp, ig, eff_bc, ts):
+    def forward(tp, ig, eff_bc, ts, keep=None):
         if ts is None:
             ts = getattr(problem, "_default_ts", None) or TracedStructure.from_problem(problem)
             problem._default_ts = ts
+        if keep is not None:
+            keep["ts"] = ts
         u0 = _dev_vec(ig, ts) if ig is not None else _zero_like(problem, eff_bc, ts)
         if solver_options.precision != "fp64":
-            return solve_mixed(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats)
-        return solve_fp64(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats)
+            return solve_mixed(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
+        return solve_fp64(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
 
     def solver_wrapper(traced_params, initial_guess=None, bc=None, traced_structure=None):
         eff_bc = bc if isinstance(bc, DirichletBC) else default_bc

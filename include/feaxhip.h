@@ -231,6 +231,56 @@ int fx_hex8_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, con
                          const int32_t* finc, const double* svar, const double* fdir, int32_t num_bc,
                          const int32_t* bc_rows, const double* bc_vals, int32_t negate, double* res);
 
+/* ------------------------------------------------------------------------
+ * Adjoint.  feax's backward pass of the linear solve (jax.custom_vjp,
+ * feax/solvers/linear.py:426-481) is lambda = J_bc^-T v followed by
+ * grads = -(dR_bc/dparams)^T lambda, the jax.vjp of the residual
+ * (feax/assembler.py:1833-1858).  These are that pullback, hand-written.
+ * ------------------------------------------------------------------------ */
+
+/* grad = scale * (dR/dp)^T lambda for the volume variable p = param of mat
+ * (cell (E,) or node (N,)).  lambda entries on rows with bc_mask set are
+ * read as 0 (rows of R_bc there do not depend on p, feax/DCboundary.py:
+ * 202-251).  Node variables need inc_ptr/inc and a workspace of
+ * fx_hex8_param_vjp_workspace_bytes(E, FX_PARAM_NODE) bytes; cell variables
+ * need neither.  With scale = -1 and lambda = J_bc^-1 v this is the
+ * parameter gradient of feax's f_bwd. */
+size_t fx_hex8_param_vjp_workspace_bytes(int32_t num_cells, int32_t param_loc);
+int fx_hex8_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x,
+                          const double* y, const double* z, const int32_t* cells, const int32_t* inc_ptr,
+                          const int32_t* inc, const fx_material* mat, const double* param, const double* u,
+                          const double* lam, const uint8_t* bc_mask, double scale, double* grad,
+                          void* workspace, size_t ws_bytes);
+
+/* Fused compliance sensitivity for a linear law with a cell variable:
+ * dc[e] = -u_e^T (dK_e/dp_e) u_e, i.e. -p rho^(p-1) (E0 - Emin) u_e^T K0_e u_e
+ * for SIMP (feax/solvers/cmg.py:312-323).  This is the self-adjoint case
+ * (lambda = u) of the gradient of C = f.u (feax/gene/responses.py:10-105)
+ * when the Dirichlet values are zero and the load does not depend on p. */
+int fx_simp_compliance_sens_f64(void* stream, int32_t num_cells, const double* x, const double* y,
+                                const double* z, const int32_t* cells, const fx_material* mat,
+                                const double* param, const double* u, double* dc);
+
+/* grad_svar[f*4+q] = scale * d(lambda . R)/d svar[f, q] for the dead traction
+ * t = fdir[f] * svar[f, q] of fx_hex8_residual_f64 (feax/assembler.py:
+ * 494-535): nanson_q * sum_a N_a(q) (fdir[f] . lambda_a), lambda read as 0
+ * on rows with bc_mask set. */
+int fx_hex8_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, const double* y,
+                             const double* z, const int32_t* cells, const int32_t* faces,
+                             const double* fdir, const double* lam, const uint8_t* bc_mask, double scale,
+                             double* grad_svar);
+
+/* Neo-Hookean strain energy Pi(u) = sum_e sum_q psi JxW with
+ * psi = mu/2 (I1 - 3) - mu ln J + kappa/2 (J - 1)^2 (feax/assembler.py:
+ * 1866-1961), written to *energy (device), and, if dpsi_dmu != NULL, its
+ * partial derivative per element dPi/dmu_e = sum_q (I1/2 - 3/2 - ln J) JxW
+ * (cell variable mu).  Fixed-order reduction; workspace of
+ * fx_hex8_energy_workspace_bytes(E) bytes. */
+size_t fx_hex8_energy_workspace_bytes(int32_t num_cells);
+int fx_hex8_energy_f64(void* stream, int32_t num_cells, const double* x, const double* y, const double* z,
+                       const int32_t* cells, const fx_material* mat, const double* param, const double* u,
+                       double* energy, double* dpsi_dmu, void* workspace, size_t ws_bytes);
+
 #ifdef __cplusplus
 }
 #endif
