@@ -274,6 +274,49 @@ def strain_energy(problem, ts, traced_params, sol, with_partial=False):
     return (out[0], dmu) if with_partial else out[0]
 
 
+def create_energy_fn(problem, traced_structure=None):
+    """energy(u_flat, traced_params) -> Pi = sum_e sum_q psi(grad u, mu) JxW
+    (feax/assembler.py:1866-1961) for the Neo-Hookean law, as a 0-d float64
+    tensor differentiable through torch.autograd in u (dPi/du = the volume
+    residual, fx_hex8_residual_f64) and in a cell variable mu (the partial
+    dPi/dmu_e of fx_hex8_energy_f64)."""
+    import torch
+
+    class EnergyFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, u, mu, tp_static):
+            ts = _ts_or_build(problem, traced_structure)
+            tp = tp_static if mu is None else _with_volume_var(tp_static, mu.detach())
+            need_mu = mu is not None and mu.requires_grad
+            Pi, dmu = strain_energy(problem, ts, tp, u.detach(), with_partial=need_mu)
+            ctx.save_for_backward(u.detach())
+            ctx.tp, ctx.dmu, ctx.ts = tp, dmu, ts
+            return Pi
+
+        @staticmethod
+        def backward(ctx, g):
+            (u,) = ctx.saved_tensors
+            gu = gm = None
+            if ctx.needs_input_grad[0]:
+                gu = g * volume_residual(problem, ctx.ts, ctx.tp, u)
+            if ctx.needs_input_grad[1]:
+                gm = g * ctx.dmu
+            return gu, gm, None
+
+    def energy(u_flat, traced_params):
+        u = getattr(u_flat, "dofs", u_flat)
+        vv = traced_params.volume_vars
+        mu = vv[0] if len(vv) and isinstance(vv[0], torch.Tensor) else None
+        return EnergyFunction.apply(u.reshape(-1), mu, traced_params)
+
+    return energy
+
+
+def _with_volume_var(tp, v):
+    from .traced_params import TracedParams
+    return TracedParams(volume_vars=(v,) + tuple(tp.volume_vars[1:]), surface_vars=list(tp.surface_vars))
+
+
 def _ts_or_build(problem, ts):
     if ts is not None:
         return ts

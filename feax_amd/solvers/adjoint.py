@@ -146,7 +146,8 @@ def _function():
             (u,) = ctx.saved_tensors
             call, keep = ctx.call, ctx.keep
             ts = keep["ts"]
-            lam = adjoint_solve(call["problem"], ts, ctx.tp, call["bc"], v, keep, call["adj_opts"])
+            adj = call["adjoint_fn"] or adjoint_solve
+            lam = adj(call["problem"], ts, ctx.tp, call["bc"], v, keep, call["adj_opts"])
             grads = parameter_gradients(call["problem"], ts, ctx.tp, call["bc"], u, lam, call["paths"],
                                         ctx.shapes, ctx.needs_input_grad[1:])
             grads = [g if g is None else g.to(dt) for g, dt in zip(grads, ctx.dtypes)]
@@ -158,7 +159,10 @@ def _function():
 _FN = None
 
 
-def differentiable_linear_solve(problem, forward, tp, ig, bc, ts, adj_opts, symmetric):
+def differentiable_solve(problem, forward, tp, ig, bc, ts, adj_opts, symmetric, adjoint_fn=None):
+    """u = forward(tp, ig, bc, ts), differentiable in the torch tensors of tp.
+    adjoint_fn(problem, ts, tp, bc, v, keep, opts) -> lambda (default: the
+    linear solve's adjoint_solve on the operator the forward kept)."""
     if not _needs_grad(tp):
         return forward(tp, ig, bc, ts)
     global _FN
@@ -166,8 +170,12 @@ def differentiable_linear_solve(problem, forward, tp, ig, bc, ts, adj_opts, symm
         _FN = _function()
     leaves = _leaves(tp)
     call = dict(problem=problem, forward=forward, tp=tp, ig=ig, bc=bc, ts=ts, adj_opts=adj_opts,
-                paths=[p for p, _ in leaves])
+                paths=[p for p, _ in leaves], adjoint_fn=adjoint_fn)
     return _FN.apply(call, *[t for _, t in leaves])
 
 
-__all__ = ["differentiable_linear_solve", "adjoint_solve", "parameter_gradients"]
+def differentiable_linear_solve(problem, forward, tp, ig, bc, ts, adj_opts, symmetric):
+    return differentiable_solve(problem, forward, tp, ig, bc, ts, adj_opts, symmetric)
+
+
+__all__ = ["differentiable_solve", "differentiable_linear_solve", "adjoint_solve", "parameter_gradients"]

@@ -1,0 +1,167 @@
+"""Newton solver with Armijo line search (feax/solvers/newton.py:809-1112 API).
+
+create_newton_solver(problem, bc, solver_options, adjoint_solver_options,
+newton_options, traced_params, symmetric_elimination, traced_structure)
+returns ``solver(traced_params, initial_guess=None, bc=None,
+traced_structure=None)``.  One Newton step is
+
+  res = R_bc(sol)                      fx_hex8_residual_f64 (FP64)
+  J   = dR_bc/du at sol                fx_hex8_assemble_* (tangent, BC-eliminated)
+  du  = PCG(J, -res, x0 = lift(sol))   fx_pcg_*  (x0 as feax/solvers/common.py:23-53)
+  sol = Armijo(sol, du)                merit 0.5 ||R_bc||^2, c1, rho, max_backtracks
+                                       (feax/solvers/newton.py:809-859)
+
+until ||res|| <= tol, ||res|| / ||res_0|| <= rel_tol, max_iter steps, or a
+failed line search (which raises NewtonLineSearchError when
+raise_on_line_search_failure, as feax does for eager calls).
+
+precision "fp64": FP64 tangent and PCG.  "mixed": FP32 difference-form
+tangent with FP64 PCG vectors (an inexact Newton step); the Newton residual
+and the line search stay FP64.
+
+Gradients (torch.autograd): lambda = J_bc(u*)^-1 v on the FP64 tangent at
+the solution, grads = -(dR/dp)^T lambda (fx_hex8_param_vjp_f64); gradients
+with respect to Dirichlet values are not provided.
+"""
+
+import numpy as np
+
+from .. import _native as nat
+from ..assembler import TracedStructure, assemble_csr, residual
+from ..DCboundary import DirichletBC
+from .common import create_x0, device_norm, pcg
+from .linear import SolveStats, _dev_vec, _zero_like
+from .options import KrylovSolverOptions, MatrixProperty, NewtonOptions, resolve_iterative_solver
+
+
+class NewtonLineSearchError(RuntimeError):
+    """Armijo backtracking found no descent step (feax/solvers/newton.py:48-57)."""
+
+
+class NewtonStats(SolveStats):
+    def __init__(self):
+        super().__init__()
+        self.newton_iters = 0
+        self.res_history = []
+        self.alphas = []
+        self.linear_iters = []
+
+    def __repr__(self):
+        return (f"NewtonStats(newton_iters={self.newton_iters}, res={self.res_history}, "
+                f"alphas={self.alphas}, linear_iters={self.linear_iters})")
+
+
+def newton_solve(problem, ts, tp, bc, sol0, lin, newton, symmetric=True, stats=None, keep=None):
+    """The Newton loop on the device; returns the FP64 solution vector."""
+    import torch
+    sol = sol0.clone()
+    res = residual(problem, ts, tp, sol, bc)
+    r0 = device_norm(res)
+    rn = r0
+    x0_fn = lin.x0_fn if lin.x0_fn is not None else create_x0(bc.bc_rows, bc.bc_vals)
+    mixed = lin.precision != "fp64"
+    shift = lin.jacobi_shift if lin.use_jacobi_preconditioner else None
+    it, found = 0, True
+    hist, alphas, lits = [r0], [], []
+    if newton.verbose:
+        print(f"[feax_amd] Newton start: ||res|| = {r0:.6e}", flush=True)
+    trial = torch.empty_like(sol)
+    while it < newton.max_iter and rn > newton.tol and rn / max(r0, 1e-30) > newton.rel_tol and found:
+        out = assemble_csr(problem, ts, tp, sol, bc, symmetric, torch.float32 if mixed else torch.float64,
+                           diff_form=mixed, jacobi_shift=shift)
+        J, dinv = out if shift is not None else (out, None)
+        b = -res
+        du, info = pcg(J, b, x0_fn(sol), lin.tol, lin.atol, lin.maxiter, dinv, None, lin.check_every)
+        lits.append(info.iters)
+        # Armijo on 0.5 ||R_bc||^2 with slope -||res||^2.
+        grad_merit = -rn * rn
+        alpha, found = 1.0, False
+        for _ in range(newton.line_search_max_backtracks):
+            nat.call("fx_axpby_f64", nat.stream(), sol.numel(), 1.0, nat.ptr(sol), alpha, nat.ptr(du),
+                     nat.ptr(trial))
+            tres = residual(problem, ts, tp, trial, bc)
+            tn = device_norm(tres)
+            if np.isfinite(tn) and 0.5 * (tn * tn - rn * rn) <= newton.line_search_c1 * alpha * grad_merit:
+                found = True
+                sol, trial = trial, sol
+                res, rn = tres, tn
+                break
+            alpha *= newton.line_search_rho
+        if not found:
+            alpha = 1e-8
+            nat.call("fx_axpby_f64", nat.stream(), sol.numel(), 1.0, nat.ptr(sol), alpha, nat.ptr(du),
+                     nat.ptr(sol))
+            res = residual(problem, ts, tp, sol, bc)
+            rn = device_norm(res)
+        it += 1
+        hist.append(rn)
+        alphas.append(alpha)
+        if newton.verbose:
+            print(f"[feax_amd] Newton iter {it}: ||res|| = {rn:.6e} (rel {rn / max(r0, 1e-30):.3e})", flush=True)
+    if stats is not None:
+        stats.newton_iters, stats.res_history, stats.alphas, stats.linear_iters = it, hist, alphas, lits
+        stats.iters = int(sum(lits))
+        stats.outer = it
+        stats.rs = rn * rn
+        stats.relres = rn / r0 if r0 > 0 else 0.0
+    if newton.raise_on_line_search_failure and not found:
+        raise NewtonLineSearchError(
+            f"Newton iter {it}: Armijo line search failed after {newton.line_search_max_backtracks} "
+            "backtracks; the Newton direction is not a descent direction for 0.5*||r||^2.")
+    if keep is not None:
+        keep.update(form="newton", A=None, dinv=None, sol=sol)
+    return sol
+
+
+def newton_adjoint(problem, ts, tp, bc, v, keep, opts, symmetric=True):
+    """lambda = J_bc(u*)^-1 v (lambda[bc] = 0) on the FP64 tangent at u*."""
+    import torch
+    sol = keep["sol"]
+    shift = opts.jacobi_shift if opts.use_jacobi_preconditioner else None
+    out = assemble_csr(problem, ts, tp, sol, bc, symmetric, torch.float64, jacobi_shift=shift)
+    J, dinv = out if shift is not None else (out, None)
+    rhs = v.detach().to(device=ts.device, dtype=torch.float64).reshape(-1).clone()
+    if bc is not None and len(bc.bc_rows):
+        rhs.index_fill_(0, bc.device(ts.device)[0].long(), 0.0)
+    lam, info = pcg(J, rhs, None, opts.tol, opts.atol, opts.maxiter, dinv, None, opts.check_every)
+    keep["adjoint_iters"] = [info.iters]
+    return lam
+
+
+def create_newton_solver(problem, bc, solver_options=None, adjoint_solver_options=None, newton_options=None,
+                         traced_params=None, symmetric_elimination=True, traced_structure=None):
+    if solver_options is None:
+        solver_options = KrylovSolverOptions(solver="cg", use_jacobi_preconditioner=True)
+    if not isinstance(solver_options, KrylovSolverOptions):
+        raise NotImplementedError("only KrylovSolverOptions (cg) are on the HIP path; direct solvers are not")
+    solver_options = resolve_iterative_solver(solver_options, MatrixProperty.SPD)
+    adj = solver_options if adjoint_solver_options is None else \
+        resolve_iterative_solver(adjoint_solver_options, MatrixProperty.SPD)
+    newton_options = newton_options or NewtonOptions()
+    default_bc = bc
+    ts_default = traced_structure
+    stats = NewtonStats()
+
+    def forward(tp, ig, eff_bc, ts, keep=None):
+        if ts is None:
+            ts = getattr(problem, "_default_ts", None) or TracedStructure.from_problem(problem)
+            problem._default_ts = ts
+        if keep is not None:
+            keep["ts"] = ts
+        u0 = _dev_vec(ig, ts) if ig is not None else _zero_like(problem, eff_bc, ts)
+        return newton_solve(problem, ts, tp, eff_bc, u0, solver_options, newton_options, symmetric_elimination,
+                            stats, keep)
+
+    def solver(traced_params, initial_guess=None, bc=None, traced_structure=None):
+        eff_bc = bc if isinstance(bc, DirichletBC) else default_bc
+        ts = traced_structure if traced_structure is not None else ts_default
+        from .adjoint import differentiable_solve
+        return differentiable_solve(problem, forward, traced_params, initial_guess, eff_bc, ts, adj,
+                                    symmetric_elimination,
+                                    adjoint_fn=lambda *a: newton_adjoint(*a, symmetric=symmetric_elimination))
+
+    solver.stats = stats
+    return solver
+
+
+__all__ = ["create_newton_solver", "newton_solve", "newton_adjoint", "NewtonLineSearchError", "NewtonStats"]
