@@ -230,6 +230,105 @@ def grad_bench(steps, warmup, n_fd=16, eps=1e-4):
     return out
 
 
+def newton_bench(n_fd=16, eps=1e-5, forms=("fp64",)):
+    """Neo-Hookean block 100x20x20 (40,000 HEX8, 133,623 DOF): mu_e ~ U(0.5,
+    1.5) (cell var), kappa = 10; bottom z = 0 clamped, top z = 20 u_z =
+    -0.4 s for s = 1..5 (x, y free), initial guess = previous state with the
+    BC values; NewtonOptions defaults, inner Jacobi-PCG.  Reports Newton
+    iterations per step, wall time, final relative residual, and dPi/dmu
+    through torch.autograd vs the partial derivative and vs central
+    differences of re-solved states on 16 seeded cells."""
+    import numpy as np
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import create_energy_fn, strain_energy
+    dims = (100, 20, 20)
+    nz = dims[2]
+
+    class NH(fe.Problem):
+        def get_energy_density(self):
+            return fe.materials.CompressibleNeoHookean(kappa=10.0)
+
+    prob = NH(fe.StructuredGrid(dims, spacing=(1.0, 1.0, 1.0)).to_mesh(), vec=3, dim=3)
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[2], 0.0), "all", 0.0),
+                               fe.DirichletBCSpec(lambda p: np.isclose(p[2], nz), "z", 0.0)]).create_bc(prob)
+    node, comp = bc.bc_rows // 3, bc.bc_rows % 3
+    is_top = np.isclose(prob.points[node, 2], nz) & (comp == 2)
+    mu0 = np.random.default_rng(20261203).uniform(0.5, 1.5, prob.num_cells)
+    ts = fe.TracedStructure.from_problem(prob)
+    rows = torch.as_tensor(bc.bc_rows.astype(np.int64), device="cuda")
+    out = {"workload": f"Neo-Hookean block 100x20x20, {ts.n} DOF, -10% z in 5 steps"}
+    energy = create_energy_fn(prob, ts)
+    for form in forms:
+        lin = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=1e-10, maxiter=20000,
+                                     use_jacobi_preconditioner=True, precision=form)
+        solver = fe.create_solver(prob, bc, lin, newton_options=fe.NewtonOptions(raise_on_line_search_failure=False),
+                                  linear=False, return_solution=False, traced_structure=ts)
+        tp = fe.TracedParams(volume_vars=(torch.as_tensor(mu0, device="cuda"),))
+        u = torch.zeros(ts.n, dtype=torch.float64, device="cuda")
+        u_prev = torch.zeros_like(u)
+        steps = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(1, 6):
+            vals = np.where(is_top, -0.4 * s, 0.0)
+            # Initial guess: step 1 zero with the BC values; later steps the
+            # secant predictor 2 u_(s-1) - u_(s-2), which carries the new BC
+            # values (the prescribed displacement is linear in s).  The guess
+            # "previous state with the new BC values" concentrates each 2%
+            # increment in the top element layer; from step 2 on its Newton
+            # iterations stall (Armijo fallback steps, indefinite tangents,
+            # inner CG at maxiter).
+            guess = u.clone() if s == 1 else 2.0 * u - u_prev
+            guess[rows] = torch.as_tensor(vals, device="cuda")
+            u_prev = u
+            u = solver(tp, guess, bc.replace_vals(vals))
+            st = solver.stats
+            steps.append({"newton_iters": st.newton_iters, "linear_iters": int(sum(st.linear_iters)),
+                          "res": st.res_history[-1], "rel": st.relres,
+                          "converged": bool(st.res_history[-1] <= 1e-6 or st.relres <= 1e-8)})
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out[form] = {"wall_s": round(wall, 4), "steps": steps}
+        log(f"newton {form}: {wall:.3f} s, iters/step {[x['newton_iters'] for x in steps]}, "
+            f"final rel {steps[-1]['rel']:.2e}")
+    # dPi/dmu at the final step (fp64 solver): autograd vs partial vs FD
+    vals = np.where(is_top, -2.0, 0.0)
+    bc5 = bc.replace_vals(vals)
+    lin = fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0, maxiter=20000, use_jacobi_preconditioner=True)
+    tight = fe.create_solver(prob, bc5, lin, newton_options=fe.NewtonOptions(tol=1e-10, rel_tol=1e-12),
+                             linear=False, return_solution=False, traced_structure=ts)
+    mu = torch.tensor(mu0, device="cuda", requires_grad=True)
+    tpg = fe.TracedParams(volume_vars=(mu,))
+    ug = tight(tpg, u)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    Pi = energy(ug, tpg)
+    Pi.backward()
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t1
+    g = mu.grad
+    _, partial = strain_energy(prob, ts, fe.TracedParams(volume_vars=(mu.detach(),)), ug.detach(), with_partial=True)
+    out["dPi_dmu_autograd_vs_partial"] = float((g - partial).abs().max() / partial.abs().max())
+    out["grad_s"] = round(t_grad, 4)
+    idx = np.random.default_rng(20261203).choice(prob.num_cells, n_fd, replace=False)
+    errs = []
+    ustar = ug.detach()
+    for e in idx:
+        vals_pm = []
+        for sgn in (1.0, -1.0):
+            m2 = mu0.copy()
+            m2[e] += sgn * eps
+            tpv = fe.TracedParams(volume_vars=(torch.as_tensor(m2, device="cuda"),))
+            vals_pm.append(float(strain_energy(prob, ts, tpv, tight(tpv, ustar))))
+        fd = (vals_pm[0] - vals_pm[1]) / (2 * eps)
+        errs.append(abs(fd - float(g[e])))
+    out["fd_max_err_16"] = float(max(errs)) / float(g.abs().max())
+    log(f"newton dPi/dmu: autograd vs partial {out['dPi_dmu_autograd_vs_partial']:.2e}, FD (16 cells) max err / "
+        f"max|g| {out['fd_max_err_16']:.2e}, backward {t_grad * 1e3:.1f} ms")
+    return out
+
+
 def kernels_only(cfg, iters=8, copy_bytes=1 << 30):
     """A short run for hardware-counter passes: `iters` PCG iterations in
     each matrix form (so pcg_spmv_kernel is dispatched `iters` times per
@@ -266,6 +365,7 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad", action="store_true", help="skip the SIMP compliance-gradient workload")
+    ap.add_argument("--no-newton", action="store_true", help="skip the Neo-Hookean Newton workload")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch only the PCG SpMV kernels and a 1 GiB copy (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -353,6 +453,7 @@ def main():
     achieved = dom["B"] / dom["t"]
     log(f"in-solve SpMV {dom['label']}: {t_live:.2f} us avg over {n_live} launches -> {achieved / 1e9:.0f} GB/s")
     grad = None if args.no_grad else grad_bench(args.steps, args.warmup)
+    newton = None if args.no_newton else newton_bench()
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(problem, bc, A64, results["fp64"]["iters"])
@@ -385,6 +486,7 @@ def main():
         "assemble_fp64_ms": round(t_asm64 * 1e3, 3),
         "grad_solve_ratio": None if grad is None else grad["ratio_adjoint"],
         "grad": grad,
+        "newton": newton,
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                      "traffic": TRAFFIC.get(dom["label"]), "kernel_us": round(dom["t"] * 1e6, 2),
