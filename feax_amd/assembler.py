@@ -210,6 +210,34 @@ def volume_residual(problem, ts, traced_params, sol, bc=None, negate=False, out=
     return res
 
 
+def jvp(problem, ts, traced_params, sol, v, bc=None, out=None):
+    """Matrix-free J_bc(sol) v (fx_hex8_jvp_f64): feax's Krylov operator
+    create_matfree_res_J_parametric (feax/assembler.py:1747-1778)."""
+    import torch
+    mat, p = resolve_material(problem, traced_params, ts)
+    u = _as_dev(sol, ts) if sol is not None else None
+    v = _as_dev(v, ts)
+    mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
+    res = torch.empty(ts.n, dtype=torch.float64, device=ts.device) if out is None else out
+    nat.call("fx_hex8_jvp_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
+             nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat), nat.ptr(p),
+             nat.ptr(u), nat.ptr(v), nat.ptr(mask), nat.ptr(res))
+    return res
+
+
+def create_matfree_res_J_parametric(problem, symmetric=True):
+    """fn(sol, tp, bc, ts) -> (res_bc, matvec) with matvec(v) = J_bc(sol) v
+    (feax/assembler.py:1747-1778).  Symmetric elimination only."""
+    if not symmetric:
+        raise NotImplementedError("the matrix-free operator is built for symmetric elimination")
+
+    def fn(sol_flat, traced_params, bc, ts=None):
+        ts_ = _ts_or_build(problem, ts)
+        res = residual(problem, ts_, traced_params, sol_flat, bc)
+        return res, (lambda v: jvp(problem, ts_, traced_params, sol_flat, v, bc))
+    return fn
+
+
 def param_vjp(problem, ts, traced_params, sol, lam, bc=None, scale=1.0):
     """scale * (dR/dp)^T lam for p = volume_vars[0] (fx_hex8_param_vjp_f64);
     lam is read as 0 on the Dirichlet rows of bc.  Returns a float64 device

@@ -23,6 +23,8 @@
 #include <cstdint>
 
 #include "hex8_common.h"
+#include "pcg_common.h"
+#include "reduce.h"
 
 namespace fx {
 
@@ -276,6 +278,138 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void residual_kernel(
   }
 }
 
+// ------------------------------------------------- matrix-free J_bc(u) v
+// feax's Krylov operator (feax/assembler.py:1747-1778):
+//   J_bc v = where(bc, v, JVP_R(u)[v * !bc]).
+// Same wave-per-node layout as the residual: lanes (q, c) form grad v (and
+// grad u for the Neo-Hookean law) at Gauss point q by a butterfly over the
+// node lanes c, contract the linearised stress dP = A(grad u) : grad v and
+// test it with grad N_a.
+template <bool NH, bool DOT>
+__global__ __launch_bounds__(64 * kNodesPerBlock) void jvp_kernel(
+    int num_nodes, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const int* __restrict__ cells, const int* __restrict__ inc_ptr, const int* __restrict__ inc, MatParams mat,
+    const double* __restrict__ param, const double* __restrict__ u, const double* __restrict__ v,
+    const uint8_t* __restrict__ bc_mask, double* __restrict__ out, const PcgState* __restrict__ st,
+    double* partials) {
+  if constexpr (DOT) {
+    if (st->done) return;  // the whole grid leaves together
+  }
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nd = blockIdx.x * kNodesPerBlock + wid;
+  const bool live = nd < num_nodes;
+  const int q = lane >> 3, c = lane & 7;
+  const double x0 = gauss_pt(q >> 2), x1 = gauss_pt((q >> 1) & 1), x2 = gauss_pt(q & 1);
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+  const int k_end = live ? inc_ptr[nd + 1] : 0;
+  for (int k = live ? inc_ptr[nd] : 0; k < k_end; ++k) {
+    const int pk = inc[k];
+    const int e = pk >> 3, a = pk & 7;
+    const int* cn = cells + 8 * static_cast<long>(e);
+    double X[8][3];
+    load_coords<double>(x, y, z, cn, X);
+    double Ji[3][3];
+    const double det = jac_inv<double>(X, x0, x1, x2, Ji);
+    double N, d0, d1, d2, gc[3], ga[3];
+    shape<double>(c, x0, x1, x2, N, d0, d1, d2);
+    phys_grad<double>(Ji, d0, d1, d2, gc);
+    shape<double>(a, x0, x1, x2, N, d0, d1, d2);
+    phys_grad<double>(Ji, d0, d1, d2, ga);
+    const int nc = cn[c];
+    double vg[3][3], ug[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double vv = (bc_mask && bc_mask[3 * nc + i]) ? 0.0 : v[3 * nc + i];
+      const double uu = NH ? u[3 * nc + i] : 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double t = vv * gc[d];
+        t = xor_sum(t, 1);
+        t = xor_sum(t, 2);
+        t = xor_sum(t, 4);
+        vg[i][d] = t;
+        if constexpr (NH) {
+          double s = uu * gc[d];
+          s = xor_sum(s, 1);
+          s = xor_sum(s, 2);
+          s = xor_sum(s, 4);
+          ug[i][d] = s;
+        }
+      }
+    }
+    const double p = param_at<double>(mat, param, e, cn, x0, x1, x2);
+    double dP[3][3];
+    if constexpr (NH) {
+      // dP_iJ = A_iJkL vg_kL = mu vg_iJ + c1 F^-T_iJ (F^-T : vg) - c2 (F^-T vg^T F^-T)_iJ
+      double FiT[3][3];
+      const double J = inv_transpose_F<double>(ug, FiT);
+      const double kap = mat.kappa;
+      const double c1 = kap * (2.0 * J - 1.0) * J, c2 = -p + kap * J * (J - 1.0);
+      double t = 0.0, M[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) t += FiT[i][d] * vg[i][d];
+#pragma unroll
+      for (int L = 0; L < 3; ++L)
+#pragma unroll
+        for (int Jc = 0; Jc < 3; ++Jc) M[L][Jc] = vg[0][L] * FiT[0][Jc] + vg[1][L] * FiT[1][Jc] + vg[2][L] * FiT[2][Jc];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int Jc = 0; Jc < 3; ++Jc)
+          dP[i][Jc] = p * vg[i][Jc] + c1 * FiT[i][Jc] * t -
+                      c2 * (FiT[i][0] * M[0][Jc] + FiT[i][1] * M[1][Jc] + FiT[i][2] * M[2][Jc]);
+    } else {
+      material_stress<double>(mat, p, vg, dP);  // linear in grad v
+    }
+    const double w = det * 0.125;
+    double ri[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ri[i] = (dP[i][0] * ga[0] + dP[i][1] * ga[1] + dP[i][2] * ga[2]) * w;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double t = ri[i];
+      t = xor_sum(t, 8);
+      t = xor_sum(t, 16);
+      t = xor_sum(t, 32);
+      ri[i] = t;
+    }
+    r0 += ri[0];
+    r1 += ri[1];
+    r2 += ri[2];
+  }
+  DVec<1> dot;
+  dot.v[0] = 0.0;
+  if (live && lane < 3) {
+    const int r = 3 * nd + lane;
+    const double val = (bc_mask && bc_mask[r]) ? v[r] : (lane == 0 ? r0 : (lane == 1 ? r1 : r2));
+    out[r] = val;
+    if constexpr (DOT) dot.v[0] = v[r] * val;
+  }
+  if constexpr (DOT) store_partials<1>(block_sum<1, 64 * kNodesPerBlock>(dot), blockIdx.x, partials);
+}
+
+int hex8_op_blocks(int num_nodes) { return (num_nodes + kNodesPerBlock - 1) / kNodesPerBlock; }
+
+void launch_hex8_op(hipStream_t s, const Hex8Op& op, const double* v, double* out, const PcgState* st,
+                    double* partials) {
+  const int grid = hex8_op_blocks(op.num_nodes);
+  const bool nh = op.mat.kind == FX_MAT_NEO_HOOKEAN;
+#define FX_JVP_LAUNCH(NHV, DOTV)                                                                                \
+  hipLaunchKernelGGL((jvp_kernel<NHV, DOTV>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s, op.num_nodes, op.x, \
+                     op.y, op.z, op.cells, op.inc_ptr, op.inc, op.mat, op.param, op.u, v, op.bc_mask, out, st,    \
+                     partials)
+  if (nh) {
+    if (st) FX_JVP_LAUNCH(true, true);
+    else FX_JVP_LAUNCH(true, false);
+  } else {
+    if (st) FX_JVP_LAUNCH(false, true);
+    else FX_JVP_LAUNCH(false, false);
+  }
+#undef FX_JVP_LAUNCH
+}
+
 __global__ void bc_res_kernel(int nbc, const int* __restrict__ rows, const double* __restrict__ vals,
                               const double* __restrict__ u, int negate, double* __restrict__ res) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -369,6 +503,24 @@ int fx_hex8_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, con
                        negate, res);
     FX_LAUNCHED("bc_res_kernel");
   }
+  return FX_OK;
+}
+
+int fx_hex8_jvp_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                    const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                    const fx_material* mat, const double* param, const double* u, const double* v,
+                    const uint8_t* bc_mask, double* out) {
+  if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !z || !cells || !inc_ptr || !inc || !mat || !param || !v ||
+      !out)
+    return fail(FX_EINVAL, "fx_hex8_jvp_f64: bad arguments");
+  if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
+    return fail(FX_EINVAL, "fx_hex8_jvp_f64: unknown material %d", mat->kind);
+  const bool nh = mat->kind == FX_MAT_NEO_HOOKEAN;
+  if (nh && !u) return fail(FX_EINVAL, "fx_hex8_jvp_f64: the Neo-Hookean tangent needs u");
+  if (out == v) return fail(FX_EINVAL, "fx_hex8_jvp_f64: out must not alias v");
+  const Hex8Op op{num_nodes, x, y, z, cells, inc_ptr, inc, to_params(mat), param, nh ? u : nullptr, bc_mask};
+  launch_hex8_op(as_stream(stream), op, v, out, nullptr, nullptr);
+  FX_LAUNCHED("jvp_kernel");
   return FX_OK;
 }
 

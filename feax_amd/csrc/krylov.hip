@@ -13,15 +13,15 @@
 //    block from `bcol` (nse/9 int32 built once by fx_csr_block3_index), and
 //    each gathered x entry is used three times.  Bytes per nonzero:
 //    s_v + 4/9 instead of s_v + 4.
-// A fixed grid of at most kMaxGridBlocks workgroups walks contiguous row
-// ranges; the block id is remapped so each XCD owns one contiguous range
-// (x entries gathered by neighbouring rows hit that XCD's L2).  Matrix
-// values and indices are streamed with non-temporal loads so they do not
-// evict the gathered vector.
+// One row / node per group and as many blocks as groups need, with plain
+// loads: on MI355X this measured faster than a persistent grid, an
+// XCD-contiguous block remap and non-temporal loads (the FP64 block-3 kernel
+// reads at the measured streaming rate).
 //
 // PCG follows jax.scipy.sparse.linalg.cg (feax/solvers/common.py:229-239)
-// with scalars kept on the device.  Per iteration: SpMV fused with the p.q
-// block partials, a 1-block finalize (alpha), the vector update fused with
+// with scalars kept on the device.  Per iteration: the operator (CSR SpMV,
+// or feax's matrix-free J_bc v from hex8.hip) fused with the p.q block
+// partials, a 1-block finalize (alpha), the vector update fused with
 // r.z / r.r partials, a 1-block finalize (beta, stop test), the direction
 // update.  FP64 partial sums, fixed order: bitwise repeatable.
 #include <algorithm>
@@ -29,10 +29,12 @@
 #include <vector>
 
 #include "common.h"
+#include "pcg_common.h"
 #include "reduce.h"
 
 namespace fx {
 
+// Streaming read of data used once (the recorded Krylov directions).
 template <typename T>
 __device__ __forceinline__ T ldnt(const T* p) {
   return __builtin_nontemporal_load(p);
@@ -254,10 +256,6 @@ __global__ __launch_bounds__(kBlock) void diag_kernel(int n, const int* __restri
 }
 
 // ---------------------------------------------------------------- PCG
-struct PcgState {
-  double alpha, beta, gamma, rs, atol2, bb;
-  int k, done, maxiter, nparts;
-};
 
 constexpr size_t kStateBytes = 256;
 // Vector-kernel partial slots (x kMaxGridBlocks doubles): r.z r.r [0, 2), init sums [2, 5).
@@ -391,15 +389,43 @@ struct PcgArgs {
   int rec_max;
 };
 
+// PCG operators: apply(x, y) computes y = A x; iter(p, q, st, partials)
+// computes q = A p unless st->done and writes grid() block partials of p.q.
 template <typename V, typename T, int L, bool B3, bool DIFF>
-void launch_pcg_iter(hipStream_t s, const PcgArgs& a, int grid_mv, int grid_v, const V* data, const T* dinv, T* x,
-                     T* r, T* z, T* p, T* q, PcgState* st, double* partials, double* spmv_partials,
-                     hipEvent_t ev0, hipEvent_t ev1) {
+struct CsrOp {
+  int n;
+  const int* indptr;
+  const int* idx;
+  const V* data;
+  int grid() const { return grid_for(B3 ? n / 3 : n, L); }
+  void apply(hipStream_t s, const T* xv, T* yv) const {
+    hipLaunchKernelGGL((spmv_kernel<V, T, L, B3, DIFF>), dim3(grid()), dim3(kBlock), 0, s, n, indptr, idx, data, xv,
+                       yv);
+  }
+  void iter(hipStream_t s, const T* p, T* q, const PcgState* st, double* partials) const {
+    hipLaunchKernelGGL((pcg_spmv_kernel<V, T, L, B3, DIFF>), dim3(grid()), dim3(kBlock), 0, s, n, indptr, idx, data,
+                       p, q, st, partials);
+  }
+};
+
+// Matrix-free J_bc v of the HEX8 mesh (feax's Krylov operator).
+struct MatfreeOp {
+  Hex8Op op;
+  int grid() const { return hex8_op_blocks(op.num_nodes); }
+  void apply(hipStream_t s, const double* xv, double* yv) const { launch_hex8_op(s, op, xv, yv, nullptr, nullptr); }
+  void iter(hipStream_t s, const double* p, double* q, const PcgState* st, double* partials) const {
+    launch_hex8_op(s, op, p, q, st, partials);
+  }
+};
+
+template <typename T, typename Op>
+void launch_pcg_iter(hipStream_t s, const PcgArgs& a, const Op& op, int grid_v, const T* dinv, T* x, T* r, T* z,
+                     T* p, T* q, PcgState* st, double* partials, double* op_partials, hipEvent_t ev0,
+                     hipEvent_t ev1) {
   if (ev0) (void)hipEventRecord(ev0, s);
-  hipLaunchKernelGGL((pcg_spmv_kernel<V, T, L, B3, DIFF>), dim3(grid_mv), dim3(kBlock), 0, s, a.n, a.indptr, a.idx,
-                     data, p, q, st, spmv_partials);
+  op.iter(s, p, q, st, op_partials);
   if (ev1) (void)hipEventRecord(ev1, s);
-  hipLaunchKernelGGL(pcg_fin_alpha, dim3(1), dim3(kFinBlock), 0, s, st, spmv_partials, grid_mv, a.rec_pq,
+  hipLaunchKernelGGL(pcg_fin_alpha, dim3(1), dim3(kFinBlock), 0, s, st, op_partials, op.grid(), a.rec_pq,
                      a.rec_max);
   hipLaunchKernelGGL((pcg_update_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, a.n, x, r, z, p, q, dinv, st,
                      partials, a.rec_p, a.rec_max);
@@ -407,28 +433,29 @@ void launch_pcg_iter(hipStream_t s, const PcgArgs& a, int grid_mv, int grid_v, c
   hipLaunchKernelGGL((pcg_dir_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, a.n, p, z, st);
 }
 
-template <typename V, typename T, bool B3, bool DIFF>
-int pcg_loop(hipStream_t s, const PcgArgs& a, int lanes, const V* data, const T* dinv, const T* b, const T* x0,
-             double tol, double atol, int maxiter, int check_every, T* x, int* iters, double* rs, void* workspace) {
+template <typename T, typename Op>
+int pcg_loop(hipStream_t s, const PcgArgs& a, const Op& op, const T* dinv, const T* b, const T* x0, double tol,
+             double atol, int maxiter, int check_every, T* x, int* iters, double* rs, void* workspace) {
   const int n = a.n;
+  if (static_cast<size_t>(op.grid()) > spmv_partial_cap(n))
+    return fail(FX_ELIMIT, "pcg: operator grid %d exceeds the partial slots", op.grid());
   char* w = static_cast<char*>(workspace);
   PcgState* st = reinterpret_cast<PcgState*>(w);
   double* partials = reinterpret_cast<double*>(w + kStateBytes);
-  double* spmv_partials = reinterpret_cast<double*>(w + kStateBytes + kPartialBytes);
+  double* op_partials = reinterpret_cast<double*>(w + kStateBytes + kPartialBytes);
   const size_t vb = align256(static_cast<size_t>(n) * sizeof(T));
   T* r = reinterpret_cast<T*>(w + kStateBytes + kPartialBytes + spmv_partial_bytes(n));
   T* z = reinterpret_cast<T*>(reinterpret_cast<char*>(r) + vb);
   T* p = reinterpret_cast<T*>(reinterpret_cast<char*>(z) + vb);
   T* q = reinterpret_cast<T*>(reinterpret_cast<char*>(p) + vb);
-  const int grid_mv = grid_for(B3 ? n / 3 : n, lanes);
   const int grid_v = vec_grid(n);
   if (x0) {
     if (x0 != x) FX_HIP(hipMemcpyAsync(x, x0, static_cast<size_t>(n) * sizeof(T), hipMemcpyDeviceToDevice, s));
   } else {
     FX_HIP(hipMemsetAsync(x, 0, static_cast<size_t>(n) * sizeof(T), s));
   }
-  int rc = launch_spmv<V, T, B3, DIFF>(s, n, a.indptr, a.idx, data, x, q, lanes);
-  if (rc) return rc;
+  op.apply(s, x, q);
+  FX_LAUNCHED("pcg_operator");
   hipLaunchKernelGGL((pcg_init_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, n, b, q, r, z, p, dinv,
                      partials + 2 * kMaxGridBlocks);
   hipLaunchKernelGGL(pcg_fin_init, dim3(1), dim3(kFinBlock), 0, s, st, partials + 2 * kMaxGridBlocks, grid_v, tol,
@@ -440,7 +467,7 @@ int pcg_loop(hipStream_t s, const PcgArgs& a, int lanes, const V* data, const T*
   } h = {0, 0};
   FX_HIP(hipMemcpyAsync(&h, &st->k, sizeof(h), hipMemcpyDeviceToHost, s));
   FX_HIP(hipStreamSynchronize(s));
-  // FX_PCG_TIME: HIP events around every SpMV launch of the executed
+  // FX_PCG_TIME: HIP events around every operator launch of the executed
   // iterations; average in rs[1] (microseconds), count in rs[2].
   const bool timing = (a.flags & FX_PCG_TIME) != 0;
   std::vector<hipEvent_t> ev;
@@ -457,8 +484,7 @@ int pcg_loop(hipStream_t s, const PcgArgs& a, int lanes, const V* data, const T*
     const int k_before = h.k;
     for (int c = 0; c < chunk; ++c) {
       hipEvent_t e0 = timing ? ev[2 * c] : nullptr, e1 = timing ? ev[2 * c + 1] : nullptr;
-      FX_LANE_SWITCH(lanes, launch_pcg_iter<V, T, LL, B3, DIFF>(s, a, grid_mv, grid_v, data, dinv, x, r, z, p, q,
-                                                                 st, partials, spmv_partials, e0, e1));
+      launch_pcg_iter<T, Op>(s, a, op, grid_v, dinv, x, r, z, p, q, st, partials, op_partials, e0, e1);
     }
     launched += chunk;
     hipError_t le = hipGetLastError();
@@ -514,14 +540,19 @@ int run_pcg(void* stream, int n, long long nse, int flags, const int* indptr, co
     if (flags & FX_PCG_TIME) rs[1] = rs[2] = 0.0;
     return FX_OK;
   }
-  PcgArgs a{n, nse, flags, indptr, b3 ? bcol : indices, rec_p, rec_pq, rec_p ? rec_max : 0};
+  const PcgArgs a{n, nse, flags, indptr, b3 ? bcol : indices, rec_p, rec_pq, rec_p ? rec_max : 0};
   const int lanes = lanes_for(n, nse, b3, sizeof(V) == 8);
-  if (diff) return pcg_loop<V, T, true, true>(s, a, lanes, data, dinv, b, x0, tol, atol, maxiter, check_every, x,
-                                               iters, rs, workspace);
-  if (b3) return pcg_loop<V, T, true, false>(s, a, lanes, data, dinv, b, x0, tol, atol, maxiter, check_every, x,
-                                             iters, rs, workspace);
-  return pcg_loop<V, T, false, false>(s, a, lanes, data, dinv, b, x0, tol, atol, maxiter, check_every, x, iters, rs,
-                                      workspace);
+  if (diff) {
+    FX_LANE_SWITCH(lanes, return pcg_loop<T>(s, a, CsrOp<V, T, LL, true, true>{n, indptr, a.idx, data}, dinv, b, x0,
+                                             tol, atol, maxiter, check_every, x, iters, rs, workspace));
+  } else if (b3) {
+    FX_LANE_SWITCH(lanes, return pcg_loop<T>(s, a, CsrOp<V, T, LL, true, false>{n, indptr, a.idx, data}, dinv, b,
+                                             x0, tol, atol, maxiter, check_every, x, iters, rs, workspace));
+  } else {
+    FX_LANE_SWITCH(lanes, return pcg_loop<T>(s, a, CsrOp<V, T, LL, false, false>{n, indptr, a.idx, data}, dinv, b,
+                                             x0, tol, atol, maxiter, check_every, x, iters, rs, workspace));
+  }
+  return FX_OK;
 }
 
 // ---------------------------------------------------------------- vectors
@@ -714,6 +745,38 @@ int fx_pcg_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int
                   double* rs, float* rec_p, double* rec_pq, int32_t rec_max, void* workspace, size_t ws_bytes) {
   return run_pcg<float, double>(stream, n, nse, flags, indptr, indices, bcol, data, dinv, b, x0, tol, atol, maxiter,
                                 check_every, x, iters, rs, rec_p, rec_pq, rec_max, workspace, ws_bytes);
+}
+
+int fx_pcg_matfree_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                       const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                       const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                       int32_t flags, const double* dinv, const double* b, const double* x0, double tol, double atol,
+                       int32_t maxiter, int32_t check_every, double* xout, int32_t* iters, double* rs,
+                       void* workspace, size_t ws_bytes) {
+  if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !z || !cells || !inc_ptr || !inc || !mat || !param || !b ||
+      !xout || !iters || !rs)
+    return fail(FX_EINVAL, "fx_pcg_matfree_f64: bad arguments");
+  if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
+    return fail(FX_EINVAL, "fx_pcg_matfree_f64: unknown material %d", mat->kind);
+  const bool nh = mat->kind == FX_MAT_NEO_HOOKEAN;
+  if (nh && !u) return fail(FX_EINVAL, "fx_pcg_matfree_f64: the Neo-Hookean tangent needs u");
+  if (maxiter < 0) return fail(FX_EINVAL, "fx_pcg_matfree_f64: maxiter < 0");
+  const int n = 3 * num_nodes;
+  const size_t need = pcg_ws_bytes(n, sizeof(double));
+  if (!workspace || ws_bytes < need)
+    return fail(FX_EWORKSPACE, "fx_pcg_matfree_f64: workspace %zu < %zu bytes", ws_bytes, need);
+  MatParams mp;
+  mp.kind = mat->kind;
+  mp.loc = mat->param_loc;
+  mp.nu = mat->nu;
+  mp.E0 = mat->E0;
+  mp.Emin = mat->Emin;
+  mp.penal = mat->penal;
+  mp.kappa = mat->kappa;
+  const MatfreeOp op{Hex8Op{num_nodes, x, y, z, cells, inc_ptr, inc, mp, param, nh ? u : nullptr, bc_mask}};
+  const PcgArgs a{n, 0, flags & FX_PCG_TIME, nullptr, nullptr, nullptr, nullptr, 0};
+  return pcg_loop<double>(as_stream(stream), a, op, dinv, b, x0, tol, atol, maxiter, check_every, xout, iters, rs,
+                          workspace);
 }
 
 size_t fx_krylov_project_workspace_bytes(int32_t nvec) {

@@ -65,6 +65,12 @@ def adjoint_solve(problem, ts, tp, bc, v, keep, opts):
     if bc is not None and len(bc.bc_rows):
         rhs.index_fill_(0, bc.device(ts.device)[0].long(), 0.0)
     A, dinv = keep["A"], keep["dinv"]
+    if keep["form"] == "matfree":
+        from .common import pcg_matfree
+        lam, info = pcg_matfree(problem, ts, tp, None, bc, rhs, None, opts.tol, opts.atol, opts.maxiter,
+                                opts.check_every)
+        keep["adjoint_iters"] = [info.iters]
+        return lam
     if keep["form"] == "fp64":
         lam, info = pcg(A, rhs, None, opts.tol, opts.atol, opts.maxiter, dinv, None, opts.check_every)
         keep["adjoint_iters"] = [info.iters]

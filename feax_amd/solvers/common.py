@@ -118,6 +118,34 @@ def pcg(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=Non
     return x, PCGInfo(iters.value, rs[0])
 
 
+def pcg_matfree(problem, ts, tp, u_lin, bc, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, check_every=32,
+                out=None, dinv=None):
+    """CG on feax's matrix-free operator J_bc(u_lin) (fx_pcg_matfree_f64);
+    returns (x, PCGInfo).  u_lin: linearisation point (Neo-Hookean; None for
+    linear laws)."""
+    import torch
+    from ..assembler import _as_dev, resolve_material
+    mat, p = resolve_material(problem, tp, ts)
+    n = ts.n
+    b = b.to(torch.float64).contiguous()
+    x = torch.empty(n, dtype=torch.float64, device=ts.device) if out is None else out
+    x0 = None if x0 is None else x0.to(torch.float64).contiguous()
+    u = _as_dev(u_lin, ts) if u_lin is not None else None
+    mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
+    flags = nat.FX_PCG_TIME if SpmvTimer._active is not None else 0
+    ws = _workspace(n, torch.float64, ts.device)
+    iters = ctypes.c_int32(0)
+    rs = (ctypes.c_double * 3)()
+    nat.call("fx_pcg_matfree_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
+             nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat), nat.ptr(p),
+             nat.ptr(u), nat.ptr(mask), flags, nat.ptr(dinv), nat.ptr(b), nat.ptr(x0), float(tol), float(atol),
+             int(maxiter), int(check_every), nat.ptr(x), ctypes.addressof(iters), ctypes.addressof(rs), nat.ptr(ws),
+             ws.numel())
+    if flags:
+        SpmvTimer._active.add("f64/matfree", rs[1], rs[2])
+    return x, PCGInfo(iters.value, rs[0])
+
+
 class KrylovRecord:
     """Device storage for the search directions of one CG solve (FP32,
     capacity x n) and their curvatures p.Ap, used to start a later solve

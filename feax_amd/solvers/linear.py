@@ -48,9 +48,12 @@ def _b3(options):
 
 
 def solve_fp64(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None, keep=None):
-    """One FP64 solve on the assembled CSR.  Returns u (FP64 device vector).
-    keep (dict): receives the operator for an adjoint solve."""
+    """One FP64 solve on the assembled CSR (or, operator="matfree", on the
+    matrix-free J_bc).  Returns u (FP64 device vector).  keep (dict): receives
+    the operator for an adjoint solve."""
     import torch
+    if opts.operator == "matfree":
+        return _solve_matfree(problem, ts, tp, bc, u0, opts, symmetric, stats, keep)
     if opts.use_jacobi_preconditioner:
         J, dinv = assemble_csr(problem, ts, tp, u0, bc, symmetric, torch.float64, jacobi_shift=opts.jacobi_shift)
     else:
@@ -63,6 +66,27 @@ def solve_fp64(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None, keep=N
         keep.update(form="fp64", A=J, dinv=dinv)
     if opts.check_convergence:
         du = check_convergence(J, du, b, opts)
+    if stats is not None:
+        stats.iters, stats.rs, stats.outer = info.iters, info.rs, 1
+        stats.inner_iters = [info.iters]
+    u = torch.empty_like(u0)
+    nat.call("fx_axpby_f64", nat.stream(), u.numel(), 1.0, nat.ptr(u0), 1.0, nat.ptr(du), nat.ptr(u))
+    return u
+
+
+def _solve_matfree(problem, ts, tp, bc, u0, opts, symmetric, stats, keep):
+    """feax's default Krylov path: CG on J_bc v without assembly
+    (feax/solvers/linear.py:394-398, feax/assembler.py:1747-1778)."""
+    import torch
+    from .common import pcg_matfree
+    if not symmetric:
+        raise NotImplementedError("the matrix-free operator is built for symmetric elimination")
+    b = residual(problem, ts, tp, u0, bc, negate=True)
+    x0_fn = opts.x0_fn if opts.x0_fn is not None else create_x0(bc.bc_rows, bc.bc_vals)
+    du, info = pcg_matfree(problem, ts, tp, None, bc, b, x0_fn(u0), opts.tol, opts.atol, opts.maxiter,
+                           opts.check_every)
+    if keep is not None:
+        keep.update(form="matfree", A=None, dinv=None)
     if stats is not None:
         stats.iters, stats.rs, stats.outer = info.iters, info.rs, 1
         stats.inner_iters = [info.iters]
@@ -178,7 +202,7 @@ def create_linear_solver(problem, bc, solver_options=None, adjoint_solver_option
         if keep is not None:
             keep["ts"] = ts
         u0 = _dev_vec(ig, ts) if ig is not None else _zero_like(problem, eff_bc, ts)
-        if solver_options.precision != "fp64":
+        if solver_options.precision != "fp64" and solver_options.operator == "csr":
             return solve_mixed(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
         return solve_fp64(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
 

@@ -16,8 +16,11 @@ failed line search (which raises NewtonLineSearchError when
 raise_on_line_search_failure, as feax does for eager calls).
 
 precision "fp64": FP64 tangent and PCG.  "mixed": FP32 difference-form
-tangent with FP64 PCG vectors (an inexact Newton step); the Newton residual
-and the line search stay FP64.
+tangent with FP64 PCG vectors inside an FP64 defect correction on the
+matrix-free FP64 tangent product, so each Newton direction meets the
+linear tolerance of the FP64 tangent; a step whose defect correction stalls
+(eps32 * cond(J) near 1, e.g. strongly compressed states) is taken with the
+FP64 tangent instead.  The Newton residual and the line search stay FP64.
 
 Gradients (torch.autograd): lambda = J_bc(u*)^-1 v on the FP64 tangent at
 the solution, grads = -(dR/dp)^T lambda (fx_hex8_param_vjp_f64); gradients
@@ -45,6 +48,7 @@ class NewtonStats(SolveStats):
         self.res_history = []
         self.alphas = []
         self.linear_iters = []
+        self.fp64_fallbacks = 0
 
     def __repr__(self):
         return (f"NewtonStats(newton_iters={self.newton_iters}, res={self.res_history}, "
@@ -61,7 +65,7 @@ def newton_solve(problem, ts, tp, bc, sol0, lin, newton, symmetric=True, stats=N
     x0_fn = lin.x0_fn if lin.x0_fn is not None else create_x0(bc.bc_rows, bc.bc_vals)
     mixed = lin.precision != "fp64"
     shift = lin.jacobi_shift if lin.use_jacobi_preconditioner else None
-    it, found = 0, True
+    it, found, fallbacks = 0, True, 0
     hist, alphas, lits = [r0], [], []
     if newton.verbose:
         print(f"[feax_amd] Newton start: ||res|| = {r0:.6e}", flush=True)
@@ -71,8 +75,22 @@ def newton_solve(problem, ts, tp, bc, sol0, lin, newton, symmetric=True, stats=N
                            diff_form=mixed, jacobi_shift=shift)
         J, dinv = out if shift is not None else (out, None)
         b = -res
-        du, info = pcg(J, b, x0_fn(sol), lin.tol, lin.atol, lin.maxiter, dinv, None, lin.check_every)
-        lits.append(info.iters)
+        du = None
+        if mixed:
+            du, n_lin = _mixed_direction(problem, ts, tp, bc, sol, b, x0_fn(sol), J, dinv, lin)
+            if du is None:
+                # The FP32 tangent is too coarse for this state (the defect
+                # correction stalls when eps32 * cond(J) approaches 1): take
+                # this step with the FP64 tangent.
+                fallbacks += 1
+                out = assemble_csr(problem, ts, tp, sol, bc, symmetric, torch.float64, jacobi_shift=shift)
+                J, dinv = out if shift is not None else (out, None)
+                du, info = pcg(J, b, x0_fn(sol), lin.tol, lin.atol, lin.maxiter, dinv, None, lin.check_every)
+                n_lin += info.iters
+        else:
+            du, info = pcg(J, b, x0_fn(sol), lin.tol, lin.atol, lin.maxiter, dinv, None, lin.check_every)
+            n_lin = info.iters
+        lits.append(n_lin)
         # Armijo on 0.5 ||R_bc||^2 with slope -||res||^2.
         grad_merit = -rn * rn
         alpha, found = 1.0, False
@@ -100,6 +118,7 @@ def newton_solve(problem, ts, tp, bc, sol0, lin, newton, symmetric=True, stats=N
             print(f"[feax_amd] Newton iter {it}: ||res|| = {rn:.6e} (rel {rn / max(r0, 1e-30):.3e})", flush=True)
     if stats is not None:
         stats.newton_iters, stats.res_history, stats.alphas, stats.linear_iters = it, hist, alphas, lits
+        stats.fp64_fallbacks = fallbacks
         stats.iters = int(sum(lits))
         stats.outer = it
         stats.rs = rn * rn
@@ -111,6 +130,43 @@ def newton_solve(problem, ts, tp, bc, sol0, lin, newton, symmetric=True, stats=N
     if keep is not None:
         keep.update(form="newton", A=None, dinv=None, sol=sol)
     return sol
+
+
+def _mixed_direction(problem, ts, tp, bc, sol, rhs, x0, A32, dinv, lin):
+    """du with J_bc(sol) du = rhs to the linear tolerance: FP32 difference-form
+    tangent inside an FP64 defect correction whose residual rhs - J_bc du is
+    the matrix-free FP64 tangent product (fx_hex8_jvp_f64); later inner
+    solves start from the projection onto the first one's directions."""
+    import torch
+    from ..assembler import jvp
+    from .linear import _record_for
+    n = ts.n
+    du = x0.clone()
+    Kx = jvp(problem, ts, tp, sol, du, bc)
+    r = rhs - Kx
+    bnorm = device_norm(rhs)
+    target = max(lin.tol * bnorm, lin.atol)
+    rn = device_norm(r)
+    d = torch.empty_like(du)
+    record = _record_for(n, min(lin.recycle, lin.maxiter), du.device) if lin.recycle > 0 else None
+    if record is not None:
+        record.count = 0
+        record.owner = object()
+    total, outer, budget = 0, 0, lin.maxiter
+    while rn > target and outer < lin.max_outer and budget > 0:
+        x0i = record.project(r) if (record is not None and outer > 0) else None
+        rec = record if (record is not None and outer == 0) else None
+        _, info = pcg(A32, r, x0i, lin.inner_tol, 0.0, budget, dinv, True, lin.check_every, out=d, record=rec)
+        du += d
+        jvp(problem, ts, tp, sol, du, bc, out=Kx)
+        torch.sub(rhs, Kx, out=r)
+        prev, rn = rn, device_norm(r)
+        total += info.iters
+        budget -= info.iters
+        outer += 1
+        if not rn < 0.5 * prev:
+            return None, total   # stalled: the caller takes the FP64 tangent
+    return (du, total) if rn <= target else (None, total)
 
 
 def newton_adjoint(problem, ts, tp, bc, v, keep, opts, symmetric=True):
@@ -126,6 +182,33 @@ def newton_adjoint(problem, ts, tp, bc, v, keep, opts, symmetric=True):
     lam, info = pcg(J, rhs, None, opts.tol, opts.atol, opts.maxiter, dinv, None, opts.check_every)
     keep["adjoint_iters"] = [info.iters]
     return lam
+
+
+def linearized_guess(problem, bc, traced_params, u_ref=None, traced_structure=None, options=None):
+    """Initial guess for a displacement-controlled load step: the solution of
+    the problem linearised at u_ref (default 0) with the Dirichlet values of
+    bc, u = u_ref + w + d with w the BC increment (w_b = bc_vals - u_ref_b)
+    and J_bc(u_ref) d = -(R(u_ref) + K(u_ref) w) on the free rows.  Spreads a
+    prescribed boundary increment through the body instead of leaving it in
+    the boundary element layer (where "previous state with the new BC
+    values" puts it)."""
+    import torch
+    from ..assembler import _ts_or_build
+    ts = _ts_or_build(problem, traced_structure)
+    opts = options or KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=20000,
+                                          use_jacobi_preconditioner=True)
+    u_ref = torch.zeros(ts.n, dtype=torch.float64, device=ts.device) if u_ref is None else _dev_vec(u_ref, ts)
+    rows, _, vals = bc.device(ts.device)
+    rows = rows.long()
+    w = torch.zeros_like(u_ref)
+    w[rows] = vals - u_ref[rows]
+    K = assemble_csr(problem, ts, traced_params, u_ref, None, True, torch.float64)
+    b = -(residual(problem, ts, traced_params, u_ref) + K.matvec(w))
+    b[rows] = 0.0
+    J, dinv = assemble_csr(problem, ts, traced_params, u_ref, bc, True, torch.float64, jacobi_shift=1e-12)
+    d, _ = pcg(J, b, None, opts.tol, opts.atol, opts.maxiter, dinv if opts.use_jacobi_preconditioner else None,
+               None, opts.check_every)
+    return u_ref + w + d
 
 
 def create_newton_solver(problem, bc, solver_options=None, adjoint_solver_options=None, newton_options=None,
@@ -164,4 +247,5 @@ def create_newton_solver(problem, bc, solver_options=None, adjoint_solver_option
     return solver
 
 
-__all__ = ["create_newton_solver", "newton_solve", "newton_adjoint", "NewtonLineSearchError", "NewtonStats"]
+__all__ = ["create_newton_solver", "newton_solve", "newton_adjoint", "linearized_guess", "NewtonLineSearchError",
+           "NewtonStats"]

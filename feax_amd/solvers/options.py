@@ -17,6 +17,11 @@ KrylovSolverOptions
              inner solve kept (FP32, recycle x n in HBM) to start the later
              inner solves from the Galerkin projection of their residual
              (0 disables).
+  operator   "csr" (default: assembled BC-eliminated CSR, Jacobi-PCG) or
+             "matfree" (feax's own default Krylov operator, the matrix-free
+             J_bc v, with no preconditioner as in feax, whose Jacobi flag is
+             ignored for a callable operator, feax/solvers/common.py:221-227).
+             precision "fp64" only.
 """
 
 from dataclasses import dataclass
@@ -72,6 +77,7 @@ class KrylovSolverOptions(AbstractSolverOptions):
     inner_tol: float = 1e-5
     max_outer: int = 30
     recycle: int = 3000
+    operator: str = "csr"
 
     def __post_init__(self):
         valid = ("auto", "cg", "bicgstab", "gmres")
@@ -81,6 +87,8 @@ class KrylovSolverOptions(AbstractSolverOptions):
             raise ValueError(f"Invalid precision: {self.precision}. Choose from ('fp64', 'mixed', 'mixed-fp32vec')")
         if self.spmv not in ("auto", "csr", "block3"):
             raise ValueError(f"Invalid spmv: {self.spmv}. Choose from ('auto', 'csr', 'block3')")
+        if self.operator not in ("csr", "matfree"):
+            raise ValueError(f"Invalid operator: {self.operator}. Choose from ('csr', 'matfree')")
 
     def uses_x0(self) -> bool:
         return True

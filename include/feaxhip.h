@@ -231,6 +231,32 @@ int fx_hex8_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, con
                          const int32_t* finc, const double* svar, const double* fdir, int32_t num_bc,
                          const int32_t* bc_rows, const double* bc_vals, int32_t negate, double* res);
 
+/* Matrix-free operator out = J_bc(u) v (FP64): feax's Krylov operator
+ * create_matfree_res_J_parametric (feax/assembler.py:1747-1778),
+ * J_bc v = where(bc, v, JVP_R(u)[v * !bc]) under symmetric elimination: rows
+ * with bc_mask set return v, the others the linearised volume residual of v
+ * with the BC entries of v read as 0.  u is the linearisation point (needed
+ * for the Neo-Hookean law, ignored and may be NULL for linear laws).  out
+ * must not alias v. */
+int fx_hex8_jvp_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                    const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                    const fx_material* mat, const double* param, const double* u, const double* v,
+                    const uint8_t* bc_mask, double* out);
+
+/* Jacobi-PCG (jax cg semantics, as fx_pcg_f64) on the matrix-free operator
+ * J_bc(u) of fx_hex8_jvp_f64 instead of an assembled CSR: feax's default
+ * Krylov path (feax/solvers/linear.py:394-398 with the operator of
+ * feax/assembler.py:1747-1778).  dinv may be NULL (plain CG, which is what
+ * feax runs there: its Jacobi flag is ignored for a callable operator,
+ * feax/solvers/common.py:221-227).  flags: 0 or FX_PCG_TIME (rs as in
+ * fx_pcg_f64).  Workspace: fx_pcg_workspace_bytes(3 * num_nodes, 8). */
+int fx_pcg_matfree_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                       const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                       const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                       int32_t flags, const double* dinv, const double* b, const double* x0, double tol, double atol,
+                       int32_t maxiter, int32_t check_every, double* xout, int32_t* iters, double* rs,
+                       void* workspace, size_t ws_bytes);
+
 /* ------------------------------------------------------------------------
  * Adjoint.  feax's backward pass of the linear solve (jax.custom_vjp,
  * feax/solvers/linear.py:426-481) is lambda = J_bc^-T v followed by

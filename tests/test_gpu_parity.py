@@ -250,3 +250,44 @@ def test_krylov_projection_matches_numpy():
     assert np.allclose(x0, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
     rec.count = 0
     assert float(rec.project(torch.as_tensor(r, device="cuda")).abs().max()) == 0.0
+
+
+def test_matfree_operator_solve_matches_direct_and_csr(c1):
+    """operator="matfree": feax's default Krylov path (CG on J_bc v, no
+    preconditioner) == the direct solve; its CG iterates follow the plain CG
+    on the assembled CSR (same operator up to rounding)."""
+    import feax_amd as fe
+    p, bc, tp = c1["problem"], c1["bc"], c1["tp"]
+    res = {}
+    for op in ("matfree", "csr"):
+        opts = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=0.0, maxiter=20000,
+                                      use_jacobi_preconditioner=False, operator=op)
+        solver = fe.create_solver(p, bc, opts, linear=True, traced_params=tp)
+        u = np.asarray(solver(tp, fe.zero_like_initial_guess(p, bc)))
+        res[op] = (u, solver.stats.iters)
+    u, it = res["matfree"]
+    err = np.linalg.norm(u - c1["u_dir"]) / np.linalg.norm(c1["u_dir"])
+    print("matfree iters", it, "csr iters", res["csr"][1], "err", err)
+    assert err < 1e-7
+    assert abs(it - res["csr"][1]) <= max(3, it // 50)
+
+
+def test_matfree_gradient_matches_csr_gradient():
+    """torch.autograd through the matrix-free solve == through the CSR solve."""
+    import torch
+    import feax_amd as fe
+    problem, bc, tp0 = cantilever((12, 3, 3), (1.0, 1.0, 1.0), 12.0, material="cell")
+    rng = np.random.default_rng(5)
+    E0 = rng.uniform(50e3, 90e3, problem.num_cells)
+    w = torch.as_tensor(rng.standard_normal(problem.num_total_dofs_all_vars), device="cuda")
+    grads = {}
+    for op in ("matfree", "csr"):
+        opts = fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0, maxiter=20000,
+                                      use_jacobi_preconditioner=(op == "csr"), operator=op)
+        solver = fe.create_solver(problem, bc, opts, linear=True, return_solution=False)
+        E = torch.tensor(E0, device="cuda", requires_grad=True)
+        L = torch.dot(w, solver(fe.TracedParams(volume_vars=(E,), surface_vars=tp0.surface_vars)))
+        L.backward()
+        grads[op] = E.grad.detach().cpu().numpy()
+    g, ref = grads["matfree"], grads["csr"]
+    assert np.abs(g - ref).max() <= 1e-7 * np.abs(ref).max()
