@@ -27,9 +27,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 
 # HBM bytes per pcg_spmv_kernel dispatch on the 200x40x40 workload, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/spmv_c2_pmc.md), each
-# scaled by the same counter's reading for a copy of known size.
-TRAFFIC = {}
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of `bench.py --kernels-only`
+# (profiles/pcg_spmv_c2_cantilever_pmc.md), each scaled by the same counter's
+# reading for a 1 GiB copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
+TRAFFIC = {"f64/block3": 761_544_739, "f32f64/diff3": 439_444_525}
 
 CONFIGS = {
     "c2": dict(dims=(200, 40, 40), spacing=1.0),
@@ -488,7 +489,7 @@ def main():
         "newton": newton,
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": TRAFFIC.get(dom["label"]), "kernel_us": round(dom["t"] * 1e6, 2),
+                     "traffic": TRAFFIC.get(dom["label"]) if args.config == "c2" else None, "kernel_us": round(dom["t"] * 1e6, 2),
                      "alg_bytes": dom["B"]},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }
