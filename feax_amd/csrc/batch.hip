@@ -1,0 +1,582 @@
+// Batched uneven instances for gfx950: one Jacobi-PCG over many independent
+// HEX8 systems at once.
+//
+// feax batches with jax.vmap over the parameters of one mesh, where a
+// batched while_loop runs until every lane has converged (test_vmap.py of
+// feax, feax/solvers/common.py:229-239 per lane).  Here the instances --
+// different meshes allowed -- are concatenated into one block-diagonal
+// system (node ranges seg_node_ptr[s] .. seg_node_ptr[s+1], CSR in feax's
+// layout, block-3 form) and every CG scalar is per instance:
+//
+//  * every block of every kernel belongs to exactly one instance (block
+//    tables built on the host), so its partial sums belong to that instance
+//    and the finalize kernels run one block per instance;
+//  * each instance has its own alpha, beta, gamma, stopping test
+//    (jax cg semantics) and done flag; converged instances' blocks exit at
+//    their first instruction while the others iterate;
+//  * a chunk of check_every iterations is captured once as a HIP graph and
+//    replayed on the caller's stream until all instances are done (one host
+//    read of the per-instance flags per chunk);
+//  * FX_PCG_AUTOTUNE times the SpMV lane counts 8 / 16 / 32 on the
+//    instance set and keeps the fastest per problem shape.
+// FP64 partial sums in a fixed order: bitwise repeatable, with or without
+// the graph.
+#include <algorithm>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "common.h"
+#include "pcg_common.h"
+#include "reduce.h"
+#include "spmv_common.h"
+
+namespace fx {
+namespace {
+
+constexpr int kSegBlock = 256;  // threads per block
+constexpr int kVecRows = 4096;  // rows per vector-kernel block
+
+struct BlockRange {
+  int seg, begin, end;  // instance, [begin, end) of nodes (SpMV) or rows (vector kernels)
+};
+
+// Sum of cnt partials starting at p (one finalize block).
+__device__ __forceinline__ double seg_sum(const double* __restrict__ p, int cnt) {
+  return sum_partials<1>(p, cnt).v[0];
+}
+
+template <typename V, typename T, int L, bool DIFF, bool DOT>
+__global__ __launch_bounds__(kSegBlock) void seg_spmv_kernel(const BlockRange* __restrict__ tab,
+                                                             const int* __restrict__ indptr,
+                                                             const int* __restrict__ bcol, const V* __restrict__ data,
+                                                             const T* __restrict__ x, T* __restrict__ y,
+                                                             const PcgState* __restrict__ st, double* partials) {
+  const BlockRange br = tab[blockIdx.x];
+  if constexpr (DOT) {
+    if (st[br.seg].done) return;  // uniform over the block
+  }
+  const int gl = threadIdx.x % L;
+  const int g = br.begin + threadIdx.x / L;
+  DVec<1> acc;
+  acc.v[0] = 0.0;
+  if (g < br.end) {
+    T y0, y1, y2;
+    b3_rows<V, T, L, DIFF>(indptr, bcol, data, x, g, gl, y0, y1, y2);
+    if (gl == 0) {
+      y[3 * g] = y0;
+      y[3 * g + 1] = y1;
+      y[3 * g + 2] = y2;
+      if constexpr (DOT)
+        acc.v[0] = static_cast<double>(x[3 * g]) * static_cast<double>(y0) +
+                   static_cast<double>(x[3 * g + 1]) * static_cast<double>(y1) +
+                   static_cast<double>(x[3 * g + 2]) * static_cast<double>(y2);
+    }
+  }
+  if constexpr (DOT) {
+    const DVec<1> t = block_sum<1, kSegBlock>(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t.v[0];
+  }
+}
+
+__global__ __launch_bounds__(kFinBlock) void seg_fin_alpha(PcgState* st, const int* __restrict__ mv_ptr,
+                                                           const double* __restrict__ partials) {
+  const int s = blockIdx.x;
+  if (st[s].done) return;
+  const double pq = seg_sum(partials + mv_ptr[s], mv_ptr[s + 1] - mv_ptr[s]);
+  if (threadIdx.x == 0) st[s].alpha = st[s].gamma / pq;
+}
+
+// x += alpha p; r -= alpha q; z = M r; partials of r.z and r.r.
+template <typename T>
+__global__ __launch_bounds__(kSegBlock) void seg_update_kernel(const BlockRange* __restrict__ vtab, T* __restrict__ x,
+                                                               T* __restrict__ r, T* __restrict__ z,
+                                                               const T* __restrict__ p, const T* __restrict__ q,
+                                                               const T* __restrict__ dinv,
+                                                               const PcgState* __restrict__ st, double* prz,
+                                                               double* prr) {
+  const BlockRange br = vtab[blockIdx.x];
+  if (st[br.seg].done) return;
+  const T alpha = static_cast<T>(st[br.seg].alpha);
+  DVec<2> acc;
+  acc.v[0] = acc.v[1] = 0.0;
+  for (int i = br.begin + threadIdx.x; i < br.end; i += kSegBlock) {
+    x[i] += alpha * p[i];
+    const T ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const T zi = dinv ? dinv[i] * ri : ri;
+    z[i] = zi;
+    acc.v[0] += static_cast<double>(ri) * static_cast<double>(zi);
+    acc.v[1] += static_cast<double>(ri) * static_cast<double>(ri);
+  }
+  const DVec<2> t = block_sum<2, kSegBlock>(acc);
+  if (threadIdx.x == 0) {
+    prz[blockIdx.x] = t.v[0];
+    prr[blockIdx.x] = t.v[1];
+  }
+}
+
+__global__ __launch_bounds__(kFinBlock) void seg_fin_beta(PcgState* st, const int* __restrict__ v_ptr,
+                                                          const double* __restrict__ prz,
+                                                          const double* __restrict__ prr) {
+  const int s = blockIdx.x;
+  if (st[s].done) return;
+  const int b0 = v_ptr[s], cnt = v_ptr[s + 1] - v_ptr[s];
+  const double g_new = seg_sum(prz + b0, cnt);
+  const double rs = seg_sum(prr + b0, cnt);
+  if (threadIdx.x == 0) {
+    PcgState& S = st[s];
+    S.beta = g_new / S.gamma;
+    S.gamma = g_new;
+    S.rs = rs;
+    S.k += 1;
+    S.done = !(S.rs > S.atol2 && S.k < S.maxiter);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSegBlock) void seg_dir_kernel(const BlockRange* __restrict__ vtab, T* __restrict__ p,
+                                                            const T* __restrict__ z,
+                                                            const PcgState* __restrict__ st) {
+  const BlockRange br = vtab[blockIdx.x];
+  if (st[br.seg].done) return;
+  const T beta = static_cast<T>(st[br.seg].beta);
+  for (int i = br.begin + threadIdx.x; i < br.end; i += kSegBlock) p[i] = z[i] + beta * p[i];
+}
+
+// r = b - q (q = A x0), z = M r, p = z; partials of r.z, r.r, b.b.
+template <typename T>
+__global__ __launch_bounds__(kSegBlock) void seg_init_kernel(const BlockRange* __restrict__ vtab,
+                                                             const T* __restrict__ b, const T* __restrict__ q,
+                                                             T* __restrict__ r, T* __restrict__ z, T* __restrict__ p,
+                                                             const T* __restrict__ dinv, double* prz, double* prr,
+                                                             double* pbb) {
+  const BlockRange br = vtab[blockIdx.x];
+  DVec<3> acc;
+  acc.v[0] = acc.v[1] = acc.v[2] = 0.0;
+  for (int i = br.begin + threadIdx.x; i < br.end; i += kSegBlock) {
+    const T bi = b[i];
+    const T ri = bi - q[i];
+    r[i] = ri;
+    const T zi = dinv ? dinv[i] * ri : ri;
+    z[i] = zi;
+    p[i] = zi;
+    acc.v[0] += static_cast<double>(ri) * static_cast<double>(zi);
+    acc.v[1] += static_cast<double>(ri) * static_cast<double>(ri);
+    acc.v[2] += static_cast<double>(bi) * static_cast<double>(bi);
+  }
+  const DVec<3> t = block_sum<3, kSegBlock>(acc);
+  if (threadIdx.x == 0) {
+    prz[blockIdx.x] = t.v[0];
+    prr[blockIdx.x] = t.v[1];
+    pbb[blockIdx.x] = t.v[2];
+  }
+}
+
+__global__ __launch_bounds__(kFinBlock) void seg_fin_init(PcgState* st, const int* __restrict__ v_ptr,
+                                                          const double* __restrict__ prz,
+                                                          const double* __restrict__ prr,
+                                                          const double* __restrict__ pbb, double tol, double atol,
+                                                          int maxiter) {
+  const int s = blockIdx.x;
+  const int b0 = v_ptr[s], cnt = v_ptr[s + 1] - v_ptr[s];
+  const double g = seg_sum(prz + b0, cnt);
+  const double rs = seg_sum(prr + b0, cnt);
+  const double bb = seg_sum(pbb + b0, cnt);
+  if (threadIdx.x == 0) {
+    PcgState& S = st[s];
+    S.alpha = S.beta = 0.0;
+    S.gamma = g;
+    S.rs = rs;
+    S.bb = bb;
+    S.atol2 = fmax(tol * tol * bb, atol * atol);
+    S.k = 0;
+    S.maxiter = maxiter;
+    S.nparts = cnt;
+    S.done = !(S.rs > S.atol2 && 0 < maxiter);
+  }
+}
+
+// Per-instance dot products out[s] = sum_{i in rows of s} a_i b_i.
+__global__ __launch_bounds__(kFinBlock) void seg_dot_kernel(const int* __restrict__ row_ptr,
+                                                            const double* __restrict__ a,
+                                                            const double* __restrict__ b, double* __restrict__ out) {
+  const int s = blockIdx.x;
+  const int r0 = row_ptr[s], r1 = row_ptr[s + 1];
+  DVec<1> acc;
+  acc.v[0] = 0.0;
+  for (int i = r0 + threadIdx.x; i < r1; i += kFinBlock) acc.v[0] += a[i] * b[i];
+  const DVec<1> t = block_sum<1, kFinBlock>(acc);
+  if (threadIdx.x == 0) out[s] = t.v[0];
+}
+
+// ----------------------------------------------------------------- host side
+inline size_t al256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+struct SegLayout {
+  int nseg = 0, n = 0, nmv = 0, nv = 0;  // instances, rows, SpMV blocks (worst case), vector blocks
+  size_t off_st = 0, off_mvtab = 0, off_mvptr = 0, off_vtab = 0, off_vptr = 0, off_rowptr = 0, off_pmv = 0,
+         off_pv = 0, off_vec = 0, vec_bytes = 0, total = 0;
+};
+
+// Worst-case SpMV block count over the lane counts used (L = 32: 8 nodes per block).
+inline int mv_blocks_cap(const int* seg_node_ptr, int nseg) {
+  long total = 0;
+  for (int s = 0; s < nseg; ++s) total += (seg_node_ptr[s + 1] - seg_node_ptr[s] + 7) / 8;
+  return static_cast<int>(std::max<long>(total, 1));
+}
+
+inline int v_blocks(const int* seg_node_ptr, int nseg) {
+  long total = 0;
+  for (int s = 0; s < nseg; ++s) total += (3L * (seg_node_ptr[s + 1] - seg_node_ptr[s]) + kVecRows - 1) / kVecRows;
+  return static_cast<int>(std::max<long>(total, 1));
+}
+
+inline SegLayout seg_layout(int nseg, const int* seg_node_ptr, size_t elem) {
+  SegLayout L;
+  L.nseg = nseg;
+  L.n = 3 * seg_node_ptr[nseg];
+  L.nmv = mv_blocks_cap(seg_node_ptr, nseg);
+  L.nv = v_blocks(seg_node_ptr, nseg);
+  size_t o = 0;
+  L.off_st = o;
+  o += al256(static_cast<size_t>(nseg) * sizeof(PcgState));
+  L.off_mvtab = o;
+  o += al256(static_cast<size_t>(L.nmv) * sizeof(BlockRange));
+  L.off_mvptr = o;
+  o += al256(static_cast<size_t>(nseg + 1) * sizeof(int));
+  L.off_vtab = o;
+  o += al256(static_cast<size_t>(L.nv) * sizeof(BlockRange));
+  L.off_vptr = o;
+  o += al256(static_cast<size_t>(nseg + 1) * sizeof(int));
+  L.off_rowptr = o;
+  o += al256(static_cast<size_t>(nseg + 1) * sizeof(int));
+  L.off_pmv = o;
+  o += al256(static_cast<size_t>(L.nmv) * sizeof(double));
+  L.off_pv = o;
+  o += al256(3 * static_cast<size_t>(L.nv) * sizeof(double));
+  L.off_vec = o;
+  L.vec_bytes = al256(static_cast<size_t>(L.n) * elem);
+  o += 4 * L.vec_bytes;
+  L.total = o;
+  return L;
+}
+
+// SpMV block table for L lanes per node (kSegBlock / L nodes per block).
+inline void mv_table(int nseg, const int* seg_node_ptr, int lanes, std::vector<BlockRange>& tab,
+                     std::vector<int>& ptr) {
+  const int gpb = kSegBlock / lanes;
+  tab.clear();
+  ptr.assign(nseg + 1, 0);
+  for (int s = 0; s < nseg; ++s) {
+    ptr[s] = static_cast<int>(tab.size());
+    for (int a = seg_node_ptr[s]; a < seg_node_ptr[s + 1]; a += gpb)
+      tab.push_back(BlockRange{s, a, std::min(a + gpb, seg_node_ptr[s + 1])});
+  }
+  ptr[nseg] = static_cast<int>(tab.size());
+}
+
+inline void v_table(int nseg, const int* seg_node_ptr, std::vector<BlockRange>& tab, std::vector<int>& ptr,
+                    std::vector<int>& rowptr) {
+  tab.clear();
+  ptr.assign(nseg + 1, 0);
+  rowptr.assign(nseg + 1, 0);
+  for (int s = 0; s < nseg; ++s) {
+    ptr[s] = static_cast<int>(tab.size());
+    const int r0 = 3 * seg_node_ptr[s], r1 = 3 * seg_node_ptr[s + 1];
+    rowptr[s] = r0;
+    for (int a = r0; a < r1; a += kVecRows) tab.push_back(BlockRange{s, a, std::min(a + kVecRows, r1)});
+  }
+  ptr[nseg] = static_cast<int>(tab.size());
+  rowptr[nseg] = 3 * seg_node_ptr[nseg];
+}
+
+template <typename V, typename T, int L, bool DIFF>
+struct SegOp {
+  const BlockRange* tab;
+  int nblocks;
+  const int *indptr, *bcol;
+  const V* data;
+  void apply(hipStream_t s, const T* x, T* y) const {
+    hipLaunchKernelGGL((seg_spmv_kernel<V, T, L, DIFF, false>), dim3(nblocks), dim3(kSegBlock), 0, s, tab, indptr,
+                       bcol, data, x, y, nullptr, nullptr);
+  }
+  void iter(hipStream_t s, const T* p, T* q, const PcgState* st, double* partials) const {
+    hipLaunchKernelGGL((seg_spmv_kernel<V, T, L, DIFF, true>), dim3(nblocks), dim3(kSegBlock), 0, s, tab, indptr,
+                       bcol, data, p, q, st, partials);
+  }
+};
+
+struct SegBuffers {
+  PcgState* st;
+  BlockRange *mvtab, *vtab;
+  int *mvptr, *vptr;
+  double *pmv, *prz, *prr, *pbb;
+  int nseg, nv;
+};
+
+template <typename T, typename Op>
+void seg_iteration(hipStream_t s, const Op& op, const SegBuffers& B, const T* dinv, T* x, T* r, T* z, T* p, T* q) {
+  op.iter(s, p, q, B.st, B.pmv);
+  hipLaunchKernelGGL(seg_fin_alpha, dim3(B.nseg), dim3(kFinBlock), 0, s, B.st, B.mvptr, B.pmv);
+  hipLaunchKernelGGL((seg_update_kernel<T>), dim3(B.nv), dim3(kSegBlock), 0, s, B.vtab, x, r, z, p, q, dinv, B.st,
+                     B.prz, B.prr);
+  hipLaunchKernelGGL(seg_fin_beta, dim3(B.nseg), dim3(kFinBlock), 0, s, B.st, B.vptr, B.prz, B.prr);
+  hipLaunchKernelGGL((seg_dir_kernel<T>), dim3(B.nv), dim3(kSegBlock), 0, s, B.vtab, p, z, B.st);
+}
+
+template <typename T, typename Op>
+int seg_pcg_loop(hipStream_t s, const Op& op, const SegBuffers& B, const T* dinv, const T* b, const T* x0, T* x,
+                 T* r, T* z, T* p, T* q, int n, double tol, double atol, int maxiter, int check_every, bool graph,
+                 int* iters, double* rs, int* chunks_out) {
+  if (x0) {
+    if (x0 != x) FX_HIP(hipMemcpyAsync(x, x0, static_cast<size_t>(n) * sizeof(T), hipMemcpyDeviceToDevice, s));
+  } else {
+    FX_HIP(hipMemsetAsync(x, 0, static_cast<size_t>(n) * sizeof(T), s));
+  }
+  op.apply(s, x, q);
+  hipLaunchKernelGGL((seg_init_kernel<T>), dim3(B.nv), dim3(kSegBlock), 0, s, B.vtab, b, q, r, z, p, dinv, B.prz,
+                     B.prr, B.pbb);
+  hipLaunchKernelGGL(seg_fin_init, dim3(B.nseg), dim3(kFinBlock), 0, s, B.st, B.vptr, B.prz, B.prr, B.pbb, tol, atol,
+                     maxiter);
+  FX_LAUNCHED("seg_pcg_init");
+  if (check_every <= 0) check_every = 32;
+  std::vector<PcgState> h(B.nseg);
+  auto all_done = [&]() {
+    for (const auto& S : h)
+      if (!S.done) return false;
+    return true;
+  };
+  FX_HIP(hipMemcpyAsync(h.data(), B.st, B.nseg * sizeof(PcgState), hipMemcpyDeviceToHost, s));
+  FX_HIP(hipStreamSynchronize(s));
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  hipStream_t cs = nullptr;
+  int rc = FX_OK;
+  if (graph && !all_done()) {
+    // Capture one chunk of iterations on a private stream (default
+    // priority); it is replayed on the caller's stream.
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      for (int c = 0; c < check_every; ++c) seg_iteration<T, Op>(cs, op, B, dinv, x, r, z, p, q);
+      e = hipStreamEndCapture(cs, &g);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) rc = hip_fail(e, "seg_pcg graph capture");
+  }
+  int launched = 0, chunks = 0;
+  while (rc == FX_OK && !all_done() && launched < maxiter) {
+    if (ge) {
+      hipError_t e = hipGraphLaunch(ge, s);
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "hipGraphLaunch");
+        break;
+      }
+    } else {
+      for (int c = 0; c < check_every; ++c) seg_iteration<T, Op>(s, op, B, dinv, x, r, z, p, q);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "seg_pcg_iter");
+        break;
+      }
+    }
+    launched += check_every;
+    ++chunks;
+    hipError_t e = hipMemcpyAsync(h.data(), B.st, B.nseg * sizeof(PcgState), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(e, "seg_pcg flags");
+  }
+  if (ge) (void)hipGraphExecDestroy(ge);
+  if (g) (void)hipGraphDestroy(g);
+  if (cs) (void)hipStreamDestroy(cs);
+  if (rc) return rc;
+  for (int i = 0; i < B.nseg; ++i) {
+    iters[i] = h[i].k;
+    rs[i] = h[i].rs;
+  }
+  if (chunks_out) *chunks_out = chunks;
+  return FX_OK;
+}
+
+// Autotune cache: problem shape -> lanes.
+std::mutex g_tune_mu;
+std::map<std::tuple<int, long long, int, int, int>, int> g_tune;
+
+template <typename V, typename T, bool DIFF>
+int tune_lanes(hipStream_t s, int nseg, const int* seg_node_ptr, long long nse, const int* indptr, const int* bcol,
+               const V* data, const T* xin, T* yout, BlockRange* dtab, int* lanes_out) {
+  const auto key = std::make_tuple(seg_node_ptr[nseg], nse, nseg, static_cast<int>(sizeof(V)), DIFF ? 1 : 0);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      *lanes_out = it->second;
+      return FX_OK;
+    }
+  }
+  hipEvent_t e0, e1;
+  FX_HIP(hipEventCreate(&e0));
+  FX_HIP(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 1e30f;
+  std::vector<BlockRange> tab;
+  std::vector<int> ptr;
+  for (int lanes : {8, 16, 32}) {
+    mv_table(nseg, seg_node_ptr, lanes, tab, ptr);
+    FX_HIP(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(BlockRange), hipMemcpyHostToDevice, s));
+    const int nb = static_cast<int>(tab.size());
+    auto launch = [&]() {
+      switch (lanes) {
+        case 8:
+          hipLaunchKernelGGL((seg_spmv_kernel<V, T, 8, DIFF, false>), dim3(nb), dim3(kSegBlock), 0, s, dtab, indptr,
+                             bcol, data, xin, yout, nullptr, nullptr);
+          break;
+        case 16:
+          hipLaunchKernelGGL((seg_spmv_kernel<V, T, 16, DIFF, false>), dim3(nb), dim3(kSegBlock), 0, s, dtab, indptr,
+                             bcol, data, xin, yout, nullptr, nullptr);
+          break;
+        default:
+          hipLaunchKernelGGL((seg_spmv_kernel<V, T, 32, DIFF, false>), dim3(nb), dim3(kSegBlock), 0, s, dtab, indptr,
+                             bcol, data, xin, yout, nullptr, nullptr);
+      }
+    };
+    launch();
+    FX_HIP(hipEventRecord(e0, s));
+    for (int rep = 0; rep < 5; ++rep) launch();
+    FX_HIP(hipEventRecord(e1, s));
+    FX_HIP(hipEventSynchronize(e1));
+    FX_LAUNCHED("seg_spmv_tune");
+    float ms = 0.f;
+    FX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = lanes;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = best;
+  }
+  *lanes_out = best;
+  return FX_OK;
+}
+
+template <typename V, typename T, bool DIFF>
+int run_batched(void* stream, int nseg, const int* seg_node_ptr, long long nse, int flags, const int* indptr,
+                const int* bcol, const V* data, const T* dinv, const T* b, const T* x0, double tol, double atol,
+                int maxiter, int check_every, T* x, int* iters, double* rs, int* lanes_used, void* workspace,
+                size_t ws_bytes) {
+  hipStream_t s = as_stream(stream);
+  if (nseg <= 0 || !seg_node_ptr || !indptr || !bcol || !data || !b || !x || !iters || !rs)
+    return fail(FX_EINVAL, "pcg_batched: bad arguments");
+  if (maxiter < 0) return fail(FX_EINVAL, "pcg_batched: maxiter < 0");
+  for (int i = 0; i < nseg; ++i)
+    if (seg_node_ptr[i + 1] < seg_node_ptr[i] || seg_node_ptr[0] != 0)
+      return fail(FX_EINVAL, "pcg_batched: seg_node_ptr must start at 0 and be non-decreasing");
+  const SegLayout Lay = seg_layout(nseg, seg_node_ptr, sizeof(T));
+  if (!workspace || ws_bytes < Lay.total)
+    return fail(FX_EWORKSPACE, "pcg_batched: workspace %zu < %zu bytes", ws_bytes, Lay.total);
+  char* w = static_cast<char*>(workspace);
+  SegBuffers B;
+  B.st = reinterpret_cast<PcgState*>(w + Lay.off_st);
+  B.mvtab = reinterpret_cast<BlockRange*>(w + Lay.off_mvtab);
+  B.mvptr = reinterpret_cast<int*>(w + Lay.off_mvptr);
+  B.vtab = reinterpret_cast<BlockRange*>(w + Lay.off_vtab);
+  B.vptr = reinterpret_cast<int*>(w + Lay.off_vptr);
+  B.pmv = reinterpret_cast<double*>(w + Lay.off_pmv);
+  B.prz = reinterpret_cast<double*>(w + Lay.off_pv);
+  B.prr = B.prz + Lay.nv;
+  B.pbb = B.prr + Lay.nv;
+  B.nseg = nseg;
+  T* r = reinterpret_cast<T*>(w + Lay.off_vec);
+  T* z = reinterpret_cast<T*>(w + Lay.off_vec + Lay.vec_bytes);
+  T* p = reinterpret_cast<T*>(w + Lay.off_vec + 2 * Lay.vec_bytes);
+  T* q = reinterpret_cast<T*>(w + Lay.off_vec + 3 * Lay.vec_bytes);
+
+  int lanes = sizeof(V) == 8 ? 32 : 16;
+  if (flags & FX_PCG_AUTOTUNE) {
+    int rc = tune_lanes<V, T, DIFF>(s, nseg, seg_node_ptr, nse, indptr, bcol, data, b, q, B.mvtab, &lanes);
+    if (rc) return rc;
+  }
+  std::vector<BlockRange> mvtab, vtab;
+  std::vector<int> mvptr, vptr, rowptr;
+  mv_table(nseg, seg_node_ptr, lanes, mvtab, mvptr);
+  v_table(nseg, seg_node_ptr, vtab, vptr, rowptr);
+  if (static_cast<int>(mvtab.size()) > Lay.nmv || static_cast<int>(vtab.size()) > Lay.nv)
+    return fail(FX_ELIMIT, "pcg_batched: block tables exceed the workspace layout");
+  B.nv = static_cast<int>(vtab.size());
+  FX_HIP(hipMemcpyAsync(B.mvtab, mvtab.data(), mvtab.size() * sizeof(BlockRange), hipMemcpyHostToDevice, s));
+  FX_HIP(hipMemcpyAsync(B.mvptr, mvptr.data(), mvptr.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  FX_HIP(hipMemcpyAsync(B.vtab, vtab.data(), vtab.size() * sizeof(BlockRange), hipMemcpyHostToDevice, s));
+  FX_HIP(hipMemcpyAsync(B.vptr, vptr.data(), vptr.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  FX_HIP(hipStreamSynchronize(s));  // the host vectors above go out of scope
+  if (lanes_used) *lanes_used = lanes;
+  const int nb = static_cast<int>(mvtab.size());
+  const bool graph = !(flags & FX_PCG_NO_GRAPH);
+  const int n = Lay.n;
+  switch (lanes) {
+    case 8:
+      return seg_pcg_loop<T>(s, SegOp<V, T, 8, DIFF>{B.mvtab, nb, indptr, bcol, data}, B, dinv, b, x0, x, r, z, p, q,
+                             n, tol, atol, maxiter, check_every, graph, iters, rs, nullptr);
+    case 16:
+      return seg_pcg_loop<T>(s, SegOp<V, T, 16, DIFF>{B.mvtab, nb, indptr, bcol, data}, B, dinv, b, x0, x, r, z, p,
+                             q, n, tol, atol, maxiter, check_every, graph, iters, rs, nullptr);
+    case 32:
+      return seg_pcg_loop<T>(s, SegOp<V, T, 32, DIFF>{B.mvtab, nb, indptr, bcol, data}, B, dinv, b, x0, x, r, z, p,
+                             q, n, tol, atol, maxiter, check_every, graph, iters, rs, nullptr);
+    default:
+      return fail(FX_EINVAL, "pcg_batched: lane count %d", lanes);
+  }
+}
+
+}  // namespace
+}  // namespace fx
+
+using namespace fx;
+
+extern "C" {
+
+size_t fx_pcg_batched_workspace_bytes(int32_t nseg, const int32_t* seg_node_ptr, int32_t vec_elem_bytes) {
+  if (nseg <= 0 || !seg_node_ptr || (vec_elem_bytes != 4 && vec_elem_bytes != 8)) return 0;
+  return seg_layout(nseg, seg_node_ptr, static_cast<size_t>(vec_elem_bytes)).total;
+}
+
+int fx_pcg_batched_f64(void* stream, int32_t nseg, const int32_t* seg_node_ptr, int64_t nse, int32_t flags,
+                       const int32_t* indptr, const int32_t* bcol, const double* data, const double* dinv,
+                       const double* b, const double* x0, double tol, double atol, int32_t maxiter,
+                       int32_t check_every, double* x, int32_t* iters, double* rs, int32_t* lanes_used,
+                       void* workspace, size_t ws_bytes) {
+  if (flags & FX_PCG_DIFF3) return fail(FX_EINVAL, "fx_pcg_batched_f64: FX_PCG_DIFF3 is for FP32 matrices");
+  return run_batched<double, double, false>(stream, nseg, seg_node_ptr, nse, flags, indptr, bcol, data, dinv, b, x0,
+                                            tol, atol, maxiter, check_every, x, iters, rs, lanes_used, workspace,
+                                            ws_bytes);
+}
+
+int fx_pcg_batched_f32f64(void* stream, int32_t nseg, const int32_t* seg_node_ptr, int64_t nse, int32_t flags,
+                          const int32_t* indptr, const int32_t* bcol, const float* data, const double* dinv,
+                          const double* b, const double* x0, double tol, double atol, int32_t maxiter,
+                          int32_t check_every, double* x, int32_t* iters, double* rs, int32_t* lanes_used,
+                          void* workspace, size_t ws_bytes) {
+  if (flags & FX_PCG_DIFF3)
+    return run_batched<float, double, true>(stream, nseg, seg_node_ptr, nse, flags, indptr, bcol, data, dinv, b, x0,
+                                            tol, atol, maxiter, check_every, x, iters, rs, lanes_used, workspace,
+                                            ws_bytes);
+  return run_batched<float, double, false>(stream, nseg, seg_node_ptr, nse, flags, indptr, bcol, data, dinv, b, x0,
+                                           tol, atol, maxiter, check_every, x, iters, rs, lanes_used, workspace,
+                                           ws_bytes);
+}
+
+int fx_seg_dot_f64(void* stream, int32_t nseg, const int32_t* row_ptr, const double* a, const double* b,
+                   double* out) {
+  if (nseg <= 0 || !row_ptr || !a || !b || !out) return fail(FX_EINVAL, "fx_seg_dot_f64: bad arguments");
+  hipLaunchKernelGGL(seg_dot_kernel, dim3(nseg), dim3(kFinBlock), 0, as_stream(stream), row_ptr, a, b, out);
+  FX_LAUNCHED("seg_dot_kernel");
+  return FX_OK;
+}
+
+}  // extern "C"

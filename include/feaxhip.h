@@ -258,6 +258,43 @@ int fx_pcg_matfree_f64(void* stream, int32_t num_nodes, int32_t num_cells, const
                        void* workspace, size_t ws_bytes);
 
 /* ------------------------------------------------------------------------
+ * Batched instances.  feax batches solves with jax.vmap over the parameters
+ * of one mesh; its batched while_loop iterates until every lane has
+ * converged (tests/test_vmap.py:63-106, feax/solvers/common.py:229-239 per
+ * lane).  Here the instances (different meshes allowed) are concatenated
+ * into one block-diagonal system: instance s owns nodes
+ * seg_node_ptr[s] .. seg_node_ptr[s+1] (HOST array, nseg+1 entries, first 0),
+ * and the CSR (block-3 form, bcol from fx_csr_block3_index) is the
+ * concatenation of the instances' CSR.  Every CG scalar, the stopping test
+ * (jax cg semantics, tol/atol/maxiter shared) and the convergence flag are
+ * per instance; iters and rs are HOST arrays of nseg entries.  A chunk of
+ * check_every iterations is captured once as a HIP graph and replayed
+ * (FX_PCG_NO_GRAPH: launch kernels directly; results are identical).
+ * FX_PCG_AUTOTUNE: time the SpMV lane counts 8/16/32 on this instance set
+ * (cached per shape); *lanes_used (nullable) reports the choice.
+ * fx_pcg_batched_f32f64: FP32 matrix (FX_PCG_DIFF3: difference form), FP64
+ * vectors.  Workspace: fx_pcg_batched_workspace_bytes(nseg, seg_node_ptr,
+ * 8).
+ * ------------------------------------------------------------------------ */
+#define FX_PCG_AUTOTUNE 8
+#define FX_PCG_NO_GRAPH 16
+size_t fx_pcg_batched_workspace_bytes(int32_t nseg, const int32_t* seg_node_ptr, int32_t vec_elem_bytes);
+int fx_pcg_batched_f64(void* stream, int32_t nseg, const int32_t* seg_node_ptr, int64_t nse, int32_t flags,
+                       const int32_t* indptr, const int32_t* bcol, const double* data, const double* dinv,
+                       const double* b, const double* x0, double tol, double atol, int32_t maxiter,
+                       int32_t check_every, double* x, int32_t* iters, double* rs, int32_t* lanes_used,
+                       void* workspace, size_t ws_bytes);
+int fx_pcg_batched_f32f64(void* stream, int32_t nseg, const int32_t* seg_node_ptr, int64_t nse, int32_t flags,
+                          const int32_t* indptr, const int32_t* bcol, const float* data, const double* dinv,
+                          const double* b, const double* x0, double tol, double atol, int32_t maxiter,
+                          int32_t check_every, double* x, int32_t* iters, double* rs, int32_t* lanes_used,
+                          void* workspace, size_t ws_bytes);
+/* out[s] = sum over the rows row_ptr[s] .. row_ptr[s+1] of a_i b_i
+ * (row_ptr: DEVICE array, nseg+1 entries; out: device, nseg). */
+int fx_seg_dot_f64(void* stream, int32_t nseg, const int32_t* row_ptr, const double* a, const double* b,
+                   double* out);
+
+/* ------------------------------------------------------------------------
  * Adjoint.  feax's backward pass of the linear solve (jax.custom_vjp,
  * feax/solvers/linear.py:426-481) is lambda = J_bc^-T v followed by
  * grads = -(dR_bc/dparams)^T lambda, the jax.vjp of the residual
