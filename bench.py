@@ -329,6 +329,95 @@ def newton_bench(n_fd=16, eps=1e-5, forms=("fp64", "mixed")):
     return out
 
 
+def _plane_x(value):
+    import numpy as np
+    return lambda p: np.isclose(p[0], value)
+
+
+def batch_bench(steps, warmup, nseg=64, seed=20261203):
+    """64 uneven SIMP cantilevers in one batch: nx ~ U{20..80}, ny, nz ~
+    U{8..24}, rho_e ~ U(0.3, 1.0), traction 10^U(-4,-2) e_z on x = nx, x = 0
+    clamped; one assembly launch and one segmented PCG (per-instance
+    stopping at rel 1e-8) for all instances, FP64 and mixed.  Compared with
+    the same instances solved one after another with create_solver (per
+    instance relative difference of the solutions)."""
+    import numpy as np
+    import torch
+    import feax_amd as fe
+    from feax_amd.batch import BatchedProblem, batch_bc, batch_params, create_batched_solver
+
+    class Simp(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.SIMPLinearElastic(nu=0.3, E0=1.0, Emin=1e-9, penal=3.0)
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.0, 0.0, 1.0))]
+
+    rng = np.random.default_rng(seed)
+    inst = []
+    t0 = time.perf_counter()
+    for _ in range(nseg):
+        nx, ny, nz = int(rng.integers(20, 81)), int(rng.integers(8, 25)), int(rng.integers(8, 25))
+        prob = Simp(fe.StructuredGrid((nx, ny, nz), spacing=(1.0, 1.0, 1.0)).to_mesh(), vec=3, dim=3,
+                    location_fns=[_plane_x(nx)])
+        bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[0], 0.0), "all", 0.0)]).create_bc(prob)
+        rho = rng.uniform(0.3, 1.0, prob.num_cells)
+        t = 10.0 ** rng.uniform(-4, -2)
+        tp = fe.TracedParams(volume_vars=(rho,),
+                             surface_vars=[(fe.TracedParams.create_uniform_surface_var(prob, t),)])
+        inst.append((prob, bc, tp))
+    assert all(len(p.boundary_inds_list[0]) > 0 for p, _, _ in inst), "every instance carries its end load"
+    bp = BatchedProblem([i[0] for i in inst])
+    bcb = batch_bc(bp, [i[1] for i in inst])
+    tpb = batch_params(bp, [i[2] for i in inst])
+    ts = fe.TracedStructure.from_problem(bp)
+    tpd = fe.TracedParams(volume_vars=(torch.as_tensor(tpb.volume_vars[0], device="cuda"),),
+                          surface_vars=[(torch.as_tensor(tpb.surface_vars[0][0], device="cuda"),)])
+    dofs = [3 * p.num_total_nodes for p, _, _ in inst]
+    log(f"batch setup {time.perf_counter() - t0:.1f} s: {nseg} instances, {ts.n} DOF "
+        f"({min(dofs)}..{max(dofs)} per instance), nse={ts.nse}")
+    out = {"workload": f"{nseg} SIMP cantilevers nx~U[20,80], ny,nz~U[8,24]: {ts.n} DOF, nse={ts.nse}, rel 1e-8",
+           "dofs_min": min(dofs), "dofs_max": max(dofs)}
+    sols = {}
+    for form in ("fp64", "mixed"):
+        opts = fe.KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=50000, use_jacobi_preconditioner=True,
+                                      precision=form)
+        solver = create_batched_solver(bp, bcb, opts, traced_structure=ts)
+        for _ in range(max(warmup, 1)):      # the first call autotunes the SpMV lanes
+            solver(tpd)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            u = solver(tpd)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t1) / steps
+        st = solver.stats
+        sols[form] = u
+        out[form] = {"s": round(wall, 5), "outer": st.outer, "lanes": st.lanes, "iters_min": min(st.iters),
+                     "iters_max": max(st.iters), "true_relres_max": max(st.relres)}
+        log(f"batch {form}: {wall:.4f} s, lanes {st.lanes}, iters {min(st.iters)}..{max(st.iters)}, "
+            f"outer {st.outer}, max true relres {max(st.relres):.2e}")
+    # the same instances one after another
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=50000, use_jacobi_preconditioner=True)
+    singles = []
+    for p, bc, tp in inst:
+        tsi = fe.TracedStructure.from_problem(p)
+        tpi = fe.TracedParams(volume_vars=(torch.as_tensor(tp.volume_vars[0], device="cuda"),),
+                              surface_vars=[(torch.as_tensor(tp.surface_vars[0][0], device="cuda"),)])
+        singles.append((fe.create_solver(p, bc, opts, linear=True, return_solution=False, traced_structure=tsi), tpi))
+    for s_, tpi in singles:
+        s_(tpi)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    us = [s_(tpi) for s_, tpi in singles]
+    torch.cuda.synchronize()
+    out["sequential_fp64_s"] = round(time.perf_counter() - t2, 5)
+    diffs = {f: max(float((a - b).norm() / b.norm()) for a, b in zip(bp.split(sols[f]), us)) for f in sols}
+    out["max_rel_diff_vs_sequential"] = diffs
+    log(f"batch: sequential create_solver {out['sequential_fp64_s']:.4f} s; max rel diff vs sequential {diffs}")
+    return out
+
+
 def kernels_only(cfg, iters=8, copy_bytes=1 << 30):
     """A short run for hardware-counter passes: `iters` PCG iterations in
     each matrix form (so pcg_spmv_kernel is dispatched `iters` times per
@@ -366,6 +455,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad", action="store_true", help="skip the SIMP compliance-gradient workload")
     ap.add_argument("--no-newton", action="store_true", help="skip the Neo-Hookean Newton workload")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances workload")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch only the PCG SpMV kernels and a 1 GiB copy (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -454,6 +544,7 @@ def main():
     log(f"in-solve SpMV {dom['label']}: {t_live:.2f} us avg over {n_live} launches -> {achieved / 1e9:.0f} GB/s")
     grad = None if args.no_grad else grad_bench(args.steps, args.warmup)
     newton = None if args.no_newton else newton_bench()
+    batch = None if args.no_batch else batch_bench(args.steps, args.warmup)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(problem, bc, A64, results["fp64"]["iters"])
@@ -487,6 +578,7 @@ def main():
         "grad_solve_ratio": None if grad is None else grad["ratio_adjoint"],
         "grad": grad,
         "newton": newton,
+        "batch": batch,
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                      "traffic": TRAFFIC.get(dom["label"]) if args.config == "c2" else None, "kernel_us": round(dom["t"] * 1e6, 2),

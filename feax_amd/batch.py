@@ -152,6 +152,10 @@ def batch_params(bp, tps):
             nf = len(p.boundary_inds_list[j])
             sv = tp.surface_vars[j][0] if (tp.surface_vars and len(tp.surface_vars) > j) else np.zeros((nf, 4))
             sv = sv.detach().cpu().numpy() if isinstance(sv, torch.Tensor) else np.asarray(sv, np.float64)
+            if nf == 0:
+                continue
+            if sv.size == 1:
+                sv = np.full(nf, float(sv.reshape(())))
             parts.append(np.broadcast_to(sv.reshape(nf, -1), (nf, NUM_FACE_QUADS)))
         surf.append((np.concatenate(parts) if parts else np.zeros((0, NUM_FACE_QUADS)),))
     return TracedParams(volume_vars=(torch.cat(vol).numpy(),), surface_vars=surf)

@@ -43,9 +43,18 @@ struct BlockRange {
   int seg, begin, end;  // instance, [begin, end) of nodes (SpMV) or rows (vector kernels)
 };
 
-// Sum of cnt partials starting at p (one finalize block).
-__device__ __forceinline__ double seg_sum(const double* __restrict__ p, int cnt) {
-  return sum_partials<1>(p, cnt).v[0];
+// Sums of cnt partials starting at p[0..NV) (one finalize block, one
+// block reduction for all NV quantities); valid in thread 0.
+template <int NV>
+__device__ __forceinline__ DVec<NV> seg_sums(const double* const (&p)[NV], int cnt) {
+  DVec<NV> acc;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc.v[k] = 0.0;
+  for (int i = threadIdx.x; i < cnt; i += kFinBlock) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc.v[k] += p[k][i];
+  }
+  return block_sum<NV, kFinBlock>(acc);
 }
 
 template <typename V, typename T, int L, bool DIFF, bool DOT>
@@ -85,8 +94,9 @@ __global__ __launch_bounds__(kFinBlock) void seg_fin_alpha(PcgState* st, const i
                                                            const double* __restrict__ partials) {
   const int s = blockIdx.x;
   if (st[s].done) return;
-  const double pq = seg_sum(partials + mv_ptr[s], mv_ptr[s + 1] - mv_ptr[s]);
-  if (threadIdx.x == 0) st[s].alpha = st[s].gamma / pq;
+  const double* src[1] = {partials + mv_ptr[s]};
+  const DVec<1> pq = seg_sums<1>(src, mv_ptr[s + 1] - mv_ptr[s]);
+  if (threadIdx.x == 0) st[s].alpha = st[s].gamma / pq.v[0];
 }
 
 // x += alpha p; r -= alpha q; z = M r; partials of r.z and r.r.
@@ -124,13 +134,14 @@ __global__ __launch_bounds__(kFinBlock) void seg_fin_beta(PcgState* st, const in
   const int s = blockIdx.x;
   if (st[s].done) return;
   const int b0 = v_ptr[s], cnt = v_ptr[s + 1] - v_ptr[s];
-  const double g_new = seg_sum(prz + b0, cnt);
-  const double rs = seg_sum(prr + b0, cnt);
+  const double* src[2] = {prz + b0, prr + b0};
+  const DVec<2> t = seg_sums<2>(src, cnt);
   if (threadIdx.x == 0) {
     PcgState& S = st[s];
+    const double g_new = t.v[0];
     S.beta = g_new / S.gamma;
     S.gamma = g_new;
-    S.rs = rs;
+    S.rs = t.v[1];
     S.k += 1;
     S.done = !(S.rs > S.atol2 && S.k < S.maxiter);
   }
@@ -182,16 +193,15 @@ __global__ __launch_bounds__(kFinBlock) void seg_fin_init(PcgState* st, const in
                                                           int maxiter) {
   const int s = blockIdx.x;
   const int b0 = v_ptr[s], cnt = v_ptr[s + 1] - v_ptr[s];
-  const double g = seg_sum(prz + b0, cnt);
-  const double rs = seg_sum(prr + b0, cnt);
-  const double bb = seg_sum(pbb + b0, cnt);
+  const double* src[3] = {prz + b0, prr + b0, pbb + b0};
+  const DVec<3> t = seg_sums<3>(src, cnt);
   if (threadIdx.x == 0) {
     PcgState& S = st[s];
     S.alpha = S.beta = 0.0;
-    S.gamma = g;
-    S.rs = rs;
-    S.bb = bb;
-    S.atol2 = fmax(tol * tol * bb, atol * atol);
+    S.gamma = t.v[0];
+    S.rs = t.v[1];
+    S.bb = t.v[2];
+    S.atol2 = fmax(tol * tol * S.bb, atol * atol);
     S.k = 0;
     S.maxiter = maxiter;
     S.nparts = cnt;

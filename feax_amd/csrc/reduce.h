@@ -24,6 +24,8 @@ struct DVec {
 
 // Sum NV values over a block of BS threads; result valid in thread 0.
 // Fixed order: butterflies inside each wave, then a serial sum over waves.
+// Call it once per kernel with all quantities (NV > 1): a second call would
+// reuse the LDS slots while thread 0 may still read the first call's.
 template <int NV, int BS>
 __device__ __forceinline__ DVec<NV> block_sum(DVec<NV> a) {
   __shared__ double red[BS / 64][NV];

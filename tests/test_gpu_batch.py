@@ -43,7 +43,8 @@ def _oracle_solution(prob, bc, rho, t):
 def batch():
     from feax_amd.batch import BatchedProblem, batch_bc, batch_params
     rng = np.random.default_rng(20261203)
-    dims = [(6, 3, 3), (9, 2, 4), (5, 4, 2), (12, 3, 3), (4, 2, 2)]
+    # the last instance spans several vector blocks (> 4096 rows)
+    dims = [(6, 3, 3), (9, 2, 4), (5, 4, 2), (12, 3, 3), (4, 2, 2), (30, 8, 8)]
     inst = [_simp_instance(d, rng) for d in dims]
     bp = BatchedProblem([i[0] for i in inst])
     bc = batch_bc(bp, [i[1] for i in inst])
