@@ -361,22 +361,21 @@ int seg_pcg_loop(hipStream_t s, const Op& op, const SegBuffers& B, const T* dinv
   };
   FX_HIP(hipMemcpyAsync(h.data(), B.st, B.nseg * sizeof(PcgState), hipMemcpyDeviceToHost, s));
   FX_HIP(hipStreamSynchronize(s));
-  hipGraph_t g = nullptr;
-  hipGraphExec_t ge = nullptr;
-  hipStream_t cs = nullptr;
+  GraphGuard gg;
   int rc = FX_OK;
   if (graph && !all_done()) {
     // Capture one chunk of iterations on a private stream (default
     // priority); it is replayed on the caller's stream.
-    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    hipError_t e = hipStreamCreateWithFlags(&gg.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(gg.stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      for (int c = 0; c < check_every; ++c) seg_iteration<T, Op>(cs, op, B, dinv, x, r, z, p, q);
-      e = hipStreamEndCapture(cs, &g);
+      for (int c = 0; c < check_every; ++c) seg_iteration<T, Op>(gg.stream, op, B, dinv, x, r, z, p, q);
+      e = hipStreamEndCapture(gg.stream, &gg.graph);
     }
-    if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0);
     if (e != hipSuccess) rc = hip_fail(e, "seg_pcg graph capture");
   }
+  const hipGraphExec_t ge = gg.exec;
   int launched = 0, chunks = 0;
   while (rc == FX_OK && !all_done() && launched < maxiter) {
     if (ge) {
@@ -399,9 +398,6 @@ int seg_pcg_loop(hipStream_t s, const Op& op, const SegBuffers& B, const T* dinv
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) rc = hip_fail(e, "seg_pcg flags");
   }
-  if (ge) (void)hipGraphExecDestroy(ge);
-  if (g) (void)hipGraphDestroy(g);
-  if (cs) (void)hipStreamDestroy(cs);
   if (rc) return rc;
   for (int i = 0; i < B.nseg; ++i) {
     iters[i] = h[i].k;
@@ -427,9 +423,11 @@ int tune_lanes(hipStream_t s, int nseg, const int* seg_node_ptr, long long nse, 
       return FX_OK;
     }
   }
-  hipEvent_t e0, e1;
-  FX_HIP(hipEventCreate(&e0));
-  FX_HIP(hipEventCreate(&e1));
+  EventGuard evg;
+  evg.ev.assign(2, nullptr);
+  FX_HIP(hipEventCreate(&evg.ev[0]));
+  FX_HIP(hipEventCreate(&evg.ev[1]));
+  hipEvent_t e0 = evg.ev[0], e1 = evg.ev[1];
   int best = 0;
   float best_ms = 1e30f;
   std::vector<BlockRange> tab;
@@ -466,8 +464,6 @@ int tune_lanes(hipStream_t s, int nseg, const int* seg_node_ptr, long long nse, 
       best = lanes;
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[key] = best;

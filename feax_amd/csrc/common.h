@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "feaxhip.h"
 
@@ -45,6 +46,33 @@ inline int hip_fail(hipError_t e, const char* where) {
   } while (0)
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Owns a captured HIP graph, its executable and the private capture stream.
+struct GraphGuard {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t stream = nullptr;
+  GraphGuard() = default;
+  GraphGuard(const GraphGuard&) = delete;
+  GraphGuard& operator=(const GraphGuard&) = delete;
+  ~GraphGuard() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// Owns HIP events.
+struct EventGuard {
+  std::vector<hipEvent_t> ev;
+  EventGuard() = default;
+  EventGuard(const EventGuard&) = delete;
+  EventGuard& operator=(const EventGuard&) = delete;
+  ~EventGuard() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
 
 // MI355X: 256 CUs in 8 XCDs; workgroups are dealt to XCDs round-robin.
 constexpr int kNumXcd = 8;

@@ -129,7 +129,10 @@ int fx_csr_diagonal_f32(void* stream, int32_t n, const int32_t* indptr, const in
 #define FX_PCG_DIFF3 2
 /* FX_PCG_TIME: time every SpMV launch of the executed iterations with HIP
  * events on `stream`; rs must then point to 3 doubles: rs[0] = ||r||^2,
- * rs[1] = mean SpMV kernel time in microseconds, rs[2] = launches timed. */
+ * rs[1] = mean SpMV kernel time in microseconds, rs[2] = launches timed.
+ * Without FX_PCG_TIME the iterations run as a HIP graph of check_every
+ * iterations captured once per call and replayed on `stream`;
+ * FX_PCG_NO_GRAPH (16, below) launches them directly (identical results). */
 #define FX_PCG_TIME 4
 /* Optional recording (all three drivers): when rec_p != NULL, iteration k <
  * rec_max stores its search direction p_k as FP32 in rec_p + k*n (d) and
