@@ -123,3 +123,15 @@ class CSRMatrix:
             raise ValueError("to_scipy() of a difference-form matrix")
         return sp.csr_matrix((self.data.detach().cpu().numpy(), self.indices.cpu().numpy(),
                               self.indptr.cpu().numpy()), shape=self.shape)
+
+
+def transpose_with_maps(J, T_perm, T_indptr, T_indices):
+    """Explicit CSR transpose from the static maps of Problem
+    (csr_T_perm / csr_T_indptr / csr_T_indices): values gathered as
+    J.data[T_perm] (feax/csr.py:125-141)."""
+    import torch
+    dev = J.data.device
+    perm = torch.as_tensor(np.asarray(T_perm, np.int64), device=dev)
+    ip = torch.as_tensor(np.asarray(T_indptr, np.int32), device=dev)
+    ix = torch.as_tensor(np.asarray(T_indices, np.int32), device=dev)
+    return CSRMatrix(J.data[perm], ip, ix, (J.shape[1], J.shape[0]), block3=J.block3)

@@ -154,6 +154,50 @@ class Problem:
             self._diag_slots = d
         return self._diag_slots
 
+    @property
+    def csr_T_perm(self):
+        """(nse,) int32: slot order of the transpose, A^T.data = data[csr_T_perm]
+        (feax/problem.py:417-447, stable sort by (col, row))."""
+        if getattr(self, "_T", None) is None:
+            self._build_transpose()
+        return self._T[0]
+
+    @property
+    def csr_T_indptr(self):
+        if getattr(self, "_T", None) is None:
+            self._build_transpose()
+        return self._T[1]
+
+    @property
+    def csr_T_indices(self):
+        if getattr(self, "_T", None) is None:
+            self._build_transpose()
+        return self._T[2]
+
+    def _build_transpose(self):
+        n = self.num_total_dofs_all_vars
+        rows = self.csr_row_of_slot.astype(np.int64)
+        cols = self.csr_indices.astype(np.int64)
+        order = np.argsort(cols * n + rows, kind="stable")
+        t_rows = cols[order]
+        indptr = np.zeros(n + 1, dtype=np.int64)
+        np.add.at(indptr, t_rows + 1, 1)
+        self._T = (order.astype(np.int32), np.cumsum(indptr).astype(np.int32), rows[order].astype(np.int32))
+
+    @property
+    def csr_volume_slots(self):
+        """(E, 576) int32: CSR slot of each element Jacobian entry jac[a*3+i,
+        b*3+j] (feax/problem.py:458-463), from the rank table:
+        indptr[3 n_a + i] + 3 rank[e, a, b] + j."""
+        if getattr(self, "_vol_slots", None) is None:
+            c = self.cells.astype(np.int64)
+            rank = self.elem_rank.reshape(-1, 8, 8).astype(np.int64)
+            rowp = self.csr_indptr.astype(np.int64)[3 * c[:, :, None] + np.arange(3)]      # (E, 8a, 3i)
+            slot = (rowp[:, :, :, None, None] + 3 * rank[:, :, None, :, None]
+                    + np.arange(3)[None, None, None, None, :])                            # (E, a, i, b, j)
+            self._vol_slots = slot.reshape(len(c), 576).astype(np.int32)
+        return self._vol_slots
+
     def _element_inds(self, cells):
         c = np.asarray(cells, np.int64)
         return (3 * c[:, :, None] + np.arange(3)[None, None, :]).reshape(len(c), -1).astype(np.int32)

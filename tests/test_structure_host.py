@@ -172,3 +172,24 @@ def test_native_errors_are_reported():
     assert "out of range" in nat.last_error()
     with pytest.raises(nat.NativeError):
         nat.check(rc, "fx_hex8_pattern")
+
+
+def test_transpose_maps_and_volume_slots():
+    """csr_T_* reproduce scipy's transpose of a random-valued matrix on the
+    pattern; csr_volume_slots equals the oracle's slot map (feax's
+    csr_full_slot volume block)."""
+    import scipy.sparse as sp
+    from conftest import cantilever
+    from oracle import hex8
+    p, _, _ = cantilever((4, 3, 2), (1.0, 1.0, 1.0), 4.0)
+    n = p.num_total_dofs_all_vars
+    rng = np.random.default_rng(0)
+    data = rng.standard_normal(p.csr_nse)
+    A = sp.csr_matrix((data, p.csr_indices, p.csr_indptr), shape=(n, n))
+    At = A.T.tocsr()
+    At.sort_indices()
+    np.testing.assert_array_equal(p.csr_T_indptr, At.indptr)
+    np.testing.assert_array_equal(p.csr_T_indices, At.indices)
+    np.testing.assert_array_equal(data[p.csr_T_perm], At.data)
+    m = hex8.Model(p.points, p.cells.astype(np.int64), p.boundary_inds_list)
+    np.testing.assert_array_equal(p.csr_volume_slots, m.csr["volume_slots"])
