@@ -1,0 +1,146 @@
+"""Writes the golden fixtures of tests/golden/ from the FP64 oracle (oracle/).
+
+    python tests/golden/make_golden.py            # small fixtures (seconds)
+    python tests/golden/make_golden.py --large    # + 200x40x40 and 120x40x20 oracle solves (CPU, minutes)
+
+Fixtures (npz, data only):
+  fixture_2x2x2.npz   the 2x2x2 unit cantilever of feax's conftest (clamped
+                      x=0, traction 1e-3 e_z on x=2, E=70e3 node var, nu=0.3):
+                      indptr, indices, BC-eliminated CSR data, residual at
+                      u=0, b, direct-solve u
+  c1_samples.npz      50x5x5 (h=2): sha256 of indptr/indices, 1000 seeded
+                      (slot, value) pairs of the BC-eliminated CSR, u at 64
+                      seeded DOFs (direct solve)
+  make_ke_3d.npz      unit-cube element stiffness (E=1, nu=0.3), feax corner order
+  simp_4x2x2.npz      SIMP compliance sensitivity on a 4x2x2 grid (rho seeded)
+  c2_oracle.npz       200x40x40 (h=1) cantilever: oracle Jacobi-PCG to rel
+                      1e-12, u at 4096 seeded DOFs, max|u|, ||u||
+  c3_oracle.npz       120x40x20 SIMP cantilever (rho ~ U(0.3,1) seeded,
+                      unit patch load): oracle PCG to rel 1e-12, compliance,
+                      closed-form dC/drho at 4096 seeded cells, max|dC/drho|
+"""
+
+import argparse
+import hashlib
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import adjoint, hex8, krylov, structure  # noqa: E402
+
+SEED = 20261203
+
+
+def cantilever_model(dims, h, t=1e-3, E=70e3):
+    pts, cells = structure.structured_grid(dims, (h, h, h))
+    L = dims[0] * h
+    faces = structure.boundary_faces(pts, cells, lambda p: np.isclose(p[0], L))
+    m = hex8.Model(pts, cells, [faces])
+    bc_rows, bc_vals = structure.dirichlet_rows(pts, [(lambda p: np.isclose(p[0], 0.0), "all", 0.0)])
+    return m, np.full(m.N, E), [hex8.constant_traction(faces, t)], bc_rows, bc_vals
+
+
+def fixture_2x2x2():
+    import scipy.sparse.linalg as spla
+    m, E, tr, rows, vals = cantilever_model((2, 2, 2), 1.0)
+    mat = hex8.Material("linear_elastic", nu=0.3)
+    u, info = krylov.linear_solve(m, mat, E, tr, rows, vals, tol=1e-14, atol=0.0)
+    ud = spla.spsolve(info["K"].tocsc(), info["b"])
+    res0 = m.residual(np.zeros(m.n), E, mat, tr)
+    np.savez(os.path.join(HERE, "fixture_2x2x2.npz"), indptr=m.csr["indptr"], indices=m.csr["indices"],
+             data_bc=info["data_bc"], res0=res0, b=info["b"], u=ud, bc_rows=rows)
+
+
+def c1_samples():
+    import scipy.sparse.linalg as spla
+    m, E, tr, rows, vals = cantilever_model((50, 5, 5), 2.0)
+    mat = hex8.Material("linear_elastic", nu=0.3)
+    _, info = krylov.linear_solve(m, mat, E, tr, rows, vals, tol=1e-12, atol=0.0)
+    ud = spla.spsolve(info["K"].tocsc(), info["b"])
+    rng = np.random.default_rng(SEED)
+    slots = np.sort(rng.choice(m.nse, 1000, replace=False))
+    dofs = np.sort(rng.choice(m.n, 64, replace=False))
+    np.savez(os.path.join(HERE, "c1_samples.npz"),
+             indptr_sha256=np.frombuffer(hashlib.sha256(m.csr["indptr"].astype(np.int32).tobytes()).digest(), np.uint8),
+             indices_sha256=np.frombuffer(hashlib.sha256(m.csr["indices"].astype(np.int32).tobytes()).digest(),
+                                          np.uint8),
+             nse=m.nse, slots=slots, values=info["data_bc"][slots], dofs=dofs, u=ud[dofs])
+
+
+def make_ke_3d():
+    pts, cells = structure.structured_grid((1, 1, 1))
+    m = hex8.Model(pts, cells, structure=False)
+    K = m.element_matrices(np.zeros(m.n), np.ones(1), hex8.Material("linear_elastic", nu=0.3))[0]
+    np.savez(os.path.join(HERE, "make_ke_3d.npz"), K=K)
+
+
+def simp_4x2x2():
+    import scipy.sparse.linalg as spla
+    pts, cells = structure.structured_grid((4, 2, 2))
+    faces = structure.boundary_faces(pts, cells, lambda p: np.isclose(p[0], 4.0))
+    m = hex8.Model(pts, cells, [faces])
+    rows, vals = structure.dirichlet_rows(pts, [(lambda p: np.isclose(p[0], 0.0), "all", 0.0)])
+    mat = hex8.Material("simp", nu=0.3, E0=1.0, Emin=1e-9, penal=3.0)
+    rho = np.random.default_rng(SEED).uniform(0.3, 1.0, m.E)
+    mask = np.zeros(m.n, bool)
+    mask[rows] = True
+    K = m.scipy_csr(hex8.apply_bc_csr(m.csr_values(np.zeros(m.n), rho, mat), m.csr, mask, rows))
+    tr = [hex8.constant_traction(faces, 0.25)]
+    f = -m.residual(np.zeros(m.n), rho, mat, tr)
+    f[rows] = 0.0
+    u = spla.spsolve(K.tocsc(), f)
+    np.savez(os.path.join(HERE, "simp_4x2x2.npz"), rho=rho, u=u,
+             compliance=adjoint.compliance(m, u, [faces], tr),
+             dc=adjoint.compliance_sens_closed_form(m, u, rho, mat))
+
+
+def c2_oracle():
+    t0 = time.time()
+    m, E, tr, rows, vals = cantilever_model((200, 40, 40), 1.0)
+    print(f"c2 model {time.time() - t0:.0f} s", flush=True)
+    mat = hex8.Material("linear_elastic", nu=0.3)
+    u, info = krylov.linear_solve(m, mat, E, tr, rows, vals, tol=1e-12, atol=0.0, maxiter=100000)
+    print(f"c2 solve {time.time() - t0:.0f} s, {info['iters']} iterations", flush=True)
+    relres = np.linalg.norm(info["K"] @ (u - 0.0) - info["b"]) / np.linalg.norm(info["b"])
+    dofs = np.sort(np.random.default_rng(SEED).choice(m.n, 4096, replace=False))
+    np.savez(os.path.join(HERE, "c2_oracle.npz"), dofs=dofs, u=u[dofs], u_max=np.abs(u).max(),
+             u_norm=np.linalg.norm(u), iters=info["iters"], relres=relres)
+
+
+def c3_oracle():
+    t0 = time.time()
+    dims = (120, 40, 20)
+    pts, cells = structure.structured_grid(dims)
+    faces = structure.boundary_faces(
+        pts, cells, lambda p: np.isclose(p[0], 120) & (np.abs(p[1] - 20) <= 1 + 1e-9) & (np.abs(p[2] - 10) <= 1 + 1e-9))
+    assert len(faces) == 4
+    m = hex8.Model(pts, cells, [faces])
+    rows, vals = structure.dirichlet_rows(pts, [(lambda p: np.isclose(p[0], 0.0), "all", 0.0)])
+    rho = np.random.default_rng(SEED).uniform(0.3, 1.0, m.E)
+    mat = hex8.Material("simp", nu=0.3, E0=1.0, Emin=1e-9, penal=3.0)
+    tr = [hex8.constant_traction(faces, 0.25)]
+    u, info = krylov.linear_solve(m, mat, rho, tr, rows, vals, tol=1e-12, atol=0.0, maxiter=100000)
+    print(f"c3 solve {time.time() - t0:.0f} s, {info['iters']} iterations", flush=True)
+    dc = adjoint.compliance_sens_closed_form(m, u, rho, mat)
+    cells_s = np.sort(np.random.default_rng(SEED).choice(m.E, 4096, replace=False))
+    np.savez(os.path.join(HERE, "c3_oracle.npz"), cells=cells_s, dc=dc[cells_s], dc_max=np.abs(dc).max(),
+             compliance=adjoint.compliance(m, u, [faces], tr), iters=info["iters"])
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--large", action="store_true")
+    ap.add_argument("--only", default=None)
+    a = ap.parse_args()
+    jobs = [fixture_2x2x2, c1_samples, make_ke_3d, simp_4x2x2] + ([c2_oracle, c3_oracle] if a.large else [])
+    for job in jobs:
+        if a.only and job.__name__ != a.only:
+            continue
+        t = time.time()
+        job()
+        print(f"{job.__name__}: {time.time() - t:.1f} s", flush=True)
