@@ -16,6 +16,7 @@ from .mesh import Mesh, StructuredGrid
 from .problem import MatrixView, Problem
 from .solution import Solution
 from .solver import create_solver
+from .solvers.gmg import GeometricMultigrid
 from .solvers.options import KrylovSolverOptions, MatrixProperty, NewtonOptions
 from .traced_params import TracedParams
 from .utils import zero_like_initial_guess

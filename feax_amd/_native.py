@@ -22,6 +22,7 @@ FX_PCG_DIFF3 = 2
 FX_PCG_TIME = 4
 FX_PCG_AUTOTUNE = 8
 FX_PCG_NO_GRAPH = 16
+FX_GMG_F32 = 32
 FX_ASM_DIFF3 = 1
 FX_ASM_COMPUTE32 = 2
 
@@ -30,6 +31,15 @@ class FxMaterial(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("param_loc", ctypes.c_int32), ("nu", ctypes.c_double),
                 ("E0", ctypes.c_double), ("Emin", ctypes.c_double), ("penal", ctypes.c_double),
                 ("kappa", ctypes.c_double)]
+
+
+class FxGmgLevel(ctypes.Structure):
+    """fx_gmg_level: one grid of the multigrid hierarchy."""
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("indptr", ctypes.c_void_p), ("bcol", ctypes.c_void_p), ("data", ctypes.c_void_p),
+                ("dinv", ctypes.c_void_p), ("bc_mask", ctypes.c_void_p), ("dense_inv", ctypes.c_void_p),
+                ("lmax", ctypes.c_double), ("lmin", ctypes.c_double), ("x", ctypes.c_void_p), ("b", ctypes.c_void_p),
+                ("r", ctypes.c_void_p), ("d", ctypes.c_void_p)]
 
 
 _P = ctypes.c_void_p
@@ -91,6 +101,14 @@ SIGNATURES = {
     "fx_hex8_traction_vjp_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P]),
     "fx_hex8_energy_workspace_bytes": (_SZ, [_I32]),
     "fx_hex8_energy_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ]),
+    "fx_gmg_eig_bounds_workspace_bytes": (_SZ, [_I32, _I32]),
+    "fx_gmg_eig_bounds_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _I32, _P, _P, _P, _SZ]),
+    "fx_gmg_vcycle": (_I32, [_P, _I32, _I32, _P, _I32, _I32, _F64, _P, _P]),
+    "fx_pcg_gmg_workspace_bytes": (_SZ, [_I32]),
+    "fx_pcg_gmg_f64": (_I32, [_P, _I32, _I32, _P, _I32, _I32, _F64, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P,
+                              _P, _P, _SZ]),
+    "fx_gmg_coarsen_f64": (_I32, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P]),
+    "fx_csr_to_dense_f64": (_I32, [_P, _I32, _P, _P, _P, _P]),
 }
 
 _lib = None

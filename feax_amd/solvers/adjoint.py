@@ -10,7 +10,8 @@ the linear solve (feax/solvers/linear.py:387-483):
             grad svar = -(dR/dsvar)^T lambda     fx_hex8_traction_vjp_f64
 
 The adjoint system reuses the forward's operator: the FP64 CSR (precision
-"fp64"), or the FP32 matrix inside an FP64 defect correction whose outer
+"fp64"; with its multigrid hierarchy under a GeometricMultigrid
+preconditioner), or the FP32 matrix inside an FP64 defect correction whose outer
 residual is the matrix-free volume residual (precision "mixed"), started
 from the Galerkin projection onto the forward's recorded Krylov directions
 when they are still held.  Initial guesses get no gradient, as in feax.
@@ -69,6 +70,10 @@ def adjoint_solve(problem, ts, tp, bc, v, keep, opts):
         from .common import pcg_matfree
         lam, info = pcg_matfree(problem, ts, tp, None, bc, rhs, None, opts.tol, opts.atol, opts.maxiter,
                                 opts.check_every)
+        keep["adjoint_iters"] = [info.iters]
+        return lam
+    if keep["form"] == "gmg":
+        lam, info = keep["gmg"].solve(rhs, None, opts.tol, opts.atol, opts.maxiter)
         keep["adjoint_iters"] = [info.iters]
         return lam
     if keep["form"] == "fp64":

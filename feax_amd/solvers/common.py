@@ -244,7 +244,8 @@ def create_iterative_solve_fn(options):
     if options.solver != "cg":
         raise NotImplementedError(f"{options.solver} is not on the HIP path (cg only)")
     if options.preconditioner is not None:
-        raise NotImplementedError("custom preconditioners are not on the HIP path (Jacobi only)")
+        raise NotImplementedError("custom preconditioners are not on the HIP path; GeometricMultigrid needs the "
+                                  "problem and runs through create_solver / create_linear_solver")
 
     def solve(A, b, x0):
         dinv = create_jacobi_preconditioner(A, options.jacobi_shift) if options.use_jacobi_preconditioner else None

@@ -7,6 +7,8 @@ computing u = u0 + du with J_bc du = -R_bc(u0) (linear.py:387-417):
 * precision "fp64": FP64 CSR assembly with the Dirichlet elimination fused
   into the store, FP64 right-hand side from the residual kernel, HIP
   Jacobi-PCG.
+* preconditioner GeometricMultigrid: FP64 CG preconditioned by a V-cycle
+  on the structured grid hierarchy (FP32 V-cycle with precision "mixed").
 * precision "mixed": FP32 CSR assembly and FP32 Jacobi-PCG (FP64 dot
   products) as the inner solver of an FP64 defect correction
   u <- u + K32^-1 (b - K u), where b - K u is the FP64 matrix-free residual
@@ -64,6 +66,32 @@ def solve_fp64(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None, keep=N
     du, info = pcg(J, b, x0, opts.tol, opts.atol, opts.maxiter, dinv, _b3(opts), opts.check_every)
     if keep is not None:
         keep.update(form="fp64", A=J, dinv=dinv)
+    if opts.check_convergence:
+        du = check_convergence(J, du, b, opts)
+    if stats is not None:
+        stats.iters, stats.rs, stats.outer = info.iters, info.rs, 1
+        stats.inner_iters = [info.iters]
+    u = torch.empty_like(u0)
+    nat.call("fx_axpby_f64", nat.stream(), u.numel(), 1.0, nat.ptr(u0), 1.0, nat.ptr(du), nat.ptr(u))
+    return u
+
+
+def solve_gmg(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None, keep=None):
+    """FP64 CSR solve preconditioned by a geometric-multigrid V-cycle
+    (KrylovSolverOptions.preconditioner = GeometricMultigrid); the V-cycle
+    runs in FP32 with precision "mixed", else in FP64."""
+    import torch
+    from .gmg import GMGHierarchy
+    if not symmetric:
+        raise NotImplementedError("GeometricMultigrid is built for symmetric elimination")
+    J, dinv = assemble_csr(problem, ts, tp, u0, bc, True, torch.float64, jacobi_shift=opts.jacobi_shift)
+    b = residual(problem, ts, tp, u0, bc, negate=True)
+    x0_fn = opts.x0_fn if opts.x0_fn is not None else create_x0(bc.bc_rows, bc.bc_vals)
+    H = GMGHierarchy(problem, ts, tp, u0, bc, J, dinv, opts.preconditioner, fp32=opts.precision != "fp64",
+                     jacobi_shift=opts.jacobi_shift)
+    du, info = H.solve(b, x0_fn(u0), opts.tol, opts.atol, opts.maxiter)
+    if keep is not None:
+        keep.update(form="gmg", A=J, dinv=dinv, gmg=H)
     if opts.check_convergence:
         du = check_convergence(J, du, b, opts)
     if stats is not None:
@@ -190,7 +218,13 @@ def create_linear_solver(problem, bc, solver_options=None, adjoint_solver_option
         adjoint_solver_options = solver_options
     else:
         adjoint_solver_options = resolve_iterative_solver(adjoint_solver_options, MatrixProperty.SPD)
-    create_iterative_solve_fn(solver_options)          # validates the options
+    from .gmg import GeometricMultigrid
+    use_gmg = isinstance(solver_options.preconditioner, GeometricMultigrid)
+    if use_gmg:
+        if solver_options.solver != "cg" or solver_options.operator != "csr":
+            raise NotImplementedError("GeometricMultigrid preconditions cg on the assembled operator")
+    else:
+        create_iterative_solve_fn(solver_options)      # validates the options
     default_bc = bc
     ts_default = traced_structure
     stats = SolveStats()
@@ -202,6 +236,8 @@ def create_linear_solver(problem, bc, solver_options=None, adjoint_solver_option
         if keep is not None:
             keep["ts"] = ts
         u0 = _dev_vec(ig, ts) if ig is not None else _zero_like(problem, eff_bc, ts)
+        if use_gmg:
+            return solve_gmg(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
         if solver_options.precision != "fp64" and solver_options.operator == "csr":
             return solve_mixed(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
         return solve_fp64(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)

@@ -347,6 +347,81 @@ int fx_hex8_energy_f64(void* stream, int32_t num_cells, const double* x, const d
                        const int32_t* cells, const fx_material* mat, const double* param, const double* u,
                        double* energy, double* dpsi_dmu, void* workspace, size_t ws_bytes);
 
+/* ------------------------------------------------------------------ */
+/* Geometric multigrid on a structured HEX8 box (StructuredGrid numbering,
+ * feax/spgrid.py:28-41).  The analogue of feax's NarrowBandCMG
+ * (feax/solvers/cmg.py:95-300: rediscretised coarse operators on cell
+ * averages, trilinear transfers, V-cycle preconditioned CG); here the
+ * smoother is a Chebyshev polynomial in D^-1 A and the coarsest level is a
+ * dense inverse (or a long Chebyshev polynomial).
+ *
+ * Level l has nx x ny x nz cells; level l+1 halves every direction or
+ * keeps some (nx_{l+1} = nx_l / 2 or nx_l, one direction at least
+ * halved).  Every level holds its BC-eliminated block-3 CSR (indptr, bcol
+ * of fx_csr_block3_index), dinv = 1 / diag, the constrained-DOF mask,
+ * lmax (largest eigenvalue of D^-1 A, fx_gmg_eig_bounds_f64; lmin, used
+ * by a Chebyshev coarsest solve only, the smallest or 0) and
+ * four vectors of 3 * nodes entries.  Values and vectors are double, or
+ * float with FX_GMG_F32.  Level 0's x and b are used only with FX_GMG_F32
+ * (the V-cycle's own copy of the CG residual and correction). */
+#define FX_GMG_F32 32
+typedef struct fx_gmg_level {
+  int32_t nx, ny, nz;
+  const int32_t* indptr;
+  const int32_t* bcol;
+  const void* data;
+  const void* dinv;
+  const uint8_t* bc_mask;   /* NULL: no constrained DOFs */
+  const void* dense_inv;    /* coarsest level: (3N)^2 row-major inverse, or NULL */
+  double lmax;
+  double lmin;              /* coarsest Chebyshev solve: lower bound (0: lmax / cheb_ratio) */
+  void* x;                  /* correction */
+  void* b;                  /* right-hand side */
+  void* r;                  /* scratch */
+  void* d;                  /* scratch */
+} fx_gmg_level;
+
+/* Bounds of the spectrum of D^-1 A (A: block-3 CSR of num_nodes nodes,
+ * SPD; D^-1 = dinv): `steps` Lanczos steps carried out as Jacobi-PCG on
+ * A x = v from a fixed v; *lmin, *lmax = extreme eigenvalues of the
+ * Lanczos tridiagonal (inside the spectrum, converging to its ends).
+ * Synchronises the stream. */
+size_t fx_gmg_eig_bounds_workspace_bytes(int32_t num_nodes, int32_t steps);
+int fx_gmg_eig_bounds_f64(void* stream, int32_t num_nodes, const int32_t* indptr, const int32_t* bcol,
+                          const double* data, const double* dinv, int32_t steps, double* lmin, double* lmax,
+                          void* workspace, size_t ws_bytes);
+
+/* z = V(r): one symmetric V-cycle over nlev levels (Chebyshev of degree
+ * smooth_degree on [lmax / cheb_ratio, 1.1 lmax] before and after the
+ * coarse correction; coarsest: dense_inv, else Chebyshev of coarse_degree).
+ * r, z: double (3N of level 0). */
+int fx_gmg_vcycle(void* stream, int32_t flags, int32_t nlev, const fx_gmg_level* levels, int32_t smooth_degree,
+                  int32_t coarse_degree, double cheb_ratio, const double* r, double* z);
+
+/* CG preconditioned by the V-cycle, with the stopping rule and iteration of
+ * jax.scipy.sparse.linalg.cg (the solve of feax/solvers/common.py:229-239;
+ * MGPCG of feax/solvers/cmg.py:276-300).  The CG operator is the FP64 CSR
+ * (levels[0].indptr, levels[0].bcol, data); x0 may be NULL (zeros).
+ * Iterations run as a HIP graph of check_every iterations unless
+ * FX_PCG_NO_GRAPH.  workspace >= fx_pcg_gmg_workspace_bytes(n). */
+size_t fx_pcg_gmg_workspace_bytes(int32_t n);
+int fx_pcg_gmg_f64(void* stream, int32_t flags, int32_t nlev, const fx_gmg_level* levels, int32_t smooth_degree,
+                   int32_t coarse_degree, double cheb_ratio, const double* data, const double* b,
+                   const double* x0, double tol, double atol, int32_t maxiter, int32_t check_every, double* x,
+                   int32_t* iters, double* rs, void* workspace, size_t ws_bytes);
+
+/* Coarse-grid copy of a fine-grid field for rediscretisation: loc
+ * FX_PARAM_NODE -> value at the same node (ncomp values per node: a node
+ * parameter, or ncomp = 3 for a displacement); FX_PARAM_CELL -> mean over
+ * the child cells (the density averaging of feax/solvers/cmg.py:375-386,
+ * 426-431).  Grids as for fx_gmg_level. */
+int fx_gmg_coarsen_f64(void* stream, int32_t loc, int32_t ncomp, int32_t nxf, int32_t nyf, int32_t nzf, int32_t nxc,
+                       int32_t nyc, int32_t nzc, const double* fine, double* coarse);
+
+/* dense[i * n + j] = A[i, j] of a CSR matrix (double), zeros elsewhere. */
+int fx_csr_to_dense_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices,
+                        const double* data, double* dense);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,3 +81,55 @@ def test_simp_fixture_gradient():
     np.testing.assert_allclose(rho.grad.cpu().numpy(), f["dc"], rtol=1e-9, atol=1e-12 * np.abs(f["dc"]).max())
     _, dc = compliance_and_sensitivity(p, fe.TracedParams(volume_vars=(f["rho"],)), u.detach(), 0.25)
     np.testing.assert_allclose(dc.cpu().numpy(), f["dc"], rtol=1e-9, atol=1e-12 * np.abs(f["dc"]).max())
+
+
+@pytest.mark.parametrize("pre", ["jacobi", "gmg-fp64", "gmg-mixed"])
+def test_c2_full_size_solution_matches_oracle_samples(pre):
+    """200x40x40 (1,013,643 DOFs): sampled DOFs of the GPU solve vs the
+    oracle's Jacobi-PCG to rel 1e-12 (c2_oracle.npz)."""
+    import feax_amd as fe
+    f = _load("c2_oracle.npz")
+    p, bc, tp = cantilever((200, 40, 40), (1.0, 1.0, 1.0), 200.0)
+    mg = fe.GeometricMultigrid() if pre.startswith("gmg") else None
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-11, atol=0.0, maxiter=20000, preconditioner=mg,
+                                  use_jacobi_preconditioner=mg is None,
+                                  precision="mixed" if pre == "gmg-mixed" else "fp64")
+    solver = fe.create_solver(p, bc, opts, linear=True, return_solution=False)
+    u = solver(tp).cpu().numpy()
+    err = np.abs(u[f["dofs"]] - f["u"]).max() / float(f["u_max"])
+    print(pre, "iters", solver.stats.iters, "max sampled err / max|u|", err)
+    assert err < 1e-8
+    assert abs(np.abs(u).max() - float(f["u_max"])) <= 1e-8 * float(f["u_max"])
+
+
+def test_c3_full_size_compliance_sensitivity_matches_oracle_samples():
+    """120x40x20 SIMP cantilever: compliance and the fused dC/drho kernel
+    after a GMG solve vs the oracle's closed form (c3_oracle.npz)."""
+    import torch
+    import feax_amd as fe
+    from feax_amd import gene
+    f = _load("c3_oracle.npz")
+    dims = (120, 40, 20)
+
+    class Simp(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.SIMPLinearElastic(nu=0.3, E0=1.0, Emin=1e-9, penal=3.0)
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.0, 0.0, 1.0))]
+
+    patch = [lambda x: np.isclose(x[0], 120) & (np.abs(x[1] - 20) <= 1 + 1e-9) & (np.abs(x[2] - 10) <= 1 + 1e-9)]
+    prob = Simp(fe.StructuredGrid(dims).to_mesh(), vec=3, dim=3, location_fns=patch)
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda x: np.isclose(x[0], 0.0), "all", 0.0)]).create_bc(prob)
+    rho = np.random.default_rng(20261203).uniform(0.3, 1.0, prob.num_cells)
+    t = fe.TracedParams.create_uniform_surface_var(prob, 0.25)
+    tp = fe.TracedParams(volume_vars=(torch.as_tensor(rho, device="cuda"),), surface_vars=[(t,)])
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-11, atol=0.0, preconditioner=fe.GeometricMultigrid())
+    solver = fe.create_solver(prob, bc, opts, linear=True, return_solution=False)
+    u = solver(tp)
+    C, dc = gene.compliance_and_sensitivity(prob, tp, u, surface_load_params=0.25)
+    dc = dc.cpu().numpy()
+    err = np.abs(dc[f["cells"]] - f["dc"]).max() / float(f["dc_max"])
+    print("c3 iters", solver.stats.iters, "dc err", err, "C", float(C), float(f["compliance"]))
+    assert err < 1e-7
+    assert abs(float(C) - float(f["compliance"])) <= 1e-9 * abs(float(f["compliance"]))

@@ -193,3 +193,17 @@ def test_transpose_maps_and_volume_slots():
     np.testing.assert_array_equal(data[p.csr_T_perm], At.data)
     m = hex8.Model(p.points, p.cells.astype(np.int64), p.boundary_inds_list)
     np.testing.assert_array_equal(p.csr_volume_slots, m.csr["volume_slots"])
+
+
+def test_structured_dims_detects_box_grids_only():
+    import feax_amd as fe
+    from feax_amd.solvers.gmg import coarse_dims, structured_dims
+    prob = fe.Problem(fe.StructuredGrid((6, 4, 5), spacing=(0.5, 1.0, 2.0)).to_mesh(), vec=3, dim=3)
+    dims, axes = structured_dims(prob)
+    assert dims == (6, 4, 5) and np.allclose(axes[2], 2.0 * np.arange(6))
+    assert coarse_dims((6, 4, 5), 2) == (3, 2, 5) and coarse_dims((3, 2, 5), 2) is None
+    pts = prob.points[::-1].copy()
+    cells = len(pts) - 1 - prob.cells
+    prob2 = fe.Problem(fe.Mesh(pts, cells), vec=3, dim=3)
+    with pytest.raises(NotImplementedError, match="structured box"):
+        structured_dims(prob2)
