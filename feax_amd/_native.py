@@ -80,11 +80,13 @@ SIGNATURES = {
     "fx_axpby_f64": (_I32, [_P, _I32, _F64, _P, _F64, _P, _P]),
     "fx_bc_lift_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P]),
     "fx_hex8_assemble_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                                    _I32, _I32, _F64, _P, _P]),
+                                    _I32, _I32, _F64, _P, _P, _P, _SZ]),
     "fx_hex8_assemble_f32": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                                    _I32, _I32, _F64, _P, _P]),
+                                    _I32, _I32, _F64, _P, _P, _P, _SZ]),
+    "fx_hex8_assemble_workspace_bytes": (_SZ, [_I32, _I32, _I32]),
+    "fx_hex8_residual_workspace_bytes": (_SZ, [_I32]),
     "fx_hex8_residual_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P,
-                                    _P, _P, _I32, _P, _P, _I32, _P]),
+                                    _P, _P, _I32, _P, _P, _I32, _P, _P, _SZ]),
     "fx_hex8_jvp_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "fx_pcg_matfree_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P, _F64,
                                   _F64, _I32, _I32, _P, _P, _P, _P, _SZ]),

@@ -140,6 +140,27 @@ def surface_values(problem, traced_params, ts):
     return torch.cat(parts).reshape(-1).contiguous()
 
 
+# Element workspace of the two-pass assembly / residual (576 or 24 values
+# per element), one growing buffer per device; above the cap the kernels
+# run single-pass with no workspace.
+ELEM_WORKSPACE_CAP = 4 << 30
+_ELEM_WS = {}
+
+
+def elem_workspace(ts, nbytes):
+    """(pointer, size) of the element workspace, or (None, 0) above the cap."""
+    import torch
+    if nbytes <= 0 or nbytes > ELEM_WORKSPACE_CAP:
+        return None, 0
+    key = str(ts.device)
+    ws = _ELEM_WS.get(key)
+    if ws is None or ws.numel() < nbytes:
+        _ELEM_WS.pop(key, None)
+        ws = torch.empty(int(nbytes), dtype=torch.uint8, device=ts.device)
+        _ELEM_WS[key] = ws
+    return ws.data_ptr(), ws.numel()
+
+
 def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, dtype=None, diff_form=False,
                  jacobi_shift=None):
     """BC-eliminated CSR Jacobian on the device (fx_hex8_assemble_*).
@@ -156,10 +177,12 @@ def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, 
     u = _as_dev(sol, ts) if sol is not None else None
     dinv = torch.empty(ts.n, dtype=torch.float64, device=ts.device) if jacobi_shift is not None else None
     name = "fx_hex8_assemble_f64" if dtype == torch.float64 else "fx_hex8_assemble_f32"
+    flags = nat.FX_ASM_DIFF3 if diff_form else 0
+    wp, wn = elem_workspace(ts, nat.lib().fx_hex8_assemble_workspace_bytes(ts.num_cells, data.element_size(), flags))
     nat.call(name, nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
              nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), nat.ptr(ts.rank), nat.ptr(ts.indptr),
              nat.ptr(ts.indices), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u), nat.ptr(mask), int(bool(symmetric)),
-             nat.FX_ASM_DIFF3 if diff_form else 0, float(jacobi_shift or 0.0), nat.ptr(dinv), nat.ptr(data))
+             flags, float(jacobi_shift or 0.0), nat.ptr(dinv), nat.ptr(data), wp, wn)
     A = CSRMatrix(data, ts.indptr, ts.indices, (ts.n, ts.n), block3=True, form="diff3" if diff_form else "csr",
                   bcol=ts.bcol())
     return A if jacobi_shift is None else (A, dinv)
@@ -183,7 +206,8 @@ def residual(problem, ts, traced_params, sol, bc=None, negate=False, out=None):
              nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u),
              nf, nat.ptr(ts.faces) if nf else None, nat.ptr(ts.face_inc_ptr) if nf else None,
              nat.ptr(ts.face_inc) if nf else None, nat.ptr(sv), nat.ptr(ts.fdir) if nf else None, nbc,
-             nat.ptr(rows), nat.ptr(vals), int(bool(negate)), nat.ptr(res))
+             nat.ptr(rows), nat.ptr(vals), int(bool(negate)), nat.ptr(res),
+             *elem_workspace(ts, nat.lib().fx_hex8_residual_workspace_bytes(ts.num_cells)))
     return res
 
 
@@ -206,7 +230,7 @@ def volume_residual(problem, ts, traced_params, sol, bc=None, negate=False, out=
     nat.call("fx_hex8_residual_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
              nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat),
              nat.ptr(p), nat.ptr(u), 0, None, None, None, None, None, nbc, nat.ptr(rows), nat.ptr(vals),
-             int(bool(negate)), nat.ptr(res))
+             int(bool(negate)), nat.ptr(res), *elem_workspace(ts, nat.lib().fx_hex8_residual_workspace_bytes(ts.num_cells)))
     return res
 
 

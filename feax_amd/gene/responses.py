@@ -45,7 +45,7 @@ def _external_work_vector(problem, ts, params):
     nat.call("fx_hex8_residual_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
              nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat),
              nat.ptr(ones), nat.ptr(zero), ts.num_faces, nat.ptr(ts.faces), nat.ptr(ts.face_inc_ptr),
-             nat.ptr(ts.face_inc), nat.ptr(svar), nat.ptr(ts.fdir), 0, None, None, 1, nat.ptr(g))
+             nat.ptr(ts.face_inc), nat.ptr(svar), nat.ptr(ts.fdir), 0, None, None, 1, nat.ptr(g), None, 0)
     return g
 
 

@@ -190,6 +190,15 @@ int fx_bc_lift_f64(void* stream, int32_t n, int32_t nbc, const int32_t* bc_rows,
  * summation order of feax/assembler.py:1253-1318): no float atomics,
  * bitwise repeatable.
  *
+ * Assembly and residual take an optional workspace.  With one
+ * (fx_hex8_assemble_workspace_bytes / fx_hex8_residual_workspace_bytes
+ * bytes) they run in two passes: one wave per element computes the element
+ * matrix (or residual) once into the workspace, then one wave (thread) per
+ * node gathers it in the same order.  workspace == NULL computes every
+ * element once per node instead (no scratch memory).  Both ways do the
+ * same arithmetic in the same order (equal up to the rounding of a fused
+ * multiply-add the compiler may form in one and not the other). 
+ *
  * x, y, z: (d) node coordinates (SoA, FP64).  cells: (d) E*8 int32.
  * param: (d) N or E values (mat->param_loc).  u: (d) 3N FP64 or NULL.
  * bc_mask: (d) 3N uint8 or NULL.
@@ -210,13 +219,16 @@ int fx_hex8_assemble_f64(void* stream, int32_t num_nodes, int32_t num_cells, con
                          const int32_t* inc, const uint8_t* elem_rank, const int32_t* indptr,
                          const int32_t* indices, const fx_material* mat, const double* param,
                          const double* u, const uint8_t* bc_mask, int32_t symmetric, int32_t flags,
-                         double jacobi_shift, double* dinv, double* data);
+                         double jacobi_shift, double* dinv, double* data, void* workspace, size_t ws_bytes);
 int fx_hex8_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, const double* x,
                          const double* y, const double* z, const int32_t* cells, const int32_t* inc_ptr,
                          const int32_t* inc, const uint8_t* elem_rank, const int32_t* indptr,
                          const int32_t* indices, const fx_material* mat, const double* param,
                          const double* u, const uint8_t* bc_mask, int32_t symmetric, int32_t flags,
-                         double jacobi_shift, double* dinv, float* data);
+                         double jacobi_shift, double* dinv, float* data, void* workspace, size_t ws_bytes);
+/* 576 element-matrix values per element: value_bytes 8 (f64) or 4 (f32;
+ * FP32 workspace only with FX_ASM_COMPUTE32). */
+size_t fx_hex8_assemble_workspace_bytes(int32_t num_cells, int32_t value_bytes, int32_t flags);
 
 /* Global residual R(u) = volume + traction (FP64): feax get_res
  * (feax/assembler.py:1618-1667).  Surface part: the loaded faces of all
@@ -232,7 +244,9 @@ int fx_hex8_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, con
                          const int32_t* inc, const fx_material* mat, const double* param, const double* u,
                          int32_t num_faces, const int32_t* faces, const int32_t* finc_ptr,
                          const int32_t* finc, const double* svar, const double* fdir, int32_t num_bc,
-                         const int32_t* bc_rows, const double* bc_vals, int32_t negate, double* res);
+                         const int32_t* bc_rows, const double* bc_vals, int32_t negate, double* res,
+                         void* workspace, size_t ws_bytes);
+size_t fx_hex8_residual_workspace_bytes(int32_t num_cells);
 
 /* Matrix-free operator out = J_bc(u) v (FP64): feax's Krylov operator
  * create_matfree_res_J_parametric (feax/assembler.py:1747-1778),

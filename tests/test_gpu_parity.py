@@ -291,3 +291,59 @@ def test_matfree_gradient_matches_csr_gradient():
         grads[op] = E.grad.detach().cpu().numpy()
     g, ref = grads["matfree"], grads["csr"]
     assert np.abs(g - ref).max() <= 1e-7 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("kind,loc", [("linear_elastic", "node"), ("simp", "cell"), ("neo_hookean", "cell")])
+def test_two_pass_and_single_pass_element_kernels_agree(kind, loc, monkeypatch):
+    """Element-workspace (two-pass) assembly and residual vs the per-node
+    single-pass kernels: the same arithmetic in the same order, so CSR values
+    (FP64, FP32, difference form), dinv and the residual agree to the
+    rounding of an operation (the compiler may contract a multiply-add
+    differently in the two kernels)."""
+    import torch
+    import feax_amd as fe
+    import feax_amd.assembler as asm
+    rng = np.random.default_rng(21)
+    laws = {"linear_elastic": fe.materials.IsotropicLinearElastic(nu=0.3),
+            "simp": fe.materials.SIMPLinearElastic(nu=0.3, E0=1.0, Emin=1e-3, penal=3.0),
+            "neo_hookean": fe.materials.CompressibleNeoHookean(kappa=10.0)}
+
+    class P(fe.Problem):
+        def get_tensor_map(self):
+            return laws[kind]
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.2, -0.1, 1.0))]
+
+    mesh = fe.StructuredGrid((7, 4, 5), spacing=(1.0, 0.9, 1.1)).to_mesh()
+    pts = mesh.points + rng.uniform(-0.12, 0.12, mesh.points.shape)
+    prob = P(fe.Mesh(pts, mesh.cells), vec=3, dim=3, location_fns=[lambda p: p[0] > 7.0 - 0.2])
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: p[0] < 0.2, "all", 0.0)]).create_bc(prob)
+    n = prob.num_total_dofs_all_vars
+    size = prob.num_total_nodes if loc == "node" else prob.num_cells
+    t = fe.TracedParams.create_uniform_surface_var(prob, 0.3)
+    tp = fe.TracedParams(volume_vars=(torch.as_tensor(rng.uniform(0.5, 1.5, size), device="cuda"),),
+                         surface_vars=[(t,)])
+    ts = fe.TracedStructure.from_problem(prob)
+    u = torch.as_tensor((0.03 if kind == "neo_hookean" else 1.0) * rng.standard_normal(n), device="cuda")
+
+    def run():
+        out = []
+        for dtype, diff in ((torch.float64, False), (torch.float32, False), (torch.float32, True)):
+            A, dinv = asm.assemble_csr(prob, ts, tp, u, bc, True, dtype, diff_form=diff, jacobi_shift=1e-12)
+            out.append((A.data.double().cpu().numpy(), dinv.cpu().numpy()))
+        res = asm.residual(prob, ts, tp, u, bc, negate=True).cpu().numpy()
+        return out, res
+
+    two, res2 = run()
+    monkeypatch.setattr(asm, "ELEM_WORKSPACE_CAP", 0)
+    one, res1 = run()
+    eps = np.finfo(np.float64).eps
+    print(kind, "residual entries differing", int((res1 != res2).sum()), "of", res1.size)
+    assert np.abs(res2 - res1).max() <= 8 * eps * np.abs(res1).max()
+    indptr = prob.csr_indptr
+    for (d2, i2), (d1, i1), ulp in zip(two, one, (eps, 2 ** -23, 2 ** -23)):
+        rowmax = np.repeat(np.maximum.reduceat(np.abs(d1), indptr[:-1]), np.diff(indptr))
+        print(kind, "csr entries differing", int((d2 != d1).sum()), "of", d1.size)
+        assert np.all(np.abs(d2 - d1) <= 8 * ulp * rowmax)
+        assert np.allclose(i2, i1, rtol=8 * eps, atol=0)
