@@ -9,7 +9,10 @@ Jacobi-PCG solve, structure and parameters already resident in HBM
 (the timing scope of examples/benchmark/bench_linear_elasticity.py:240-250
 after TracedStructure.from_problem).  Forms timed: "fp64" (FP64 assembly,
 SpMV, CG) and "mixed" (FP32 difference-form matrix, FP64 vectors, FP64
-defect correction).
+defect correction) -- the headline -- and, beside them, "gmg-fp64" /
+"gmg-mixed": the same step with CG preconditioned by the geometric-multigrid
+V-cycle (FP64 / FP32 V-cycle; hierarchy built inside the step).  Each form
+reports its error against the FP64 oracle's solution.
 
 Prints one JSON line (see the fields below); --gpus N > 1 is refused: this
 build runs on one GPU.
@@ -456,6 +459,7 @@ def main():
     ap.add_argument("--no-grad", action="store_true", help="skip the SIMP compliance-gradient workload")
     ap.add_argument("--no-newton", action="store_true", help="skip the Neo-Hookean Newton workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances workload")
+    ap.add_argument("--no-gmg", action="store_true", help="skip the multigrid-preconditioned forms")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch only the PCG SpMV kernels and a 1 GiB copy (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -481,17 +485,28 @@ def main():
     n, nse, Ecount, N = ts.n, ts.nse, ts.num_cells, ts.num_nodes
     u0 = fe.zero_like_initial_guess(problem, bc, device="cuda")
 
-    def solver_for(precision):
+    def solver_for(form):
+        gmg = form.startswith("gmg-")
+        precision = form[4:] if gmg else form
         opts = fe.KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=20000,
-                                      use_jacobi_preconditioner=True, precision=precision)
+                                      use_jacobi_preconditioner=not gmg, precision=precision,
+                                      preconditioner=fe.GeometricMultigrid() if gmg else None)
         return fe.create_solver(problem, bc, opts, linear=True, traced_params=tp, traced_structure=ts,
                                 return_solution=False)
+
+    # FP64 oracle (tests/golden/c2_oracle.npz: the oracle's Jacobi-PCG to rel
+    # 1e-12 on this workload, u at 4096 seeded DOFs): error of each form.
+    oracle_fix = None
+    fix_path = os.path.join(ROOT, "tests", "golden", "c2_oracle.npz")
+    if args.config == "c2" and os.path.exists(fix_path):
+        oracle_fix = np.load(fix_path, allow_pickle=False)
 
     results = {}
     b0 = residual(problem, ts, tp, u0, bc, negate=True)
     bnorm = device_norm(b0)
     spmv_live = {}
-    for form in ("fp64", "mixed"):
+    forms = ("fp64", "mixed") + (() if args.no_gmg else ("gmg-fp64", "gmg-mixed"))
+    for form in forms:
         solver = solver_for(form)
         for _ in range(args.warmup):
             solver(tp, u0)
@@ -508,10 +523,14 @@ def main():
             torch.cuda.synchronize()
         spmv_live.update({k: (v[0] / v[1], v[1]) for k, v in timer.kernels.items() if v[1]})
         r = residual(problem, ts, tp, u, bc, negate=True)
+        err = None
+        if oracle_fix is not None:
+            us = u[torch.as_tensor(oracle_fix["dofs"], device=u.device)].cpu().numpy()
+            err = float(np.abs(us - oracle_fix["u"]).max() / float(oracle_fix["u_max"]))
         results[form] = dict(s=el, iters=solver.stats.iters, outer=solver.stats.outer,
-                             inner=solver.stats.inner_iters, relres=device_norm(r) / bnorm, u=u)
+                             inner=solver.stats.inner_iters, relres=device_norm(r) / bnorm, u=u, oracle_err=err)
         log(f"{form}: {el:.4f} s/step  iters={solver.stats.iters} outer={solver.stats.outer} "
-            f"true relres={results[form]['relres']:.3e}")
+            f"true relres={results[form]['relres']:.3e} oracle err={err}")
     du = results["mixed"]["u"] - results["fp64"]["u"]
     mixed_vs_fp64 = float(du.norm() / results["fp64"]["u"].norm())
 
@@ -567,7 +586,8 @@ def main():
                                f"h={cfg['spacing']}, {n} DOF, nse={nse}, Jacobi-PCG rel 1e-8",
                    "form": headline},
         "forms": {k: {"s": round(v["s"], 6), "iters": v["iters"], "outer": v["outer"],
-                      "true_relres": v["relres"]} for k, v in results.items()},
+                      "true_relres": v["relres"], "oracle_err": v["oracle_err"]} for k, v in results.items()},
+        "oracle_err_def": "max over 4096 seeded DOFs |u - u_oracle| / max|u_oracle| (FP64 oracle, rel 1e-12)",
         "mixed_vs_fp64_rel_diff": mixed_vs_fp64,
         "spmv": {"fp64_b3_us": round(t_spmv64 * 1e6, 2), "fp64_alg_GBps": round(B64 / t_spmv64 / 1e9, 1),
                  "diff3_us": round(t_diff * 1e6, 2), "diff3_alg_GBps": round(Bd / t_diff / 1e9, 1),
