@@ -16,7 +16,7 @@ multigrid CG for structured bands, NarrowBandCMG (feax/solvers/cmg.py:
   and the constrained DOFs injected from the fine BC mask.
 * smoother: Chebyshev of ``smooth_degree`` in D^-1 A on
   [lmax / cheb_ratio, 1.1 lmax], lmax from ``lmax_iters`` Lanczos steps.
-* coarsest: dense inverse (Cholesky of the coarsest CSR, at most
+* coarsest: dense inverse (rocSOLVER Cholesky of the coarsest CSR, at most
   ``max_dense_dofs`` DOFs), else Chebyshev of ``coarse_degree`` on
   [lmin, 1.1 lmax] (both Lanczos estimates).
 * KrylovSolverOptions.precision "fp64" runs the V-cycle in FP64, "mixed"
@@ -176,6 +176,18 @@ def _workspace(nbytes, device):
     return ws
 
 
+def _dense_inverse(A, dev):
+    """A^-1 (FP64, dense) of the coarsest SPD operator: the CSR scattered to
+    a dense matrix on the device, Cholesky factor and inverse from
+    rocSOLVER (torch.linalg)."""
+    import torch
+    n = A.shape[0]
+    dense = torch.empty((n, n), dtype=torch.float64, device=dev)
+    nat.call("fx_csr_to_dense_f64", nat.stream(), n, nat.ptr(A.indptr), nat.ptr(A.indices), nat.ptr(A.data),
+             nat.ptr(dense))
+    return torch.cholesky_inverse(torch.linalg.cholesky(dense))
+
+
 class GMGHierarchy:
     """The values of one hierarchy: per-level CSR (FP64, and FP32 copies for
     an FP32 V-cycle), dinv, masks, lmax, vectors, dense coarsest inverse."""
@@ -241,11 +253,7 @@ class GMGHierarchy:
         nc = S.ts[last].n
         self.dense = None
         if mg.coarse == "dense" and nc <= mg.max_dense_dofs:
-            Ac = self.A64[last]
-            dense = torch.empty((nc, nc), dtype=torch.float64, device=dev)
-            nat.call("fx_csr_to_dense_f64", stream, nc, nat.ptr(Ac.indptr), nat.ptr(Ac.indices),
-                     nat.ptr(Ac.data), nat.ptr(dense))
-            inv = torch.cholesky_inverse(torch.linalg.cholesky(dense))
+            inv = _dense_inverse(self.A64[last], dev)
             if self.fp32:
                 inv32 = torch.empty((nc, nc), dtype=torch.float32, device=dev)
                 nat.call("fx_convert_f64_to_f32", stream, nc * nc, 1.0, nat.ptr(inv), nat.ptr(inv32))
