@@ -31,6 +31,7 @@ def _load_params(problem, surface_load_params):
 def _external_work_vector(problem, ts, params):
     """g with C = g . u: minus the traction residual of the loads `params`."""
     import torch
+    from ..assembler import elem_workspace
     parts = []
     for i, p in enumerate(params):
         nf = int(ts.face_offsets[i + 1] - ts.face_offsets[i])
@@ -45,7 +46,8 @@ def _external_work_vector(problem, ts, params):
     nat.call("fx_hex8_residual_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
              nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat),
              nat.ptr(ones), nat.ptr(zero), ts.num_faces, nat.ptr(ts.faces), nat.ptr(ts.face_inc_ptr),
-             nat.ptr(ts.face_inc), nat.ptr(svar), nat.ptr(ts.fdir), 0, None, None, 1, nat.ptr(g), None, 0)
+             nat.ptr(ts.face_inc), nat.ptr(svar), nat.ptr(ts.fdir), 0, None, None, 1, nat.ptr(g),
+             *elem_workspace(ts, nat.lib().fx_hex8_residual_workspace_bytes(ts.num_cells)))
     return g
 
 
