@@ -103,6 +103,9 @@ SIGNATURES = {
     "fx_hex8_traction_vjp_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P]),
     "fx_hex8_energy_workspace_bytes": (_SZ, [_I32]),
     "fx_hex8_energy_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ]),
+    "fx_bicgstab_workspace_bytes": (_SZ, [_I32]),
+    "fx_bicgstab_f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P, _P,
+                               _P, _SZ]),
     "fx_gmg_eig_bounds_workspace_bytes": (_SZ, [_I32, _I32]),
     "fx_gmg_eig_bounds_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _I32, _P, _P, _P, _SZ]),
     "fx_gmg_vcycle": (_I32, [_P, _I32, _I32, _P, _I32, _I32, _F64, _P, _P]),

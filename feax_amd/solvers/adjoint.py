@@ -61,7 +61,8 @@ def adjoint_solve(problem, ts, tp, bc, v, keep, opts):
     """lambda = J_bc^-1 v with lambda[bc] = 0, on the forward's operator."""
     import torch
     from ..assembler import volume_residual
-    from .common import device_norm, pcg
+    from ..csr import transpose_with_maps
+    from .common import device_norm, krylov, pcg
     rhs = v.detach().to(device=ts.device, dtype=torch.float64).reshape(-1).clone()
     if bc is not None and len(bc.bc_rows):
         rhs.index_fill_(0, bc.device(ts.device)[0].long(), 0.0)
@@ -77,7 +78,15 @@ def adjoint_solve(problem, ts, tp, bc, v, keep, opts):
         keep["adjoint_iters"] = [info.iters]
         return lam
     if keep["form"] == "fp64":
-        lam, info = pcg(A, rhs, None, opts.tol, opts.atol, opts.maxiter, dinv, None, opts.check_every)
+        if opts.solver == "bicgstab":
+            # J_bc^T: the matrix itself under symmetric elimination, else its
+            # transpose through the precomputed maps (feax/solvers/linear.py:
+            # 442-456, feax/csr.py:130-141).
+            At = A if keep.get("symmetric", True) else transpose_with_maps(
+                A, problem.csr_T_perm, problem.csr_T_indptr, problem.csr_T_indices)
+            lam, info = krylov(At, rhs, None, opts, dinv)
+        else:
+            lam, info = pcg(A, rhs, None, opts.tol, opts.atol, opts.maxiter, dinv, None, opts.check_every)
         keep["adjoint_iters"] = [info.iters]
         return lam
     # FP32 operator in an FP64 defect correction.

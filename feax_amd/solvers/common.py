@@ -2,7 +2,9 @@
 
 create_iterative_solve_fn(options) -> solve(A, b, x0) runs the HIP
 Jacobi-PCG (fx_pcg_f64 / fx_pcg_f32) with jax.scipy.sparse.linalg.cg
-stopping semantics (feax/solvers/common.py:213-239).  Unlike feax's
+stopping semantics (feax/solvers/common.py:213-239), or BiCGSTAB
+(fx_bicgstab_f64, jax bicgstab semantics, common.py:241-252) for
+solver="bicgstab".  Unlike feax's
 matrix-free default, ``use_jacobi_preconditioner`` is honoured (the
 operator here is the assembled CSR).
 """
@@ -116,6 +118,47 @@ def pcg(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=Non
         form = "diff3" if flags & nat.FX_PCG_DIFF3 else ("block3" if flags & nat.FX_PCG_BLOCK3 else "csr")
         timer.add(f"{name[len('fx_pcg_'):]}/{form}", rs[1], rs[2])
     return x, PCGInfo(iters.value, rs[0])
+
+
+_BCG_WS = {}
+
+
+def bicgstab(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=None, block3=None,
+             check_every=16, out=None):
+    """BiCGSTAB on the device (fx_bicgstab_f64, FP64) with the iteration and
+    stopping rule of jax.scipy.sparse.linalg.bicgstab (feax/solvers/
+    common.py:241-252); M = Jacobi (dinv) or none.  Returns (x, PCGInfo)."""
+    import torch
+    if A.form != "csr" or A.data.dtype != torch.float64:
+        raise NotImplementedError("bicgstab runs on an FP64 CSR operator")
+    n = A.shape[0]
+    b = b.to(torch.float64).contiguous()
+    x = torch.empty(n, dtype=torch.float64, device=A.data.device) if out is None else out
+    x0 = None if x0 is None else x0.to(torch.float64).contiguous()
+    dinv = None if dinv is None else dinv.to(torch.float64).contiguous()
+    use_b3 = A.block3 if block3 is None else (bool(block3) and A.block3)
+    flags = nat.FX_PCG_BLOCK3 if use_b3 else 0
+    need = int(nat.lib().fx_bicgstab_workspace_bytes(n))
+    key = str(A.data.device)
+    ws = _BCG_WS.get(key)
+    if ws is None or ws.numel() < need:
+        ws = torch.empty(need, dtype=torch.uint8, device=A.data.device)
+        _BCG_WS[key] = ws
+    iters = ctypes.c_int32(0)
+    rs = ctypes.c_double(0.0)
+    nat.call("fx_bicgstab_f64", nat.stream(), n, A.nse, flags, nat.ptr(A.indptr), nat.ptr(A.indices),
+             nat.ptr(A.bcol if use_b3 else None), nat.ptr(A.data), nat.ptr(dinv), nat.ptr(b), nat.ptr(x0),
+             float(tol), float(atol), int(maxiter), int(check_every), nat.ptr(x), ctypes.byref(iters),
+             ctypes.byref(rs), nat.ptr(ws), ws.numel())
+    return x, PCGInfo(iters.value, rs.value)
+
+
+def krylov(A, b, x0, opts, dinv=None, block3=None, out=None):
+    """The Krylov method opts.solver names ("cg" or "bicgstab") on A."""
+    if opts.solver == "bicgstab":
+        return bicgstab(A, b, x0, opts.tol, opts.atol, opts.maxiter, dinv, block3, min(opts.check_every, 16),
+                        out=out)
+    return pcg(A, b, x0, opts.tol, opts.atol, opts.maxiter, dinv, block3, opts.check_every, out=out)
 
 
 def pcg_matfree(problem, ts, tp, u_lin, bc, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, check_every=32,
@@ -241,8 +284,8 @@ def create_iterative_solve_fn(options):
     if options.solver == "auto":
         raise ValueError("KrylovSolverOptions.solver is 'auto' (unresolved). "
                          "Call resolve_iterative_solver() before create_iterative_solve_fn().")
-    if options.solver != "cg":
-        raise NotImplementedError(f"{options.solver} is not on the HIP path (cg only)")
+    if options.solver not in ("cg", "bicgstab"):
+        raise NotImplementedError(f"{options.solver} is not on the HIP path (cg and bicgstab only)")
     if options.preconditioner is not None:
         raise NotImplementedError("custom preconditioners are not on the HIP path; GeometricMultigrid needs the "
                                   "problem and runs through create_solver / create_linear_solver")
@@ -250,7 +293,7 @@ def create_iterative_solve_fn(options):
     def solve(A, b, x0):
         dinv = create_jacobi_preconditioner(A, options.jacobi_shift) if options.use_jacobi_preconditioner else None
         b3 = {"auto": None, "csr": False, "block3": True}[options.spmv]
-        x, info = pcg(A, b, x0, options.tol, options.atol, options.maxiter, dinv, b3, options.check_every)
+        x, info = krylov(A, b, x0, options, dinv, b3)
         solve.last_info = info
         return x
     solve.last_info = None

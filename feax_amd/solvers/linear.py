@@ -9,6 +9,8 @@ computing u = u0 + du with J_bc du = -R_bc(u0) (linear.py:387-417):
   Jacobi-PCG.
 * preconditioner GeometricMultigrid: FP64 CG preconditioned by a V-cycle
   on the structured grid hierarchy (FP32 V-cycle with precision "mixed").
+* solver "bicgstab": FP64 CSR, HIP BiCGSTAB (jax bicgstab semantics): the
+  method for the non-symmetric operator of symmetric_elimination=False.
 * precision "mixed": FP32 CSR assembly and FP32 Jacobi-PCG (FP64 dot
   products) as the inner solver of an FP64 defect correction
   u <- u + K32^-1 (b - K u), where b - K u is the FP64 matrix-free residual
@@ -26,7 +28,7 @@ from .. import _native as nat
 from ..assembler import TracedStructure, assemble_csr, residual
 from ..DCboundary import DirichletBC
 from .common import (KrylovRecord, check_convergence, create_iterative_solve_fn, create_jacobi_preconditioner,
-                     create_x0, device_norm, pcg)
+                     create_x0, device_norm, krylov, pcg)
 from .options import KrylovSolverOptions, MatrixProperty, resolve_iterative_solver
 
 
@@ -63,9 +65,9 @@ def solve_fp64(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None, keep=N
     b = residual(problem, ts, tp, u0, bc, negate=True)
     x0_fn = opts.x0_fn if opts.x0_fn is not None else create_x0(bc.bc_rows, bc.bc_vals)
     x0 = x0_fn(u0)
-    du, info = pcg(J, b, x0, opts.tol, opts.atol, opts.maxiter, dinv, _b3(opts), opts.check_every)
+    du, info = krylov(J, b, x0, opts, dinv, _b3(opts))
     if keep is not None:
-        keep.update(form="fp64", A=J, dinv=dinv)
+        keep.update(form="fp64", A=J, dinv=dinv, symmetric=symmetric)
     if opts.check_convergence:
         du = check_convergence(J, du, b, opts)
     if stats is not None:
@@ -238,6 +240,9 @@ def create_linear_solver(problem, bc, solver_options=None, adjoint_solver_option
         u0 = _dev_vec(ig, ts) if ig is not None else _zero_like(problem, eff_bc, ts)
         if use_gmg:
             return solve_gmg(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
+        if solver_options.solver == "bicgstab" and (solver_options.precision != "fp64"
+                                                    or solver_options.operator != "csr"):
+            raise NotImplementedError("bicgstab runs on the FP64 CSR operator (precision='fp64', operator='csr')")
         if solver_options.precision != "fp64" and solver_options.operator == "csr":
             return solve_mixed(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
         return solve_fp64(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)

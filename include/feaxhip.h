@@ -362,6 +362,22 @@ int fx_hex8_energy_f64(void* stream, int32_t num_cells, const double* x, const d
                        double* energy, double* dpsi_dmu, void* workspace, size_t ws_bytes);
 
 /* ------------------------------------------------------------------ */
+/* BiCGSTAB with the iteration and stopping rule of
+ * jax.scipy.sparse.linalg.bicgstab (the bicgstab branch of feax's
+ * create_iterative_solve_fn, feax/solvers/common.py:241-252): for the
+ * non-symmetric operator of symmetric_elimination=False or a caller's
+ * solver="bicgstab".  FP64; flags: FX_PCG_BLOCK3 (bcol, n % 3 == 0) or 0
+ * (scalar CSR, indices), FX_PCG_NO_GRAPH.  dinv == NULL: no
+ * preconditioner.  Stops when ||r||^2 <= max(tol^2 ||b||^2, atol^2), at
+ * maxiter, or on breakdown (alpha = 0 or omega = 0); *iters = k, *rs =
+ * ||r_k||^2.  workspace >= fx_bicgstab_workspace_bytes(n). */
+size_t fx_bicgstab_workspace_bytes(int32_t n);
+int fx_bicgstab_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
+                    const int32_t* indices, const int32_t* bcol, const double* data, const double* dinv,
+                    const double* b, const double* x0, double tol, double atol, int32_t maxiter,
+                    int32_t check_every, double* x, int32_t* iters, double* rs, void* workspace, size_t ws_bytes);
+
+/* ------------------------------------------------------------------ */
 /* Geometric multigrid on a structured HEX8 box (StructuredGrid numbering,
  * feax/spgrid.py:28-41).  The analogue of feax's NarrowBandCMG
  * (feax/solvers/cmg.py:95-300: rediscretised coarse operators on cell

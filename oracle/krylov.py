@@ -10,6 +10,15 @@ Restates:
     k < maxiter: q = A p, alpha = gamma / (p.q), x += alpha p, r -= alpha q,
     z = M r, gamma' = r.z, beta = gamma'/gamma, p = z + beta p.  The
     recursively updated r is the one tested; no error on maxiter.
+  jax.scipy.sparse.linalg.bicgstab (third-party; called at
+    feax/solvers/common.py:241-252), restated from the published algorithm:
+    r = b - A x0, rhat = p = q = r, rho = alpha = omega = 1; loop while
+    ||r||^2 > max(tol^2 ||b||^2, atol^2), k < maxiter, no breakdown:
+    rho' = rhat.r, beta = rho'/rho alpha/omega, p = r + beta (p - omega q),
+    phat = M p, q = A phat, alpha = rho'/rhat.q, s = r - alpha q; if
+    s.s < atol2: x += alpha phat, r = s; else shat = M s, t = A shat,
+    omega = t.s/t.t, x += alpha phat + omega shat, r = s - omega t;
+    breakdown when alpha = 0 or omega = 0 (after a full step).
   feax/solvers/common.py:86-94   M(x) = x / (diag + jacobi_shift)
   feax/solvers/common.py:23-53   x0 = lift of BC increments
   feax/solvers/common.py:376-395 check_convergence: NaN unless
@@ -17,10 +26,14 @@ Restates:
   feax/solvers/linear.py:387-417 u = u0 + CG(J_bc, -R_bc(u0), x0)
 
 Pinned by: a scipy.sparse.linalg.spsolve direct solve of the same system
-(tests/test_oracle_krylov.py), the stopping-rule invariants (returned
+(tests/test_oracle.py), the stopping-rule invariants (returned
 residual below the threshold, iteration cap honoured, maxiter=0 returns x0),
-and exact convergence in <= n steps on small SPD systems.  The iterate
-trajectory of JAX's own cg is parity unpinned (needs an import).
+and exact convergence in <= n steps on small SPD systems; bicgstab by the
+direct solve of non-symmetric systems (the row-only Dirichlet elimination
+and a random non-symmetric matrix), by agreeing with scipy's own
+bicgstab's solution, and by one exact step on A = I.  The iterate
+trajectories of JAX's own cg and bicgstab are parity unpinned (need an
+import).
 """
 
 import numpy as np
@@ -48,6 +61,44 @@ def pcg(A, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=None):
         beta = gamma_new / gamma
         p = z + beta * p
         gamma = gamma_new
+        rs = float(r @ r)
+        k += 1
+    return x, k, rs
+
+
+def bicgstab(A, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=None):
+    """Return (x, iters, ||r||^2) following jax.scipy.sparse.linalg.bicgstab
+    (iters = steps taken; a breakdown ends the loop)."""
+    matvec = A if callable(A) else (lambda v: A @ v)
+    M = (lambda v: v) if dinv is None else (lambda v: dinv * v)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=float)
+    atol2 = max(tol * tol * float(b @ b), atol * atol)
+    r = b - matvec(x)
+    rhat, p, q = r.copy(), r.copy(), r.copy()
+    rho = alpha = omega = 1.0
+    k = 0
+    rs = float(r @ r)
+    broke = False
+    while rs > atol2 and k < maxiter and not broke:
+        rho_new = float(rhat @ r)
+        beta = rho_new / rho * alpha / omega
+        p = r + beta * (p - omega * q)
+        phat = M(p)
+        q = matvec(phat)
+        alpha = rho_new / float(rhat @ q)
+        s = r - alpha * q
+        if float(s @ s) < atol2:
+            x = x + alpha * phat
+            r = s
+            broke = alpha == 0.0
+        else:
+            shat = M(s)
+            t = matvec(shat)
+            omega = float(t @ s) / float(t @ t)
+            x = x + alpha * phat + omega * shat
+            r = s - omega * t
+            broke = alpha == 0.0 or omega == 0.0
+        rho = rho_new
         rs = float(r @ r)
         k += 1
     return x, k, rs

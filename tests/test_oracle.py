@@ -263,3 +263,56 @@ def test_x0_lift_and_check_convergence():
     A = np.eye(3)
     assert np.all(np.isnan(krylov.check_convergence(A, np.zeros(3), np.ones(3), 0.1)))
     assert np.all(krylov.check_convergence(A, np.ones(3), np.ones(3), 0.1) == 1.0)
+
+
+def test_bicgstab_oracle_solves_nonsymmetric_systems():
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from oracle import krylov
+    rng = np.random.default_rng(4)
+    n = 60
+    A = sp.random(n, n, density=0.1, random_state=5) + sp.diags(4.0 + rng.random(n))
+    A = A.tocsr()
+    b = rng.standard_normal(n)
+    x, k, rs = krylov.bicgstab(A, b, tol=1e-12, atol=0.0, maxiter=500, dinv=1.0 / A.diagonal())
+    xd = spla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
+    assert rs <= (1e-12) ** 2 * (b @ b) and 0 < k < 500
+    xs, info = spla.bicgstab(A, b, rtol=1e-12, atol=0.0, maxiter=500)
+    assert info == 0 and np.linalg.norm(x - xs) <= 1e-8 * np.linalg.norm(xd)
+    # A = I: one step is exact (s = 0 -> early exit)
+    x1, k1, rs1 = krylov.bicgstab(sp.identity(n).tocsr(), b, tol=1e-12, atol=0.0)
+    assert k1 == 1 and np.allclose(x1, b) and rs1 == 0.0
+    # maxiter caps the count; maxiter = 0 returns x0
+    _, k2, _ = krylov.bicgstab(A, b, tol=1e-30, atol=0.0, maxiter=3)
+    assert k2 == 3
+    x3, k3, _ = krylov.bicgstab(A, b, x0=np.ones(n), maxiter=0)
+    assert k3 == 0 and np.all(x3 == 1.0)
+
+
+def test_bicgstab_oracle_on_row_only_dirichlet_elimination():
+    """symmetric_elimination=False keeps the BC columns: a non-symmetric
+    operator whose solution equals the symmetric elimination's."""
+    import scipy.sparse.linalg as spla
+    from oracle import hex8, krylov, structure
+    pts, cells = structure.structured_grid((4, 2, 2), (1.0, 1.0, 1.0))
+    faces = structure.boundary_faces(pts, cells, lambda p: np.isclose(p[0], 4.0))
+    m = hex8.Model(pts, cells, [faces])
+    rows, vals = structure.dirichlet_rows(pts, [(lambda p: np.isclose(p[0], 0.0), "all", 0.0),
+                                                (lambda p: np.isclose(p[0], 4.0), 2, 0.01)])
+    mat = hex8.Material("linear_elastic", nu=0.3)
+    tr = [hex8.constant_traction(faces, 1e-3)]
+    E = np.full(m.N, 70e3)
+    u_s, info_s = krylov.linear_solve(m, mat, E, tr, rows, vals, tol=1e-12, atol=0.0)
+    u0 = np.zeros(m.n)
+    u0[rows] = vals
+    mask = np.zeros(m.n, bool)
+    mask[rows] = True
+    data = hex8.apply_bc_csr(m.csr_values(u0, E, mat), m.csr, mask, rows, symmetric=False)
+    K = m.scipy_csr(data)
+    assert abs(K - K.T).max() > 0  # not symmetric
+    b = -hex8.apply_bc_res(m.residual(u0, E, mat, tr), u0, rows, vals)
+    x0 = krylov.x0_lift(u0, rows, vals)
+    du, k, _ = krylov.bicgstab(K, b, x0, tol=1e-12, atol=0.0, dinv=krylov.jacobi_dinv(K.diagonal()))
+    assert np.linalg.norm(u0 + du - u_s) <= 1e-8 * np.linalg.norm(u_s)
+    assert np.linalg.norm(du - spla.spsolve(K.tocsc(), b)) <= 1e-8 * np.linalg.norm(du)
