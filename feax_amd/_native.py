@@ -114,6 +114,8 @@ SIGNATURES = {
                               _P, _P, _SZ]),
     "fx_gmg_coarsen_f64": (_I32, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P]),
     "fx_csr_to_dense_f64": (_I32, [_P, _I32, _P, _P, _P, _P]),
+    "fx_gmg_galerkin_workspace_bytes": (_SZ, [_I32, _I32, _I32, _I32, _I32, _I32]),
+    "fx_gmg_galerkin_dense_f64": (_I32, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _SZ]),
 }
 
 _lib = None

@@ -11,8 +11,8 @@
 //   levels    the BC-eliminated CSR of each grid, assembled on the coarse
 //             grid by the HEX8 assembly kernel (rediscretisation)
 //   transfer  trilinear prolongation P between node grids (each direction
-//             halved, 2n -> n cells, or kept where its cell count is odd)
-//             and restriction R = P^T, constrained DOFs of both grids masked
+//             halved, n -> ceil(n/2) cells, or kept) and restriction
+//             R = P^T, constrained DOFs of both grids masked
 //   smoother  Chebyshev polynomial in D^-1 A on [lmax / ratio, 1.1 lmax]
 //             (Saad, Iterative Methods, Alg. 12.1; lmax from a few Lanczos
 //             steps, fx_gmg_eig_bounds_f64), the same polynomial
@@ -106,6 +106,14 @@ __device__ __forceinline__ void node_ijk(int id, int NY, int NZ, int& I, int& J,
 
 __device__ __forceinline__ bool free_dof(const uint8_t* __restrict__ mask, int i) { return !mask || !mask[i]; }
 
+// Coarsening of one direction: kept (nc = nf, factor 1) or halved
+// (nc = ceil(nf / 2), factor 2; for odd nf the last coarse cell reaches one
+// fine cell past the fine grid, as in feax/solvers/cmg.py:82-87, 178-182).
+__host__ __device__ __forceinline__ int cfactor(int nf, int nc) { return nf == nc ? 1 : 2; }
+__host__ __device__ __forceinline__ bool coarsening_ok(int nf, int nc) {
+  return nc >= 1 && (nc == nf || (nf >= 2 && nc == (nf + 1) / 2));
+}
+
 // Coarse nodes under fine index I along a direction coarsened by f (1 or
 // 2): f = 2 takes floor(I/2) .. ceil(I/2) (one node for even I, two with
 // weight 1/2 for odd I, trilinear); f = 1 takes I.
@@ -129,9 +137,9 @@ __global__ __launch_bounds__(kBlock) void prolong_add_kernel(int nxf, int nyf, i
   int I, J, K;
   node_ijk(f, NYf, NZf, I, J, K);
   int i0, j0, k0, ni, nj, nk;
-  coarse_span(I, nxf / nxc, i0, ni);
-  coarse_span(J, nyf / nyc, j0, nj);
-  coarse_span(K, nzf / nzc, k0, nk);
+  coarse_span(I, cfactor(nxf, nxc), i0, ni);
+  coarse_span(J, cfactor(nyf, nyc), j0, nj);
+  coarse_span(K, cfactor(nzf, nzc), k0, nk);
   const T w = T(1) / T(ni * nj * nk);
   T a0 = T(0), a1 = T(0), a2 = T(0);
   for (int a = 0; a < ni; ++a)
@@ -158,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void restrict_kernel(int nxf, int nyf, int 
                                                           const PcgState* __restrict__ st) {
   if (finished(st)) return;
   const int NXf = nxf + 1, NYf = nyf + 1, NZf = nzf + 1, NYc = nyc + 1, NZc = nzc + 1;
-  const int fx = nxf / nxc, fy = nyf / nyc, fz = nzf / nzc;
+  const int fx = cfactor(nxf, nxc), fy = cfactor(nyf, nyc), fz = cfactor(nzf, nzc);
   const int C = blockIdx.x * kBlock + threadIdx.x;
   if (C >= (nxc + 1) * NYc * NZc) return;
   int i, j, k;
@@ -424,7 +432,10 @@ __global__ __launch_bounds__(kBlock) void coarsen_node_kernel(int nxf, int nyf, 
   if (C >= (nxc + 1) * (nyc + 1) * (nzc + 1)) return;
   int i, j, k;
   node_ijk(C, nyc + 1, nzc + 1, i, j, k);
-  const int f = ((nxf / nxc) * i * (nyf + 1) + (nyf / nyc) * j) * (nzf + 1) + (nzf / nzc) * k;
+  // a coarse node past the end of an odd fine direction takes the last fine node
+  const int I = min(cfactor(nxf, nxc) * i, nxf), J = min(cfactor(nyf, nyc) * j, nyf),
+            K = min(cfactor(nzf, nzc) * k, nzf);
+  const int f = (I * (nyf + 1) + J) * (nzf + 1) + K;
   for (int c = 0; c < ncomp; ++c) coarse[ncomp * C + c] = fine[ncomp * f + c];
 }
 
@@ -433,14 +444,105 @@ __global__ __launch_bounds__(kBlock) void coarsen_cell_kernel(int nxf, int nyf, 
                                                               double* __restrict__ coarse) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= nxc * nyc * nzc) return;
-  const int fx = nxf / nxc, fy = nyf / nyc, fz = nzf / nzc;
+  const int fx = cfactor(nxf, nxc), fy = cfactor(nyf, nyc), fz = cfactor(nzf, nzc);
   const int ck = e % nzc, t = e / nzc;
   const int cj = t % nyc, ci = t / nyc;
   double s = 0.0;
-  for (int a = 0; a < fx; ++a)
-    for (int b = 0; b < fy; ++b)
-      for (int c = 0; c < fz; ++c) s += fine[((fx * ci + a) * nyf + fy * cj + b) * nzf + fz * ck + c];
-  coarse[e] = s / static_cast<double>(fx * fy * fz);
+  int cnt = 0;  // child cells inside the fine grid (cmg.py:375-386 counts them the same way)
+  for (int a = 0; a < fx; ++a) {
+    if (fx * ci + a >= nxf) continue;
+    for (int b = 0; b < fy; ++b) {
+      if (fy * cj + b >= nyf) continue;
+      for (int c = 0; c < fz; ++c) {
+        if (fz * ck + c >= nzf) continue;
+        s += fine[((fx * ci + a) * nyf + fy * cj + b) * nzf + fz * ck + c];
+        ++cnt;
+      }
+    }
+  }
+  coarse[e] = s / static_cast<double>(cnt);
+}
+
+// ------------------------------------------------------------ Galerkin coarsest
+// Weight of coarse node index i in the prolongation row of fine index I
+// along one direction (factor f): 1 (even I, i = I/2), 1/2 (odd I, i one of
+// its two neighbours), or 1 (kept direction, i = I); else 0.
+__device__ __forceinline__ double pweight(int I, int i, int f) {
+  if (f == 1) return I == i ? 1.0 : 0.0;
+  if (I & 1) return (i == (I >> 1) || i == (I >> 1) + 1) ? 0.5 : 0.0;
+  return i == (I >> 1) ? 1.0 : 0.0;
+}
+
+// B = A P' (n_f x n_c, row-major), P' = M_f P M_c: one thread per (fine row
+// a, coarse DOF J); sums the CSR row of a over its columns b of J's
+// component that are free, times P[b][J].
+__global__ __launch_bounds__(kBlock) void galerkin_ap_kernel(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+                                                             const int* __restrict__ indptr,
+                                                             const int* __restrict__ indices,
+                                                             const double* __restrict__ data,
+                                                             const uint8_t* __restrict__ mask_f,
+                                                             const uint8_t* __restrict__ mask_c,
+                                                             double* __restrict__ B) {
+  const int nf = 3 * (nxf + 1) * (nyf + 1) * (nzf + 1), nc = 3 * (nxc + 1) * (nyc + 1) * (nzc + 1);
+  const long idx = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= static_cast<long>(nf) * nc) return;
+  const int a = static_cast<int>(idx / nc), J = static_cast<int>(idx - static_cast<long>(a) * nc);
+  double v = 0.0;
+  if (free_dof(mask_c, J)) {
+    const int cJ = J % 3;
+    int ci, cj, ck;
+    node_ijk(J / 3, nyc + 1, nzc + 1, ci, cj, ck);
+    const int fx = cfactor(nxf, nxc), fy = cfactor(nyf, nyc), fz = cfactor(nzf, nzc);
+    for (int k = indptr[a]; k < indptr[a + 1]; ++k) {
+      const int b = indices[k];
+      if (b % 3 != cJ || !free_dof(mask_f, b)) continue;
+      int I, Jn, K;
+      node_ijk(b / 3, nyf + 1, nzf + 1, I, Jn, K);
+      const double w = pweight(I, ci, fx) * pweight(Jn, cj, fy) * pweight(K, ck, fz);
+      if (w != 0.0) v += data[k] * w;
+    }
+  }
+  B[idx] = v;
+}
+
+// Ac = P'^T B (n_c x n_c): coarse DOF I gathers the free fine DOFs of its
+// restriction stencil; constrained coarse DOFs get identity rows/columns.
+__global__ __launch_bounds__(kBlock) void galerkin_ptb_kernel(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+                                                              const double* __restrict__ B,
+                                                              const uint8_t* __restrict__ mask_f,
+                                                              const uint8_t* __restrict__ mask_c,
+                                                              double* __restrict__ Ac) {
+  const int nc = 3 * (nxc + 1) * (nyc + 1) * (nzc + 1);
+  const long idx = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= static_cast<long>(nc) * nc) return;
+  const int I = static_cast<int>(idx / nc), J = static_cast<int>(idx - static_cast<long>(I) * nc);
+  if (!free_dof(mask_c, I) || !free_dof(mask_c, J)) {
+    Ac[idx] = I == J ? 1.0 : 0.0;
+    return;
+  }
+  const int NXf = nxf + 1, NYf = nyf + 1, NZf = nzf + 1;
+  const int fx = cfactor(nxf, nxc), fy = cfactor(nyf, nyc), fz = cfactor(nzf, nzc);
+  const int cI = I % 3;
+  int i, j, k;
+  node_ijk(I / 3, nyc + 1, nzc + 1, i, j, k);
+  double v = 0.0;
+  for (int oi = 1 - fx; oi <= fx - 1; ++oi) {
+    const int X = fx * i + oi;
+    if (X < 0 || X >= NXf) continue;
+    for (int oj = 1 - fy; oj <= fy - 1; ++oj) {
+      const int Y = fy * j + oj;
+      if (Y < 0 || Y >= NYf) continue;
+      for (int ok = 1 - fz; ok <= fz - 1; ++ok) {
+        const int Z = fz * k + ok;
+        if (Z < 0 || Z >= NZf) continue;
+        const int a = 3 * ((X * NYf + Y) * NZf + Z) + cI;
+        if (!free_dof(mask_f, a)) continue;
+        const double w = (oi ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (ok ? 0.5 : 1.0);
+        v += w * B[static_cast<long>(a) * nc + J];
+      }
+    }
+  }
+  Ac[idx] = v;
 }
 
 // ------------------------------------------------------------ host side
@@ -541,14 +643,14 @@ int check_levels(int flags, int nlev, const fx_gmg_level* lev, int smooth_degree
     if (L.nx < 1 || L.ny < 1 || L.nz < 1) return fail(FX_EINVAL, "gmg: level %d has no cells", l);
     if (static_cast<long long>(L.nx + 1) * (L.ny + 1) * (L.nz + 1) > (1 << 29) / 3)
       return fail(FX_ELIMIT, "gmg: level %d too large for int32 DOF indices", l);
-    if (!L.indptr || !L.bcol || !L.data || !L.dinv || !L.r || !L.d)
+    const bool dense_coarsest = l == nlev - 1 && L.dense_inv;  // uses only dense_inv, x, b, bc_mask
+    if (!dense_coarsest && (!L.indptr || !L.bcol || !L.data || !L.dinv || !L.r || !L.d))
       return fail(FX_EINVAL, "gmg: level %d lacks its CSR, dinv or scratch vectors", l);
     if ((l > 0 || (flags & FX_GMG_F32)) && (!L.x || !L.b))
       return fail(FX_EINVAL, "gmg: level %d needs x and b", l);
     if (l + 1 < nlev) {
       const fx_gmg_level& C = lev[l + 1];
-      auto halved_or_kept = [](int f, int c) { return f == c || f == 2 * c; };
-      if (!halved_or_kept(L.nx, C.nx) || !halved_or_kept(L.ny, C.ny) || !halved_or_kept(L.nz, C.nz) ||
+      if (!coarsening_ok(L.nx, C.nx) || !coarsening_ok(L.ny, C.ny) || !coarsening_ok(L.nz, C.nz) ||
           (L.nx == C.nx && L.ny == C.ny && L.nz == C.nz))
         return fail(FX_EINVAL,
                     "gmg: level %d grid (%d,%d,%d) -> level %d (%d,%d,%d): each direction must be halved or kept, "
@@ -827,10 +929,40 @@ int fx_pcg_gmg_f64(void* stream, int32_t flags, int32_t nlev, const fx_gmg_level
                          maxiter, check_every, x, iters, rs, w);
 }
 
+size_t fx_gmg_galerkin_workspace_bytes(int32_t nxf, int32_t nyf, int32_t nzf, int32_t nxc, int32_t nyc,
+                                       int32_t nzc) {
+  const size_t nf = 3 * static_cast<size_t>(nxf + 1) * (nyf + 1) * (nzf + 1);
+  const size_t nc = 3 * static_cast<size_t>(nxc + 1) * (nyc + 1) * (nzc + 1);
+  return nf * nc * sizeof(double);
+}
+
+int fx_gmg_galerkin_dense_f64(void* stream, int32_t nxf, int32_t nyf, int32_t nzf, int32_t nxc, int32_t nyc,
+                              int32_t nzc, const int32_t* indptr, const int32_t* indices, const double* data,
+                              const uint8_t* mask_f, const uint8_t* mask_c, double* Ac, void* workspace,
+                              size_t ws_bytes) {
+  if (!coarsening_ok(nxf, nxc) || !coarsening_ok(nyf, nyc) || !coarsening_ok(nzf, nzc) || !indptr || !indices ||
+      !data || !Ac)
+    return fail(FX_EINVAL, "fx_gmg_galerkin_dense_f64: grids (%d,%d,%d) -> (%d,%d,%d) or null pointers", nxf, nyf,
+                nzf, nxc, nyc, nzc);
+  const size_t need = fx_gmg_galerkin_workspace_bytes(nxf, nyf, nzf, nxc, nyc, nzc);
+  const size_t nc = 3 * static_cast<size_t>(nxc + 1) * (nyc + 1) * (nzc + 1);
+  if (nc > 20000) return fail(FX_ELIMIT, "fx_gmg_galerkin_dense_f64: %zu coarse DOFs is too many for a dense matrix", nc);
+  if (!workspace || ws_bytes < need)
+    return fail(FX_EWORKSPACE, "fx_gmg_galerkin_dense_f64: workspace %zu < %zu", ws_bytes, need);
+  hipStream_t s = as_stream(stream);
+  double* B = static_cast<double*>(workspace);
+  const size_t nb = need / sizeof(double);
+  hipLaunchKernelGGL(galerkin_ap_kernel, dim3(static_cast<unsigned>((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                     nxf, nyf, nzf, nxc, nyc, nzc, indptr, indices, data, mask_f, mask_c, B);
+  hipLaunchKernelGGL(galerkin_ptb_kernel, dim3(static_cast<unsigned>((nc * nc + kBlock - 1) / kBlock)), dim3(kBlock),
+                     0, s, nxf, nyf, nzf, nxc, nyc, nzc, B, mask_f, mask_c, Ac);
+  FX_LAUNCHED("fx_gmg_galerkin_dense_f64");
+  return FX_OK;
+}
+
 int fx_gmg_coarsen_f64(void* stream, int32_t loc, int32_t ncomp, int32_t nxf, int32_t nyf, int32_t nzf, int32_t nxc,
                        int32_t nyc, int32_t nzc, const double* fine, double* coarse) {
-  auto ok = [](int f, int c) { return c >= 1 && (f == c || f == 2 * c); };
-  if (!ok(nxf, nxc) || !ok(nyf, nyc) || !ok(nzf, nzc) || !fine || !coarse)
+  if (!coarsening_ok(nxf, nxc) || !coarsening_ok(nyf, nyc) || !coarsening_ok(nzf, nzc) || !fine || !coarse)
     return fail(FX_EINVAL, "fx_gmg_coarsen_f64: grids (%d,%d,%d) -> (%d,%d,%d) or null pointers", nxf, nyf, nzf, nxc,
                 nyc, nzc);
   hipStream_t s = as_stream(stream);

@@ -8,7 +8,8 @@ multigrid CG for structured bands, NarrowBandCMG (feax/solvers/cmg.py:
 
 * levels: the mesh must be a full StructuredGrid box (feax/spgrid.py:
   28-41); each coarser grid halves every direction whose cell count is
-  even (the others are kept), down to ``min_cells`` cells per direction.
+  even (the others are kept) or, when none is even, ceil-halves the odd
+  ones, until a level has at most ``coarse_dofs`` DOFs.
 * coarse operators: the HEX8 assembly kernel on the coarse grid
   (rediscretisation) with the coarse parameter = node value at the same
   node (node var) or mean over the child cells (cell var; the density
@@ -16,9 +17,11 @@ multigrid CG for structured bands, NarrowBandCMG (feax/solvers/cmg.py:
   and the constrained DOFs injected from the fine BC mask.
 * smoother: Chebyshev of ``smooth_degree`` in D^-1 A on
   [lmax / cheb_ratio, 1.1 lmax], lmax from ``lmax_iters`` Lanczos steps.
-* coarsest: dense inverse (rocSOLVER Cholesky of the coarsest CSR, at most
-  ``max_dense_dofs`` DOFs), else Chebyshev of ``coarse_degree`` on
-  [lmin, 1.1 lmax] (both Lanczos estimates).
+* coarsest: dense inverse (rocSOLVER Cholesky) of the Galerkin product
+  P'^T A P' of the level above (fx_gmg_galerkin_dense_f64; at most
+  ``max_dense_dofs`` DOFs), else Chebyshev of ``coarse_degree`` on the
+  rediscretised coarsest operator over [lmin, 1.1 lmax] (both Lanczos
+  estimates).
 * KrylovSolverOptions.precision "fp64" runs the V-cycle in FP64, "mixed"
   in FP32 (FP32 copies of every level); the CG is FP64 either way and stops
   on the jax cg rule ||r||^2 <= max(tol^2 ||b||^2, atol^2).
@@ -38,12 +41,13 @@ from .. import _native as nat
 @dataclass(frozen=True)
 class GeometricMultigrid:
     smooth_degree: int = 2
-    cheb_ratio: float = 8.0
+    cheb_ratio: float = 12.0
     coarse: str = "dense"
     coarse_degree: int = 40
     max_dense_dofs: int = 12000
     max_levels: int = 12
     min_cells: int = 2
+    coarse_dofs: int = 1500
     lmax_iters: int = 10
     check_every: int = 4
 
@@ -54,8 +58,8 @@ class GeometricMultigrid:
             raise ValueError("smooth_degree, coarse_degree and lmax_iters must be >= 1")
         if not self.cheb_ratio > 1.0:
             raise ValueError("cheb_ratio must be > 1")
-        if self.max_levels < 1 or self.min_cells < 1 or self.check_every < 1:
-            raise ValueError("max_levels, min_cells and check_every must be >= 1")
+        if self.max_levels < 1 or self.min_cells < 1 or self.check_every < 1 or self.coarse_dofs < 1:
+            raise ValueError("max_levels, min_cells, coarse_dofs and check_every must be >= 1")
 
 
 def structured_dims(problem):
@@ -84,8 +88,27 @@ def structured_dims(problem):
 
 
 def coarse_dims(dims, min_cells):
-    out = tuple(d // 2 if (d % 2 == 0 and d // 2 >= min_cells) else d for d in dims)
-    return None if out == tuple(dims) else out
+    """Next coarser grid: halve every direction with an even cell count that
+    keeps >= min_cells cells; when none is even, ceil-halve the odd ones
+    (the last coarse cell then reaches one fine cell past the grid, as
+    feax's NarrowBandCMG coarsens, feax/solvers/cmg.py:82-87, 178-182).
+    None when no direction can be coarsened."""
+    even = tuple(d // 2 if (d % 2 == 0 and d // 2 >= min_cells) else d for d in dims)
+    if even != tuple(dims):
+        return even
+    odd = tuple((d + 1) // 2 if (d >= 3 and (d + 1) // 2 >= min_cells) else d for d in dims)
+    return None if odd == tuple(dims) else odd
+
+
+def _coarse_axis(a, nf, nc):
+    """Node coordinates of a coarse direction (every other fine node; one
+    extrapolated node past the end of an odd fine direction)."""
+    if nc == nf:
+        return a
+    c = a[::2]
+    if len(c) < nc + 1:
+        c = np.append(c, a[-1] + (a[-1] - a[-2]))
+    return c
 
 
 def _law(problem):
@@ -113,19 +136,20 @@ class GMGStructure:
         self.ts = [ts]
         self.problems = [problem]
         self.inject = []        # host: coarse node -> fine node of the previous level
-        while len(self.dims) < mg.max_levels:
+        while len(self.dims) < mg.max_levels and 3 * int(np.prod([d + 1 for d in self.dims[-1]])) > mg.coarse_dofs:
             c = coarse_dims(self.dims[-1], mg.min_cells)
             if c is None:
                 break
-            f = [a // b for a, b in zip(self.dims[-1], c)]
-            axes = [a[::k] for a, k in zip(axes, f)]
+            f = [1 if a == b else 2 for a, b in zip(self.dims[-1], c)]
+            axes = [_coarse_axis(a, nf, nc) for a, nf, nc in zip(axes, self.dims[-1], c)]
             pts = np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1).reshape(-1, 3)
             cells = StructuredGrid(c).cell_to_nodes(np.arange(int(np.prod(c))))
             cp = LevelProblem(Mesh(pts, cells), vec=3, dim=3)
             cp.law = _law(problem)
-            fy, fz = self.dims[-1][1] + 1, self.dims[-1][2] + 1
+            nf = self.dims[-1]
             i, j, k = np.meshgrid(*[np.arange(n + 1) for n in c], indexing="ij")
-            self.inject.append(((f[0] * i * fy + f[1] * j) * fz + f[2] * k).reshape(-1))
+            I, J, K = (np.minimum(fk * ik, n) for fk, ik, n in zip(f, (i, j, k), nf))
+            self.inject.append(((I * (nf[1] + 1) + J) * (nf[2] + 1) + K).reshape(-1))
             self.dims.append(c)
             self.problems.append(cp)
             self.ts.append(TracedStructure.from_problem(cp, ts.device))
@@ -154,7 +178,7 @@ class GMGStructure:
 
 
 def structure_for(problem, ts, mg):
-    key = (mg.max_levels, mg.min_cells, str(ts.device))
+    key = (mg.max_levels, mg.min_cells, mg.coarse_dofs, str(ts.device))
     cache = problem.__dict__.setdefault("_gmg_structure", {})
     st = cache.get(key)
     if st is None or st.ts[0] is not ts:
@@ -211,8 +235,21 @@ class GMGHierarchy:
         self.A64 = [A]
         self.dinv64 = [dinv]
         structs = []
+        last = S.nlev - 1
+        dense_last = mg.coarse == "dense" and S.ts[last].n <= mg.max_dense_dofs
         for l in range(S.nlev):
             cts = S.ts[l]
+            mask = bcs[l].device(dev)[1] if (bcs[l] is not None and len(bcs[l].bc_rows)) else None
+            if l == last and l > 0 and dense_last:
+                # Dense coarsest: only its right-hand side, correction and
+                # mask are used (the operator is the Galerkin product below).
+                x = torch.empty(cts.n, dtype=vdt, device=dev)
+                b = torch.empty(cts.n, dtype=vdt, device=dev)
+                self._keep += [mask, x, b]
+                nx, ny, nz = S.dims[l]
+                structs.append(nat.FxGmgLevel(nx, ny, nz, None, None, None, None, nat.ptr(mask), None, 0.0, 0.0,
+                                              nat.ptr(x), nat.ptr(b), None, None))
+                continue
             if l > 0:
                 prev = S.dims[l - 1]
                 pc = torch.empty(cts.num_nodes if loc == nat.FX_PARAM_NODE else cts.num_cells, dtype=torch.float64,
@@ -234,7 +271,6 @@ class GMGHierarchy:
                 nat.call("fx_convert_f64_to_f32", stream, cts.nse, 1.0, nat.ptr(Al.data), nat.ptr(data))
                 dv = torch.empty(cts.n, dtype=torch.float32, device=dev)
                 nat.call("fx_convert_f64_to_f32", stream, cts.n, 1.0, nat.ptr(dl), nat.ptr(dv))
-            mask = bcs[l].device(dev)[1] if (bcs[l] is not None and len(bcs[l].bc_rows)) else None
             need_xb = l > 0 or self.fp32
             x = torch.empty(cts.n, dtype=vdt, device=dev) if need_xb else None
             b = torch.empty(cts.n, dtype=vdt, device=dev) if need_xb else None
@@ -248,12 +284,24 @@ class GMGHierarchy:
         self.levels = (nat.FxGmgLevel * S.nlev)(*structs)
         flags = nat.FX_GMG_F32 if self.fp32 else 0
         self.flags = flags
-        # Coarsest level: dense inverse when small enough.
-        last = S.nlev - 1
+        # Coarsest level: dense inverse when small enough, of the Galerkin
+        # product P'^T A P' of the level above (exact across a ceil-halved
+        # direction, where rediscretisation is not).
         nc = S.ts[last].n
         self.dense = None
-        if mg.coarse == "dense" and nc <= mg.max_dense_dofs:
-            inv = _dense_inverse(self.A64[last], dev)
+        if dense_last:
+            if last > 0:
+                Af = self.A64[last - 1]
+                mf = bcs[last - 1].device(dev)[1] if len(bcs[last - 1].bc_rows) else None
+                mc = bcs[last].device(dev)[1] if len(bcs[last].bc_rows) else None
+                Ac = torch.empty((nc, nc), dtype=torch.float64, device=dev)
+                ws = _workspace(nat.lib().fx_gmg_galerkin_workspace_bytes(*S.dims[last - 1], *S.dims[last]), dev)
+                nat.call("fx_gmg_galerkin_dense_f64", stream, *S.dims[last - 1], *S.dims[last], nat.ptr(Af.indptr),
+                         nat.ptr(Af.indices), nat.ptr(Af.data), nat.ptr(mf), nat.ptr(mc), nat.ptr(Ac), nat.ptr(ws),
+                         ws.numel())
+                inv = torch.cholesky_inverse(torch.linalg.cholesky(Ac))
+            else:
+                inv = _dense_inverse(self.A64[last], dev)
             if self.fp32:
                 inv32 = torch.empty((nc, nc), dtype=torch.float32, device=dev)
                 nat.call("fx_convert_f64_to_f32", stream, nc * nc, 1.0, nat.ptr(inv), nat.ptr(inv32))

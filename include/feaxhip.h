@@ -385,15 +385,16 @@ int fx_bicgstab_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const i
  * smoother is a Chebyshev polynomial in D^-1 A and the coarsest level is a
  * dense inverse (or a long Chebyshev polynomial).
  *
- * Level l has nx x ny x nz cells; level l+1 halves every direction or
- * keeps some (nx_{l+1} = nx_l / 2 or nx_l, one direction at least
- * halved).  Every level holds its BC-eliminated block-3 CSR (indptr, bcol
+ * Level l has nx x ny x nz cells; level l+1 halves or keeps each
+ * direction (nx_{l+1} = ceil(nx_l / 2) or nx_l, one direction at least
+ * halved; an odd count's last coarse cell reaches past the fine grid).  Every level holds its BC-eliminated block-3 CSR (indptr, bcol
  * of fx_csr_block3_index), dinv = 1 / diag, the constrained-DOF mask,
  * lmax (largest eigenvalue of D^-1 A, fx_gmg_eig_bounds_f64; lmin, used
  * by a Chebyshev coarsest solve only, the smallest or 0) and
  * four vectors of 3 * nodes entries.  Values and vectors are double, or
  * float with FX_GMG_F32.  Level 0's x and b are used only with FX_GMG_F32
- * (the V-cycle's own copy of the CG residual and correction). */
+ * (the V-cycle's own copy of the CG residual and correction); a coarsest
+ * level with dense_inv uses only dense_inv, x, b and bc_mask. */
 #define FX_GMG_F32 32
 typedef struct fx_gmg_level {
   int32_t nx, ny, nz;
@@ -447,6 +448,20 @@ int fx_pcg_gmg_f64(void* stream, int32_t flags, int32_t nlev, const fx_gmg_level
  * 426-431).  Grids as for fx_gmg_level. */
 int fx_gmg_coarsen_f64(void* stream, int32_t loc, int32_t ncomp, int32_t nxf, int32_t nyf, int32_t nzf, int32_t nxc,
                        int32_t nyc, int32_t nzc, const double* fine, double* coarse);
+
+/* Dense Galerkin operator Ac = P'^T A P' of the next coarser grid
+ * (P' = the trilinear prolongation with the constrained DOFs of both grids
+ * masked; constrained coarse DOFs get identity rows and columns): the exact
+ * coarse operator for the dense coarsest level, also across a ceil-halved
+ * direction.  A: the fine level's BC-eliminated CSR (scalar indices).
+ * Ac: (3 Nc)^2 row-major, at most 20000 coarse DOFs.  workspace >=
+ * fx_gmg_galerkin_workspace_bytes (the dense A P', 3 Nf x 3 Nc). */
+size_t fx_gmg_galerkin_workspace_bytes(int32_t nxf, int32_t nyf, int32_t nzf, int32_t nxc, int32_t nyc,
+                                       int32_t nzc);
+int fx_gmg_galerkin_dense_f64(void* stream, int32_t nxf, int32_t nyf, int32_t nzf, int32_t nxc, int32_t nyc,
+                              int32_t nzc, const int32_t* indptr, const int32_t* indices, const double* data,
+                              const uint8_t* mask_f, const uint8_t* mask_c, double* Ac, void* workspace,
+                              size_t ws_bytes);
 
 /* dense[i * n + j] = A[i, j] of a CSR matrix (double), zeros elsewhere. */
 int fx_csr_to_dense_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices,

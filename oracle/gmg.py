@@ -23,6 +23,8 @@ Pinned by (tests/test_oracle_gmg.py):
                 Chebyshev series)
   vcycle        symmetric positive definite as an explicit matrix; one
                 level with the exact inverse is A^-1
+  galerkin_dense  equals the rediscretised coarse operator for a uniform
+                material on even grids (the Galerkin identity above)
   gmg_pcg       solution equals scipy's direct solve
   lanczos_bounds  inside the spectrum of D^-1 A and converging to its ends
                 (numpy eigh of the symmetrically scaled operator)
@@ -34,8 +36,13 @@ import scipy.sparse as sp
 
 def coarse_dims(dims, min_cells=2):
     """Halve each direction whose cell count is even and stays >= min_cells;
-    None when no direction can be halved."""
+    when none can be, ceil-halve the odd ones (cmg.py:82-87, 178-182: the last
+    coarse cell reaches one fine cell past the grid); None when nothing can
+    be coarsened."""
     out = tuple(d // 2 if (d % 2 == 0 and d // 2 >= min_cells) else d for d in dims)
+    if out != tuple(dims):
+        return out
+    out = tuple((d + 1) // 2 if (d >= 3 and (d + 1) // 2 >= min_cells) else d for d in dims)
     return None if out == tuple(dims) else out
 
 
@@ -44,7 +51,7 @@ def prolongation_1d(nf, nc):
     (nf = 2 nc) -> weights 1 (even fine index) or 1/2, 1/2 (odd); kept -> I."""
     if nf == nc:
         return sp.identity(nf + 1, format="csr")
-    assert nf == 2 * nc
+    assert nc == (nf + 1) // 2
     rows, cols, vals = [], [], []
     for i in range(nf + 1):
         if i % 2 == 0:
@@ -69,27 +76,43 @@ def prolongation(dims_f, dims_c, free_f=None, free_c=None):
     return P.tocsr()
 
 
+def _factors(dims_f, dims_c):
+    return tuple(1 if f == c else 2 for f, c in zip(dims_f, dims_c))
+
+
 def node_injection(dims_f, dims_c):
-    """Fine node id of every coarse node (same coordinates)."""
-    fx, fy, fz = (f // c for f, c in zip(dims_f, dims_c))
+    """Fine node id of every coarse node (same coordinates; a coarse node
+    past the end of an odd fine direction takes the last fine node)."""
+    fx, fy, fz = _factors(dims_f, dims_c)
     nyf, nzf = dims_f[1] + 1, dims_f[2] + 1
     i, j, k = np.meshgrid(np.arange(dims_c[0] + 1), np.arange(dims_c[1] + 1), np.arange(dims_c[2] + 1),
                           indexing="ij")
-    return ((fx * i * nyf + fy * j) * nzf + fz * k).reshape(-1)
+    I, J, K = np.minimum(fx * i, dims_f[0]), np.minimum(fy * j, dims_f[1]), np.minimum(fz * k, dims_f[2])
+    return ((I * nyf + J) * nzf + K).reshape(-1)
 
 
 def cell_children(dims_f, dims_c):
     """(E_c, k) fine cell ids inside every coarse cell (k = product of the
-    per-direction factors)."""
-    fx, fy, fz = (f // c for f, c in zip(dims_f, dims_c))
-    ny, nz = dims_f[1], dims_f[2]
+    per-direction factors), -1 where the coarse cell reaches past the fine
+    grid."""
+    fx, fy, fz = _factors(dims_f, dims_c)
+    nx, ny, nz = dims_f
     ci, cj, ck = np.meshgrid(np.arange(dims_c[0]), np.arange(dims_c[1]), np.arange(dims_c[2]), indexing="ij")
     out = []
     for a in range(fx):
         for b in range(fy):
             for c in range(fz):
-                out.append(((fx * ci + a) * ny + fy * cj + b) * nz + fz * ck + c)
+                I, J, K = fx * ci + a, fy * cj + b, fz * ck + c
+                ids = (I * ny + J) * nz + K
+                out.append(np.where((I < nx) & (J < ny) & (K < nz), ids, -1))
     return np.stack([o.reshape(-1) for o in out], axis=1)
+
+
+def coarsen_cell_var(p, dims_f, dims_c):
+    """Mean of a cell field over the child cells inside the fine grid."""
+    ch = cell_children(dims_f, dims_c)
+    ok = ch >= 0
+    return np.where(ok, p[np.maximum(ch, 0)], 0.0).sum(axis=1) / ok.sum(axis=1)
 
 
 def chebyshev(A, dinv, b, x, lmax, ratio, degree, zero_init, lmin=0.0):
@@ -185,14 +208,27 @@ def gmg_pcg(A, b, x0, M, tol=1e-10, atol=0.0, maxiter=1000):
     return x, k
 
 
+def galerkin_dense(fine_level, coarse_mask):
+    """P'^T A P' (dense) of a level's operator with its masked transfer to
+    the next grid; constrained coarse DOFs get identity rows and columns."""
+    P = fine_level["P"]
+    G = (P.T @ fine_level["A"] @ P).toarray()
+    fixed = np.asarray(coarse_mask, bool)
+    G[fixed, :] = 0.0
+    G[:, fixed] = 0.0
+    G[fixed, fixed] = 1.0
+    return G
+
+
 def hierarchy(dims, param, mat, bc_mask, nlev=None, min_cells=2, lmax_iters=10, dense=True, shift=1e-12,
-              spacing=(1.0, 1.0, 1.0)):
+              spacing=(1.0, 1.0, 1.0), coarse_dofs=1):
     """Level list for vcycle(): rediscretised BC-eliminated operators on the
-    grids of coarse_dims (node var: value at the same node; cell var: mean
-    of the child cells), constrained DOFs injected from bc_mask, dinv =
+    grids of coarse_dims until a level has <= coarse_dofs DOFs (node var:
+    value at the same node; cell var: mean of the child cells inside the
+    fine grid), constrained DOFs injected from bc_mask, dinv =
     1 / (diag + shift), lmax (and the coarsest lmin) by lanczos_bounds,
-    masked transfers, dense inverse of the coarsest operator when
-    ``dense``."""
+    masked transfers, and with ``dense`` the inverse of the Galerkin
+    product P'^T A P' of the level above as the coarsest solve."""
     from . import hex8, structure
     levels = []
     d = tuple(dims)
@@ -208,21 +244,22 @@ def hierarchy(dims, param, mat, bc_mask, nlev=None, min_cells=2, lmax_iters=10, 
         lev = dict(A=A, dims=d, free=~mask, dinv=1.0 / (A.diagonal() + shift), param=p, mask=mask)
         levels.append(lev)
         c = coarse_dims(d, min_cells)
-        if c is None or (nlev is not None and len(levels) == nlev):
+        if c is None or (nlev is not None and len(levels) == nlev) or model.n <= coarse_dofs:
             break
         inj = node_injection(d, c)
         if p.shape[0] == model.N:
             pc = p[inj]
         else:
-            pc = p[cell_children(d, c)].mean(axis=1)
+            pc = coarsen_cell_var(p, d, c)
         maskc = mask.reshape(-1, 3)[inj].reshape(-1)
         lev["P"] = prolongation(d, c, ~mask, ~maskc)
-        h = h * np.asarray([a // b for a, b in zip(d, c)], float)
+        h = h * np.asarray(_factors(d, c), float)
         d, p, mask = c, pc, maskc
     for i, lev in enumerate(levels):
         last = i == len(levels) - 1
         if last and dense:
-            lev["dense_inv"] = np.linalg.inv(lev["A"].toarray())
+            lev["dense_inv"] = np.linalg.inv(galerkin_dense(levels[i - 1], lev["mask"]) if i > 0
+                                             else lev["A"].toarray())
             lev["lmax"] = lev["lmin"] = 0.0
         else:
             lo, hi = lanczos_bounds(lev["A"], lev["dinv"], lmax_iters)

@@ -54,17 +54,17 @@ def _oracle_levels(prob, bc, rho, mg, dense=True):
     dims, _ = structured_dims(prob)
     mat = hex8.Material("simp", nu=0.3, E0=1.0, Emin=1e-3, penal=3.0)
     return gmg.hierarchy(dims, rho, mat, bc.bc_mask, min_cells=mg.min_cells, lmax_iters=mg.lmax_iters,
-                         nlev=mg.max_levels, dense=dense)
+                         nlev=mg.max_levels, dense=dense, coarse_dofs=mg.coarse_dofs)
 
 
 @pytest.mark.parametrize("dims,coarse", [((8, 4, 4), "dense"), ((8, 4, 6), "dense"), ((8, 4, 4), "chebyshev"),
-                                         ((6, 2, 2), "chebyshev")])
+                                         ((6, 2, 2), "chebyshev"), ((10, 6, 6), "dense"), ((5, 3, 7), "chebyshev")])
 def test_vcycle_matches_oracle(dims, coarse):
     import torch
     from oracle import gmg
     from feax_amd.solvers.gmg import GeometricMultigrid
     prob, bc, rho = _simp_problem(dims)
-    mg = GeometricMultigrid(coarse=coarse, min_cells=1, coarse_degree=20)
+    mg = GeometricMultigrid(coarse=coarse, min_cells=1, coarse_degree=20, coarse_dofs=1)
     H, ts = _hierarchy(prob, bc, rho, mg)
     levels = _oracle_levels(prob, bc, rho, mg, dense=coarse == "dense")
     assert H.nlev == len(levels) and [tuple(d) for d in H.S.dims] == [l["dims"] for l in levels]
@@ -88,7 +88,7 @@ def test_fp32_vcycle_close_to_oracle():
     from oracle import gmg
     from feax_amd.solvers.gmg import GeometricMultigrid
     prob, bc, rho = _simp_problem((8, 4, 4))
-    mg = GeometricMultigrid(min_cells=1)
+    mg = GeometricMultigrid(min_cells=1, coarse_dofs=1)
     H, ts = _hierarchy(prob, bc, rho, mg, fp32=True)
     levels = _oracle_levels(prob, bc, rho, mg)
     r = np.where(bc.bc_mask, 0.0, np.random.default_rng(8).standard_normal(ts.n))
@@ -108,7 +108,7 @@ def test_gmg_solve_matches_direct_solve(precision):
     from feax_amd.solvers.gmg import GeometricMultigrid
     prob, bc, rho = _simp_problem((16, 8, 8), seed=9)
     opts = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=0.0, precision=precision,
-                                  preconditioner=GeometricMultigrid())
+                                  preconditioner=GeometricMultigrid(coarse_dofs=200))
     solver = fe.create_solver(prob, bc, opts, linear=True)
     t = fe.TracedParams.create_uniform_surface_var(prob, 0.5)
     tp = fe.TracedParams(volume_vars=(torch.as_tensor(rho, device="cuda"),), surface_vars=[(t,)])
@@ -129,7 +129,7 @@ def test_graph_and_direct_launches_agree_bitwise():
     import torch
     from feax_amd.solvers.gmg import GeometricMultigrid
     prob, bc, rho = _simp_problem((8, 4, 4))
-    H, ts = _hierarchy(prob, bc, rho, GeometricMultigrid(min_cells=1, check_every=3))
+    H, ts = _hierarchy(prob, bc, rho, GeometricMultigrid(min_cells=1, check_every=3, coarse_dofs=1))
     b = torch.as_tensor(np.where(bc.bc_mask, 0.0, np.random.default_rng(2).standard_normal(ts.n)), device="cuda")
     x1, i1 = H.solve(b, None, 1e-10, 0.0, 200, graph=True)
     x2, i2 = H.solve(b, None, 1e-10, 0.0, 200, graph=False)
@@ -147,7 +147,7 @@ def test_gradient_through_gmg_solver_matches_jacobi_path():
     prob, bc, rho = _simp_problem((8, 4, 4), seed=4)
     t = fe.TracedParams.create_uniform_surface_var(prob, 0.5)
     grads = []
-    for pre in (GeometricMultigrid(min_cells=1), None):
+    for pre in (GeometricMultigrid(min_cells=1, coarse_dofs=1), None):
         opts = fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0, preconditioner=pre,
                                       use_jacobi_preconditioner=pre is None)
         solver = fe.create_solver(prob, bc, opts, linear=True, return_solution=False)

@@ -28,6 +28,25 @@ def test_prolongation_reproduces_trilinear_fields(dims):
     assert np.array_equal(pf[inj], pc)
 
 
+@pytest.mark.parametrize("dims", [(5, 3, 3), (7, 1, 3)])
+def test_ceil_halving_prolongation_reproduces_trilinear_fields(dims):
+    """Odd counts are ceil-halved: the coarse grid (spacing 2) reaches one
+    fine cell past the fine one and P still reproduces trilinear fields."""
+    c = gmg.coarse_dims(dims, 1)
+    assert all(b == (a + 1) // 2 or b == a for a, b in zip(dims, c)) and c != dims
+    h = [1 if a == b else 2 for a, b in zip(dims, c)]
+    pf, _ = structure.structured_grid(dims)
+    pc, _ = structure.structured_grid(c, h)
+    P = gmg.prolongation(dims, c)
+    assert np.abs(P @ _field(pc).reshape(-1) - _field(pf).reshape(-1)).max() < 1e-12
+    # children inside the fine grid only; the mean of a constant is the constant
+    ch = gmg.cell_children(dims, c)
+    assert np.array_equal(np.sort(ch[ch >= 0]), np.arange(int(np.prod(dims))))
+    assert np.allclose(gmg.coarsen_cell_var(np.full(int(np.prod(dims)), 3.5), dims, c), 3.5)
+    inj = gmg.node_injection(dims, c)
+    assert inj.max() < len(pf)
+
+
 @pytest.mark.parametrize("dims", [(4, 2, 2), (4, 2, 3), (2, 4, 4)])
 def test_galerkin_product_equals_rediscretised_operator(dims):
     """Nested Q1 spaces with exact 2x2x2 Gauss on both grids: P^T K_h P = K_H
@@ -44,6 +63,21 @@ def test_galerkin_product_equals_rediscretised_operator(dims):
     P = gmg.prolongation(dims, c)
     G = (P.T @ K @ P).toarray()
     assert np.abs(G - K2).max() <= 1e-13 * np.abs(K2).max()
+
+
+def test_galerkin_coarsest_equals_rediscretised_for_uniform_material():
+    """Uniform material, even grid: the Galerkin coarsest operator (with the
+    masks and identity rows) equals the rediscretised BC-eliminated one."""
+    dims = (4, 2, 2)
+    pts, cells = structure.structured_grid(dims)
+    mask = np.zeros(3 * len(pts), bool)
+    mask[(3 * np.nonzero(pts[:, 0] == 0.0)[0][:, None] + np.arange(3)).reshape(-1)] = True
+    mat = hex8.Material("linear_elastic", nu=0.3)
+    levels = gmg.hierarchy(dims, np.full(len(cells), 5.0), mat, mask, nlev=2, min_cells=1)
+    G = gmg.galerkin_dense(levels[0], levels[1]["mask"])
+    A1 = levels[1]["A"].toarray()
+    assert np.abs(G - A1).max() <= 1e-12 * np.abs(A1).max()
+    assert np.allclose(levels[1]["dense_inv"], np.linalg.inv(A1), rtol=1e-9, atol=1e-12 * np.abs(np.linalg.inv(A1)).max())
 
 
 def test_cell_children_and_coarsened_average():
@@ -113,6 +147,18 @@ def test_single_level_dense_vcycle_is_the_inverse():
     x, k = gmg.gmg_pcg(A, b, None, lambda r: gmg.vcycle(levels, r, 3, 40, 15.0), tol=1e-12)
     assert k == 1
     assert np.allclose(A @ x, b, atol=1e-10 * np.abs(b).max())
+
+
+def test_gmg_pcg_with_ceil_halved_levels_matches_direct_solve():
+    dims = (10, 6, 6)
+    pts, cells, mask, mat, param = _cantilever(dims, "simp", seed=8)
+    levels = gmg.hierarchy(dims, param, mat, mask, min_cells=1)
+    assert [l["dims"] for l in levels][:3] == [(10, 6, 6), (5, 3, 3), (3, 2, 2)]
+    A = levels[0]["A"]
+    b = np.where(mask, 0.0, np.random.default_rng(2).standard_normal(A.shape[0]))
+    x, k = gmg.gmg_pcg(A, b, None, lambda r: gmg.vcycle(levels, r, 2, 40, 8.0), tol=1e-10)
+    xd = spla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd) and k < 40
 
 
 @pytest.mark.parametrize("kind", ["simp", "linear"])

@@ -201,7 +201,9 @@ def test_structured_dims_detects_box_grids_only():
     prob = fe.Problem(fe.StructuredGrid((6, 4, 5), spacing=(0.5, 1.0, 2.0)).to_mesh(), vec=3, dim=3)
     dims, axes = structured_dims(prob)
     assert dims == (6, 4, 5) and np.allclose(axes[2], 2.0 * np.arange(6))
-    assert coarse_dims((6, 4, 5), 2) == (3, 2, 5) and coarse_dims((3, 2, 5), 2) is None
+    assert coarse_dims((6, 4, 5), 2) == (3, 2, 5)          # even counts first
+    assert coarse_dims((3, 2, 5), 2) == (2, 2, 3)          # then odd ones, ceil-halved
+    assert coarse_dims((2, 2, 2), 2) is None
     pts = prob.points[::-1].copy()
     cells = len(pts) - 1 - prob.cells
     prob2 = fe.Problem(fe.Mesh(pts, cells), vec=3, dim=3)
