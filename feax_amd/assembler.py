@@ -386,9 +386,34 @@ def get_res(problem, sol_list, traced_params, ts=None):
     return problem.unflatten_fn_sol_list(residual(problem, ts, traced_params, flat))
 
 
+def view_csr(problem, ts, A):
+    """The problem's matrix_view of a FULL CSRMatrix: the UPPER (j >= i) or
+    LOWER (j <= i) slots of the pattern, i.e. what feax assembles for that
+    view (the dedupe of the filtered (I, J), feax/problem.py:218-231) --
+    feax's BC elimination acts slot by slot, so filtering commutes with it.
+    FULL returns A itself."""
+    import torch
+    from .problem import MatrixView
+    if problem.matrix_view == MatrixView.FULL:
+        return A
+    key = ("view", problem.matrix_view)
+    if key not in ts._cache:
+        indptr, indices, slots = problem.view_pattern()
+        dev = ts.device
+        ts._cache[key] = (torch.as_tensor(np.ascontiguousarray(indptr, np.int32), device=dev),
+                          torch.as_tensor(np.ascontiguousarray(indices, np.int32), device=dev),
+                          torch.as_tensor(np.ascontiguousarray(slots, np.int64), device=dev))
+    indptr, indices, slots = ts._cache[key]
+    return CSRMatrix(A.data.index_select(0, slots), indptr, indices, A.shape, block3=False)
+
+
 def create_J_bc_csr_parametric(problem, symmetric=True, dtype=None):
+    """fn(sol, traced_params, bc, ts) -> J_bc as a CSRMatrix in the
+    problem's matrix_view (feax/assembler.py:1695-1713).  The solvers of
+    this package always work on the FULL matrix."""
     def fn(sol_flat, traced_params, bc, ts=None):
-        return assemble_csr(problem, _ts_or_build(problem, ts), traced_params, sol_flat, bc, symmetric, dtype)
+        ts = _ts_or_build(problem, ts)
+        return view_csr(problem, ts, assemble_csr(problem, ts, traced_params, sol_flat, bc, symmetric, dtype))
     return fn
 
 

@@ -405,3 +405,33 @@ def test_bicgstab_gradient_nonsymmetric_elimination_matches_cg():
     err = np.abs(grads[0] - grads[1]).max() / np.abs(grads[1]).max()
     print("gradient rel diff bicgstab/row-only vs cg/symmetric", err)
     assert err < 1e-7
+
+
+@pytest.mark.parametrize("view", ["UPPER", "LOWER"])
+def test_matrix_view_csr_is_the_triangle_of_the_full_matrix(view):
+    """create_J_bc_csr_parametric honours matrix_view: the UPPER / LOWER
+    pattern of feax's filtered (I, J) with the values of the BC-eliminated
+    FULL matrix (tests/test_matrix_view.py of feax checks the pattern)."""
+    import scipy.sparse as sp
+    import feax_amd as fe
+    from feax_amd.assembler import create_J_bc_csr_parametric
+
+    class LE(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.IsotropicLinearElastic(nu=0.3)
+
+    mesh = fe.StructuredGrid((4, 2, 3)).to_mesh()
+    E = np.random.default_rng(2).uniform(1.0, 2.0, 4 * 2 * 3)
+    full = LE(mesh, vec=3, dim=3)
+    pv = LE(mesh, vec=3, dim=3, matrix_view=view)
+    tps = fe.TracedParams(volume_vars=(E,))
+    bcs = [fe.DirichletBCConfig([fe.DirichletBCSpec(lambda x: np.isclose(x[0], 0.0), "all", 0.0)]).create_bc(q)
+           for q in (full, pv)]
+    Af = create_J_bc_csr_parametric(full)(None, tps, bcs[0])
+    Av = create_J_bc_csr_parametric(pv)(None, tps, bcs[1])
+    n = full.num_total_dofs_all_vars
+    F = sp.csr_matrix((Af.data.cpu().numpy(), Af.indices.cpu().numpy(), Af.indptr.cpu().numpy()), shape=(n, n))
+    V = sp.csr_matrix((Av.data.cpu().numpy(), Av.indices.cpu().numpy(), Av.indptr.cpu().numpy()), shape=(n, n))
+    ref = sp.triu(F) if view == "UPPER" else sp.tril(F)
+    assert V.nnz == pv.view_pattern()[1].size
+    assert abs(V - ref).max() == 0.0
