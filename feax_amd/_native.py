@@ -106,6 +106,9 @@ SIGNATURES = {
     "fx_bicgstab_workspace_bytes": (_SZ, [_I32]),
     "fx_bicgstab_f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P, _P,
                                _P, _SZ]),
+    "fx_gmres_workspace_bytes": (_SZ, [_I32, _I32, _I32]),
+    "fx_gmres_f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _I32, _P, _P,
+                            _P, _P, _SZ]),
     "fx_gmg_eig_bounds_workspace_bytes": (_SZ, [_I32, _I32]),
     "fx_gmg_eig_bounds_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _I32, _P, _P, _P, _SZ]),
     "fx_gmg_vcycle": (_I32, [_P, _I32, _I32, _P, _I32, _I32, _F64, _P, _P]),

@@ -78,7 +78,7 @@ def adjoint_solve(problem, ts, tp, bc, v, keep, opts):
         keep["adjoint_iters"] = [info.iters]
         return lam
     if keep["form"] == "fp64":
-        if opts.solver == "bicgstab":
+        if opts.solver in ("bicgstab", "gmres"):
             # J_bc^T: the matrix itself under symmetric elimination, else its
             # transpose through the precomputed maps (feax/solvers/linear.py:
             # 442-456, feax/csr.py:130-141).

@@ -4,7 +4,8 @@ create_iterative_solve_fn(options) -> solve(A, b, x0) runs the HIP
 Jacobi-PCG (fx_pcg_f64 / fx_pcg_f32) with jax.scipy.sparse.linalg.cg
 stopping semantics (feax/solvers/common.py:213-239), or BiCGSTAB
 (fx_bicgstab_f64, jax bicgstab semantics, common.py:241-252) for
-solver="bicgstab".  Unlike feax's
+solver="bicgstab", or restarted GMRES (fx_gmres_f64, jax gmres semantics,
+common.py:254-265) for solver="gmres".  Unlike feax's
 matrix-free default, ``use_jacobi_preconditioner`` is honoured (the
 operator here is the assembled CSR).
 """
@@ -153,8 +154,48 @@ def bicgstab(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, din
     return x, PCGInfo(iters.value, rs.value)
 
 
+_GM_WS = {}
+
+
+def gmres(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, restart=20, maxiter=10000, dinv=None, block3=None,
+          check_every=1, out=None):
+    """Restarted GMRES on the device (fx_gmres_f64, FP64) with the iteration
+    and stopping rule of jax.scipy.sparse.linalg.gmres, solve_method=
+    'batched' (feax/solvers/common.py:254-265): left preconditioning with
+    M = Jacobi (dinv) or none, maxiter counts restart cycles.  Returns
+    (x, PCGInfo) with iters = cycles and rs = ||M (b - A x)||^2."""
+    import torch
+    if A.form != "csr" or A.data.dtype != torch.float64:
+        raise NotImplementedError("gmres runs on an FP64 CSR operator")
+    restart = 20 if restart is None else int(restart)
+    if not 1 <= restart <= 64:
+        raise ValueError(f"gmres restart must lie in [1, 64], got {restart}")
+    n = A.shape[0]
+    b = b.to(torch.float64).contiguous()
+    x = torch.empty(n, dtype=torch.float64, device=A.data.device) if out is None else out
+    x0 = None if x0 is None else x0.to(torch.float64).contiguous()
+    dinv = None if dinv is None else dinv.to(torch.float64).contiguous()
+    use_b3 = A.block3 if block3 is None else (bool(block3) and A.block3)
+    flags = nat.FX_PCG_BLOCK3 if use_b3 else 0
+    need = int(nat.lib().fx_gmres_workspace_bytes(n, restart, flags))
+    key = str(A.data.device)
+    ws = _GM_WS.get(key)
+    if ws is None or ws.numel() < need:
+        ws = torch.empty(need, dtype=torch.uint8, device=A.data.device)
+        _GM_WS[key] = ws
+    cycles = ctypes.c_int32(0)
+    rn = ctypes.c_double(0.0)
+    nat.call("fx_gmres_f64", nat.stream(), n, A.nse, flags, nat.ptr(A.indptr), nat.ptr(A.indices),
+             nat.ptr(A.bcol if use_b3 else None), nat.ptr(A.data), nat.ptr(dinv), nat.ptr(b), nat.ptr(x0),
+             float(tol), float(atol), restart, int(maxiter), int(check_every), nat.ptr(x), ctypes.byref(cycles),
+             ctypes.byref(rn), nat.ptr(ws), ws.numel())
+    return x, PCGInfo(cycles.value, rn.value * rn.value)
+
+
 def krylov(A, b, x0, opts, dinv=None, block3=None, out=None):
-    """The Krylov method opts.solver names ("cg" or "bicgstab") on A."""
+    """The Krylov method opts.solver names ("cg", "bicgstab" or "gmres") on A."""
+    if opts.solver == "gmres":
+        return gmres(A, b, x0, opts.tol, opts.atol, opts.restart, opts.maxiter, dinv, block3, out=out)
     if opts.solver == "bicgstab":
         return bicgstab(A, b, x0, opts.tol, opts.atol, opts.maxiter, dinv, block3, min(opts.check_every, 16),
                         out=out)
@@ -284,8 +325,8 @@ def create_iterative_solve_fn(options):
     if options.solver == "auto":
         raise ValueError("KrylovSolverOptions.solver is 'auto' (unresolved). "
                          "Call resolve_iterative_solver() before create_iterative_solve_fn().")
-    if options.solver not in ("cg", "bicgstab"):
-        raise NotImplementedError(f"{options.solver} is not on the HIP path (cg and bicgstab only)")
+    if options.solver not in ("cg", "bicgstab", "gmres"):
+        raise NotImplementedError(f"{options.solver} is not on the HIP path")
     if options.preconditioner is not None:
         raise NotImplementedError("custom preconditioners are not on the HIP path; GeometricMultigrid needs the "
                                   "problem and runs through create_solver / create_linear_solver")

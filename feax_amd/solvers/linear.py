@@ -9,8 +9,9 @@ computing u = u0 + du with J_bc du = -R_bc(u0) (linear.py:387-417):
   Jacobi-PCG.
 * preconditioner GeometricMultigrid: FP64 CG preconditioned by a V-cycle
   on the structured grid hierarchy (FP32 V-cycle with precision "mixed").
-* solver "bicgstab": FP64 CSR, HIP BiCGSTAB (jax bicgstab semantics): the
-  method for the non-symmetric operator of symmetric_elimination=False.
+* solver "bicgstab" / "gmres": FP64 CSR, HIP BiCGSTAB or restarted GMRES
+  (jax bicgstab / gmres semantics): the methods for the non-symmetric
+  operator of symmetric_elimination=False.
 * precision "mixed": FP32 CSR assembly and FP32 Jacobi-PCG (FP64 dot
   products) as the inner solver of an FP64 defect correction
   u <- u + K32^-1 (b - K u), where b - K u is the FP64 matrix-free residual
@@ -240,9 +241,10 @@ def create_linear_solver(problem, bc, solver_options=None, adjoint_solver_option
         u0 = _dev_vec(ig, ts) if ig is not None else _zero_like(problem, eff_bc, ts)
         if use_gmg:
             return solve_gmg(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
-        if solver_options.solver == "bicgstab" and (solver_options.precision != "fp64"
-                                                    or solver_options.operator != "csr"):
-            raise NotImplementedError("bicgstab runs on the FP64 CSR operator (precision='fp64', operator='csr')")
+        if solver_options.solver in ("bicgstab", "gmres") and (solver_options.precision != "fp64"
+                                                               or solver_options.operator != "csr"):
+            raise NotImplementedError(f"{solver_options.solver} runs on the FP64 CSR operator "
+                                      "(precision='fp64', operator='csr')")
         if solver_options.precision != "fp64" and solver_options.operator == "csr":
             return solve_mixed(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
         return solve_fp64(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)

@@ -378,6 +378,23 @@ int fx_bicgstab_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const i
                     int32_t check_every, double* x, int32_t* iters, double* rs, void* workspace, size_t ws_bytes);
 
 /* ------------------------------------------------------------------ */
+/* Restarted GMRES with the iteration and stopping rule of
+ * jax.scipy.sparse.linalg.gmres, solve_method='batched' (the gmres branch
+ * of feax's create_iterative_solve_fn, feax/solvers/common.py:254-265,
+ * restart = options.restart or 20): left-preconditioned (M = Jacobi dinv,
+ * or the identity when dinv == NULL), restart clipped to n, maxiter counts
+ * restart cycles, stops when ||M (b - A x)|| <= ||M b|| min(1, max(tol
+ * ||b||, atol) / ||b||).  FP64; flags as fx_bicgstab_f64.  1 <= restart
+ * <= 64.  *cycles = restart cycles run, *rn = ||M (b - A x)||.  workspace
+ * >= fx_gmres_workspace_bytes(n, restart, flags) (the (restart+1) x n
+ * Krylov basis included); 0 for an invalid restart. */
+size_t fx_gmres_workspace_bytes(int32_t n, int32_t restart, int32_t flags);
+int fx_gmres_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr, const int32_t* indices,
+                 const int32_t* bcol, const double* data, const double* dinv, const double* b, const double* x0,
+                 double tol, double atol, int32_t restart, int32_t maxiter, int32_t check_every, double* x,
+                 int32_t* cycles, double* rn, void* workspace, size_t ws_bytes);
+
+/* ------------------------------------------------------------------ */
 /* Geometric multigrid on a structured HEX8 box (StructuredGrid numbering,
  * feax/spgrid.py:28-41).  The analogue of feax's NarrowBandCMG
  * (feax/solvers/cmg.py:95-300: rediscretised coarse operators on cell

@@ -19,6 +19,14 @@ Restates:
     s.s < atol2: x += alpha phat, r = s; else shat = M s, t = A shat,
     omega = t.s/t.t, x += alpha phat + omega shat, r = s - omega t;
     breakdown when alpha = 0 or omega = 0 (after a full step).
+  jax.scipy.sparse.linalg.gmres (third-party; called at
+    feax/solvers/common.py:254-265 with restart 20 unless options.restart),
+    restated from the published algorithm with its default
+    solve_method='batched': left preconditioning, maxiter counts restart
+    cycles, stop on ||M r|| <= ||M b|| min(1, max(tol ||b||, atol)/||b||);
+    each cycle runs up to `restart` Arnoldi steps (one classical
+    Gram-Schmidt pass, breakdown below eps ||M A v||) and solves the
+    projected least squares problem through its normal equations.
   feax/solvers/common.py:86-94   M(x) = x / (diag + jacobi_shift)
   feax/solvers/common.py:23-53   x0 = lift of BC increments
   feax/solvers/common.py:376-395 check_convergence: NaN unless
@@ -31,9 +39,13 @@ residual below the threshold, iteration cap honoured, maxiter=0 returns x0),
 and exact convergence in <= n steps on small SPD systems; bicgstab by the
 direct solve of non-symmetric systems (the row-only Dirichlet elimination
 and a random non-symmetric matrix), by agreeing with scipy's own
-bicgstab's solution, and by one exact step on A = I.  The iterate
-trajectories of JAX's own cg and bicgstab are parity unpinned (need an
-import).
+bicgstab's solution, and by one exact step on A = I; gmres by the
+direct solve, by the exact solution after one cycle when restart >= n, by
+the minimal-residual property of a cycle (its residual equals the least
+squares optimum over x0 + K_m(MA, M r0) computed from an explicit Krylov
+basis), by the Arnoldi relation M A V_m = V_{m+1} H^T and by a breakdown
+on A = I.  The iterate trajectories of JAX's own cg, bicgstab and gmres
+are parity unpinned (need an import).
 """
 
 import numpy as np
@@ -102,6 +114,82 @@ def bicgstab(A, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=None):
         rs = float(r @ r)
         k += 1
     return x, k, rs
+
+
+def _safe_normalize(v, thresh=None):
+    """(v / ||v||, ||v||), or (0, 0) when ||v|| <= thresh (default: machine
+    epsilon, an absolute threshold) -- jax's GMRES helper."""
+    nrm = float(np.sqrt(v @ v))
+    if thresh is None:
+        thresh = np.finfo(np.float64).eps
+    if nrm > thresh:
+        return v / nrm, nrm
+    return np.zeros_like(v), 0.0
+
+
+def gmres_cycle(matvec, M, b, x, unit_residual, residual_norm, restart):
+    """One restart cycle of jax.scipy.sparse.linalg.gmres, solve_method=
+    'batched': Arnoldi on M A from the unit preconditioned residual (one
+    classical Gram-Schmidt pass; a step whose orthogonalised norm is at most
+    eps * ||M A v|| is a breakdown and ends the cycle), then the least
+    squares problem min ||beta e0 - H^T y|| solved through its normal
+    equations (H H^T) y = H beta by Cholesky, H = eye(restart, restart+1)
+    with row j overwritten by step j; x += V[:, :restart] y; the new
+    preconditioned residual M (b - A x) is normalised.
+    Returns (x, unit_residual, residual_norm, steps)."""
+    n = b.size
+    eps = np.finfo(np.float64).eps
+    V = np.zeros((n, restart + 1))
+    V[:, 0] = unit_residual
+    H = np.eye(restart, restart + 1)
+    j = 0
+    breakdown = False
+    while j < restart and not breakdown:
+        w = M(matvec(V[:, j]))
+        _, w0 = _safe_normalize(w)
+        h = V.T @ w
+        w = w - V @ h
+        unit_w, w1 = _safe_normalize(w, eps * w0)
+        V[:, j + 1] = unit_w
+        h[j + 1] = w1
+        H[j, :] = h
+        breakdown = w1 == 0.0
+        j += 1
+    beta = np.zeros(restart + 1)
+    beta[0] = residual_norm
+    a = H.T
+    L = np.linalg.cholesky(a.T @ a)
+    y = np.linalg.solve(L.T, np.linalg.solve(L, a.T @ beta))
+    x = x + V[:, :-1] @ y
+    unit, nrm = _safe_normalize(M(b - matvec(x)))
+    return x, unit, nrm, j
+
+
+def gmres(A, b, x0=None, tol=1e-10, atol=1e-10, restart=20, maxiter=10000, dinv=None):
+    """Return (x, cycles, ||M (b - A x)||) following jax.scipy.sparse.linalg.
+    gmres (solve_method='batched', its default; called at
+    feax/solvers/common.py:254-265 with restart = options.restart or 20):
+    left-preconditioned, restart = min(restart, n), maxiter counts restart
+    cycles, and the loop runs while cycles < maxiter and the preconditioned
+    residual norm exceeds ptol = ||M b|| * min(1, max(tol ||b||, atol) / ||b||)
+    (NaN, so no cycle, when b = 0 and atol = 0)."""
+    matvec = A if callable(A) else (lambda v: A @ v)
+    M = (lambda v: v) if dinv is None else (lambda v: dinv * v)
+    n = b.size
+    restart = min(restart, n)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=float)
+    bn = float(np.sqrt(b @ b))
+    atol_eff = max(tol * bn, atol)
+    Mb = M(b)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ratio = np.float64(atol_eff) / np.float64(bn)
+    ptol = float(np.sqrt(Mb @ Mb)) * (np.nan if np.isnan(ratio) else min(1.0, float(ratio)))
+    unit, rn = _safe_normalize(M(b - matvec(x)))
+    k = 0
+    while k < maxiter and rn > ptol:
+        x, unit, rn, _ = gmres_cycle(matvec, M, b, x, unit, rn, restart)
+        k += 1
+    return x, k, rn
 
 
 def jacobi_dinv(diag, shift=1e-12):

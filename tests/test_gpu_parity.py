@@ -435,3 +435,125 @@ def test_matrix_view_csr_is_the_triangle_of_the_full_matrix(view):
     ref = sp.triu(F) if view == "UPPER" else sp.tril(F)
     assert V.nnz == pv.view_pattern()[1].size
     assert abs(V - ref).max() == 0.0
+
+
+def _simp_box(dims, seed=3):
+    import feax_amd as fe
+
+    class Simp(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.SIMPLinearElastic(nu=0.3, E0=1.0, Emin=1e-3, penal=3.0)
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.0, 0.0, 1.0))]
+
+    nx = dims[0]
+    prob = Simp(fe.StructuredGrid(dims).to_mesh(), vec=3, dim=3, location_fns=[lambda x: np.isclose(x[0], nx)])
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda x: np.isclose(x[0], 0.0), "all", 0.0)]).create_bc(prob)
+    rho = np.random.default_rng(seed).uniform(0.3, 1.0, prob.num_cells)
+    return prob, bc, rho
+
+
+@pytest.mark.parametrize("dims,restart,spmv", [((6, 4, 4), 20, "block3"), ((10, 6, 6), 20, "csr"),
+                                               ((10, 6, 6), 64, "block3")])
+def test_gmres_nonsymmetric_elimination_matches_oracle(dims, restart, spmv):
+    """solver="gmres" with symmetric_elimination=False: the solution equals
+    the direct solve of the row-only elimination and the restart-cycle count
+    the oracle's GMRES (jax gmres semantics, Jacobi, left-preconditioned)."""
+    import torch
+    import scipy.sparse.linalg as spla
+    import feax_amd as fe
+    from oracle import hex8, krylov
+    prob, bc, rho = _simp_box(dims)
+    opts = fe.KrylovSolverOptions(solver="gmres", tol=1e-10, atol=0.0, restart=restart,
+                                  use_jacobi_preconditioner=True, spmv=spmv)
+    solver = fe.create_solver(prob, bc, opts, linear=True, symmetric_elimination=False)
+    t = fe.TracedParams.create_uniform_surface_var(prob, 0.5)
+    u = np.asarray(solver(fe.TracedParams(volume_vars=(torch.as_tensor(rho, device="cuda"),), surface_vars=[(t,)])))
+    m = hex8.Model(prob.points, prob.cells.astype(np.int64), prob.boundary_inds_list)
+    mat = hex8.Material("simp", nu=0.3, E0=1.0, Emin=1e-3, penal=3.0)
+    u0 = np.zeros(m.n)
+    mask = np.zeros(m.n, bool)
+    mask[bc.bc_rows] = True
+    K = m.scipy_csr(hex8.apply_bc_csr(m.csr_values(u0, rho, mat), m.csr, mask, bc.bc_rows, symmetric=False))
+    b = -hex8.apply_bc_res(m.residual(u0, rho, mat, [hex8.constant_traction(prob.boundary_inds_list[0], 0.5)]),
+                           u0, bc.bc_rows, bc.bc_vals)
+    xd = spla.spsolve(K.tocsc(), b)
+    _, k_ref, _ = krylov.gmres(K, b, np.zeros(m.n), 1e-10, 0.0, restart, 10000, krylov.jacobi_dinv(K.diagonal()))
+    err = np.linalg.norm(u - xd) / np.linalg.norm(xd)
+    print("gmres", dims, restart, spmv, solver.stats, "err", err, "oracle cycles", k_ref)
+    assert err < 1e-8
+    assert abs(solver.stats.iters - k_ref) <= max(1, 0.1 * k_ref)
+
+
+def test_gmres_kernel_matches_oracle_on_a_general_sparse_matrix():
+    """fx_gmres_f64 on a random non-symmetric scalar CSR against the oracle:
+    same cycles and iterate; restart clipped to n (one exact cycle); a
+    breakdown at the first step (A = I); maxiter = 0 and b = 0 return x0."""
+    import torch
+    import scipy.sparse as sp
+    from oracle import krylov
+    from feax_amd.csr import CSRMatrix
+    from feax_amd.solvers.common import gmres
+
+    def dev(K):
+        K = K.tocsr()
+        K.sort_indices()
+        return CSRMatrix(torch.as_tensor(K.data, device="cuda"), torch.as_tensor(K.indptr.astype(np.int32),
+                         device="cuda"), torch.as_tensor(K.indices.astype(np.int32), device="cuda"), K.shape)
+
+    rng = np.random.default_rng(11)
+    n = 300
+    K = (sp.random(n, n, density=0.02, random_state=12) + sp.diags(2.0 + rng.random(n))).tocsr()
+    b = rng.standard_normal(n)
+    x0 = rng.standard_normal(n)
+    dinv = 1.0 / K.diagonal()
+    A = dev(K)
+    tb, tx0, td = (torch.as_tensor(v, device="cuda") for v in (b, x0, dinv))
+    for restart in (5, 20):
+        x, info = gmres(A, tb, tx0, 1e-11, 0.0, restart, 10000, td)
+        xr, kr, rnr = krylov.gmres(K, b, x0, 1e-11, 0.0, restart, 10000, dinv)
+        print("restart", restart, "cycles", info.iters, kr, "rn", np.sqrt(info.rs), rnr)
+        assert abs(info.iters - kr) <= 1
+        assert np.linalg.norm(_np(x) - xr) <= 1e-9 * np.linalg.norm(xr)
+    # no preconditioner, fixed cycle count: the same iterate as the oracle
+    x, info = gmres(A, tb, None, 1e-30, 0.0, 7, 3, None)
+    xr, kr, _ = krylov.gmres(K, b, None, 1e-30, 0.0, 7, 3, None)
+    assert info.iters == kr == 3
+    assert np.abs(_np(x) - xr).max() <= 1e-10 * np.abs(xr).max()
+    # restart >= n: one cycle solves exactly
+    small = (sp.random(40, 40, density=0.1, random_state=3) + sp.diags(3.0 + rng.random(40))).tocsr()
+    bs = rng.standard_normal(40)
+    x, info = gmres(dev(small), torch.as_tensor(bs, device="cuda"), None, 1e-12, 0.0, 64, 100, None)
+    assert info.iters == 1
+    assert np.linalg.norm(small @ _np(x) - bs) <= 1e-10 * np.linalg.norm(bs)
+    # A = I: breakdown at the first Arnoldi step, x = b
+    x, info = gmres(dev(sp.identity(n)), tb, None, 1e-12, 0.0, 20, 100, None)
+    assert info.iters == 1 and np.abs(_np(x) - b).max() <= 1e-15 * np.abs(b).max()
+    # maxiter = 0, and b = 0 with atol = 0 (NaN bound): x0 back, no cycle
+    x, info = gmres(A, tb, tx0, 1e-10, 0.0, 20, 0, td)
+    assert info.iters == 0 and torch.equal(x, tx0)
+    x, info = gmres(A, torch.zeros_like(tb), tx0, 1e-10, 0.0, 20, 100, td)
+    assert info.iters == 0 and torch.equal(x, tx0)
+    with pytest.raises(ValueError, match="restart"):
+        gmres(A, tb, None, 1e-10, 0.0, 65, 10, None)
+
+
+def test_gmres_gradient_nonsymmetric_elimination_matches_cg():
+    """Adjoint through GMRES on J^T (transpose maps) equals CG's gradient."""
+    import torch
+    import feax_amd as fe
+    prob, bc, rho = _simp_box((8, 4, 4), seed=6)
+    t = fe.TracedParams.create_uniform_surface_var(prob, 0.5)
+    grads = []
+    for solver_name, sym in (("gmres", False), ("cg", True)):
+        opts = fe.KrylovSolverOptions(solver=solver_name, tol=1e-12, atol=0.0, restart=64,
+                                      use_jacobi_preconditioner=True)
+        solver = fe.create_solver(prob, bc, opts, linear=True, symmetric_elimination=sym, return_solution=False)
+        r = torch.as_tensor(rho, device="cuda").requires_grad_(True)
+        u = solver(fe.TracedParams(volume_vars=(r,), surface_vars=[(t,)]))
+        (u * u).sum().backward()
+        grads.append(_np(r.grad))
+    err = np.abs(grads[0] - grads[1]).max() / np.abs(grads[1]).max()
+    print("gradient rel diff gmres/row-only vs cg/symmetric", err)
+    assert err < 1e-7

@@ -316,3 +316,83 @@ def test_bicgstab_oracle_on_row_only_dirichlet_elimination():
     du, k, _ = krylov.bicgstab(K, b, x0, tol=1e-12, atol=0.0, dinv=krylov.jacobi_dinv(K.diagonal()))
     assert np.linalg.norm(u0 + du - u_s) <= 1e-8 * np.linalg.norm(u_s)
     assert np.linalg.norm(du - spla.spsolve(K.tocsc(), b)) <= 1e-8 * np.linalg.norm(du)
+
+
+def _nonsym(n, seed):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    A = (sp.random(n, n, density=0.15, random_state=seed) + sp.diags(3.0 + rng.random(n))).tocsr()
+    return A, rng.standard_normal(n)
+
+
+def test_gmres_cycle_is_the_minimal_residual_over_the_krylov_space():
+    """One cycle of m steps returns x0 + the minimiser of ||M (b - A x)||
+    over x0 + K_m(MA, M r0), computed here from an explicit basis."""
+    A, b = _nonsym(40, 6)
+    dinv = 1.0 / A.diagonal()
+    x0 = np.random.default_rng(1).standard_normal(40)
+    r0 = dinv * (b - A @ x0)
+    u, rn = krylov._safe_normalize(r0)
+    for m in (1, 3, 8):
+        x, _, rn1, steps = krylov.gmres_cycle(lambda v: A @ v, lambda v: dinv * v, b, x0, u, rn, m)
+        K = np.empty((40, m))
+        v = r0.copy()
+        for j in range(m):
+            K[:, j] = v / np.linalg.norm(v)
+            v = dinv * (A @ K[:, j])
+        Q, _ = np.linalg.qr(K)
+        B = dinv[:, None] * (A @ Q)
+        y, *_ = np.linalg.lstsq(B, r0, rcond=None)
+        xs = x0 + Q @ y
+        assert steps == m
+        assert abs(rn1 - np.linalg.norm(dinv * (b - A @ xs))) <= 1e-10 * rn
+        assert np.linalg.norm(x - xs) <= 1e-9 * np.linalg.norm(xs)
+
+
+def test_gmres_oracle_solves_and_stops_like_jax():
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    A, b = _nonsym(60, 4)
+    dinv = 1.0 / A.diagonal()
+    xd = spla.spsolve(A.tocsc(), b)
+    # restart >= n: one cycle is exact (restart is clipped to n)
+    x, k, rn = krylov.gmres(A, b, tol=1e-12, atol=0.0, restart=100, dinv=dinv)
+    assert k == 1 and np.linalg.norm(x - xd) <= 1e-10 * np.linalg.norm(xd)
+    # restarted: converges, stops once ||M r|| <= ptol = ||M b|| tol
+    x, k, rn = krylov.gmres(A, b, tol=1e-11, atol=0.0, restart=5, dinv=dinv)
+    assert k > 1 and rn <= np.linalg.norm(dinv * b) * 1e-11
+    assert abs(rn - np.linalg.norm(dinv * (b - A @ x))) <= 1e-14 * np.linalg.norm(dinv * b)
+    assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
+    # ... and that cycle count is the first at which the bound holds
+    xp, kp, rnp = krylov.gmres(A, b, tol=1e-11, atol=0.0, restart=5, dinv=dinv, maxiter=k - 1)
+    assert kp == k - 1 and rnp > np.linalg.norm(dinv * b) * 1e-11
+    # A = I: breakdown at the first step, x = b after one cycle
+    xi, ki, rni = krylov.gmres(sp.identity(60).tocsr(), b, tol=1e-12, atol=0.0)
+    assert ki == 1 and np.allclose(xi, b, rtol=1e-15, atol=1e-15) and rni <= 1e-15
+    # maxiter = 0 returns x0; b = 0 with atol = 0 runs no cycle (NaN bound)
+    x3, k3, _ = krylov.gmres(A, b, x0=np.ones(60), maxiter=0)
+    assert k3 == 0 and np.all(x3 == 1.0)
+    x4, k4, _ = krylov.gmres(A, np.zeros(60), x0=np.ones(60), tol=1e-10, atol=0.0)
+    assert k4 == 0 and np.all(x4 == 1.0)
+    # b = 0 with atol > 0: ptol = ||M b|| = 0, runs to the exact zero
+    x5, k5, rn5 = krylov.gmres(A, np.zeros(60), x0=np.ones(60), tol=1e-10, atol=1e-10, restart=60)
+    assert k5 >= 1 and np.abs(x5).max() < 1e-12
+
+
+def test_gmres_oracle_on_row_only_dirichlet_elimination():
+    import scipy.sparse.linalg as spla
+    from oracle import hex8, structure
+    pts, cells = structure.structured_grid((4, 2, 2), (1.0, 1.0, 1.0))
+    faces = structure.boundary_faces(pts, cells, lambda p: np.isclose(p[0], 4.0))
+    m = hex8.Model(pts, cells, [faces])
+    rows, vals = structure.dirichlet_rows(pts, [(lambda p: np.isclose(p[0], 0.0), "all", 0.0)])
+    mat = hex8.Material("linear_elastic", nu=0.3)
+    E = np.full(m.N, 70e3)
+    u0 = np.zeros(m.n)
+    mask = np.zeros(m.n, bool)
+    mask[rows] = True
+    K = m.scipy_csr(hex8.apply_bc_csr(m.csr_values(u0, E, mat), m.csr, mask, rows, symmetric=False))
+    b = -hex8.apply_bc_res(m.residual(u0, E, mat, [hex8.constant_traction(faces, 1e-3)]), u0, rows, vals)
+    du, k, _ = krylov.gmres(K, b, np.zeros(m.n), tol=1e-12, atol=0.0, dinv=krylov.jacobi_dinv(K.diagonal()))
+    xd = spla.spsolve(K.tocsc(), b)
+    assert np.linalg.norm(du - xd) <= 1e-9 * np.linalg.norm(xd) and k > 1
