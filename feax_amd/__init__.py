@@ -2,8 +2,8 @@
 
 Drop-in for the KrylovSolverOptions route of feax.create_solver: Problem,
 DirichletBCSpec / DirichletBCConfig, TracedParams, TracedStructure,
-KrylovSolverOptions, create_solver, Solution, zero_like_initial_guess,
-CSRMatrix.  All compute runs in libfeaxhip.so (hand-written HIP for
+KrylovSolverOptions, DirectSolverOptions, create_solver, Solution,
+zero_like_initial_guess, CSRMatrix.  All compute runs in libfeaxhip.so (hand-written HIP for
 gfx950); the package itself holds only host-side structure and dispatch.
 """
 
@@ -17,7 +17,7 @@ from .problem import MatrixView, Problem
 from .solution import Solution
 from .solver import create_solver
 from .solvers.gmg import GeometricMultigrid
-from .solvers.options import KrylovSolverOptions, MatrixProperty, NewtonOptions
+from .solvers.options import DirectSolverOptions, KrylovSolverOptions, MatrixProperty, NewtonOptions
 from .traced_params import TracedParams
 from .utils import zero_like_initial_guess
 

@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libfeaxhip.so")
 
-SOURCES = ["pattern.cpp", "krylov.hip", "hex8.hip", "adjoint.hip", "batch.hip", "gmg.hip", "bicgstab.hip", "gmres.hip"]
+SOURCES = ["pattern.cpp", "krylov.hip", "hex8.hip", "adjoint.hip", "batch.hip", "gmg.hip", "bicgstab.hip", "gmres.hip",
+           "direct.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -23,6 +24,10 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+
+
+# rocBLAS / rocSOLVER: the dense blocks of the banded Cholesky (direct.hip).
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def hipcc():
@@ -36,7 +41,7 @@ def build(verbose=False):
     os.makedirs(OUT_DIR, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs, "-o", tmp]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs, *LIBS, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -106,6 +106,11 @@ SIGNATURES = {
     "fx_bicgstab_workspace_bytes": (_SZ, [_I32]),
     "fx_bicgstab_f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P, _P,
                                _P, _SZ]),
+    "fx_band_ldab": (_I64, [_I32]),
+    "fx_band_workspace_bytes": (_SZ, [_I32]),
+    "fx_band_from_csr_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _I32, _I64, _P, _P, _SZ]),
+    "fx_band_cholesky_f64": (_I32, [_P, _I32, _I32, _I64, _P, _P, _P, _SZ]),
+    "fx_band_solve_f64": (_I32, [_P, _I32, _I32, _I64, _P, _P, _P, _P, _P, _SZ]),
     "fx_gmres_workspace_bytes": (_SZ, [_I32, _I32, _I32]),
     "fx_gmres_f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _I32, _P, _P,
                             _P, _P, _SZ]),

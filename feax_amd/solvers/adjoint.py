@@ -67,6 +67,25 @@ def adjoint_solve(problem, ts, tp, bc, v, keep, opts):
     if bc is not None and len(bc.bc_rows):
         rhs.index_fill_(0, bc.device(ts.device)[0].long(), 0.0)
     A, dinv = keep["A"], keep["dinv"]
+    from .options import DirectSolverOptions
+    if isinstance(opts, DirectSolverOptions) or keep["form"] == "direct":
+        from .direct import BandCholesky, direct_adjoint
+        if keep["form"] != "direct":
+            # a direct adjoint of an iterative forward: factor the FP64 J_bc
+            if A is None or A.data.dtype != torch.float64 or A.form != "csr":
+                from ..assembler import assemble_csr
+                A = assemble_csr(problem, ts, tp, keep.get("u0"), bc, True, torch.float64)
+            keep.update(form="direct", A=A, factor=BandCholesky(problem, ts.device).factor(A))
+        if isinstance(opts, DirectSolverOptions):
+            keep["adjoint_iters"] = [0]
+            return direct_adjoint(problem, ts, keep, rhs)
+        # an iterative adjoint of a direct forward: Krylov on the kept J_bc
+        if opts.use_jacobi_preconditioner:
+            from .common import create_jacobi_preconditioner
+            dinv = create_jacobi_preconditioner(A, opts.jacobi_shift)
+        lam, info = krylov(A, rhs, None, opts, dinv)
+        keep["adjoint_iters"] = [info.iters]
+        return lam
     if keep["form"] == "matfree":
         from .common import pcg_matfree
         lam, info = pcg_matfree(problem, ts, tp, None, bc, rhs, None, opts.tol, opts.atol, opts.maxiter,

@@ -30,7 +30,7 @@ from ..assembler import TracedStructure, assemble_csr, residual
 from ..DCboundary import DirichletBC
 from .common import (KrylovRecord, check_convergence, create_iterative_solve_fn, create_jacobi_preconditioner,
                      create_x0, device_norm, krylov, pcg)
-from .options import KrylovSolverOptions, MatrixProperty, resolve_iterative_solver
+from .options import DirectSolverOptions, KrylovSolverOptions, MatrixProperty, resolve_iterative_solver
 
 
 class SolveStats:
@@ -214,19 +214,25 @@ def create_linear_solver(problem, bc, solver_options=None, adjoint_solver_option
                          symmetric_elimination=True, traced_structure=None):
     if solver_options is None:
         solver_options = KrylovSolverOptions()
-    if not isinstance(solver_options, KrylovSolverOptions):
-        raise NotImplementedError("only KrylovSolverOptions (cg) are on the HIP path; direct solvers are not")
-    solver_options = resolve_iterative_solver(solver_options, MatrixProperty.SPD)
+    for o in (solver_options, adjoint_solver_options):
+        if o is not None and not isinstance(o, (KrylovSolverOptions, DirectSolverOptions)):
+            raise TypeError(f"Expected DirectSolverOptions or KrylovSolverOptions, got {type(o).__name__}.")
+    use_direct = isinstance(solver_options, DirectSolverOptions)
+    if use_direct and not symmetric_elimination:
+        raise NotImplementedError("the direct solver factors the symmetric elimination's SPD J_bc "
+                                  "(symmetric_elimination=True)")
+    if not use_direct:
+        solver_options = resolve_iterative_solver(solver_options, MatrixProperty.SPD)
     if adjoint_solver_options is None:
         adjoint_solver_options = solver_options
-    else:
+    elif isinstance(adjoint_solver_options, KrylovSolverOptions):
         adjoint_solver_options = resolve_iterative_solver(adjoint_solver_options, MatrixProperty.SPD)
     from .gmg import GeometricMultigrid
-    use_gmg = isinstance(solver_options.preconditioner, GeometricMultigrid)
+    use_gmg = not use_direct and isinstance(solver_options.preconditioner, GeometricMultigrid)
     if use_gmg:
         if solver_options.solver != "cg" or solver_options.operator != "csr":
             raise NotImplementedError("GeometricMultigrid preconditions cg on the assembled operator")
-    else:
+    elif not use_direct:
         create_iterative_solve_fn(solver_options)      # validates the options
     default_bc = bc
     ts_default = traced_structure
@@ -239,6 +245,9 @@ def create_linear_solver(problem, bc, solver_options=None, adjoint_solver_option
         if keep is not None:
             keep["ts"] = ts
         u0 = _dev_vec(ig, ts) if ig is not None else _zero_like(problem, eff_bc, ts)
+        if use_direct:
+            from .direct import solve_direct
+            return solve_direct(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
         if use_gmg:
             return solve_gmg(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
         if solver_options.solver in ("bicgstab", "gmres") and (solver_options.precision != "fp64"

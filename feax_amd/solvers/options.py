@@ -94,6 +94,29 @@ class KrylovSolverOptions(AbstractSolverOptions):
         return True
 
 
+@dataclass(frozen=True)
+class DirectSolverOptions(AbstractSolverOptions):
+    """feax.solvers.options.DirectSolverOptions (options.py:595-635).  Every
+    solver name runs the device banded Cholesky of the SPD J_bc
+    (symmetric elimination): "auto" and "cudss" (feax's GPU solver), and the
+    host solvers "spsolve", "umfpack", "cholmod" alike.  cudss_options and
+    sksparse_options are accepted and unused; the forward factor is always
+    reused by the adjoint solve (J_bc is symmetric), which feax does under
+    reuse_factorization."""
+    solver: str = "auto"
+    cudss_options: Optional[object] = None
+    sksparse_options: Optional[object] = None
+    reuse_factorization: bool = False
+
+    def __post_init__(self):
+        valid = ("auto", "cudss", "spsolve", "umfpack", "cholmod")
+        if self.solver not in valid:
+            raise ValueError(f"Invalid direct solver: {self.solver}. Choose from {valid}")
+
+    def uses_x0(self) -> bool:
+        return False
+
+
 def resolve_iterative_solver(options, matrix_property):
     """'auto' -> cg / bicgstab / gmres by matrix property (options.py:837-889)."""
     if options.solver != "auto":

@@ -30,6 +30,7 @@ extern "C" {
 #define FX_EINVAL 1001     /* bad argument / shape */
 #define FX_ELIMIT 1002     /* a size limit of a kernel exceeded */
 #define FX_EWORKSPACE 1003 /* workspace too small */
+#define FX_ELIB 1004       /* a rocBLAS / rocSOLVER call failed */
 
 /* Material kinds understood by the element kernels. */
 #define FX_MAT_LINEAR_ELASTIC 0 /* sigma = lam tr(eps) I + 2 mu eps, E = param        */
@@ -393,6 +394,39 @@ int fx_gmres_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int3
                  const int32_t* bcol, const double* data, const double* dinv, const double* b, const double* x0,
                  double tol, double atol, int32_t restart, int32_t maxiter, int32_t check_every, double* x,
                  int32_t* cycles, double* rn, void* workspace, size_t ws_bytes);
+
+/* ------------------------------------------------------------------ */
+/* Direct solve of the SPD J_bc (symmetric elimination): banded Cholesky on
+ * the device, the stand-in for feax's GPU direct solver (DirectSolverOptions
+ * "auto" / "cudss", feax/solvers/common.py:116-210; the direct branch of
+ * create_linear_solver, feax/solvers/linear.py:308-417: u = u0 +
+ * J_bc^-1 (-R_bc(u0)), the factor reused by the adjoint since J_bc is
+ * symmetric, linear.py:351-378).
+ *
+ * Band storage (LAPACK lower band): column j at ab + j * ldab, entry (i, j)
+ * (0 <= i - j <= bw) at offset i - j, in the caller's ordering: row i' of
+ * the band is row perm[i'] of the CSR (inv_perm[perm[i']] = i'; NULL =
+ * identity).  ab holds n * ldab doubles, ldab >= fx_band_ldab(bw).
+ * workspace >= fx_band_workspace_bytes(n) for each call.
+ *
+ * fx_band_from_csr_f64: zero ab and scatter the lower triangle of the
+ *   (symmetric) CSR; FX_EINVAL if an entry lies past bw.
+ * fx_band_cholesky_f64: in-place L (A = L L^T), blocked right-looking
+ *   (rocSOLVER potrf on 128-column diagonal blocks, rocBLAS trsm / syrk on
+ *   the band below); *info = 0, or the 1-based column of the first
+ *   non-positive pivot (the matrix is not positive definite).  Uses one
+ *   rocBLAS handle per device, set to `stream` at each call.
+ * fx_band_solve_f64: x = P^T L^-T L^-1 P b with (P b)[i'] = b[perm[i']];
+ *   x may alias b. */
+int64_t fx_band_ldab(int32_t bw);
+size_t fx_band_workspace_bytes(int32_t n);
+int fx_band_from_csr_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, const double* data,
+                         const int32_t* inv_perm, int32_t bw, int64_t ldab, double* ab, void* workspace,
+                         size_t ws_bytes);
+int fx_band_cholesky_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, double* ab, int32_t* info,
+                         void* workspace, size_t ws_bytes);
+int fx_band_solve_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, const double* ab, const int32_t* perm,
+                      const double* b, double* x, void* workspace, size_t ws_bytes);
 
 /* ------------------------------------------------------------------ */
 /* Geometric multigrid on a structured HEX8 box (StructuredGrid numbering,

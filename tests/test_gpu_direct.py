@@ -1,0 +1,164 @@
+"""GPU parity of the direct solver (banded Cholesky, fx_band_*).
+
+The factor against the matrix it factors (L L^T = P J P^T), the solution
+against scipy's direct solve of the oracle system (c1 and a box whose
+bandwidth spans several factor blocks), gradients against the CG path, the
+RCM ordering of a shuffled mesh, and the non-positive-definite report.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import cantilever, oracle_model
+from test_structure_host import shuffled_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _simp(dims=None, pts_cells=None, seed=3):
+    import feax_amd as fe
+
+    class Simp(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.SIMPLinearElastic(nu=0.3, E0=1.0, Emin=1e-3, penal=3.0)
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.0, 0.0, 1.0))]
+
+    mesh = fe.StructuredGrid(dims).to_mesh() if dims is not None else fe.Mesh(*pts_cells)
+    xmax = mesh.points[:, 0].max()
+    prob = Simp(mesh, vec=3, dim=3, location_fns=[lambda x: np.isclose(x[0], xmax)])
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda x: np.isclose(x[0], 0.0), "all", 0.0)]).create_bc(prob)
+    rho = np.random.default_rng(seed).uniform(0.3, 1.0, prob.num_cells)
+    return prob, bc, rho
+
+
+def _tp(prob, rho, t=0.5, grad=False):
+    import torch
+    import feax_amd as fe
+    r = torch.as_tensor(rho, device="cuda")
+    if grad:
+        r.requires_grad_(True)
+    return fe.TracedParams(volume_vars=(r,), surface_vars=[(fe.TracedParams.create_uniform_surface_var(prob, t),)]), r
+
+
+def _oracle_system(prob, bc, rho, t=0.5):
+    from oracle import hex8, krylov
+    m = hex8.Model(prob.points, prob.cells.astype(np.int64), prob.boundary_inds_list)
+    mat = hex8.Material("simp", nu=0.3, E0=1.0, Emin=1e-3, penal=3.0)
+    tr = [hex8.constant_traction(prob.boundary_inds_list[0], t)]
+    _, info = krylov.linear_solve(m, mat, rho, tr, bc.bc_rows, bc.bc_vals, maxiter=1)
+    return info["K"], info["b"]
+
+
+@pytest.mark.parametrize("case", ["grid", "shuffled"])
+def test_band_factor_reproduces_the_matrix(case):
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import assemble_csr
+    from feax_amd.solvers.direct import BandCholesky
+    if case == "grid":
+        prob, bc, rho = _simp((6, 3, 4))
+    else:
+        prob, bc, rho = _simp(pts_cells=shuffled_mesh((6, 3, 3), 5))
+    ts = fe.TracedStructure.from_problem(prob)
+    tp, _ = _tp(prob, rho)
+    J = assemble_csr(prob, ts, tp, None, bc, True, torch.float64)
+    F = BandCholesky(prob, ts.device).factor(J)
+    torch.cuda.synchronize()
+    n, ldab, bw = F.n, F.ldab, F.bw
+    ab = _np(F.ab[: n * ldab]).reshape(n, ldab)
+    L = np.zeros((n, n))
+    for j in range(n):
+        m = min(bw + 1, n - j)
+        L[j:j + m, j] = ab[j, :m]
+    import scipy.sparse as sp
+    K = sp.csr_matrix((_np(J.data), _np(J.indices), _np(J.indptr)), shape=(n, n)).toarray()
+    perm = np.arange(n) if F.perm is None else _np(F.perm)
+    PK = K[np.ix_(perm, perm)]
+    err = np.abs(L @ L.T - PK).max() / np.abs(K).max()
+    print(case, "n", n, "bw", bw, "ordering", "natural" if F.perm is None else "rcm", "|LL^T - PKP^T|", err)
+    assert err < 1e-13
+    assert (case == "shuffled") == (F.perm is not None)
+
+
+@pytest.mark.parametrize("case", ["c1", "grid12x8x8"])
+def test_direct_solve_matches_scipy_direct_solve(case):
+    import scipy.sparse.linalg as spla
+    import feax_amd as fe
+    if case == "c1":
+        prob, bc, tp = cantilever((50, 5, 5), (2.0, 2.0, 2.0), 100.0)
+        from oracle import hex8, krylov
+        m = oracle_model(prob)
+        mat = hex8.Material("linear_elastic", nu=0.3)
+        _, info = krylov.linear_solve(m, mat, np.full(m.N, 70e3),
+                                      [hex8.constant_traction(prob.boundary_inds_list[0], 1e-3)],
+                                      bc.bc_rows, bc.bc_vals, maxiter=1)
+        K, b = info["K"], info["b"]
+    else:
+        prob, bc, rho = _simp((12, 8, 8))
+        tp, _ = _tp(prob, rho)
+        K, b = _oracle_system(prob, bc, rho)
+    solver = fe.create_solver(prob, bc, fe.DirectSolverOptions(), linear=True)
+    u = np.asarray(solver(tp))
+    ud = spla.spsolve(K.tocsc(), b)
+    err = np.linalg.norm(u - ud) / np.linalg.norm(ud)
+    res = np.linalg.norm(K @ u - b) / np.linalg.norm(b)
+    res_d = np.linalg.norm(K @ ud - b) / np.linalg.norm(b)
+    print(case, "rel err vs scipy", err, "relres", res, "scipy's relres", res_d)
+    # both are backward-stable direct solves: residuals at the rounding
+    # level of the (ill-conditioned, slender c1) system, errors ~ cond * eps
+    assert err < 1e-9 and res < 100 * res_d + 1e-14
+    # the adjoint solve reuses the factor: a second call refactors cleanly
+    u2 = np.asarray(solver(tp))
+    assert np.array_equal(u, u2)
+
+
+def test_direct_gradient_matches_cg_gradient():
+    import feax_amd as fe
+    prob, bc, rho = _simp((8, 4, 4), seed=6)
+    grads = []
+    for opts in (fe.DirectSolverOptions(), fe.KrylovSolverOptions(solver="cg", tol=1e-13, atol=0.0,
+                                                                  use_jacobi_preconditioner=True)):
+        solver = fe.create_solver(prob, bc, opts, linear=True, return_solution=False)
+        tp, r = _tp(prob, rho, grad=True)
+        u = solver(tp)
+        (u * u).sum().backward()
+        grads.append(_np(r.grad))
+    err = np.abs(grads[0] - grads[1]).max() / np.abs(grads[1]).max()
+    print("gradient rel diff direct vs cg", err)
+    assert err < 1e-9
+    # mixed: direct forward, CG adjoint; and CG forward, direct adjoint
+    for fwd, adj in ((fe.DirectSolverOptions(), fe.KrylovSolverOptions(solver="cg", tol=1e-13, atol=0.0,
+                                                                        use_jacobi_preconditioner=True)),
+                     (fe.KrylovSolverOptions(solver="cg", tol=1e-13, atol=0.0, use_jacobi_preconditioner=True),
+                      fe.DirectSolverOptions())):
+        solver = fe.create_solver(prob, bc, fwd, adjoint_solver_options=adj, linear=True, return_solution=False)
+        tp, r = _tp(prob, rho, grad=True)
+        (solver(tp) ** 2).sum().backward()
+        assert np.abs(_np(r.grad) - grads[1]).max() <= 1e-9 * np.abs(grads[1]).max()
+
+
+def test_direct_reports_a_matrix_that_is_not_positive_definite():
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import assemble_csr
+    from feax_amd.solvers.direct import BandCholesky
+    prob, bc, rho = _simp((4, 2, 2))
+    ts = fe.TracedStructure.from_problem(prob)
+    tp, _ = _tp(prob, rho)
+    J = assemble_csr(prob, ts, tp, None, bc, True, torch.float64)
+    neg = J.with_data(-J.data)
+    with pytest.raises(RuntimeError, match="not positive definite"):
+        BandCholesky(prob, ts.device).factor(neg)
+    # the next factorisation of the SPD matrix is unaffected
+    F = BandCholesky(prob, ts.device).factor(J)
+    b = torch.ones(ts.n, dtype=torch.float64, device="cuda")
+    x = F.solve(b)
+    import scipy.sparse as sp
+    K = sp.csr_matrix((_np(J.data), _np(J.indices), _np(J.indptr)), shape=(ts.n, ts.n))
+    assert np.linalg.norm(K @ _np(x) - 1.0) <= 1e-10 * np.sqrt(ts.n)
