@@ -26,7 +26,8 @@ HIPCC_FLAGS = [
 ]
 
 
-# rocBLAS / rocSOLVER: the dense blocks of the banded Cholesky (direct.hip).
+# rocSOLVER / rocBLAS: the potrf / trsm / syrk blocks of the banded Cholesky
+# (direct.hip).
 LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 

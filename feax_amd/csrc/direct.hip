@@ -17,11 +17,13 @@
 //
 // The entries of L21 past the bandwidth are stored zeros and stay zero
 // (Cholesky fills only inside the envelope).  The triangular solves are
-// hand-written: one launch per 64-column block, the block's diagonal
-// triangle solved by one wave from LDS, the coupling to the next block
-// (forward) or the previous bandwidth rows (backward) as coalesced
-// multiply-adds.  Rows are in the caller's ordering `perm` (new -> old),
-// chosen to reduce the bandwidth.
+// hand-written: one launch per 256-row block, whose diagonal triangle is
+// solved in 64-row sub-blocks (one wave from LDS, then all threads update
+// the rows below / above), while the other workgroups apply the block's
+// coupling to the next bandwidth rows (forward) or the previous ones
+// (backward) as coalesced multiply-adds; they read only entries within the
+// bandwidth.  Rows are in the caller's ordering `perm` (new -> old), chosen
+// to reduce the bandwidth.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -37,7 +39,8 @@ namespace fx {
 namespace {
 
 constexpr int kFactorNb = 128;  // block columns of the factorisation
-constexpr int kSolveNb = 64;    // block of the triangular solves (one wave)
+constexpr int kSolveNb = 256;   // block of the triangular solves (one launch each)
+constexpr int kSub = 64;        // diagonal sub-block solved by one wave
 constexpr int kThreads = 256;
 
 __device__ __forceinline__ size_t boff(int64_t i, int64_t j, int64_t ldab) {
@@ -81,105 +84,193 @@ __global__ __launch_bounds__(kThreads) void permute_kernel(int n, const int* __r
   }
 }
 
-// Diagonal block [c, c + w) of L into LDS: Ld[i][j] = L(c + i, c + j), i >= j.
-__device__ __forceinline__ void load_diag(const double* __restrict__ ab, int64_t ldab, int c, int w,
-                                          double (*Ld)[kSolveNb + 1]) {
-  const int lane = threadIdx.x & 63;
-  for (int j = 0; j < w; ++j)
-    if (lane >= j && lane < w) Ld[lane][j] = ab[boff(c + lane, c + j, ldab)];
+// Diagonal triangle [c, c + w) (w <= kSub) of L into LDS, all threads:
+// Ld[i][j] = L(c + i, c + j) for i > j (0 past the bandwidth) and
+// Ld[j][j] = 1 / L(c + j, c + j), so the sequential steps multiply.
+__device__ __forceinline__ void load_diag(const double* __restrict__ ab, int64_t ldab, int bw, int c, int w,
+                                          double (*Ld)[kSub + 1]) {
+#pragma unroll
+  for (int q = 0; q < kSub * kSub / kThreads; ++q) {
+    const int e = threadIdx.x + q * kThreads;
+    const int i = e % kSub, j = e / kSub;
+    if (j < w && i >= j && i < w) {
+      const double l = (i - j <= bw) ? ab[boff(c + i, c + j, ldab)] : 0.0;
+      Ld[i][j] = i == j ? 1.0 / l : l;
+    }
+  }
 }
 
-// Forward step.  On entry y[cK, cK + wK) is final (cK < 0: nothing yet).
-// Block 0: rows of the next block [c1, c1 + w1) take the coupling to block
-// K, then wave 0 solves the next diagonal triangle.  Other blocks: rows
-// [c1 + w1, min(n, cK + wK + bw)) take the coupling to block K.
+// sum_{j in [j0, w)} L(r, c + j) v[j] for this thread's row r; for one j
+// the threads' consecutive rows read consecutive addresses (coalesced).
+// kBatch loads are issued before the first is used (latency, not
+// bandwidth, bounds these short dots).
+constexpr int kBatch = 32;
+__device__ __forceinline__ double row_dot(const double* __restrict__ ab, int64_t ldab, int r, int c, int j0, int w,
+                                          const double* __restrict__ v) {
+  double acc = 0.0;
+  for (int base = j0; base < w; base += kBatch) {
+    double l[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) l[u] = base + u < w ? ab[boff(r, c + base + u, ldab)] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      if (base + u < w) acc += l[u] * v[base + u];
+  }
+  return acc;
+}
+
+// Wave-wide sums s[q] = sum_{r < wr, c + r - j_q <= bw} L(c + r, j_q) v[r]
+// for NJ columns j_q = j0 + q * stride (j_q < jend), lanes over r
+// (coalesced); all loads issued before the reductions.  Valid in every lane.
+template <int NJ>
+__device__ __forceinline__ void col_dots(const double* __restrict__ ab, int64_t ldab, int bw, int c, int wr, int j0,
+                                         int stride, int jend, const double* __restrict__ v, double (&s)[NJ]) {
+  const int lane = threadIdx.x & 63;
+  constexpr int kQ = kSolveNb / 64;
+  double l[NJ][kQ];
+#pragma unroll
+  for (int q = 0; q < NJ; ++q) {
+    const int j = j0 + q * stride;
+#pragma unroll
+    for (int h = 0; h < kQ; ++h) {
+      const int r = h * 64 + lane;
+      l[q][h] = (j < jend && r < wr && c + r - j <= bw) ? ab[boff(c + r, j, ldab)] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NJ; ++q) {
+    double a = 0.0;
+#pragma unroll
+    for (int h = 0; h < kQ; ++h) {
+      const int r = h * 64 + lane;
+      if (r < wr) a += l[q][h] * v[r];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    s[q] = a;
+  }
+}
+
+// Forward step (L y = b, in place).  On entry y[cK, cK + wK) is final (cK
+// < 0: nothing yet).  Block 0: the next block [c1, c1 + w1) takes its
+// coupling to block K, then solves its triangle in kSub sub-blocks (wave 0
+// solves a sub-block from LDS, all threads update the rows below it).
+// Other blocks: rows [c1 + w1, min(n, cK + wK + bw)) take the coupling to
+// block K.  Only entries within the bandwidth are read.
 __global__ __launch_bounds__(kThreads) void band_fwd_kernel(int n, int bw, int64_t ldab,
                                                             const double* __restrict__ ab, double* __restrict__ y,
                                                             int cK, int wK) {
   __shared__ double yk[kSolveNb];
-  __shared__ double Ld[kSolveNb][kSolveNb + 1];
+  __shared__ double yb[kSolveNb];
+  __shared__ double Ld[kSub][kSub + 1];
+  const int t = threadIdx.x;
   const int c1 = cK < 0 ? 0 : cK + wK;
   const int w1 = min(kSolveNb, n - c1);
-  if (cK >= 0 && threadIdx.x < wK) yk[threadIdx.x] = y[cK + threadIdx.x];
+  if (t < wK) yk[t] = y[cK + t];
   __syncthreads();
   if (blockIdx.x == 0) {
-    const int t = threadIdx.x;
-    double v = 0.0;
-    if (t < 64) {
-      load_diag(ab, ldab, c1, w1, Ld);
-      if (t < w1) {
-        v = y[c1 + t];
-        if (cK >= 0)
-          for (int j = 0; j < wK; ++j) v -= ab[boff(c1 + t, cK + j, ldab)] * yk[j];
+    if (t < w1) {
+      double v = y[c1 + t];
+      if (wK > 0) v -= row_dot(ab, ldab, c1 + t, cK, max(0, c1 + t - bw - cK), wK, yk);
+      yb[t] = v;
+    }
+    for (int s0 = 0; s0 < w1; s0 += kSub) {
+      const int sw = min(kSub, w1 - s0);
+      load_diag(ab, ldab, bw, c1 + s0, sw, Ld);
+      __syncthreads();
+      if (t < 64) {
+        double vv = t < sw ? yb[s0 + t] : 0.0;
+        for (int jj = 0; jj < sw; ++jj) {
+          const double xj = __shfl(vv, jj, 64) * Ld[jj][jj];
+          if (t == jj)
+            vv = xj;
+          else if (t > jj && t < sw)
+            vv -= Ld[t][jj] * xj;
+        }
+        if (t < sw) yb[s0 + t] = vv;
       }
+      __syncthreads();
+      if (t >= s0 + sw && t < w1)
+        yb[t] -= row_dot(ab, ldab, c1 + t, c1 + s0, max(0, t - bw - s0), sw, yb + s0);
+      __syncthreads();
     }
-    __syncthreads();
-    if (t >= 64) return;
-    for (int jj = 0; jj < w1; ++jj) {
-      const double xj = __shfl(v, jj, 64) / Ld[jj][jj];
-      if (t == jj)
-        v = xj;
-      else if (t > jj && t < w1)
-        v -= Ld[t][jj] * xj;
-    }
-    if (t < w1) y[c1 + t] = v;
+    if (t < w1) y[c1 + t] = yb[t];
     return;
   }
-  const int r = c1 + w1 + (blockIdx.x - 1) * kThreads + threadIdx.x;
+  const int r = c1 + w1 + (blockIdx.x - 1) * kThreads + t;
   const int rend = min(n, cK + wK + bw);
   if (r >= rend) return;
-  double s = 0.0;
-  for (int j = 0; j < wK; ++j) s += ab[boff(r, cK + j, ldab)] * yk[j];
-  y[r] -= s;
+  y[r] -= row_dot(ab, ldab, r, cK, max(0, r - bw - cK), wK, yk);
 }
 
-// Backward step (L^T x = y).  On entry y[cK, cK + wK) holds final x (the
-// last block's launch has cK = n: nothing yet).  Rows j of the previous
-// bandwidth take sum_{r in K} L(r, j) x_r (one wave per j, coalesced in
-// r); block 0 owns the previous block [c1, c1 + 64) and then solves its
-// transposed triangle with wave 0.
+// Backward step (L^T x = y, in place).  On entry y[cK, cK + wK) holds final
+// x (the last block's launch has wK = 0).  Other blocks: rows j of the
+// previous bandwidth take sum_{r in K} L(r, j) x_r, one wave per j.  Block
+// 0 does the same for its block [c1, c1 + w1), then solves the transposed
+// triangle in kSub sub-blocks from the bottom.
 __global__ __launch_bounds__(kThreads) void band_bwd_kernel(int n, int bw, int64_t ldab,
                                                             const double* __restrict__ ab, double* __restrict__ y,
                                                             int cK, int wK, int c1, int w1) {
   __shared__ double xk[kSolveNb];
-  __shared__ double Ld[kSolveNb][kSolveNb + 1];
-  __shared__ double upd[kSolveNb];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (threadIdx.x < wK) xk[threadIdx.x] = y[cK + threadIdx.x];
+  __shared__ double xb[kSolveNb];
+  __shared__ double Ld[kSub][kSub + 1];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  constexpr int kWaves = kThreads / 64;
+  if (t < wK) xk[t] = y[cK + t];
   __syncthreads();
   if (blockIdx.x == 0) {
-    // coupling of the rows of block [c1, c1 + w1) to block K
-    for (int j = wid; j < w1; j += kThreads / 64) {
-      double s = 0.0;
-      if (lane < wK) s = ab[boff(cK + lane, c1 + j, ldab)] * xk[lane];
+    constexpr int kJ = 8;
+    for (int jj = wid; jj < w1; jj += kWaves * kJ) {
+      double s[kJ];
+      if (wK > 0) {
+        col_dots<kJ>(ab, ldab, bw, cK, wK, c1 + jj, kWaves, c1 + w1, xk, s);
+      } else {
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-      if (lane == 0) upd[j] = s;
+        for (int q = 0; q < kJ; ++q) s[q] = 0.0;
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < kJ; ++q)
+          if (jj + q * kWaves < w1) xb[jj + q * kWaves] = y[c1 + jj + q * kWaves] - s[q];
+      }
     }
-    if (wid == 0) load_diag(ab, ldab, c1, w1, Ld);
-    __syncthreads();
-    if (wid != 0) return;
-    const int t = lane;
-    double v = t < w1 ? y[c1 + t] - (wK > 0 ? upd[t] : 0.0) : 0.0;
-    for (int jj = w1 - 1; jj >= 0; --jj) {
-      const double xj = __shfl(v, jj, 64) / Ld[jj][jj];
-      if (t == jj)
-        v = xj;
-      else if (t < jj)
-        v -= Ld[jj][t] * xj;
+    for (int s0 = ((w1 - 1) / kSub) * kSub; s0 >= 0; s0 -= kSub) {
+      const int sw = min(kSub, w1 - s0);
+      load_diag(ab, ldab, bw, c1 + s0, sw, Ld);
+      __syncthreads();
+      if (t < 64) {
+        double vv = t < sw ? xb[s0 + t] : 0.0;
+        for (int jj = sw - 1; jj >= 0; --jj) {
+          const double xj = __shfl(vv, jj, 64) * Ld[jj][jj];
+          if (t == jj)
+            vv = xj;
+          else if (t < jj)
+            vv -= Ld[jj][t] * xj;
+        }
+        if (t < sw) xb[s0 + t] = vv;
+      }
+      __syncthreads();
+      for (int jj = wid; jj < s0; jj += kWaves * kJ) {
+        double s[kJ];
+        col_dots<kJ>(ab, ldab, bw, c1 + s0, sw, c1 + jj, kWaves, c1 + s0, xb + s0, s);
+        if (lane == 0) {
+#pragma unroll
+          for (int q = 0; q < kJ; ++q)
+            if (jj + q * kWaves < s0) xb[jj + q * kWaves] -= s[q];
+        }
+      }
+      __syncthreads();
     }
-    if (t < w1) y[c1 + t] = v;
+    if (t < w1) y[c1 + t] = xb[t];
     return;
   }
   if (wK == 0) return;
   const int jlo = max(0, cK - bw);
-  const int j = c1 - 1 - ((blockIdx.x - 1) * (kThreads / 64) + wid);
+  const int j = c1 - 1 - ((blockIdx.x - 1) * kWaves + wid);
   if (j < jlo) return;
-  double s = 0.0;
-  if (lane < wK) s = ab[boff(cK + lane, j, ldab)] * xk[lane];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if (lane == 0) y[j] -= s;
+  double s[1];
+  col_dots<1>(ab, ldab, bw, cK, wK, j, 0, j + 1, xk, s);
+  if (lane == 0) y[j] -= s[0];
 }
 
 // One rocBLAS handle per device for the process (rocSOLVER runs on it).

@@ -412,10 +412,10 @@ int fx_gmres_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int3
  * fx_band_from_csr_f64: zero ab and scatter the lower triangle of the
  *   (symmetric) CSR; FX_EINVAL if an entry lies past bw.
  * fx_band_cholesky_f64: in-place L (A = L L^T), blocked right-looking
- *   (rocSOLVER potrf on 128-column diagonal blocks, rocBLAS trsm / syrk on
- *   the band below); *info = 0, or the 1-based column of the first
- *   non-positive pivot (the matrix is not positive definite).  Uses one
- *   rocBLAS handle per device, set to `stream` at each call.
+ *   (rocSOLVER potrf on 128-column diagonal blocks, rocBLAS
+ *   trsm / syrk on the band below); *info = 0, or the 1-based column of the
+ *   first non-positive pivot (the matrix is not positive definite).  Uses
+ *   one rocBLAS handle per device, set to `stream` at each call.
  * fx_band_solve_f64: x = P^T L^-T L^-1 P b with (P b)[i'] = b[perm[i']];
  *   x may alias b. */
 int64_t fx_band_ldab(int32_t bw);
