@@ -23,6 +23,7 @@ FX_PCG_TIME = 4
 FX_PCG_AUTOTUNE = 8
 FX_PCG_NO_GRAPH = 16
 FX_GMG_F32 = 32
+FX_BAND_LDLT = 1
 FX_ASM_DIFF3 = 1
 FX_ASM_COMPUTE32 = 2
 
@@ -107,10 +108,11 @@ SIGNATURES = {
     "fx_bicgstab_f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P, _P,
                                _P, _SZ]),
     "fx_band_ldab": (_I64, [_I32]),
-    "fx_band_workspace_bytes": (_SZ, [_I32]),
+    "fx_band_workspace_bytes": (_SZ, [_I32, _I32]),
     "fx_band_from_csr_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _I32, _I64, _P, _P, _SZ]),
     "fx_band_cholesky_f64": (_I32, [_P, _I32, _I32, _I64, _P, _P, _P, _SZ]),
-    "fx_band_solve_f64": (_I32, [_P, _I32, _I32, _I64, _P, _P, _P, _P, _P, _SZ]),
+    "fx_band_ldlt_f64": (_I32, [_P, _I32, _I32, _I64, _P, _P, _P, _SZ]),
+    "fx_band_solve_f64": (_I32, [_P, _I32, _I32, _I64, _I32, _P, _P, _P, _P, _P, _SZ]),
     "fx_gmres_workspace_bytes": (_SZ, [_I32, _I32, _I32]),
     "fx_gmres_f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _I32, _P, _P,
                             _P, _P, _SZ]),

@@ -87,15 +87,19 @@ __global__ __launch_bounds__(kThreads) void permute_kernel(int n, const int* __r
 // Diagonal triangle [c, c + w) (w <= kSub) of L into LDS, all threads:
 // Ld[i][j] = L(c + i, c + j) for i > j (0 past the bandwidth) and
 // Ld[j][j] = 1 / L(c + j, c + j), so the sequential steps multiply.
+// unit: L has a unit diagonal (LDL^T; the stored diagonal is D).
 __device__ __forceinline__ void load_diag(const double* __restrict__ ab, int64_t ldab, int bw, int c, int w,
-                                          double (*Ld)[kSub + 1]) {
+                                          bool unit, double (*Ld)[kSub + 1]) {
 #pragma unroll
   for (int q = 0; q < kSub * kSub / kThreads; ++q) {
     const int e = threadIdx.x + q * kThreads;
     const int i = e % kSub, j = e / kSub;
     if (j < w && i >= j && i < w) {
-      const double l = (i - j <= bw) ? ab[boff(c + i, c + j, ldab)] : 0.0;
-      Ld[i][j] = i == j ? 1.0 / l : l;
+      if (i == j) {
+        Ld[i][j] = unit ? 1.0 : 1.0 / ab[boff(c + i, c + j, ldab)];
+      } else {
+        Ld[i][j] = (i - j <= bw) ? ab[boff(c + i, c + j, ldab)] : 0.0;
+      }
     }
   }
 }
@@ -159,7 +163,7 @@ __device__ __forceinline__ void col_dots(const double* __restrict__ ab, int64_t 
 // block K.  Only entries within the bandwidth are read.
 __global__ __launch_bounds__(kThreads) void band_fwd_kernel(int n, int bw, int64_t ldab,
                                                             const double* __restrict__ ab, double* __restrict__ y,
-                                                            int cK, int wK) {
+                                                            int cK, int wK, int unit) {
   __shared__ double yk[kSolveNb];
   __shared__ double yb[kSolveNb];
   __shared__ double Ld[kSub][kSub + 1];
@@ -176,7 +180,7 @@ __global__ __launch_bounds__(kThreads) void band_fwd_kernel(int n, int bw, int64
     }
     for (int s0 = 0; s0 < w1; s0 += kSub) {
       const int sw = min(kSub, w1 - s0);
-      load_diag(ab, ldab, bw, c1 + s0, sw, Ld);
+      load_diag(ab, ldab, bw, c1 + s0, sw, unit, Ld);
       __syncthreads();
       if (t < 64) {
         double vv = t < sw ? yb[s0 + t] : 0.0;
@@ -210,7 +214,7 @@ __global__ __launch_bounds__(kThreads) void band_fwd_kernel(int n, int bw, int64
 // triangle in kSub sub-blocks from the bottom.
 __global__ __launch_bounds__(kThreads) void band_bwd_kernel(int n, int bw, int64_t ldab,
                                                             const double* __restrict__ ab, double* __restrict__ y,
-                                                            int cK, int wK, int c1, int w1) {
+                                                            int cK, int wK, int c1, int w1, int unit) {
   __shared__ double xk[kSolveNb];
   __shared__ double xb[kSolveNb];
   __shared__ double Ld[kSub][kSub + 1];
@@ -236,7 +240,7 @@ __global__ __launch_bounds__(kThreads) void band_bwd_kernel(int n, int bw, int64
     }
     for (int s0 = ((w1 - 1) / kSub) * kSub; s0 >= 0; s0 -= kSub) {
       const int sw = min(kSub, w1 - s0);
-      load_diag(ab, ldab, bw, c1 + s0, sw, Ld);
+      load_diag(ab, ldab, bw, c1 + s0, sw, unit, Ld);
       __syncthreads();
       if (t < 64) {
         double vv = t < sw ? xb[s0 + t] : 0.0;
@@ -273,6 +277,39 @@ __global__ __launch_bounds__(kThreads) void band_bwd_kernel(int n, int bw, int64
   if (lane == 0) y[j] -= s[0];
 }
 
+// y[i] /= D[i] (the stored diagonal of an LDL^T factor).
+__global__ __launch_bounds__(kThreads) void diag_scale_kernel(int n, int64_t ldab, const double* __restrict__ ab,
+                                                              double* __restrict__ y) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) y[i] /= ab[boff(i, i, ldab)];
+}
+
+// Mirror the lower triangle of the w x w diagonal block into the upper
+// triangle of its dense view (leading dimension ldab - 1), which LU reads:
+// view entry (i, j), i < j, is the unused tail of band column c0 + j - 1
+// at distance ldab - (j - i) > bw + kFactorNb, never read elsewhere.
+__global__ __launch_bounds__(kThreads) void mirror_block_kernel(int64_t ldab, double* __restrict__ ab, int c0,
+                                                                int w) {
+  const int64_t ld = ldab - 1;
+  for (int e = blockIdx.x * kThreads + threadIdx.x; e < w * w; e += gridDim.x * kThreads) {
+    const int i = e % w, j = e / w;
+    if (i < j) ab[(c0 + i) + (c0 + j) * ld] = ab[(c0 + j) + (c0 + i) * ld];
+  }
+}
+
+// LDL^T panel of block column [c0, c0 + w): the band holds W = A21 L11^-T
+// (h x w); copy W to wt (column-major, leading dimension h) and overwrite
+// the band with L21 = W D1^-1.
+__global__ __launch_bounds__(kThreads) void ldlt_panel_kernel(int64_t ldab, double* __restrict__ ab, int c0, int w,
+                                                              int h, double* __restrict__ wt) {
+  const int r = blockIdx.x * kThreads + threadIdx.x;
+  const int j = blockIdx.y;
+  if (r >= h || j >= w) return;
+  const size_t o = boff(c0 + w + r, c0 + j, ldab);
+  const double v = ab[o];
+  wt[static_cast<size_t>(j) * h + r] = v;
+  ab[o] = v / ab[boff(c0 + j, c0 + j, ldab)];
+}
+
 // One rocBLAS handle per device for the process (rocSOLVER runs on it).
 rocblas_handle blas_handle(int& err) {
   static std::mutex mu;
@@ -304,10 +341,12 @@ extern "C" {
 
 int64_t fx_band_ldab(int32_t bw) { return bw < 0 ? -1 : static_cast<int64_t>(bw) + 2 * kFactorNb; }
 
-size_t fx_band_workspace_bytes(int32_t n) {
-  if (n < 0) return 0;
+size_t fx_band_workspace_bytes(int32_t n, int32_t bw) {
+  if (n < 0 || bw < 0) return 0;
   const size_t nb = (static_cast<size_t>(n) + kFactorNb - 1) / kFactorNb;
-  return 256 + ((nb * sizeof(int) + 255) & ~static_cast<size_t>(255)) + static_cast<size_t>(n) * 8;
+  return 256 + ((nb * sizeof(int) + 255) & ~static_cast<size_t>(255)) +
+         ((static_cast<size_t>(n) * 8 + 255) & ~static_cast<size_t>(255)) +
+         static_cast<size_t>(bw) * kFactorNb * 8;
 }
 
 int fx_band_from_csr_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, const double* data,
@@ -315,8 +354,8 @@ int fx_band_from_csr_f64(void* stream, int32_t n, const int32_t* indptr, const i
                          size_t ws_bytes) {
   if (n < 0 || bw < 0 || ldab < fx_band_ldab(bw) || (n > 0 && (!indptr || !indices || !data || !ab)))
     return fail(FX_EINVAL, "fx_band_from_csr_f64: bad arguments");
-  if (!workspace || ws_bytes < fx_band_workspace_bytes(n))
-    return fail(FX_EWORKSPACE, "fx_band_from_csr_f64: workspace %zu < %zu", ws_bytes, fx_band_workspace_bytes(n));
+  if (!workspace || ws_bytes < fx_band_workspace_bytes(n, bw))
+    return fail(FX_EWORKSPACE, "fx_band_from_csr_f64: workspace %zu < %zu", ws_bytes, fx_band_workspace_bytes(n, bw));
   if (n == 0) return FX_OK;
   hipStream_t s = as_stream(stream);
   int* bad = static_cast<int*>(workspace);
@@ -333,23 +372,29 @@ int fx_band_from_csr_f64(void* stream, int32_t n, const int32_t* indptr, const i
   return FX_OK;
 }
 
-int fx_band_cholesky_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, double* ab, int32_t* info,
-                         void* workspace, size_t ws_bytes) {
-  if (n < 0 || bw < 0 || ldab < fx_band_ldab(bw) || !info || (n > 0 && !ab))
-    return fail(FX_EINVAL, "fx_band_cholesky_f64: bad arguments");
-  if (!workspace || ws_bytes < fx_band_workspace_bytes(n))
-    return fail(FX_EWORKSPACE, "fx_band_cholesky_f64: workspace %zu < %zu", ws_bytes, fx_band_workspace_bytes(n));
+}  // extern "C"
+
+namespace fx {
+namespace {
+
+// Blocked right-looking band factorisation, Cholesky (L L^T) or, ldlt,
+// L D L^T without pivoting (the diagonal block, mirrored to full, by
+// rocSOLVER's LU without pivoting: U11 = D1 L11^T; W = A21 L11^-T with unit
+// L11; L21 = W D1^-1; A22 -= L21 W^T by syrkx).
+int band_factor(const char* name, bool ldlt, hipStream_t s, int n, int bw, int64_t ldab, double* ab, int* info,
+                char* wsp) {
   *info = 0;
   if (n == 0) return FX_OK;
-  if (ldab - 1 > INT32_MAX) return fail(FX_ELIMIT, "fx_band_cholesky_f64: ldab %lld too large", (long long)ldab);
+  if (ldab - 1 > INT32_MAX) return fail(FX_ELIMIT, "%s: ldab %lld too large", name, (long long)ldab);
   int err = 0;
   rocblas_handle h = blas_handle(err);
-  if (err) return fail(FX_ELIB, "fx_band_cholesky_f64: rocblas_create_handle failed");
-  hipStream_t s = as_stream(stream);
+  if (err) return fail(FX_ELIB, "%s: rocblas_create_handle failed", name);
   rocblas_status st = rocblas_set_stream(h, s);
   if (st != rocblas_status_success) return blas_fail(st, "rocblas_set_stream");
   const int nblocks = (n + kFactorNb - 1) / kFactorNb;
-  int* dinfo = reinterpret_cast<int*>(static_cast<char*>(workspace) + 256);
+  int* dinfo = reinterpret_cast<int*>(wsp + 256);
+  double* wt = reinterpret_cast<double*>(wsp + 256 + ((nblocks * sizeof(int) + 255) & ~static_cast<size_t>(255)) +
+                                         ((static_cast<size_t>(n) * 8 + 255) & ~static_cast<size_t>(255)));
   const int ld = static_cast<int>(ldab - 1);
   auto A = [&](int64_t i, int64_t j) { return ab + i + j * static_cast<int64_t>(ld); };
   const double one = 1.0, minus_one = -1.0;
@@ -357,12 +402,29 @@ int fx_band_cholesky_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, doub
     const int c0 = k * kFactorNb;
     const int w = std::min(kFactorNb, n - c0);
     const int hh = std::min(bw, n - c0 - w);
-    st = rocsolver_dpotrf(h, rocblas_fill_lower, w, A(c0, c0), ld, dinfo + k);
-    if (st != rocblas_status_success) return blas_fail(st, "rocsolver_dpotrf");
-    if (hh > 0) {
-      st = rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                         rocblas_diagonal_non_unit, hh, w, &one, A(c0, c0), ld, A(c0 + w, c0), ld);
-      if (st != rocblas_status_success) return blas_fail(st, "rocblas_dtrsm");
+    if (ldlt) {
+      hipLaunchKernelGGL(mirror_block_kernel, dim3((w * w + kThreads - 1) / kThreads), dim3(kThreads), 0, s, ldab,
+                         ab, c0, w);
+      FX_LAUNCHED("mirror_block_kernel");
+      st = rocsolver_dgetrf_npvt(h, w, w, A(c0, c0), ld, dinfo + k);
+      if (st != rocblas_status_success) return blas_fail(st, "rocsolver_dgetrf_npvt");
+    } else {
+      st = rocsolver_dpotrf(h, rocblas_fill_lower, w, A(c0, c0), ld, dinfo + k);
+      if (st != rocblas_status_success) return blas_fail(st, "rocsolver_dpotrf");
+    }
+    if (hh <= 0) continue;
+    st = rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                       ldlt ? rocblas_diagonal_unit : rocblas_diagonal_non_unit, hh, w, &one, A(c0, c0), ld,
+                       A(c0 + w, c0), ld);
+    if (st != rocblas_status_success) return blas_fail(st, "rocblas_dtrsm");
+    if (ldlt) {
+      hipLaunchKernelGGL(ldlt_panel_kernel, dim3((hh + kThreads - 1) / kThreads, w), dim3(kThreads), 0, s, ldab, ab,
+                         c0, w, hh, wt);
+      FX_LAUNCHED("ldlt_panel_kernel");
+      st = rocblas_dsyrkx(h, rocblas_fill_lower, rocblas_operation_none, hh, w, &minus_one, A(c0 + w, c0), ld, wt,
+                          hh, &one, A(c0 + w, c0 + w), ld);
+      if (st != rocblas_status_success) return blas_fail(st, "rocblas_dsyrkx");
+    } else {
       st = rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, hh, w, &minus_one, A(c0 + w, c0), ld, &one,
                          A(c0 + w, c0 + w), ld);
       if (st != rocblas_status_success) return blas_fail(st, "rocblas_dsyrk");
@@ -380,18 +442,44 @@ int fx_band_cholesky_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, doub
   return FX_OK;
 }
 
-int fx_band_solve_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, const double* ab, const int32_t* perm,
-                      const double* b, double* x, void* workspace, size_t ws_bytes) {
-  if (n < 0 || bw < 0 || ldab < fx_band_ldab(bw) || (n > 0 && (!ab || !b || !x)))
+}  // namespace
+}  // namespace fx
+
+extern "C" {
+
+int fx_band_cholesky_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, double* ab, int32_t* info,
+                         void* workspace, size_t ws_bytes) {
+  if (n < 0 || bw < 0 || ldab < fx_band_ldab(bw) || !info || (n > 0 && !ab))
+    return fail(FX_EINVAL, "fx_band_cholesky_f64: bad arguments");
+  if (!workspace || ws_bytes < fx_band_workspace_bytes(n, bw))
+    return fail(FX_EWORKSPACE, "fx_band_cholesky_f64: workspace %zu < %zu", ws_bytes, fx_band_workspace_bytes(n, bw));
+  return band_factor("fx_band_cholesky_f64", false, as_stream(stream), n, bw, ldab, ab, info,
+                     static_cast<char*>(workspace));
+}
+
+int fx_band_ldlt_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, double* ab, int32_t* info, void* workspace,
+                     size_t ws_bytes) {
+  if (n < 0 || bw < 0 || ldab < fx_band_ldab(bw) || !info || (n > 0 && !ab))
+    return fail(FX_EINVAL, "fx_band_ldlt_f64: bad arguments");
+  if (!workspace || ws_bytes < fx_band_workspace_bytes(n, bw))
+    return fail(FX_EWORKSPACE, "fx_band_ldlt_f64: workspace %zu < %zu", ws_bytes, fx_band_workspace_bytes(n, bw));
+  return band_factor("fx_band_ldlt_f64", true, as_stream(stream), n, bw, ldab, ab, info,
+                     static_cast<char*>(workspace));
+}
+
+int fx_band_solve_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, int32_t flags, const double* ab,
+                      const int32_t* perm, const double* b, double* x, void* workspace, size_t ws_bytes) {
+  if (n < 0 || bw < 0 || ldab < fx_band_ldab(bw) || (flags & ~FX_BAND_LDLT) || (n > 0 && (!ab || !b || !x)))
     return fail(FX_EINVAL, "fx_band_solve_f64: bad arguments");
-  if (!workspace || ws_bytes < fx_band_workspace_bytes(n))
-    return fail(FX_EWORKSPACE, "fx_band_solve_f64: workspace %zu < %zu", ws_bytes, fx_band_workspace_bytes(n));
+  if (!workspace || ws_bytes < fx_band_workspace_bytes(n, bw))
+    return fail(FX_EWORKSPACE, "fx_band_solve_f64: workspace %zu < %zu", ws_bytes, fx_band_workspace_bytes(n, bw));
   if (n == 0) return FX_OK;
   hipStream_t s = as_stream(stream);
   const size_t nblk = (static_cast<size_t>(n) + kFactorNb - 1) / kFactorNb;
   double* y = reinterpret_cast<double*>(static_cast<char*>(workspace) + 256 +
                                         ((nblk * sizeof(int) + 255) & ~static_cast<size_t>(255)));
   const int gv = std::max(1, std::min((n + kThreads - 1) / kThreads, 2048));
+  const int unit = (flags & FX_BAND_LDLT) ? 1 : 0;
   hipLaunchKernelGGL(permute_kernel, dim3(gv), dim3(kThreads), 0, s, n, perm, b, y, 0);
   // L y = P b
   const int nsb = (n + kSolveNb - 1) / kSolveNb;
@@ -403,9 +491,10 @@ int fx_band_solve_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, const d
     const int rend = k < 0 ? 0 : std::min(n, cK + wK + bw);
     const int others = std::max(0, rend - (c1 + w1));
     hipLaunchKernelGGL(band_fwd_kernel, dim3(1 + (others + kThreads - 1) / kThreads), dim3(kThreads), 0, s, n, bw,
-                       ldab, ab, y, cK, wK);
+                       ldab, ab, y, cK, wK, unit);
   }
   FX_LAUNCHED("fx_band_solve_f64 forward");
+  if (unit) hipLaunchKernelGGL(diag_scale_kernel, dim3(gv), dim3(kThreads), 0, s, n, ldab, ab, y);
   // L^T x = y, from the last block
   for (int k = nsb; k >= 1; --k) {
     const int cK = k < nsb ? k * kSolveNb : n;
@@ -416,7 +505,7 @@ int fx_band_solve_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, const d
     const int others = wK > 0 ? std::max(0, c1 - jlo) : 0;
     const int wpb = kThreads / 64;
     hipLaunchKernelGGL(band_bwd_kernel, dim3(1 + (others + wpb - 1) / wpb), dim3(kThreads), 0, s, n, bw, ldab, ab, y,
-                       cK, wK, c1, w1);
+                       cK, wK, c1, w1, unit);
   }
   FX_LAUNCHED("fx_band_solve_f64 backward");
   hipLaunchKernelGGL(permute_kernel, dim3(gv), dim3(kThreads), 0, s, n, perm, y, x, 1);

@@ -1,4 +1,4 @@
-"""Direct linear solve on the device: banded Cholesky of the SPD J_bc.
+"""Direct linear solve on the device: banded Cholesky (or L D L^T) of J_bc.
 
 The stand-in for feax's GPU direct solver (DirectSolverOptions "auto" /
 "cudss", feax/solvers/common.py:116-210).  feax's direct branch of
@@ -6,7 +6,10 @@ create_linear_solver (feax/solvers/linear.py:308-417) computes
 u = u0 + J_bc^-1 (-R_bc(u0)) with no initial iterate, and under a symmetric
 operator reuses the forward factorisation for the adjoint solve
 (linear.py:351-378); this module does the same with
-fx_band_from_csr_f64 / fx_band_cholesky_f64 / fx_band_solve_f64.
+fx_band_from_csr_f64 / fx_band_cholesky_f64 / fx_band_solve_f64.  A
+symmetric matrix with a non-positive pivot (a Neo-Hookean tangent away from
+equilibrium) is factored again as L D L^T without pivoting
+(fx_band_ldlt_f64), as a symmetric-indefinite direct solver would.
 
 Ordering: the DOFs are renumbered node-wise (3 DOFs of a node stay
 together) by whichever of the mesh's own numbering and reverse
@@ -76,7 +79,7 @@ class BandCholesky:
         self.device = device
         self.perm = None if perm is None else torch.as_tensor(perm, device=device)
         self.inv = None if inv is None else torch.as_tensor(inv, device=device)
-        self.ws = torch.empty(int(nat.lib().fx_band_workspace_bytes(self.n)), dtype=torch.uint8, device=device)
+        self.ws = torch.empty(int(nat.lib().fx_band_workspace_bytes(self.n, bw)), dtype=torch.uint8, device=device)
         key = (str(device), self.n * self.ldab)
         ab = BandCholesky._buffers.get(key)
         if ab is None:
@@ -84,22 +87,33 @@ class BandCholesky:
             BandCholesky._buffers[key] = ab
         self.ab = ab
         self.token = None
+        self.ldlt = False
 
-    def factor(self, A, token=None):
-        """Factor the (symmetric positive definite) FP64 CSRMatrix A."""
-        import torch
-        if A.data.dtype != torch.float64 or A.form != "csr":
-            raise NotImplementedError("the direct solver factors an FP64 CSR matrix")
+    def _scatter(self, A):
         nat.call("fx_band_from_csr_f64", nat.stream(), self.n, nat.ptr(A.indptr), nat.ptr(A.indices),
                  nat.ptr(A.data), nat.ptr(self.inv), self.bw, self.ldab, nat.ptr(self.ab), nat.ptr(self.ws),
                  self.ws.numel())
+
+    def factor(self, A, token=None):
+        """Factor the symmetric FP64 CSRMatrix A: Cholesky, or L D L^T
+        (no pivoting) when a pivot is not positive (an indefinite tangent)."""
+        import torch
+        if A.data.dtype != torch.float64 or A.form != "csr":
+            raise NotImplementedError("the direct solver factors an FP64 CSR matrix")
+        self.ab.owner = None
+        self.token = None
         info = ctypes.c_int32(0)
+        self._scatter(A)
         nat.call("fx_band_cholesky_f64", nat.stream(), self.n, self.bw, self.ldab, nat.ptr(self.ab),
                  ctypes.byref(info), nat.ptr(self.ws), self.ws.numel())
-        if info.value:
-            self.ab.owner = None
-            raise RuntimeError(f"direct solver: J_bc is not positive definite (pivot {info.value} of {self.n}); "
-                               "the banded Cholesky needs symmetric elimination of an SPD operator")
+        self.ldlt = info.value != 0
+        if self.ldlt:
+            self._scatter(A)
+            nat.call("fx_band_ldlt_f64", nat.stream(), self.n, self.bw, self.ldab, nat.ptr(self.ab),
+                     ctypes.byref(info), nat.ptr(self.ws), self.ws.numel())
+            if info.value:
+                raise RuntimeError(f"direct solver: zero pivot {info.value} of {self.n} in the L D L^T factorisation "
+                                   "(J_bc is singular)")
         self.token = token if token is not None else object()
         self.ab.owner = self.token
         return self
@@ -118,8 +132,8 @@ class BandCholesky:
             raise RuntimeError("direct solver: the factorisation was overwritten by a later one")
         b = b.to(torch.float64).contiguous()
         x = torch.empty_like(b) if out is None else out
-        nat.call("fx_band_solve_f64", nat.stream(), self.n, self.bw, self.ldab, nat.ptr(self.ab), nat.ptr(self.perm),
-                 nat.ptr(b), nat.ptr(x), nat.ptr(self.ws), self.ws.numel())
+        nat.call("fx_band_solve_f64", nat.stream(), self.n, self.bw, self.ldab, nat.FX_BAND_LDLT if self.ldlt else 0,
+                 nat.ptr(self.ab), nat.ptr(self.perm), nat.ptr(b), nat.ptr(x), nat.ptr(self.ws), self.ws.numel())
         return x
 
 

@@ -34,7 +34,7 @@ from ..assembler import TracedStructure, assemble_csr, residual
 from ..DCboundary import DirichletBC
 from .common import create_x0, device_norm, pcg
 from .linear import SolveStats, _dev_vec, _zero_like
-from .options import KrylovSolverOptions, MatrixProperty, NewtonOptions, resolve_iterative_solver
+from .options import DirectSolverOptions, KrylovSolverOptions, MatrixProperty, NewtonOptions, resolve_iterative_solver
 
 
 class NewtonLineSearchError(RuntimeError):
@@ -62,9 +62,15 @@ def newton_solve(problem, ts, tp, bc, sol0, lin, newton, symmetric=True, stats=N
     res = residual(problem, ts, tp, sol, bc)
     r0 = device_norm(res)
     rn = r0
-    x0_fn = lin.x0_fn if lin.x0_fn is not None else create_x0(bc.bc_rows, bc.bc_vals)
-    mixed = lin.precision != "fp64"
-    shift = lin.jacobi_shift if lin.use_jacobi_preconditioner else None
+    direct = isinstance(lin, DirectSolverOptions)
+    if direct:
+        from .direct import BandCholesky
+        factor = BandCholesky(problem, ts.device)
+        x0_fn, mixed, shift = None, False, None
+    else:
+        x0_fn = lin.x0_fn if lin.x0_fn is not None else create_x0(bc.bc_rows, bc.bc_vals)
+        mixed = lin.precision != "fp64"
+        shift = lin.jacobi_shift if lin.use_jacobi_preconditioner else None
     it, found, fallbacks = 0, True, 0
     hist, alphas, lits = [r0], [], []
     if newton.verbose:
@@ -76,7 +82,10 @@ def newton_solve(problem, ts, tp, bc, sol0, lin, newton, symmetric=True, stats=N
         J, dinv = out if shift is not None else (out, None)
         b = -res
         du = None
-        if mixed:
+        if direct:
+            # feax's direct branch: du = J_bc^-1 b, no initial iterate
+            du, n_lin = factor.factor(J).solve(b), 0
+        elif mixed:
             du, n_lin = _mixed_direction(problem, ts, tp, bc, sol, b, x0_fn(sol), J, dinv, lin)
             if du is None:
                 # The FP32 tangent is too coarse for this state (the defect
@@ -173,12 +182,17 @@ def newton_adjoint(problem, ts, tp, bc, v, keep, opts, symmetric=True):
     """lambda = J_bc(u*)^-1 v (lambda[bc] = 0) on the FP64 tangent at u*."""
     import torch
     sol = keep["sol"]
-    shift = opts.jacobi_shift if opts.use_jacobi_preconditioner else None
+    direct = isinstance(opts, DirectSolverOptions)
+    shift = None if direct or not opts.use_jacobi_preconditioner else opts.jacobi_shift
     out = assemble_csr(problem, ts, tp, sol, bc, symmetric, torch.float64, jacobi_shift=shift)
     J, dinv = out if shift is not None else (out, None)
     rhs = v.detach().to(device=ts.device, dtype=torch.float64).reshape(-1).clone()
     if bc is not None and len(bc.bc_rows):
         rhs.index_fill_(0, bc.device(ts.device)[0].long(), 0.0)
+    if direct:
+        from .direct import BandCholesky
+        keep["adjoint_iters"] = [0]
+        return BandCholesky(problem, ts.device).factor(J).solve(rhs)
     lam, info = pcg(J, rhs, None, opts.tol, opts.atol, opts.maxiter, dinv, None, opts.check_every)
     keep["adjoint_iters"] = [info.iters]
     return lam
@@ -215,11 +229,18 @@ def create_newton_solver(problem, bc, solver_options=None, adjoint_solver_option
                          traced_params=None, symmetric_elimination=True, traced_structure=None):
     if solver_options is None:
         solver_options = KrylovSolverOptions(solver="cg", use_jacobi_preconditioner=True)
-    if not isinstance(solver_options, KrylovSolverOptions):
-        raise NotImplementedError("only KrylovSolverOptions (cg) are on the HIP path; direct solvers are not")
-    solver_options = resolve_iterative_solver(solver_options, MatrixProperty.SPD)
-    adj = solver_options if adjoint_solver_options is None else \
-        resolve_iterative_solver(adjoint_solver_options, MatrixProperty.SPD)
+    for o in (solver_options, adjoint_solver_options):
+        if o is not None and not isinstance(o, (KrylovSolverOptions, DirectSolverOptions)):
+            raise TypeError(f"Expected DirectSolverOptions or KrylovSolverOptions, got {type(o).__name__}.")
+    if isinstance(solver_options, DirectSolverOptions) or isinstance(adjoint_solver_options, DirectSolverOptions):
+        if not symmetric_elimination:
+            raise NotImplementedError("the direct solver factors the symmetric elimination's SPD J_bc "
+                                      "(symmetric_elimination=True)")
+
+    def _resolve(o):
+        return o if isinstance(o, DirectSolverOptions) else resolve_iterative_solver(o, MatrixProperty.SPD)
+    solver_options = _resolve(solver_options)
+    adj = solver_options if adjoint_solver_options is None else _resolve(adjoint_solver_options)
     newton_options = newton_options or NewtonOptions()
     default_bc = bc
     ts_default = traced_structure

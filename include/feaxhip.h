@@ -407,7 +407,7 @@ int fx_gmres_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int3
  * (0 <= i - j <= bw) at offset i - j, in the caller's ordering: row i' of
  * the band is row perm[i'] of the CSR (inv_perm[perm[i']] = i'; NULL =
  * identity).  ab holds n * ldab doubles, ldab >= fx_band_ldab(bw).
- * workspace >= fx_band_workspace_bytes(n) for each call.
+ * workspace >= fx_band_workspace_bytes(n, bw) for each call.
  *
  * fx_band_from_csr_f64: zero ab and scatter the lower triangle of the
  *   (symmetric) CSR; FX_EINVAL if an entry lies past bw.
@@ -416,17 +416,25 @@ int fx_gmres_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int3
  *   trsm / syrk on the band below); *info = 0, or the 1-based column of the
  *   first non-positive pivot (the matrix is not positive definite).  Uses
  *   one rocBLAS handle per device, set to `stream` at each call.
- * fx_band_solve_f64: x = P^T L^-T L^-1 P b with (P b)[i'] = b[perm[i']];
- *   x may alias b. */
+ * fx_band_ldlt_f64: in-place L D L^T without pivoting (unit L below the
+ *   diagonal, D on it) for a symmetric indefinite matrix such as a
+ *   Neo-Hookean tangent away from equilibrium; *info = 0, or the 1-based
+ *   column of the first zero pivot.
+ * fx_band_solve_f64: x = P^T L^-T L^-1 P b, or with FX_BAND_LDLT
+ *   x = P^T L^-T D^-1 L^-1 P b, where (P b)[i'] = b[perm[i']]; x may alias
+ *   b. */
+#define FX_BAND_LDLT 1
 int64_t fx_band_ldab(int32_t bw);
-size_t fx_band_workspace_bytes(int32_t n);
+size_t fx_band_workspace_bytes(int32_t n, int32_t bw);
 int fx_band_from_csr_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, const double* data,
                          const int32_t* inv_perm, int32_t bw, int64_t ldab, double* ab, void* workspace,
                          size_t ws_bytes);
 int fx_band_cholesky_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, double* ab, int32_t* info,
                          void* workspace, size_t ws_bytes);
-int fx_band_solve_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, const double* ab, const int32_t* perm,
-                      const double* b, double* x, void* workspace, size_t ws_bytes);
+int fx_band_ldlt_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, double* ab, int32_t* info, void* workspace,
+                     size_t ws_bytes);
+int fx_band_solve_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, int32_t flags, const double* ab,
+                      const int32_t* perm, const double* b, double* x, void* workspace, size_t ws_bytes);
 
 /* ------------------------------------------------------------------ */
 /* Geometric multigrid on a structured HEX8 box (StructuredGrid numbering,

@@ -143,22 +143,43 @@ def test_direct_gradient_matches_cg_gradient():
         assert np.abs(_np(r.grad) - grads[1]).max() <= 1e-9 * np.abs(grads[1]).max()
 
 
-def test_direct_reports_a_matrix_that_is_not_positive_definite():
+def test_indefinite_matrices_take_ldlt_and_singular_ones_are_reported():
+    """A symmetric indefinite matrix (J - sigma I, and -J) fails Cholesky and
+    is factored as L D L^T without pivoting (solution vs scipy); a matrix
+    with an empty free row and column has an exactly zero pivot."""
     import torch
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
     import feax_amd as fe
     from feax_amd.assembler import assemble_csr
     from feax_amd.solvers.direct import BandCholesky
-    prob, bc, rho = _simp((4, 2, 2))
+    prob, bc, rho = _simp((6, 3, 3))
     ts = fe.TracedStructure.from_problem(prob)
     tp, _ = _tp(prob, rho)
     J = assemble_csr(prob, ts, tp, None, bc, True, torch.float64)
-    neg = J.with_data(-J.data)
-    with pytest.raises(RuntimeError, match="not positive definite"):
-        BandCholesky(prob, ts.device).factor(neg)
-    # the next factorisation of the SPD matrix is unaffected
+    n = ts.n
+    K = sp.csr_matrix((_np(J.data), _np(J.indices), _np(J.indptr)), shape=(n, n))
+    rows = np.repeat(np.arange(n), np.diff(K.indptr))
+    sigma = 0.37 * np.median(K.diagonal())
+    data_shift = np.where(rows == K.indices, K.data - sigma, K.data)
+    b = torch.as_tensor(np.cos(np.arange(n)), device="cuda")
+    for data in (data_shift, -K.data):
+        A = J.with_data(torch.as_tensor(data, device="cuda"))
+        F = BandCholesky(prob, ts.device).factor(A)
+        assert F.ldlt
+        x = _np(F.solve(b))
+        Ks = sp.csr_matrix((data, K.indices, K.indptr), shape=(n, n))
+        xd = spla.spsolve(Ks.tocsc(), _np(b))
+        err = np.linalg.norm(x - xd) / np.linalg.norm(xd)
+        print("ldlt", "rel err vs scipy", err, "negative eigenvalues",
+              int((np.linalg.eigvalsh(Ks.toarray()) < 0).sum()))
+        assert err < 1e-9
+    free = int(np.nonzero(~bc.bc_mask)[0][5])
+    empty = np.where((rows == free) | (K.indices == free), 0.0, K.data)
+    with pytest.raises(RuntimeError, match="zero pivot"):
+        BandCholesky(prob, ts.device).factor(J.with_data(torch.as_tensor(empty, device="cuda")))
+    # the SPD matrix still takes Cholesky, and solves after the failures
     F = BandCholesky(prob, ts.device).factor(J)
-    b = torch.ones(ts.n, dtype=torch.float64, device="cuda")
+    assert not F.ldlt
     x = F.solve(b)
-    import scipy.sparse as sp
-    K = sp.csr_matrix((_np(J.data), _np(J.indices), _np(J.indptr)), shape=(ts.n, ts.n))
-    assert np.linalg.norm(K @ _np(x) - 1.0) <= 1e-10 * np.sqrt(ts.n)
+    assert np.linalg.norm(K @ _np(x) - _np(b)) <= 1e-10 * np.linalg.norm(_np(b))

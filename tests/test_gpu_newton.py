@@ -82,7 +82,7 @@ def test_nh_residual_matches_oracle():
     assert np.abs(r - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "direct"])
 def test_newton_load_steps_match_oracle(precision):
     """-10% z-compression of a block in 3 displacement steps (bottom clamped,
     top u_z prescribed, initial guess = previous state with the BC values)."""
@@ -93,8 +93,11 @@ def test_newton_load_steps_match_oracle(precision):
     prob, bc, is_top = _nh_problem(dims)
     rng = np.random.default_rng(20261203)
     mu = rng.uniform(0.5, 1.5, prob.num_cells)
-    lin = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=0.0, maxiter=20000, use_jacobi_preconditioner=True,
-                                 precision=precision)
+    if precision == "direct":   # the oracle's own linear solver: a direct solve
+        lin = fe.DirectSolverOptions()
+    else:
+        lin = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=0.0, maxiter=20000,
+                                     use_jacobi_preconditioner=True, precision=precision)
     nopts = fe.NewtonOptions(tol=1e-10, rel_tol=1e-12)
     solver = fe.create_solver(prob, bc, lin, newton_options=nopts, linear=False, return_solution=False)
     m = hex8.Model(prob.points, prob.cells.astype(np.int64))
@@ -118,7 +121,8 @@ def test_newton_load_steps_match_oracle(precision):
         assert st.newton_iters <= it_o + (3 if precision == "mixed" else 1)
 
 
-def test_energy_gradient_through_newton():
+@pytest.mark.parametrize("linear", ["cg", "direct"])
+def test_energy_gradient_through_newton(linear):
     """dPi/dmu of the displacement-controlled equilibrium via torch.autograd
     (adjoint through the Newton solver + energy VJP) == the partial dPi/dmu_e
     (R_free = 0, du_bc/dmu = 0), and == central differences of GPU re-solves."""
@@ -131,7 +135,8 @@ def test_energy_gradient_through_newton():
     mu0 = rng.uniform(0.5, 1.5, prob.num_cells)
     vals = np.where(is_top, -0.1 * dims[2], 0.0)
     bcs = bc.replace_vals(vals)
-    lin = fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0, maxiter=20000, use_jacobi_preconditioner=True)
+    lin = fe.DirectSolverOptions() if linear == "direct" else \
+        fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0, maxiter=20000, use_jacobi_preconditioner=True)
     nopts = fe.NewtonOptions(tol=1e-11, rel_tol=1e-13)
     solver = fe.create_solver(prob, bcs, lin, newton_options=nopts, linear=False, return_solution=False)
     energy = create_energy_fn(prob)
