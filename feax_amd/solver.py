@@ -7,7 +7,7 @@ Newton solver; wraps the flat result in a Solution when
 
 from .solution import Solution
 from .solvers.linear import create_linear_solver
-from .solvers.options import KrylovSolverOptions
+from .solvers.options import DirectSolverOptions
 
 
 def create_solver(problem, bc, solver_options=None, adjoint_solver_options=None, newton_options=None,
@@ -18,8 +18,11 @@ def create_solver(problem, bc, solver_options=None, adjoint_solver_options=None,
     if extra_residual_fn is not None:
         raise NotImplementedError("extra_residual_fn is not on the HIP path")
     if solver_options is None:
-        # feax defaults to a direct solver; the HIP path is Krylov only.
-        solver_options = KrylovSolverOptions(solver="cg", use_jacobi_preconditioner=True)
+        # feax's default (feax/solver.py:293-301): a direct solver.  Its
+        # "auto" needs no sample traced_params here (the factorisation picks
+        # Cholesky or L D L^T by itself), so feax's ValueError for a missing
+        # traced_params does not arise.
+        solver_options = DirectSolverOptions()
 
     def _wrap(fn):
         if not return_solution:

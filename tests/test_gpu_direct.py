@@ -113,9 +113,12 @@ def test_direct_solve_matches_scipy_direct_solve(case):
     # both are backward-stable direct solves: residuals at the rounding
     # level of the (ill-conditioned, slender c1) system, errors ~ cond * eps
     assert err < 1e-9 and res < 100 * res_d + 1e-14
-    # the adjoint solve reuses the factor: a second call refactors cleanly
+    # a second call refactors cleanly; create_solver's default options are
+    # this direct solver, as feax's are (feax/solver.py:293-301)
     u2 = np.asarray(solver(tp))
     assert np.array_equal(u, u2)
+    u3 = np.asarray(fe.create_solver(prob, bc, linear=True)(tp))
+    assert np.array_equal(u, u3)
 
 
 def test_direct_gradient_matches_cg_gradient():
