@@ -12,7 +12,9 @@ SpMV, CG) and "mixed" (FP32 difference-form matrix, FP64 vectors, FP64
 defect correction) -- the headline -- and, beside them, "gmg-fp64" /
 "gmg-mixed": the same step with CG preconditioned by the geometric-multigrid
 V-cycle (FP64 / FP32 V-cycle; hierarchy built inside the step).  Each form
-reports its error against the FP64 oracle's solution.
+reports its error against the FP64 oracle's solution.  "direct": one solve
+with DirectSolverOptions (feax's default; banded Cholesky), factorisation
+and triangular solves timed apart.
 
 Prints one JSON line (see the fields below); --gpus N > 1 is refused: this
 build runs on one GPU.
@@ -421,6 +423,52 @@ def batch_bench(steps, warmup, nseg=64, seed=20261203):
     return out
 
 
+def direct_bench(problem, bc, tp, ts, u0, bnorm, oracle_fix):
+    """DirectSolverOptions (feax's default) on the same workload: one warm
+    solve (ordering, band allocation, rocBLAS set-up), one timed solve, and
+    the factorisation / triangular solves timed apart."""
+    import numpy as np
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import assemble_csr, residual
+    from feax_amd.solvers.common import device_norm
+    from feax_amd.solvers.direct import BandCholesky
+    solver = fe.create_solver(problem, bc, fe.DirectSolverOptions(), linear=True, traced_params=tp,
+                              traced_structure=ts, return_solution=False)
+    solver(tp, u0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    u = solver(tp, u0)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    r = residual(problem, ts, tp, u, bc, negate=True)
+    err = None
+    if oracle_fix is not None:
+        us = u[torch.as_tensor(oracle_fix["dofs"], device=u.device)].cpu().numpy()
+        err = float(np.abs(us - oracle_fix["u"]).max() / float(oracle_fix["u_max"]))
+    J = assemble_csr(problem, ts, tp, None, bc, True, torch.float64)
+    b = residual(problem, ts, tp, u0, bc, negate=True)
+    F = BandCholesky(problem, ts.device)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    F.factor(J)
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    F.solve(b)
+    torch.cuda.synchronize()
+    tsol = time.perf_counter() - t1
+    out = dict(s=round(el, 4), factor_s=round(tf, 4), solve_s=round(tsol, 4), bw=F.bw,
+               band_GB=round(F.n * F.ldab * 8 / 1e9, 1), factor_TFLOPs=round(F.n * F.bw ** 2 / tf / 1e12, 2),
+               true_relres=device_norm(r) / bnorm, oracle_err=err, ldlt=F.ldlt)
+    log(f"direct: {el:.3f} s (factor {tf:.3f} s, solve {tsol:.3f} s, bw {F.bw}, band {out['band_GB']} GB) "
+        f"relres {out['true_relres']:.3e} oracle err {err}")
+    del F, J, u, solver
+    BandCholesky.release()
+    torch.cuda.empty_cache()
+    return out
+
+
 def kernels_only(cfg, iters=8, copy_bytes=1 << 30):
     """A short run for hardware-counter passes: `iters` PCG iterations in
     each matrix form (so pcg_spmv_kernel is dispatched `iters` times per
@@ -460,6 +508,7 @@ def main():
     ap.add_argument("--no-newton", action="store_true", help="skip the Neo-Hookean Newton workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances workload")
     ap.add_argument("--no-gmg", action="store_true", help="skip the multigrid-preconditioned forms")
+    ap.add_argument("--no-direct", action="store_true", help="skip the banded-Cholesky direct solve")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch only the PCG SpMV kernels and a 1 GiB copy (for rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -533,6 +582,9 @@ def main():
             f"true relres={results[form]['relres']:.3e} oracle err={err}")
     du = results["mixed"]["u"] - results["fp64"]["u"]
     mixed_vs_fp64 = float(du.norm() / results["fp64"]["u"].norm())
+    for v in results.values():
+        v.pop("u", None)
+    direct = None if args.no_direct else direct_bench(problem, bc, tp, ts, u0, bnorm, oracle_fix)
 
     # Dominant kernel: the SpMV inside the PCG iterations.  Standalone launches
     # of the same kernels, timed with HIP events.
@@ -595,6 +647,7 @@ def main():
                  "in_solve_launches": {k: v[1] for k, v in spmv_live.items()},
                  "B_alg_fp64": B64, "B_alg_diff3": Bd, "B_min_fp64": Bmin64, "B_min_diff3": Bmind},
         "assemble_fp64_ms": round(t_asm64 * 1e3, 3),
+        "direct": direct,
         "grad_solve_ratio": None if grad is None else grad["ratio_adjoint"],
         "grad": grad,
         "newton": newton,
