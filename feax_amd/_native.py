@@ -55,6 +55,16 @@ SIGNATURES = {
     "fx_version": (_I32, [ctypes.c_char_p, _SZ]),
     "fx_hex8_pattern": (_I32, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "fx_hex8_face_incidence": (_I32, [_I32, _I32, _P, _I32, _P, _P, _P]),
+    "fx_tet4_pattern": (_I32, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "fx_tet4_face_incidence": (_I32, [_I32, _I32, _P, _I32, _P, _P, _P]),
+    "fx_tet4_assemble_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                    _I32, _I32, _F64, _P, _P, _P, _SZ]),
+    "fx_tet4_assemble_f32": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                    _I32, _I32, _F64, _P, _P, _P, _SZ]),
+    "fx_tet4_assemble_workspace_bytes": (_SZ, [_I32, _I32, _I32]),
+    "fx_tet4_residual_workspace_bytes": (_SZ, [_I32]),
+    "fx_tet4_residual_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P,
+                                    _P, _P, _I32, _P, _P, _I32, _P, _P, _SZ]),
     "fx_csr_is_block3": (_I32, [_I32, _P, _P, _P]),
     "fx_csr_spmv_f64": (_I32, [_P, _I32, _I64, _P, _P, _P, _P, _P]),
     "fx_csr_spmv_f32": (_I32, [_P, _I32, _I64, _P, _P, _P, _P, _P]),

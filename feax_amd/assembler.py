@@ -29,6 +29,9 @@ class TracedStructure:
         i32 = dict(dtype=torch.int32, device=dev)
         P = problem.points
         self.device = dev
+        self.ele_type = problem.ele_type
+        self.tag = "hex8" if problem.ele_type == "HEX8" else "tet4"
+        self.num_face_quads = problem.fes[0].num_face_quads
         self.num_nodes = problem.num_total_nodes
         self.num_cells = problem.num_cells
         self.n = problem.num_total_dofs_all_vars
@@ -122,10 +125,12 @@ def resolve_material(problem, traced_params, ts):
 
 
 def surface_values(problem, traced_params, ts):
-    """(F*4,) float64: surface_vars[i][0] of every loaded boundary, concatenated."""
+    """(F*q,) float64: surface_vars[i][0] of every loaded boundary,
+    concatenated (q = 4 face Gauss points for HEX8, 1 for TET4)."""
     import torch
     if ts.num_faces == 0:
         return None
+    q = ts.num_face_quads
     sv = traced_params.surface_vars if traced_params is not None else []
     parts = []
     for i in range(len(problem.boundary_inds_list)):
@@ -133,11 +138,17 @@ def surface_values(problem, traced_params, ts):
         if i < len(sv) and len(sv[i]):
             t = _as_dev(sv[i][0], ts)
             if t.numel() == nf:
-                t = t.reshape(nf, 1).expand(nf, 4)
-            parts.append(t.reshape(nf, 4))
+                t = t.reshape(nf, 1).expand(nf, q)
+            parts.append(t.reshape(nf, q))
         else:
-            parts.append(torch.zeros((nf, 4), dtype=torch.float64, device=ts.device))
+            parts.append(torch.zeros((nf, q), dtype=torch.float64, device=ts.device))
     return torch.cat(parts).reshape(-1).contiguous()
+
+
+def hex8_only(ts, what):
+    """The kernels of `what` exist for HEX8 only."""
+    if ts.tag != "hex8":
+        raise NotImplementedError(f"{what} is implemented for HEX8 meshes only (got {ts.ele_type})")
 
 
 # Element workspace of the two-pass assembly / residual (576 or 24 values
@@ -148,9 +159,10 @@ _ELEM_WS = {}
 
 
 def elem_workspace(ts, nbytes):
-    """(pointer, size) of the element workspace, or (None, 0) above the cap."""
+    """(pointer, size) of the element workspace, or (None, 0) above the cap
+    (HEX8 then runs single-pass; the TET4 kernels always take it)."""
     import torch
-    if nbytes <= 0 or nbytes > ELEM_WORKSPACE_CAP:
+    if nbytes <= 0 or (nbytes > ELEM_WORKSPACE_CAP and ts.tag == "hex8"):
         return None, 0
     key = str(ts.device)
     ws = _ELEM_WS.get(key)
@@ -163,7 +175,7 @@ def elem_workspace(ts, nbytes):
 
 def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, dtype=None, diff_form=False,
                  jacobi_shift=None):
-    """BC-eliminated CSR Jacobian on the device (fx_hex8_assemble_*).
+    """BC-eliminated CSR Jacobian on the device (fx_hex8_assemble_* / fx_tet4_assemble_*).
 
     diff_form: FP32 difference form for the mixed solver.  jacobi_shift:
     also return dinv = 1 / (diag + shift) (FP64) computed in the same pass.
@@ -176,9 +188,10 @@ def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, 
     mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
     u = _as_dev(sol, ts) if sol is not None else None
     dinv = torch.empty(ts.n, dtype=torch.float64, device=ts.device) if jacobi_shift is not None else None
-    name = "fx_hex8_assemble_f64" if dtype == torch.float64 else "fx_hex8_assemble_f32"
+    name = f"fx_{ts.tag}_assemble_f64" if dtype == torch.float64 else f"fx_{ts.tag}_assemble_f32"
     flags = nat.FX_ASM_DIFF3 if diff_form else 0
-    wp, wn = elem_workspace(ts, nat.lib().fx_hex8_assemble_workspace_bytes(ts.num_cells, data.element_size(), flags))
+    wsb = getattr(nat.lib(), f"fx_{ts.tag}_assemble_workspace_bytes")(ts.num_cells, data.element_size(), flags)
+    wp, wn = elem_workspace(ts, wsb)
     nat.call(name, nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
              nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), nat.ptr(ts.rank), nat.ptr(ts.indptr),
              nat.ptr(ts.indices), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u), nat.ptr(mask), int(bool(symmetric)),
@@ -202,12 +215,12 @@ def residual(problem, ts, traced_params, sol, bc=None, negate=False, out=None):
         rows = vals = None
         nbc = 0
     nf = ts.num_faces
-    nat.call("fx_hex8_residual_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
+    nat.call(f"fx_{ts.tag}_residual_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
              nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u),
              nf, nat.ptr(ts.faces) if nf else None, nat.ptr(ts.face_inc_ptr) if nf else None,
              nat.ptr(ts.face_inc) if nf else None, nat.ptr(sv), nat.ptr(ts.fdir) if nf else None, nbc,
              nat.ptr(rows), nat.ptr(vals), int(bool(negate)), nat.ptr(res),
-             *elem_workspace(ts, nat.lib().fx_hex8_residual_workspace_bytes(ts.num_cells)))
+             *elem_workspace(ts, getattr(nat.lib(), f"fx_{ts.tag}_residual_workspace_bytes")(ts.num_cells)))
     return res
 
 
@@ -227,10 +240,10 @@ def volume_residual(problem, ts, traced_params, sol, bc=None, negate=False, out=
     else:
         rows = vals = None
         nbc = 0
-    nat.call("fx_hex8_residual_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
+    nat.call(f"fx_{ts.tag}_residual_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
              nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat),
              nat.ptr(p), nat.ptr(u), 0, None, None, None, None, None, nbc, nat.ptr(rows), nat.ptr(vals),
-             int(bool(negate)), nat.ptr(res), *elem_workspace(ts, nat.lib().fx_hex8_residual_workspace_bytes(ts.num_cells)))
+             int(bool(negate)), nat.ptr(res), *elem_workspace(ts, getattr(nat.lib(), f"fx_{ts.tag}_residual_workspace_bytes")(ts.num_cells)))
     return res
 
 
@@ -243,6 +256,7 @@ def jvp(problem, ts, traced_params, sol, v, bc=None, out=None):
     v = _as_dev(v, ts)
     mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
     res = torch.empty(ts.n, dtype=torch.float64, device=ts.device) if out is None else out
+    hex8_only(ts, "the matrix-free operator")
     nat.call("fx_hex8_jvp_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
              nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat), nat.ptr(p),
              nat.ptr(u), nat.ptr(v), nat.ptr(mask), nat.ptr(res))
@@ -274,6 +288,7 @@ def param_vjp(problem, ts, traced_params, sol, lam, bc=None, scale=1.0):
     grad = torch.empty_like(p, dtype=torch.float64)
     wsb = int(nat.lib().fx_hex8_param_vjp_workspace_bytes(ts.num_cells, mat.param_loc))
     ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=ts.device)
+    hex8_only(ts, "the parameter VJP (gradients)")
     nat.call("fx_hex8_param_vjp_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
              nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat),
              nat.ptr(p), nat.ptr(u), nat.ptr(lam), nat.ptr(mask), float(scale), nat.ptr(grad), nat.ptr(ws), wsb)
@@ -289,6 +304,7 @@ def traction_vjp(problem, ts, lam, bc=None, scale=1.0):
     lam = _as_dev(lam, ts)
     mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
     g = torch.empty(ts.num_faces * 4, dtype=torch.float64, device=ts.device)
+    hex8_only(ts, "the traction VJP (gradients)")
     nat.call("fx_hex8_traction_vjp_f64", nat.stream(), ts.num_faces, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
              nat.ptr(ts.cells), nat.ptr(ts.faces), nat.ptr(ts.fdir), nat.ptr(lam), nat.ptr(mask), float(scale),
              nat.ptr(g))
@@ -320,6 +336,7 @@ def strain_energy(problem, ts, traced_params, sol, with_partial=False):
     dmu = torch.empty(ts.num_cells, dtype=torch.float64, device=ts.device) if with_partial else None
     wsb = int(nat.lib().fx_hex8_energy_workspace_bytes(ts.num_cells))
     ws = torch.empty(wsb, dtype=torch.uint8, device=ts.device)
+    hex8_only(ts, "the strain energy")
     nat.call("fx_hex8_energy_f64", nat.stream(), ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
              nat.ptr(ts.cells), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u), nat.ptr(out), nat.ptr(dmu),
              nat.ptr(ws), wsb)

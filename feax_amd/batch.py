@@ -71,7 +71,7 @@ class BatchedProblem(Problem):
         self.instances = problems
         self._tensor_map, self._surface_maps = tm0, maps0
         self._energy = p0.get_energy_density() is not None and p0.get_tensor_map() is None
-        self.vec, self.dim, self.ele_type, self.gauss_order = 3, 3, "HEX8", None
+        self.vec, self.dim, self.ele_type, self.gauss_order, self.npe = 3, 3, "HEX8", None, 8
         self.location_fns, self.matrix_view, self.additional_info = None, MatrixView.FULL, ()
         self.nseg = len(problems)
         self.seg_node_ptr = np.cumsum([0] + [p.num_total_nodes for p in problems]).astype(np.int32)

@@ -28,75 +28,15 @@
 // residual has the same two forms.
 #include <cstdint>
 
+#include "cell_common.h"
 #include "hex8_common.h"
 #include "pcg_common.h"
 #include "reduce.h"
 
 namespace fx {
 
-constexpr int kNodesPerBlock = 4;  // one wave per node, 4 waves per block
-constexpr int kMaxDeg = 64;        // LDS accumulator: 9 * kMaxDeg values per wave
 
 // ------------------------------------------------------------------ assembly
-// The node's three CSR rows from its LDS accumulator (acc: row i, block kb,
-// column j at i * len + 3 kb + j): Jacobi dinv, the difference form, and the
-// store with the Dirichlet elimination of feax/DCboundary.py:147-199.
-template <typename T, typename C>
-__device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, int row0, int nval,
-                                            const int* __restrict__ inc_ptr, const int* __restrict__ inc,
-                                            const uint8_t* __restrict__ rank, const int* __restrict__ indices,
-                                            const uint8_t* __restrict__ bc_mask, int symmetric, int diff_form,
-                                            double jacobi_shift, double* __restrict__ dinv, T* __restrict__ data) {
-  __builtin_amdgcn_wave_barrier();
-  // Position of node nd in its own neighbour list (its diagonal 3x3 block).
-  int kself = 0;
-  {
-    const int pk = inc[inc_ptr[nd]];
-    const int a0 = pk & 7;
-    kself = rank[64 * static_cast<long>(pk >> 3) + 9 * a0];
-  }
-  if (dinv && lane < 3) {
-    // Jacobi: 1 / (K_bc[r, r] + shift) (feax/solvers/common.py:86-94); BC rows have diagonal 1.
-    const int r = 3 * nd + lane;
-    const double d = (bc_mask && bc_mask[r]) ? 1.0 : static_cast<double>(acc[lane * len + 3 * kself + lane]);
-    dinv[r] = 1.0 / (d + jacobi_shift);
-  }
-  if (diff_form) {
-    // Difference form: the diagonal block slot holds S = block row sums of
-    // the BC-eliminated matrix.  Unconstrained element rows sum to zero
-    // (sum_b grad N_b = 0), so for a free row S[c][d] = -(sum of its entries
-    // in eliminated columns (m, d)); a BC row has S[c][d] = delta_cd.
-    C S = C(0);
-    const int c = lane / 3, d = lane - 3 * (lane / 3);
-    if (lane < 9) {
-      if (bc_mask && bc_mask[3 * nd + c]) {
-        S = (c == d) ? C(1) : C(0);
-      } else if (bc_mask) {
-        for (int kb = 0; kb < len / 3; ++kb) {
-          const int m = indices[row0 + 3 * kb] / 3;
-          if (bc_mask[3 * m + d]) S -= acc[c * len + 3 * kb + d];
-        }
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (lane < 9) acc[c * len + 3 * kself + d] = S;
-    __builtin_amdgcn_wave_barrier();
-  }
-  for (int idx = lane; idx < nval; idx += 64) {
-    C v = acc[idx];
-    const int slot = row0 + idx;
-    const bool own = ((idx % len) / 3) == kself;
-    if (bc_mask && !(diff_form && own)) {
-      const int r = 3 * nd + idx / len;
-      const int c = indices[slot];
-      const bool zr = bc_mask[r] != 0;
-      if (zr || (symmetric && bc_mask[c] != 0)) v = C(0);
-      if (zr && r == c) v = C(1);
-    }
-    data[slot] = static_cast<T>(v);
-  }
-}
-
 // NH: Neo-Hookean tangent at the state u (the Gauss-point deformation
 // gradient is formed from the element's 8 nodal displacements across the
 // 8 lanes of each Gauss-point group).
@@ -345,41 +285,6 @@ __global__ __launch_bounds__(64 * kElemPerBlock) void elem_matrix_kernel(
     C* dst = ke + 576 * static_cast<long>(e);
     for (int i = lane; i < 576; i += 64) dst[i] = KS[i];
   }
-}
-
-template <typename T, typename C>
-__global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
-    int num_nodes, const int* __restrict__ inc_ptr, const int* __restrict__ inc, const uint8_t* __restrict__ rank,
-    const int* __restrict__ indptr, const int* __restrict__ indices, const C* __restrict__ ke,
-    const uint8_t* __restrict__ bc_mask, int symmetric, int diff_form, double jacobi_shift,
-    double* __restrict__ dinv, T* __restrict__ data) {
-  __shared__ C acc_all[kNodesPerBlock][9 * kMaxDeg];
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int nd = blockIdx.x * kNodesPerBlock + wid;
-  if (nd >= num_nodes) return;
-  C* acc = acc_all[wid];
-  const int row0 = indptr[3 * nd];
-  const int len = indptr[3 * nd + 1] - row0;  // 3 * deg
-  const int nval = 3 * len;
-  if (len > 3 * kMaxDeg) return;  // host guarantees deg <= kMaxDeg
-  for (int i = lane; i < nval; i += 64) acc[i] = C(0);
-  __builtin_amdgcn_wave_barrier();
-  for (int k = inc_ptr[nd]; k < inc_ptr[nd + 1]; ++k) {
-    const int pk = inc[k];
-    const int e = pk >> 3, a = pk & 7;
-    const C* src = ke + 576 * static_cast<long>(e) + 72 * a;  // blocks (a, b), b = 0..7
-    const uint8_t* rk = rank + 64 * static_cast<long>(e) + 8 * a;
-    // 72 values: lane idx -> block b = idx / 9, entry t = idx % 9; distinct
-    // lanes touch distinct accumulator entries.
-    for (int idx = lane; idx < 72; idx += 64) {
-      const int b = idx / 9, t = idx - 9 * b;
-      const int i = t / 3, j = t - 3 * i;
-      acc[i * len + 3 * rk[b] + j] += src[idx];
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-  finish_rows<T, C>(nd, lane, acc, len, row0, nval, inc_ptr, inc, rank, indices, bc_mask, symmetric, diff_form,
-                    jacobi_shift, dinv, data);
 }
 
 // ------------------------------------------------------------------ residual
@@ -700,15 +605,6 @@ void launch_hex8_op(hipStream_t s, const Hex8Op& op, const double* v, double* ou
     else FX_JVP_LAUNCH(false, false);
   }
 #undef FX_JVP_LAUNCH
-}
-
-__global__ void bc_res_kernel(int nbc, const int* __restrict__ rows, const double* __restrict__ vals,
-                              const double* __restrict__ u, int negate, double* __restrict__ res) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nbc) return;
-  const int r = rows[i];
-  const double v = (u ? u[r] : 0.0) - vals[i];
-  res[r] = negate ? -v : v;
 }
 
 // Bytes of the element-matrix workspace: 576 values per element, FP64

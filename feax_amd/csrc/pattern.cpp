@@ -1,4 +1,4 @@
-// Host-side structure builders for HEX8, vec = 3 meshes.
+// Host-side structure builders for HEX8 and TET4, vec = 3 meshes.
 //
 // fx_hex8_pattern produces the CSR pattern that feax builds in
 // Problem._build_csr_assembly_structure (feax/problem.py:316-484): the
@@ -21,30 +21,33 @@ namespace {
 const int kFaceVerts[6][4] = {{0, 1, 3, 2}, {0, 1, 4, 5}, {0, 3, 4, 7},
                               {1, 2, 5, 6}, {3, 2, 7, 6}, {4, 5, 7, 6}};
 
-}  // namespace
+// basix tetrahedron facets: facet i is the triangle opposite vertex i
+// (feax keeps basix's TET4 vertex order, feax/basis.py:78-83).
+const int kTetFaceVerts[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
 
-extern "C" int fx_hex8_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells,
-                               int64_t* nse_out, int32_t* max_degree, int32_t* indptr,
-                               int32_t* indices, uint8_t* elem_rank, int32_t* inc_ptr,
-                               int32_t* inc) {
+// The CSR pattern, element ranks and node incidence of a mesh of NPE-node
+// cells (the incidence entry of node a of element e is e * 8 + a).
+int cell_pattern(const char* name, int npe, int32_t num_nodes, int32_t num_cells, const int32_t* cells,
+                 int64_t* nse_out, int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                 int32_t* inc_ptr, int32_t* inc) {
   if (num_nodes <= 0 || num_cells < 0 || !cells || !nse_out || !indptr)
-    return fx::fail(FX_EINVAL, "fx_hex8_pattern: bad arguments");
+    return fx::fail(FX_EINVAL, "%s: bad arguments", name);
   if (static_cast<int64_t>(num_nodes) * 3 >= INT32_MAX || static_cast<int64_t>(num_cells) * 8 >= INT32_MAX)
-    return fx::fail(FX_ELIMIT, "fx_hex8_pattern: mesh too large for int32 indices");
+    return fx::fail(FX_ELIMIT, "%s: mesh too large for int32 indices", name);
   const int64_t E = num_cells, N = num_nodes;
   // node -> (element, local) incidence, ascending element id per node.
   std::vector<int32_t> iptr(N + 1, 0);
-  for (int64_t k = 0; k < E * 8; ++k) {
+  for (int64_t k = 0; k < E * npe; ++k) {
     int32_t nd = cells[k];
-    if (nd < 0 || nd >= num_nodes) return fx::fail(FX_EINVAL, "fx_hex8_pattern: node id %d out of range", nd);
+    if (nd < 0 || nd >= num_nodes) return fx::fail(FX_EINVAL, "%s: node id %d out of range", name, nd);
     iptr[nd + 1]++;
   }
   for (int64_t i = 0; i < N; ++i) iptr[i + 1] += iptr[i];
-  std::vector<int32_t> ilist(E * 8);
+  std::vector<int32_t> ilist(E * npe);
   {
     std::vector<int32_t> pos(iptr.begin(), iptr.end() - 1);
     for (int64_t e = 0; e < E; ++e)
-      for (int a = 0; a < 8; ++a) ilist[pos[cells[e * 8 + a]]++] = static_cast<int32_t>(e * 8 + a);
+      for (int a = 0; a < npe; ++a) ilist[pos[cells[e * npe + a]]++] = static_cast<int32_t>(e * 8 + a);
   }
   // neighbour lists
   std::vector<int32_t> nptr(N + 1, 0);
@@ -56,7 +59,7 @@ extern "C" int fx_hex8_pattern(int32_t num_nodes, int32_t num_cells, const int32
     tmp.clear();
     for (int32_t k = iptr[nd]; k < iptr[nd + 1]; ++k) {
       int32_t e = ilist[k] >> 3;
-      for (int b = 0; b < 8; ++b) tmp.push_back(cells[static_cast<int64_t>(e) * 8 + b]);
+      for (int b = 0; b < npe; ++b) tmp.push_back(cells[static_cast<int64_t>(e) * npe + b]);
     }
     std::sort(tmp.begin(), tmp.end());
     tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
@@ -64,14 +67,14 @@ extern "C" int fx_hex8_pattern(int32_t num_nodes, int32_t num_cells, const int32
     nptr[nd + 1] = static_cast<int32_t>(nbr.size());
     maxdeg = std::max<int32_t>(maxdeg, static_cast<int32_t>(tmp.size()));
   }
-  if (maxdeg > 255) return fx::fail(FX_ELIMIT, "fx_hex8_pattern: node degree %d > 255", maxdeg);
+  if (maxdeg > 255) return fx::fail(FX_ELIMIT, "%s: node degree %d > 255", name, maxdeg);
   int64_t nse = 0;
   indptr[0] = 0;
   for (int64_t nd = 0; nd < N; ++nd) {
     int64_t deg = nptr[nd + 1] - nptr[nd];
     for (int i = 0; i < 3; ++i) {
       nse += 3 * deg;
-      if (nse >= INT32_MAX) return fx::fail(FX_ELIMIT, "fx_hex8_pattern: nse exceeds int32");
+      if (nse >= INT32_MAX) return fx::fail(FX_ELIMIT, "%s: nse exceeds int32", name);
       indptr[nd * 3 + i + 1] = static_cast<int32_t>(nse);
     }
   }
@@ -87,19 +90,35 @@ extern "C" int fx_hex8_pattern(int32_t num_nodes, int32_t num_cells, const int32
   }
   if (elem_rank) {
     for (int64_t e = 0; e < E; ++e)
-      for (int a = 0; a < 8; ++a) {
-        int32_t nd = cells[e * 8 + a];
+      for (int a = 0; a < npe; ++a) {
+        int32_t nd = cells[e * npe + a];
         const int32_t* lo = nbr.data() + nptr[nd];
         const int32_t* hi = nbr.data() + nptr[nd + 1];
-        for (int b = 0; b < 8; ++b) {
-          const int32_t* it = std::lower_bound(lo, hi, cells[e * 8 + b]);
-          elem_rank[e * 64 + a * 8 + b] = static_cast<uint8_t>(it - lo);
+        for (int b = 0; b < npe; ++b) {
+          const int32_t* it = std::lower_bound(lo, hi, cells[e * npe + b]);
+          elem_rank[(e * npe + a) * npe + b] = static_cast<uint8_t>(it - lo);
         }
       }
   }
   if (inc_ptr) std::copy(iptr.begin(), iptr.end(), inc_ptr);
   if (inc) std::copy(ilist.begin(), ilist.end(), inc);
   return FX_OK;
+}
+
+}  // namespace
+
+extern "C" int fx_hex8_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse_out,
+                               int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                               int32_t* inc_ptr, int32_t* inc) {
+  return cell_pattern("fx_hex8_pattern", 8, num_nodes, num_cells, cells, nse_out, max_degree, indptr, indices,
+                      elem_rank, inc_ptr, inc);
+}
+
+extern "C" int fx_tet4_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse_out,
+                               int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                               int32_t* inc_ptr, int32_t* inc) {
+  return cell_pattern("fx_tet4_pattern", 4, num_nodes, num_cells, cells, nse_out, max_degree, indptr, indices,
+                      elem_rank, inc_ptr, inc);
 }
 
 extern "C" int fx_hex8_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells,
@@ -122,6 +141,31 @@ extern "C" int fx_hex8_face_incidence(int32_t num_nodes, int32_t num_cells, cons
     for (int v = 0; v < 4; ++v) {
       int a = kFaceVerts[lf][v];
       finc[pos[cells[static_cast<int64_t>(e) * 8 + a]]++] = f * 8 + a;
+    }
+  }
+  return FX_OK;
+}
+
+extern "C" int fx_tet4_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells,
+                                      int32_t num_faces, const int32_t* faces, int32_t* finc_ptr,
+                                      int32_t* finc) {
+  if (num_nodes <= 0 || num_faces < 0 || !cells || !finc_ptr || (num_faces > 0 && (!faces || !finc)))
+    return fx::fail(FX_EINVAL, "fx_tet4_face_incidence: bad arguments");
+  std::vector<int32_t> cnt(static_cast<size_t>(num_nodes) + 1, 0);
+  for (int32_t f = 0; f < num_faces; ++f) {
+    int32_t e = faces[2 * f], lf = faces[2 * f + 1];
+    if (e < 0 || e >= num_cells || lf < 0 || lf > 3)
+      return fx::fail(FX_EINVAL, "fx_tet4_face_incidence: bad face (%d, %d)", e, lf);
+    for (int v = 0; v < 3; ++v) cnt[cells[static_cast<int64_t>(e) * 4 + kTetFaceVerts[lf][v]] + 1]++;
+  }
+  for (int32_t i = 0; i < num_nodes; ++i) cnt[i + 1] += cnt[i];
+  std::copy(cnt.begin(), cnt.end(), finc_ptr);
+  std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
+  for (int32_t f = 0; f < num_faces; ++f) {
+    int32_t e = faces[2 * f], lf = faces[2 * f + 1];
+    for (int v = 0; v < 3; ++v) {
+      int a = kTetFaceVerts[lf][v];
+      finc[pos[cells[static_cast<int64_t>(e) * 4 + a]]++] = f * 8 + a;
     }
   }
   return FX_OK;

@@ -1,4 +1,4 @@
-"""HEX8 reference-element facts needed on the host.
+"""HEX8 / TET4 reference-element facts needed on the host.
 
 Geometry itself (shape gradients, JxW, Nanson scale) is never stored: the
 HIP kernels recompute it per Gauss point from the node coordinates.  The
@@ -14,6 +14,18 @@ HEX8_FACE_INDS = np.array([[0, 1, 3, 2], [0, 1, 4, 5], [0, 3, 4, 7],
                            [1, 2, 5, 6], [3, 2, 7, 6], [4, 5, 7, 6]], dtype=np.int64)
 NUM_QUADS = 8        # 2x2x2 Gauss points
 NUM_FACE_QUADS = 4   # 2x2 per facet
+
+# TET4 (feax/basis.py:78-83): basix tetrahedron facets, facet i opposite
+# vertex i; one Gauss point per cell and per facet.
+TET4_FACE_INDS = np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]], dtype=np.int64)
+ELEMENTS = {"HEX8": dict(npe=8, faces=HEX8_FACE_INDS, num_quads=NUM_QUADS, num_face_quads=NUM_FACE_QUADS),
+            "TET4": dict(npe=4, faces=TET4_FACE_INDS, num_quads=1, num_face_quads=1)}
+
+
+def element_info(ele_type):
+    if ele_type not in ELEMENTS:
+        raise NotImplementedError(f"the HIP path covers {sorted(ELEMENTS)} (got {ele_type})")
+    return ELEMENTS[ele_type]
 
 
 def eval_points(fn, pts):
@@ -38,14 +50,14 @@ def eval_points(fn, pts):
     return np.asarray([fn(p) for p in pts])
 
 
-def boundary_conditions_inds(points, cells, location_fns):
-    """List of (F, 2) int arrays (cell, local face) whose 4 vertices all satisfy
+def boundary_conditions_inds(points, cells, location_fns, ele_type="HEX8"):
+    """List of (F, 2) int arrays (cell, local face) whose vertices all satisfy
     each location_fn, in np.argwhere (cell, face) order -- feax/fe.py:267-318.
     Two-argument location functions receive (point, node index)."""
     out = []
     if not location_fns:
         return out
-    fverts = cells[:, HEX8_FACE_INDS]                      # (E, 6, 4) node ids
+    fverts = cells[:, element_info(ele_type)["faces"]]     # (E, faces, verts) node ids
     for fn in location_fns:
         nargs = fn.__code__.co_argcount if hasattr(fn, "__code__") else 1
         nodes = np.arange(len(points))

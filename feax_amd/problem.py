@@ -21,7 +21,7 @@ from typing import Any, Callable, List, Optional, Tuple, Union
 import numpy as np
 
 from . import _native as nat
-from .fe import NUM_FACE_QUADS, NUM_QUADS, boundary_conditions_inds
+from .fe import boundary_conditions_inds, element_info
 from .mesh import Mesh
 
 
@@ -36,19 +36,21 @@ class FiniteElement:
     """The per-variable facts feax.FiniteElement exposes that callers read
     (feax/fe.py:20-94): counts only; geometry lives in the kernels."""
 
-    def __init__(self, mesh, vec, dim):
+    def __init__(self, mesh, vec, dim, ele_type="HEX8"):
+        info = element_info(ele_type)
         self.mesh = mesh
         self.points = mesh.points
         self.cells = mesh.cells
         self.vec = vec
         self.dim = dim
+        self.ele_type = ele_type
         self.num_cells = len(mesh.cells)
         self.num_total_nodes = len(mesh.points)
         self.num_total_dofs = self.num_total_nodes * vec
-        self.num_nodes = 8
-        self.num_quads = NUM_QUADS
-        self.num_face_quads = NUM_FACE_QUADS
-        self.num_faces = 6
+        self.num_nodes = info["npe"]
+        self.num_quads = info["num_quads"]
+        self.num_face_quads = info["num_face_quads"]
+        self.num_faces = len(info["faces"])
 
 
 @dataclass
@@ -75,44 +77,49 @@ class Problem:
             self.mesh, self.vec = self.mesh[0], self.vec[0]
             self.ele_type = self.ele_type[0] if isinstance(self.ele_type, list) else self.ele_type
             self.gauss_order = self.gauss_order[0] if isinstance(self.gauss_order, list) else self.gauss_order
-        if self.ele_type != "HEX8" or self.vec != 3 or self.dim != 3:
+        if self.ele_type not in ("HEX8", "TET4") or self.vec != 3 or self.dim != 3:
             raise NotImplementedError(
-                f"the HIP path covers HEX8 with vec=3, dim=3 (got {self.ele_type}, vec={self.vec}, dim={self.dim})")
-        if self.gauss_order not in (None, 2):
-            raise NotImplementedError("HEX8 uses the 2x2x2 Gauss rule (gauss_order=2)")
+                f"the HIP path covers HEX8 and TET4 with vec=3, dim=3 (got {self.ele_type}, vec={self.vec}, "
+                f"dim={self.dim})")
+        default_order = 2 if self.ele_type == "HEX8" else 0
+        if self.gauss_order not in (None, default_order):
+            raise NotImplementedError(f"{self.ele_type} uses feax's default rule (gauss_order={default_order})")
+        self.npe = element_info(self.ele_type)["npe"]
         self.num_vars = 1
-        self.fes = [FiniteElement(self.mesh, self.vec, self.dim)]
+        self.fes = [FiniteElement(self.mesh, self.vec, self.dim, self.ele_type)]
         self.points = np.ascontiguousarray(self.mesh.points, dtype=np.float64)
         self.cells = np.ascontiguousarray(self.mesh.cells, dtype=np.int32)
-        if self.cells.ndim != 2 or self.cells.shape[1] != 8:
-            raise ValueError(f"HEX8 cells must be (E, 8), got {self.cells.shape}")
+        if self.cells.ndim != 2 or self.cells.shape[1] != self.npe:
+            raise ValueError(f"{self.ele_type} cells must be (E, {self.npe}), got {self.cells.shape}")
         self.cells_list = [self.cells]
         self.num_cells = len(self.cells)
         self.num_total_nodes = len(self.points)
         self.num_total_dofs_all_vars = 3 * self.num_total_nodes
         self.offset = [0]
-        self.boundary_inds_list = boundary_conditions_inds(self.points, self.cells, self.location_fns)
+        self.boundary_inds_list = boundary_conditions_inds(self.points, self.cells, self.location_fns, self.ele_type)
         self._build_csr_assembly_structure()
         self.custom_init(*self.additional_info)
 
     # ------------------------------------------------------------------ structure
     def _build_csr_assembly_structure(self):
-        """CSR pattern + assembly maps (feax/problem.py:316-484) via fx_hex8_pattern."""
-        N, E = self.num_total_nodes, self.num_cells
+        """CSR pattern + assembly maps (feax/problem.py:316-484) via fx_hex8_pattern / fx_tet4_pattern."""
+        N, E, npe = self.num_total_nodes, self.num_cells, self.npe
         n = 3 * N
         L = nat.lib()
+        tag = "hex8" if self.ele_type == "HEX8" else "tet4"
+        pattern = getattr(L, f"fx_{tag}_pattern")
         nse = ctypes.c_int64(0)
         maxdeg = ctypes.c_int32(0)
         indptr = np.zeros(n + 1, dtype=np.int32)
-        nat.check(L.fx_hex8_pattern(N, E, nat.ptr(self.cells), ctypes.addressof(nse), ctypes.addressof(maxdeg),
-                                    nat.ptr(indptr), None, None, None, None), "fx_hex8_pattern")
+        nat.check(pattern(N, E, nat.ptr(self.cells), ctypes.addressof(nse), ctypes.addressof(maxdeg),
+                          nat.ptr(indptr), None, None, None, None), f"fx_{tag}_pattern")
         indices = np.empty(nse.value, dtype=np.int32)
-        rank = np.empty(E * 64, dtype=np.uint8)
+        rank = np.empty(E * npe * npe, dtype=np.uint8)
         inc_ptr = np.empty(N + 1, dtype=np.int32)
-        inc = np.empty(E * 8, dtype=np.int32)
-        nat.check(L.fx_hex8_pattern(N, E, nat.ptr(self.cells), ctypes.addressof(nse), ctypes.addressof(maxdeg),
-                                    nat.ptr(indptr), nat.ptr(indices), nat.ptr(rank), nat.ptr(inc_ptr),
-                                    nat.ptr(inc)), "fx_hex8_pattern")
+        inc = np.empty(E * npe, dtype=np.int32)
+        nat.check(pattern(N, E, nat.ptr(self.cells), ctypes.addressof(nse), ctypes.addressof(maxdeg),
+                          nat.ptr(indptr), nat.ptr(indices), nat.ptr(rank), nat.ptr(inc_ptr), nat.ptr(inc)),
+                  f"fx_{tag}_pattern")
         self.csr_nse = int(nse.value)
         self.csr_indptr = indptr
         self.csr_indices = indices
@@ -128,9 +135,10 @@ class Problem:
         allf = (np.concatenate(faces) if faces else np.zeros((0, 2), np.int64)).astype(np.int32)
         self.all_faces = np.ascontiguousarray(allf)
         finc_ptr = np.zeros(N + 1, dtype=np.int32)
-        finc = np.zeros(max(4 * len(allf), 1), dtype=np.int32)
-        nat.check(L.fx_hex8_face_incidence(N, E, nat.ptr(self.cells), len(allf), nat.ptr(self.all_faces),
-                                           nat.ptr(finc_ptr), nat.ptr(finc)), "fx_hex8_face_incidence")
+        finc = np.zeros(max(len(element_info(self.ele_type)["faces"][0]) * len(allf), 1), dtype=np.int32)
+        nat.check(getattr(L, f"fx_{tag}_face_incidence")(N, E, nat.ptr(self.cells), len(allf),
+                                                         nat.ptr(self.all_faces), nat.ptr(finc_ptr), nat.ptr(finc)),
+                  f"fx_{tag}_face_incidence")
         self.face_inc_ptr, self.face_inc = finc_ptr, finc
         if self.csr_max_degree > 64:
             raise NotImplementedError(f"node degree {self.csr_max_degree} > 64 not supported by the assembly kernel")
@@ -186,16 +194,16 @@ class Problem:
 
     @property
     def csr_volume_slots(self):
-        """(E, 576) int32: CSR slot of each element Jacobian entry jac[a*3+i,
+        """(E, (3 npe)^2) int32: CSR slot of each element Jacobian entry jac[a*3+i,
         b*3+j] (feax/problem.py:458-463), from the rank table:
         indptr[3 n_a + i] + 3 rank[e, a, b] + j."""
         if getattr(self, "_vol_slots", None) is None:
             c = self.cells.astype(np.int64)
-            rank = self.elem_rank.reshape(-1, 8, 8).astype(np.int64)
+            rank = self.elem_rank.reshape(-1, self.npe, self.npe).astype(np.int64)
             rowp = self.csr_indptr.astype(np.int64)[3 * c[:, :, None] + np.arange(3)]      # (E, 8a, 3i)
             slot = (rowp[:, :, :, None, None] + 3 * rank[:, :, None, :, None]
                     + np.arange(3)[None, None, None, None, :])                            # (E, a, i, b, j)
-            self._vol_slots = slot.reshape(len(c), 576).astype(np.int32)
+            self._vol_slots = slot.reshape(len(c), (3 * self.npe) ** 2).astype(np.int32)
         return self._vol_slots
 
     def _element_inds(self, cells):
@@ -205,8 +213,9 @@ class Problem:
     def _coo(self):
         inds = [self._element_inds(self.cells)] + [self._element_inds(self.cells[b[:, 0]])
                                                    for b in self.boundary_inds_list]
-        I = np.concatenate([np.repeat(i[:, :, None], 24, axis=2).reshape(-1) for i in inds])
-        J = np.concatenate([np.repeat(i[:, None, :], 24, axis=1).reshape(-1) for i in inds])
+        nd = 3 * self.npe
+        I = np.concatenate([np.repeat(i[:, :, None], nd, axis=2).reshape(-1) for i in inds])
+        J = np.concatenate([np.repeat(i[:, None, :], nd, axis=1).reshape(-1) for i in inds])
         return I, J
 
     @property

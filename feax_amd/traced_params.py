@@ -11,7 +11,7 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
-from .fe import NUM_FACE_QUADS, NUM_QUADS, eval_points
+from .fe import eval_points
 
 
 @dataclass(frozen=True)
@@ -35,12 +35,12 @@ class TracedParams:
 
     @staticmethod
     def create_uniform_volume_var(problem, value, var_index=0):
-        return np.full((problem.num_cells, NUM_QUADS), float(value))
+        return np.full((problem.num_cells, problem.fes[var_index].num_quads), float(value))
 
     @staticmethod
     def create_uniform_surface_var(problem, value, surface_index=0):
         nf = len(problem.boundary_inds_list[surface_index])
-        return np.full((nf, NUM_FACE_QUADS), float(value))
+        return np.full((nf, problem.fes[0].num_face_quads), float(value))
 
     @staticmethod
     def create_node_var_from_fn(problem, var_fn, var_index=0):

@@ -249,6 +249,43 @@ int fx_hex8_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, con
                          void* workspace, size_t ws_bytes);
 size_t fx_hex8_residual_workspace_bytes(int32_t num_cells);
 
+/* ------------------------------------------------------------------ */
+/* TET4 (feax/basis.py:78-83: basix P1 tetrahedron, one Gauss point; facet i
+ * opposite vertex i, one point): the structure builders and the two-pass
+ * kernels of the HEX8 entry points above, with cells (E, 4), elem_rank
+ * E * 16 (rank of node b in node a's neighbour list at 16 e + 4 a + b),
+ * node incidence entries e * 8 + a, faces (F, 2) (cell, local facet) and
+ * one surface value per face (svar F, fdir F * 3).  The workspace is
+ * required (fx_tet4_*_workspace_bytes: 144 values per element for the
+ * matrix, 12 doubles for the residual).  Same materials, flags and
+ * Dirichlet handling as fx_hex8_assemble_* / fx_hex8_residual_f64
+ * (feax/assembler.py:1362-1404, 1618-1667; feax/DCboundary.py:147-251). */
+int fx_tet4_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse,
+                    int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                    int32_t* inc_ptr, int32_t* inc);
+int fx_tet4_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int32_t num_faces,
+                           const int32_t* faces, int32_t* finc_ptr, int32_t* finc);
+int fx_tet4_assemble_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                         const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                         const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                         const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                         int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, double* data,
+                         void* workspace, size_t ws_bytes);
+int fx_tet4_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                         const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                         const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                         const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                         int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, float* data,
+                         void* workspace, size_t ws_bytes);
+size_t fx_tet4_assemble_workspace_bytes(int32_t num_cells, int32_t value_bytes, int32_t flags);
+int fx_tet4_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                         const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                         const fx_material* mat, const double* param, const double* u, int32_t num_faces,
+                         const int32_t* faces, const int32_t* finc_ptr, const int32_t* finc, const double* svar,
+                         const double* fdir, int32_t num_bc, const int32_t* bc_rows, const double* bc_vals,
+                         int32_t negate, double* res, void* workspace, size_t ws_bytes);
+size_t fx_tet4_residual_workspace_bytes(int32_t num_cells);
+
 /* Matrix-free operator out = J_bc(u) v (FP64): feax's Krylov operator
  * create_matfree_res_J_parametric (feax/assembler.py:1747-1778),
  * J_bc v = where(bc, v, JVP_R(u)[v * !bc]) under symmetric elimination: rows

@@ -95,7 +95,9 @@ def test_unsupported_problems_raise():
     with pytest.raises(NotImplementedError):
         fe.Problem(mesh, vec=1, dim=3)
     with pytest.raises(NotImplementedError):
-        fe.Problem(fe.Mesh(mesh.points, mesh.cells[:, :4], ele_type="TET4"), vec=3, dim=3, ele_type="TET4")
+        fe.Problem(mesh, vec=3, dim=3, ele_type="HEX20")
+    with pytest.raises(ValueError, match=r"\(E, 4\)"):
+        fe.Problem(mesh, vec=3, dim=3, ele_type="TET4")
 
 
 def test_dirichlet_bc_matches_oracle():
