@@ -1,0 +1,182 @@
+"""GPU parity of the TET4 kernels against the TET4 oracle (oracle/tet4.py):
+CSR assembly (FP64 / FP32, the three material laws), the residual with dead
+traction and Dirichlet rows, linear solves (CG and the direct solver) against
+scipy's direct solve, and Neo-Hookean Newton load steps against the oracle
+Newton."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPS64 = np.finfo(np.float64).eps
+EPS32 = np.finfo(np.float32).eps
+NU = 0.3
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _problem(kind, dims=(6, 3, 3), h=(1.0, 1.0, 1.0), seed=0, shuffle=True, load=True):
+    import feax_amd as fe
+    from oracle import tet4
+    pts, cells = tet4.box_tets(dims, h)
+    if shuffle:
+        rng = np.random.default_rng(seed)
+        perm = rng.permutation(len(pts))
+        pts, cells = pts[perm], np.argsort(perm)[cells][rng.permutation(len(cells))]
+    L = dims[0] * h[0]
+
+    class P(fe.Problem):
+        def get_tensor_map(self):
+            if kind == "simp":
+                return fe.materials.SIMPLinearElastic(nu=NU, E0=1.0, Emin=1e-3, penal=3.0)
+            if kind == "neo_hookean":
+                return None
+            return fe.materials.IsotropicLinearElastic(nu=NU)
+
+        def get_energy_density(self):
+            return fe.materials.CompressibleNeoHookean(kappa=10.0) if kind == "neo_hookean" else None
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.0, 0.0, 1.0))]
+
+    locs = [lambda p: np.isclose(p[0], L)] if load else []
+    prob = P(fe.Mesh(pts, cells, ele_type="TET4"), vec=3, dim=3, ele_type="TET4", location_fns=locs)
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[0], 0.0), "all", 0.0)]).create_bc(prob)
+    return prob, bc
+
+
+def _oracle(prob):
+    from oracle import tet4
+    return tet4.Model(prob.points, prob.cells.astype(np.int64), prob.boundary_inds_list)
+
+
+def _mat(kind):
+    from oracle import hex8
+    if kind == "simp":
+        return hex8.Material("simp", nu=NU, E0=1.0, Emin=1e-3, penal=3.0)
+    if kind == "neo_hookean":
+        return hex8.Material("neo_hookean", kappa=10.0)
+    return hex8.Material("linear_elastic", nu=NU)
+
+
+def _param(kind, prob, seed=1):
+    rng = np.random.default_rng(seed)
+    if kind == "linear_elastic":
+        return rng.uniform(50.0, 90.0, prob.num_total_nodes)      # node variable
+    if kind == "simp":
+        return rng.uniform(0.3, 1.0, prob.num_cells)
+    return rng.uniform(0.5, 1.5, prob.num_cells)
+
+
+@pytest.mark.parametrize("kind", ["linear_elastic", "simp", "neo_hookean"])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_tet4_assembly_matches_oracle(kind, dtype):
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import assemble_csr
+    from oracle import hex8
+    prob, bc = _problem(kind)
+    m = _oracle(prob)
+    ts = fe.TracedStructure.from_problem(prob)
+    param = _param(kind, prob)
+    u = 0.03 * np.random.default_rng(2).standard_normal(m.n) if kind == "neo_hookean" else np.zeros(m.n)
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    A = assemble_csr(prob, ts, fe.TracedParams(volume_vars=(param,)), torch.as_tensor(u, device="cuda"), bc, True, dt)
+    assert np.array_equal(_np(A.indptr), m.csr["indptr"]) and np.array_equal(_np(A.indices), m.csr["indices"])
+    mask = np.zeros(m.n, bool)
+    mask[bc.bc_rows] = True
+    ref = hex8.apply_bc_csr(m.csr_values(u, param, _mat(kind)), m.csr, mask, bc.bc_rows, True)
+    K = m.scipy_csr(ref)
+    scale = np.repeat(np.abs(K).max(axis=1).toarray().ravel(), np.diff(K.indptr))
+    err = np.abs(_np(A.data).astype(np.float64) - ref) / scale
+    tol = 64 * (EPS64 if dtype == "f64" else EPS32)
+    print(kind, dtype, "assembly max err / row max", err.max())
+    assert err.max() <= tol
+
+
+@pytest.mark.parametrize("kind", ["linear_elastic", "neo_hookean"])
+def test_tet4_residual_matches_oracle(kind):
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import residual
+    from oracle import hex8, tet4
+    prob, bc = _problem(kind)
+    m = _oracle(prob)
+    ts = fe.TracedStructure.from_problem(prob)
+    param = _param(kind, prob)
+    u = 0.03 * np.random.default_rng(4).standard_normal(m.n)
+    t = 0.37
+    tp = fe.TracedParams(volume_vars=(param,), surface_vars=[(fe.TracedParams.create_uniform_surface_var(prob, t),)])
+    r = _np(residual(prob, ts, tp, torch.as_tensor(u, device="cuda"), bc))
+    ref = m.residual(u, param, _mat(kind), [tet4.constant_traction(prob.boundary_inds_list[0], t)])
+    ref = hex8.apply_bc_res(ref, u, bc.bc_rows, bc.bc_vals)
+    err = np.abs(r - ref).max() / np.abs(ref).max()
+    print(kind, "residual rel err", err)
+    assert err <= 1e-13
+
+
+@pytest.mark.parametrize("opts", ["cg", "direct"])
+def test_tet4_linear_solve_matches_direct_solve(opts):
+    import scipy.sparse.linalg as spla
+    import feax_amd as fe
+    from oracle import tet4, krylov
+    prob, bc = _problem("simp", dims=(8, 3, 3))
+    m = _oracle(prob)
+    rho = _param("simp", prob)
+    t = 0.5
+    tp = fe.TracedParams(volume_vars=(rho,), surface_vars=[(fe.TracedParams.create_uniform_surface_var(prob, t),)])
+    o = fe.DirectSolverOptions() if opts == "direct" else \
+        fe.KrylovSolverOptions(solver="cg", tol=1e-11, atol=0.0, use_jacobi_preconditioner=True)
+    u = np.asarray(fe.create_solver(prob, bc, o, linear=True)(tp))
+    _, info = krylov.linear_solve(m, _mat("simp"), rho, [tet4.constant_traction(prob.boundary_inds_list[0], t)],
+                                  bc.bc_rows, bc.bc_vals, maxiter=1)
+    ud = spla.spsolve(info["K"].tocsc(), info["b"])
+    err = np.linalg.norm(u - ud) / np.linalg.norm(ud)
+    print(opts, "tet4 solve rel err vs scipy", err)
+    assert err < 1e-8
+
+
+def test_tet4_newton_matches_oracle_newton():
+    import torch
+    import feax_amd as fe
+    from oracle import newton as onewton
+    dims = (3, 3, 4)
+    prob, _ = _problem("neo_hookean", dims=dims, load=False, shuffle=False)
+    nz = dims[2]
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[2], 0.0), "all", 0.0),
+                               fe.DirichletBCSpec(lambda p: np.isclose(p[2], nz), "z", 0.0)]).create_bc(prob)
+    node, comp = bc.bc_rows // 3, bc.bc_rows % 3
+    is_top = np.isclose(prob.points[node, 2], nz) & (comp == 2)
+    m = _oracle(prob)
+    mu = _param("neo_hookean", prob)
+    solver = fe.create_solver(prob, bc, fe.DirectSolverOptions(), newton_options=fe.NewtonOptions(tol=1e-10,
+                              rel_tol=1e-12), linear=False, return_solution=False)
+    tp = fe.TracedParams(volume_vars=(mu,))
+    u = torch.zeros(m.n, dtype=torch.float64, device="cuda")
+    uo = np.zeros(m.n)
+    rows = torch.as_tensor(bc.bc_rows.astype(np.int64), device="cuda")
+    for s in (1, 2):
+        vals = np.where(is_top, -0.05 * nz * s, 0.0)
+        u[rows] = torch.as_tensor(vals, device="cuda")
+        u = solver(tp, u, bc.replace_vals(vals))
+        uo[bc.bc_rows] = vals
+        uo, it_o, _ = onewton.newton(m, _mat("neo_hookean"), mu, bc.bc_rows, vals, uo, tol=1e-10, rel_tol=1e-12)
+        err = np.linalg.norm(_np(u) - uo) / np.linalg.norm(uo)
+        print("tet4 newton step", s, "iters", solver.stats.newton_iters, "oracle", it_o, "err", err)
+        assert err < 1e-9 and solver.stats.newton_iters == it_o
+
+
+def test_tet4_gradients_are_reported_as_not_implemented():
+    import torch
+    import feax_amd as fe
+    prob, bc = _problem("simp", dims=(3, 2, 2))
+    r = torch.as_tensor(_param("simp", prob), device="cuda").requires_grad_(True)
+    tp = fe.TracedParams(volume_vars=(r,), surface_vars=[(fe.TracedParams.create_uniform_surface_var(prob, 0.5),)])
+    solver = fe.create_solver(prob, bc, fe.KrylovSolverOptions(solver="cg", use_jacobi_preconditioner=True),
+                              linear=True, return_solution=False)
+    u = solver(tp)
+    with pytest.raises(NotImplementedError, match="HEX8"):
+        (u * u).sum().backward()
