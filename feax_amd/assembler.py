@@ -277,7 +277,8 @@ def create_matfree_res_J_parametric(problem, symmetric=True):
 
 
 def param_vjp(problem, ts, traced_params, sol, lam, bc=None, scale=1.0):
-    """scale * (dR/dp)^T lam for p = volume_vars[0] (fx_hex8_param_vjp_f64);
+    """scale * (dR/dp)^T lam for p = volume_vars[0] (fx_hex8_param_vjp_f64 /
+    fx_tet4_param_vjp_f64);
     lam is read as 0 on the Dirichlet rows of bc.  Returns a float64 device
     tensor shaped like the parameter (cell (E,) or node (N,))."""
     import torch
@@ -286,29 +287,29 @@ def param_vjp(problem, ts, traced_params, sol, lam, bc=None, scale=1.0):
     lam = _as_dev(lam, ts)
     mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
     grad = torch.empty_like(p, dtype=torch.float64)
-    wsb = int(nat.lib().fx_hex8_param_vjp_workspace_bytes(ts.num_cells, mat.param_loc))
+    wsb = int(getattr(nat.lib(), f"fx_{ts.tag}_param_vjp_workspace_bytes")(ts.num_cells, mat.param_loc))
     ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=ts.device)
-    hex8_only(ts, "the parameter VJP (gradients)")
-    nat.call("fx_hex8_param_vjp_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
+    nat.call(f"fx_{ts.tag}_param_vjp_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
              nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat),
              nat.ptr(p), nat.ptr(u), nat.ptr(lam), nat.ptr(mask), float(scale), nat.ptr(grad), nat.ptr(ws), wsb)
     return grad
 
 
 def traction_vjp(problem, ts, lam, bc=None, scale=1.0):
-    """scale * d(lam . R)/d svar per loaded boundary: a list of (F_i, 4)
-    float64 device tensors (fx_hex8_traction_vjp_f64)."""
+    """scale * d(lam . R)/d svar per loaded boundary: a list of (F_i, q)
+    float64 device tensors (fx_hex8_traction_vjp_f64, q = 4 /
+    fx_tet4_traction_vjp_f64, q = 1)."""
     import torch
     if ts.num_faces == 0:
         return []
     lam = _as_dev(lam, ts)
     mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
-    g = torch.empty(ts.num_faces * 4, dtype=torch.float64, device=ts.device)
-    hex8_only(ts, "the traction VJP (gradients)")
-    nat.call("fx_hex8_traction_vjp_f64", nat.stream(), ts.num_faces, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
-             nat.ptr(ts.cells), nat.ptr(ts.faces), nat.ptr(ts.fdir), nat.ptr(lam), nat.ptr(mask), float(scale),
-             nat.ptr(g))
-    g = g.reshape(-1, 4)
+    q = ts.num_face_quads
+    g = torch.empty(ts.num_faces * q, dtype=torch.float64, device=ts.device)
+    nat.call(f"fx_{ts.tag}_traction_vjp_f64", nat.stream(), ts.num_faces, nat.ptr(ts.x), nat.ptr(ts.y),
+             nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.faces), nat.ptr(ts.fdir), nat.ptr(lam), nat.ptr(mask),
+             float(scale), nat.ptr(g))
+    g = g.reshape(-1, q)
     off = ts.face_offsets
     return [g[off[i]:off[i + 1]] for i in range(len(off) - 1)]
 
@@ -321,7 +322,7 @@ def compliance_sensitivity(problem, ts, traced_params, sol):
     mat, p = resolve_material(problem, traced_params, ts)
     u = _as_dev(sol, ts)
     dc = torch.empty(ts.num_cells, dtype=torch.float64, device=ts.device)
-    nat.call("fx_simp_compliance_sens_f64", nat.stream(), ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
+    nat.call("fx_simp_compliance_sens_f64" if ts.tag == "hex8" else "fx_tet4_simp_compliance_sens_f64", nat.stream(), ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
              nat.ptr(ts.cells), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u), nat.ptr(dc))
     return dc
 

@@ -232,6 +232,122 @@ __global__ __launch_bounds__(256) void tet_node_residual_kernel(
   res[3 * nd + 2] = sgn * r2;
 }
 
+// ------------------------------------------------------------- adjoint
+// The HEX8 pullbacks (adjoint.hip) at the single Gauss point: per element
+// w_e = dR_e/dp . lambda_e (SELF: lambda = u, the compliance case), with
+//   linear / SIMP: sigma_1(grad u) : grad lambda JxW dE/dp,
+//   Neo-Hookean:   (F - F^-T) : grad lambda JxW;
+// a cell variable gets scale * w_e, a node variable scale * sum over its
+// elements of N_a(centroid) w_e = w_e / 4.  lambda is read as 0 on the rows
+// with bc_mask set.
+template <bool SELF>
+__global__ __launch_bounds__(kTetThreads) void tet_param_vjp_kernel(
+    int num_cells, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const int* __restrict__ cells, MatParams mat, const double* __restrict__ param, const double* __restrict__ u,
+    const double* __restrict__ lam, const uint8_t* __restrict__ bc_mask, double scale, double* __restrict__ out_cell,
+    double* __restrict__ we) {
+  const int e = blockIdx.x * kTetThreads + threadIdx.x;
+  if (e >= num_cells) return;
+  const int* cn = cells + 4 * static_cast<long>(e);
+  double g[4][3];
+  const double jxw = tet_grads<double>(x, y, z, cn, g) / 6.0;
+  double ug[3][3], lg[3][3];
+  tet_ugrad<double>(u, cn, g, ug);
+  if constexpr (SELF) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) lg[i][d] = ug[i][d];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int r = 3 * cn[c] + i;
+          v += ((bc_mask && bc_mask[r]) ? 0.0 : lam[r]) * g[c][d];
+        }
+        lg[i][d] = v;
+      }
+  }
+  double dens = 0.0;
+  if (mat.kind == FX_MAT_NEO_HOOKEAN) {
+    double FiT[3][3];
+    inv_transpose_F<double>(ug, FiT);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dens += (ug[i][d] + (i == d ? 1.0 : 0.0) - FiT[i][d]) * lg[i][d];
+  } else {
+    const double nu = mat.nu;
+    const double lam0 = nu / ((1 + nu) * (1 - 2 * nu)), mu0 = 1 / (2 * (1 + nu));
+    const double tr = ug[0][0] + ug[1][1] + ug[2][2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dens += (mu0 * (ug[i][d] + ug[d][i]) + (i == d ? lam0 * tr : 0.0)) * lg[i][d];
+    double dEdp;
+    modulus_of<double>(mat, tet_param<double>(mat, param, e, cn), &dEdp);
+    dens *= dEdp;
+  }
+  const double w = dens * jxw;
+  if (out_cell)
+    out_cell[e] = scale * w;
+  else
+    we[e] = w;
+}
+
+__global__ __launch_bounds__(256) void tet_param_vjp_node_kernel(int num_nodes, const int* __restrict__ inc_ptr,
+                                                                 const int* __restrict__ inc,
+                                                                 const double* __restrict__ we, double scale,
+                                                                 double* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= num_nodes) return;
+  double s = 0.0;
+  for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += 0.25 * we[inc[k] >> 3];
+  out[i] = scale * s;
+}
+
+// grad_svar[f] = scale * nanson * sum over the facet's vertices a of
+// (dir_f . lambda_a) / 3 (facet lf is opposite vertex lf).
+__global__ __launch_bounds__(256) void tet_traction_vjp_kernel(int num_faces, const double* __restrict__ x,
+                                                               const double* __restrict__ y,
+                                                               const double* __restrict__ z,
+                                                               const int* __restrict__ cells,
+                                                               const int* __restrict__ faces,
+                                                               const double* __restrict__ fdir,
+                                                               const double* __restrict__ lam,
+                                                               const uint8_t* __restrict__ bc_mask, double scale,
+                                                               double* __restrict__ out) {
+  const int f = blockIdx.x * 256 + threadIdx.x;
+  if (f >= num_faces) return;
+  const int e = faces[2 * f], lf = faces[2 * f + 1];
+  const int* cn = cells + 4 * static_cast<long>(e);
+  double g[4][3];
+  const double det = tet_grads<double>(x, y, z, cn, g);
+  double n[3], wf;
+  tet_facet(lf, n, wf);
+  double m[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) m[d] = n[0] * g[1][d] + n[1] * g[2][d] + n[2] * g[3][d];
+  const double nanson = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) * det * wf;
+  double s = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if (a == lf) continue;
+    double dl = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int r = 3 * cn[a] + i;
+      dl += fdir[3 * f + i] * ((bc_mask && bc_mask[r]) ? 0.0 : lam[r]);
+    }
+    s += (1.0 / 3.0) * dl;
+  }
+  out[f] = scale * s * nanson;
+}
+
 inline size_t tet_matrix_bytes(int num_cells, int value_bytes, int flags) {
   const size_t c = (value_bytes == 4 && (flags & FX_ASM_COMPUTE32)) ? 4 : 8;
   return 144 * c * static_cast<size_t>(num_cells);
@@ -348,6 +464,71 @@ int fx_tet4_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, con
                        negate, res);
     FX_LAUNCHED("bc_res_kernel");
   }
+  return FX_OK;
+}
+
+size_t fx_tet4_param_vjp_workspace_bytes(int32_t num_cells, int32_t param_loc) {
+  if (num_cells < 0) return 0;
+  return param_loc == FX_PARAM_NODE ? static_cast<size_t>(num_cells) * sizeof(double) : 0;
+}
+
+int fx_tet4_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const fx_material* mat, const double* param, const double* u, const double* lam,
+                          const uint8_t* bc_mask, double scale, double* grad, void* workspace, size_t ws_bytes) {
+  if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !z || !cells || !mat || !param || !u || !lam || !grad)
+    return fail(FX_EINVAL, "fx_tet4_param_vjp_f64: bad arguments");
+  if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
+    return fail(FX_EINVAL, "fx_tet4_param_vjp_f64: unknown material %d", mat->kind);
+  if (mat->param_loc != FX_PARAM_NODE && mat->param_loc != FX_PARAM_CELL)
+    return fail(FX_EINVAL, "fx_tet4_param_vjp_f64: unknown parameter location %d", mat->param_loc);
+  hipStream_t s = as_stream(stream);
+  const MatParams mp = to_params(mat);
+  const dim3 eg((num_cells + kTetThreads - 1) / kTetThreads);
+  if (mat->param_loc == FX_PARAM_CELL) {
+    hipLaunchKernelGGL(tet_param_vjp_kernel<false>, eg, dim3(kTetThreads), 0, s, num_cells, x, y, z, cells, mp,
+                       param, u, lam, bc_mask, scale, grad, nullptr);
+    FX_LAUNCHED("tet_param_vjp_kernel");
+    return FX_OK;
+  }
+  if (!inc_ptr || !inc) return fail(FX_EINVAL, "fx_tet4_param_vjp_f64: node variables need the incidence");
+  const size_t need = fx_tet4_param_vjp_workspace_bytes(num_cells, FX_PARAM_NODE);
+  if (!workspace || ws_bytes < need)
+    return fail(FX_EWORKSPACE, "fx_tet4_param_vjp_f64: workspace %zu < %zu bytes", ws_bytes, need);
+  double* we = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(tet_param_vjp_kernel<false>, eg, dim3(kTetThreads), 0, s, num_cells, x, y, z, cells, mp, param,
+                     u, lam, bc_mask, 1.0, nullptr, we);
+  hipLaunchKernelGGL(tet_param_vjp_node_kernel, dim3((num_nodes + 255) / 256), dim3(256), 0, s, num_nodes, inc_ptr,
+                     inc, we, scale, grad);
+  FX_LAUNCHED("tet_param_vjp_node_kernel");
+  return FX_OK;
+}
+
+int fx_tet4_simp_compliance_sens_f64(void* stream, int32_t num_cells, const double* x, const double* y,
+                                     const double* z, const int32_t* cells, const fx_material* mat,
+                                     const double* param, const double* u, double* dc) {
+  if (num_cells <= 0 || !x || !y || !z || !cells || !mat || !param || !u || !dc)
+    return fail(FX_EINVAL, "fx_tet4_simp_compliance_sens_f64: bad arguments");
+  if (mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_LINEAR_ELASTIC)
+    return fail(FX_EINVAL, "fx_tet4_simp_compliance_sens_f64: needs a linear law (SIMP or linear elastic)");
+  if (mat->param_loc != FX_PARAM_CELL)
+    return fail(FX_EINVAL, "fx_tet4_simp_compliance_sens_f64: needs a cell variable");
+  hipLaunchKernelGGL(tet_param_vjp_kernel<true>, dim3((num_cells + kTetThreads - 1) / kTetThreads),
+                     dim3(kTetThreads), 0, as_stream(stream), num_cells, x, y, z, cells, to_params(mat), param, u,
+                     nullptr, nullptr, -1.0, dc, nullptr);
+  FX_LAUNCHED("tet_param_vjp_kernel");
+  return FX_OK;
+}
+
+int fx_tet4_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, const double* y, const double* z,
+                             const int32_t* cells, const int32_t* faces, const double* fdir, const double* lam,
+                             const uint8_t* bc_mask, double scale, double* grad_svar) {
+  if (num_faces < 0 || (num_faces > 0 && (!x || !y || !z || !cells || !faces || !fdir || !lam || !grad_svar)))
+    return fail(FX_EINVAL, "fx_tet4_traction_vjp_f64: bad arguments");
+  if (num_faces == 0) return FX_OK;
+  hipLaunchKernelGGL(tet_traction_vjp_kernel, dim3((num_faces + 255) / 256), dim3(256), 0, as_stream(stream),
+                     num_faces, x, y, z, cells, faces, fdir, lam, bc_mask, scale, grad_svar);
+  FX_LAUNCHED("tet_traction_vjp_kernel");
   return FX_OK;
 }
 

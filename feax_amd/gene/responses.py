@@ -36,18 +36,19 @@ def _external_work_vector(problem, ts, params):
     for i, p in enumerate(params):
         nf = int(ts.face_offsets[i + 1] - ts.face_offsets[i])
         v = torch.as_tensor(np.asarray(p, np.float64), device=ts.device)
-        parts.append(torch.broadcast_to(v.reshape(-1, 1) if v.numel() == nf else v, (nf, 4)).to(torch.float64))
+        q = ts.num_face_quads
+        parts.append(torch.broadcast_to(v.reshape(-1, 1) if v.numel() == nf else v, (nf, q)).to(torch.float64))
     svar = torch.cat(parts).reshape(-1).contiguous()
     # At u = 0 the volume term vanishes for every law here (stress(0) = 0).
     mat = nat.material_struct(nat.FX_MAT_LINEAR_ELASTIC, nat.FX_PARAM_CELL, nu=0.3)
     ones = torch.ones(ts.num_cells, dtype=torch.float64, device=ts.device)
     zero = torch.zeros(ts.n, dtype=torch.float64, device=ts.device)
     g = torch.empty(ts.n, dtype=torch.float64, device=ts.device)
-    nat.call("fx_hex8_residual_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
+    nat.call(f"fx_{ts.tag}_residual_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
              nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat),
              nat.ptr(ones), nat.ptr(zero), ts.num_faces, nat.ptr(ts.faces), nat.ptr(ts.face_inc_ptr),
              nat.ptr(ts.face_inc), nat.ptr(svar), nat.ptr(ts.fdir), 0, None, None, 1, nat.ptr(g),
-             *elem_workspace(ts, nat.lib().fx_hex8_residual_workspace_bytes(ts.num_cells)))
+             *elem_workspace(ts, getattr(nat.lib(), f"fx_{ts.tag}_residual_workspace_bytes")(ts.num_cells)))
     return g
 
 

@@ -285,6 +285,21 @@ int fx_tet4_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, con
                          const double* fdir, int32_t num_bc, const int32_t* bc_rows, const double* bc_vals,
                          int32_t negate, double* res, void* workspace, size_t ws_bytes);
 size_t fx_tet4_residual_workspace_bytes(int32_t num_cells);
+/* TET4 pullbacks, as fx_hex8_param_vjp_f64 / fx_simp_compliance_sens_f64 /
+ * fx_hex8_traction_vjp_f64 at the single Gauss point (node variables need
+ * fx_tet4_param_vjp_workspace_bytes: one double per element); the traction
+ * gradient has one value per face. */
+size_t fx_tet4_param_vjp_workspace_bytes(int32_t num_cells, int32_t param_loc);
+int fx_tet4_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const fx_material* mat, const double* param, const double* u, const double* lam,
+                          const uint8_t* bc_mask, double scale, double* grad, void* workspace, size_t ws_bytes);
+int fx_tet4_simp_compliance_sens_f64(void* stream, int32_t num_cells, const double* x, const double* y,
+                                     const double* z, const int32_t* cells, const fx_material* mat,
+                                     const double* param, const double* u, double* dc);
+int fx_tet4_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, const double* y, const double* z,
+                             const int32_t* cells, const int32_t* faces, const double* fdir, const double* lam,
+                             const uint8_t* bc_mask, double scale, double* grad_svar);
 
 /* Matrix-free operator out = J_bc(u) v (FP64): feax's Krylov operator
  * create_matfree_res_J_parametric (feax/assembler.py:1747-1778),
