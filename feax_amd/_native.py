@@ -117,6 +117,8 @@ SIGNATURES = {
                                      _P, _SZ]),
     "fx_tet4_simp_compliance_sens_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "fx_tet4_traction_vjp_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P]),
+    "fx_tet4_energy_workspace_bytes": (_SZ, [_I32]),
+    "fx_tet4_energy_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ]),
     "fx_hex8_energy_workspace_bytes": (_SZ, [_I32]),
     "fx_hex8_energy_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ]),
     "fx_bicgstab_workspace_bytes": (_SZ, [_I32]),

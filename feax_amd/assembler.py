@@ -329,16 +329,15 @@ def compliance_sensitivity(problem, ts, traced_params, sol):
 
 def strain_energy(problem, ts, traced_params, sol, with_partial=False):
     """Neo-Hookean strain energy Pi(u) (device scalar) and, with_partial,
-    dPi/dmu_e per element (fx_hex8_energy_f64)."""
+    dPi/dmu_e per element (fx_hex8_energy_f64 / fx_tet4_energy_f64)."""
     import torch
     mat, p = resolve_material(problem, traced_params, ts)
     u = _as_dev(sol, ts)
     out = torch.empty(1, dtype=torch.float64, device=ts.device)
     dmu = torch.empty(ts.num_cells, dtype=torch.float64, device=ts.device) if with_partial else None
-    wsb = int(nat.lib().fx_hex8_energy_workspace_bytes(ts.num_cells))
+    wsb = int(getattr(nat.lib(), f"fx_{ts.tag}_energy_workspace_bytes")(ts.num_cells))
     ws = torch.empty(wsb, dtype=torch.uint8, device=ts.device)
-    hex8_only(ts, "the strain energy")
-    nat.call("fx_hex8_energy_f64", nat.stream(), ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
+    nat.call(f"fx_{ts.tag}_energy_f64", nat.stream(), ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
              nat.ptr(ts.cells), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u), nat.ptr(out), nat.ptr(dmu),
              nat.ptr(ws), wsb)
     return (out[0], dmu) if with_partial else out[0]

@@ -19,6 +19,7 @@
 #include "common.h"
 #include "feaxhip.h"
 #include "hex8_common.h"
+#include "reduce.h"
 
 namespace fx {
 namespace {
@@ -348,6 +349,50 @@ __global__ __launch_bounds__(256) void tet_traction_vjp_kernel(int num_faces, co
   out[f] = scale * s * nanson;
 }
 
+// Neo-Hookean strain energy psi = mu/2 (I1 - 3) - mu ln J + kappa/2 (J - 1)^2
+// (feax/assembler.py:1866-1961) at the single Gauss point: per element
+// psi JxW and its partial d/dmu_e, block partial sums of the total.
+__global__ __launch_bounds__(kTetThreads) void tet_energy_kernel(int num_cells, const double* __restrict__ x,
+                                                                 const double* __restrict__ y,
+                                                                 const double* __restrict__ z,
+                                                                 const int* __restrict__ cells, MatParams mat,
+                                                                 const double* __restrict__ param,
+                                                                 const double* __restrict__ u,
+                                                                 double* __restrict__ dpsi_dmu,
+                                                                 double* __restrict__ partials) {
+  const int e = blockIdx.x * kTetThreads + threadIdx.x;
+  double psi = 0.0;
+  if (e < num_cells) {
+    const int* cn = cells + 4 * static_cast<long>(e);
+    double g[4][3], ug[3][3], FiT[3][3];
+    const double jxw = tet_grads<double>(x, y, z, cn, g) / 6.0;
+    tet_ugrad<double>(u, cn, g, ug);
+    const double J = inv_transpose_F<double>(ug, FiT);
+    double I1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double F = ug[i][d] + (i == d ? 1.0 : 0.0);
+        I1 += F * F;
+      }
+    const double mu = tet_param<double>(mat, param, e, cn);
+    const double dmu = (0.5 * (I1 - 3.0) - log(J)) * jxw;
+    psi = mu * dmu + 0.5 * mat.kappa * (J - 1.0) * (J - 1.0) * jxw;
+    if (dpsi_dmu) dpsi_dmu[e] = dmu;
+  }
+  DVec<1> acc;
+  acc.v[0] = psi;
+  const DVec<1> t = block_sum<1, kTetThreads>(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t.v[0];
+}
+
+__global__ __launch_bounds__(kFinBlock) void tet_energy_fin(const double* __restrict__ partials, int nparts,
+                                                            double* __restrict__ out) {
+  const DVec<1> t = sum_partials<1>(partials, nparts);
+  if (threadIdx.x == 0) *out = t.v[0];
+}
+
 inline size_t tet_matrix_bytes(int num_cells, int value_bytes, int flags) {
   const size_t c = (value_bytes == 4 && (flags & FX_ASM_COMPUTE32)) ? 4 : 8;
   return 144 * c * static_cast<size_t>(num_cells);
@@ -529,6 +574,33 @@ int fx_tet4_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, c
   hipLaunchKernelGGL(tet_traction_vjp_kernel, dim3((num_faces + 255) / 256), dim3(256), 0, as_stream(stream),
                      num_faces, x, y, z, cells, faces, fdir, lam, bc_mask, scale, grad_svar);
   FX_LAUNCHED("tet_traction_vjp_kernel");
+  return FX_OK;
+}
+
+size_t fx_tet4_energy_workspace_bytes(int32_t num_cells) {
+  if (num_cells < 0) return 0;
+  return static_cast<size_t>((num_cells + kTetThreads - 1) / kTetThreads) * sizeof(double);
+}
+
+int fx_tet4_energy_f64(void* stream, int32_t num_cells, const double* x, const double* y, const double* z,
+                       const int32_t* cells, const fx_material* mat, const double* param, const double* u,
+                       double* energy, double* dpsi_dmu, void* workspace, size_t ws_bytes) {
+  if (num_cells <= 0 || !x || !y || !z || !cells || !mat || !param || !u || !energy)
+    return fail(FX_EINVAL, "fx_tet4_energy_f64: bad arguments");
+  if (mat->kind != FX_MAT_NEO_HOOKEAN)
+    return fail(FX_EINVAL, "fx_tet4_energy_f64: strain energy is built for the Neo-Hookean law");
+  if (dpsi_dmu && mat->param_loc != FX_PARAM_CELL)
+    return fail(FX_EINVAL, "fx_tet4_energy_f64: dpsi/dmu per element needs a cell variable");
+  const size_t need = fx_tet4_energy_workspace_bytes(num_cells);
+  if (!workspace || ws_bytes < need)
+    return fail(FX_EWORKSPACE, "fx_tet4_energy_f64: workspace %zu < %zu bytes", ws_bytes, need);
+  hipStream_t s = as_stream(stream);
+  const int grid = (num_cells + kTetThreads - 1) / kTetThreads;
+  double* partials = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(tet_energy_kernel, dim3(grid), dim3(kTetThreads), 0, s, num_cells, x, y, z, cells,
+                     to_params(mat), param, u, dpsi_dmu, partials);
+  hipLaunchKernelGGL(tet_energy_fin, dim3(1), dim3(kFinBlock), 0, s, partials, grid, energy);
+  FX_LAUNCHED("tet_energy_kernel");
   return FX_OK;
 }
 

@@ -300,6 +300,12 @@ int fx_tet4_simp_compliance_sens_f64(void* stream, int32_t num_cells, const doub
 int fx_tet4_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, const double* y, const double* z,
                              const int32_t* cells, const int32_t* faces, const double* fdir, const double* lam,
                              const uint8_t* bc_mask, double scale, double* grad_svar);
+/* TET4 Neo-Hookean strain energy, as fx_hex8_energy_f64 at the single
+ * Gauss point (workspace fx_tet4_energy_workspace_bytes). */
+size_t fx_tet4_energy_workspace_bytes(int32_t num_cells);
+int fx_tet4_energy_f64(void* stream, int32_t num_cells, const double* x, const double* y, const double* z,
+                       const int32_t* cells, const fx_material* mat, const double* param, const double* u,
+                       double* energy, double* dpsi_dmu, void* workspace, size_t ws_bytes);
 
 /* Matrix-free operator out = J_bc(u) v (FP64): feax's Krylov operator
  * create_matfree_res_J_parametric (feax/assembler.py:1747-1778),

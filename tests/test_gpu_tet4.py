@@ -219,3 +219,47 @@ def test_tet4_gradients_match_finite_differences(kind):
         create_compliance_fn(prob, surface_load_params=0.5)(u2).backward()
         ref = _np(r2.grad)
         assert np.abs(_np(dC) - ref).max() <= 1e-8 * np.abs(ref).max()
+
+
+def test_tet4_energy_gradient_through_newton():
+    """dPi/dmu of the displacement-controlled equilibrium (autograd through
+    the Newton solver and the TET4 energy kernel) equals the partial
+    dPi/dmu_e (R_free = 0) and central differences of re-solved states."""
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import create_energy_fn, strain_energy
+    dims = (2, 2, 3)
+    prob, _ = _problem("neo_hookean", dims=dims, load=False, shuffle=False)
+    nz = dims[2]
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[2], 0.0), "all", 0.0),
+                               fe.DirichletBCSpec(lambda p: np.isclose(p[2], nz), "z", 0.0)]).create_bc(prob)
+    node, comp = bc.bc_rows // 3, bc.bc_rows % 3
+    vals = np.where(np.isclose(prob.points[node, 2], nz) & (comp == 2), -0.05 * nz, 0.0)
+    bcs = bc.replace_vals(vals)
+    mu0 = _param("neo_hookean", prob, seed=7)
+    solver = fe.create_solver(prob, bcs, fe.DirectSolverOptions(), newton_options=fe.NewtonOptions(tol=1e-11,
+                              rel_tol=1e-13), linear=False, return_solution=False)
+    energy = create_energy_fn(prob)
+    ig = torch.zeros(prob.num_total_dofs_all_vars, dtype=torch.float64, device="cuda")
+    ig[torch.as_tensor(bc.bc_rows.astype(np.int64), device="cuda")] = torch.as_tensor(vals, device="cuda")
+    mu = torch.tensor(mu0, device="cuda", requires_grad=True)
+    tp = fe.TracedParams(volume_vars=(mu,))
+    u = solver(tp, ig)
+    energy(u, tp).backward()
+    g = _np(mu.grad)
+    ts = prob._default_ts
+    _, partial = strain_energy(prob, ts, fe.TracedParams(volume_vars=(mu0,)), u.detach(), with_partial=True)
+    partial = _np(partial)
+    assert np.abs(g - partial).max() <= 1e-8 * np.abs(partial).max()
+
+    def Pof(mv):
+        with torch.no_grad():
+            tpv = fe.TracedParams(volume_vars=(torch.as_tensor(mv, device="cuda"),))
+            return float(strain_energy(prob, ts, tpv, solver(tpv, u.detach())))
+    rng = np.random.default_rng(3)
+    for e in rng.choice(prob.num_cells, 3, replace=False):
+        mp, mm = mu0.copy(), mu0.copy()
+        mp[e] += 1e-5
+        mm[e] -= 1e-5
+        fd = (Pof(mp) - Pof(mm)) / 2e-5
+        assert abs(fd - g[e]) <= 1e-6 * np.abs(g).max(), (e, fd, g[e])
