@@ -30,7 +30,8 @@ class TracedStructure:
         P = problem.points
         self.device = dev
         self.ele_type = problem.ele_type
-        self.tag = "hex8" if problem.ele_type == "HEX8" else "tet4"
+        self.tag = problem.ele_type.lower()
+        self.vec = problem.vec
         self.num_face_quads = problem.fes[0].num_face_quads
         self.num_nodes = problem.num_total_nodes
         self.num_cells = problem.num_cells
@@ -38,7 +39,7 @@ class TracedStructure:
         self.nse = problem.csr_nse
         self.x = torch.as_tensor(np.ascontiguousarray(P[:, 0]), **f64)
         self.y = torch.as_tensor(np.ascontiguousarray(P[:, 1]), **f64)
-        self.z = torch.as_tensor(np.ascontiguousarray(P[:, 2]), **f64)
+        self.z = torch.as_tensor(np.ascontiguousarray(P[:, 2]), **f64) if P.shape[1] > 2 else None
         self.cells = torch.as_tensor(problem.cells.reshape(-1), **i32)
         self.inc_ptr = torch.as_tensor(problem.node_inc_ptr, **i32)
         self.inc = torch.as_tensor(problem.node_inc, **i32)
@@ -52,14 +53,17 @@ class TracedStructure:
         self.face_offsets = problem.face_offsets
         # Per-face traction direction from the problem's surface maps.
         maps = problem.get_surface_maps() or []
-        fdir = np.zeros((self.num_faces, 3))
+        fdir = np.zeros((self.num_faces, self.vec))
         for i in range(len(problem.boundary_inds_list)):
             if i < len(maps):
                 m = maps[i]
                 if not isinstance(m, SurfaceTraction):
                     raise NotImplementedError(
                         "surface maps on the HIP path are feax_amd.materials.SurfaceTraction (dead loads)")
-                fdir[self.face_offsets[i]:self.face_offsets[i + 1]] = m.direction
+                d = np.asarray(m.direction, float)
+                if d.shape != (self.vec,):
+                    raise ValueError(f"SurfaceTraction direction {d} must have {self.vec} components here")
+                fdir[self.face_offsets[i]:self.face_offsets[i + 1]] = d
         self.fdir = torch.as_tensor(fdir.reshape(-1), **f64)
         self._cache = {}
         self._bcol = None
@@ -175,7 +179,8 @@ def elem_workspace(ts, nbytes):
 
 def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, dtype=None, diff_form=False,
                  jacobi_shift=None):
-    """BC-eliminated CSR Jacobian on the device (fx_hex8_assemble_* / fx_tet4_assemble_*).
+    """BC-eliminated CSR Jacobian on the device (fx_hex8_assemble_* / fx_tet4_assemble_* /
+    fx_quad4_assemble_*).
 
     diff_form: FP32 difference form for the mixed solver.  jacobi_shift:
     also return dinv = 1 / (diag + shift) (FP64) computed in the same pass.
@@ -196,8 +201,9 @@ def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, 
              nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), nat.ptr(ts.rank), nat.ptr(ts.indptr),
              nat.ptr(ts.indices), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u), nat.ptr(mask), int(bool(symmetric)),
              flags, float(jacobi_shift or 0.0), nat.ptr(dinv), nat.ptr(data), wp, wn)
-    A = CSRMatrix(data, ts.indptr, ts.indices, (ts.n, ts.n), block3=True, form="diff3" if diff_form else "csr",
-                  bcol=ts.bcol())
+    b3 = ts.vec == 3
+    A = CSRMatrix(data, ts.indptr, ts.indices, (ts.n, ts.n), block3=b3, form="diff3" if diff_form else "csr",
+                  bcol=ts.bcol() if b3 else None)
     return A if jacobi_shift is None else (A, dinv)
 
 
@@ -278,7 +284,7 @@ def create_matfree_res_J_parametric(problem, symmetric=True):
 
 def param_vjp(problem, ts, traced_params, sol, lam, bc=None, scale=1.0):
     """scale * (dR/dp)^T lam for p = volume_vars[0] (fx_hex8_param_vjp_f64 /
-    fx_tet4_param_vjp_f64);
+    fx_tet4_param_vjp_f64 / fx_quad4_param_vjp_f64);
     lam is read as 0 on the Dirichlet rows of bc.  Returns a float64 device
     tensor shaped like the parameter (cell (E,) or node (N,))."""
     import torch
@@ -298,7 +304,7 @@ def param_vjp(problem, ts, traced_params, sol, lam, bc=None, scale=1.0):
 def traction_vjp(problem, ts, lam, bc=None, scale=1.0):
     """scale * d(lam . R)/d svar per loaded boundary: a list of (F_i, q)
     float64 device tensors (fx_hex8_traction_vjp_f64, q = 4 /
-    fx_tet4_traction_vjp_f64, q = 1)."""
+    fx_tet4_traction_vjp_f64, q = 1 / fx_quad4_traction_vjp_f64, q = 2)."""
     import torch
     if ts.num_faces == 0:
         return []
@@ -316,13 +322,15 @@ def traction_vjp(problem, ts, lam, bc=None, scale=1.0):
 
 def compliance_sensitivity(problem, ts, traced_params, sol):
     """Fused dC/drho_e = -u_e^T dK_e/drho_e u_e for a linear law with a cell
-    variable (fx_simp_compliance_sens_f64): the self-adjoint gradient of the
-    compliance when the Dirichlet values are zero."""
+    variable (fx_simp_compliance_sens_f64 and the TET4 / QUAD4 variants): the
+    self-adjoint gradient of the compliance when the Dirichlet values are
+    zero."""
     import torch
     mat, p = resolve_material(problem, traced_params, ts)
     u = _as_dev(sol, ts)
     dc = torch.empty(ts.num_cells, dtype=torch.float64, device=ts.device)
-    nat.call("fx_simp_compliance_sens_f64" if ts.tag == "hex8" else "fx_tet4_simp_compliance_sens_f64", nat.stream(), ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
+    name = "fx_simp_compliance_sens_f64" if ts.tag == "hex8" else f"fx_{ts.tag}_simp_compliance_sens_f64"
+    nat.call(name, nat.stream(), ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
              nat.ptr(ts.cells), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u), nat.ptr(dc))
     return dc
 
@@ -335,6 +343,8 @@ def strain_energy(problem, ts, traced_params, sol, with_partial=False):
     u = _as_dev(sol, ts)
     out = torch.empty(1, dtype=torch.float64, device=ts.device)
     dmu = torch.empty(ts.num_cells, dtype=torch.float64, device=ts.device) if with_partial else None
+    if ts.tag == "quad4":
+        raise NotImplementedError("the strain energy is implemented for HEX8 and TET4 meshes")
     wsb = int(getattr(nat.lib(), f"fx_{ts.tag}_energy_workspace_bytes")(ts.num_cells))
     ws = torch.empty(wsb, dtype=torch.uint8, device=ts.device)
     nat.call(f"fx_{ts.tag}_energy_f64", nat.stream(), ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
@@ -397,7 +407,7 @@ def _ts_or_build(problem, ts):
 
 
 def get_res(problem, sol_list, traced_params, ts=None):
-    """Residual as a list of (N, 3) arrays (feax/assembler.py:1618-1667)."""
+    """Residual as a list of (N, vec) arrays (feax/assembler.py:1618-1667)."""
     ts = _ts_or_build(problem, ts)
     flat = sol_list[0].reshape(-1) if isinstance(sol_list, (list, tuple)) else sol_list
     return problem.unflatten_fn_sol_list(residual(problem, ts, traced_params, flat))

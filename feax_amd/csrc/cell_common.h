@@ -17,7 +17,7 @@ constexpr int kMaxDeg = 64;        // LDS accumulator: 9 * kMaxDeg values per wa
 // The node's three CSR rows from its LDS accumulator (acc: row i, block kb,
 // column j at i * len + 3 kb + j): Jacobi dinv, the difference form, and the
 // store with the Dirichlet elimination of feax/DCboundary.py:147-199.
-template <typename T, typename C, int NPE = 8>
+template <typename T, typename C, int NPE = 8, int V = 3>
 __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, int row0, int nval,
                                             const int* __restrict__ inc_ptr, const int* __restrict__ inc,
                                             const uint8_t* __restrict__ rank, const int* __restrict__ indices,
@@ -31,10 +31,10 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
     const int a0 = pk & 7;
     kself = rank[NPE * NPE * static_cast<long>(pk >> 3) + (NPE + 1) * a0];
   }
-  if (dinv && lane < 3) {
+  if (dinv && lane < V) {
     // Jacobi: 1 / (K_bc[r, r] + shift) (feax/solvers/common.py:86-94); BC rows have diagonal 1.
-    const int r = 3 * nd + lane;
-    const double d = (bc_mask && bc_mask[r]) ? 1.0 : static_cast<double>(acc[lane * len + 3 * kself + lane]);
+    const int r = V * nd + lane;
+    const double d = (bc_mask && bc_mask[r]) ? 1.0 : static_cast<double>(acc[lane * len + V * kself + lane]);
     dinv[r] = 1.0 / (d + jacobi_shift);
   }
   if (diff_form) {
@@ -43,27 +43,27 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
     // (sum_b grad N_b = 0), so for a free row S[c][d] = -(sum of its entries
     // in eliminated columns (m, d)); a BC row has S[c][d] = delta_cd.
     C S = C(0);
-    const int c = lane / 3, d = lane - 3 * (lane / 3);
-    if (lane < 9) {
-      if (bc_mask && bc_mask[3 * nd + c]) {
+    const int c = lane / V, d = lane - V * (lane / V);
+    if (lane < V * V) {
+      if (bc_mask && bc_mask[V * nd + c]) {
         S = (c == d) ? C(1) : C(0);
       } else if (bc_mask) {
-        for (int kb = 0; kb < len / 3; ++kb) {
-          const int m = indices[row0 + 3 * kb] / 3;
-          if (bc_mask[3 * m + d]) S -= acc[c * len + 3 * kb + d];
+        for (int kb = 0; kb < len / V; ++kb) {
+          const int m = indices[row0 + V * kb] / V;
+          if (bc_mask[V * m + d]) S -= acc[c * len + V * kb + d];
         }
       }
     }
     __builtin_amdgcn_wave_barrier();
-    if (lane < 9) acc[c * len + 3 * kself + d] = S;
+    if (lane < V * V) acc[c * len + V * kself + d] = S;
     __builtin_amdgcn_wave_barrier();
   }
   for (int idx = lane; idx < nval; idx += 64) {
     C v = acc[idx];
     const int slot = row0 + idx;
-    const bool own = ((idx % len) / 3) == kself;
+    const bool own = ((idx % len) / V) == kself;
     if (bc_mask && !(diff_form && own)) {
-      const int r = 3 * nd + idx / len;
+      const int r = V * nd + idx / len;
       const int c = indices[slot];
       const bool zr = bc_mask[r] != 0;
       if (zr || (symmetric && bc_mask[c] != 0)) v = C(0);
@@ -73,9 +73,10 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
   }
 }
 
-// NPE nodes per element (8: HEX8, 4: TET4): element matrices of 9 NPE^2
-// values, blocks (a, b) at 9 (NPE a + b), ranks at NPE (NPE e + a) + b.
-template <typename T, typename C, int NPE = 8>
+// NPE nodes per element (8: HEX8, 4: TET4 / QUAD4), V DOFs per node (3, or
+// 2 in the plane): element matrices of V^2 NPE^2 values, blocks (a, b) at
+// V^2 (NPE a + b), ranks at NPE (NPE e + a) + b.
+template <typename T, typename C, int NPE = 8, int V = 3>
 __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
     int num_nodes, const int* __restrict__ inc_ptr, const int* __restrict__ inc, const uint8_t* __restrict__ rank,
     const int* __restrict__ indptr, const int* __restrict__ indices, const C* __restrict__ ke,
@@ -86,27 +87,27 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
   const int nd = blockIdx.x * kNodesPerBlock + wid;
   if (nd >= num_nodes) return;
   C* acc = acc_all[wid];
-  const int row0 = indptr[3 * nd];
-  const int len = indptr[3 * nd + 1] - row0;  // 3 * deg
-  const int nval = 3 * len;
-  if (len > 3 * kMaxDeg) return;  // host guarantees deg <= kMaxDeg
+  const int row0 = indptr[V * nd];
+  const int len = indptr[V * nd + 1] - row0;  // V * deg
+  const int nval = V * len;
+  if (len > V * kMaxDeg) return;  // host guarantees deg <= kMaxDeg
   for (int i = lane; i < nval; i += 64) acc[i] = C(0);
   __builtin_amdgcn_wave_barrier();
   for (int k = inc_ptr[nd]; k < inc_ptr[nd + 1]; ++k) {
     const int pk = inc[k];
     const int e = pk >> 3, a = pk & 7;
-    const C* src = ke + 9 * NPE * NPE * static_cast<long>(e) + 9 * NPE * a;  // blocks (a, b), b < NPE
+    const C* src = ke + V * V * NPE * NPE * static_cast<long>(e) + V * V * NPE * a;  // blocks (a, b), b < NPE
     const uint8_t* rk = rank + NPE * NPE * static_cast<long>(e) + NPE * a;
-    // 9 NPE values: lane idx -> block b = idx / 9, entry t = idx % 9;
+    // V^2 NPE values: lane idx -> block b = idx / V^2, entry t = idx % V^2;
     // distinct lanes touch distinct accumulator entries.
-    for (int idx = lane; idx < 9 * NPE; idx += 64) {
-      const int b = idx / 9, t = idx - 9 * b;
-      const int i = t / 3, j = t - 3 * i;
-      acc[i * len + 3 * rk[b] + j] += src[idx];
+    for (int idx = lane; idx < V * V * NPE; idx += 64) {
+      const int b = idx / (V * V), t = idx - V * V * b;
+      const int i = t / V, j = t - V * i;
+      acc[i * len + V * rk[b] + j] += src[idx];
     }
     __builtin_amdgcn_wave_barrier();
   }
-  finish_rows<T, C, NPE>(nd, lane, acc, len, row0, nval, inc_ptr, inc, rank, indices, bc_mask, symmetric,
+  finish_rows<T, C, NPE, V>(nd, lane, acc, len, row0, nval, inc_ptr, inc, rank, indices, bc_mask, symmetric,
                          diff_form, jacobi_shift, dinv, data);
 }
 

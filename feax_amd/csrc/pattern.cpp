@@ -1,4 +1,5 @@
-// Host-side structure builders for HEX8 and TET4, vec = 3 meshes.
+// Host-side structure builders for HEX8 and TET4 (vec = 3) and QUAD4
+// (vec = 2) meshes.
 //
 // fx_hex8_pattern produces the CSR pattern that feax builds in
 // Problem._build_csr_assembly_structure (feax/problem.py:316-484): the
@@ -27,12 +28,12 @@ const int kTetFaceVerts[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
 
 // The CSR pattern, element ranks and node incidence of a mesh of NPE-node
 // cells (the incidence entry of node a of element e is e * 8 + a).
-int cell_pattern(const char* name, int npe, int32_t num_nodes, int32_t num_cells, const int32_t* cells,
+int cell_pattern(const char* name, int npe, int vec, int32_t num_nodes, int32_t num_cells, const int32_t* cells,
                  int64_t* nse_out, int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
                  int32_t* inc_ptr, int32_t* inc) {
   if (num_nodes <= 0 || num_cells < 0 || !cells || !nse_out || !indptr)
     return fx::fail(FX_EINVAL, "%s: bad arguments", name);
-  if (static_cast<int64_t>(num_nodes) * 3 >= INT32_MAX || static_cast<int64_t>(num_cells) * 8 >= INT32_MAX)
+  if (static_cast<int64_t>(num_nodes) * vec >= INT32_MAX || static_cast<int64_t>(num_cells) * 8 >= INT32_MAX)
     return fx::fail(FX_ELIMIT, "%s: mesh too large for int32 indices", name);
   const int64_t E = num_cells, N = num_nodes;
   // node -> (element, local) incidence, ascending element id per node.
@@ -72,20 +73,20 @@ int cell_pattern(const char* name, int npe, int32_t num_nodes, int32_t num_cells
   indptr[0] = 0;
   for (int64_t nd = 0; nd < N; ++nd) {
     int64_t deg = nptr[nd + 1] - nptr[nd];
-    for (int i = 0; i < 3; ++i) {
-      nse += 3 * deg;
+    for (int i = 0; i < vec; ++i) {
+      nse += vec * deg;
       if (nse >= INT32_MAX) return fx::fail(FX_ELIMIT, "%s: nse exceeds int32", name);
-      indptr[nd * 3 + i + 1] = static_cast<int32_t>(nse);
+      indptr[nd * vec + i + 1] = static_cast<int32_t>(nse);
     }
   }
   *nse_out = nse;
   if (max_degree) *max_degree = maxdeg;
   if (!indices) return FX_OK;
   for (int64_t nd = 0; nd < N; ++nd) {
-    for (int i = 0; i < 3; ++i) {
-      int64_t s = indptr[nd * 3 + i];
+    for (int i = 0; i < vec; ++i) {
+      int64_t s = indptr[nd * vec + i];
       for (int32_t k = nptr[nd]; k < nptr[nd + 1]; ++k)
-        for (int j = 0; j < 3; ++j) indices[s++] = nbr[k] * 3 + j;
+        for (int j = 0; j < vec; ++j) indices[s++] = nbr[k] * vec + j;
     }
   }
   if (elem_rank) {
@@ -110,14 +111,21 @@ int cell_pattern(const char* name, int npe, int32_t num_nodes, int32_t num_cells
 extern "C" int fx_hex8_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse_out,
                                int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
                                int32_t* inc_ptr, int32_t* inc) {
-  return cell_pattern("fx_hex8_pattern", 8, num_nodes, num_cells, cells, nse_out, max_degree, indptr, indices,
+  return cell_pattern("fx_hex8_pattern", 8, 3, num_nodes, num_cells, cells, nse_out, max_degree, indptr, indices,
                       elem_rank, inc_ptr, inc);
 }
 
 extern "C" int fx_tet4_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse_out,
                                int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
                                int32_t* inc_ptr, int32_t* inc) {
-  return cell_pattern("fx_tet4_pattern", 4, num_nodes, num_cells, cells, nse_out, max_degree, indptr, indices,
+  return cell_pattern("fx_tet4_pattern", 4, 3, num_nodes, num_cells, cells, nse_out, max_degree, indptr, indices,
+                      elem_rank, inc_ptr, inc);
+}
+
+extern "C" int fx_quad4_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse_out,
+                                int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                                int32_t* inc_ptr, int32_t* inc) {
+  return cell_pattern("fx_quad4_pattern", 4, 2, num_nodes, num_cells, cells, nse_out, max_degree, indptr, indices,
                       elem_rank, inc_ptr, inc);
 }
 
@@ -165,6 +173,34 @@ extern "C" int fx_tet4_face_incidence(int32_t num_nodes, int32_t num_cells, cons
     int32_t e = faces[2 * f], lf = faces[2 * f + 1];
     for (int v = 0; v < 3; ++v) {
       int a = kTetFaceVerts[lf][v];
+      finc[pos[cells[static_cast<int64_t>(e) * 4 + a]]++] = f * 8 + a;
+    }
+  }
+  return FX_OK;
+}
+
+// QUAD4 edges in feax numbering: basix quadrilateral facets through the
+// re-order [0, 1, 3, 2] (feax/basis.py:90-95): bottom, left, right, top.
+extern "C" int fx_quad4_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells,
+                                       int32_t num_faces, const int32_t* faces, int32_t* finc_ptr,
+                                       int32_t* finc) {
+  static const int kQuadEdges[4][2] = {{0, 1}, {0, 3}, {1, 2}, {3, 2}};
+  if (num_nodes <= 0 || num_faces < 0 || !cells || !finc_ptr || (num_faces > 0 && (!faces || !finc)))
+    return fx::fail(FX_EINVAL, "fx_quad4_face_incidence: bad arguments");
+  std::vector<int32_t> cnt(static_cast<size_t>(num_nodes) + 1, 0);
+  for (int32_t f = 0; f < num_faces; ++f) {
+    int32_t e = faces[2 * f], lf = faces[2 * f + 1];
+    if (e < 0 || e >= num_cells || lf < 0 || lf > 3)
+      return fx::fail(FX_EINVAL, "fx_quad4_face_incidence: bad face (%d, %d)", e, lf);
+    for (int v = 0; v < 2; ++v) cnt[cells[static_cast<int64_t>(e) * 4 + kQuadEdges[lf][v]] + 1]++;
+  }
+  for (int32_t i = 0; i < num_nodes; ++i) cnt[i + 1] += cnt[i];
+  std::copy(cnt.begin(), cnt.end(), finc_ptr);
+  std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
+  for (int32_t f = 0; f < num_faces; ++f) {
+    int32_t e = faces[2 * f], lf = faces[2 * f + 1];
+    for (int v = 0; v < 2; ++v) {
+      int a = kQuadEdges[lf][v];
       finc[pos[cells[static_cast<int64_t>(e) * 4 + a]]++] = f * 8 + a;
     }
   }

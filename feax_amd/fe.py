@@ -1,4 +1,4 @@
-"""HEX8 / TET4 reference-element facts needed on the host.
+"""HEX8 / TET4 / QUAD4 reference-element facts needed on the host.
 
 Geometry itself (shape gradients, JxW, Nanson scale) is never stored: the
 HIP kernels recompute it per Gauss point from the node coordinates.  The
@@ -18,8 +18,12 @@ NUM_FACE_QUADS = 4   # 2x2 per facet
 # TET4 (feax/basis.py:78-83): basix tetrahedron facets, facet i opposite
 # vertex i; one Gauss point per cell and per facet.
 TET4_FACE_INDS = np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]], dtype=np.int64)
-ELEMENTS = {"HEX8": dict(npe=8, faces=HEX8_FACE_INDS, num_quads=NUM_QUADS, num_face_quads=NUM_FACE_QUADS),
-            "TET4": dict(npe=4, faces=TET4_FACE_INDS, num_quads=1, num_face_quads=1)}
+# QUAD4 (feax/basis.py:90-95): basix quadrilateral edges through the
+# re-order [0, 1, 3, 2]; 2x2 Gauss points, 2 per edge; dim = vec = 2.
+QUAD4_FACE_INDS = np.array([[0, 1], [0, 3], [1, 2], [3, 2]], dtype=np.int64)
+ELEMENTS = {"HEX8": dict(npe=8, faces=HEX8_FACE_INDS, num_quads=NUM_QUADS, num_face_quads=NUM_FACE_QUADS, dim=3),
+            "TET4": dict(npe=4, faces=TET4_FACE_INDS, num_quads=1, num_face_quads=1, dim=3),
+            "QUAD4": dict(npe=4, faces=QUAD4_FACE_INDS, num_quads=4, num_face_quads=2, dim=2)}
 
 
 def element_info(ele_type):

@@ -94,7 +94,9 @@ class SurfaceTraction:
     sign (feax/assembler.py:529-533)."""
 
     def __init__(self, direction=(0.0, 0.0, 1.0)):
-        self.direction = np.asarray(direction, float).reshape(3)
+        self.direction = np.asarray(direction, float).reshape(-1)
+        if self.direction.shape not in ((2,), (3,)):
+            raise ValueError(f"traction direction must have 2 or 3 components, got {self.direction}")
 
     def __call__(self, u, x, t):
         return self.direction * t

@@ -92,3 +92,21 @@ class StructuredGrid:
         cells = self.cell_to_nodes(ids)
         used = np.unique(cells)
         return Mesh(self.node_coords(used), np.searchsorted(used, cells), ele_type=self.ele_type)
+
+
+def rectangle_mesh(Nx, Ny, domain_x=1.0, domain_y=1.0, origin=(0.0, 0.0), ele_type="QUAD4"):
+    """Structured QUAD4 mesh of a rectangle (feax/mesh.py:447-502): node
+    i (Ny + 1) + j at (x_i, y_j), cell (i Ny + j) = counter-clockwise
+    [(i, j), (i+1, j), (i+1, j+1), (i, j+1)]."""
+    if ele_type != "QUAD4":
+        raise NotImplementedError(f"rectangle_mesh builds QUAD4 here (got {ele_type})")
+    x0, y0 = origin
+    x = np.linspace(x0, x0 + domain_x, Nx + 1)
+    y = np.linspace(y0, y0 + domain_y, Ny + 1)
+    xv, yv = np.meshgrid(x, y, indexing="ij")
+    points = np.stack([xv.reshape(-1), yv.reshape(-1)], axis=1)
+    i, j = np.meshgrid(np.arange(Nx), np.arange(Ny), indexing="ij")
+    i, j = i.reshape(-1), j.reshape(-1)
+    n0 = i * (Ny + 1) + j
+    cells = np.stack([n0, n0 + (Ny + 1), n0 + (Ny + 1) + 1, n0 + 1], axis=1).astype(np.int32)
+    return Mesh(points, cells, ele_type="QUAD4")

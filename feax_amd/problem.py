@@ -1,4 +1,5 @@
-"""Problem: FE structure for HEX8, vec = 3 problems (the feax.Problem API).
+"""Problem: FE structure for HEX8 / TET4 (vec = 3) and QUAD4 (vec = 2)
+problems (the feax.Problem API).
 
 Mirrors feax.Problem (feax/problem.py:37-846): constructor arguments,
 ``location_fns`` / ``boundary_inds_list``, ``matrix_view``, the physics
@@ -55,7 +56,8 @@ class FiniteElement:
 
 @dataclass
 class Problem:
-    """HEX8 vector problem (vec = 3, dim = 3) on an explicit mesh."""
+    """Vector problem on an explicit mesh: HEX8 / TET4 (vec = dim = 3) or
+    QUAD4 (vec = dim = 2)."""
     mesh: Union[Mesh, List[Mesh]]
     vec: Union[int, List[int]]
     dim: int
@@ -77,11 +79,13 @@ class Problem:
             self.mesh, self.vec = self.mesh[0], self.vec[0]
             self.ele_type = self.ele_type[0] if isinstance(self.ele_type, list) else self.ele_type
             self.gauss_order = self.gauss_order[0] if isinstance(self.gauss_order, list) else self.gauss_order
-        if self.ele_type not in ("HEX8", "TET4") or self.vec != 3 or self.dim != 3:
+        ok = ((self.ele_type in ("HEX8", "TET4") and self.vec == 3 and self.dim == 3)
+              or (self.ele_type == "QUAD4" and self.vec == 2 and self.dim == 2))
+        if not ok:
             raise NotImplementedError(
-                f"the HIP path covers HEX8 and TET4 with vec=3, dim=3 (got {self.ele_type}, vec={self.vec}, "
-                f"dim={self.dim})")
-        default_order = 2 if self.ele_type == "HEX8" else 0
+                f"the HIP path covers HEX8 and TET4 with vec=3, dim=3 and QUAD4 with vec=2, dim=2 (got "
+                f"{self.ele_type}, vec={self.vec}, dim={self.dim})")
+        default_order = 0 if self.ele_type == "TET4" else 2
         if self.gauss_order not in (None, default_order):
             raise NotImplementedError(f"{self.ele_type} uses feax's default rule (gauss_order={default_order})")
         self.npe = element_info(self.ele_type)["npe"]
@@ -94,7 +98,7 @@ class Problem:
         self.cells_list = [self.cells]
         self.num_cells = len(self.cells)
         self.num_total_nodes = len(self.points)
-        self.num_total_dofs_all_vars = 3 * self.num_total_nodes
+        self.num_total_dofs_all_vars = self.vec * self.num_total_nodes
         self.offset = [0]
         self.boundary_inds_list = boundary_conditions_inds(self.points, self.cells, self.location_fns, self.ele_type)
         self._build_csr_assembly_structure()
@@ -102,11 +106,12 @@ class Problem:
 
     # ------------------------------------------------------------------ structure
     def _build_csr_assembly_structure(self):
-        """CSR pattern + assembly maps (feax/problem.py:316-484) via fx_hex8_pattern / fx_tet4_pattern."""
+        """CSR pattern + assembly maps (feax/problem.py:316-484) via fx_hex8_pattern / fx_tet4_pattern /
+        fx_quad4_pattern."""
         N, E, npe = self.num_total_nodes, self.num_cells, self.npe
-        n = 3 * N
+        n = self.vec * N
         L = nat.lib()
-        tag = "hex8" if self.ele_type == "HEX8" else "tet4"
+        tag = self.ele_type.lower()
         pattern = getattr(L, f"fx_{tag}_pattern")
         nse = ctypes.c_int64(0)
         maxdeg = ctypes.c_int32(0)
@@ -194,26 +199,28 @@ class Problem:
 
     @property
     def csr_volume_slots(self):
-        """(E, (3 npe)^2) int32: CSR slot of each element Jacobian entry jac[a*3+i,
-        b*3+j] (feax/problem.py:458-463), from the rank table:
-        indptr[3 n_a + i] + 3 rank[e, a, b] + j."""
+        """(E, (vec npe)^2) int32: CSR slot of each element Jacobian entry
+        jac[a*vec+i, b*vec+j] (feax/problem.py:458-463), from the rank table:
+        indptr[vec n_a + i] + vec rank[e, a, b] + j."""
         if getattr(self, "_vol_slots", None) is None:
             c = self.cells.astype(np.int64)
+            v = self.vec
             rank = self.elem_rank.reshape(-1, self.npe, self.npe).astype(np.int64)
-            rowp = self.csr_indptr.astype(np.int64)[3 * c[:, :, None] + np.arange(3)]      # (E, 8a, 3i)
-            slot = (rowp[:, :, :, None, None] + 3 * rank[:, :, None, :, None]
-                    + np.arange(3)[None, None, None, None, :])                            # (E, a, i, b, j)
-            self._vol_slots = slot.reshape(len(c), (3 * self.npe) ** 2).astype(np.int32)
+            rowp = self.csr_indptr.astype(np.int64)[v * c[:, :, None] + np.arange(v)]      # (E, a, i)
+            slot = (rowp[:, :, :, None, None] + v * rank[:, :, None, :, None]
+                    + np.arange(v)[None, None, None, None, :])                            # (E, a, i, b, j)
+            self._vol_slots = slot.reshape(len(c), (v * self.npe) ** 2).astype(np.int32)
         return self._vol_slots
 
     def _element_inds(self, cells):
         c = np.asarray(cells, np.int64)
-        return (3 * c[:, :, None] + np.arange(3)[None, None, :]).reshape(len(c), -1).astype(np.int32)
+        v = self.vec
+        return (v * c[:, :, None] + np.arange(v)[None, None, :]).reshape(len(c), -1).astype(np.int32)
 
     def _coo(self):
         inds = [self._element_inds(self.cells)] + [self._element_inds(self.cells[b[:, 0]])
                                                    for b in self.boundary_inds_list]
-        nd = 3 * self.npe
+        nd = self.vec * self.npe
         I = np.concatenate([np.repeat(i[:, :, None], nd, axis=2).reshape(-1) for i in inds])
         J = np.concatenate([np.repeat(i[:, None, :], nd, axis=1).reshape(-1) for i in inds])
         return I, J
@@ -269,7 +276,7 @@ class Problem:
     # ------------------------------------------------------------------ layout
     def unflatten_fn_sol_list(self, flat):
         flat = getattr(flat, "dofs", flat)
-        return [flat.reshape(self.num_total_nodes, 3)]
+        return [flat.reshape(self.num_total_nodes, self.vec)]
 
     # ------------------------------------------------------------------ hooks
     def custom_init(self, *args):

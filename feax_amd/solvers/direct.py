@@ -11,7 +11,7 @@ symmetric matrix with a non-positive pivot (a Neo-Hookean tangent away from
 equilibrium) is factored again as L D L^T without pivoting
 (fx_band_ldlt_f64), as a symmetric-indefinite direct solver would.
 
-Ordering: the DOFs are renumbered node-wise (3 DOFs of a node stay
+Ordering: the DOFs are renumbered node-wise (the vec DOFs of a node stay
 together) by whichever of the mesh's own numbering and reverse
 Cuthill-McKee of the node graph gives the smaller bandwidth bw; the band
 holds n (bw + 256) FP64 values (44 GB at the 200x40x40 cantilever).
@@ -36,8 +36,8 @@ def _bandwidth(rows, cols, inv):
 def band_ordering(problem):
     """(perm, inv, bw): DOF renumbering new -> old and old -> new (None for
     the identity) with its bandwidth; cached on the problem.  Reverse
-    Cuthill-McKee runs on the node graph (each node's 3 DOFs stay
-    together) and is kept only when it narrows the band."""
+    Cuthill-McKee runs on the node graph (each node's DOFs stay together)
+    and is kept only when it narrows the band."""
     cached = getattr(problem, "_band_ordering", None)
     if cached is not None:
         return cached
@@ -48,12 +48,13 @@ def band_ordering(problem):
     n = len(indptr) - 1
     rows = np.repeat(np.arange(n, dtype=np.int32), np.diff(indptr))
     best = (None, None, _bandwidth(rows, cols, None))
-    if n % 3 == 0 and n > 3:
-        sel = (rows % 3 == 0) & (cols % 3 == 0)
-        N = n // 3
-        G = sp.csr_matrix((np.ones(int(sel.sum()), np.int8), (rows[sel] // 3, cols[sel] // 3)), shape=(N, N))
+    v = int(getattr(problem, "vec", 3))
+    if n % v == 0 and n > v:
+        sel = (rows % v == 0) & (cols % v == 0)
+        N = n // v
+        G = sp.csr_matrix((np.ones(int(sel.sum()), np.int8), (rows[sel] // v, cols[sel] // v)), shape=(N, N))
         node_perm = reverse_cuthill_mckee(G, symmetric_mode=True).astype(np.int64)
-        perm = (3 * node_perm[:, None] + np.arange(3)).reshape(-1)
+        perm = (v * node_perm[:, None] + np.arange(v)).reshape(-1)
         inv = np.empty(n, np.int64)
         inv[perm] = np.arange(n)
         bw = _bandwidth(rows, cols, inv)

@@ -249,12 +249,16 @@ def create_linear_solver(problem, bc, solver_options=None, adjoint_solver_option
             from .direct import solve_direct
             return solve_direct(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
         if use_gmg:
+            if problem.ele_type != "HEX8":
+                raise NotImplementedError("GeometricMultigrid is built on the structured HEX8 box")
             return solve_gmg(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
         if solver_options.solver in ("bicgstab", "gmres") and (solver_options.precision != "fp64"
                                                                or solver_options.operator != "csr"):
             raise NotImplementedError(f"{solver_options.solver} runs on the FP64 CSR operator "
                                       "(precision='fp64', operator='csr')")
         if solver_options.precision != "fp64" and solver_options.operator == "csr":
+            if problem.vec != 3:
+                raise NotImplementedError("mixed precision uses the block-3 difference form (vec = 3)")
             return solve_mixed(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
         return solve_fp64(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
 

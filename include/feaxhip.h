@@ -307,6 +307,55 @@ int fx_tet4_energy_f64(void* stream, int32_t num_cells, const double* x, const d
                        const int32_t* cells, const fx_material* mat, const double* param, const double* u,
                        double* energy, double* dpsi_dmu, void* workspace, size_t ws_bytes);
 
+/* ------------------------------------------------------------------ */
+/* QUAD4, dim = 2, vec = 2 (feax/basis.py:90-95: basix P1 quadrilateral
+ * through re-order [0, 1, 3, 2], 2x2 Gauss points; edges bottom (0,1),
+ * left (0,3), right (1,2), top (3,2), 2 points each): the structure
+ * builders (rows 2 n + i; incidence e * 8 + a; elem_rank 16 per element)
+ * and the two-pass kernels with the signatures of the HEX8 / TET4 entry
+ * points (z is ignored and may be NULL; svar holds 2 values per edge,
+ * fdir 2 per edge).  The laws are the plane-strain restrictions of the
+ * 3-D ones (du_z = 0).  FP64 element arithmetic only (no FX_ASM_DIFF3 /
+ * FX_ASM_COMPUTE32); the workspace is required (64 values per element for
+ * the matrix, 8 doubles for the residual, 4 per element for node-variable
+ * pullbacks). */
+int fx_quad4_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse,
+                     int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                     int32_t* inc_ptr, int32_t* inc);
+int fx_quad4_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int32_t num_faces,
+                            const int32_t* faces, int32_t* finc_ptr, int32_t* finc);
+int fx_quad4_assemble_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                          const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                          int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, double* data,
+                          void* workspace, size_t ws_bytes);
+int fx_quad4_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                          const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                          int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, float* data,
+                          void* workspace, size_t ws_bytes);
+size_t fx_quad4_assemble_workspace_bytes(int32_t num_cells, int32_t value_bytes, int32_t flags);
+int fx_quad4_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const fx_material* mat, const double* param, const double* u, int32_t num_faces,
+                          const int32_t* faces, const int32_t* finc_ptr, const int32_t* finc, const double* svar,
+                          const double* fdir, int32_t num_bc, const int32_t* bc_rows, const double* bc_vals,
+                          int32_t negate, double* res, void* workspace, size_t ws_bytes);
+size_t fx_quad4_residual_workspace_bytes(int32_t num_cells);
+size_t fx_quad4_param_vjp_workspace_bytes(int32_t num_cells, int32_t param_loc);
+int fx_quad4_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                           const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                           const fx_material* mat, const double* param, const double* u, const double* lam,
+                           const uint8_t* bc_mask, double scale, double* grad, void* workspace, size_t ws_bytes);
+int fx_quad4_simp_compliance_sens_f64(void* stream, int32_t num_cells, const double* x, const double* y,
+                                      const double* z, const int32_t* cells, const fx_material* mat,
+                                      const double* param, const double* u, double* dc);
+int fx_quad4_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, const double* y, const double* z,
+                              const int32_t* cells, const int32_t* faces, const double* fdir, const double* lam,
+                              const uint8_t* bc_mask, double scale, double* grad_svar);
+
 /* Matrix-free operator out = J_bc(u) v (FP64): feax's Krylov operator
  * create_matfree_res_J_parametric (feax/assembler.py:1747-1778),
  * J_bc v = where(bc, v, JVP_R(u)[v * !bc]) under symmetric elimination: rows

@@ -1,0 +1,239 @@
+"""GPU parity of the QUAD4 kernels (dim 2, vec 2) against the QUAD4 oracle
+(oracle/quad4.py): CSR assembly (FP64 / FP32, the three material laws), the
+residual with a per-quadrature-point dead traction and Dirichlet rows,
+linear solves (CG and the direct solver) against scipy's direct solve,
+Neo-Hookean Newton load steps against the oracle Newton, and gradients
+against central differences and the autograd compliance gradient."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPS64 = np.finfo(np.float64).eps
+EPS32 = np.finfo(np.float32).eps
+NU = 0.3
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _mesh(nx, ny, lx, ly, seed, distort):
+    from oracle import quad4
+    pts, cells = quad4.rectangle(nx, ny, lx, ly)
+    rng = np.random.default_rng(seed)
+    if distort:
+        inner = (pts[:, 0] > 1e-9) & (pts[:, 0] < lx - 1e-9) & (pts[:, 1] > 1e-9) & (pts[:, 1] < ly - 1e-9)
+        pts = pts.copy()
+        pts[inner] += rng.uniform(-0.2, 0.2, (inner.sum(), 2)) * min(lx / nx, ly / ny)
+    perm = rng.permutation(len(pts))
+    return pts[perm], np.argsort(perm)[cells][rng.permutation(len(cells))]
+
+
+def _problem(kind, n=(12, 5), L=(3.0, 1.0), seed=0, distort=True, load=True):
+    import feax_amd as fe
+    pts, cells = _mesh(n[0], n[1], L[0], L[1], seed, distort)
+
+    class P(fe.Problem):
+        def get_tensor_map(self):
+            if kind == "simp":
+                return fe.materials.SIMPLinearElastic(nu=NU, E0=1.0, Emin=1e-3, penal=3.0)
+            if kind == "neo_hookean":
+                return None
+            return fe.materials.IsotropicLinearElastic(nu=NU)
+
+        def get_energy_density(self):
+            return fe.materials.CompressibleNeoHookean(kappa=10.0) if kind == "neo_hookean" else None
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.0, 1.0))]
+
+    locs = [lambda p: np.isclose(p[0], L[0])] if load else []
+    prob = P(fe.Mesh(pts, cells, ele_type="QUAD4"), vec=2, dim=2, ele_type="QUAD4", location_fns=locs)
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[0], 0.0), "all", 0.0)]).create_bc(prob)
+    return prob, bc
+
+
+def _oracle(prob):
+    from oracle import quad4
+    return quad4.Model(prob.points, prob.cells.astype(np.int64), prob.boundary_inds_list)
+
+
+def _mat(kind):
+    from oracle import hex8
+    if kind == "simp":
+        return hex8.Material("simp", nu=NU, E0=1.0, Emin=1e-3, penal=3.0)
+    if kind == "neo_hookean":
+        return hex8.Material("neo_hookean", kappa=10.0)
+    return hex8.Material("linear_elastic", nu=NU)
+
+
+def _param(kind, prob, seed=1):
+    rng = np.random.default_rng(seed)
+    if kind == "linear_elastic":
+        return rng.uniform(50.0, 90.0, prob.num_total_nodes)      # node variable
+    if kind == "simp":
+        return rng.uniform(0.3, 1.0, prob.num_cells)
+    return rng.uniform(0.5, 1.5, prob.num_cells)
+
+
+@pytest.mark.parametrize("kind", ["linear_elastic", "simp", "neo_hookean"])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_quad4_assembly_matches_oracle(kind, dtype):
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import assemble_csr
+    from oracle import hex8
+    prob, bc = _problem(kind)
+    m = _oracle(prob)
+    ts = fe.TracedStructure.from_problem(prob)
+    param = _param(kind, prob)
+    u = 0.03 * np.random.default_rng(2).standard_normal(m.n) if kind == "neo_hookean" else np.zeros(m.n)
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    print("quad4 assembly", kind, dtype, "cells", prob.num_cells, flush=True)
+    A = assemble_csr(prob, ts, fe.TracedParams(volume_vars=(param,)), torch.as_tensor(u, device="cuda"), bc, True, dt)
+    assert np.array_equal(_np(A.indptr), m.csr["indptr"]) and np.array_equal(_np(A.indices), m.csr["indices"])
+    mask = np.zeros(m.n, bool)
+    mask[bc.bc_rows] = True
+    ref = hex8.apply_bc_csr(m.csr_values(u, param, _mat(kind)), m.csr, mask, bc.bc_rows, True)
+    K = m.scipy_csr(ref)
+    scale = np.repeat(np.abs(K).max(axis=1).toarray().ravel(), np.diff(K.indptr))
+    err = np.abs(_np(A.data).astype(np.float64) - ref) / scale
+    tol = 64 * (EPS64 if dtype == "f64" else EPS32)
+    print(kind, dtype, "assembly max err / row max", err.max())
+    assert err.max() <= tol
+
+
+@pytest.mark.parametrize("kind", ["linear_elastic", "neo_hookean"])
+def test_quad4_residual_matches_oracle(kind):
+    """Per-quadrature-point traction values (F, 2) pin the edge Gauss point
+    order between the kernel and the oracle."""
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import residual
+    from oracle import hex8, quad4
+    prob, bc = _problem(kind)
+    m = _oracle(prob)
+    ts = fe.TracedStructure.from_problem(prob)
+    param = _param(kind, prob)
+    rng = np.random.default_rng(4)
+    u = 0.03 * rng.standard_normal(m.n)
+    t = rng.uniform(0.2, 0.6, (len(prob.boundary_inds_list[0]), 2))
+    tp = fe.TracedParams(volume_vars=(param,), surface_vars=[(t,)])
+    print("quad4 residual", kind, flush=True)
+    r = _np(residual(prob, ts, tp, torch.as_tensor(u, device="cuda"), bc))
+    ref = m.residual(u, param, _mat(kind), [quad4.constant_traction(prob.boundary_inds_list[0], t)])
+    ref = hex8.apply_bc_res(ref, u, bc.bc_rows, bc.bc_vals)
+    err = np.abs(r - ref).max() / np.abs(ref).max()
+    print(kind, "residual rel err", err)
+    assert err <= 1e-13
+
+
+@pytest.mark.parametrize("opts", ["cg", "direct"])
+def test_quad4_linear_solve_matches_direct_solve(opts):
+    import scipy.sparse.linalg as spla
+    import feax_amd as fe
+    from oracle import krylov, quad4
+    prob, bc = _problem("simp", n=(24, 8))
+    m = _oracle(prob)
+    rho = _param("simp", prob)
+    t = 0.5
+    tp = fe.TracedParams(volume_vars=(rho,), surface_vars=[(fe.TracedParams.create_uniform_surface_var(prob, t),)])
+    o = fe.DirectSolverOptions() if opts == "direct" else \
+        fe.KrylovSolverOptions(solver="cg", tol=1e-11, atol=0.0, use_jacobi_preconditioner=True)
+    print("quad4 solve", opts, flush=True)
+    u = np.asarray(fe.create_solver(prob, bc, o, linear=True)(tp))
+    _, info = krylov.linear_solve(m, _mat("simp"), rho, [quad4.constant_traction(prob.boundary_inds_list[0], t)],
+                                  bc.bc_rows, bc.bc_vals, maxiter=1)
+    ud = spla.spsolve(info["K"].tocsc(), info["b"])
+    err = np.linalg.norm(u - ud) / np.linalg.norm(ud)
+    print(opts, "quad4 solve rel err vs scipy", err)
+    assert err < 1e-8
+
+
+def test_quad4_newton_matches_oracle_newton():
+    """Compression of a Neo-Hookean strip: bottom clamped, top pushed down."""
+    import torch
+    import feax_amd as fe
+    from oracle import newton as onewton
+    ny = 4
+    prob, _ = _problem("neo_hookean", n=(3, ny), L=(3.0, float(ny)), load=False, distort=False)
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[1], 0.0), "all", 0.0),
+                               fe.DirichletBCSpec(lambda p: np.isclose(p[1], ny), "y", 0.0)]).create_bc(prob)
+    node, comp = bc.bc_rows // 2, bc.bc_rows % 2
+    is_top = np.isclose(prob.points[node, 1], ny) & (comp == 1)
+    m = _oracle(prob)
+    mu = _param("neo_hookean", prob)
+    solver = fe.create_solver(prob, bc, fe.DirectSolverOptions(), newton_options=fe.NewtonOptions(tol=1e-10,
+                              rel_tol=1e-12), linear=False, return_solution=False)
+    tp = fe.TracedParams(volume_vars=(mu,))
+    u = torch.zeros(m.n, dtype=torch.float64, device="cuda")
+    uo = np.zeros(m.n)
+    rows = torch.as_tensor(bc.bc_rows.astype(np.int64), device="cuda")
+    for s in (1, 2):
+        vals = np.where(is_top, -0.05 * ny * s, 0.0)
+        u[rows] = torch.as_tensor(vals, device="cuda")
+        print("quad4 newton step", s, flush=True)
+        u = solver(tp, u, bc.replace_vals(vals))
+        uo[bc.bc_rows] = vals
+        uo, it_o, _ = onewton.newton(m, _mat("neo_hookean"), mu, bc.bc_rows, vals, uo, tol=1e-10, rel_tol=1e-12)
+        err = np.linalg.norm(_np(u) - uo) / np.linalg.norm(uo)
+        print("quad4 newton step", s, "iters", solver.stats.newton_iters, "oracle", it_o, "err", err)
+        assert err < 1e-9 and solver.stats.newton_iters == it_o
+
+
+@pytest.mark.parametrize("kind", ["simp", "linear_elastic"])
+def test_quad4_gradients_match_finite_differences(kind):
+    """d(u.u)/dp through the adjoint (QUAD4 parameter VJP; cell rho for SIMP,
+    node E for linear elasticity) and d(u.u)/dt for each edge Gauss point's
+    traction through the traction VJP against central differences of
+    re-solved states; the fused compliance sensitivity against the autograd
+    compliance gradient."""
+    import torch
+    import feax_amd as fe
+    prob, bc = _problem(kind, n=(8, 3))
+    p0 = _param(kind, prob)
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-13, atol=0.0, use_jacobi_preconditioner=True)
+    solver = fe.create_solver(prob, bc, opts, linear=True, return_solution=False)
+    t0 = np.random.default_rng(6).uniform(0.3, 0.7, (len(prob.boundary_inds_list[0]), 2))
+
+    def J(pv, tv):
+        with torch.no_grad():
+            u = solver(fe.TracedParams(volume_vars=(torch.as_tensor(pv, device="cuda"),),
+                                       surface_vars=[(torch.as_tensor(tv, device="cuda"),)]))
+            return float((u * u).sum())
+
+    r = torch.as_tensor(p0, device="cuda").requires_grad_(True)
+    tt = torch.as_tensor(t0, device="cuda").requires_grad_(True)
+    print("quad4 gradients", kind, flush=True)
+    u = solver(fe.TracedParams(volume_vars=(r,), surface_vars=[(tt,)]))
+    (u * u).sum().backward()
+    g, gt = _np(r.grad), _np(tt.grad)
+    rng = np.random.default_rng(9)
+    for i in rng.choice(len(p0), 5, replace=False):
+        eps = 1e-5 * max(1.0, abs(p0[i]))
+        pp, pm = p0.copy(), p0.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        fd = (J(pp, t0) - J(pm, t0)) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * np.abs(g).max() + 1e-9 * abs(fd), (i, fd, g[i])
+    for f in rng.choice(len(t0), 2, replace=False):
+        for q in range(2):
+            tp_, tm_ = t0.copy(), t0.copy()
+            tp_[f, q] += 1e-5
+            tm_[f, q] -= 1e-5
+            fd = (J(p0, tp_) - J(p0, tm_)) / 2e-5
+            assert abs(fd - gt[f, q]) <= 1e-6 * np.abs(gt).max(), (f, q, fd, gt[f, q])
+    if kind == "simp":
+        from feax_amd.gene import compliance_and_sensitivity, create_compliance_fn
+        tu = fe.TracedParams.create_uniform_surface_var(prob, 0.5)
+        tp = fe.TracedParams(volume_vars=(torch.as_tensor(p0, device="cuda"),), surface_vars=[(tu,)])
+        with torch.no_grad():
+            us = solver(tp)
+        C, dC = compliance_and_sensitivity(prob, tp, us, surface_load_params=0.5)
+        r2 = torch.as_tensor(p0, device="cuda").requires_grad_(True)
+        u2 = solver(fe.TracedParams(volume_vars=(r2,), surface_vars=[(tu,)]))
+        create_compliance_fn(prob, surface_load_params=0.5)(u2).backward()
+        ref = _np(r2.grad)
+        assert np.abs(_np(dC) - ref).max() <= 1e-8 * np.abs(ref).max()
