@@ -58,6 +58,8 @@ def solve_fp64(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None, keep=N
     the operator for an adjoint solve."""
     import torch
     if opts.operator == "matfree":
+        if problem.ele_type != "HEX8":
+            raise NotImplementedError("the matrix-free operator is the HEX8 one")
         return _solve_matfree(problem, ts, tp, bc, u0, opts, symmetric, stats, keep)
     if opts.use_jacobi_preconditioner:
         J, dinv = assemble_csr(problem, ts, tp, u0, bc, symmetric, torch.float64, jacobi_shift=opts.jacobi_shift)

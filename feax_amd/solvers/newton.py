@@ -71,6 +71,8 @@ def newton_solve(problem, ts, tp, bc, sol0, lin, newton, symmetric=True, stats=N
         x0_fn = lin.x0_fn if lin.x0_fn is not None else create_x0(bc.bc_rows, bc.bc_vals)
         mixed = lin.precision != "fp64"
         shift = lin.jacobi_shift if lin.use_jacobi_preconditioner else None
+        if mixed and problem.vec != 3:
+            raise NotImplementedError("mixed precision uses the block-3 difference form (vec = 3)")
     it, found, fallbacks = 0, True, 0
     hist, alphas, lits = [r0], [], []
     if newton.verbose:

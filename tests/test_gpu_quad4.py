@@ -237,3 +237,17 @@ def test_quad4_gradients_match_finite_differences(kind):
         create_compliance_fn(prob, surface_load_params=0.5)(u2).backward()
         ref = _np(r2.grad)
         assert np.abs(_np(dC) - ref).max() <= 1e-8 * np.abs(ref).max()
+
+
+def test_quad4_paths_built_for_vec3_hex8_raise():
+    """Mixed precision (block-3 difference form), multigrid and the
+    matrix-free operator (HEX8) refuse a QUAD4 problem instead of running."""
+    import feax_amd as fe
+    prob, bc = _problem("simp", n=(4, 2))
+    tp = fe.TracedParams(volume_vars=(_param("simp", prob),),
+                         surface_vars=[(fe.TracedParams.create_uniform_surface_var(prob, 0.5),)])
+    for o in (fe.KrylovSolverOptions(solver="cg", precision="mixed"),
+              fe.KrylovSolverOptions(solver="cg", operator="matfree"),
+              fe.KrylovSolverOptions(solver="cg", preconditioner=fe.GeometricMultigrid())):
+        with pytest.raises(NotImplementedError):
+            fe.create_solver(prob, bc, o, linear=True)(tp)
