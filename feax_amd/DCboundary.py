@@ -25,8 +25,21 @@ class DirichletBC:
     total_dofs: int
 
     def replace_vals(self, new_bc_vals):
-        """Same DOFs, new prescribed values (feax/DCboundary.py:36-61)."""
+        """Same DOFs, new prescribed values (feax/DCboundary.py:36-61).  A torch
+        tensor is kept as it is, so a solve can differentiate with respect to
+        it (the bc_vals cotangent of feax's custom VJPs)."""
+        if type(new_bc_vals).__module__.startswith("torch"):
+            if tuple(new_bc_vals.shape) != (len(self.bc_rows),):
+                raise ValueError(f"bc_vals of shape {tuple(new_bc_vals.shape)} for {len(self.bc_rows)} rows")
+            return DirichletBC(self.bc_rows, self.bc_mask, new_bc_vals, self.total_dofs)
         return DirichletBC(self.bc_rows, self.bc_mask, np.asarray(new_bc_vals, np.float64), self.total_dofs)
+
+    def vals_numpy(self):
+        """bc_vals as a float64 numpy array (a tensor is detached and copied)."""
+        v = self.bc_vals
+        if type(v).__module__.startswith("torch"):
+            return v.detach().to("cpu", dtype=__import__("torch").float64).numpy()
+        return np.asarray(v, np.float64)
 
     @staticmethod
     def from_specs(problem, specs):
@@ -38,9 +51,11 @@ class DirichletBC:
         key = str(dev)
         cache = self.__dict__.setdefault("_dev", {})
         if key not in cache:
+            vals = self.bc_vals
+            vals = vals.detach().to(device=dev, dtype=torch.float64).contiguous() if isinstance(vals, torch.Tensor) \
+                else torch.as_tensor(np.ascontiguousarray(vals, np.float64), device=dev)
             cache[key] = (torch.as_tensor(np.ascontiguousarray(self.bc_rows, np.int32), device=dev),
-                          torch.as_tensor(np.ascontiguousarray(self.bc_mask, np.uint8), device=dev),
-                          torch.as_tensor(np.ascontiguousarray(self.bc_vals, np.float64), device=dev))
+                          torch.as_tensor(np.ascontiguousarray(self.bc_mask, np.uint8), device=dev), vals)
         return cache[key]
 
 

@@ -126,7 +126,7 @@ def batch_bc(bp, bcs):
     rows, vals, mask = [], [], []
     for bc, off in zip(bcs, bp.seg_node_ptr[:-1]):
         rows.append(np.asarray(bc.bc_rows, np.int64) + 3 * int(off))
-        vals.append(np.asarray(bc.bc_vals, np.float64))
+        vals.append(bc.vals_numpy())
         mask.append(np.asarray(bc.bc_mask, bool))
     return DirichletBC(np.concatenate(rows).astype(np.int32), np.concatenate(mask), np.concatenate(vals),
                        bp.num_total_dofs_all_vars)

@@ -42,7 +42,8 @@ class IsotropicLinearElastic(TensorMap):
         lam = E * self.nu / ((1 + self.nu) * (1 - 2 * self.nu))
         mu = E / (2 * (1 + self.nu))
         eps = 0.5 * (u_grad + np.swapaxes(u_grad, -1, -2))
-        return lam * np.trace(eps, axis1=-2, axis2=-1)[..., None, None] * np.eye(3) + 2 * mu * eps
+        d = np.shape(u_grad)[-1]
+        return lam * np.trace(eps, axis1=-2, axis2=-1)[..., None, None] * np.eye(d) + 2 * mu * eps
 
     def material(self, param_loc):
         return nat.material_struct(self.kind, param_loc, nu=self.nu)
@@ -78,7 +79,7 @@ class CompressibleNeoHookean(TensorMap):
         self.kappa = float(kappa)
 
     def __call__(self, u_grad, mu):
-        F = np.eye(3) + u_grad
+        F = np.eye(np.shape(u_grad)[-1]) + u_grad
         J = np.linalg.det(F)
         FiT = np.swapaxes(np.linalg.inv(F), -1, -2)
         return mu * (F - FiT) + self.kappa * (J - 1) * J * FiT

@@ -54,8 +54,42 @@ class Solution:
     def __rmul__(self, o):
         return self._other(o) * self.dofs
 
+    def __truediv__(self, o):
+        return self.dofs / self._other(o)
+
+    def __rtruediv__(self, o):
+        return self._other(o) / self.dofs
+
+    def __pow__(self, o):
+        return self.dofs ** self._other(o)
+
     def __neg__(self):
         return -self.dofs
+
+    def __matmul__(self, o):
+        return self.dofs @ self._other(o)
+
+    def __rmatmul__(self, o):
+        return self._other(o) @ self.dofs
+
+    def block_until_ready(self):
+        """Waits for the device work that produces the DOFs."""
+        if hasattr(self.dofs, "is_cuda") and self.dofs.is_cuda:
+            import torch
+            torch.cuda.synchronize(self.dofs.device)
+        return self
+
+    @classmethod
+    def __torch_function__(cls, func, types, args=(), kwargs=None):
+        """torch functions see the flat DOF vector (the role of feax's
+        __jax_array__): torch.sum(sol), torch.dot(sol, v), ..."""
+        def unwrap(a):
+            if isinstance(a, Solution):
+                return a.dofs
+            if isinstance(a, (list, tuple)):
+                return type(a)(unwrap(x) for x in a)
+            return a
+        return func(*unwrap(args), **{k: unwrap(v) for k, v in (kwargs or {}).items()})
 
     def __getitem__(self, key):
         return self.dofs[key]

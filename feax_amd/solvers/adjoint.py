@@ -8,6 +8,17 @@ the linear solve (feax/solvers/linear.py:387-483):
                                                  elimination; lambda[bc] = 0
             grad p    = -(dR/dp)^T lambda        fx_hex8_param_vjp_f64
             grad svar = -(dR/dsvar)^T lambda     fx_hex8_traction_vjp_f64
+            grad bc_vals = v_b - (K lambda_f)_b  K the un-eliminated FP64
+                                                 Jacobian at u (symmetric for
+                                                 the laws here)
+
+The bc_vals cotangent follows feax: -(d R_bc / d bc_vals)^T lambda with
+R_bc[b] = u_b - bc_vals gives lambda_b of the full adjoint J^T lambda = v.
+Where J keeps the Dirichlet columns (non-symmetric elimination) or the
+Newton adjoint corrects lambda_b (feax/solvers/newton.py:200-222,
+1031-1069) that is v_b - K_fb^T lambda_f; the linear solve under symmetric
+elimination leaves lambda_b = v_b (feax/solvers/linear.py:421-483), its
+free DOFs seeing the Dirichlet values only through the initial guess.
 
 The adjoint system reuses the forward's operator: the FP64 CSR (precision
 "fp64"; with its multigrid hierarchy under a GeometricMultigrid
@@ -18,6 +29,29 @@ when they are still held.  Initial guesses get no gradient, as in feax.
 """
 
 import numpy as np
+
+
+def _bc_leaf(bc):
+    """bc.bc_vals when it is a torch tensor that requires grad, else None."""
+    import torch
+    v = getattr(bc, "bc_vals", None)
+    return v if isinstance(v, torch.Tensor) and v.requires_grad else None
+
+
+def bc_gradient(problem, ts, tp, bc, u, v, lam, coupling):
+    """d(v . u)/d bc_vals: v_b, minus (K(u) lambda_f)_b when the Dirichlet
+    columns couple (coupling)."""
+    import torch
+    rows = bc.device(ts.device)[0].long()
+    vv = v.detach().to(device=ts.device, dtype=torch.float64).reshape(-1)
+    g = vv[rows].clone()
+    if coupling:
+        from ..assembler import assemble_csr
+        lf = lam.detach().to(torch.float64).clone()
+        lf[rows] = 0.0
+        K = assemble_csr(problem, ts, tp, u.detach(), None, True, torch.float64)
+        g -= K.matvec(lf)[rows]
+    return g
 
 
 def _needs_grad(tp):
@@ -47,6 +81,8 @@ def _leaves(tp):
 
 def _rebuild(tp, paths, tensors):
     from ..traced_params import TracedParams
+    if tp is None:
+        return None
     vv = list(tp.volume_vars)
     sv = [list(s) for s in tp.surface_vars]
     for path, t in zip(paths, tensors):
@@ -171,12 +207,13 @@ def _function():
     class LinearSolveFunction(torch.autograd.Function):
         @staticmethod
         def forward(ctx, call, *tensors):
-            tp = _rebuild(call["tp"], call["paths"], [t.detach() for t in tensors])
+            n = len(call["paths"])
+            tp = _rebuild(call["tp"], call["paths"], [t.detach() for t in tensors[:n]])
             keep = {}
             u = call["forward"](tp, call["ig"], call["bc"], call["ts"], keep=keep)
             ctx.call, ctx.keep, ctx.tp = call, keep, tp
             ctx.shapes = [tuple(t.shape) for t in tensors]
-            ctx.dtypes = [t.dtype for t in tensors]
+            ctx.dtypes = [(t.dtype, t.device) for t in tensors]
             ctx.save_for_backward(u)
             return u
 
@@ -185,11 +222,19 @@ def _function():
             (u,) = ctx.saved_tensors
             call, keep = ctx.call, ctx.keep
             ts = keep["ts"]
-            adj = call["adjoint_fn"] or adjoint_solve
-            lam = adj(call["problem"], ts, ctx.tp, call["bc"], v, keep, call["adj_opts"])
+            n = len(call["paths"])
+            needs = ctx.needs_input_grad[1:]
+            want_bc = call["has_bc"] and needs[n]
+            lam = None
+            if any(needs[:n]) or (want_bc and call["bc_coupling"]):
+                adj = call["adjoint_fn"] or adjoint_solve
+                lam = adj(call["problem"], ts, ctx.tp, call["bc"], v, keep, call["adj_opts"])
             grads = parameter_gradients(call["problem"], ts, ctx.tp, call["bc"], u, lam, call["paths"],
-                                        ctx.shapes, ctx.needs_input_grad[1:])
-            grads = [g if g is None else g.to(dt) for g, dt in zip(grads, ctx.dtypes)]
+                                        ctx.shapes[:n], needs[:n])
+            if call["has_bc"]:
+                grads.append(bc_gradient(call["problem"], ts, ctx.tp, call["bc"], u, v, lam, call["bc_coupling"])
+                             if want_bc else None)
+            grads = [g if g is None else g.to(device=d, dtype=dt) for g, (dt, d) in zip(grads, ctx.dtypes)]
             return (None, *grads)
 
     return LinearSolveFunction
@@ -198,23 +243,31 @@ def _function():
 _FN = None
 
 
-def differentiable_solve(problem, forward, tp, ig, bc, ts, adj_opts, symmetric, adjoint_fn=None):
-    """u = forward(tp, ig, bc, ts), differentiable in the torch tensors of tp.
+def differentiable_solve(problem, forward, tp, ig, bc, ts, adj_opts, symmetric, adjoint_fn=None, bc_coupling=True):
+    """u = forward(tp, ig, bc, ts), differentiable in the torch tensors of tp
+    and in bc.bc_vals when that is a tensor requiring grad.
     adjoint_fn(problem, ts, tp, bc, v, keep, opts) -> lambda (default: the
-    linear solve's adjoint_solve on the operator the forward kept)."""
-    if not _needs_grad(tp):
+    linear solve's adjoint_solve on the operator the forward kept);
+    bc_coupling: whether the bc_vals gradient carries -(K lambda_f)_b."""
+    bc_leaf = _bc_leaf(bc)
+    if not _needs_grad(tp) and bc_leaf is None:
         return forward(tp, ig, bc, ts)
     global _FN
     if _FN is None:
         _FN = _function()
-    leaves = _leaves(tp)
+    leaves = _leaves(tp) if tp is not None else []
     call = dict(problem=problem, forward=forward, tp=tp, ig=ig, bc=bc, ts=ts, adj_opts=adj_opts,
-                paths=[p for p, _ in leaves], adjoint_fn=adjoint_fn)
-    return _FN.apply(call, *[t for _, t in leaves])
+                paths=[p for p, _ in leaves], adjoint_fn=adjoint_fn, has_bc=bc_leaf is not None,
+                bc_coupling=bool(bc_coupling))
+    tensors = [t for _, t in leaves] + ([bc_leaf] if bc_leaf is not None else [])
+    return _FN.apply(call, *tensors)
 
 
 def differentiable_linear_solve(problem, forward, tp, ig, bc, ts, adj_opts, symmetric):
-    return differentiable_solve(problem, forward, tp, ig, bc, ts, adj_opts, symmetric)
+    # Under symmetric elimination the forward's free DOFs see the Dirichlet
+    # values only through the initial guess, so the bc_vals gradient is v_b.
+    return differentiable_solve(problem, forward, tp, ig, bc, ts, adj_opts, symmetric, bc_coupling=not symmetric)
 
 
-__all__ = ["differentiable_solve", "differentiable_linear_solve", "adjoint_solve", "parameter_gradients"]
+__all__ = ["differentiable_solve", "differentiable_linear_solve", "adjoint_solve", "parameter_gradients",
+           "bc_gradient"]

@@ -276,7 +276,7 @@ def create_x0(bc_rows=None, bc_vals=None):
         rows = torch.as_tensor(np.asarray(bc_rows, np.int32), device=x0.device) \
             if not isinstance(bc_rows, torch.Tensor) else bc_rows.to(torch.int32)
         vals = torch.as_tensor(np.asarray(bc_vals, np.float64), device=x0.device) \
-            if not isinstance(bc_vals, torch.Tensor) else bc_vals.to(torch.float64)
+            if not isinstance(bc_vals, torch.Tensor) else bc_vals.detach().to(device=x0.device, dtype=torch.float64)
         u0 = current_sol.to(torch.float64).contiguous()
         nat.call("fx_bc_lift_f64", nat.stream(), x0.numel(), len(rows), nat.ptr(rows.contiguous()),
                  nat.ptr(vals.contiguous()), nat.ptr(u0), nat.ptr(x0))

@@ -246,7 +246,10 @@ def create_linear_solver(problem, bc, solver_options=None, adjoint_solver_option
             problem._default_ts = ts
         if keep is not None:
             keep["ts"] = ts
-        u0 = _dev_vec(ig, ts) if ig is not None else _zero_like(problem, eff_bc, ts)
+        # No initial guess: feax's default, zero_like_initial_guess of the
+        # solver's own bc (feax/solvers/linear.py:471-483), also when another
+        # bc is passed to the call.
+        u0 = _dev_vec(ig, ts) if ig is not None else _zero_like(problem, default_bc, ts)
         if use_direct:
             from .direct import solve_direct
             return solve_direct(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)

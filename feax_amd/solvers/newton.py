@@ -98,6 +98,10 @@ def newton_solve(problem, ts, tp, bc, sol0, lin, newton, symmetric=True, stats=N
                 J, dinv = out if shift is not None else (out, None)
                 du, info = pcg(J, b, x0_fn(sol), lin.tol, lin.atol, lin.maxiter, dinv, None, lin.check_every)
                 n_lin += info.iters
+        elif lin.solver in ("bicgstab", "gmres"):
+            from .common import krylov
+            du, info = krylov(J, b, x0_fn(sol), lin, dinv)
+            n_lin = info.iters
         else:
             du, info = pcg(J, b, x0_fn(sol), lin.tol, lin.atol, lin.maxiter, dinv, None, lin.check_every)
             n_lin = info.iters

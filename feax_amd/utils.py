@@ -9,7 +9,7 @@ def zero_like_initial_guess(problem, bc, device=None):
     Returns a numpy array (device=None) or a device tensor (device given).
     """
     u = np.zeros(problem.num_total_dofs_all_vars)
-    u[np.asarray(bc.bc_rows, np.int64)] = bc.bc_vals
+    u[np.asarray(bc.bc_rows, np.int64)] = bc.vals_numpy()
     if device is None:
         return u
     import torch

@@ -211,3 +211,41 @@ def test_structured_dims_detects_box_grids_only():
     prob2 = fe.Problem(fe.Mesh(pts, cells), vec=3, dim=3)
     with pytest.raises(NotImplementedError, match="structured box"):
         structured_dims(prob2)
+
+
+def test_dirichlet_values_may_be_a_tensor():
+    """replace_vals keeps a torch tensor (so a solve can differentiate with
+    respect to it); vals_numpy and zero_like_initial_guess read it; a wrong
+    length is refused."""
+    import torch
+    import feax_amd as fe
+    mesh = fe.mesh.rectangle_mesh(2, 2)
+
+    class P(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.IsotropicLinearElastic(nu=0.25, E=2.5)
+
+    prob = P(mesh, vec=2, dim=2, ele_type="QUAD4")
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[0], 0.0), "all", 0.0)]).create_bc(prob)
+    t = torch.linspace(0.1, 0.6, len(bc.bc_rows), dtype=torch.float64, requires_grad=True)
+    bc2 = bc.replace_vals(t)
+    assert bc2.bc_vals is t and np.allclose(bc2.vals_numpy(), t.detach().numpy())
+    u = fe.zero_like_initial_guess(prob, bc2)
+    assert np.allclose(u[bc.bc_rows], t.detach().numpy()) and np.count_nonzero(u) == len(bc.bc_rows)
+    with pytest.raises(ValueError):
+        bc.replace_vals(torch.zeros(len(bc.bc_rows) + 1))
+
+
+def test_solution_behaves_as_the_flat_dof_vector():
+    """Arithmetic, indexing and torch functions see the flat DOFs
+    (feax/solution.py:60-170)."""
+    import torch
+    from feax_amd.solution import Solution
+    d = torch.arange(12, dtype=torch.float64)
+    s = Solution(d, ((4, 3),))
+    assert torch.equal(s ** 2, d ** 2) and torch.equal(s / 2, d / 2) and torch.equal(2 / (s + 1), 2 / (d + 1))
+    assert torch.equal(s - s, torch.zeros(12, dtype=torch.float64)) and float(s @ s) == float(d @ d)
+    assert float(torch.sum(s)) == 66.0 and float(torch.dot(s, d)) == float(d @ d)
+    assert torch.equal(torch.stack([s, d]), torch.stack([d, d]))
+    assert s.field(0).shape == (4, 3) and torch.equal(s.node_var(1), d.reshape(4, 3)[:, 1])
+    assert np.array_equal(np.asarray(s), d.numpy()) and s.block_until_ready() is s
