@@ -637,7 +637,7 @@ def main():
         "config": {"workload": f"hex8 cantilever {cfg['dims'][0]}x{cfg['dims'][1]}x{cfg['dims'][2]} "
                                f"h={cfg['spacing']}, {n} DOF, nse={nse}, Jacobi-PCG rel 1e-8",
                    "form": headline},
-        "forms": {k: {"s": round(v["s"], 6), "iters": v["iters"], "outer": v["outer"],
+        "forms": {k: {"s": round(v["s"], 6), "iters": v["iters"], "outer": v["outer"], "inner": v["inner"],
                       "true_relres": v["relres"], "oracle_err": v["oracle_err"]} for k, v in results.items()},
         "oracle_err_def": "max over 4096 seeded DOFs |u - u_oracle| / max|u_oracle| (FP64 oracle, rel 1e-12)",
         "mixed_vs_fp64_rel_diff": mixed_vs_fp64,

@@ -98,7 +98,7 @@ def adjoint_solve(problem, ts, tp, bc, v, keep, opts):
     import torch
     from ..assembler import volume_residual
     from ..csr import transpose_with_maps
-    from .common import device_norm, krylov, pcg
+    from .common import device_norm, inner_tolerance, krylov, pcg
     rhs = v.detach().to(device=ts.device, dtype=torch.float64).reshape(-1).clone()
     if bc is not None and len(bc.bc_rows):
         rhs.index_fill_(0, bc.device(ts.device)[0].long(), 0.0)
@@ -156,8 +156,8 @@ def adjoint_solve(problem, ts, tp, bc, v, keep, opts):
     rn, outer, budget, inner = bnorm, 0, opts.maxiter, []
     while rn > target and outer < opts.max_outer and budget > 0:
         x0 = rec.project(r) if rec is not None else None
-        _, info = pcg(A, r, x0, opts.inner_tol, 0.0, budget, dinv, True, opts.check_every, out=d,
-                      vec_dtype=torch.float64)
+        _, info = pcg(A, r, x0, inner_tolerance(opts, target, rn), 0.0, budget, dinv, True, opts.check_every,
+                      out=d, vec_dtype=torch.float64)
         lam += d
         res = volume_residual(problem, ts, tp, lam, bc, negate=True)   # -K_bc lam
         torch.add(rhs, res, out=r)

@@ -264,6 +264,15 @@ def create_jacobi_preconditioner(A: CSRMatrix, shift=1e-12):
     return d
 
 
+def inner_tolerance(opts, target, rn):
+    """Relative tolerance of a mixed-precision inner solve whose right-hand
+    side has norm rn when the outer loop stops at target: inner_tol, or
+    the smaller reduction the target still needs (times inner_safety)."""
+    if opts.inner_safety <= 0 or rn <= 0:
+        return opts.inner_tol
+    return min(0.5, max(opts.inner_tol, opts.inner_safety * target / rn))
+
+
 def create_x0(bc_rows=None, bc_vals=None):
     """x0_fn(current_sol) lifting BC increments (feax/solvers/common.py:23-53)."""
     import torch

@@ -29,7 +29,7 @@ from .. import _native as nat
 from ..assembler import TracedStructure, assemble_csr, residual
 from ..DCboundary import DirichletBC
 from .common import (KrylovRecord, check_convergence, create_iterative_solve_fn, create_jacobi_preconditioner,
-                     create_x0, device_norm, krylov, pcg)
+                     create_x0, device_norm, inner_tolerance, krylov, pcg)
 from .options import DirectSolverOptions, KrylovSolverOptions, MatrixProperty, resolve_iterative_solver
 
 
@@ -170,7 +170,8 @@ def solve_mixed(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None, keep=
             nat.call("fx_convert_f64_to_f32", nat.stream(), n, 1.0 / rn, nat.ptr(r), nat.ptr(rv))
         x0 = record.project(rv) if (record is not None and outer > 0) else None
         rec = record if (record is not None and outer == 0) else None
-        _, info = pcg(A, rv, x0, opts.inner_tol, 0.0, budget, dinv, True, opts.check_every, out=d, record=rec)
+        tin = inner_tolerance(opts, target, rn)
+        _, info = pcg(A, rv, x0, tin, 0.0, budget, dinv, True, opts.check_every, out=d, record=rec)
         if fp32vec:
             nat.call("fx_axpy_f32_into_f64", nat.stream(), n, rn, nat.ptr(d), nat.ptr(u))
         else:

@@ -32,7 +32,7 @@ import numpy as np
 from .. import _native as nat
 from ..assembler import TracedStructure, assemble_csr, residual
 from ..DCboundary import DirichletBC
-from .common import create_x0, device_norm, pcg
+from .common import create_x0, device_norm, inner_tolerance, pcg
 from .linear import SolveStats, _dev_vec, _zero_like
 from .options import DirectSolverOptions, KrylovSolverOptions, MatrixProperty, NewtonOptions, resolve_iterative_solver
 
@@ -171,7 +171,8 @@ def _mixed_direction(problem, ts, tp, bc, sol, rhs, x0, A32, dinv, lin):
     while rn > target and outer < lin.max_outer and budget > 0:
         x0i = record.project(r) if (record is not None and outer > 0) else None
         rec = record if (record is not None and outer == 0) else None
-        _, info = pcg(A32, r, x0i, lin.inner_tol, 0.0, budget, dinv, True, lin.check_every, out=d, record=rec)
+        _, info = pcg(A32, r, x0i, inner_tolerance(lin, target, rn), 0.0, budget, dinv, True, lin.check_every,
+                      out=d, record=rec)
         du += d
         jvp(problem, ts, tp, sol, du, bc, out=Kx)
         torch.sub(rhs, Kx, out=r)

@@ -13,6 +13,10 @@ KrylovSolverOptions
   spmv       "auto" | "csr" | "block3": SpMV kernel for the CG iterations.
   check_every  iterations between host reads of the convergence flag.
   inner_tol, max_outer  mixed-precision inner CG tolerance and outer cap.
+  inner_safety  mixed precision: a later inner solve stops at relative
+             tolerance max(inner_tol, inner_safety * target / ||r||), the
+             reduction the outer target still needs with a margin (capped
+             at 0.5; 0 keeps inner_tol throughout).
   recycle    mixed precision: number of search directions of the first
              inner solve kept (FP32, recycle x n in HBM) to start the later
              inner solves from the Galerkin projection of their residual
@@ -75,6 +79,7 @@ class KrylovSolverOptions(AbstractSolverOptions):
     spmv: str = "auto"
     check_every: int = 32
     inner_tol: float = 1e-5
+    inner_safety: float = 0.3
     max_outer: int = 30
     recycle: int = 3000
     operator: str = "csr"
