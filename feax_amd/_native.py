@@ -106,6 +106,10 @@ SIGNATURES = {
                                   _P, _P, _P, _P, _SZ]),
     "fx_pcg_batched_f32f64": (_I32, [_P, _I32, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32,
                                      _P, _P, _P, _P, _P, _SZ]),
+    "fx_pcg_batched_record_f32f64": (_I32, [_P, _I32, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32,
+                                            _I32, _P, _P, _P, _P, _P, _P, _I32, _P, _SZ]),
+    "fx_seg_project_workspace_bytes": (_SZ, [_I32, _P, _I32]),
+    "fx_seg_project_f32": (_I32, [_P, _I32, _P, _I32, _P, _P, _P, _P, _P, _P, _SZ]),
     "fx_seg_dot_f64": (_I32, [_P, _I32, _P, _P, _P, _P]),
     "fx_hex8_param_vjp_workspace_bytes": (_SZ, [_I32, _I32]),
     "fx_hex8_param_vjp_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P,

@@ -195,9 +195,50 @@ def _row_seg(bp, ts):
 _WS = {}
 
 
-def pcg_batched(bp, A, b, x0, dinv, tol, atol, maxiter, check_every=32, out=None, autotune=True, graph=True):
+class SegKrylovRecord:
+    """Per-instance search directions of one batched inner solve (FP32,
+    capacity x n) and their p.A p (capacity x nseg); project(r) is the
+    per-instance Galerkin projection of r onto them (fx_seg_project_f32)."""
+
+    def __init__(self, bp, n, capacity, device):
+        import torch
+        self.bp, self.n, self.capacity = bp, n, int(capacity)
+        self.P = torch.empty((self.capacity, n), dtype=torch.float32, device=device)
+        self.pq = torch.empty((self.capacity, bp.nseg), dtype=torch.float64, device=device)
+        self.counts = np.zeros(bp.nseg, np.int32)
+
+    def project(self, r):
+        import torch
+        snp = np.ascontiguousarray(self.bp.seg_node_ptr, dtype=np.int32)
+        kmax = int(self.counts.max()) if len(self.counts) else 0
+        need = int(nat.lib().fx_seg_project_workspace_bytes(self.bp.nseg, nat.ptr(snp), kmax))
+        ws = torch.empty(need, dtype=torch.uint8, device=r.device)
+        x0 = torch.empty(self.n, dtype=torch.float64, device=r.device)
+        nat.call("fx_seg_project_f32", nat.stream(), self.bp.nseg, nat.ptr(snp), kmax, nat.ptr(self.counts),
+                 nat.ptr(self.P), nat.ptr(self.pq), nat.ptr(r.to(torch.float64).contiguous()), nat.ptr(x0),
+                 nat.ptr(ws), need)
+        return x0
+
+
+_SEG_RECORDS = {}
+
+
+def _seg_record_for(bp, n, capacity, device):
+    key = (str(device), n, bp.nseg)
+    rec = _SEG_RECORDS.get(key)
+    if rec is None or rec.capacity < capacity or rec.bp is not bp:
+        _SEG_RECORDS.pop(key, None)
+        rec = SegKrylovRecord(bp, n, capacity, device)
+        _SEG_RECORDS[key] = rec
+    return rec
+
+
+def pcg_batched(bp, A, b, x0, dinv, tol, atol, maxiter, check_every=32, out=None, autotune=True, graph=True,
+                record=None):
     """Segmented Jacobi-PCG over the instances (fx_pcg_batched_f64 /
-    _f32f64).  Returns (x, iters[nseg], rs[nseg], lanes)."""
+    _f32f64; with record, an FP32-matrix solve that keeps each instance's
+    search directions in it, fx_pcg_batched_record_f32f64).  Returns (x,
+    iters[nseg], rs[nseg], lanes)."""
     import torch
     n = A.shape[0]
     dev = A.data.device
@@ -221,10 +262,17 @@ def pcg_batched(bp, A, b, x0, dinv, tol, atol, maxiter, check_every=32, out=None
     iters = np.zeros(bp.nseg, np.int32)
     rs = np.zeros(bp.nseg, np.float64)
     lanes = ctypes.c_int32(0)
-    nat.call(name, nat.stream(), bp.nseg, nat.ptr(snp), A.nse, flags, nat.ptr(A.indptr), nat.ptr(A.bcol),
-             nat.ptr(A.data), nat.ptr(dinv), nat.ptr(b), nat.ptr(x0), float(tol), float(atol), int(maxiter),
-             int(check_every), nat.ptr(x), nat.ptr(iters), nat.ptr(rs), ctypes.addressof(lanes), nat.ptr(ws),
-             ws.numel())
+    common = (nat.stream(), bp.nseg, nat.ptr(snp), A.nse, flags, nat.ptr(A.indptr), nat.ptr(A.bcol),
+              nat.ptr(A.data), nat.ptr(dinv), nat.ptr(b), nat.ptr(x0), float(tol), float(atol), int(maxiter),
+              int(check_every), nat.ptr(x), nat.ptr(iters), nat.ptr(rs), ctypes.addressof(lanes))
+    if record is not None:
+        if A.data.dtype != torch.float32:
+            raise ValueError("recording is for the FP32-matrix batched solve")
+        nat.call("fx_pcg_batched_record_f32f64", *common, nat.ptr(record.P), nat.ptr(record.pq), record.capacity,
+                 nat.ptr(ws), ws.numel())
+        record.counts = np.minimum(iters, record.capacity).astype(np.int32)
+    else:
+        nat.call(name, *common, nat.ptr(ws), ws.numel())
     return x, iters, rs, lanes.value
 
 
@@ -237,7 +285,7 @@ def create_batched_solver(bp, bc, solver_options=None, traced_structure=None, au
     norms decide which instances take another inner solve."""
     import torch
     from .assembler import TracedStructure, assemble_csr, residual
-    from .solvers.common import create_x0
+    from .solvers.common import create_x0, inner_tolerance
     from .solvers.options import KrylovSolverOptions
     from .solvers.linear import _dev_vec, _zero_like
     opts = solver_options or KrylovSolverOptions(solver="cg", use_jacobi_preconditioner=True)
@@ -271,11 +319,19 @@ def create_batched_solver(bp, bc, solver_options=None, traced_structure=None, au
         total = np.zeros(bp.nseg, np.int64)
         d = torch.empty_like(u)
         outer = 0
+        # Krylov recycling per instance: the first inner solve records its
+        # directions, later ones start from the projection onto them.
+        record = _seg_record_for(bp, ts.n, min(opts.recycle, opts.maxiter), ts.device) if opts.recycle > 0 else None
         while np.any(rn > target) and outer < opts.max_outer:
-            active = torch.as_tensor((rn > target).astype(np.float64), device=ts.device)
+            act = rn > target
+            active = torch.as_tensor(act.astype(np.float64), device=ts.device)
             rhs = r * active[row_seg]
-            _, iters, _, lanes = pcg_batched(bp, A, rhs, None, dinv, opts.inner_tol, 0.0, opts.maxiter,
-                                             opts.check_every, out=d, autotune=autotune, graph=graph)
+            # one tolerance for all instances: the strictest need among the active ones
+            tin = min(inner_tolerance(opts, t_, r_) for t_, r_ in zip(target[act], rn[act]))
+            x0i = record.project(rhs) if (record is not None and outer > 0) else None
+            _, iters, _, lanes = pcg_batched(bp, A, rhs, x0i, dinv, tin, 0.0, opts.maxiter, opts.check_every,
+                                             out=d, autotune=autotune, graph=graph,
+                                             record=record if outer == 0 else None)
             u += d
             residual(bp, ts, traced_params, u, bc, negate=True, out=r)
             rn = _seg_norms(bp, ts, r)
@@ -291,4 +347,5 @@ def create_batched_solver(bp, bc, solver_options=None, traced_structure=None, au
     return solver
 
 
-__all__ = ["BatchedProblem", "batch_bc", "batch_params", "create_batched_solver", "pcg_batched", "BatchedStats"]
+__all__ = ["BatchedProblem", "batch_bc", "batch_params", "create_batched_solver", "pcg_batched", "BatchedStats",
+           "SegKrylovRecord"]

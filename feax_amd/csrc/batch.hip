@@ -18,7 +18,11 @@
 //    replayed on the caller's stream until all instances are done (one host
 //    read of the per-instance flags per chunk);
 //  * FX_PCG_AUTOTUNE times the SpMV lane counts 8 / 16 / 32 on the
-//    instance set and keeps the fastest per problem shape.
+//    instance set and keeps the fastest per problem shape;
+//  * optionally each instance's search directions p_k (FP32) and p_k.A p_k
+//    are recorded at its own iteration count k, and fx_seg_project_f32
+//    forms the per-instance Galerkin projection onto them (Krylov recycling
+//    for the later inner solves of a mixed-precision defect correction).
 // FP64 partial sums in a fixed order: bitwise repeatable, with or without
 // the graph.
 #include <algorithm>
@@ -90,13 +94,24 @@ __global__ __launch_bounds__(kSegBlock) void seg_spmv_kernel(const BlockRange* _
   }
 }
 
+// Recording of search directions: P[k n + i] = p_k[i] (FP32) for the rows
+// of an instance at its iteration k < rec_max, pq[k nseg + s] = p_k.A p_k.
+struct SegRecord {
+  float* P;
+  double* pq;
+  int rec_max, n, nseg;
+};
+
 __global__ __launch_bounds__(kFinBlock) void seg_fin_alpha(PcgState* st, const int* __restrict__ mv_ptr,
-                                                           const double* __restrict__ partials) {
+                                                           const double* __restrict__ partials, SegRecord rec) {
   const int s = blockIdx.x;
   if (st[s].done) return;
   const double* src[1] = {partials + mv_ptr[s]};
   const DVec<1> pq = seg_sums<1>(src, mv_ptr[s + 1] - mv_ptr[s]);
-  if (threadIdx.x == 0) st[s].alpha = st[s].gamma / pq.v[0];
+  if (threadIdx.x == 0) {
+    st[s].alpha = st[s].gamma / pq.v[0];
+    if (rec.pq && st[s].k < rec.rec_max) rec.pq[static_cast<size_t>(st[s].k) * rec.nseg + s] = pq.v[0];
+  }
 }
 
 // x += alpha p; r -= alpha q; z = M r; partials of r.z and r.r.
@@ -106,13 +121,16 @@ __global__ __launch_bounds__(kSegBlock) void seg_update_kernel(const BlockRange*
                                                                const T* __restrict__ p, const T* __restrict__ q,
                                                                const T* __restrict__ dinv,
                                                                const PcgState* __restrict__ st, double* prz,
-                                                               double* prr) {
+                                                               double* prr, SegRecord rec) {
   const BlockRange br = vtab[blockIdx.x];
   if (st[br.seg].done) return;
   const T alpha = static_cast<T>(st[br.seg].alpha);
+  const int k = st[br.seg].k;
+  float* rp = (rec.P && k < rec.rec_max) ? rec.P + static_cast<size_t>(k) * rec.n : nullptr;
   DVec<2> acc;
   acc.v[0] = acc.v[1] = 0.0;
   for (int i = br.begin + threadIdx.x; i < br.end; i += kSegBlock) {
+    if (rp) rp[i] = static_cast<float>(p[i]);
     x[i] += alpha * p[i];
     const T ri = r[i] - alpha * q[i];
     r[i] = ri;
@@ -222,6 +240,62 @@ __global__ __launch_bounds__(kFinBlock) void seg_dot_kernel(const int* __restric
   if (threadIdx.x == 0) out[s] = t.v[0];
 }
 
+// Per-instance Galerkin projection onto recorded directions:
+//   x0[i] = sum_{k < cnt[s]} c[k, s] P[k, i],  c[k, s] = (p_k . r)_s / pq[k, s]
+// for the rows i of instance s.  Dots per (vector block, k), then a fixed
+// order sum over the instance's blocks.
+__global__ __launch_bounds__(kSegBlock) void seg_proj_dot_kernel(const BlockRange* __restrict__ vtab, int n, int nv,
+                                                                 const int* __restrict__ cnt,
+                                                                 const float* __restrict__ P,
+                                                                 const double* __restrict__ r, double* partials) {
+  const BlockRange br = vtab[blockIdx.x];
+  const int k = blockIdx.y;
+  DVec<1> acc;
+  acc.v[0] = 0.0;
+  if (k < cnt[br.seg]) {
+    const float* pk = P + static_cast<size_t>(k) * n;
+    for (int i = br.begin + threadIdx.x; i < br.end; i += kSegBlock)
+      acc.v[0] += static_cast<double>(pk[i]) * r[i];
+  }
+  const DVec<1> t = block_sum<1, kSegBlock>(acc);
+  if (threadIdx.x == 0) partials[static_cast<size_t>(k) * nv + blockIdx.x] = t.v[0];
+}
+
+__global__ __launch_bounds__(kSegBlock) void seg_proj_coef_kernel(int nseg, int kmax, int nv,
+                                                                  const int* __restrict__ vptr,
+                                                                  const int* __restrict__ cnt,
+                                                                  const double* __restrict__ partials,
+                                                                  const double* __restrict__ pq,
+                                                                  double* __restrict__ c) {
+  const long idx = static_cast<long>(blockIdx.x) * kSegBlock + threadIdx.x;
+  if (idx >= static_cast<long>(kmax) * nseg) return;
+  const int k = static_cast<int>(idx / nseg), s = static_cast<int>(idx % nseg);
+  double v = 0.0;
+  if (k < cnt[s]) {
+    const double* src = partials + static_cast<size_t>(k) * nv;
+    double d = 0.0;
+    for (int b = vptr[s]; b < vptr[s + 1]; ++b) d += src[b];
+    const double den = pq[static_cast<size_t>(k) * nseg + s];
+    v = den != 0.0 ? d / den : 0.0;
+  }
+  c[idx] = v;
+}
+
+__global__ __launch_bounds__(kSegBlock) void seg_proj_combine_kernel(const BlockRange* __restrict__ vtab, int n,
+                                                                     int nseg, const int* __restrict__ cnt,
+                                                                     const float* __restrict__ P,
+                                                                     const double* __restrict__ c,
+                                                                     double* __restrict__ x0) {
+  const BlockRange br = vtab[blockIdx.x];
+  const int K = cnt[br.seg];
+  for (int i = br.begin + threadIdx.x; i < br.end; i += kSegBlock) {
+    double v = 0.0;
+    for (int k = 0; k < K; ++k)
+      v += c[static_cast<size_t>(k) * nseg + br.seg] * static_cast<double>(P[static_cast<size_t>(k) * n + i]);
+    x0[i] = v;
+  }
+}
+
 // ----------------------------------------------------------------- host side
 inline size_t al256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
@@ -325,14 +399,15 @@ struct SegBuffers {
   int *mvptr, *vptr;
   double *pmv, *prz, *prr, *pbb;
   int nseg, nv;
+  SegRecord rec;
 };
 
 template <typename T, typename Op>
 void seg_iteration(hipStream_t s, const Op& op, const SegBuffers& B, const T* dinv, T* x, T* r, T* z, T* p, T* q) {
   op.iter(s, p, q, B.st, B.pmv);
-  hipLaunchKernelGGL(seg_fin_alpha, dim3(B.nseg), dim3(kFinBlock), 0, s, B.st, B.mvptr, B.pmv);
+  hipLaunchKernelGGL(seg_fin_alpha, dim3(B.nseg), dim3(kFinBlock), 0, s, B.st, B.mvptr, B.pmv, B.rec);
   hipLaunchKernelGGL((seg_update_kernel<T>), dim3(B.nv), dim3(kSegBlock), 0, s, B.vtab, x, r, z, p, q, dinv, B.st,
-                     B.prz, B.prr);
+                     B.prz, B.prr, B.rec);
   hipLaunchKernelGGL(seg_fin_beta, dim3(B.nseg), dim3(kFinBlock), 0, s, B.st, B.vptr, B.prz, B.prr);
   hipLaunchKernelGGL((seg_dir_kernel<T>), dim3(B.nv), dim3(kSegBlock), 0, s, B.vtab, p, z, B.st);
 }
@@ -476,10 +551,12 @@ template <typename V, typename T, bool DIFF>
 int run_batched(void* stream, int nseg, const int* seg_node_ptr, long long nse, int flags, const int* indptr,
                 const int* bcol, const V* data, const T* dinv, const T* b, const T* x0, double tol, double atol,
                 int maxiter, int check_every, T* x, int* iters, double* rs, int* lanes_used, void* workspace,
-                size_t ws_bytes) {
+                size_t ws_bytes, SegRecord rec = SegRecord{nullptr, nullptr, 0, 0, 0}) {
   hipStream_t s = as_stream(stream);
   if (nseg <= 0 || !seg_node_ptr || !indptr || !bcol || !data || !b || !x || !iters || !rs)
     return fail(FX_EINVAL, "pcg_batched: bad arguments");
+  if ((rec.P != nullptr) != (rec.pq != nullptr) || (rec.P && rec.rec_max <= 0))
+    return fail(FX_EINVAL, "pcg_batched: a record needs P, pq and rec_max > 0");
   if (maxiter < 0) return fail(FX_EINVAL, "pcg_batched: maxiter < 0");
   for (int i = 0; i < nseg; ++i)
     if (seg_node_ptr[i + 1] < seg_node_ptr[i] || seg_node_ptr[0] != 0)
@@ -499,6 +576,9 @@ int run_batched(void* stream, int nseg, const int* seg_node_ptr, long long nse, 
   B.prr = B.prz + Lay.nv;
   B.pbb = B.prr + Lay.nv;
   B.nseg = nseg;
+  B.rec = rec;
+  B.rec.n = 3 * seg_node_ptr[nseg];
+  B.rec.nseg = nseg;
   T* r = reinterpret_cast<T*>(w + Lay.off_vec);
   T* z = reinterpret_cast<T*>(w + Lay.off_vec + Lay.vec_bytes);
   T* p = reinterpret_cast<T*>(w + Lay.off_vec + 2 * Lay.vec_bytes);
@@ -575,6 +655,74 @@ int fx_pcg_batched_f32f64(void* stream, int32_t nseg, const int32_t* seg_node_pt
   return run_batched<float, double, false>(stream, nseg, seg_node_ptr, nse, flags, indptr, bcol, data, dinv, b, x0,
                                            tol, atol, maxiter, check_every, x, iters, rs, lanes_used, workspace,
                                            ws_bytes);
+}
+
+int fx_pcg_batched_record_f32f64(void* stream, int32_t nseg, const int32_t* seg_node_ptr, int64_t nse,
+                                 int32_t flags, const int32_t* indptr, const int32_t* bcol, const float* data,
+                                 const double* dinv, const double* b, const double* x0, double tol, double atol,
+                                 int32_t maxiter, int32_t check_every, double* x, int32_t* iters, double* rs,
+                                 int32_t* lanes_used, float* rec_p, double* rec_pq, int32_t rec_max,
+                                 void* workspace, size_t ws_bytes) {
+  if (!rec_p || !rec_pq || rec_max <= 0) return fail(FX_EINVAL, "fx_pcg_batched_record_f32f64: bad record");
+  const SegRecord rec{rec_p, rec_pq, rec_max, 0, 0};
+  if (flags & FX_PCG_DIFF3)
+    return run_batched<float, double, true>(stream, nseg, seg_node_ptr, nse, flags, indptr, bcol, data, dinv, b, x0,
+                                            tol, atol, maxiter, check_every, x, iters, rs, lanes_used, workspace,
+                                            ws_bytes, rec);
+  return run_batched<float, double, false>(stream, nseg, seg_node_ptr, nse, flags, indptr, bcol, data, dinv, b, x0,
+                                           tol, atol, maxiter, check_every, x, iters, rs, lanes_used, workspace,
+                                           ws_bytes, rec);
+}
+
+size_t fx_seg_project_workspace_bytes(int32_t nseg, const int32_t* seg_node_ptr, int32_t kmax) {
+  if (nseg <= 0 || !seg_node_ptr || kmax < 0) return 0;
+  const size_t nv = static_cast<size_t>(v_blocks(seg_node_ptr, nseg));
+  return al256(nv * sizeof(BlockRange)) + 2 * al256((static_cast<size_t>(nseg) + 1) * sizeof(int)) +
+         al256(static_cast<size_t>(std::max(kmax, 1)) * nv * sizeof(double)) +
+         al256(static_cast<size_t>(std::max(kmax, 1)) * nseg * sizeof(double));
+}
+
+int fx_seg_project_f32(void* stream, int32_t nseg, const int32_t* seg_node_ptr, int32_t kmax, const int32_t* counts,
+                       const float* P, const double* pq, const double* r, double* x0, void* workspace,
+                       size_t ws_bytes) {
+  if (nseg <= 0 || !seg_node_ptr || kmax < 0 || !counts || !r || !x0 || (kmax > 0 && (!P || !pq)))
+    return fail(FX_EINVAL, "fx_seg_project_f32: bad arguments");
+  for (int s = 0; s < nseg; ++s)
+    if (counts[s] < 0 || counts[s] > kmax || seg_node_ptr[s + 1] < seg_node_ptr[s] || seg_node_ptr[0] != 0)
+      return fail(FX_EINVAL, "fx_seg_project_f32: counts must lie in [0, kmax], seg_node_ptr non-decreasing from 0");
+  if (kmax > 65535) return fail(FX_ELIMIT, "fx_seg_project_f32: kmax %d > 65535", kmax);
+  if (!workspace || ws_bytes < fx_seg_project_workspace_bytes(nseg, seg_node_ptr, kmax))
+    return fail(FX_EWORKSPACE, "fx_seg_project_f32: workspace too small");
+  hipStream_t s = as_stream(stream);
+  std::vector<BlockRange> vtab;
+  std::vector<int> vptr, rowptr;
+  v_table(nseg, seg_node_ptr, vtab, vptr, rowptr);
+  const int nv = static_cast<int>(vtab.size());
+  const int n = 3 * seg_node_ptr[nseg];
+  char* w = static_cast<char*>(workspace);
+  BlockRange* dvtab = reinterpret_cast<BlockRange*>(w);
+  w += al256(static_cast<size_t>(nv) * sizeof(BlockRange));
+  int* dvptr = reinterpret_cast<int*>(w);
+  w += al256((static_cast<size_t>(nseg) + 1) * sizeof(int));
+  int* dcnt = reinterpret_cast<int*>(w);
+  w += al256((static_cast<size_t>(nseg) + 1) * sizeof(int));
+  double* partials = reinterpret_cast<double*>(w);
+  w += al256(static_cast<size_t>(std::max(kmax, 1)) * nv * sizeof(double));
+  double* c = reinterpret_cast<double*>(w);
+  FX_HIP(hipMemcpyAsync(dvtab, vtab.data(), vtab.size() * sizeof(BlockRange), hipMemcpyHostToDevice, s));
+  FX_HIP(hipMemcpyAsync(dvptr, vptr.data(), vptr.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  FX_HIP(hipMemcpyAsync(dcnt, counts, static_cast<size_t>(nseg) * sizeof(int), hipMemcpyHostToDevice, s));
+  if (kmax > 0) {
+    hipLaunchKernelGGL(seg_proj_dot_kernel, dim3(nv, kmax), dim3(kSegBlock), 0, s, dvtab, n, nv, dcnt, P, r,
+                       partials);
+    const long nc = static_cast<long>(kmax) * nseg;
+    hipLaunchKernelGGL(seg_proj_coef_kernel, dim3(static_cast<unsigned>((nc + kSegBlock - 1) / kSegBlock)),
+                       dim3(kSegBlock), 0, s, nseg, kmax, nv, dvptr, dcnt, partials, pq, c);
+  }
+  hipLaunchKernelGGL(seg_proj_combine_kernel, dim3(nv), dim3(kSegBlock), 0, s, dvtab, n, nseg, dcnt, P, c, x0);
+  FX_LAUNCHED("seg_proj kernels");
+  FX_HIP(hipStreamSynchronize(s));  // the host tables above go out of scope
+  return FX_OK;
 }
 
 int fx_seg_dot_f64(void* stream, int32_t nseg, const int32_t* row_ptr, const double* a, const double* b,

@@ -414,6 +414,30 @@ int fx_pcg_batched_f32f64(void* stream, int32_t nseg, const int32_t* seg_node_pt
                           const double* b, const double* x0, double tol, double atol, int32_t maxiter,
                           int32_t check_every, double* x, int32_t* iters, double* rs, int32_t* lanes_used,
                           void* workspace, size_t ws_bytes);
+/* fx_pcg_batched_record_f32f64: fx_pcg_batched_f32f64 that also records
+ * each instance's search directions at its own iteration count k <
+ * rec_max: rec_p[k n + i] = p_k[i] (FP32, n = 3 seg_node_ptr[nseg]) for the
+ * instance's rows, rec_pq[k nseg + s] = p_k . A p_k (device arrays of
+ * rec_max x n and rec_max x nseg).  Instance s holds min(iters[s], rec_max)
+ * directions afterwards.  No reference counterpart (feax has no mixed
+ * precision): Krylov recycling for the later inner solves of the batched
+ * defect correction. */
+int fx_pcg_batched_record_f32f64(void* stream, int32_t nseg, const int32_t* seg_node_ptr, int64_t nse,
+                                 int32_t flags, const int32_t* indptr, const int32_t* bcol, const float* data,
+                                 const double* dinv, const double* b, const double* x0, double tol, double atol,
+                                 int32_t maxiter, int32_t check_every, double* x, int32_t* iters, double* rs,
+                                 int32_t* lanes_used, float* rec_p, double* rec_pq, int32_t rec_max,
+                                 void* workspace, size_t ws_bytes);
+/* x0 = per-instance Galerkin projection of r onto the recorded directions:
+ * x0[i] = sum_{k < counts[s]} (p_k . r)_s / rec_pq[k nseg + s] p_k[i] for
+ * the rows i of instance s (counts: HOST array, each in [0, kmax]; P, pq as
+ * recorded, with at least kmax directions).  Workspace:
+ * fx_seg_project_workspace_bytes(nseg, seg_node_ptr, kmax).  Synchronises
+ * the stream before returning. */
+size_t fx_seg_project_workspace_bytes(int32_t nseg, const int32_t* seg_node_ptr, int32_t kmax);
+int fx_seg_project_f32(void* stream, int32_t nseg, const int32_t* seg_node_ptr, int32_t kmax, const int32_t* counts,
+                       const float* P, const double* pq, const double* r, double* x0, void* workspace,
+                       size_t ws_bytes);
 /* out[s] = sum over the rows row_ptr[s] .. row_ptr[s+1] of a_i b_i
  * (row_ptr: DEVICE array, nseg+1 entries; out: device, nseg). */
 int fx_seg_dot_f64(void* stream, int32_t nseg, const int32_t* row_ptr, const double* a, const double* b,

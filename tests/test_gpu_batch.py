@@ -117,3 +117,73 @@ def test_same_mesh_parameter_batch():
     for ui, r in zip(bp.split(u), rhos):
         ref = _oracle_solution(p, bc, r, t)
         assert np.linalg.norm(np.asarray(ui.cpu()) - ref) / np.linalg.norm(ref) < 1e-8
+
+
+def test_recorded_directions_reproduce_each_instance_cg_solution(batch):
+    """The Galerkin projection of b onto the directions recorded while CG
+    solved A x = b is that CG solution (per instance; FP32 storage of the
+    directions limits the agreement), and instances whose right-hand side is
+    zero project to zero.  A is the plain FP32 matrix (exactly symmetric);
+    the difference form's diagonal blocks carry an antisymmetric rounding
+    part of order eps32 |K|, and CG run deep (rel 1e-10) on small instances
+    loses the global conjugacy this identity needs."""
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import assemble_csr, residual
+    from feax_amd.batch import SegKrylovRecord, pcg_batched
+    bp, bc, tp = batch["bp"], batch["bc"], batch["tp"]
+    ts = fe.TracedStructure.from_problem(bp)
+    tpd = fe.TracedParams(volume_vars=(torch.as_tensor(tp.volume_vars[0], device="cuda"),),
+                          surface_vars=[(torch.as_tensor(tp.surface_vars[0][0], device="cuda"),)])
+    A, dinv = assemble_csr(bp, ts, tpd, None, bc, True, torch.float32, jacobi_shift=1e-12)
+    b = residual(bp, ts, tpd, torch.zeros(ts.n, dtype=torch.float64, device="cuda"), bc, negate=True)
+    cap = 1000
+    rec = SegKrylovRecord(bp, ts.n, cap, "cuda")
+    print("batched recording solve", flush=True)
+    x, iters, _, _ = pcg_batched(bp, A, b, None, dinv, 1e-10, 0.0, 5000, record=rec)
+    assert np.array_equal(rec.counts, np.minimum(iters, cap)) and rec.counts.min() > 0 and iters.max() <= cap
+    x0 = rec.project(b)
+    P, pq, bn = rec.P.cpu().numpy().astype(np.float64), rec.pq.cpu().numpy(), b.cpu().numpy()
+    rows = np.concatenate([[0], np.cumsum(3 * np.diff(bp.seg_node_ptr))])
+    for s, (xs, ps) in enumerate(zip(bp.split(x), bp.split(x0))):
+        # the kernels against the same projection evaluated here from the record
+        K, sl = int(rec.counts[s]), slice(rows[s], rows[s + 1])
+        want = ((P[:K, sl] @ bn[sl]) / pq[:K, s]) @ P[:K, sl]
+        got = ps.cpu().numpy()
+        assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+        # exact arithmetic: the projection is the CG solution; here FP32
+        # directions and finite-precision conjugacy leave ~1e-4
+        err = float((ps - xs).norm() / xs.norm())
+        print("instance", s, "iters", int(iters[s]), "projection vs CG solution", err)
+        assert err < 1e-3
+    # zero right-hand side on one instance -> zero projection there
+    row_seg = np.repeat(np.arange(bp.nseg), 3 * np.diff(bp.seg_node_ptr))
+    mask = torch.as_tensor((row_seg != 2).astype(np.float64), device="cuda")
+    x2 = rec.project(b * mask)
+    assert float(bp.split(x2)[2].abs().max()) == 0.0
+    assert float((bp.split(x2)[0] - bp.split(x0)[0]).abs().max()) == 0.0
+
+
+def test_batched_mixed_recycling_cuts_inner_iterations(batch):
+    """The mixed batched solve with per-instance recycling agrees with the
+    one without and takes fewer inner iterations after the first outer
+    step."""
+    import torch
+    import feax_amd as fe
+    from feax_amd.batch import create_batched_solver
+    bp, bc, tp = batch["bp"], batch["bc"], batch["tp"]
+    tpd = fe.TracedParams(volume_vars=(torch.as_tensor(tp.volume_vars[0], device="cuda"),),
+                          surface_vars=[(torch.as_tensor(tp.surface_vars[0][0], device="cuda"),)])
+    out = {}
+    for rec in (0, 3000):
+        opts = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=0.0, maxiter=20000, use_jacobi_preconditioner=True,
+                                      precision="mixed", recycle=rec)
+        s = create_batched_solver(bp, bc, opts)
+        print("batched mixed, recycle", rec, flush=True)
+        u = s(tpd)
+        out[rec] = (u, list(s.stats.iters), s.stats.outer, max(s.stats.relres))
+        print("recycle", rec, "iters", out[rec][1], "outer", out[rec][2], "relres", out[rec][3])
+        assert out[rec][3] <= 1e-10
+    errs = _errors(bp, out[3000][0], batch["ref"])
+    assert max(errs) < 1e-8
+    assert sum(out[3000][1]) < sum(out[0][1])
