@@ -347,5 +347,52 @@ def create_batched_solver(bp, bc, solver_options=None, traced_structure=None, au
     return solver
 
 
+def create_vmap_solver(problem, bc, solver_options=None, autotune=True, graph=True):
+    """solve(traced_params, bc_vals=None) -> (B, n) FP64 tensor: the
+    jax.vmap(solver) use of feax (tests/test_vmap.py:63-106,
+    tests/test_bc_vmap.py) on one HEX8 mesh.  Every tensor / array of
+    traced_params carries a leading batch axis B; bc_vals, when given, is
+    (B, len(bc.bc_rows)) (one set of Dirichlet values per lane).  The lanes
+    are solved together by the batched engine (one assembly launch, one
+    segmented PCG; per-lane stopping tests as in a vmapped jax cg)."""
+    import torch
+    from .solvers.options import KrylovSolverOptions
+    opts = solver_options or KrylovSolverOptions(solver="cg", use_jacobi_preconditioner=True)
+    cache = {}
+
+    def lane(x, i):
+        return x[i] if x is not None else None
+
+    def solve(traced_params, bc_vals=None):
+        leaves = list(traced_params.volume_vars) + [v for sv in traced_params.surface_vars for v in sv]
+        sizes = {int(v.shape[0]) for v in leaves if hasattr(v, "shape") and len(v.shape)}
+        if bc_vals is not None:
+            sizes.add(int(bc_vals.shape[0]))
+        if len(sizes) != 1:
+            raise ValueError(f"every batched input needs the same leading axis (got {sorted(sizes)})")
+        B = sizes.pop()
+        if B not in cache:
+            from .assembler import TracedStructure
+            bp = BatchedProblem([problem] * B)
+            cache[B] = (bp, TracedStructure.from_problem(bp))
+        bp, ts = cache[B]
+        vals = bc_vals.detach().cpu().numpy() if isinstance(bc_vals, torch.Tensor) else bc_vals
+        bcs = [bc if vals is None else bc.replace_vals(np.asarray(vals[i], np.float64)) for i in range(B)]
+        tps = [TracedParams(volume_vars=tuple(lane(v, i) for v in traced_params.volume_vars),
+                            surface_vars=[tuple(lane(v, i) for v in sv) for sv in traced_params.surface_vars])
+               for i in range(B)]
+        solver = create_batched_solver(bp, batch_bc(bp, bcs), opts, traced_structure=ts, autotune=autotune,
+                                       graph=graph)
+        tpb = batch_params(bp, tps)
+        tpd = TracedParams(volume_vars=tuple(torch.as_tensor(v, device=ts.device) for v in tpb.volume_vars),
+                           surface_vars=[tuple(torch.as_tensor(v, device=ts.device) for v in sv)
+                                         for sv in tpb.surface_vars])
+        u = solver(tpd)
+        solve.stats = solver.stats
+        return u.reshape(B, -1)
+
+    return solve
+
+
 __all__ = ["BatchedProblem", "batch_bc", "batch_params", "create_batched_solver", "pcg_batched", "BatchedStats",
-           "SegKrylovRecord"]
+           "SegKrylovRecord", "create_vmap_solver"]

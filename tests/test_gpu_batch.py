@@ -187,3 +187,33 @@ def test_batched_mixed_recycling_cuts_inner_iterations(batch):
     errs = _errors(bp, out[3000][0], batch["ref"])
     assert max(errs) < 1e-8
     assert sum(out[3000][1]) < sum(out[0][1])
+
+
+def test_vmap_solver_matches_one_solve_per_lane():
+    """create_vmap_solver over a leading batch axis of the traced variables
+    and of the Dirichlet values (the vmap uses of the reference's
+    tests/test_vmap.py and tests/test_bc_vmap.py) equals one create_solver
+    call per lane."""
+    import torch
+    import feax_amd as fe
+    from feax_amd.batch import create_vmap_solver
+    rng = np.random.default_rng(11)
+    p, bc, tp, rho, t = _simp_instance((6, 3, 3), rng)
+    B = 3
+    rhos = rng.uniform(0.3, 1.0, (B, p.num_cells))
+    ts_ = np.stack([fe.TracedParams.create_uniform_surface_var(p, 10.0 ** rng.uniform(-3, -2)) for _ in range(B)])
+    vals = np.stack([bc.vals_numpy() + 0.01 * (i + 1) * (np.asarray(bc.bc_rows) % 3 == 2) for i in range(B)])
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=0.0, maxiter=20000, use_jacobi_preconditioner=True)
+    solve = create_vmap_solver(p, bc, opts)
+    print("vmap solve", flush=True)
+    U = solve(fe.TracedParams(volume_vars=(torch.as_tensor(rhos, device="cuda"),),
+                              surface_vars=[(torch.as_tensor(ts_, device="cuda"),)]), bc_vals=vals)
+    assert tuple(U.shape) == (B, p.num_total_dofs_all_vars)
+    single = fe.create_solver(p, bc, opts, linear=True, return_solution=False)
+    for i in range(B):
+        bci = bc.replace_vals(vals[i])
+        ui = single(fe.TracedParams(volume_vars=(rhos[i],), surface_vars=[(ts_[i],)]),
+                    fe.zero_like_initial_guess(p, bci), bc=bci)
+        err = float((U[i] - ui).norm() / ui.norm())
+        print("lane", i, "vs single solve", err)
+        assert err < 1e-8
