@@ -363,7 +363,7 @@ def create_vmap_solver(problem, bc, solver_options=None, autotune=True, graph=Tr
     def lane(x, i):
         return x[i] if x is not None else None
 
-    def solve(traced_params, bc_vals=None):
+    def run(traced_params, bc_vals, keep=None):
         leaves = list(traced_params.volume_vars) + [v for sv in traced_params.surface_vars for v in sv]
         sizes = {int(v.shape[0]) for v in leaves if hasattr(v, "shape") and len(v.shape)}
         if bc_vals is not None:
@@ -389,9 +389,65 @@ def create_vmap_solver(problem, bc, solver_options=None, autotune=True, graph=Tr
                                          for sv in tpb.surface_vars])
         u = solver(tpd)
         solve.stats = solver.stats
+        if keep is not None:
+            keep.update(bp=bp, ts=ts, bc=batch_bc(bp, bcs), tp=tpd)
         return u.reshape(B, -1)
 
+    def solve(traced_params, bc_vals=None):
+        from .solvers.adjoint import _leaves, _needs_grad
+        if not _needs_grad(traced_params):
+            return run(traced_params, bc_vals)
+        leaves = _leaves(traced_params)
+        return _vmap_function().apply(dict(run=run, tp=traced_params, bc_vals=bc_vals, opts=opts,
+                                           paths=[q for q, _ in leaves]), *[t for _, t in leaves])
+
     return solve
+
+
+_VMAP_FN = None
+
+
+def _vmap_function():
+    """autograd of the vmapped linear solve: one batched FP64 adjoint solve
+    lambda = J_bc^-1 V (symmetric elimination, lambda[bc] = 0) on the stacked
+    problem, then the parameter and traction pullbacks of the stacked
+    problem, whose cells and loaded faces are the lanes' in lane order."""
+    global _VMAP_FN
+    if _VMAP_FN is not None:
+        return _VMAP_FN
+    import torch
+    from .solvers.adjoint import _rebuild, parameter_gradients
+
+    class VmapSolve(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, call, *tensors):
+            tp = _rebuild(call["tp"], call["paths"], [t.detach() for t in tensors])
+            keep = {}
+            U = call["run"](tp, call["bc_vals"], keep=keep)
+            ctx.call, ctx.keep = call, keep
+            ctx.shapes = [tuple(t.shape) for t in tensors]
+            ctx.meta = [(t.dtype, t.device) for t in tensors]
+            ctx.save_for_backward(U)
+            return U
+
+        @staticmethod
+        def backward(ctx, V):
+            from .assembler import assemble_csr
+            (U,) = ctx.saved_tensors
+            k, opts = ctx.keep, ctx.call["opts"]
+            bp, ts, bcb, tpd = k["bp"], k["ts"], k["bc"], k["tp"]
+            J, dinv = assemble_csr(bp, ts, tpd, None, bcb, True, torch.float64, jacobi_shift=opts.jacobi_shift)
+            rhs = V.detach().to(device=ts.device, dtype=torch.float64).reshape(-1).clone()
+            if len(bcb.bc_rows):
+                rhs.index_fill_(0, bcb.device(ts.device)[0].long(), 0.0)
+            lam, _, _, _ = pcg_batched(bp, J, rhs, None, dinv if opts.use_jacobi_preconditioner else None,
+                                       opts.tol, opts.atol, opts.maxiter, opts.check_every)
+            grads = parameter_gradients(bp, ts, tpd, bcb, U.reshape(-1), lam, ctx.call["paths"], ctx.shapes,
+                                        ctx.needs_input_grad[1:])
+            return (None, *[g if g is None else g.to(device=d, dtype=dt) for g, (dt, d) in zip(grads, ctx.meta)])
+
+    _VMAP_FN = VmapSolve
+    return _VMAP_FN
 
 
 __all__ = ["BatchedProblem", "batch_bc", "batch_params", "create_batched_solver", "pcg_batched", "BatchedStats",

@@ -217,3 +217,36 @@ def test_vmap_solver_matches_one_solve_per_lane():
         err = float((U[i] - ui).norm() / ui.norm())
         print("lane", i, "vs single solve", err)
         assert err < 1e-8
+
+
+def test_vmap_solver_gradients_match_per_lane_autograd():
+    """grad of sum_i w_i |u_i|^2 through the vmapped solve (the reference's
+    vmap-of-grad composition, tests/test_vmap.py:243-290) equals each lane's
+    gradient through its own create_solver call, for the cell densities
+    (B, E) and the per-face-point tractions (B, F, 4)."""
+    import torch
+    import feax_amd as fe
+    from feax_amd.batch import create_vmap_solver
+    rng = np.random.default_rng(12)
+    p, bc, tp, rho, t = _simp_instance((6, 3, 3), rng)
+    B = 3
+    rhos = rng.uniform(0.3, 1.0, (B, p.num_cells))
+    ts_ = np.stack([fe.TracedParams.create_uniform_surface_var(p, 10.0 ** rng.uniform(-3, -2)) for _ in range(B)])
+    ts_ = ts_ * rng.uniform(0.5, 1.5, ts_.shape)
+    w = torch.tensor([1.0, 2.0, 0.5], dtype=torch.float64, device="cuda")
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0, maxiter=20000, use_jacobi_preconditioner=True)
+    r = torch.tensor(rhos, device="cuda", requires_grad=True)
+    s = torch.tensor(ts_, device="cuda", requires_grad=True)
+    print("vmap gradient", flush=True)
+    U = create_vmap_solver(p, bc, opts)(fe.TracedParams(volume_vars=(r,), surface_vars=[(s,)]))
+    (w * (U * U).sum(dim=1)).sum().backward()
+    single = fe.create_solver(p, bc, opts, linear=True, return_solution=False)
+    for i in range(B):
+        ri = torch.tensor(rhos[i], device="cuda", requires_grad=True)
+        si = torch.tensor(ts_[i], device="cuda", requires_grad=True)
+        ui = single(fe.TracedParams(volume_vars=(ri,), surface_vars=[(si,)]))
+        (w[i] * (ui * ui).sum()).backward()
+        er = float((r.grad[i] - ri.grad).norm() / ri.grad.norm())
+        es = float((s.grad[i] - si.grad).norm() / si.grad.norm())
+        print("lane", i, "rho grad", er, "traction grad", es)
+        assert er < 1e-8 and es < 1e-8
