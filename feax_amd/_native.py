@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libfeaxhip.so")
 FX_MAT_LINEAR_ELASTIC = 0
 FX_MAT_SIMP = 1
 FX_MAT_NEO_HOOKEAN = 2
+FX_MAT_CONDUCTION = 3
+FX_MAT_SIMP_CONDUCTION = 4
 FX_PARAM_NODE = 0
 FX_PARAM_CELL = 1
 FX_PCG_BLOCK3 = 1
@@ -121,6 +123,21 @@ SIGNATURES = {
                                      _P, _SZ]),
     "fx_tet4_simp_compliance_sens_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "fx_tet4_traction_vjp_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P]),
+    "fx_hex8s_pattern": (_I32, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "fx_hex8s_face_incidence": (_I32, [_I32, _I32, _P, _I32, _P, _P, _P]),
+    "fx_hex8s_assemble_workspace_bytes": (_SZ, [_I32, _I32, _I32]),
+    "fx_hex8s_assemble_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                     _I32, _I32, _F64, _P, _P, _P, _SZ]),
+    "fx_hex8s_assemble_f32": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                     _I32, _I32, _F64, _P, _P, _P, _SZ]),
+    "fx_hex8s_residual_workspace_bytes": (_SZ, [_I32]),
+    "fx_hex8s_residual_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P,
+                                     _P, _P, _I32, _P, _P, _I32, _P, _P, _SZ]),
+    "fx_hex8s_param_vjp_workspace_bytes": (_SZ, [_I32, _I32]),
+    "fx_hex8s_param_vjp_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P,
+                                      _P, _SZ]),
+    "fx_hex8s_simp_compliance_sens_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "fx_hex8s_traction_vjp_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P]),
     "fx_quad4_pattern": (_I32, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "fx_quad4_face_incidence": (_I32, [_I32, _I32, _P, _I32, _P, _P, _P]),
     "fx_quad4_assemble_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,

@@ -30,7 +30,7 @@ class TracedStructure:
         P = problem.points
         self.device = dev
         self.ele_type = problem.ele_type
-        self.tag = problem.ele_type.lower()
+        self.tag = problem.kernel_tag
         self.vec = problem.vec
         self.num_face_quads = problem.fes[0].num_face_quads
         self.num_nodes = problem.num_total_nodes
@@ -62,7 +62,7 @@ class TracedStructure:
                         "surface maps on the HIP path are feax_amd.materials.SurfaceTraction (dead loads)")
                 d = np.asarray(m.direction, float)
                 if d.shape != (self.vec,):
-                    raise ValueError(f"SurfaceTraction direction {d} must have {self.vec} components here")
+                    raise ValueError(f"surface map direction {d} must have {self.vec} components here")
                 fdir[self.face_offsets[i]:self.face_offsets[i + 1]] = d
         self.fdir = torch.as_tensor(fdir.reshape(-1), **f64)
         self._cache = {}
@@ -118,7 +118,7 @@ def resolve_material(problem, traced_params, ts):
         else:
             raise NotImplementedError(f"volume var of shape {tuple(p.shape)} (node (N,) or cell (E,) only)")
     else:
-        E = getattr(tm, "E", None)
+        E = getattr(tm, "fixed_param", getattr(tm, "E", None))
         if E is None:
             raise ValueError("tensor map needs its modulus from TracedParams.volume_vars[0]")
         key = ("constE", E)
@@ -343,8 +343,8 @@ def strain_energy(problem, ts, traced_params, sol, with_partial=False):
     u = _as_dev(sol, ts)
     out = torch.empty(1, dtype=torch.float64, device=ts.device)
     dmu = torch.empty(ts.num_cells, dtype=torch.float64, device=ts.device) if with_partial else None
-    if ts.tag == "quad4":
-        raise NotImplementedError("the strain energy is implemented for HEX8 and TET4 meshes")
+    if ts.tag not in ("hex8", "tet4"):
+        raise NotImplementedError("the strain energy is implemented for HEX8 and TET4 meshes (vec = 3)")
     wsb = int(getattr(nat.lib(), f"fx_{ts.tag}_energy_workspace_bytes")(ts.num_cells))
     ws = torch.empty(wsb, dtype=torch.uint8, device=ts.device)
     nat.call(f"fx_{ts.tag}_energy_f64", nat.stream(), ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),

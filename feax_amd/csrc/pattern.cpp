@@ -129,6 +129,13 @@ extern "C" int fx_quad4_pattern(int32_t num_nodes, int32_t num_cells, const int3
                       elem_rank, inc_ptr, inc);
 }
 
+extern "C" int fx_hex8s_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse_out,
+                                int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                                int32_t* inc_ptr, int32_t* inc) {
+  return cell_pattern("fx_hex8s_pattern", 8, 1, num_nodes, num_cells, cells, nse_out, max_degree, indptr, indices,
+                      elem_rank, inc_ptr, inc);
+}
+
 extern "C" int fx_hex8_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells,
                                       int32_t num_faces, const int32_t* faces, int32_t* finc_ptr,
                                       int32_t* finc) {
@@ -239,4 +246,11 @@ extern "C" int fx_version(char* buf, size_t len) {
   if (!buf || len == 0) return FX_EINVAL;
   snprintf(buf, len, "feaxhip 0.1 gfx950");
   return FX_OK;
+}
+
+// The face incidence depends on the nodes only: the scalar problem shares it.
+extern "C" int fx_hex8s_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells,
+                                       int32_t num_faces, const int32_t* faces, int32_t* finc_ptr,
+                                       int32_t* finc) {
+  return fx_hex8_face_incidence(num_nodes, num_cells, cells, num_faces, faces, finc_ptr, finc);
 }

@@ -96,8 +96,58 @@ class SurfaceTraction:
 
     def __init__(self, direction=(0.0, 0.0, 1.0)):
         self.direction = np.asarray(direction, float).reshape(-1)
-        if self.direction.shape not in ((2,), (3,)):
-            raise ValueError(f"traction direction must have 2 or 3 components, got {self.direction}")
+        if self.direction.shape not in ((1,), (2,), (3,)):
+            raise ValueError(f"traction direction must have 1, 2 or 3 components, got {self.direction}")
 
     def __call__(self, u, x, t):
         return self.direction * t
+
+
+class IsotropicConduction(TensorMap):
+    """Scalar flux q = k grad T (vec = 1): the tensor map
+    ``lambda grad_T, k: k * grad_T`` of feax's tests/test_params_bridge.py:
+    131-133 and examples/basic/thermo_mech_coupling.py:64-69.  ``k`` fixed
+    here, or (k=None) taken from TracedParams.volume_vars[0] as a node or
+    cell variable."""
+    kind = nat.FX_MAT_CONDUCTION
+
+    def __init__(self, k=None):
+        self.k = None if k is None else float(k)
+
+    @property
+    def fixed_param(self):
+        return self.k
+
+    def __call__(self, grad_T, k=None):
+        return (self.k if k is None else k) * grad_T
+
+    def material(self, param_loc):
+        return nat.material_struct(self.kind, param_loc)
+
+
+class SIMPConduction(IsotropicConduction):
+    """q = k(rho) grad T with k = kmin + rho^penal (k0 - kmin), rho a cell or
+    node variable (thermal topology optimisation)."""
+    kind = nat.FX_MAT_SIMP_CONDUCTION
+
+    def __init__(self, k0=1.0, kmin=1e-3, penal=3.0):
+        super().__init__(None)
+        self.k0, self.kmin, self.penal = float(k0), float(kmin), float(penal)
+
+    def conductivity(self, rho):
+        return self.kmin + np.asarray(rho) ** self.penal * (self.k0 - self.kmin)
+
+    def __call__(self, grad_T, rho):
+        return self.conductivity(rho) * grad_T
+
+    def material(self, param_loc):
+        return nat.material_struct(self.kind, param_loc, E0=self.k0, Emin=self.kmin, penal=self.penal)
+
+
+class SurfaceFlux(SurfaceTraction):
+    """Prescribed normal flux q = scale * s on a loaded boundary of a scalar
+    problem (the one-component surface map ``lambda u, x, q: np.array([q])``),
+    added with a + sign like a traction."""
+
+    def __init__(self, scale=1.0):
+        super().__init__((float(scale),))

@@ -1,5 +1,5 @@
-"""Problem: FE structure for HEX8 / TET4 (vec = 3) and QUAD4 (vec = 2)
-problems (the feax.Problem API).
+"""Problem: FE structure for HEX8 / TET4 (vec = 3), QUAD4 (vec = 2) and
+scalar HEX8 (vec = 1) problems (the feax.Problem API).
 
 Mirrors feax.Problem (feax/problem.py:37-846): constructor arguments,
 ``location_fns`` / ``boundary_inds_list``, ``matrix_view``, the physics
@@ -80,11 +80,12 @@ class Problem:
             self.ele_type = self.ele_type[0] if isinstance(self.ele_type, list) else self.ele_type
             self.gauss_order = self.gauss_order[0] if isinstance(self.gauss_order, list) else self.gauss_order
         ok = ((self.ele_type in ("HEX8", "TET4") and self.vec == 3 and self.dim == 3)
-              or (self.ele_type == "QUAD4" and self.vec == 2 and self.dim == 2))
+              or (self.ele_type == "QUAD4" and self.vec == 2 and self.dim == 2)
+              or (self.ele_type == "HEX8" and self.vec == 1 and self.dim == 3))
         if not ok:
             raise NotImplementedError(
-                f"the HIP path covers HEX8 and TET4 with vec=3, dim=3 and QUAD4 with vec=2, dim=2 (got "
-                f"{self.ele_type}, vec={self.vec}, dim={self.dim})")
+                f"the HIP path covers HEX8 and TET4 with vec=3, dim=3, QUAD4 with vec=2, dim=2 and scalar HEX8 "
+                f"(vec=1, dim=3) (got {self.ele_type}, vec={self.vec}, dim={self.dim})")
         default_order = 0 if self.ele_type == "TET4" else 2
         if self.gauss_order not in (None, default_order):
             raise NotImplementedError(f"{self.ele_type} uses feax's default rule (gauss_order={default_order})")
@@ -104,6 +105,13 @@ class Problem:
         self._build_csr_assembly_structure()
         self.custom_init(*self.additional_info)
 
+    @property
+    def kernel_tag(self):
+        """Name of the native kernel family: hex8, tet4, quad4, or hex8s
+        (scalar HEX8, vec = 1)."""
+        tag = self.ele_type.lower()
+        return tag + "s" if (tag == "hex8" and self.vec == 1) else tag
+
     # ------------------------------------------------------------------ structure
     def _build_csr_assembly_structure(self):
         """CSR pattern + assembly maps (feax/problem.py:316-484) via fx_hex8_pattern / fx_tet4_pattern /
@@ -111,7 +119,7 @@ class Problem:
         N, E, npe = self.num_total_nodes, self.num_cells, self.npe
         n = self.vec * N
         L = nat.lib()
-        tag = self.ele_type.lower()
+        tag = self.kernel_tag
         pattern = getattr(L, f"fx_{tag}_pattern")
         nse = ctypes.c_int64(0)
         maxdeg = ctypes.c_int32(0)

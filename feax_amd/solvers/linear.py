@@ -58,8 +58,8 @@ def solve_fp64(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None, keep=N
     the operator for an adjoint solve."""
     import torch
     if opts.operator == "matfree":
-        if problem.ele_type != "HEX8":
-            raise NotImplementedError("the matrix-free operator is the HEX8 one")
+        if problem.ele_type != "HEX8" or problem.vec != 3:
+            raise NotImplementedError("the matrix-free operator is the HEX8 (vec = 3) one")
         return _solve_matfree(problem, ts, tp, bc, u0, opts, symmetric, stats, keep)
     if opts.use_jacobi_preconditioner:
         J, dinv = assemble_csr(problem, ts, tp, u0, bc, symmetric, torch.float64, jacobi_shift=opts.jacobi_shift)
@@ -255,8 +255,8 @@ def create_linear_solver(problem, bc, solver_options=None, adjoint_solver_option
             from .direct import solve_direct
             return solve_direct(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
         if use_gmg:
-            if problem.ele_type != "HEX8":
-                raise NotImplementedError("GeometricMultigrid is built on the structured HEX8 box")
+            if problem.ele_type != "HEX8" or problem.vec != 3:
+                raise NotImplementedError("GeometricMultigrid is built on the structured HEX8 box (vec = 3)")
             return solve_gmg(problem, ts, tp, eff_bc, u0, solver_options, symmetric_elimination, stats, keep)
         if solver_options.solver in ("bicgstab", "gmres") and (solver_options.precision != "fp64"
                                                                or solver_options.operator != "csr"):

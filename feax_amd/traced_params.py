@@ -26,6 +26,35 @@ class TracedParams:
         object.__setattr__(self, "volume_vars", vv)
 
     @staticmethod
+    def node_var_from_solution(problem, sol, component=None, var_index=0):
+        """A solved field as a (num_nodes,) node variable of the next solve
+        on the same mesh (feax/traced_params.py:378-423); a torch solution
+        stays differentiable.  component is required when vec > 1."""
+        sol = getattr(sol, "dofs", sol)
+        if var_index != 0:
+            raise IndexError("single-variable problems have var_index 0 only")
+        u = sol.reshape(problem.num_total_nodes, problem.vec) if hasattr(sol, "reshape") else \
+            np.asarray(sol).reshape(problem.num_total_nodes, problem.vec)
+        if problem.vec == 1:
+            return u[:, 0]
+        if component is None:
+            raise ValueError(f"solution has vec={problem.vec} components; pass component=")
+        return u[:, component]
+
+    def replace_volume_var(self, index, new_var):
+        """A copy with volume_vars[index] replaced (feax/traced_params.py:425-442)."""
+        vv = list(self.volume_vars)
+        vv[index] = new_var
+        return TracedParams(volume_vars=tuple(vv), surface_vars=list(self.surface_vars))
+
+    def replace_surface_var(self, surface_index, var_index, new_var):
+        """A copy with surface_vars[surface_index][var_index] replaced
+        (feax/traced_params.py:444-466)."""
+        sv = [list(t) for t in self.surface_vars]
+        sv[surface_index][var_index] = new_var
+        return TracedParams(volume_vars=self.volume_vars, surface_vars=[tuple(t) for t in sv])
+
+    @staticmethod
     def create_node_var(problem, value, var_index=0):
         return np.full(problem.fes[var_index].num_total_nodes, float(value))
 

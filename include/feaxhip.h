@@ -36,6 +36,8 @@ extern "C" {
 #define FX_MAT_LINEAR_ELASTIC 0 /* sigma = lam tr(eps) I + 2 mu eps, E = param        */
 #define FX_MAT_SIMP 1           /* as above with E = Emin + rho^p (E0 - Emin)         */
 #define FX_MAT_NEO_HOOKEAN 2    /* P = mu (F - F^-T) + kappa (J-1) J F^-T, mu = param */
+#define FX_MAT_CONDUCTION 3     /* vec = 1: flux k grad T, k = param                  */
+#define FX_MAT_SIMP_CONDUCTION 4 /* vec = 1: k = Emin + rho^penal (E0 - Emin)          */
 
 /* Where the per-material parameter lives. */
 #define FX_PARAM_NODE 0 /* (N,) interpolated with the shape functions at quads */
@@ -656,6 +658,54 @@ int fx_gmg_galerkin_dense_f64(void* stream, int32_t nxf, int32_t nyf, int32_t nz
 /* dense[i * n + j] = A[i, j] of a CSR matrix (double), zeros elsewhere. */
 int fx_csr_to_dense_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices,
                         const double* data, double* dense);
+
+/* ------------------------------------------------------------------------
+ * Scalar HEX8 (vec = 1): steady conduction div(k grad T) = 0 with a surface
+ * flux -- feax's scalar Problem (vec=1, tensor map ``k * grad_T``,
+ * examples/basic/thermo_mech_coupling.py:64-69, tests/test_params_bridge.py:
+ * 131-150) on the kernels of feax/assembler.py:406-448, 494-535, 808-818.
+ * Same arguments as the fx_hex8_* entry points; DOF a = node a; the material
+ * is FX_MAT_CONDUCTION or FX_MAT_SIMP_CONDUCTION; fdir is one number per
+ * face (flux = fdir[f] * svar[4 f + q]); u is ignored by the assembly (the
+ * law is linear).  The pattern and the face incidence are those of
+ * feax/problem.py:316-484 with vec = 1.
+ * ------------------------------------------------------------------------ */
+int fx_hex8s_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse,
+                     int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                     int32_t* inc_ptr, int32_t* inc);
+int fx_hex8s_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int32_t num_faces,
+                            const int32_t* faces, int32_t* finc_ptr, int32_t* finc);
+size_t fx_hex8s_assemble_workspace_bytes(int32_t num_cells, int32_t value_bytes, int32_t flags);
+int fx_hex8s_assemble_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                          const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                          int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, double* data,
+                          void* workspace, size_t ws_bytes);
+int fx_hex8s_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                          const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                          int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, float* data,
+                          void* workspace, size_t ws_bytes);
+size_t fx_hex8s_residual_workspace_bytes(int32_t num_cells);
+int fx_hex8s_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const fx_material* mat, const double* param, const double* u, int32_t num_faces,
+                          const int32_t* faces, const int32_t* finc_ptr, const int32_t* finc, const double* svar,
+                          const double* fdir, int32_t num_bc, const int32_t* bc_rows, const double* bc_vals,
+                          int32_t negate, double* res, void* workspace, size_t ws_bytes);
+size_t fx_hex8s_param_vjp_workspace_bytes(int32_t num_cells, int32_t param_loc);
+int fx_hex8s_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                           const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                           const fx_material* mat, const double* param, const double* u, const double* lam,
+                           const uint8_t* bc_mask, double scale, double* grad, void* workspace, size_t ws_bytes);
+int fx_hex8s_simp_compliance_sens_f64(void* stream, int32_t num_cells, const double* x, const double* y,
+                                      const double* z, const int32_t* cells, const fx_material* mat,
+                                      const double* param, const double* u, double* dc);
+int fx_hex8s_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, const double* y, const double* z,
+                              const int32_t* cells, const int32_t* faces, const double* fdir, const double* lam,
+                              const uint8_t* bc_mask, double scale, double* grad_svar);
 
 #ifdef __cplusplus
 }

@@ -93,9 +93,12 @@ def test_unsupported_problems_raise():
     import feax_amd as fe
     mesh = fe.StructuredGrid((1, 1, 1)).to_mesh()
     with pytest.raises(NotImplementedError):
-        fe.Problem(mesh, vec=1, dim=3)
+        fe.Problem(mesh, vec=2, dim=3)
+    with pytest.raises(NotImplementedError):
+        fe.Problem(fe.mesh.rectangle_mesh(1, 1), vec=1, dim=2, ele_type="QUAD4")
     with pytest.raises(NotImplementedError):
         fe.Problem(mesh, vec=3, dim=3, ele_type="HEX20")
+    assert fe.Problem(mesh, vec=1, dim=3).kernel_tag == "hex8s"
     with pytest.raises(ValueError, match=r"\(E, 4\)"):
         fe.Problem(mesh, vec=3, dim=3, ele_type="TET4")
 
