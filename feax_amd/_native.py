@@ -33,7 +33,8 @@ FX_ASM_COMPUTE32 = 2
 class FxMaterial(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("param_loc", ctypes.c_int32), ("nu", ctypes.c_double),
                 ("E0", ctypes.c_double), ("Emin", ctypes.c_double), ("penal", ctypes.c_double),
-                ("kappa", ctypes.c_double)]
+                ("kappa", ctypes.c_double), ("temperature", ctypes.c_void_p), ("alpha", ctypes.c_double),
+                ("t_ref", ctypes.c_double)]
 
 
 class FxGmgLevel(ctypes.Structure):
@@ -116,6 +117,8 @@ SIGNATURES = {
     "fx_hex8_param_vjp_workspace_bytes": (_SZ, [_I32, _I32]),
     "fx_hex8_param_vjp_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P,
                                      _P, _SZ]),
+    "fx_hex8_thermal_vjp_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P, _P,
+                                       _SZ]),
     "fx_simp_compliance_sens_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "fx_hex8_traction_vjp_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P]),
     "fx_tet4_param_vjp_workspace_bytes": (_SZ, [_I32, _I32]),
@@ -235,5 +238,7 @@ def stream():
     return torch.cuda.current_stream().cuda_stream
 
 
-def material_struct(kind, param_loc, nu=0.3, E0=1.0, Emin=1e-9, penal=3.0, kappa=0.0):
-    return FxMaterial(int(kind), int(param_loc), float(nu), float(E0), float(Emin), float(penal), float(kappa))
+def material_struct(kind, param_loc, nu=0.3, E0=1.0, Emin=1e-9, penal=3.0, kappa=0.0, temperature=None,
+                    alpha=0.0, t_ref=0.0):
+    return FxMaterial(int(kind), int(param_loc), float(nu), float(E0), float(Emin), float(penal), float(kappa),
+                      temperature, float(alpha), float(t_ref))

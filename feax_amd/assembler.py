@@ -125,7 +125,38 @@ def resolve_material(problem, traced_params, ts):
         if key not in ts._cache:
             ts._cache[key] = torch.full((ts.num_cells,), float(E), dtype=torch.float64, device=ts.device)
         p, loc = ts._cache[key], nat.FX_PARAM_CELL
-    return tm.material(loc), p
+    mat = tm.material(loc)
+    alpha = getattr(tm, "alpha", None)
+    if alpha is not None:
+        # thermo-elastic law: the temperature is volume_vars[1], a node var
+        if ts.tag != "hex8":
+            raise NotImplementedError("thermal strain is built for HEX8 meshes (vec = 3)")
+        if len(vv) < 2:
+            raise ValueError("a thermo-elastic law takes the temperature as volume_vars[1] (node variable)")
+        T = _as_dev(vv[1], ts).to(torch.float64).contiguous()
+        if T.ndim != 1 or T.shape[0] != ts.num_nodes:
+            raise NotImplementedError(f"the temperature must be a node variable ({ts.num_nodes},)")
+        mat.temperature, mat.alpha, mat.t_ref = T.data_ptr(), float(alpha), float(tm.t_ref)
+        mat._temperature = T      # keeps the device array alive with the struct
+    return mat, p
+
+
+def thermal_vjp(problem, ts, traced_params, lam, bc=None, scale=1.0):
+    """scale * (dR/dT)^T lam for the temperature node variable
+    (volume_vars[1]) of a thermo-elastic law (fx_hex8_thermal_vjp_f64)."""
+    import torch
+    mat, p = resolve_material(problem, traced_params, ts)
+    if not mat.temperature:
+        raise ValueError("the tensor map has no temperature field")
+    lam = _as_dev(lam, ts)
+    mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
+    grad = torch.empty(ts.num_nodes, dtype=torch.float64, device=ts.device)
+    wsb = int(nat.lib().fx_hex8_param_vjp_workspace_bytes(ts.num_cells, nat.FX_PARAM_NODE))
+    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=ts.device)
+    nat.call("fx_hex8_thermal_vjp_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
+             nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat),
+             nat.ptr(p), nat.ptr(lam), nat.ptr(mask), float(scale), nat.ptr(grad), nat.ptr(ws), wsb)
+    return grad
 
 
 def surface_values(problem, traced_params, ts):

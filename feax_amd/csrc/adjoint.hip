@@ -150,8 +150,15 @@ __global__ __launch_bounds__(kElemBlock) void param_vjp_kernel(
 #pragma unroll
         for (int d = 0; d < 3; ++d)
           dens += (mu0 * (ug[i][d] + ug[d][i]) + (i == d ? lam0 * tr : 0.0)) * lg[i][d];
+      const double pq = param_at<double>(mat, param, static_cast<int>(e), cn, x0, x1, x2);
+      if (mat.temp) {
+        // thermal stress -E per_E I: d/dE contributes -per_E tr(grad lambda)
+        double per_E;
+        thermal_shift<double>(mat, pq, cn, x0, x1, x2, &per_E);
+        dens -= per_E * (lg[0][0] + lg[1][1] + lg[2][2]);
+      }
       double dEdp;
-      modulus_of<double>(mat, param_at<double>(mat, param, static_cast<int>(e), cn, x0, x1, x2), &dEdp);
+      modulus_of<double>(mat, pq, &dEdp);
       dens *= dEdp;
     }
     w = dens * jxw;
@@ -165,6 +172,31 @@ __global__ __launch_bounds__(kElemBlock) void param_vjp_kernel(
 }
 
 // Node variables: g_a = scale * sum_{e in inc(a)} sum_q N_a(q) w_eq.
+// Pullback onto the temperature node variable of the thermal strain, per
+// (element, Gauss point): the thermal part of R_ai is -kth d_i N_a with
+// kth = E c3 alpha (T_q - t_ref), so lambda . dR/dT_q = -E c3 alpha
+// div(lambda) JxW (T_q = sum_b N_b(q) T_b: the node kernel spreads it).
+__global__ __launch_bounds__(kElemBlock) void thermal_vjp_kernel(
+    int num_cells, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const int* __restrict__ cells, MatParams mat, const double* __restrict__ param, const double* __restrict__ lam,
+    const uint8_t* __restrict__ bc_mask, double* __restrict__ wq) {
+  const long t = static_cast<long>(blockIdx.x) * kElemBlock + threadIdx.x;
+  const long e = t >> 3;
+  const int q = static_cast<int>(t & 7);
+  if (e >= num_cells) return;
+  const int* cn = cells + 8 * e;
+  double X[8][3];
+  load_coords<double>(x, y, z, cn, X);
+  double x0, x1, x2;
+  gauss_point(q, x0, x1, x2);
+  double ug[3][3], lg[3][3];
+  const double jxw = field_grads<false>(X, cn, x0, x1, x2, lam, lam, bc_mask, ug, lg);
+  const double E = modulus_of<double>(mat, param_at<double>(mat, param, static_cast<int>(e), cn, x0, x1, x2));
+  const double nu = mat.nu;
+  const double c3 = 3.0 * nu / ((1 + nu) * (1 - 2 * nu)) + 1.0 / (1 + nu);
+  wq[8 * e + q] = -E * c3 * mat.alpha * (lg[0][0] + lg[1][1] + lg[2][2]) * jxw;
+}
+
 __global__ __launch_bounds__(256) void param_vjp_node_kernel(int num_nodes, const int* __restrict__ inc_ptr,
                                                              const int* __restrict__ inc,
                                                              const double* __restrict__ wq, double scale,
@@ -298,6 +330,8 @@ int fx_hex8_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, co
     return fail(FX_EINVAL, "fx_hex8_param_vjp_f64: bad arguments");
   if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
     return fail(FX_EINVAL, "fx_hex8_param_vjp_f64: unknown material %d", mat->kind);
+  if (mat->temperature && mat->kind == FX_MAT_NEO_HOOKEAN)
+    return fail(FX_EINVAL, "fx_hex8_param_vjp_f64: thermal strain is built for the linear / SIMP laws");
   if (mat->param_loc != FX_PARAM_NODE && mat->param_loc != FX_PARAM_CELL)
     return fail(FX_EINVAL, "fx_hex8_param_vjp_f64: unknown parameter location %d", mat->param_loc);
   hipStream_t s = as_stream(stream);
@@ -322,11 +356,36 @@ int fx_hex8_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, co
   return FX_OK;
 }
 
+int fx_hex8_thermal_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                            const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                            const fx_material* mat, const double* param, const double* lam, const uint8_t* bc_mask,
+                            double scale, double* grad, void* workspace, size_t ws_bytes) {
+  if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !z || !cells || !inc_ptr || !inc || !mat || !param || !lam ||
+      !grad)
+    return fail(FX_EINVAL, "fx_hex8_thermal_vjp_f64: bad arguments");
+  if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP)
+    return fail(FX_EINVAL, "fx_hex8_thermal_vjp_f64: thermal strain is built for the linear / SIMP laws");
+  if (!mat->temperature) return fail(FX_EINVAL, "fx_hex8_thermal_vjp_f64: the material has no temperature field");
+  const size_t need = fx_hex8_param_vjp_workspace_bytes(num_cells, FX_PARAM_NODE);
+  if (!workspace || ws_bytes < need)
+    return fail(FX_EWORKSPACE, "fx_hex8_thermal_vjp_f64: workspace %zu < %zu bytes", ws_bytes, need);
+  hipStream_t s = as_stream(stream);
+  double* wq = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(thermal_vjp_kernel, dim3(elem_grid(num_cells)), dim3(kElemBlock), 0, s, num_cells, x, y, z,
+                     cells, to_params(mat), param, lam, bc_mask, wq);
+  hipLaunchKernelGGL(param_vjp_node_kernel, dim3((num_nodes + 255) / 256), dim3(256), 0, s, num_nodes, inc_ptr, inc,
+                     wq, scale, grad);
+  FX_LAUNCHED("thermal_vjp_kernel");
+  return FX_OK;
+}
+
 int fx_simp_compliance_sens_f64(void* stream, int32_t num_cells, const double* x, const double* y, const double* z,
                                 const int32_t* cells, const fx_material* mat, const double* param, const double* u,
                                 double* dc) {
   if (num_cells <= 0 || !x || !y || !z || !cells || !mat || !param || !u || !dc)
     return fail(FX_EINVAL, "fx_simp_compliance_sens_f64: bad arguments");
+  if (mat->temperature)
+    return fail(FX_EINVAL, "fx_simp_compliance_sens_f64: a thermal load depends on E; use the adjoint");
   if (mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_LINEAR_ELASTIC)
     return fail(FX_EINVAL, "fx_simp_compliance_sens_f64: needs a linear law (SIMP or linear elastic)");
   if (mat->param_loc != FX_PARAM_CELL)

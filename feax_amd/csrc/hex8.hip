@@ -326,7 +326,12 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void residual_kernel(
         ug[i][d] = v;
       }
     double sig[3][3];
-    material_stress<double>(mat, param_at<double>(mat, param, e, cn, x0, x1, x2), ug, sig);
+    const double pq = param_at<double>(mat, param, e, cn, x0, x1, x2);
+    material_stress<double>(mat, pq, ug, sig);
+    const double kth = thermal_shift<double>(mat, pq, cn, x0, x1, x2);
+    sig[0][0] -= kth;
+    sig[1][1] -= kth;
+    sig[2][2] -= kth;
     const double w = det * 0.125;
     double ri[3];
 #pragma unroll
@@ -413,7 +418,12 @@ __global__ __launch_bounds__(64 * kElemPerBlock) void elem_residual_kernel(
       ug[i][d] = v;
     }
   double sig[3][3];
-  material_stress<double>(mat, param_at<double>(mat, param, e, cn, x0, x1, x2), ug, sig);
+  const double pq = param_at<double>(mat, param, e, cn, x0, x1, x2);
+  material_stress<double>(mat, pq, ug, sig);
+  const double kth = thermal_shift<double>(mat, pq, cn, x0, x1, x2);
+  sig[0][0] -= kth;
+  sig[1][1] -= kth;
+  sig[2][2] -= kth;
   const double w = det * 0.125;
   double ri[3];
 #pragma unroll
@@ -712,6 +722,8 @@ int fx_hex8_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, con
     return fail(FX_EINVAL, "fx_hex8_residual_f64: bad arguments");
   if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
     return fail(FX_EINVAL, "fx_hex8_residual_f64: unknown material %d", mat->kind);
+  if (mat->temperature && mat->kind == FX_MAT_NEO_HOOKEAN)
+    return fail(FX_EINVAL, "fx_hex8_residual_f64: thermal strain is built for the linear / SIMP laws");
   if (num_faces > 0 && (!faces || !finc_ptr || !finc || !svar || !fdir))
     return fail(FX_EINVAL, "fx_hex8_residual_f64: face arrays missing");
   hipStream_t s = as_stream(stream);

@@ -84,6 +84,8 @@ __device__ __forceinline__ void phys_grad(const T (&Ji)[3][3], T d0, T d1, T d2,
 struct MatParams {
   int kind, loc;
   double nu, E0, Emin, penal, kappa;
+  const double* temp;  // thermal strain: node temperatures, or nullptr
+  double alpha, tref;
 };
 
 inline MatParams to_params(const fx_material* m) {
@@ -95,6 +97,9 @@ inline MatParams to_params(const fx_material* m) {
   p.Emin = m->Emin;
   p.penal = m->penal;
   p.kappa = m->kappa;
+  p.temp = m->temperature;
+  p.alpha = m->alpha;
+  p.tref = m->t_ref;
   return p;
 }
 
@@ -210,6 +215,28 @@ __device__ __forceinline__ void material_stress(const MatParams& m, T p, const T
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int d = 0; d < 3; ++d) S[i][d] = mu * (ug[i][d] + ug[d][i]) + (i == d ? lam * tr : T(0));
+}
+
+// Thermal strain of the linear / SIMP laws: sigma = C : (eps - alpha dT I)
+// shifts the stress by -kth I, kth = E (3 lam0 + 2 mu0) alpha (T_q - t_ref)
+// (the thermo-elastic tensor map of feax's examples/basic/
+// thermo_mech_coupling.py:95-103).  0 without a temperature field.
+template <typename T>
+__device__ __forceinline__ T thermal_shift(const MatParams& m, T p, const int* __restrict__ cn, T x0, T x1, T x2,
+                                           T* dkth_dE = nullptr) {
+  if (!m.temp) return T(0);
+  T tq = T(0);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    T N, d0, d1, d2;
+    shape<T>(c, x0, x1, x2, N, d0, d1, d2);
+    tq += N * static_cast<T>(m.temp[cn[c]]);
+  }
+  const T nu = static_cast<T>(m.nu);
+  const T c3 = T(3) * nu / ((T(1) + nu) * (T(1) - T(2) * nu)) + T(1) / (T(1) + nu);  // 3 lam0 + 2 mu0
+  const T per_E = c3 * static_cast<T>(m.alpha) * (tq - static_cast<T>(m.tref));
+  if (dkth_dE) *dkth_dE = per_E;
+  return modulus_of<T>(m, p) * per_E;
 }
 
 // Face quadrature point (lf, q) in the reference cell: the 2x2 Gauss points

@@ -268,6 +268,7 @@ int s_assemble(void* stream, int num_nodes, int num_cells, const double* x, cons
       !indices || !mat || !param || !data)
     return fail(FX_EINVAL, "fx_hex8s_assemble: bad arguments");
   if (!conduction_law(mat)) return fail(FX_EINVAL, "fx_hex8s_assemble: needs a conduction law (got %d)", mat->kind);
+  if (mat->temperature) return fail(FX_EINVAL, "fx_hex8s_assemble: thermal strain is for the elastic laws");
   if (flags & (FX_ASM_DIFF3 | FX_ASM_COMPUTE32))
     return fail(FX_EINVAL, "fx_hex8s_assemble: the difference form and FP32 arithmetic are built for vec = 3");
   const size_t need = 64 * sizeof(double) * static_cast<size_t>(num_cells);
@@ -334,6 +335,7 @@ int fx_hex8s_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, co
       !res)
     return fail(FX_EINVAL, "fx_hex8s_residual_f64: bad arguments");
   if (!conduction_law(mat)) return fail(FX_EINVAL, "fx_hex8s_residual_f64: needs a conduction law (got %d)", mat->kind);
+  if (mat->temperature) return fail(FX_EINVAL, "fx_hex8s_residual_f64: thermal strain is for the elastic laws");
   if (num_faces > 0 && (!faces || !finc_ptr || !finc || !svar || !fdir))
     return fail(FX_EINVAL, "fx_hex8s_residual_f64: face arrays missing");
   const size_t need = fx_hex8s_residual_workspace_bytes(num_cells);
@@ -370,6 +372,7 @@ int fx_hex8s_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, c
     return fail(FX_EINVAL, "fx_hex8s_param_vjp_f64: bad arguments");
   if (!conduction_law(mat))
     return fail(FX_EINVAL, "fx_hex8s_param_vjp_f64: needs a conduction law (got %d)", mat->kind);
+  if (mat->temperature) return fail(FX_EINVAL, "fx_hex8s_param_vjp_f64: thermal strain is for the elastic laws");
   if (mat->param_loc != FX_PARAM_NODE && mat->param_loc != FX_PARAM_CELL)
     return fail(FX_EINVAL, "fx_hex8s_param_vjp_f64: unknown parameter location %d", mat->param_loc);
   hipStream_t s = as_stream(stream);

@@ -417,6 +417,7 @@ int quad_assemble(void* stream, int num_nodes, int num_cells, const double* x, c
     return fail(FX_EINVAL, "fx_quad4_assemble: bad arguments");
   if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
     return fail(FX_EINVAL, "fx_quad4_assemble: unknown material %d", mat->kind);
+  if (mat->temperature) return fail(FX_EINVAL, "fx_quad4_assemble: thermal strain is built for HEX8");
   if (flags & (FX_ASM_DIFF3 | FX_ASM_COMPUTE32))
     return fail(FX_EINVAL, "fx_quad4_assemble: the difference form and FP32 arithmetic are built for vec = 3");
   const bool nh = mat->kind == FX_MAT_NEO_HOOKEAN;
@@ -491,6 +492,7 @@ int fx_quad4_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, co
     return fail(FX_EINVAL, "fx_quad4_residual_f64: bad arguments");
   if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
     return fail(FX_EINVAL, "fx_quad4_residual_f64: unknown material %d", mat->kind);
+  if (mat->temperature) return fail(FX_EINVAL, "fx_quad4_residual_f64: thermal strain is built for HEX8");
   if (num_faces > 0 && (!faces || !finc_ptr || !finc || !svar || !fdir))
     return fail(FX_EINVAL, "fx_quad4_residual_f64: face arrays missing");
   const size_t need = fx_quad4_residual_workspace_bytes(num_cells);
@@ -528,6 +530,7 @@ int fx_quad4_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, c
     return fail(FX_EINVAL, "fx_quad4_param_vjp_f64: bad arguments");
   if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
     return fail(FX_EINVAL, "fx_quad4_param_vjp_f64: unknown material %d", mat->kind);
+  if (mat->temperature) return fail(FX_EINVAL, "fx_quad4_param_vjp_f64: thermal strain is built for HEX8");
   if (mat->param_loc != FX_PARAM_NODE && mat->param_loc != FX_PARAM_CELL)
     return fail(FX_EINVAL, "fx_quad4_param_vjp_f64: unknown parameter location %d", mat->param_loc);
   hipStream_t s = as_stream(stream);

@@ -409,6 +409,7 @@ int tet_assemble(void* stream, int num_nodes, int num_cells, const double* x, co
     return fail(FX_EINVAL, "fx_tet4_assemble: bad arguments");
   if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
     return fail(FX_EINVAL, "fx_tet4_assemble: unknown material %d", mat->kind);
+  if (mat->temperature) return fail(FX_EINVAL, "fx_tet4_assemble: thermal strain is built for HEX8");
   const bool nh = mat->kind == FX_MAT_NEO_HOOKEAN;
   if (nh && !u) return fail(FX_EINVAL, "fx_tet4_assemble: the Neo-Hookean tangent needs u");
   const size_t need = tet_matrix_bytes(num_cells, sizeof(T), flags);
@@ -489,6 +490,7 @@ int fx_tet4_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, con
     return fail(FX_EINVAL, "fx_tet4_residual_f64: bad arguments");
   if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
     return fail(FX_EINVAL, "fx_tet4_residual_f64: unknown material %d", mat->kind);
+  if (mat->temperature) return fail(FX_EINVAL, "fx_tet4_residual_f64: thermal strain is built for HEX8");
   if (num_faces > 0 && (!faces || !finc_ptr || !finc || !svar || !fdir))
     return fail(FX_EINVAL, "fx_tet4_residual_f64: face arrays missing");
   const size_t need = fx_tet4_residual_workspace_bytes(num_cells);
@@ -525,6 +527,7 @@ int fx_tet4_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, co
     return fail(FX_EINVAL, "fx_tet4_param_vjp_f64: bad arguments");
   if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
     return fail(FX_EINVAL, "fx_tet4_param_vjp_f64: unknown material %d", mat->kind);
+  if (mat->temperature) return fail(FX_EINVAL, "fx_tet4_param_vjp_f64: thermal strain is built for HEX8");
   if (mat->param_loc != FX_PARAM_NODE && mat->param_loc != FX_PARAM_CELL)
     return fail(FX_EINVAL, "fx_tet4_param_vjp_f64: unknown parameter location %d", mat->param_loc);
   hipStream_t s = as_stream(stream);

@@ -151,3 +151,35 @@ class SurfaceFlux(SurfaceTraction):
 
     def __init__(self, scale=1.0):
         super().__init__((float(scale),))
+
+
+class ThermoElastic(IsotropicLinearElastic):
+    """sigma = lam tr(eps_m) I + 2 mu eps_m with eps_m = eps - alpha (T - t_ref) I:
+    the thermo-elastic tensor map ``stress(u_grad, E, T)`` of feax's
+    tests/test_params_bridge.py:136-143 and examples/basic/
+    thermo_mech_coupling.py:95-103.  E is volume_vars[0] (node or cell
+    variable, or fixed with E=...) and the temperature T is volume_vars[1],
+    a node variable (e.g. TracedParams.node_var_from_solution of a thermal
+    solve).  HEX8 only."""
+
+    def __init__(self, nu=0.3, alpha=1e-5, t_ref=0.0, E=None):
+        super().__init__(nu, E)
+        self.alpha, self.t_ref = float(alpha), float(t_ref)
+
+    def __call__(self, u_grad, E=None, T=0.0):
+        u_grad = np.asarray(u_grad, float)
+        shift = self.alpha * (np.asarray(T, float) - self.t_ref)
+        return super().__call__(u_grad - shift[..., None, None] * np.eye(u_grad.shape[-1]), E)
+
+
+class SIMPThermoElastic(SIMPLinearElastic):
+    """ThermoElastic with E = Emin + rho^penal (E0 - Emin) (the lattice law of
+    examples/basic/thermo_mech_coupling.py:95-103 with Emin = 0)."""
+
+    def __init__(self, nu=0.3, E0=1.0, Emin=1e-9, penal=3.0, alpha=1e-5, t_ref=0.0):
+        super().__init__(nu, E0, Emin, penal)
+        self.alpha, self.t_ref = float(alpha), float(t_ref)
+
+    def __call__(self, u_grad, rho, T=0.0):
+        E = self.modulus(rho)
+        return ThermoElastic(self.nu, self.alpha, self.t_ref)(u_grad, E, T)

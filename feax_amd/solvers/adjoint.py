@@ -194,6 +194,10 @@ def parameter_gradients(problem, ts, tp, bc, u, lam, paths, shapes, needs):
                 grads.append(g.sum().reshape(shape))
             else:
                 raise NotImplementedError(f"gradient for a surface var of shape {tuple(shape)}")
+        elif path == ("v", 1) and getattr(problem.get_tensor_map(), "alpha", None) is not None:
+            # the temperature of a thermo-elastic law
+            from ..assembler import thermal_vjp
+            grads.append(thermal_vjp(problem, ts, tp, lam, bc, scale=-1.0).reshape(shape))
         else:
             # Further volume/surface vars do not enter the HIP residual.
             import torch

@@ -49,6 +49,12 @@ typedef struct fx_material {
   double nu;              /* Poisson ratio (linear elastic / SIMP)  */
   double E0, Emin, penal; /* SIMP law                               */
   double kappa;           /* Neo-Hookean bulk modulus               */
+  /* Thermal strain of the linear / SIMP laws (HEX8 only):
+   * sigma = C(E) : (eps - alpha (T - t_ref) I) with T the node variable
+   * `temperature` (device, num_nodes), interpolated at the Gauss points;
+   * NULL: no thermal strain.  The tangent does not depend on T. */
+  const double* temperature;
+  double alpha, t_ref;
 } fx_material;
 
 /* Last error message of this thread (NUL-terminated, truncated to len). */
@@ -471,6 +477,18 @@ int fx_hex8_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, co
  * for SIMP (feax/solvers/cmg.py:312-323).  This is the self-adjoint case
  * (lambda = u) of the gradient of C = f.u (feax/gene/responses.py:10-105)
  * when the Dirichlet values are zero and the load does not depend on p. */
+/* d(lambda . R)/dT for the temperature node variable of a thermo-elastic
+ * material (mat->temperature set; linear / SIMP law): grad[b] = scale *
+ * sum over the elements and Gauss points at node b of
+ * -N_b E (3 lam0 + 2 mu0) alpha div(lambda) JxW (lambda read as 0 on
+ * bc_mask rows).  Workspace: fx_hex8_param_vjp_workspace_bytes(E,
+ * FX_PARAM_NODE).  Replaces the temperature cotangent jax.vjp forms through
+ * the thermo-elastic tensor map (examples/basic/thermo_mech_coupling.py:
+ * 95-103, feax/solvers/linear.py:466-483). */
+int fx_hex8_thermal_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                            const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                            const fx_material* mat, const double* param, const double* lam, const uint8_t* bc_mask,
+                            double scale, double* grad, void* workspace, size_t ws_bytes);
 int fx_simp_compliance_sens_f64(void* stream, int32_t num_cells, const double* x, const double* y,
                                 const double* z, const int32_t* cells, const fx_material* mat,
                                 const double* param, const double* u, double* dc);
