@@ -354,7 +354,9 @@ def create_vmap_solver(problem, bc, solver_options=None, autotune=True, graph=Tr
     traced_params carries a leading batch axis B; bc_vals, when given, is
     (B, len(bc.bc_rows)) (one set of Dirichlet values per lane).  The lanes
     are solved together by the batched engine (one assembly launch, one
-    segmented PCG; per-lane stopping tests as in a vmapped jax cg)."""
+    segmented PCG; per-lane stopping tests as in a vmapped jax cg); other
+    Krylov methods (bicgstab, gmres) and preconditioned or matrix-free forms
+    run one create_solver call per lane."""
     import torch
     from .solvers.options import KrylovSolverOptions
     opts = solver_options or KrylovSolverOptions(solver="cg", use_jacobi_preconditioner=True)
@@ -362,6 +364,25 @@ def create_vmap_solver(problem, bc, solver_options=None, autotune=True, graph=Tr
 
     def lane(x, i):
         return x[i] if x is not None else None
+
+    lane_solver = []
+
+    def per_lane(traced_params, bc_vals):
+        # Krylov methods other than CG (and preconditioned / matrix-free
+        # forms): one create_solver call per lane, stacked; autograd runs per
+        # lane through the solver's own backward.
+        from .solver import create_solver
+        if not lane_solver:
+            lane_solver.append(create_solver(problem, bc, opts, linear=True, return_solution=False))
+        leaves = list(traced_params.volume_vars) + [v for sv in traced_params.surface_vars for v in sv]
+        B = int(leaves[0].shape[0]) if leaves else int(bc_vals.shape[0])
+        out = []
+        for i in range(B):
+            tpi = TracedParams(volume_vars=tuple(lane(v, i) for v in traced_params.volume_vars),
+                               surface_vars=[tuple(lane(v, i) for v in sv) for sv in traced_params.surface_vars])
+            bci = bc if bc_vals is None else bc.replace_vals(bc_vals[i])
+            out.append(lane_solver[0](tpi, None, bc=bci))
+        return torch.stack(out)
 
     def run(traced_params, bc_vals, keep=None):
         leaves = list(traced_params.volume_vars) + [v for sv in traced_params.surface_vars for v in sv]
@@ -395,6 +416,9 @@ def create_vmap_solver(problem, bc, solver_options=None, autotune=True, graph=Tr
 
     def solve(traced_params, bc_vals=None):
         from .solvers.adjoint import _leaves, _needs_grad
+        if opts.solver != "cg" or opts.precision not in ("fp64", "mixed") or opts.operator != "csr" \
+                or getattr(opts, "preconditioner", None) is not None:
+            return per_lane(traced_params, bc_vals)
         if not _needs_grad(traced_params):
             return run(traced_params, bc_vals)
         leaves = _leaves(traced_params)

@@ -250,3 +250,29 @@ def test_vmap_solver_gradients_match_per_lane_autograd():
         es = float((s.grad[i] - si.grad).norm() / si.grad.norm())
         print("lane", i, "rho grad", er, "traction grad", es)
         assert er < 1e-8 and es < 1e-8
+
+
+@pytest.mark.parametrize("method", ["bicgstab", "gmres"])
+def test_vmap_solver_other_krylov_methods(method):
+    """The reference's tests/test_vmap.py:109-190 cases: vmapped BiCGSTAB and
+    GMRES solves equal one solve per lane, and their gradients too."""
+    import torch
+    import feax_amd as fe
+    from feax_amd.batch import create_vmap_solver
+    rng = np.random.default_rng(13)
+    p, bc, tp, rho, t = _simp_instance((6, 3, 3), rng)
+    B = 2
+    rhos = rng.uniform(0.3, 1.0, (B, p.num_cells))
+    ts_ = np.stack([fe.TracedParams.create_uniform_surface_var(p, 1e-2) for _ in range(B)])
+    opts = fe.KrylovSolverOptions(solver=method, tol=1e-12, atol=0.0, maxiter=20000, use_jacobi_preconditioner=True)
+    r = torch.tensor(rhos, device="cuda", requires_grad=True)
+    print("vmap", method, flush=True)
+    U = create_vmap_solver(p, bc, opts)(fe.TracedParams(volume_vars=(r,), surface_vars=[(torch.as_tensor(ts_),)]))
+    (U * U).sum().backward()
+    single = fe.create_solver(p, bc, opts, linear=True, return_solution=False)
+    for i in range(B):
+        ri = torch.tensor(rhos[i], device="cuda", requires_grad=True)
+        ui = single(fe.TracedParams(volume_vars=(ri,), surface_vars=[(ts_[i],)]))
+        (ui * ui).sum().backward()
+        assert float((U[i] - ui).norm() / ui.norm()) < 1e-12
+        assert float((r.grad[i] - ri.grad).norm() / ri.grad.norm()) < 1e-12
