@@ -33,13 +33,13 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 
 # HBM bytes per pcg_spmv_kernel dispatch on the 200x40x40 workload, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of `bench.py --kernels-only`
-# (profiles/pcg_spmv_c2_cantilever_pmc.md), each scaled by the same counter's
+# (profiles/pcg_spmv_cantilever_200x40x40_pmc.md), each scaled by the same counter's
 # reading for a 1 GiB copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
 TRAFFIC = {"f64/block3": 761_544_739, "f32f64/diff3": 439_444_525}
 
 CONFIGS = {
-    "c2": dict(dims=(200, 40, 40), spacing=1.0),
-    "c1": dict(dims=(50, 5, 5), spacing=2.0),
+    "cantilever_200x40x40": dict(dims=(200, 40, 40), spacing=1.0),
+    "cantilever_50x5x5": dict(dims=(50, 5, 5), spacing=2.0),
 }
 
 
@@ -502,7 +502,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="cantilever_200x40x40", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad", action="store_true", help="skip the SIMP compliance-gradient workload")
     ap.add_argument("--no-newton", action="store_true", help="skip the Neo-Hookean Newton workload")
@@ -543,11 +543,11 @@ def main():
         return fe.create_solver(problem, bc, opts, linear=True, traced_params=tp, traced_structure=ts,
                                 return_solution=False)
 
-    # FP64 oracle (tests/golden/c2_oracle.npz: the oracle's Jacobi-PCG to rel
+    # FP64 oracle (tests/golden/cantilever_200x40x40_oracle.npz: the oracle's Jacobi-PCG to rel
     # 1e-12 on this workload, u at 4096 seeded DOFs): error of each form.
     oracle_fix = None
-    fix_path = os.path.join(ROOT, "tests", "golden", "c2_oracle.npz")
-    if args.config == "c2" and os.path.exists(fix_path):
+    fix_path = os.path.join(ROOT, "tests", "golden", "cantilever_200x40x40_oracle.npz")
+    if args.config == "cantilever_200x40x40" and os.path.exists(fix_path):
         oracle_fix = np.load(fix_path, allow_pickle=False)
 
     results = {}
@@ -654,7 +654,7 @@ def main():
         "batch": batch,
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": TRAFFIC.get(dom["label"]) if args.config == "c2" else None, "kernel_us": round(dom["t"] * 1e6, 2),
+                     "traffic": TRAFFIC.get(dom["label"]) if args.config == "cantilever_200x40x40" else None, "kernel_us": round(dom["t"] * 1e6, 2),
                      "alg_bytes": dom["B"]},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }

@@ -8,14 +8,14 @@ Fixtures (npz, data only):
                       x=0, traction 1e-3 e_z on x=2, E=70e3 node var, nu=0.3):
                       indptr, indices, BC-eliminated CSR data, residual at
                       u=0, b, direct-solve u
-  c1_samples.npz      50x5x5 (h=2): sha256 of indptr/indices, 1000 seeded
+  cantilever_50x5x5_samples.npz      50x5x5 (h=2): sha256 of indptr/indices, 1000 seeded
                       (slot, value) pairs of the BC-eliminated CSR, u at 64
                       seeded DOFs (direct solve)
   make_ke_3d.npz      unit-cube element stiffness (E=1, nu=0.3), feax corner order
   simp_4x2x2.npz      SIMP compliance sensitivity on a 4x2x2 grid (rho seeded)
-  c2_oracle.npz       200x40x40 (h=1) cantilever: oracle Jacobi-PCG to rel
+  cantilever_200x40x40_oracle.npz       200x40x40 (h=1) cantilever: oracle Jacobi-PCG to rel
                       1e-12, u at 4096 seeded DOFs, max|u|, ||u||
-  c3_oracle.npz       120x40x20 SIMP cantilever (rho ~ U(0.3,1) seeded,
+  simp_120x40x20_oracle.npz       120x40x20 SIMP cantilever (rho ~ U(0.3,1) seeded,
                       unit patch load): oracle PCG to rel 1e-12, compliance,
                       closed-form dC/drho at 4096 seeded cells, max|dC/drho|
 """
@@ -56,7 +56,7 @@ def fixture_2x2x2():
              data_bc=info["data_bc"], res0=res0, b=info["b"], u=ud, bc_rows=rows)
 
 
-def c1_samples():
+def cantilever_50x5x5_samples():
     import scipy.sparse.linalg as spla
     m, E, tr, rows, vals = cantilever_model((50, 5, 5), 2.0)
     mat = hex8.Material("linear_elastic", nu=0.3)
@@ -65,7 +65,7 @@ def c1_samples():
     rng = np.random.default_rng(SEED)
     slots = np.sort(rng.choice(m.nse, 1000, replace=False))
     dofs = np.sort(rng.choice(m.n, 64, replace=False))
-    np.savez(os.path.join(HERE, "c1_samples.npz"),
+    np.savez(os.path.join(HERE, "cantilever_50x5x5_samples.npz"),
              indptr_sha256=np.frombuffer(hashlib.sha256(m.csr["indptr"].astype(np.int32).tobytes()).digest(), np.uint8),
              indices_sha256=np.frombuffer(hashlib.sha256(m.csr["indices"].astype(np.int32).tobytes()).digest(),
                                           np.uint8),
@@ -99,20 +99,20 @@ def simp_4x2x2():
              dc=adjoint.compliance_sens_closed_form(m, u, rho, mat))
 
 
-def c2_oracle():
+def cantilever_200x40x40_oracle():
     t0 = time.time()
     m, E, tr, rows, vals = cantilever_model((200, 40, 40), 1.0)
-    print(f"c2 model {time.time() - t0:.0f} s", flush=True)
+    print(f"200x40x40 model {time.time() - t0:.0f} s", flush=True)
     mat = hex8.Material("linear_elastic", nu=0.3)
     u, info = krylov.linear_solve(m, mat, E, tr, rows, vals, tol=1e-12, atol=0.0, maxiter=100000)
-    print(f"c2 solve {time.time() - t0:.0f} s, {info['iters']} iterations", flush=True)
+    print(f"200x40x40 solve {time.time() - t0:.0f} s, {info['iters']} iterations", flush=True)
     relres = np.linalg.norm(info["K"] @ (u - 0.0) - info["b"]) / np.linalg.norm(info["b"])
     dofs = np.sort(np.random.default_rng(SEED).choice(m.n, 4096, replace=False))
-    np.savez(os.path.join(HERE, "c2_oracle.npz"), dofs=dofs, u=u[dofs], u_max=np.abs(u).max(),
+    np.savez(os.path.join(HERE, "cantilever_200x40x40_oracle.npz"), dofs=dofs, u=u[dofs], u_max=np.abs(u).max(),
              u_norm=np.linalg.norm(u), iters=info["iters"], relres=relres)
 
 
-def c3_oracle():
+def simp_120x40x20_oracle():
     t0 = time.time()
     dims = (120, 40, 20)
     pts, cells = structure.structured_grid(dims)
@@ -128,7 +128,7 @@ def c3_oracle():
     print(f"c3 solve {time.time() - t0:.0f} s, {info['iters']} iterations", flush=True)
     dc = adjoint.compliance_sens_closed_form(m, u, rho, mat)
     cells_s = np.sort(np.random.default_rng(SEED).choice(m.E, 4096, replace=False))
-    np.savez(os.path.join(HERE, "c3_oracle.npz"), cells=cells_s, dc=dc[cells_s], dc_max=np.abs(dc).max(),
+    np.savez(os.path.join(HERE, "simp_120x40x20_oracle.npz"), cells=cells_s, dc=dc[cells_s], dc_max=np.abs(dc).max(),
              compliance=adjoint.compliance(m, u, [faces], tr), iters=info["iters"])
 
 
@@ -137,7 +137,8 @@ if __name__ == "__main__":
     ap.add_argument("--large", action="store_true")
     ap.add_argument("--only", default=None)
     a = ap.parse_args()
-    jobs = [fixture_2x2x2, c1_samples, make_ke_3d, simp_4x2x2] + ([c2_oracle, c3_oracle] if a.large else [])
+    jobs = [fixture_2x2x2, cantilever_50x5x5_samples, make_ke_3d, simp_4x2x2] + (
+        [cantilever_200x40x40_oracle, simp_120x40x20_oracle] if a.large else [])
     for job in jobs:
         if a.only and job.__name__ != a.only:
             continue

@@ -32,9 +32,9 @@ def test_fixture_2x2x2_pattern_and_values():
 
 
 def test_c1_pattern_hash_matches_fixture():
-    f = _load("c1_samples.npz")
+    f = _load("cantilever_50x5x5_samples.npz")
     p, _, _ = cantilever((50, 5, 5), (2.0, 2.0, 2.0), 100.0)
-    assert p.csr_nse == int(f["nse"]) == 347904
+    assert p.csr_nse == int(f["nse"]) == 347904                       # nnz of feax's pattern, 50x5x5 box
     h1 = hashlib.sha256(p.csr_indptr.astype(np.int32).tobytes()).digest()
     h2 = hashlib.sha256(p.csr_indices.astype(np.int32).tobytes()).digest()
     assert h1 == bytes(f["indptr_sha256"]) and h2 == bytes(f["indices_sha256"])

@@ -1,7 +1,7 @@
 """GPU parity of the direct solver (banded Cholesky, fx_band_*).
 
 The factor against the matrix it factors (L L^T = P J P^T), the solution
-against scipy's direct solve of the oracle system (c1 and a box whose
+against scipy's direct solve of the oracle system (the 50x5x5 cantilever and a box whose
 bandwidth spans several factor blocks), gradients against the CG path, the
 RCM ordering of a shuffled mesh, and the non-positive-definite report.
 """
@@ -86,11 +86,11 @@ def test_band_factor_reproduces_the_matrix(case):
     assert (case == "shuffled") == (F.perm is not None)
 
 
-@pytest.mark.parametrize("case", ["c1", "grid12x8x8"])
+@pytest.mark.parametrize("case", ["cantilever50x5x5", "grid12x8x8"])
 def test_direct_solve_matches_scipy_direct_solve(case):
     import scipy.sparse.linalg as spla
     import feax_amd as fe
-    if case == "c1":
+    if case == "cantilever50x5x5":
         prob, bc, tp = cantilever((50, 5, 5), (2.0, 2.0, 2.0), 100.0)
         from oracle import hex8, krylov
         m = oracle_model(prob)
@@ -111,7 +111,7 @@ def test_direct_solve_matches_scipy_direct_solve(case):
     res_d = np.linalg.norm(K @ ud - b) / np.linalg.norm(b)
     print(case, "rel err vs scipy", err, "relres", res, "scipy's relres", res_d)
     # both are backward-stable direct solves: residuals at the rounding
-    # level of the (ill-conditioned, slender c1) system, errors ~ cond * eps
+    # level of the (ill-conditioned, slender 50x5x5) system, errors ~ cond * eps
     assert err < 1e-9 and res < 100 * res_d + 1e-14
     # a second call refactors cleanly; create_solver's default options are
     # this direct solver, as feax's are (feax/solver.py:293-301)

@@ -39,7 +39,7 @@ def test_c1_sampled_values_and_solution():
     import torch
     import feax_amd as fe
     from feax_amd.assembler import assemble_csr
-    f = _load("c1_samples.npz")
+    f = _load("cantilever_50x5x5_samples.npz")
     p, bc, tp = cantilever((50, 5, 5), (2.0, 2.0, 2.0), 100.0)
     ts = fe.TracedStructure.from_problem(p)
     data = assemble_csr(p, ts, tp, None, bc, True, torch.float64).data.cpu().numpy()
@@ -86,9 +86,9 @@ def test_simp_fixture_gradient():
 @pytest.mark.parametrize("pre", ["jacobi", "gmg-fp64", "gmg-mixed"])
 def test_c2_full_size_solution_matches_oracle_samples(pre):
     """200x40x40 (1,013,643 DOFs): sampled DOFs of the GPU solve vs the
-    oracle's Jacobi-PCG to rel 1e-12 (c2_oracle.npz)."""
+    oracle's Jacobi-PCG to rel 1e-12 (cantilever_200x40x40_oracle.npz)."""
     import feax_amd as fe
-    f = _load("c2_oracle.npz")
+    f = _load("cantilever_200x40x40_oracle.npz")
     p, bc, tp = cantilever((200, 40, 40), (1.0, 1.0, 1.0), 200.0)
     mg = fe.GeometricMultigrid() if pre.startswith("gmg") else None
     opts = fe.KrylovSolverOptions(solver="cg", tol=1e-11, atol=0.0, maxiter=20000, preconditioner=mg,
@@ -104,11 +104,11 @@ def test_c2_full_size_solution_matches_oracle_samples(pre):
 
 def test_c3_full_size_compliance_sensitivity_matches_oracle_samples():
     """120x40x20 SIMP cantilever: compliance and the fused dC/drho kernel
-    after a GMG solve vs the oracle's closed form (c3_oracle.npz)."""
+    after a GMG solve vs the oracle's closed form (simp_120x40x20_oracle.npz)."""
     import torch
     import feax_amd as fe
     from feax_amd import gene
-    f = _load("c3_oracle.npz")
+    f = _load("simp_120x40x20_oracle.npz")
     dims = (120, 40, 20)
 
     class Simp(fe.Problem):
@@ -130,6 +130,6 @@ def test_c3_full_size_compliance_sensitivity_matches_oracle_samples():
     C, dc = gene.compliance_and_sensitivity(prob, tp, u, surface_load_params=0.25)
     dc = dc.cpu().numpy()
     err = np.abs(dc[f["cells"]] - f["dc"]).max() / float(f["dc_max"])
-    print("c3 iters", solver.stats.iters, "dc err", err, "C", float(C), float(f["compliance"]))
+    print("simp 120x40x20 iters", solver.stats.iters, "dc err", err, "C", float(C), float(f["compliance"]))
     assert err < 1e-7
     assert abs(float(C) - float(f["compliance"])) <= 1e-9 * abs(float(f["compliance"]))

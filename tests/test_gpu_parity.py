@@ -17,8 +17,8 @@ def _np(t):
 
 
 @pytest.fixture(scope="module")
-def c1():
-    """Config c1: 50x5x5 HEX8, spacing 2, clamped x=0, traction on x=100."""
+def small_cantilever():
+    """50x5x5 HEX8, spacing 2, clamped x=0, traction on x=100."""
     from oracle import hex8, krylov
     problem, bc, tp = cantilever((50, 5, 5), (2.0, 2.0, 2.0), 100.0)
     m = oracle_model(problem)
@@ -38,20 +38,20 @@ def _row_scale(K):
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-def test_assembly_matches_oracle(c1, dtype):
+def test_assembly_matches_oracle(small_cantilever, dtype):
     import torch
     import feax_amd as fe
     from feax_amd.assembler import assemble_csr
-    p, bc, tp = c1["problem"], c1["bc"], c1["tp"]
+    p, bc, tp = small_cantilever["problem"], small_cantilever["bc"], small_cantilever["tp"]
     ts = fe.TracedStructure.from_problem(p)
     dt = torch.float64 if dtype == "f64" else torch.float32
     A = assemble_csr(p, ts, tp, None, bc, True, dt)
     torch.cuda.synchronize()
-    assert np.array_equal(_np(A.indptr), c1["m"].csr["indptr"])
-    assert np.array_equal(_np(A.indices), c1["m"].csr["indices"])
-    ref = c1["info"]["data_bc"]
+    assert np.array_equal(_np(A.indptr), small_cantilever["m"].csr["indptr"])
+    assert np.array_equal(_np(A.indices), small_cantilever["m"].csr["indices"])
+    ref = small_cantilever["info"]["data_bc"]
     got = _np(A.data).astype(np.float64)
-    scale = _row_scale(c1["info"]["K"])
+    scale = _row_scale(small_cantilever["info"]["K"])
     tol = (64 * EPS64) if dtype == "f64" else (64 * EPS32)
     err = np.abs(got - ref) / scale
     print(f"assembly {dtype}: max err / row max = {err.max():.3e} (tol {tol:.1e})")
@@ -84,11 +84,11 @@ def test_assembly_unbc_and_simp_general_geometry():
     assert err.max() <= 64 * EPS64, err.max()
 
 
-def test_assembly_bitwise_repeatable(c1):
+def test_assembly_bitwise_repeatable(small_cantilever):
     import torch
     import feax_amd as fe
     from feax_amd.assembler import assemble_csr
-    p, bc, tp = c1["problem"], c1["bc"], c1["tp"]
+    p, bc, tp = small_cantilever["problem"], small_cantilever["bc"], small_cantilever["tp"]
     ts = fe.TracedStructure.from_problem(p)
     a = _np(assemble_csr(p, ts, tp, None, bc).data)
     b = _np(assemble_csr(p, ts, tp, None, bc).data)
@@ -97,10 +97,10 @@ def test_assembly_bitwise_repeatable(c1):
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("kernel", ["csr", "block3"])
-def test_spmv_matches_scipy(c1, dtype, kernel):
+def test_spmv_matches_scipy(small_cantilever, dtype, kernel):
     import torch
     from feax_amd.csr import CSRMatrix
-    K = c1["info"]["K"]
+    K = small_cantilever["info"]["K"]
     dt = torch.float64 if dtype == "f64" else torch.float32
     npdt = np.float64 if dtype == "f64" else np.float32
     rng = np.random.default_rng(1)
@@ -119,18 +119,18 @@ def test_spmv_matches_scipy(c1, dtype, kernel):
     assert np.all(np.abs(y - yref) <= bound), np.max(np.abs(y - yref) / bound)
 
 
-def test_residual_matches_oracle(c1):
+def test_residual_matches_oracle(small_cantilever):
     import torch
     import feax_amd as fe
     from feax_amd.assembler import residual
-    p, bc, tp, m = c1["problem"], c1["bc"], c1["tp"], c1["m"]
+    p, bc, tp, m = small_cantilever["problem"], small_cantilever["bc"], small_cantilever["tp"], small_cantilever["m"]
     ts = fe.TracedStructure.from_problem(p)
     rng = np.random.default_rng(3)
     u = rng.standard_normal(m.n) * 1e-3
     r = _np(residual(p, ts, tp, torch.as_tensor(u, device="cuda")))
-    rref = m.residual(u, c1["E"], c1["mat"], c1["tr"])
-    K0 = m.scipy_csr(m.csr_values(u, c1["E"], c1["mat"]))
-    scale = abs(K0) @ np.abs(u) + np.abs(m.residual(np.zeros(m.n), c1["E"], c1["mat"], c1["tr"]))
+    rref = m.residual(u, small_cantilever["E"], small_cantilever["mat"], small_cantilever["tr"])
+    K0 = m.scipy_csr(m.csr_values(u, small_cantilever["E"], small_cantilever["mat"]))
+    scale = abs(K0) @ np.abs(u) + np.abs(m.residual(np.zeros(m.n), small_cantilever["E"], small_cantilever["mat"], small_cantilever["tr"]))
     assert np.all(np.abs(r - rref) <= 1e-12 * scale + 1e-300)
     # BC-applied and negated: b = -R_bc (feax/solvers/linear.py:400)
     b = _np(residual(p, ts, tp, torch.as_tensor(u, device="cuda"), bc, negate=True))
@@ -140,33 +140,33 @@ def test_residual_matches_oracle(c1):
 
 
 @pytest.mark.parametrize("spmv", ["csr", "block3"])
-def test_fp64_solve_matches_direct(c1, spmv):
+def test_fp64_solve_matches_direct(small_cantilever, spmv):
     import feax_amd as fe
-    p, bc, tp = c1["problem"], c1["bc"], c1["tp"]
+    p, bc, tp = small_cantilever["problem"], small_cantilever["bc"], small_cantilever["tp"]
     opts = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=0.0, maxiter=10000,
                                   use_jacobi_preconditioner=True, spmv=spmv)
     solver = fe.create_solver(p, bc, opts, linear=True, traced_params=tp)
     u = np.asarray(solver(tp, fe.zero_like_initial_guess(p, bc)))
-    err = np.linalg.norm(u - c1["u_dir"]) / np.linalg.norm(c1["u_dir"])
+    err = np.linalg.norm(u - small_cantilever["u_dir"]) / np.linalg.norm(small_cantilever["u_dir"])
     from oracle import krylov
-    K, b = c1["info"]["K"], c1["info"]["b"]
-    _, it_ref, _ = krylov.pcg(K, b, c1["info"]["x0"], 1e-10, 0.0, 10000, krylov.jacobi_dinv(K.diagonal()))
+    K, b = small_cantilever["info"]["K"], small_cantilever["info"]["b"]
+    _, it_ref, _ = krylov.pcg(K, b, small_cantilever["info"]["x0"], 1e-10, 0.0, 10000, krylov.jacobi_dinv(K.diagonal()))
     print("fp64", spmv, solver.stats, "err", err, "oracle iters", it_ref)
     assert err < 1e-8
     # same algorithm and stopping rule; FP rounding may move the count slightly
     assert abs(solver.stats.iters - it_ref) <= max(3, 0.02 * it_ref)
 
 
-def test_diff_form_operator_matches_oracle(c1):
+def test_diff_form_operator_matches_oracle(small_cantilever):
     """FP32 difference-form CSR applied to FP64 vectors equals K x up to FP32
     rounding of the entries acting on differences x_m - x_n."""
     import torch
     import feax_amd as fe
     from feax_amd.assembler import assemble_csr
-    p, bc, tp = c1["problem"], c1["bc"], c1["tp"]
+    p, bc, tp = small_cantilever["problem"], small_cantilever["bc"], small_cantilever["tp"]
     ts = fe.TracedStructure.from_problem(p)
     A, dinv = assemble_csr(p, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12)
-    K = c1["info"]["K"]
+    K = small_cantilever["info"]["K"]
     rng = np.random.default_rng(5)
     x = 1.0 + 1e-3 * rng.standard_normal(K.shape[0])      # large common translation + small variation
     y = _np(A.matvec(torch.as_tensor(x, device="cuda")))
@@ -177,7 +177,7 @@ def test_diff_form_operator_matches_oracle(c1):
     absdiff = np.abs(np.where(own, 0.0, K.data)) * np.abs(x[K.indices] - xn)
     # S (block row sums) of rows coupled to eliminated columns: from the
     # unconstrained oracle values, |S| <= sum of |entries| in BC columns.
-    full = c1["info"]["data"]
+    full = small_cantilever["info"]["data"]
     bcm = np.zeros(K.shape[0], bool)
     bcm[bc.bc_rows] = True
     srow = np.bincount(rows, np.where(bcm[K.indices] & ~bcm[rows], np.abs(full), 0.0) * np.abs(xn),
@@ -191,25 +191,25 @@ def test_diff_form_operator_matches_oracle(c1):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "mixed-fp32vec"])
-def test_mixed_solve_matches_direct(c1, precision):
+def test_mixed_solve_matches_direct(small_cantilever, precision):
     import feax_amd as fe
-    p, bc, tp = c1["problem"], c1["bc"], c1["tp"]
+    p, bc, tp = small_cantilever["problem"], small_cantilever["bc"], small_cantilever["tp"]
     opts = fe.KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=20000,
                                   use_jacobi_preconditioner=True, precision=precision)
     solver = fe.create_solver(p, bc, opts, linear=True, traced_params=tp)
     u = np.asarray(solver(tp, fe.zero_like_initial_guess(p, bc)))
-    err = np.linalg.norm(u - c1["u_dir"]) / np.linalg.norm(c1["u_dir"])
+    err = np.linalg.norm(u - small_cantilever["u_dir"]) / np.linalg.norm(small_cantilever["u_dir"])
     st = solver.stats
     print(precision, st, "err", err)
     assert st.relres <= 1e-8
     assert err < 1e-6
 
 
-def test_pcg_repeatable_and_maxiter(c1):
+def test_pcg_repeatable_and_maxiter(small_cantilever):
     import torch
     from feax_amd.csr import CSRMatrix
     from feax_amd.solvers.common import pcg
-    K, b = c1["info"]["K"], c1["info"]["b"]
+    K, b = small_cantilever["info"]["K"], small_cantilever["info"]["b"]
     A = CSRMatrix(torch.as_tensor(K.data, device="cuda"), torch.as_tensor(K.indptr.astype(np.int32), device="cuda"),
                   torch.as_tensor(K.indices.astype(np.int32), device="cuda"), K.shape, block3=True)
     bt = torch.as_tensor(b, device="cuda")
@@ -252,12 +252,12 @@ def test_krylov_projection_matches_numpy():
     assert float(rec.project(torch.as_tensor(r, device="cuda")).abs().max()) == 0.0
 
 
-def test_matfree_operator_solve_matches_direct_and_csr(c1):
+def test_matfree_operator_solve_matches_direct_and_csr(small_cantilever):
     """operator="matfree": feax's default Krylov path (CG on J_bc v, no
     preconditioner) == the direct solve; its CG iterates follow the plain CG
     on the assembled CSR (same operator up to rounding)."""
     import feax_amd as fe
-    p, bc, tp = c1["problem"], c1["bc"], c1["tp"]
+    p, bc, tp = small_cantilever["problem"], small_cantilever["bc"], small_cantilever["tp"]
     res = {}
     for op in ("matfree", "csr"):
         opts = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=0.0, maxiter=20000,
@@ -266,7 +266,7 @@ def test_matfree_operator_solve_matches_direct_and_csr(c1):
         u = np.asarray(solver(tp, fe.zero_like_initial_guess(p, bc)))
         res[op] = (u, solver.stats.iters)
     u, it = res["matfree"]
-    err = np.linalg.norm(u - c1["u_dir"]) / np.linalg.norm(c1["u_dir"])
+    err = np.linalg.norm(u - small_cantilever["u_dir"]) / np.linalg.norm(small_cantilever["u_dir"])
     print("matfree iters", it, "csr iters", res["csr"][1], "err", err)
     assert err < 1e-7
     assert abs(it - res["csr"][1]) <= max(3, it // 50)
@@ -350,19 +350,19 @@ def test_two_pass_and_single_pass_element_kernels_agree(kind, loc, monkeypatch):
 
 
 @pytest.mark.parametrize("spmv", ["block3", "csr"])
-def test_bicgstab_nonsymmetric_elimination_matches_oracle(c1, spmv):
+def test_bicgstab_nonsymmetric_elimination_matches_oracle(small_cantilever, spmv):
     """solver="bicgstab" with symmetric_elimination=False: the row-only
     Dirichlet elimination (a non-symmetric operator); the solution equals
     the direct solve and the iteration count the oracle's BiCGSTAB."""
     import feax_amd as fe
     from oracle import hex8, krylov
-    p, bc, tp = c1["problem"], c1["bc"], c1["tp"]
+    p, bc, tp = small_cantilever["problem"], small_cantilever["bc"], small_cantilever["tp"]
     opts = fe.KrylovSolverOptions(solver="bicgstab", tol=1e-10, atol=0.0, maxiter=10000,
                                   use_jacobi_preconditioner=True, spmv=spmv)
     solver = fe.create_solver(p, bc, opts, linear=True, symmetric_elimination=False)
     u = np.asarray(solver(tp, fe.zero_like_initial_guess(p, bc)))
-    err = np.linalg.norm(u - c1["u_dir"]) / np.linalg.norm(c1["u_dir"])
-    m, info = c1["m"], c1["info"]
+    err = np.linalg.norm(u - small_cantilever["u_dir"]) / np.linalg.norm(small_cantilever["u_dir"])
+    m, info = small_cantilever["m"], small_cantilever["info"]
     mask = np.zeros(m.n, bool)
     mask[bc.bc_rows] = True
     K = m.scipy_csr(hex8.apply_bc_csr(info["data"], m.csr, mask, bc.bc_rows, symmetric=False))

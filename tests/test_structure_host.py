@@ -31,13 +31,13 @@ def shuffled_mesh(dims, seed):
     return pts[perm], inv[cells][rng.permutation(len(cells))]
 
 
-@pytest.mark.parametrize("case", ["fixture", "c1", "odd", "shuffled"])
+@pytest.mark.parametrize("case", ["fixture", "cantilever50x5x5", "odd", "shuffled"])
 def test_native_pattern_matches_oracle_argsort(case):
     import feax_amd as fe
     if case == "fixture":
         pts, cells = st.structured_grid((2, 2, 2), (5.0, 5.0, 5.0))
         L = 10.0
-    elif case == "c1":
+    elif case == "cantilever50x5x5":
         pts, cells = st.structured_grid((50, 5, 5), (2.0, 2.0, 2.0))
         L = 100.0
     elif case == "odd":
