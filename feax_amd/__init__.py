@@ -8,8 +8,9 @@ gfx950); the package itself holds only host-side structure and dispatch.
 """
 
 from . import gene, materials
-from .assembler import (TracedStructure, create_energy_fn, create_J_bc_csr_parametric, create_res_bc_parametric,
-                        create_res_J_bc_csr_parametric, get_res)
+from .assembler import (TracedStructure, create_energy_fn, create_J_bc_csr_function, create_J_bc_csr_parametric,
+                        create_res_bc_function, create_res_bc_parametric, create_res_J_bc_csr_parametric,
+                        get_jacobian, get_jacobian_info, get_res)
 from .csr import CSRMatrix
 from .DCboundary import DirichletBC, DirichletBCConfig, DirichletBCSpec, dirichlet_bc_config
 from .mesh import Mesh, StructuredGrid

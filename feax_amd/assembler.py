@@ -444,6 +444,45 @@ def get_res(problem, sol_list, traced_params, ts=None):
     return problem.unflatten_fn_sol_list(residual(problem, ts, traced_params, flat))
 
 
+def get_jacobian(problem, sol_list, traced_params, ts=None):
+    """The assembled tangent without Dirichlet conditions as a CSRMatrix in
+    the problem's view (feax/assembler.py:1517-1566)."""
+    import torch
+    ts = _ts_or_build(problem, ts)
+    flat = sol_list[0].reshape(-1) if isinstance(sol_list, (list, tuple)) else sol_list
+    return view_csr(problem, ts, assemble_csr(problem, ts, traced_params, flat, None, True, torch.float64))
+
+
+def get_jacobian_info(problem, sol_list, traced_params):
+    """nnz, shape and matrix_view of the Jacobian without assembling it
+    (feax/assembler.py:1568-1616)."""
+    n = problem.num_total_dofs_all_vars
+    nnz = len(problem.view_pattern()[1]) if getattr(problem, "matrix_view", None) is not None and \
+        problem.matrix_view.name != "FULL" else int(problem.csr_nse)
+    return {"nnz": int(nnz), "shape": (n, n), "matrix_view": getattr(problem, "matrix_view", None)}
+
+
+def create_J_bc_csr_function(problem, bc, symmetric=True, traced_structure=None):
+    """(sol_flat, traced_params) -> the BC-applied Jacobian as a CSRMatrix
+    (feax/assembler.py:1670-1693)."""
+    import torch
+
+    def J_bc_func(sol_flat, traced_params):
+        ts = _ts_or_build(problem, traced_structure)
+        A = assemble_csr(problem, ts, traced_params, sol_flat, bc, symmetric, torch.float64)
+        return view_csr(problem, ts, A)
+    return J_bc_func
+
+
+def create_res_bc_function(problem, bc, traced_structure=None):
+    """(sol_flat, traced_params) -> the residual with the Dirichlet rows set
+    to u - bc_vals (feax/assembler.py:1797-1830)."""
+    def res_bc_func(sol_flat, traced_params):
+        ts = _ts_or_build(problem, traced_structure)
+        return residual(problem, ts, traced_params, sol_flat, bc)
+    return res_bc_func
+
+
 def view_csr(problem, ts, A):
     """The problem's matrix_view of a FULL CSRMatrix: the UPPER (j >= i) or
     LOWER (j <= i) slots of the pattern, i.e. what feax assembles for that

@@ -557,3 +557,27 @@ def test_gmres_gradient_nonsymmetric_elimination_matches_cg():
     err = np.abs(grads[0] - grads[1]).max() / np.abs(grads[1]).max()
     print("gradient rel diff gmres/row-only vs cg/symmetric", err)
     assert err < 1e-7
+
+
+def test_public_assembly_functions(small_cantilever):
+    """get_jacobian (no Dirichlet conditions), create_J_bc_csr_function and
+    create_res_bc_function against the oracle's matrices and residual."""
+    import torch
+    import feax_amd as fe
+    from oracle import hex8
+    p, bc, tp, m = small_cantilever["problem"], small_cantilever["bc"], small_cantilever["tp"], small_cantilever["m"]
+    mat = small_cantilever["mat"]
+    E = small_cantilever["E"]
+    u = 1e-3 * np.random.default_rng(8).standard_normal(m.n)
+    print("public assembly functions", flush=True)
+    K = fe.get_jacobian(p, [u.reshape(-1, 3)], tp)
+    ref = m.csr_values(u, E, mat)
+    assert np.abs(_np(K.data) - ref).max() <= 1e-13 * np.abs(ref).max()
+    Jbc = fe.create_J_bc_csr_function(p, bc)(torch.as_tensor(u, device="cuda"), tp)
+    mask = np.zeros(m.n, bool)
+    mask[bc.bc_rows] = True
+    ref_bc = hex8.apply_bc_csr(ref, m.csr, mask, bc.bc_rows, True)
+    assert np.abs(_np(Jbc.data) - ref_bc).max() <= 1e-13 * np.abs(ref_bc).max()
+    r = _np(fe.create_res_bc_function(p, bc)(torch.as_tensor(u, device="cuda"), tp))
+    rref = hex8.apply_bc_res(m.residual(u, E, mat, small_cantilever["tr"]), u, bc.bc_rows, bc.bc_vals)
+    assert np.abs(r - rref).max() <= 1e-12 * np.abs(rref).max()

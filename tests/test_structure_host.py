@@ -252,3 +252,15 @@ def test_solution_behaves_as_the_flat_dof_vector():
     assert torch.equal(torch.stack([s, d]), torch.stack([d, d]))
     assert s.field(0).shape == (4, 3) and torch.equal(s.node_var(1), d.reshape(4, 3)[:, 1])
     assert np.array_equal(np.asarray(s), d.numpy()) and s.block_until_ready() is s
+
+
+def test_jacobian_info_without_assembly():
+    import feax_amd as fe
+    mesh = fe.StructuredGrid((3, 2, 2)).to_mesh()
+    p = fe.Problem(mesh, vec=3, dim=3)
+    info = fe.get_jacobian_info(p, [np.zeros((p.num_total_nodes, 3))], None)
+    assert info["shape"] == (p.num_total_dofs_all_vars,) * 2 and info["nnz"] == p.csr_nse
+    assert info["matrix_view"] == fe.MatrixView.FULL
+    pu = fe.Problem(mesh, vec=3, dim=3, matrix_view=fe.MatrixView.UPPER)
+    n = pu.num_total_dofs_all_vars
+    assert fe.get_jacobian_info(pu, None, None)["nnz"] == (p.csr_nse + n) // 2
