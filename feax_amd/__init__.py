@@ -23,3 +23,7 @@ from .traced_params import TracedParams
 from .utils import zero_like_initial_guess
 
 __version__ = "0.1.0"
+from .solvers.common import (check_convergence, create_direct_solve_fn, create_iterative_solve_fn,
+                             create_jacobi_preconditioner, create_linear_solve_fn, create_x0)
+from .solvers.linear import create_linear_solver, linear_solve
+from .solvers.options import AbstractSolverOptions, resolve_iterative_solver

@@ -348,3 +348,48 @@ def create_iterative_solve_fn(options):
         return x
     solve.last_info = None
     return solve
+
+
+class _Pattern:
+    """A CSR pattern standing in for a Problem in band_ordering (one DOF per
+    graph node)."""
+
+    def __init__(self, A):
+        self.csr_indptr = A.indptr.detach().cpu().numpy()
+        self.csr_indices = A.indices.detach().cpu().numpy()
+        self.vec = 1
+
+
+def create_direct_solve_fn(options, *, cache_namespace="global"):
+    """solve(A, b, x0=None) -> x for DirectSolverOptions
+    (feax/solvers/common.py:115-210): the device banded factorisation of the
+    symmetric FP64 CSRMatrix A (Cholesky, or L D L^T without pivoting),
+    ordered by reverse Cuthill-McKee on A's own graph; the factorisation of
+    the last pattern is reused while A's values are unchanged only under
+    reuse_factorization."""
+    from .direct import BandCholesky
+    state = {}
+
+    def solve(A, b, x0=None):
+        key = (A.indptr.data_ptr(), A.indices.data_ptr(), A.shape)
+        if state.get("key") != key:
+            state.update(key=key, chol=BandCholesky(_Pattern(A), A.data.device), data=None)
+        chol = state["chol"]
+        same = options.reuse_factorization and state["data"] is A.data and chol.valid()
+        if not same:
+            chol.factor(A)
+            state["data"] = A.data
+        return chol.solve(b)
+    return solve
+
+
+def create_linear_solve_fn(solver_options, *, cache_namespace="global"):
+    """The solve(A, b, x0) of DirectSolverOptions or KrylovSolverOptions
+    (feax/solvers/common.py:273-296)."""
+    from .options import DirectSolverOptions, KrylovSolverOptions
+    if isinstance(solver_options, DirectSolverOptions):
+        return create_direct_solve_fn(solver_options, cache_namespace=cache_namespace)
+    if isinstance(solver_options, KrylovSolverOptions):
+        return create_iterative_solve_fn(solver_options)
+    raise TypeError("Unsupported solver option type. Expected DirectSolverOptions or KrylovSolverOptions, got "
+                    f"{type(solver_options).__name__}.")

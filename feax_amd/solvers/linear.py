@@ -213,6 +213,29 @@ def _dev_vec(a, ts):
     return torch.as_tensor(np.ascontiguousarray(a, np.float64), device=ts.device)
 
 
+def linear_solve(J_bc_applied, res_bc_applied, initial_guess, bc, solver_options, matrix_view=None,
+                 traced_params=None, P_mat=None, linear_solve_fn=None, x0_fn=None):
+    """One linear step u = u0 + J_bc^-1 (-R_bc(u0)) from user-supplied
+    J_bc / R_bc functions (feax/solvers/linear.py:119-160; P_mat, the
+    reduced-space prolongation, is not on this path)."""
+    from .common import create_linear_solve_fn
+    if P_mat is not None:
+        raise NotImplementedError("prolongation-reduced solves (P_mat) are not on the HIP path")
+    if linear_solve_fn is None:
+        linear_solve_fn = create_linear_solve_fn(solver_options)
+    args = (initial_guess, traced_params) if traced_params is not None else (initial_guess,)
+    res = res_bc_applied(*args)
+    J = J_bc_applied(*args)
+    b = -res
+    x0 = None
+    if solver_options.uses_x0():
+        x0 = (x0_fn or create_x0(bc_rows=bc.bc_rows, bc_vals=bc.bc_vals))(initial_guess)
+    du = linear_solve_fn(J, b, x0)
+    if solver_options.check_convergence:
+        du = check_convergence(J, du, b, solver_options, matrix_view, "Linear solver")
+    return initial_guess + du, None
+
+
 def create_linear_solver(problem, bc, solver_options=None, adjoint_solver_options=None, traced_params=None,
                          symmetric_elimination=True, traced_structure=None):
     if solver_options is None:

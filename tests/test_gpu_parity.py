@@ -581,3 +581,24 @@ def test_public_assembly_functions(small_cantilever):
     r = _np(fe.create_res_bc_function(p, bc)(torch.as_tensor(u, device="cuda"), tp))
     rref = hex8.apply_bc_res(m.residual(u, E, mat, small_cantilever["tr"]), u, bc.bc_rows, bc.bc_vals)
     assert np.abs(r - rref).max() <= 1e-12 * np.abs(rref).max()
+
+
+@pytest.mark.parametrize("opts", ["cg", "direct"])
+def test_linear_solve_from_user_functions(small_cantilever, opts):
+    """feax's linear_solve(J_bc, R_bc, u0, bc, options) with the public
+    assembly functions and create_linear_solve_fn (Krylov or direct) equals
+    scipy's direct solve of the oracle system."""
+    import torch
+    import feax_amd as fe
+    p, bc, tp = small_cantilever["problem"], small_cantilever["bc"], small_cantilever["tp"]
+    o = fe.DirectSolverOptions() if opts == "direct" else \
+        fe.resolve_iterative_solver(fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0,
+                                                           use_jacobi_preconditioner=True), fe.MatrixProperty.SPD)
+    u0 = fe.zero_like_initial_guess(p, bc, device="cuda")
+    print("linear_solve", opts, flush=True)
+    u, _ = fe.linear_solve(fe.create_J_bc_csr_function(p, bc), fe.create_res_bc_function(p, bc), u0, bc, o,
+                           p.matrix_view, traced_params=tp)
+    ud = small_cantilever["u_dir"]
+    err = np.linalg.norm(_np(u) - ud) / np.linalg.norm(ud)
+    print(opts, "linear_solve err vs scipy", err)
+    assert err < 1e-8
