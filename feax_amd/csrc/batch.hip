@@ -289,10 +289,18 @@ __global__ __launch_bounds__(kSegBlock) void seg_proj_combine_kernel(const Block
   const BlockRange br = vtab[blockIdx.x];
   const int K = cnt[br.seg];
   for (int i = br.begin + threadIdx.x; i < br.end; i += kSegBlock) {
-    double v = 0.0;
-    for (int k = 0; k < K; ++k)
-      v += c[static_cast<size_t>(k) * nseg + br.seg] * static_cast<double>(P[static_cast<size_t>(k) * n + i]);
-    x0[i] = v;
+    // four independent partial sums (more loads in flight), added in a fixed order
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    int k = 0;
+    for (; k + 3 < K; k += 4) {
+      v0 += c[static_cast<size_t>(k) * nseg + br.seg] * static_cast<double>(P[static_cast<size_t>(k) * n + i]);
+      v1 += c[static_cast<size_t>(k + 1) * nseg + br.seg] * static_cast<double>(P[static_cast<size_t>(k + 1) * n + i]);
+      v2 += c[static_cast<size_t>(k + 2) * nseg + br.seg] * static_cast<double>(P[static_cast<size_t>(k + 2) * n + i]);
+      v3 += c[static_cast<size_t>(k + 3) * nseg + br.seg] * static_cast<double>(P[static_cast<size_t>(k + 3) * n + i]);
+    }
+    for (; k < K; ++k)
+      v0 += c[static_cast<size_t>(k) * nseg + br.seg] * static_cast<double>(P[static_cast<size_t>(k) * n + i]);
+    x0[i] = (v0 + v1) + (v2 + v3);
   }
 }
 

@@ -586,9 +586,17 @@ __global__ __launch_bounds__(kBlock) void proj_combine_kernel(int n, int nvec, c
                                                               const double* __restrict__ c, double* __restrict__ x0) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  double s = 0.0;
-  for (int k = 0; k < nvec; ++k) s += c[k] * static_cast<double>(ldnt(P + static_cast<size_t>(k) * n + i));
-  x0[i] = s;
+  // four independent partial sums (more loads in flight), added in a fixed order
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int k = 0;
+  for (; k + 3 < nvec; k += 4) {
+    s0 += c[k] * static_cast<double>(ldnt(P + static_cast<size_t>(k) * n + i));
+    s1 += c[k + 1] * static_cast<double>(ldnt(P + static_cast<size_t>(k + 1) * n + i));
+    s2 += c[k + 2] * static_cast<double>(ldnt(P + static_cast<size_t>(k + 2) * n + i));
+    s3 += c[k + 3] * static_cast<double>(ldnt(P + static_cast<size_t>(k + 3) * n + i));
+  }
+  for (; k < nvec; ++k) s0 += c[k] * static_cast<double>(ldnt(P + static_cast<size_t>(k) * n + i));
+  x0[i] = (s0 + s1) + (s2 + s3);
 }
 
 // x0 = 0 except x0[bc_rows] = bc_vals - u0[bc_rows] (feax/solvers/common.py:23-53).
