@@ -401,6 +401,178 @@ __global__ __launch_bounds__(256) void quad_traction_vjp_kernel(int num_faces, c
   out[2 * f + qq] = scale * acc * nanson;
 }
 
+// ---------------------------------------------------------------- scalar QUAD4
+// vec = 1 conduction in the plane (the scalar Problem of feax with flux
+// k grad T on QUAD4): K_ab = sum_q k_q grad N_a . grad N_b JxW_q, R_a =
+// sum_q k_q grad N_a . grad T_q JxW_q, edge flux + sum_q q N_a nanson;
+// k = p or the SIMP interpolation (FX_MAT_SIMP_CONDUCTION).
+__device__ __forceinline__ double qs_conductivity(const MatParams& m, double p, double* dkdp) {
+  if (m.kind == FX_MAT_SIMP_CONDUCTION) {
+    const double span = m.E0 - m.Emin;
+    if (dkdp) *dkdp = m.penal * pow(p, m.penal - 1.0) * span;
+    return m.Emin + pow(p, m.penal) * span;
+  }
+  if (dkdp) *dkdp = 1.0;
+  return p;
+}
+
+__device__ __forceinline__ void qs_grad(const double* __restrict__ f, const uint8_t* __restrict__ mask,
+                                        const int* __restrict__ cn, const double (&g)[4][3], double& G0, double& G1) {
+  G0 = G1 = 0.0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double v = (mask && mask[cn[c]]) ? 0.0 : f[cn[c]];
+    G0 += v * g[c][0];
+    G1 += v * g[c][1];
+  }
+}
+
+// One thread per (element, a, b): ke[16 e + 4 a + b].
+__global__ __launch_bounds__(kQuadThreads) void qs_elem_matrix_kernel(int num_cells, const double* __restrict__ x,
+                                                                      const double* __restrict__ y,
+                                                                      const int* __restrict__ cells, MatParams mat,
+                                                                      const double* __restrict__ param,
+                                                                      double* __restrict__ ke) {
+  const long t = static_cast<long>(blockIdx.x) * kQuadThreads + threadIdx.x;
+  const long e = t >> 4;
+  if (e >= num_cells) return;
+  const int a = static_cast<int>((t >> 2) & 3), b = static_cast<int>(t & 3);
+  const int* cn = cells + 4 * e;
+  double kq[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double s, tt, g[4][3];
+    quad_gauss(q, s, tt);
+    const double w = quad_grads(x, y, cn, s, tt, g) * 0.25;
+    const double k = qs_conductivity(mat, quad_param(mat, param, static_cast<int>(e), cn, s, tt), nullptr);
+    kq[q] = k * w * (g[a][0] * g[b][0] + g[a][1] * g[b][1]);
+  }
+  ke[16 * e + 4 * a + b] = (kq[0] + kq[1]) + (kq[2] + kq[3]);
+}
+
+// One thread per element: re[4 e + a].
+__global__ __launch_bounds__(kQuadThreads) void qs_elem_residual_kernel(int num_cells, const double* __restrict__ x,
+                                                                        const double* __restrict__ y,
+                                                                        const int* __restrict__ cells, MatParams mat,
+                                                                        const double* __restrict__ param,
+                                                                        const double* __restrict__ u,
+                                                                        double* __restrict__ re) {
+  const int e = blockIdx.x * kQuadThreads + threadIdx.x;
+  if (e >= num_cells) return;
+  const int* cn = cells + 4 * static_cast<long>(e);
+  double rq[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double s, t, g[4][3], G0, G1;
+    quad_gauss(q, s, t);
+    const double w = quad_grads(x, y, cn, s, t, g) * 0.25;
+    qs_grad(u, nullptr, cn, g, G0, G1);
+    const double kw = qs_conductivity(mat, quad_param(mat, param, e, cn, s, t), nullptr) * w;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) rq[q][a] = kw * (G0 * g[a][0] + G1 * g[a][1]);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) re[4 * static_cast<long>(e) + a] = (rq[0][a] + rq[1][a]) + (rq[2][a] + rq[3][a]);
+}
+
+// One thread per node: element residuals in ascending order, then the
+// loaded edges: flux(f, q) = fdir[f] * svar[2 f + q].
+__global__ __launch_bounds__(256) void qs_node_residual_kernel(
+    int num_nodes, const double* __restrict__ x, const double* __restrict__ y, const int* __restrict__ cells,
+    const int* __restrict__ inc_ptr, const int* __restrict__ inc, const double* __restrict__ re,
+    const int* __restrict__ faces, const int* __restrict__ finc_ptr, const int* __restrict__ finc,
+    const double* __restrict__ svar, const double* __restrict__ fdir, int negate, double* __restrict__ res) {
+  const int nd = blockIdx.x * 256 + threadIdx.x;
+  if (nd >= num_nodes) return;
+  double r = 0.0;
+  for (int k = inc_ptr[nd]; k < inc_ptr[nd + 1]; ++k) {
+    const int pk = inc[k];
+    r += re[4 * static_cast<long>(pk >> 3) + (pk & 7)];
+  }
+  if (finc_ptr) {
+    for (int k = finc_ptr[nd]; k < finc_ptr[nd + 1]; ++k) {
+      const int pk = finc[k];
+      const int f = pk >> 3, a = pk & 7;
+      const int e = faces[2 * f], lf = faces[2 * f + 1];
+      const int* cn = cells + 4 * static_cast<long>(e);
+      double vq[2];
+      for (int qq = 0; qq < 2; ++qq) {
+        double s, t, N, ds, dt;
+        const double nanson = quad_edge_point(x, y, cn, lf, qq, s, t);
+        quad_shape(a, s, t, N, ds, dt);
+        vq[qq] = svar[2 * f + qq] * N * nanson;
+      }
+      r += fdir[f] * (vq[0] + vq[1]);
+    }
+  }
+  res[nd] = negate ? -r : r;
+}
+
+// Pullbacks per (element, Gauss point) w_eq = dk/dp grad lambda . grad T
+// JxW (SELF: lambda = T); cell variables: scale * sum_q, else we[4 e + q].
+template <bool SELF>
+__global__ __launch_bounds__(kQuadThreads) void qs_param_vjp_kernel(
+    int num_cells, const double* __restrict__ x, const double* __restrict__ y, const int* __restrict__ cells,
+    MatParams mat, const double* __restrict__ param, const double* __restrict__ u, const double* __restrict__ lam,
+    const uint8_t* __restrict__ bc_mask, double scale, double* __restrict__ out_cell, double* __restrict__ we) {
+  const int e = blockIdx.x * kQuadThreads + threadIdx.x;
+  if (e >= num_cells) return;
+  const int* cn = cells + 4 * static_cast<long>(e);
+  double wq[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double s, t, g[4][3], G0, G1, L0, L1;
+    quad_gauss(q, s, t);
+    const double w = quad_grads(x, y, cn, s, t, g) * 0.25;
+    qs_grad(u, nullptr, cn, g, G0, G1);
+    if constexpr (SELF) {
+      L0 = G0;
+      L1 = G1;
+    } else {
+      qs_grad(lam, bc_mask, cn, g, L0, L1);
+    }
+    double dkdp;
+    qs_conductivity(mat, quad_param(mat, param, e, cn, s, t), &dkdp);
+    wq[q] = dkdp * w * (G0 * L0 + G1 * L1);
+  }
+  if (out_cell) {
+    out_cell[e] = scale * ((wq[0] + wq[1]) + (wq[2] + wq[3]));
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) we[4 * static_cast<long>(e) + q] = wq[q];
+  }
+}
+
+// grad_svar[2 f + q] = scale * nanson_q * fdir[f] * sum_a N_a(q) lambda_a.
+__global__ __launch_bounds__(256) void qs_flux_vjp_kernel(int num_faces, const double* __restrict__ x,
+                                                          const double* __restrict__ y, const int* __restrict__ cells,
+                                                          const int* __restrict__ faces,
+                                                          const double* __restrict__ fdir,
+                                                          const double* __restrict__ lam,
+                                                          const uint8_t* __restrict__ bc_mask, double scale,
+                                                          double* __restrict__ out) {
+  const long t = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+  const long f = t >> 1;
+  const int qq = static_cast<int>(t & 1);
+  if (f >= num_faces) return;
+  const int e = faces[2 * f], lf = faces[2 * f + 1];
+  const int* cn = cells + 4 * static_cast<long>(e);
+  double s, tt;
+  const double nanson = quad_edge_point(x, y, cn, lf, qq, s, tt);
+  double acc = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    double N, ds, dt;
+    quad_shape(a, s, tt, N, ds, dt);
+    acc += N * ((bc_mask && bc_mask[cn[a]]) ? 0.0 : lam[cn[a]]);
+  }
+  out[2 * f + qq] = scale * fdir[f] * acc * nanson;
+}
+
+inline bool qs_law(const fx_material* m) {
+  return m->kind == FX_MAT_CONDUCTION || m->kind == FX_MAT_SIMP_CONDUCTION;
+}
+
 inline size_t quad_matrix_bytes(int num_cells, int value_bytes, int flags) {
   const size_t c = (value_bytes == 4 && (flags & FX_ASM_COMPUTE32)) ? 4 : 8;
   return 64 * c * static_cast<size_t>(num_cells);
@@ -583,6 +755,174 @@ int fx_quad4_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, 
   hipLaunchKernelGGL(quad_traction_vjp_kernel, dim3(static_cast<int>((threads + 255) / 256)), dim3(256), 0,
                      as_stream(stream), num_faces, x, y, cells, faces, fdir, lam, bc_mask, scale, grad_svar);
   FX_LAUNCHED("quad_traction_vjp_kernel");
+  return FX_OK;
+}
+
+size_t fx_quad4s_assemble_workspace_bytes(int32_t num_cells, int32_t value_bytes, int32_t flags) {
+  (void)value_bytes;
+  (void)flags;
+  return num_cells < 0 ? 0 : 16 * sizeof(double) * static_cast<size_t>(num_cells);
+}
+
+}  // extern "C"
+
+namespace fx {
+namespace {
+
+template <typename T>
+int qs_assemble(void* stream, int num_nodes, int num_cells, const double* x, const double* y,
+                       const int* cells, const int* inc_ptr, const int* inc, const uint8_t* rank, const int* indptr,
+                       const int* indices, const fx_material* mat, const double* param, const uint8_t* bc_mask,
+                       int symmetric, int flags, double jacobi_shift, double* dinv, T* data, void* workspace,
+                       size_t ws_bytes) {
+  if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !cells || !inc_ptr || !inc || !rank || !indptr || !indices ||
+      !mat || !param || !data)
+    return fail(FX_EINVAL, "fx_quad4s_assemble: bad arguments");
+  if (!qs_law(mat) || mat->temperature) return fail(FX_EINVAL, "fx_quad4s_assemble: needs a conduction law");
+  if (flags & (FX_ASM_DIFF3 | FX_ASM_COMPUTE32))
+    return fail(FX_EINVAL, "fx_quad4s_assemble: the difference form and FP32 arithmetic are built for vec = 3");
+  const size_t need = fx_quad4s_assemble_workspace_bytes(num_cells, sizeof(T), flags);
+  if (!workspace || ws_bytes < need)
+    return fail(FX_EWORKSPACE, "fx_quad4s_assemble: workspace %zu < %zu", workspace ? ws_bytes : 0, need);
+  hipStream_t s = as_stream(stream);
+  double* ke = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(qs_elem_matrix_kernel, dim3(static_cast<int>((16L * num_cells + kQuadThreads - 1) / kQuadThreads)),
+                     dim3(kQuadThreads), 0, s, num_cells, x, y, cells, to_params(mat), param, ke);
+  hipLaunchKernelGGL((gather_rows_kernel<T, double, 4, 1>), dim3((num_nodes + kNodesPerBlock - 1) / kNodesPerBlock),
+                     dim3(64 * kNodesPerBlock), 0, s, num_nodes, inc_ptr, inc, rank, indptr, indices, ke, bc_mask,
+                     symmetric, 0, jacobi_shift, dinv, data);
+  FX_LAUNCHED("fx_quad4s_assemble");
+  return FX_OK;
+}
+
+}  // namespace
+}  // namespace fx
+
+extern "C" {
+
+int fx_quad4s_assemble_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                           const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                           const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                           const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                           int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, double* data,
+                           void* workspace, size_t ws_bytes) {
+  (void)z;
+  (void)u;
+  return qs_assemble<double>(stream, num_nodes, num_cells, x, y, cells, inc_ptr, inc, elem_rank, indptr, indices, mat,
+                             param, bc_mask, symmetric, flags, jacobi_shift, dinv, data, workspace, ws_bytes);
+}
+
+int fx_quad4s_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                           const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                           const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                           const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                           int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, float* data,
+                           void* workspace, size_t ws_bytes) {
+  (void)z;
+  (void)u;
+  return qs_assemble<float>(stream, num_nodes, num_cells, x, y, cells, inc_ptr, inc, elem_rank, indptr, indices, mat,
+                            param, bc_mask, symmetric, flags, jacobi_shift, dinv, data, workspace, ws_bytes);
+}
+
+size_t fx_quad4s_residual_workspace_bytes(int32_t num_cells) {
+  return num_cells < 0 ? 0 : 4 * sizeof(double) * static_cast<size_t>(num_cells);
+}
+
+int fx_quad4s_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                           const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                           const fx_material* mat, const double* param, const double* u, int32_t num_faces,
+                           const int32_t* faces, const int32_t* finc_ptr, const int32_t* finc, const double* svar,
+                           const double* fdir, int32_t num_bc, const int32_t* bc_rows, const double* bc_vals,
+                           int32_t negate, double* res, void* workspace, size_t ws_bytes) {
+  (void)z;
+  if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !cells || !inc_ptr || !inc || !mat || !param || !u || !res)
+    return fail(FX_EINVAL, "fx_quad4s_residual_f64: bad arguments");
+  if (!qs_law(mat) || mat->temperature) return fail(FX_EINVAL, "fx_quad4s_residual_f64: needs a conduction law");
+  if (num_faces > 0 && (!faces || !finc_ptr || !finc || !svar || !fdir))
+    return fail(FX_EINVAL, "fx_quad4s_residual_f64: face arrays missing");
+  const size_t need = fx_quad4s_residual_workspace_bytes(num_cells);
+  if (!workspace || ws_bytes < need)
+    return fail(FX_EWORKSPACE, "fx_quad4s_residual_f64: workspace %zu < %zu", workspace ? ws_bytes : 0, need);
+  hipStream_t s = as_stream(stream);
+  const bool hf = num_faces > 0;
+  double* re = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(qs_elem_residual_kernel, dim3((num_cells + kQuadThreads - 1) / kQuadThreads), dim3(kQuadThreads),
+                     0, s, num_cells, x, y, cells, to_params(mat), param, u, re);
+  hipLaunchKernelGGL(qs_node_residual_kernel, dim3((num_nodes + 255) / 256), dim3(256), 0, s, num_nodes, x, y, cells,
+                     inc_ptr, inc, re, hf ? faces : nullptr, hf ? finc_ptr : nullptr, hf ? finc : nullptr, svar, fdir,
+                     negate, res);
+  FX_LAUNCHED("fx_quad4s_residual_f64");
+  if (num_bc > 0 && bc_rows) {
+    if (!bc_vals) return fail(FX_EINVAL, "fx_quad4s_residual_f64: bc_vals missing");
+    hipLaunchKernelGGL(bc_res_kernel, dim3((num_bc + 255) / 256), dim3(256), 0, s, num_bc, bc_rows, bc_vals, u,
+                       negate, res);
+    FX_LAUNCHED("bc_res_kernel");
+  }
+  return FX_OK;
+}
+
+size_t fx_quad4s_param_vjp_workspace_bytes(int32_t num_cells, int32_t param_loc) {
+  if (num_cells < 0) return 0;
+  return param_loc == FX_PARAM_NODE ? static_cast<size_t>(num_cells) * 4 * sizeof(double) : 0;
+}
+
+int fx_quad4s_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                            const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                            const fx_material* mat, const double* param, const double* u, const double* lam,
+                            const uint8_t* bc_mask, double scale, double* grad, void* workspace, size_t ws_bytes) {
+  (void)z;
+  if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !cells || !mat || !param || !u || !lam || !grad)
+    return fail(FX_EINVAL, "fx_quad4s_param_vjp_f64: bad arguments");
+  if (!qs_law(mat) || mat->temperature) return fail(FX_EINVAL, "fx_quad4s_param_vjp_f64: needs a conduction law");
+  if (mat->param_loc != FX_PARAM_NODE && mat->param_loc != FX_PARAM_CELL)
+    return fail(FX_EINVAL, "fx_quad4s_param_vjp_f64: unknown parameter location %d", mat->param_loc);
+  hipStream_t s = as_stream(stream);
+  const MatParams mp = to_params(mat);
+  const dim3 eg((num_cells + kQuadThreads - 1) / kQuadThreads);
+  if (mat->param_loc == FX_PARAM_CELL) {
+    hipLaunchKernelGGL(qs_param_vjp_kernel<false>, eg, dim3(kQuadThreads), 0, s, num_cells, x, y, cells, mp, param, u,
+                       lam, bc_mask, scale, grad, nullptr);
+    FX_LAUNCHED("qs_param_vjp_kernel");
+    return FX_OK;
+  }
+  if (!inc_ptr || !inc) return fail(FX_EINVAL, "fx_quad4s_param_vjp_f64: node variables need the incidence");
+  const size_t need = fx_quad4s_param_vjp_workspace_bytes(num_cells, FX_PARAM_NODE);
+  if (!workspace || ws_bytes < need)
+    return fail(FX_EWORKSPACE, "fx_quad4s_param_vjp_f64: workspace %zu < %zu bytes", ws_bytes, need);
+  double* we = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(qs_param_vjp_kernel<false>, eg, dim3(kQuadThreads), 0, s, num_cells, x, y, cells, mp, param, u,
+                     lam, bc_mask, 1.0, nullptr, we);
+  hipLaunchKernelGGL(quad_param_vjp_node_kernel, dim3((num_nodes + 255) / 256), dim3(256), 0, s, num_nodes, inc_ptr,
+                     inc, we, scale, grad);
+  FX_LAUNCHED("quad_param_vjp_node_kernel");
+  return FX_OK;
+}
+
+int fx_quad4s_simp_compliance_sens_f64(void* stream, int32_t num_cells, const double* x, const double* y,
+                                       const double* z, const int32_t* cells, const fx_material* mat,
+                                       const double* param, const double* u, double* dc) {
+  (void)z;
+  if (num_cells <= 0 || !x || !y || !cells || !mat || !param || !u || !dc)
+    return fail(FX_EINVAL, "fx_quad4s_simp_compliance_sens_f64: bad arguments");
+  if (!qs_law(mat) || mat->temperature || mat->param_loc != FX_PARAM_CELL)
+    return fail(FX_EINVAL, "fx_quad4s_simp_compliance_sens_f64: needs a conduction law with a cell variable");
+  hipLaunchKernelGGL(qs_param_vjp_kernel<true>, dim3((num_cells + kQuadThreads - 1) / kQuadThreads),
+                     dim3(kQuadThreads), 0, as_stream(stream), num_cells, x, y, cells, to_params(mat), param, u,
+                     nullptr, nullptr, -1.0, dc, nullptr);
+  FX_LAUNCHED("qs_param_vjp_kernel");
+  return FX_OK;
+}
+
+int fx_quad4s_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, const double* y, const double* z,
+                               const int32_t* cells, const int32_t* faces, const double* fdir, const double* lam,
+                               const uint8_t* bc_mask, double scale, double* grad_svar) {
+  (void)z;
+  if (num_faces < 0 || (num_faces > 0 && (!x || !y || !cells || !faces || !fdir || !lam || !grad_svar)))
+    return fail(FX_EINVAL, "fx_quad4s_traction_vjp_f64: bad arguments");
+  if (num_faces == 0) return FX_OK;
+  hipLaunchKernelGGL(qs_flux_vjp_kernel, dim3(static_cast<int>((2L * num_faces + 255) / 256)), dim3(256), 0,
+                     as_stream(stream), num_faces, x, y, cells, faces, fdir, lam, bc_mask, scale, grad_svar);
+  FX_LAUNCHED("qs_flux_vjp_kernel");
   return FX_OK;
 }
 

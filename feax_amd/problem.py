@@ -81,7 +81,8 @@ class Problem:
             self.gauss_order = self.gauss_order[0] if isinstance(self.gauss_order, list) else self.gauss_order
         ok = ((self.ele_type in ("HEX8", "TET4") and self.vec == 3 and self.dim == 3)
               or (self.ele_type == "QUAD4" and self.vec == 2 and self.dim == 2)
-              or (self.ele_type == "HEX8" and self.vec == 1 and self.dim == 3))
+              or (self.ele_type == "HEX8" and self.vec == 1 and self.dim == 3)
+              or (self.ele_type == "QUAD4" and self.vec == 1 and self.dim == 2))
         if not ok:
             raise NotImplementedError(
                 f"the HIP path covers HEX8 and TET4 with vec=3, dim=3, QUAD4 with vec=2, dim=2 and scalar HEX8 "
@@ -107,10 +108,10 @@ class Problem:
 
     @property
     def kernel_tag(self):
-        """Name of the native kernel family: hex8, tet4, quad4, or hex8s
-        (scalar HEX8, vec = 1)."""
+        """Name of the native kernel family: hex8, tet4, quad4, or hex8s /
+        quad4s (scalar HEX8 / QUAD4, vec = 1)."""
         tag = self.ele_type.lower()
-        return tag + "s" if (tag == "hex8" and self.vec == 1) else tag
+        return tag + "s" if (tag in ("hex8", "quad4") and self.vec == 1) else tag
 
     # ------------------------------------------------------------------ structure
     def _build_csr_assembly_structure(self):

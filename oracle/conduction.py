@@ -73,3 +73,46 @@ class Model:
     def scipy_csr(self, data):
         import scipy.sparse as sp
         return sp.csr_matrix((data, self.csr["indices"], self.csr["indptr"]), shape=(self.n, self.n))
+
+
+class QuadModel:
+    """The scalar oracle on QUAD4 (plane conduction): oracle/quad4.py's
+    geometry (2 x 2 Gauss points, 2 per edge) with the flux law above."""
+
+    def __init__(self, points, cells, boundary_inds_list=()):
+        from . import quad4
+        self.points = np.asarray(points, float)[:, :2]
+        self.cells = np.asarray(cells, np.int64)
+        self.E, self.N = len(self.cells), len(self.points)
+        self.n = self.N
+        self.boundary_inds_list = [np.asarray(b, np.int64).reshape(-1, 2) for b in boundary_inds_list]
+        self.grads, self.JxW = quad4.geometry(self.points, self.cells)            # (E, 4, 4, 2), (E, 4)
+        self.faces = [quad4.face_geometry(self.points, self.cells, b) for b in self.boundary_inds_list]
+        self.inds = st.build_inds(self.cells, vec=1)
+        I, J, _ = st.coo_ij(self.cells, self.boundary_inds_list, vec=1)
+        self.csr = st.csr_structure(I, J, self.n, I, J, num_cells=self.E, ndof_vol=4)
+        self.nse = self.csr["nse"]
+        self._pq = quad4.param_at_quads
+
+    def _kw(self, param, kind, **law):
+        return conductivity(kind, self._pq(param, self.cells, self.N), **law) * self.JxW
+
+    def element_matrices(self, param, kind="conduction", **law):
+        return np.einsum("eqad,eqbd,eq->eab", self.grads, self.grads, self._kw(param, kind, **law))
+
+    def residual(self, T, param, kind="conduction", fluxes=(), **law):
+        gT = np.einsum("ea,eqad->eqd", T[self.cells], self.grads)
+        vals = [np.einsum("eqd,eqad,eq->ea", gT, self.grads, self._kw(param, kind, **law)).reshape(-1)]
+        dofs = [self.inds.reshape(-1)]
+        for (fv, ns), b, q in zip(self.faces, self.boundary_inds_list, fluxes):
+            vals.append(np.einsum("fq,fqa,fq->fa", np.broadcast_to(q, ns.shape), fv, ns).reshape(-1))
+            dofs.append(self.inds[b[:, 0]].reshape(-1))
+        return np.bincount(np.concatenate(dofs).astype(np.int64), np.concatenate(vals), minlength=self.n)
+
+    def csr_values(self, param, kind="conduction", **law):
+        slots = self.csr["volume_slots"].reshape(-1).astype(np.int64)
+        return np.bincount(slots, self.element_matrices(param, kind, **law).reshape(-1), minlength=self.nse + 1)[:self.nse]
+
+    def scipy_csr(self, data):
+        import scipy.sparse as sp
+        return sp.csr_matrix((data, self.csr["indices"], self.csr["indptr"]), shape=(self.n, self.n))

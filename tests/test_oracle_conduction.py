@@ -103,3 +103,26 @@ def test_native_scalar_structure_matches_oracle():
     assert np.array_equal(fe.TracedParams.node_var_from_solution(p, T), T)
     tp = fe.TracedParams(volume_vars=(np.ones(len(cells)),)).replace_volume_var(0, np.zeros(len(cells)))
     assert np.all(tp.volume_vars[0] == 0.0)
+
+
+def test_plane_conduction_patch_test_and_native_structure():
+    """QUAD4 conduction: linear field on a distorted mesh (interior residual
+    0, edge reaction k g.n length, energy), and the native quad4s pattern."""
+    import feax_amd as fe
+    from oracle import quad4
+    pts, cells = quad4.rectangle(4, 3, 2.0, 1.5)
+    inner = (pts[:, 0] > 1e-9) & (pts[:, 0] < 2.0 - 1e-9) & (pts[:, 1] > 1e-9) & (pts[:, 1] < 1.5 - 1e-9)
+    pts = pts.copy()
+    pts[inner] += np.random.default_rng(2).uniform(-0.1, 0.1, (inner.sum(), 2))
+    m = conduction.QuadModel(pts, cells)
+    k, g = 2.0, np.array([0.6, -0.3])
+    T = pts @ g
+    r = m.residual(T, np.full(m.E, k))
+    assert np.abs(r[inner]).max() <= 1e-13 * np.abs(r).max()
+    assert abs(r[np.isclose(pts[:, 0], 2.0)].sum() - k * g[0] * 1.5) <= 1e-12
+    K = m.scipy_csr(m.csr_values(np.full(m.E, k)))
+    assert abs(0.5 * T @ (K @ T) - 0.5 * 3.0 * k * g @ g) <= 1e-12
+    p = fe.Problem(fe.Mesh(pts, cells, ele_type="QUAD4"), vec=1, dim=2, ele_type="QUAD4")
+    assert p.kernel_tag == "quad4s" and p.csr_nse == m.nse
+    assert np.array_equal(p.csr_indptr, m.csr["indptr"]) and np.array_equal(p.csr_indices, m.csr["indices"])
+    assert np.array_equal(p.csr_volume_slots, m.csr["volume_slots"])
