@@ -141,6 +141,21 @@ SIGNATURES = {
                                       _P, _SZ]),
     "fx_quad4s_simp_compliance_sens_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "fx_quad4s_traction_vjp_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P]),
+    "fx_tet4s_pattern": (_I32, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "fx_tet4s_face_incidence": (_I32, [_I32, _I32, _P, _I32, _P, _P, _P]),
+    "fx_tet4s_assemble_workspace_bytes": (_SZ, [_I32, _I32, _I32]),
+    "fx_tet4s_assemble_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                     _I32, _I32, _F64, _P, _P, _P, _SZ]),
+    "fx_tet4s_assemble_f32": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                     _I32, _I32, _F64, _P, _P, _P, _SZ]),
+    "fx_tet4s_residual_workspace_bytes": (_SZ, [_I32]),
+    "fx_tet4s_residual_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P,
+                                     _P, _P, _I32, _P, _P, _I32, _P, _P, _SZ]),
+    "fx_tet4s_param_vjp_workspace_bytes": (_SZ, [_I32, _I32]),
+    "fx_tet4s_param_vjp_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P,
+                                      _P, _SZ]),
+    "fx_tet4s_simp_compliance_sens_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "fx_tet4s_traction_vjp_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P]),
     "fx_hex8s_pattern": (_I32, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "fx_hex8s_face_incidence": (_I32, [_I32, _I32, _P, _I32, _P, _P, _P]),
     "fx_hex8s_assemble_workspace_bytes": (_SZ, [_I32, _I32, _I32]),

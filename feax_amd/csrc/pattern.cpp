@@ -267,3 +267,16 @@ extern "C" int fx_quad4s_face_incidence(int32_t num_nodes, int32_t num_cells, co
                                         int32_t* finc) {
   return fx_quad4_face_incidence(num_nodes, num_cells, cells, num_faces, faces, finc_ptr, finc);
 }
+
+extern "C" int fx_tet4s_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse_out,
+                                int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                                int32_t* inc_ptr, int32_t* inc) {
+  return cell_pattern("fx_tet4s_pattern", 4, 1, num_nodes, num_cells, cells, nse_out, max_degree, indptr, indices,
+                      elem_rank, inc_ptr, inc);
+}
+
+extern "C" int fx_tet4s_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells,
+                                       int32_t num_faces, const int32_t* faces, int32_t* finc_ptr,
+                                       int32_t* finc) {
+  return fx_tet4_face_incidence(num_nodes, num_cells, cells, num_faces, faces, finc_ptr, finc);
+}

@@ -442,6 +442,190 @@ int tet_assemble(void* stream, int num_nodes, int num_cells, const double* x, co
   return FX_OK;
 }
 
+// ---------------------------------------------------------------- scalar TET4
+// vec = 1 conduction on tets: K_ab = k grad N_a . grad N_b JxW, R_a =
+// k grad N_a . grad T JxW at the single Gauss point, facet flux q / 3 per
+// facet vertex times the Nanson scale; k = p or its SIMP interpolation.
+__device__ __forceinline__ double ts_conductivity(const MatParams& m, double p, double* dkdp) {
+  if (m.kind == FX_MAT_SIMP_CONDUCTION) {
+    const double span = m.E0 - m.Emin;
+    if (dkdp) *dkdp = m.penal * pow(p, m.penal - 1.0) * span;
+    return m.Emin + pow(p, m.penal) * span;
+  }
+  if (dkdp) *dkdp = 1.0;
+  return p;
+}
+
+__device__ __forceinline__ void ts_grad(const double* __restrict__ f, const uint8_t* __restrict__ mask,
+                                        const int* __restrict__ cn, const double (&g)[4][3], double (&G)[3]) {
+  G[0] = G[1] = G[2] = 0.0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double v = (mask && mask[cn[c]]) ? 0.0 : f[cn[c]];
+    G[0] += v * g[c][0];
+    G[1] += v * g[c][1];
+    G[2] += v * g[c][2];
+  }
+}
+
+__device__ __forceinline__ double ts_facet_nanson(const double (&g)[4][3], double det, int lf) {
+  double n[3], wf;
+  tet_facet(lf, n, wf);
+  double m[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) m[d] = n[0] * g[1][d] + n[1] * g[2][d] + n[2] * g[3][d];
+  return sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) * det * wf;
+}
+
+// One thread per (element, a, b): ke[16 e + 4 a + b].
+__global__ __launch_bounds__(kTetThreads) void ts_elem_matrix_kernel(int num_cells, const double* __restrict__ x,
+                                                                     const double* __restrict__ y,
+                                                                     const double* __restrict__ z,
+                                                                     const int* __restrict__ cells, MatParams mat,
+                                                                     const double* __restrict__ param,
+                                                                     double* __restrict__ ke) {
+  const long t = static_cast<long>(blockIdx.x) * kTetThreads + threadIdx.x;
+  const long e = t >> 4;
+  if (e >= num_cells) return;
+  const int a = static_cast<int>((t >> 2) & 3), b = static_cast<int>(t & 3);
+  const int* cn = cells + 4 * e;
+  double g[4][3];
+  const double w = tet_grads<double>(x, y, z, cn, g) / 6.0;
+  const double k = ts_conductivity(mat, tet_param<double>(mat, param, static_cast<int>(e), cn), nullptr);
+  ke[16 * e + 4 * a + b] = k * w * (g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2]);
+}
+
+// One thread per element: re[4 e + a].
+__global__ __launch_bounds__(kTetThreads) void ts_elem_residual_kernel(int num_cells, const double* __restrict__ x,
+                                                                       const double* __restrict__ y,
+                                                                       const double* __restrict__ z,
+                                                                       const int* __restrict__ cells, MatParams mat,
+                                                                       const double* __restrict__ param,
+                                                                       const double* __restrict__ u,
+                                                                       double* __restrict__ re) {
+  const int e = blockIdx.x * kTetThreads + threadIdx.x;
+  if (e >= num_cells) return;
+  const int* cn = cells + 4 * static_cast<long>(e);
+  double g[4][3], G[3];
+  const double w = tet_grads<double>(x, y, z, cn, g) / 6.0;
+  ts_grad(u, nullptr, cn, g, G);
+  const double kw = ts_conductivity(mat, tet_param<double>(mat, param, e, cn), nullptr) * w;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) re[4 * static_cast<long>(e) + a] = kw * (G[0] * g[a][0] + G[1] * g[a][1] + G[2] * g[a][2]);
+}
+
+// One thread per node: element residuals in ascending order, then the
+// loaded facets: flux = fdir[f] * svar[f] at the facet centroid.
+__global__ __launch_bounds__(256) void ts_node_residual_kernel(
+    int num_nodes, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const int* __restrict__ cells, const int* __restrict__ inc_ptr, const int* __restrict__ inc,
+    const double* __restrict__ re, const int* __restrict__ faces, const int* __restrict__ finc_ptr,
+    const int* __restrict__ finc, const double* __restrict__ svar, const double* __restrict__ fdir, int negate,
+    double* __restrict__ res) {
+  const int nd = blockIdx.x * 256 + threadIdx.x;
+  if (nd >= num_nodes) return;
+  double r = 0.0;
+  for (int k = inc_ptr[nd]; k < inc_ptr[nd + 1]; ++k) {
+    const int pk = inc[k];
+    r += re[4 * static_cast<long>(pk >> 3) + (pk & 7)];
+  }
+  if (finc_ptr) {
+    for (int k = finc_ptr[nd]; k < finc_ptr[nd + 1]; ++k) {
+      const int f = finc[k] >> 3;
+      const int e = faces[2 * f], lf = faces[2 * f + 1];
+      double g[4][3];
+      const double det = tet_grads<double>(x, y, z, cells + 4 * static_cast<long>(e), g);
+      r += fdir[f] * (svar[f] * (1.0 / 3.0) * ts_facet_nanson(g, det, lf));
+    }
+  }
+  res[nd] = negate ? -r : r;
+}
+
+// w_e = dk/dp grad lambda . grad T JxW (SELF: lambda = T); cell variables:
+// scale * w_e; node variables keep we[e] (tet_param_vjp_node_kernel).
+template <bool SELF>
+__global__ __launch_bounds__(kTetThreads) void ts_param_vjp_kernel(
+    int num_cells, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const int* __restrict__ cells, MatParams mat, const double* __restrict__ param, const double* __restrict__ u,
+    const double* __restrict__ lam, const uint8_t* __restrict__ bc_mask, double scale, double* __restrict__ out_cell,
+    double* __restrict__ we) {
+  const int e = blockIdx.x * kTetThreads + threadIdx.x;
+  if (e >= num_cells) return;
+  const int* cn = cells + 4 * static_cast<long>(e);
+  double g[4][3], G[3], L[3];
+  const double w = tet_grads<double>(x, y, z, cn, g) / 6.0;
+  ts_grad(u, nullptr, cn, g, G);
+  if constexpr (SELF) {
+    L[0] = G[0];
+    L[1] = G[1];
+    L[2] = G[2];
+  } else {
+    ts_grad(lam, bc_mask, cn, g, L);
+  }
+  double dkdp;
+  ts_conductivity(mat, tet_param<double>(mat, param, e, cn), &dkdp);
+  const double v = dkdp * w * (G[0] * L[0] + G[1] * L[1] + G[2] * L[2]);
+  if (out_cell) {
+    out_cell[e] = scale * v;
+  } else {
+    we[e] = v;
+  }
+}
+
+// grad_svar[f] = scale * fdir[f] * nanson * sum over the facet's vertices of lambda_a / 3.
+__global__ __launch_bounds__(256) void ts_flux_vjp_kernel(int num_faces, const double* __restrict__ x,
+                                                          const double* __restrict__ y, const double* __restrict__ z,
+                                                          const int* __restrict__ cells,
+                                                          const int* __restrict__ faces,
+                                                          const double* __restrict__ fdir,
+                                                          const double* __restrict__ lam,
+                                                          const uint8_t* __restrict__ bc_mask, double scale,
+                                                          double* __restrict__ out) {
+  const int f = blockIdx.x * 256 + threadIdx.x;
+  if (f >= num_faces) return;
+  const int e = faces[2 * f], lf = faces[2 * f + 1];
+  const int* cn = cells + 4 * static_cast<long>(e);
+  double g[4][3];
+  const double det = tet_grads<double>(x, y, z, cn, g);
+  double s = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if (a == lf) continue;
+    s += (1.0 / 3.0) * ((bc_mask && bc_mask[cn[a]]) ? 0.0 : lam[cn[a]]);
+  }
+  out[f] = scale * fdir[f] * s * ts_facet_nanson(g, det, lf);
+}
+
+inline bool ts_law(const fx_material* m) {
+  return (m->kind == FX_MAT_CONDUCTION || m->kind == FX_MAT_SIMP_CONDUCTION) && !m->temperature;
+}
+
+template <typename T>
+int ts_assemble(void* stream, int num_nodes, int num_cells, const double* x, const double* y, const double* z,
+                const int* cells, const int* inc_ptr, const int* inc, const uint8_t* rank, const int* indptr,
+                const int* indices, const fx_material* mat, const double* param, const uint8_t* bc_mask,
+                int symmetric, int flags, double jacobi_shift, double* dinv, T* data, void* workspace,
+                size_t ws_bytes) {
+  if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !z || !cells || !inc_ptr || !inc || !rank || !indptr ||
+      !indices || !mat || !param || !data)
+    return fail(FX_EINVAL, "fx_tet4s_assemble: bad arguments");
+  if (!ts_law(mat)) return fail(FX_EINVAL, "fx_tet4s_assemble: needs a conduction law");
+  if (flags & (FX_ASM_DIFF3 | FX_ASM_COMPUTE32))
+    return fail(FX_EINVAL, "fx_tet4s_assemble: the difference form and FP32 arithmetic are built for vec = 3");
+  const size_t need = 16 * sizeof(double) * static_cast<size_t>(num_cells);
+  if (!workspace || ws_bytes < need)
+    return fail(FX_EWORKSPACE, "fx_tet4s_assemble: workspace %zu < %zu", workspace ? ws_bytes : 0, need);
+  hipStream_t s = as_stream(stream);
+  double* ke = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(ts_elem_matrix_kernel, dim3(static_cast<int>((16L * num_cells + kTetThreads - 1) / kTetThreads)),
+                     dim3(kTetThreads), 0, s, num_cells, x, y, z, cells, to_params(mat), param, ke);
+  hipLaunchKernelGGL((gather_rows_kernel<T, double, 4, 1>), dim3((num_nodes + kNodesPerBlock - 1) / kNodesPerBlock),
+                     dim3(64 * kNodesPerBlock), 0, s, num_nodes, inc_ptr, inc, rank, indptr, indices, ke, bc_mask,
+                     symmetric, 0, jacobi_shift, dinv, data);
+  FX_LAUNCHED("fx_tet4s_assemble");
+  return FX_OK;
+}
+
 }  // namespace
 }  // namespace fx
 
@@ -604,6 +788,133 @@ int fx_tet4_energy_f64(void* stream, int32_t num_cells, const double* x, const d
                      to_params(mat), param, u, dpsi_dmu, partials);
   hipLaunchKernelGGL(tet_energy_fin, dim3(1), dim3(kFinBlock), 0, s, partials, grid, energy);
   FX_LAUNCHED("tet_energy_kernel");
+  return FX_OK;
+}
+
+size_t fx_tet4s_assemble_workspace_bytes(int32_t num_cells, int32_t value_bytes, int32_t flags) {
+  (void)value_bytes;
+  (void)flags;
+  return num_cells < 0 ? 0 : 16 * sizeof(double) * static_cast<size_t>(num_cells);
+}
+
+int fx_tet4s_assemble_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                          const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                          int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, double* data,
+                          void* workspace, size_t ws_bytes) {
+  (void)u;
+  return ts_assemble<double>(stream, num_nodes, num_cells, x, y, z, cells, inc_ptr, inc, elem_rank, indptr, indices,
+                             mat, param, bc_mask, symmetric, flags, jacobi_shift, dinv, data, workspace, ws_bytes);
+}
+
+int fx_tet4s_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                          const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                          int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, float* data,
+                          void* workspace, size_t ws_bytes) {
+  (void)u;
+  return ts_assemble<float>(stream, num_nodes, num_cells, x, y, z, cells, inc_ptr, inc, elem_rank, indptr, indices,
+                            mat, param, bc_mask, symmetric, flags, jacobi_shift, dinv, data, workspace, ws_bytes);
+}
+
+size_t fx_tet4s_residual_workspace_bytes(int32_t num_cells) {
+  return num_cells < 0 ? 0 : 4 * sizeof(double) * static_cast<size_t>(num_cells);
+}
+
+int fx_tet4s_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const fx_material* mat, const double* param, const double* u, int32_t num_faces,
+                          const int32_t* faces, const int32_t* finc_ptr, const int32_t* finc, const double* svar,
+                          const double* fdir, int32_t num_bc, const int32_t* bc_rows, const double* bc_vals,
+                          int32_t negate, double* res, void* workspace, size_t ws_bytes) {
+  if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !z || !cells || !inc_ptr || !inc || !mat || !param || !u ||
+      !res)
+    return fail(FX_EINVAL, "fx_tet4s_residual_f64: bad arguments");
+  if (!ts_law(mat)) return fail(FX_EINVAL, "fx_tet4s_residual_f64: needs a conduction law");
+  if (num_faces > 0 && (!faces || !finc_ptr || !finc || !svar || !fdir))
+    return fail(FX_EINVAL, "fx_tet4s_residual_f64: face arrays missing");
+  const size_t need = fx_tet4s_residual_workspace_bytes(num_cells);
+  if (!workspace || ws_bytes < need)
+    return fail(FX_EWORKSPACE, "fx_tet4s_residual_f64: workspace %zu < %zu", workspace ? ws_bytes : 0, need);
+  hipStream_t s = as_stream(stream);
+  const bool hf = num_faces > 0;
+  double* re = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(ts_elem_residual_kernel, dim3((num_cells + kTetThreads - 1) / kTetThreads), dim3(kTetThreads),
+                     0, s, num_cells, x, y, z, cells, to_params(mat), param, u, re);
+  hipLaunchKernelGGL(ts_node_residual_kernel, dim3((num_nodes + 255) / 256), dim3(256), 0, s, num_nodes, x, y, z,
+                     cells, inc_ptr, inc, re, hf ? faces : nullptr, hf ? finc_ptr : nullptr, hf ? finc : nullptr, svar,
+                     fdir, negate, res);
+  FX_LAUNCHED("fx_tet4s_residual_f64");
+  if (num_bc > 0 && bc_rows) {
+    if (!bc_vals) return fail(FX_EINVAL, "fx_tet4s_residual_f64: bc_vals missing");
+    hipLaunchKernelGGL(bc_res_kernel, dim3((num_bc + 255) / 256), dim3(256), 0, s, num_bc, bc_rows, bc_vals, u,
+                       negate, res);
+    FX_LAUNCHED("bc_res_kernel");
+  }
+  return FX_OK;
+}
+
+size_t fx_tet4s_param_vjp_workspace_bytes(int32_t num_cells, int32_t param_loc) {
+  if (num_cells < 0) return 0;
+  return param_loc == FX_PARAM_NODE ? static_cast<size_t>(num_cells) * sizeof(double) : 0;
+}
+
+int fx_tet4s_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                           const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                           const fx_material* mat, const double* param, const double* u, const double* lam,
+                           const uint8_t* bc_mask, double scale, double* grad, void* workspace, size_t ws_bytes) {
+  if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !z || !cells || !mat || !param || !u || !lam || !grad)
+    return fail(FX_EINVAL, "fx_tet4s_param_vjp_f64: bad arguments");
+  if (!ts_law(mat)) return fail(FX_EINVAL, "fx_tet4s_param_vjp_f64: needs a conduction law");
+  if (mat->param_loc != FX_PARAM_NODE && mat->param_loc != FX_PARAM_CELL)
+    return fail(FX_EINVAL, "fx_tet4s_param_vjp_f64: unknown parameter location %d", mat->param_loc);
+  hipStream_t s = as_stream(stream);
+  const MatParams mp = to_params(mat);
+  const dim3 eg((num_cells + kTetThreads - 1) / kTetThreads);
+  if (mat->param_loc == FX_PARAM_CELL) {
+    hipLaunchKernelGGL(ts_param_vjp_kernel<false>, eg, dim3(kTetThreads), 0, s, num_cells, x, y, z, cells, mp, param,
+                       u, lam, bc_mask, scale, grad, nullptr);
+    FX_LAUNCHED("ts_param_vjp_kernel");
+    return FX_OK;
+  }
+  if (!inc_ptr || !inc) return fail(FX_EINVAL, "fx_tet4s_param_vjp_f64: node variables need the incidence");
+  const size_t need = fx_tet4s_param_vjp_workspace_bytes(num_cells, FX_PARAM_NODE);
+  if (!workspace || ws_bytes < need)
+    return fail(FX_EWORKSPACE, "fx_tet4s_param_vjp_f64: workspace %zu < %zu bytes", ws_bytes, need);
+  double* we = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(ts_param_vjp_kernel<false>, eg, dim3(kTetThreads), 0, s, num_cells, x, y, z, cells, mp, param, u,
+                     lam, bc_mask, 1.0, nullptr, we);
+  hipLaunchKernelGGL(tet_param_vjp_node_kernel, dim3((num_nodes + 255) / 256), dim3(256), 0, s, num_nodes, inc_ptr,
+                     inc, we, scale, grad);
+  FX_LAUNCHED("tet_param_vjp_node_kernel");
+  return FX_OK;
+}
+
+int fx_tet4s_simp_compliance_sens_f64(void* stream, int32_t num_cells, const double* x, const double* y,
+                                      const double* z, const int32_t* cells, const fx_material* mat,
+                                      const double* param, const double* u, double* dc) {
+  if (num_cells <= 0 || !x || !y || !z || !cells || !mat || !param || !u || !dc)
+    return fail(FX_EINVAL, "fx_tet4s_simp_compliance_sens_f64: bad arguments");
+  if (!ts_law(mat) || mat->param_loc != FX_PARAM_CELL)
+    return fail(FX_EINVAL, "fx_tet4s_simp_compliance_sens_f64: needs a conduction law with a cell variable");
+  hipLaunchKernelGGL(ts_param_vjp_kernel<true>, dim3((num_cells + kTetThreads - 1) / kTetThreads), dim3(kTetThreads),
+                     0, as_stream(stream), num_cells, x, y, z, cells, to_params(mat), param, u, nullptr, nullptr, -1.0,
+                     dc, nullptr);
+  FX_LAUNCHED("ts_param_vjp_kernel");
+  return FX_OK;
+}
+
+int fx_tet4s_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, const double* y, const double* z,
+                              const int32_t* cells, const int32_t* faces, const double* fdir, const double* lam,
+                              const uint8_t* bc_mask, double scale, double* grad_svar) {
+  if (num_faces < 0 || (num_faces > 0 && (!x || !y || !z || !cells || !faces || !fdir || !lam || !grad_svar)))
+    return fail(FX_EINVAL, "fx_tet4s_traction_vjp_f64: bad arguments");
+  if (num_faces == 0) return FX_OK;
+  hipLaunchKernelGGL(ts_flux_vjp_kernel, dim3((num_faces + 255) / 256), dim3(256), 0, as_stream(stream), num_faces, x,
+                     y, z, cells, faces, fdir, lam, bc_mask, scale, grad_svar);
+  FX_LAUNCHED("ts_flux_vjp_kernel");
   return FX_OK;
 }
 

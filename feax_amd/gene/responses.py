@@ -40,7 +40,7 @@ def _external_work_vector(problem, ts, params):
         parts.append(torch.broadcast_to(v.reshape(-1, 1) if v.numel() == nf else v, (nf, q)).to(torch.float64))
     svar = torch.cat(parts).reshape(-1).contiguous()
     # At u = 0 the volume term vanishes for every law here (stress(0) = 0).
-    kind = nat.FX_MAT_CONDUCTION if ts.tag in ("hex8s", "quad4s") else nat.FX_MAT_LINEAR_ELASTIC
+    kind = nat.FX_MAT_CONDUCTION if ts.tag in ("hex8s", "tet4s", "quad4s") else nat.FX_MAT_LINEAR_ELASTIC
     mat = nat.material_struct(kind, nat.FX_PARAM_CELL, nu=0.3)
     ones = torch.ones(ts.num_cells, dtype=torch.float64, device=ts.device)
     zero = torch.zeros(ts.n, dtype=torch.float64, device=ts.device)

@@ -82,7 +82,8 @@ class Problem:
         ok = ((self.ele_type in ("HEX8", "TET4") and self.vec == 3 and self.dim == 3)
               or (self.ele_type == "QUAD4" and self.vec == 2 and self.dim == 2)
               or (self.ele_type == "HEX8" and self.vec == 1 and self.dim == 3)
-              or (self.ele_type == "QUAD4" and self.vec == 1 and self.dim == 2))
+              or (self.ele_type == "QUAD4" and self.vec == 1 and self.dim == 2)
+              or (self.ele_type == "TET4" and self.vec == 1 and self.dim == 3))
         if not ok:
             raise NotImplementedError(
                 f"the HIP path covers HEX8 and TET4 with vec=3, dim=3, QUAD4 with vec=2, dim=2 and scalar HEX8 "
@@ -109,9 +110,9 @@ class Problem:
     @property
     def kernel_tag(self):
         """Name of the native kernel family: hex8, tet4, quad4, or hex8s /
-        quad4s (scalar HEX8 / QUAD4, vec = 1)."""
+        tet4s / quad4s (the scalar, vec = 1, problems)."""
         tag = self.ele_type.lower()
-        return tag + "s" if (tag in ("hex8", "quad4") and self.vec == 1) else tag
+        return tag + "s" if self.vec == 1 else tag
 
     # ------------------------------------------------------------------ structure
     def _build_csr_assembly_structure(self):

@@ -765,6 +765,45 @@ int fx_quad4s_traction_vjp_f64(void* stream, int32_t num_faces, const double* x,
                               const int32_t* cells, const int32_t* faces, const double* fdir, const double* lam,
                               const uint8_t* bc_mask, double scale, double* grad_svar);
 
+/* Scalar TET4 (vec = 1): the fx_hex8s_* entry points on TET4 geometry
+ * (one Gauss point; flux = fdir[f] * svar[f] at the facet centroid). */
+int fx_tet4s_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse,
+                     int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                     int32_t* inc_ptr, int32_t* inc);
+int fx_tet4s_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int32_t num_faces,
+                            const int32_t* faces, int32_t* finc_ptr, int32_t* finc);
+size_t fx_tet4s_assemble_workspace_bytes(int32_t num_cells, int32_t value_bytes, int32_t flags);
+int fx_tet4s_assemble_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                          const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                          int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, double* data,
+                          void* workspace, size_t ws_bytes);
+int fx_tet4s_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                          const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                          int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, float* data,
+                          void* workspace, size_t ws_bytes);
+size_t fx_tet4s_residual_workspace_bytes(int32_t num_cells);
+int fx_tet4s_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const fx_material* mat, const double* param, const double* u, int32_t num_faces,
+                          const int32_t* faces, const int32_t* finc_ptr, const int32_t* finc, const double* svar,
+                          const double* fdir, int32_t num_bc, const int32_t* bc_rows, const double* bc_vals,
+                          int32_t negate, double* res, void* workspace, size_t ws_bytes);
+size_t fx_tet4s_param_vjp_workspace_bytes(int32_t num_cells, int32_t param_loc);
+int fx_tet4s_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                           const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                           const fx_material* mat, const double* param, const double* u, const double* lam,
+                           const uint8_t* bc_mask, double scale, double* grad, void* workspace, size_t ws_bytes);
+int fx_tet4s_simp_compliance_sens_f64(void* stream, int32_t num_cells, const double* x, const double* y,
+                                      const double* z, const int32_t* cells, const fx_material* mat,
+                                      const double* param, const double* u, double* dc);
+int fx_tet4s_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, const double* y, const double* z,
+                              const int32_t* cells, const int32_t* faces, const double* fdir, const double* lam,
+                              const uint8_t* bc_mask, double scale, double* grad_svar);
+
 #ifdef __cplusplus
 }
 #endif

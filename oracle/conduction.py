@@ -116,3 +116,23 @@ class QuadModel:
     def scipy_csr(self, data):
         import scipy.sparse as sp
         return sp.csr_matrix((data, self.csr["indices"], self.csr["indptr"]), shape=(self.n, self.n))
+
+
+class TetModel(QuadModel):
+    """The scalar oracle on TET4: oracle/tet4.py's geometry (one Gauss point,
+    facet flux at the facet centroid) with the flux law above."""
+
+    def __init__(self, points, cells, boundary_inds_list=()):
+        from . import tet4
+        self.points = np.asarray(points, float)
+        self.cells = np.asarray(cells, np.int64)
+        self.E, self.N = len(self.cells), len(self.points)
+        self.n = self.N
+        self.boundary_inds_list = [np.asarray(b, np.int64).reshape(-1, 2) for b in boundary_inds_list]
+        self.grads, self.JxW = tet4.geometry(self.points, self.cells)            # (E, 1, 4, 3), (E, 1)
+        self.faces = [tet4.face_geometry(self.points, self.cells, b) for b in self.boundary_inds_list]
+        self.inds = st.build_inds(self.cells, vec=1)
+        I, J, _ = st.coo_ij(self.cells, self.boundary_inds_list, vec=1)
+        self.csr = st.csr_structure(I, J, self.n, I, J, num_cells=self.E, ndof_vol=4)
+        self.nse = self.csr["nse"]
+        self._pq = tet4.param_at_quads

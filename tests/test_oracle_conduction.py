@@ -126,3 +126,17 @@ def test_plane_conduction_patch_test_and_native_structure():
     assert p.kernel_tag == "quad4s" and p.csr_nse == m.nse
     assert np.array_equal(p.csr_indptr, m.csr["indptr"]) and np.array_equal(p.csr_indices, m.csr["indices"])
     assert np.array_equal(p.csr_volume_slots, m.csr["volume_slots"])
+
+
+def test_tet_conduction_patch_test():
+    from oracle import tet4
+    pts, cells = tet4.box_tets((3, 2, 2), (1.0, 0.5, 2.0))
+    m = conduction.TetModel(pts, cells)
+    k, g = 1.5, np.array([0.2, -0.7, 0.4])
+    T = pts @ g
+    r = m.residual(T, np.full(m.E, k))
+    L = np.array([3.0, 1.0, 4.0])
+    interior = np.all((pts > 1e-9) & (pts < L - 1e-9), axis=1)
+    assert np.abs(r[interior]).max() <= 1e-13 * np.abs(r).max()
+    K = m.scipy_csr(m.csr_values(np.full(m.E, k)))
+    assert abs(0.5 * T @ (K @ T) - 0.5 * 12.0 * k * g @ g) <= 1e-12 * 12.0

@@ -95,9 +95,10 @@ def test_unsupported_problems_raise():
     with pytest.raises(NotImplementedError):
         fe.Problem(mesh, vec=2, dim=3)
     assert fe.Problem(fe.mesh.rectangle_mesh(1, 1), vec=1, dim=2, ele_type="QUAD4").kernel_tag == "quad4s"
-    tets = fe.Mesh(np.eye(4, 3), np.array([[0, 1, 2, 3]]), ele_type="TET4")
+    tets = fe.Mesh(np.vstack([np.zeros(3), np.eye(3)]), np.array([[0, 1, 2, 3]]), ele_type="TET4")
+    assert fe.Problem(tets, vec=1, dim=3, ele_type="TET4").kernel_tag == "tet4s"
     with pytest.raises(NotImplementedError):
-        fe.Problem(tets, vec=1, dim=3, ele_type="TET4")
+        fe.Problem(tets, vec=2, dim=3, ele_type="TET4")
     with pytest.raises(NotImplementedError):
         fe.Problem(mesh, vec=3, dim=3, ele_type="HEX20")
     assert fe.Problem(mesh, vec=1, dim=3).kernel_tag == "hex8s"
