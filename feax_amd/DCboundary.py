@@ -14,7 +14,7 @@ from typing import Callable, List, Union
 
 import numpy as np
 
-from .fe import eval_points
+from .fe import eval_points, location_flags
 
 
 @dataclass(frozen=True)
@@ -102,21 +102,19 @@ class DirichletBCConfig:
                 continue
             comps = list(range(vec)) if spec.component == "all" else (
                 [spec.component] if spec.component < vec else [])
-            nargs = spec.location.__code__.co_argcount if hasattr(spec.location, "__code__") else 1
+            # The location is evaluated once per spec (feax evaluates it per
+            # component, with the same result).
+            flag = location_flags(spec.location, pts, "got") if comps else None
+            nodes = np.nonzero(flag)[0] if comps else np.zeros(0, np.int64)
+            if len(nodes) == 0:
+                continue
+            v = np.asarray(eval_points(spec.value, pts[nodes]), np.float64).reshape(-1)
+            if v.shape != (len(nodes),):
+                raise ValueError(f"a Dirichlet value function must return one value per point "
+                                 f"(got {v.size} values for {len(nodes)} points)")
             for c in comps:
-                if nargs == 1:
-                    flag = eval_points(spec.location, pts).astype(bool)
-                elif nargs == 2:
-                    flag = np.asarray([bool(spec.location(p, i)) for i, p in enumerate(pts)])
-                else:
-                    raise ValueError(f"Wrong number of arguments for location_fn: must be 1 or 2, got {nargs}")
-                nodes = np.nonzero(flag)[0]
-                if len(nodes) == 0:
-                    continue
                 rows.append(nodes * vec + c)
-                v = eval_points(spec.value, pts[nodes])
-                vals.append(np.broadcast_to(np.asarray(v, np.float64).reshape(-1), (len(nodes),)).copy()
-                            if np.size(v) in (1, len(nodes)) else np.asarray(v, np.float64))
+                vals.append(v)
         if rows:
             r = np.concatenate(rows)
             v = np.concatenate(vals)

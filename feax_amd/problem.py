@@ -22,7 +22,7 @@ from typing import Any, Callable, List, Optional, Tuple, Union
 import numpy as np
 
 from . import _native as nat
-from .fe import boundary_conditions_inds, element_info
+from .fe import boundary_conditions_inds, element_info, physical_quad_points, physical_surface_quad_points
 from .mesh import Mesh
 
 
@@ -113,6 +113,19 @@ class Problem:
         tet4s / quad4s (the scalar, vec = 1, problems)."""
         tag = self.ele_type.lower()
         return tag + "s" if self.vec == 1 else tag
+
+    @property
+    def physical_quad_points(self):
+        """(E, num_quads, dim) coordinates of the volume quadrature points
+        (feax/problem.py:276-277)."""
+        return physical_quad_points(self.points, self.cells, self.ele_type)
+
+    @property
+    def physical_surface_quad_points(self):
+        """One (F_i, num_face_quads, dim) array per location_fn
+        (feax/problem.py:282-308)."""
+        return [physical_surface_quad_points(self.points, self.cells, b, self.ele_type)
+                for b in self.boundary_inds_list]
 
     # ------------------------------------------------------------------ structure
     def _build_csr_assembly_structure(self):

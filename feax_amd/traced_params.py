@@ -73,9 +73,29 @@ class TracedParams:
 
     @staticmethod
     def create_node_var_from_fn(problem, var_fn, var_index=0):
-        return np.asarray(eval_points(var_fn, problem.points), float)
+        """var_fn at every node (feax/traced_params.py:234-266)."""
+        return np.asarray(eval_points(var_fn, problem.fes[var_index].points), float)
 
     @staticmethod
     def create_cell_var_from_fn(problem, var_fn, var_index=0):
-        cent = problem.points[problem.cells].mean(axis=1)
+        """var_fn at every cell centroid, the mean of its vertices
+        (feax/traced_params.py:269-302)."""
+        fe = problem.fes[var_index]
+        cent = np.asarray(fe.points, float)[np.asarray(fe.cells)].mean(axis=1)
         return np.asarray(eval_points(var_fn, cent), float)
+
+    @staticmethod
+    def create_spatially_varying_volume_var(problem, var_fn, var_index=0):
+        """var_fn at every volume quadrature point, shape (E, num_quads)
+        (feax/traced_params.py:304-340)."""
+        qp = problem.physical_quad_points
+        E, nq, dim = qp.shape
+        return np.asarray(eval_points(var_fn, qp.reshape(-1, dim)), float).reshape(E, nq)
+
+    @staticmethod
+    def create_spatially_varying_surface_var(problem, var_fn, surface_index=0):
+        """var_fn at every face quadrature point of boundary surface_index,
+        shape (F, num_face_quads) (feax/traced_params.py:342-375)."""
+        sp = problem.physical_surface_quad_points[surface_index]
+        F, nfq, dim = sp.shape
+        return np.asarray(eval_points(var_fn, sp.reshape(-1, dim)), float).reshape(F, nfq)

@@ -16,7 +16,7 @@ Restates:
   feax/assembler.py:1253-1318      CSR values = ordered reduction over slots
   feax/DCboundary.py:147-251       Dirichlet elimination on CSR / residual
 
-Pinned by (tests/test_oracle_hex8.py): partition of unity and exact
+Pinned by (tests/test_oracle.py): partition of unity and exact
 gradients of the trilinear basis; the closed-form unit-cube stiffness
 ``make_KE_3d`` of feax/solvers/cmg.py:45-74 after the corner permutation;
 symmetry, 6 rigid-body modes and linearity in E; central differences of the

@@ -18,7 +18,9 @@ Pinned by: the reference's known answers (len(I)=6912, UPPER/LOWER 3600,
 shape (81, 81) for the 2x2x2 fixture, tests/test_matrix_view.py:107-133,
 174-175, 199-200), the closed form nse = 9*prod(3(n_d+1)-2) for structured
 boxes, len(I) = 576*(E+F), and a brute-force dense adjacency check on tiny
-meshes (tests/test_oracle_structure.py).
+meshes (tests/test_oracle.py, tests/test_structure_host.py); location and
+value functions are evaluated point by point, and pinned against
+explicit per-point loops (tests/test_structure_host.py).
 """
 
 import numpy as np
@@ -65,18 +67,29 @@ def structured_grid(dims, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0)):
     return points, np.searchsorted(used, cells)
 
 
+def _location_per_node(points, location_fn):
+    """Boolean flag of every node, ``location_fn`` called once per point as
+    jax.vmap calls it (feax/fe.py:288-301, feax/DCboundary.py:448-460): a
+    one-argument function gets the (dim,) coordinates, a two-argument one
+    also the node index."""
+    nargs = location_fn.__code__.co_argcount if hasattr(location_fn, "__code__") else 1
+    if nargs == 1:
+        return np.array([bool(location_fn(p)) for p in points], dtype=bool)
+    if nargs == 2:
+        return np.array([bool(location_fn(p, i)) for i, p in enumerate(points)], dtype=bool)
+    raise ValueError(f"Wrong number of arguments for location_fn: must be 1 or 2, got {nargs}")
+
+
 def boundary_faces(points, cells, location_fn):
     """(F, 2) int64 array of (cell, local face) whose 4 vertices all satisfy
     ``location_fn`` -- feax/fe.py:267-318 (np.argwhere => (cell, face) order).
 
-    ``location_fn`` is called on a (3, M) array of coordinates (x = p[0] ...)
-    and must return a length-M boolean array, the vectorised equivalent of
-    jax.vmap over points.
+    ``location_fn`` is evaluated point by point.  A face vertex is a mesh
+    node, so evaluating each node once and gathering the flags per face
+    vertex gives what feax's vmap over (cell, face, vertex) gives.
     """
-    fp = points[cells[:, HEX8_FACE_INDS]]          # (E, 6, 4, 3)
-    flat = fp.reshape(-1, 3)
-    flags = np.asarray(location_fn(flat.T), dtype=bool).reshape(fp.shape[:3])
-    return np.argwhere(flags.all(axis=2)).astype(np.int64)
+    flag = _location_per_node(points, location_fn)
+    return np.argwhere(flag[cells[:, HEX8_FACE_INDS]].all(axis=2)).astype(np.int64)
 
 
 def build_inds(cells, vec=3, offset=0):
@@ -178,21 +191,22 @@ def dirichlet_rows(points, specs, vec=3):
     """(bc_rows int32 sorted unique, bc_vals f64) -- DCboundary.py:395-501.
 
     ``specs`` is a list of (location_fn, component, value) where component is
-    an int or 'all' and value a float or a vectorised fn of (3, M) coords.
-    Rows are concatenated per spec and component, stably sorted, and
-    duplicates keep their first occurrence (np.unique return_index).
+    an int or 'all' and value a float or a fn of one point.  Location and
+    value functions are called point by point (the jax.vmap of
+    feax/DCboundary.py:457-470).  Rows are concatenated per spec and
+    component, stably sorted, and duplicates keep their first occurrence
+    (np.unique return_index).
     """
     rows, vals = [], []
     for loc, comp, value in specs:
         comps = list(range(vec)) if comp == "all" else [int(comp)]
         for c in comps:
-            nodes = np.argwhere(np.asarray(loc(points.T), dtype=bool)).reshape(-1)
+            nodes = np.argwhere(_location_per_node(points, loc)).reshape(-1)
             if len(nodes) == 0:
                 continue
             rows.append(nodes * vec + c)
             if callable(value):
-                vals.append(np.asarray(value(points[nodes].T), float).reshape(-1)
-                            * np.ones(len(nodes)))
+                vals.append(np.array([float(np.asarray(value(p)).reshape(())) for p in points[nodes]]))
             else:
                 vals.append(np.full(len(nodes), float(value)))
     if not rows:
