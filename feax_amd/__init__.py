@@ -26,4 +26,5 @@ __version__ = "0.1.0"
 from .solvers.common import (check_convergence, create_direct_solve_fn, create_iterative_solve_fn,
                              create_jacobi_preconditioner, create_linear_solve_fn, create_x0)
 from .solvers.linear import create_linear_solver, linear_solve
-from .solvers.options import AbstractSolverOptions, resolve_iterative_solver
+from .solvers.options import (AbstractSolverOptions, detect_matrix_property, resolve_direct_solver,
+                              resolve_iterative_solver)

@@ -95,6 +95,23 @@ class CSRMatrix:
     def __matmul__(self, v):
         return self.matvec(v)
 
+    def transpose(self):
+        """The explicit transpose as a CSRMatrix (feax/csr.py:102-113 gives a
+        matvec-only view; here the transposed values are one device gather
+        of ``data`` through a slot map built once per pattern on the host)."""
+        if self.form != "csr":
+            raise ValueError("transpose() of a difference-form matrix")
+        ip, ix, perm = _transpose_maps(self)
+        return CSRMatrix(self.data[perm], ip, ix, (self.shape[1], self.shape[0]), block3=self.block3)
+
+    @property
+    def T(self):
+        return self.transpose()
+
+    def rmatvec(self, w):
+        """A^T w (feax/csr.py:92-100)."""
+        return self.transpose().matvec(w)
+
     def diagonal(self):
         import torch
         if self.form != "csr":
@@ -125,6 +142,32 @@ class CSRMatrix:
                               self.indptr.cpu().numpy()), shape=self.shape)
 
 
+_T_CACHE = {}
+
+
+def _transpose_maps(A):
+    """(indptr, indices, perm) of A's transposed pattern, cached per pattern:
+    slot k of the transpose holds A.data[perm[k]], rows sorted, columns
+    sorted within a row (feax/problem.py:417-451)."""
+    import torch
+    key = (A.indptr.data_ptr(), A.indices.data_ptr(), A.shape)
+    hit = _T_CACHE.get(key)
+    if hit is None:
+        n, m = A.shape
+        indptr = A.indptr.cpu().numpy().astype(np.int64)
+        cols = A.indices.cpu().numpy().astype(np.int64)
+        rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(indptr))
+        perm = np.argsort(cols * n + rows, kind="stable")
+        t_indptr = np.zeros(m + 1, np.int64)
+        np.cumsum(np.bincount(cols, minlength=m), out=t_indptr[1:])
+        dev = A.data.device
+        hit = (torch.as_tensor(t_indptr.astype(np.int32), device=dev),
+               torch.as_tensor(rows[perm].astype(np.int32), device=dev),
+               torch.as_tensor(perm, device=dev), A.indptr, A.indices)
+        _T_CACHE[key] = hit
+    return hit[:3]
+
+
 def transpose_with_maps(J, T_perm, T_indptr, T_indices):
     """Explicit CSR transpose from the static maps of Problem
     (csr_T_perm / csr_T_indptr / csr_T_indices): values gathered as
@@ -135,3 +178,51 @@ def transpose_with_maps(J, T_perm, T_indptr, T_indices):
     ip = torch.as_tensor(np.asarray(T_indptr, np.int32), device=dev)
     ix = torch.as_tensor(np.asarray(T_indices, np.int32), device=dev)
     return CSRMatrix(J.data[perm], ip, ix, (J.shape[1], J.shape[0]), block3=J.block3)
+
+
+_VIEW_CACHE = {}
+
+
+def expand_view(A, matrix_view):
+    """The full symmetric matrix a triangular view stands for: A + A^T -
+    diag(A) (feax/solvers/common.py:364-373, _matvec_with_matrix_view).
+
+    ``matrix_view`` is a MatrixView (or its name); FULL and None return A.
+    The full pattern and the gather map from the triangle's slots are built
+    once per pattern on the host (rows sorted, columns sorted within a row,
+    as feax's CSR); the values are one device gather per call.
+    """
+    import torch
+    name = getattr(matrix_view, "name", matrix_view)
+    if name in (None, "FULL"):
+        return A
+    if name not in ("UPPER", "LOWER"):
+        raise ValueError(f"unknown matrix_view {matrix_view!r}")
+    if A.form != "csr":
+        raise ValueError("expand_view needs a plain CSR matrix")
+    key = (A.indptr.data_ptr(), A.indices.data_ptr(), A.shape, name)
+    hit = _VIEW_CACHE.get(key)
+    if hit is None:
+        n = A.shape[0]
+        indptr = A.indptr.cpu().numpy().astype(np.int64)
+        cols = A.indices.cpu().numpy().astype(np.int64)
+        rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(indptr))
+        keep = cols >= rows if name == "UPPER" else cols <= rows
+        if not np.all(keep):
+            raise ValueError(f"matrix_view {name}: the matrix has entries outside that triangle")
+        off = cols != rows
+        slot = np.arange(len(cols), dtype=np.int64)
+        r_all = np.concatenate([rows, cols[off]])
+        c_all = np.concatenate([cols, rows[off]])
+        s_all = np.concatenate([slot, slot[off]])
+        order = np.argsort(r_all * n + c_all, kind="stable")
+        full_indptr = np.zeros(n + 1, np.int64)
+        np.cumsum(np.bincount(r_all, minlength=n), out=full_indptr[1:])
+        dev = A.data.device
+        hit = (torch.as_tensor(full_indptr.astype(np.int32), device=dev),
+               torch.as_tensor(c_all[order].astype(np.int32), device=dev),
+               torch.as_tensor(s_all[order], device=dev),
+               A.indptr, A.indices)          # keep the keyed tensors alive
+        _VIEW_CACHE[key] = hit
+    ip, ix, src = hit[:3]
+    return CSRMatrix(A.data[src], ip, ix, A.shape)

@@ -315,15 +315,23 @@ def _dot_ws(device):
 
 
 def check_convergence(A, x, b, solver_options, matrix_view=None, solver_label="Linear solver"):
-    """NaN-fill unless ||Ax - b|| / (||b|| + 1e-12) < threshold (common.py:376-395)."""
+    """NaN-fill unless ||Ax - b|| / (||b|| + 1e-12) < threshold
+    (feax/solvers/common.py:376-395).  With an UPPER or LOWER matrix_view,
+    A x is the product with the full symmetric matrix A + A^T - diag(A)
+    (feax/solvers/common.py:364-373)."""
     import torch
-    Ax = A(x) if callable(A) and not isinstance(A, CSRMatrix) else A @ x
+    from ..csr import expand_view
+    if isinstance(A, CSRMatrix):
+        Ax = expand_view(A, matrix_view) @ x
+    else:
+        Ax = A(x) if callable(A) else A @ x
     r = torch.empty_like(b, dtype=torch.float64)
     nat.call("fx_axpby_f64", nat.stream(), r.numel(), 1.0, nat.ptr(Ax.to(torch.float64).contiguous()), -1.0,
              nat.ptr(b.to(torch.float64).contiguous()), nat.ptr(r))
-    rel = device_norm(r) / (device_norm(b) + 1e-12)
+    rn = device_norm(r)
+    rel = rn / (device_norm(b) + 1e-12)
     if getattr(solver_options, "verbose", False):
-        print(f"{solver_label} residual: rel={rel:.6e}")
+        print(f"{solver_label} residual: abs={rn:.6e}, rel={rel:.6e}")
     if rel < getattr(solver_options, "convergence_threshold", 0.1):
         return x
     return torch.full_like(x, float("nan"))
@@ -368,6 +376,9 @@ def create_direct_solve_fn(options, *, cache_namespace="global"):
     the last pattern is reused while A's values are unchanged only under
     reuse_factorization."""
     from .direct import BandCholesky
+    if options.solver == "auto":
+        raise ValueError("DirectSolverOptions.solver is 'auto' (unresolved). "
+                         "Call resolve_direct_solver() before create_direct_solve_fn().")
     state = {}
 
     def solve(A, b, x0=None):

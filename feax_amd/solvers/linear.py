@@ -27,6 +27,7 @@ import numpy as np
 
 from .. import _native as nat
 from ..assembler import TracedStructure, assemble_csr, residual
+from ..csr import CSRMatrix
 from ..DCboundary import DirichletBC
 from .common import (KrylovRecord, check_convergence, create_iterative_solve_fn, create_jacobi_preconditioner,
                      create_x0, device_norm, inner_tolerance, krylov, pcg)
@@ -217,7 +218,13 @@ def linear_solve(J_bc_applied, res_bc_applied, initial_guess, bc, solver_options
                  traced_params=None, P_mat=None, linear_solve_fn=None, x0_fn=None):
     """One linear step u = u0 + J_bc^-1 (-R_bc(u0)) from user-supplied
     J_bc / R_bc functions (feax/solvers/linear.py:119-160; P_mat, the
-    reduced-space prolongation, is not on this path)."""
+    reduced-space prolongation, is not on this path).
+
+    A J_bc in an UPPER or LOWER matrix_view stores one triangle of a
+    symmetric matrix; the solve and the convergence check use the full
+    matrix A + A^T - diag(A) it stands for (feax/solvers/common.py:364-373;
+    feax's direct backends take the view as a factorisation flag)."""
+    from ..csr import expand_view
     from .common import create_linear_solve_fn
     if P_mat is not None:
         raise NotImplementedError("prolongation-reduced solves (P_mat) are not on the HIP path")
@@ -226,13 +233,15 @@ def linear_solve(J_bc_applied, res_bc_applied, initial_guess, bc, solver_options
     args = (initial_guess, traced_params) if traced_params is not None else (initial_guess,)
     res = res_bc_applied(*args)
     J = J_bc_applied(*args)
+    if isinstance(J, CSRMatrix):
+        J = expand_view(J, matrix_view)
     b = -res
     x0 = None
     if solver_options.uses_x0():
         x0 = (x0_fn or create_x0(bc_rows=bc.bc_rows, bc_vals=bc.bc_vals))(initial_guess)
     du = linear_solve_fn(J, b, x0)
     if solver_options.check_convergence:
-        du = check_convergence(J, du, b, solver_options, matrix_view, "Linear solver")
+        du = check_convergence(J, du, b, solver_options, None, "Linear solver")
     return initial_guess + du, None
 
 

@@ -131,3 +131,47 @@ def resolve_iterative_solver(options, matrix_property):
     import dataclasses
     logger.info(f"IterativeSolver auto: matrix_property={matrix_property.name} -> {name}")
     return dataclasses.replace(options, solver=name)
+
+
+def detect_matrix_property(A, sym_tol: float = 1e-8, matrix_view=None) -> MatrixProperty:
+    """SPD / SYMMETRIC / GENERAL of an assembled device CSRMatrix
+    (feax/solvers/options.py:188-259).
+
+    Symmetry: ||A x - A^T x|| / (||A x|| + 1e-30) < sym_tol for a seeded
+    standard-normal x (feax draws x from jax.random.PRNGKey(42); the
+    decision does not depend on the draw for an exactly symmetric or a
+    clearly non-symmetric matrix), skipped for an UPPER or LOWER view.
+    Definiteness (heuristic, as in feax): every diagonal entry positive.
+    """
+    import numpy as np
+    import torch
+    from .common import device_norm
+    name = getattr(matrix_view, "name", matrix_view)
+    if name in ("UPPER", "LOWER"):
+        symmetric = True
+    else:
+        n = A.shape[1]
+        x = torch.as_tensor(np.random.default_rng(42).standard_normal(n), dtype=A.data.dtype,
+                            device=A.data.device)
+        Ax = A @ x
+        ATx = A.rmatvec(x)
+        symmetric = device_norm(Ax - ATx) / (device_norm(Ax) + 1e-30) < sym_tol
+    if not symmetric:
+        return MatrixProperty.GENERAL
+    if bool((A.diagonal() > 0).all()):
+        return MatrixProperty.SPD
+    return MatrixProperty.SYMMETRIC
+
+
+def resolve_direct_solver(options, matrix_property, matrix_view=None):
+    """'auto' -> the device solver (feax/solvers/options.py:637-780 picks
+    cuDSS on a GPU backend).  The device banded factorisation is Cholesky /
+    L D L^T of a symmetric matrix, so a GENERAL matrix is refused."""
+    if matrix_property == MatrixProperty.GENERAL:
+        raise NotImplementedError("the direct solver factors symmetric matrices; the detected matrix is "
+                                  "GENERAL (use KrylovSolverOptions(solver='gmres' or 'bicgstab'))")
+    if options.solver != "auto":
+        return options
+    import dataclasses
+    logger.info(f"DirectSolver auto: matrix_property={matrix_property.name} -> cudss")
+    return dataclasses.replace(options, solver="cudss")

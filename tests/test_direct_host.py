@@ -64,6 +64,17 @@ def test_direct_options_validate_and_route():
     p = _LE.problem(mesh.points, mesh.cells)
     bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda x: np.isclose(x[0], 0.0), "all", 0.0)]).create_bc(p)
     with pytest.raises(NotImplementedError, match="symmetric"):
-        fe.create_solver(p, bc, fe.DirectSolverOptions(), linear=True, symmetric_elimination=False)
+        fe.create_solver(p, bc, fe.DirectSolverOptions(solver="cudss"), linear=False, symmetric_elimination=False)
+    # feax/solver.py:316-321: "auto" (and missing options) need a sample traced_params
+    for opts in (None, fe.DirectSolverOptions(), fe.KrylovSolverOptions()):
+        with pytest.raises(ValueError, match="traced_params is required"):
+            fe.create_solver(p, bc, opts, linear=True)
+    with pytest.raises(ValueError, match="traced_params is required"):
+        fe.create_solver(p, bc, fe.KrylovSolverOptions(solver="cg"), fe.KrylovSolverOptions(), linear=True)
+    with pytest.raises(ValueError, match="unresolved"):
+        fe.create_direct_solve_fn(fe.DirectSolverOptions())
+    with pytest.raises(NotImplementedError, match="GENERAL"):
+        fe.resolve_direct_solver(fe.DirectSolverOptions(), fe.MatrixProperty.GENERAL)
+    assert fe.resolve_direct_solver(fe.DirectSolverOptions(), fe.MatrixProperty.SPD).solver == "cudss"
     with pytest.raises(TypeError, match="Expected"):
         fe.create_solver(p, bc, object(), linear=True)

@@ -68,8 +68,10 @@ def test_bc_grad_linear_nonsymmetric():
     respond to bc_vals and the gradient carries -(K lambda_f)_b."""
     import feax_amd as fe
     problem, bc, tp, vals, initial = _setup()
-    solver = fe.create_solver(problem, bc, solver_options=fe.KrylovSolverOptions(solver="bicgstab", tol=1e-12),
-                              linear=True, symmetric_elimination=False)
+    # feax's create_solver(linear=True) always eliminates symmetrically; the
+    # row-only elimination of the linear solve is create_linear_solver's option
+    solver = fe.create_linear_solver(problem, bc, fe.KrylovSolverOptions(solver="bicgstab", tol=1e-12),
+                                     symmetric_elimination=False)
     g, _ = _check(solver, bc, tp, vals, initial, 1e-6)
     assert not np.allclose(g, 2 * vals, atol=1e-6)
 
@@ -78,7 +80,7 @@ def test_bc_grad_linear_nonsymmetric():
 def test_bc_grad_newton(opts):
     import feax_amd as fe
     problem, bc, tp, vals, initial = _setup()
-    o = fe.DirectSolverOptions() if opts == "direct" else fe.KrylovSolverOptions(solver="cg")
+    o = fe.DirectSolverOptions(solver="cudss") if opts == "direct" else fe.KrylovSolverOptions(solver="cg")
     solver = fe.create_solver(problem, bc, solver_options=o, linear=False, newton_options=fe.NewtonOptions())
     _check(solver, bc, tp, vals, initial, 1e-6)
 
@@ -112,7 +114,7 @@ def test_bc_grad_hex8_neo_hookean_newton():
     rng = np.random.default_rng(2)
     mu0 = rng.uniform(0.5, 1.5, problem.num_cells)
     vals = bc.vals_numpy() + np.where(bc.vals_numpy() != 0, rng.uniform(-0.02, 0.02, len(bc.bc_rows)), 0.0)
-    solver = fe.create_solver(problem, bc, fe.DirectSolverOptions(),
+    solver = fe.create_solver(problem, bc, fe.DirectSolverOptions(solver="cudss"),
                               newton_options=fe.NewtonOptions(tol=1e-12, rel_tol=1e-14), linear=False)
     ig = fe.zero_like_initial_guess(problem, bc.replace_vals(vals))
 

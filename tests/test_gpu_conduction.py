@@ -118,7 +118,7 @@ def test_conduction_solves(opts):
     prob, _, L = _problem("conduction", dims=(12, 4, 4), flux=False)
     bc2 = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[0], 0.0), "all", 0.0),
                                 fe.DirichletBCSpec(lambda p: np.isclose(p[0], L[0]), "all", 100.0)]).create_bc(prob)
-    o = fe.DirectSolverOptions() if opts == "direct" else \
+    o = fe.DirectSolverOptions(solver="cudss") if opts == "direct" else \
         fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0, use_jacobi_preconditioner=True)
     k = fe.TracedParams.create_cell_var(prob, 1.7)
     print("conduction bar", opts, flush=True)

@@ -66,7 +66,7 @@ def test_plane_conduction_assembly_residual_and_solves():
     f[bc.bc_rows] = 0.0
     Td = spla.spsolve(K.tocsc(), f)
     for o in (fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0, use_jacobi_preconditioner=True),
-              fe.DirectSolverOptions()):
+              fe.DirectSolverOptions(solver="cudss")):
         Ts = np.asarray(fe.create_solver(prob, bc, o, linear=True)(tp))
         assert np.linalg.norm(Ts - Td) / np.linalg.norm(Td) < 1e-9
 

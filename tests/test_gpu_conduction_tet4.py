@@ -62,7 +62,7 @@ def test_tet_conduction_assembly_residual_solves_and_gradients():
     f[bc.bc_rows] = 0.0
     Td = spla.spsolve(m.scipy_csr(ref).tocsc(), f)
     for o in (fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0, use_jacobi_preconditioner=True),
-              fe.DirectSolverOptions()):
+              fe.DirectSolverOptions(solver="cudss")):
         assert np.linalg.norm(np.asarray(fe.create_solver(prob, bc, o, linear=True)(tp)) - Td) < 1e-9 * np.linalg.norm(Td)
     solver = fe.create_solver(prob, bc, fe.KrylovSolverOptions(solver="cg", tol=1e-13, atol=0.0,
                               use_jacobi_preconditioner=True), linear=True, return_solution=False)

@@ -103,7 +103,7 @@ def test_direct_solve_matches_scipy_direct_solve(case):
         prob, bc, rho = _simp((12, 8, 8))
         tp, _ = _tp(prob, rho)
         K, b = _oracle_system(prob, bc, rho)
-    solver = fe.create_solver(prob, bc, fe.DirectSolverOptions(), linear=True)
+    solver = fe.create_solver(prob, bc, fe.DirectSolverOptions(), linear=True, traced_params=tp)
     u = np.asarray(solver(tp))
     ud = spla.spsolve(K.tocsc(), b)
     err = np.linalg.norm(u - ud) / np.linalg.norm(ud)
@@ -117,7 +117,7 @@ def test_direct_solve_matches_scipy_direct_solve(case):
     # this direct solver, as feax's are (feax/solver.py:293-301)
     u2 = np.asarray(solver(tp))
     assert np.array_equal(u, u2)
-    u3 = np.asarray(fe.create_solver(prob, bc, linear=True)(tp))
+    u3 = np.asarray(fe.create_solver(prob, bc, linear=True, traced_params=tp)(tp))
     assert np.array_equal(u, u3)
 
 
@@ -125,7 +125,7 @@ def test_direct_gradient_matches_cg_gradient():
     import feax_amd as fe
     prob, bc, rho = _simp((8, 4, 4), seed=6)
     grads = []
-    for opts in (fe.DirectSolverOptions(), fe.KrylovSolverOptions(solver="cg", tol=1e-13, atol=0.0,
+    for opts in (fe.DirectSolverOptions(solver="cudss"), fe.KrylovSolverOptions(solver="cg", tol=1e-13, atol=0.0,
                                                                   use_jacobi_preconditioner=True)):
         solver = fe.create_solver(prob, bc, opts, linear=True, return_solution=False)
         tp, r = _tp(prob, rho, grad=True)
@@ -136,10 +136,10 @@ def test_direct_gradient_matches_cg_gradient():
     print("gradient rel diff direct vs cg", err)
     assert err < 1e-9
     # mixed: direct forward, CG adjoint; and CG forward, direct adjoint
-    for fwd, adj in ((fe.DirectSolverOptions(), fe.KrylovSolverOptions(solver="cg", tol=1e-13, atol=0.0,
+    for fwd, adj in ((fe.DirectSolverOptions(solver="cudss"), fe.KrylovSolverOptions(solver="cg", tol=1e-13, atol=0.0,
                                                                         use_jacobi_preconditioner=True)),
                      (fe.KrylovSolverOptions(solver="cg", tol=1e-13, atol=0.0, use_jacobi_preconditioner=True),
-                      fe.DirectSolverOptions())):
+                      fe.DirectSolverOptions(solver="cudss"))):
         solver = fe.create_solver(prob, bc, fwd, adjoint_solver_options=adj, linear=True, return_solution=False)
         tp, r = _tp(prob, rho, grad=True)
         (solver(tp) ** 2).sum().backward()

@@ -94,7 +94,7 @@ def test_newton_load_steps_match_oracle(precision):
     rng = np.random.default_rng(20261203)
     mu = rng.uniform(0.5, 1.5, prob.num_cells)
     if precision == "direct":   # the oracle's own linear solver: a direct solve
-        lin = fe.DirectSolverOptions()
+        lin = fe.DirectSolverOptions(solver="cudss")
     else:
         lin = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=0.0, maxiter=20000,
                                      use_jacobi_preconditioner=True, precision=precision)
@@ -135,7 +135,7 @@ def test_energy_gradient_through_newton(linear):
     mu0 = rng.uniform(0.5, 1.5, prob.num_cells)
     vals = np.where(is_top, -0.1 * dims[2], 0.0)
     bcs = bc.replace_vals(vals)
-    lin = fe.DirectSolverOptions() if linear == "direct" else \
+    lin = fe.DirectSolverOptions(solver="cudss") if linear == "direct" else \
         fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0, maxiter=20000, use_jacobi_preconditioner=True)
     nopts = fe.NewtonOptions(tol=1e-11, rel_tol=1e-13)
     solver = fe.create_solver(prob, bcs, lin, newton_options=nopts, linear=False, return_solution=False)

@@ -359,8 +359,8 @@ def test_bicgstab_nonsymmetric_elimination_matches_oracle(small_cantilever, spmv
     p, bc, tp = small_cantilever["problem"], small_cantilever["bc"], small_cantilever["tp"]
     opts = fe.KrylovSolverOptions(solver="bicgstab", tol=1e-10, atol=0.0, maxiter=10000,
                                   use_jacobi_preconditioner=True, spmv=spmv)
-    solver = fe.create_solver(p, bc, opts, linear=True, symmetric_elimination=False)
-    u = np.asarray(solver(tp, fe.zero_like_initial_guess(p, bc)))
+    solver = fe.create_linear_solver(p, bc, opts, symmetric_elimination=False)
+    u = _np(solver(tp, fe.zero_like_initial_guess(p, bc)))
     err = np.linalg.norm(u - small_cantilever["u_dir"]) / np.linalg.norm(small_cantilever["u_dir"])
     m, info = small_cantilever["m"], small_cantilever["info"]
     mask = np.zeros(m.n, bool)
@@ -397,7 +397,7 @@ def test_bicgstab_gradient_nonsymmetric_elimination_matches_cg():
     grads = []
     for solver_name, sym in (("bicgstab", False), ("cg", True)):
         opts = fe.KrylovSolverOptions(solver=solver_name, tol=1e-12, atol=0.0, use_jacobi_preconditioner=True)
-        solver = fe.create_solver(prob, bc, opts, linear=True, symmetric_elimination=sym, return_solution=False)
+        solver = fe.create_linear_solver(prob, bc, opts, symmetric_elimination=sym)
         r = torch.as_tensor(rho, device="cuda").requires_grad_(True)
         u = solver(fe.TracedParams(volume_vars=(r,), surface_vars=[(t,)]))
         (u * u).sum().backward()
@@ -467,9 +467,9 @@ def test_gmres_nonsymmetric_elimination_matches_oracle(dims, restart, spmv):
     prob, bc, rho = _simp_box(dims)
     opts = fe.KrylovSolverOptions(solver="gmres", tol=1e-10, atol=0.0, restart=restart,
                                   use_jacobi_preconditioner=True, spmv=spmv)
-    solver = fe.create_solver(prob, bc, opts, linear=True, symmetric_elimination=False)
+    solver = fe.create_linear_solver(prob, bc, opts, symmetric_elimination=False)
     t = fe.TracedParams.create_uniform_surface_var(prob, 0.5)
-    u = np.asarray(solver(fe.TracedParams(volume_vars=(torch.as_tensor(rho, device="cuda"),), surface_vars=[(t,)])))
+    u = _np(solver(fe.TracedParams(volume_vars=(torch.as_tensor(rho, device="cuda"),), surface_vars=[(t,)])))
     m = hex8.Model(prob.points, prob.cells.astype(np.int64), prob.boundary_inds_list)
     mat = hex8.Material("simp", nu=0.3, E0=1.0, Emin=1e-3, penal=3.0)
     u0 = np.zeros(m.n)
@@ -549,7 +549,7 @@ def test_gmres_gradient_nonsymmetric_elimination_matches_cg():
     for solver_name, sym in (("gmres", False), ("cg", True)):
         opts = fe.KrylovSolverOptions(solver=solver_name, tol=1e-12, atol=0.0, restart=64,
                                       use_jacobi_preconditioner=True)
-        solver = fe.create_solver(prob, bc, opts, linear=True, symmetric_elimination=sym, return_solution=False)
+        solver = fe.create_linear_solver(prob, bc, opts, symmetric_elimination=sym)
         r = torch.as_tensor(rho, device="cuda").requires_grad_(True)
         u = solver(fe.TracedParams(volume_vars=(r,), surface_vars=[(t,)]))
         (u * u).sum().backward()
@@ -591,7 +591,7 @@ def test_linear_solve_from_user_functions(small_cantilever, opts):
     import torch
     import feax_amd as fe
     p, bc, tp = small_cantilever["problem"], small_cantilever["bc"], small_cantilever["tp"]
-    o = fe.DirectSolverOptions() if opts == "direct" else \
+    o = fe.DirectSolverOptions(solver="cudss") if opts == "direct" else \
         fe.resolve_iterative_solver(fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0,
                                                            use_jacobi_preconditioner=True), fe.MatrixProperty.SPD)
     u0 = fe.zero_like_initial_guess(p, bc, device="cuda")
@@ -602,3 +602,55 @@ def test_linear_solve_from_user_functions(small_cantilever, opts):
     err = np.linalg.norm(_np(u) - ud) / np.linalg.norm(ud)
     print(opts, "linear_solve err vs scipy", err)
     assert err < 1e-8
+
+
+@pytest.mark.parametrize("view", ["UPPER", "LOWER"])
+@pytest.mark.parametrize("opts", ["cg", "direct"])
+def test_linear_solve_on_a_matrix_view(view, opts):
+    """linear_solve with an UPPER / LOWER J_bc solves the full symmetric
+    system A + A^T - diag(A) (feax/solvers/common.py:364-373), for Krylov and
+    direct options, and check_convergence with the view keeps the solution."""
+    import scipy.sparse as sp
+    import feax_amd as fe
+    from feax_amd.solvers.common import check_convergence
+
+    class LE(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.IsotropicLinearElastic(nu=0.3)
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.0, 0.0, 1.0))]
+
+    dims = (8, 3, 3)
+    mesh = fe.StructuredGrid(dims).to_mesh()
+    full = LE(mesh, vec=3, dim=3, location_fns=[lambda x: np.isclose(x[0], 8.0)])
+    pv = LE(mesh, vec=3, dim=3, location_fns=[lambda x: np.isclose(x[0], 8.0)], matrix_view=view)
+    E = np.random.default_rng(4).uniform(1.0, 2.0, pv.num_cells)
+    bcs = [fe.DirichletBCConfig([fe.DirichletBCSpec(lambda x: np.isclose(x[0], 0.0), "all", 0.0)]).create_bc(q)
+           for q in (full, pv)]
+    t = fe.TracedParams.create_uniform_surface_var(pv, 1e-2)
+    tp = fe.TracedParams(volume_vars=(E,), surface_vars=[(t,)])
+    Af = fe.create_J_bc_csr_function(full, bcs[0])(fe.zero_like_initial_guess(full, bcs[0], device="cuda"), tp)
+    b = -_np(fe.create_res_bc_function(full, bcs[0])(fe.zero_like_initial_guess(full, bcs[0], device="cuda"), tp))
+    ud = spl.spsolve(Af.to_scipy().tocsc(), b)
+    o = fe.DirectSolverOptions(solver="cudss") if opts == "direct" else \
+        fe.resolve_iterative_solver(fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0,
+                                                           use_jacobi_preconditioner=True, check_convergence=True),
+                                    fe.MatrixProperty.SPD)
+    u0 = fe.zero_like_initial_guess(pv, bcs[1], device="cuda")
+    print("linear_solve on a view", view, opts, flush=True)
+    Jv = fe.create_J_bc_csr_function(pv, bcs[1])(u0, tp)
+    n = pv.num_total_dofs_all_vars
+    V = sp.csr_matrix((_np(Jv.data), _np(Jv.indices), _np(Jv.indptr)), shape=(n, n))
+    assert (sp.tril(V, -1).nnz == 0) if view == "UPPER" else (sp.triu(V, 1).nnz == 0)
+    u, _ = fe.linear_solve(fe.create_J_bc_csr_function(pv, bcs[1]), fe.create_res_bc_function(pv, bcs[1]), u0,
+                           bcs[1], o, pv.matrix_view, traced_params=tp)
+    err = np.linalg.norm(_np(u) - ud) / np.linalg.norm(ud)
+    print(view, opts, "err vs scipy (full matrix)", err)
+    assert err < 1e-9
+    import torch
+    du = torch.as_tensor(ud, device="cuda")
+    kept = check_convergence(Jv, du, torch.as_tensor(b, device="cuda"), o, pv.matrix_view)
+    assert torch.isfinite(kept).all()
+    dropped = check_convergence(Jv, du, torch.as_tensor(b, device="cuda"), o, fe.MatrixView.FULL)
+    assert torch.isnan(dropped).all()        # the triangle alone is not the operator

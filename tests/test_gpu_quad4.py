@@ -140,7 +140,7 @@ def test_quad4_linear_solve_matches_direct_solve(opts):
     rho = _param("simp", prob)
     t = 0.5
     tp = fe.TracedParams(volume_vars=(rho,), surface_vars=[(fe.TracedParams.create_uniform_surface_var(prob, t),)])
-    o = fe.DirectSolverOptions() if opts == "direct" else \
+    o = fe.DirectSolverOptions(solver="cudss") if opts == "direct" else \
         fe.KrylovSolverOptions(solver="cg", tol=1e-11, atol=0.0, use_jacobi_preconditioner=True)
     print("quad4 solve", opts, flush=True)
     u = np.asarray(fe.create_solver(prob, bc, o, linear=True)(tp))
@@ -165,7 +165,7 @@ def test_quad4_newton_matches_oracle_newton():
     is_top = np.isclose(prob.points[node, 1], ny) & (comp == 1)
     m = _oracle(prob)
     mu = _param("neo_hookean", prob)
-    solver = fe.create_solver(prob, bc, fe.DirectSolverOptions(), newton_options=fe.NewtonOptions(tol=1e-10,
+    solver = fe.create_solver(prob, bc, fe.DirectSolverOptions(solver="cudss"), newton_options=fe.NewtonOptions(tol=1e-10,
                               rel_tol=1e-12), linear=False, return_solution=False)
     tp = fe.TracedParams(volume_vars=(mu,))
     u = torch.zeros(m.n, dtype=torch.float64, device="cuda")

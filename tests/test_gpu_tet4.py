@@ -128,7 +128,7 @@ def test_tet4_linear_solve_matches_direct_solve(opts):
     rho = _param("simp", prob)
     t = 0.5
     tp = fe.TracedParams(volume_vars=(rho,), surface_vars=[(fe.TracedParams.create_uniform_surface_var(prob, t),)])
-    o = fe.DirectSolverOptions() if opts == "direct" else \
+    o = fe.DirectSolverOptions(solver="cudss") if opts == "direct" else \
         fe.KrylovSolverOptions(solver="cg", tol=1e-11, atol=0.0, use_jacobi_preconditioner=True)
     u = np.asarray(fe.create_solver(prob, bc, o, linear=True)(tp))
     _, info = krylov.linear_solve(m, _mat("simp"), rho, [tet4.constant_traction(prob.boundary_inds_list[0], t)],
@@ -152,7 +152,7 @@ def test_tet4_newton_matches_oracle_newton():
     is_top = np.isclose(prob.points[node, 2], nz) & (comp == 2)
     m = _oracle(prob)
     mu = _param("neo_hookean", prob)
-    solver = fe.create_solver(prob, bc, fe.DirectSolverOptions(), newton_options=fe.NewtonOptions(tol=1e-10,
+    solver = fe.create_solver(prob, bc, fe.DirectSolverOptions(solver="cudss"), newton_options=fe.NewtonOptions(tol=1e-10,
                               rel_tol=1e-12), linear=False, return_solution=False)
     tp = fe.TracedParams(volume_vars=(mu,))
     u = torch.zeros(m.n, dtype=torch.float64, device="cuda")
@@ -237,7 +237,7 @@ def test_tet4_energy_gradient_through_newton():
     vals = np.where(np.isclose(prob.points[node, 2], nz) & (comp == 2), -0.05 * nz, 0.0)
     bcs = bc.replace_vals(vals)
     mu0 = _param("neo_hookean", prob, seed=7)
-    solver = fe.create_solver(prob, bcs, fe.DirectSolverOptions(), newton_options=fe.NewtonOptions(tol=1e-11,
+    solver = fe.create_solver(prob, bcs, fe.DirectSolverOptions(solver="cudss"), newton_options=fe.NewtonOptions(tol=1e-11,
                               rel_tol=1e-13), linear=False, return_solution=False)
     energy = create_energy_fn(prob)
     ig = torch.zeros(prob.num_total_dofs_all_vars, dtype=torch.float64, device="cuda")

@@ -71,7 +71,7 @@ def test_free_expansion_on_rollers_is_exact(opts):
     bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[0], 0.0), "x", 0.0),
                                fe.DirichletBCSpec(lambda p: np.isclose(p[1], 0.0), "y", 0.0),
                                fe.DirichletBCSpec(lambda p: np.isclose(p[2], 0.0), "z", 0.0)]).create_bc(prob)
-    o = fe.DirectSolverOptions() if opts == "direct" else \
+    o = fe.DirectSolverOptions(solver="cudss") if opts == "direct" else \
         fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0, use_jacobi_preconditioner=True)
     dT = 60.0
     tp = fe.TracedParams(volume_vars=(fe.TracedParams.create_cell_var(prob, 70.0),
