@@ -67,16 +67,20 @@ def test_auto_resolves_gmres_for_row_only_newton(capsys):
     p, bc, tp = _small()
     opts = fe.KrylovSolverOptions(tol=1e-12, atol=0.0, use_jacobi_preconditioner=True, restart=50)
     print("case: auto Newton, symmetric_elimination=False", flush=True)
-    solver = fe.create_solver(p, bc, opts, newton_options=fe.NewtonOptions(tol=1e-12, rel_tol=1e-12),
-                              linear=False, traced_params=tp, symmetric_elimination=False)
+    # stop at a relative residual of 1e-10: one accurate Newton step from the
+    # lifted guess reaches it (a linear material), and a tighter target would
+    # ask the line search for descent at the rounding level of ||R||
+    nopts = fe.NewtonOptions(tol=1e-14, rel_tol=1e-10)
+    solver = fe.create_solver(p, bc, opts, newton_options=nopts, linear=False, traced_params=tp,
+                              symmetric_elimination=False)
     assert "matrix_property=GENERAL -> gmres" in capsys.readouterr().out
     u = np.asarray(solver(tp))
     sym = fe.create_solver(p, bc, fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0,
                                                          use_jacobi_preconditioner=True),
-                           newton_options=fe.NewtonOptions(tol=1e-12, rel_tol=1e-12), linear=False)
+                           newton_options=nopts, linear=False)
     us = np.asarray(sym(tp))
     err = np.linalg.norm(u - us) / np.linalg.norm(us)
-    print("row-only gmres Newton vs symmetric cg Newton", err)
+    print("row-only gmres Newton vs symmetric cg Newton", err, solver.stats, sym.stats)
     assert err < 1e-8
     # the linear path of feax ignores symmetric_elimination (always symmetric)
     lin = fe.create_solver(p, bc, fe.KrylovSolverOptions(solver="cg", tol=1e-12, atol=0.0,
