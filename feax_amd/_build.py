@@ -38,17 +38,35 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
-    os.makedirs(OUT_DIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs, *LIBS, "-o", tmp]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError(f"hipcc failed ({r.returncode})")
+        raise RuntimeError(f"hipcc failed ({r.returncode}): {cmd[-1]}")
+
+
+def build(verbose=False, jobs=None):
+    """Compile every source to an object in parallel, then link the shared
+    library (one hipcc per source: each carries its own gfx950 code object)."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OUT_DIR, exist_ok=True)
+    obj_dir = os.path.join(OUT_DIR, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    cc = hipcc()
+    compile_flags = [fl for fl in HIPCC_FLAGS if fl != "-shared"]
+    cmds, objs = [], []
+    for src in SOURCES:
+        obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        cmds.append([cc, *compile_flags, "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-c",
+                     os.path.join(CSRC, src), "-o", obj])
+    jobs = jobs or max(1, min(len(cmds), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(jobs) as pool:
+        list(pool.map(lambda c: _run(c, verbose), cmds))
+    tmp = LIB + ".tmp"
+    _run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, *LIBS, "-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -229,6 +229,12 @@ def lib():
             raise NativeError(
                 f"{LIB_PATH} is missing: build it with `python -m feax_amd._build` "
                 "(there is no CPU fallback)")
+        # torch ships its own HIP runtime under the same soname
+        # (libamdhip64.so.7) as the one the library links against.  Loading
+        # torch first makes the process use one runtime, torch's, for the
+        # streams and device memory it hands to the library; loading the
+        # library first would put a second runtime beside torch's.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
