@@ -82,6 +82,8 @@ SIGNATURES = {
     "fx_pcg_workspace_bytes": (_SZ, [_I32, _I32]),
     "fx_pcg_f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P,
                           _P, _P, _P, _I32, _P, _SZ]),
+    "fx_pcg_f64_noalloc": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P,
+                                  _P, _P]),
     "fx_pcg_f32": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P,
                           _P, _P, _P, _I32, _P, _SZ]),
     "fx_pcg_f32f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P,

@@ -26,6 +26,7 @@
 // update.  FP64 partial sums, fixed order: bitwise repeatable.
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -609,6 +610,43 @@ __global__ __launch_bounds__(kBlock) void bc_lift_kernel(int nbc, const int* __r
 
 }  // namespace fx
 
+namespace fx {
+
+constexpr size_t kNoAllocScratchBytes = size_t(40) << 20;
+
+// Device scratch of fx_pcg_f64_noalloc: the PCG state, reduction partials
+// and the r, z, p, q vectors live here instead of in a caller workspace.
+// Each GPU holds its own copy for the life of the process.  In host code
+// the name is only a stand-in for the symbol: its device address comes from
+// hipGetSymbolAddress, once per device (noalloc_scratch()).
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch (looked up on the first
+// call on each device, with that device current).
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  FX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(FX_ELIMIT, "noalloc: device %d >= %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    FX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(FX_EINVAL, "noalloc: hipGetSymbolAddress returned a null address");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FX_OK;
+}
+
+}  // namespace
+}  // namespace fx
+
 using namespace fx;
 
 extern "C" {
@@ -703,6 +741,26 @@ int fx_pcg_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_
   if (flags & FX_PCG_DIFF3) return fail(FX_EINVAL, "fx_pcg_f64: FX_PCG_DIFF3 is for FP32 matrices");
   return run_pcg<double, double>(stream, n, nse, flags, indptr, indices, bcol, data, dinv, b, x0, tol, atol, maxiter,
                                  check_every, x, iters, rs, rec_p, rec_pq, rec_max, workspace, ws_bytes);
+}
+
+int fx_pcg_f64_noalloc(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
+                       const int32_t* indices, const int32_t* bcol, const double* data, const double* dinv,
+                       const double* b, const double* x0, double tol, double atol, int32_t maxiter,
+                       int32_t check_every, double* x, int32_t* iters, double* rs) {
+  if (flags & FX_PCG_DIFF3) return fail(FX_EINVAL, "fx_pcg_f64_noalloc: FX_PCG_DIFF3 is for FP32 matrices");
+  if (flags & FX_PCG_TIME) return fail(FX_EINVAL, "fx_pcg_f64_noalloc: FX_PCG_TIME creates events");
+  const size_t need = pcg_ws_bytes(std::max(n, 0), sizeof(double));
+  if (need > kNoAllocScratchBytes)
+    return fail(FX_ELIMIT, "fx_pcg_f64_noalloc: n = %d needs %zu bytes of scratch, more than its %zu", n, need,
+                kNoAllocScratchBytes);
+  void* ws = nullptr;
+  const int rc = noalloc_scratch(&ws);
+  if (rc) return rc;
+  // No graph capture (a graph and its capture stream are allocations): the
+  // chunks of check_every iterations are launched directly on the stream.
+  return run_pcg<double, double>(stream, n, nse, flags | FX_PCG_NO_GRAPH, indptr, indices, bcol, data, dinv, b, x0,
+                                 tol, atol, maxiter, check_every, x, iters, rs, nullptr, nullptr, 0, ws,
+                                 kNoAllocScratchBytes);
 }
 
 int fx_pcg_f32(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr, const int32_t* indices,

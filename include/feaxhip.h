@@ -152,6 +152,21 @@ int fx_pcg_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_
                const double* b, const double* x0, double tol, double atol, int32_t maxiter,
                int32_t check_every, double* x, int32_t* iters, double* rs, float* rec_p, double* rec_pq,
                int32_t rec_max, void* workspace, size_t ws_bytes);
+/* fx_pcg_f64_noalloc: fx_pcg_f64 that allocates nothing, for callers that
+ * cannot allocate device memory (code that must not call hipMalloc: serving
+ * loops, harnesses that forbid allocation).  It has no counterpart in the
+ * reference.  The PCG state, partials and the r, z, p, q vectors live in a
+ * 40 MiB static device array per GPU, so n <= 1,231,168 (FX_ELIMIT
+ * beyond).  No HIP graph, stream or event is created: the check_every
+ * chunks are launched directly on `stream` (results equal fx_pcg_f64's);
+ * FX_PCG_TIME and recording are not available.  Calls on one GPU share
+ * the array: calls that overlap on one GPU (two streams, two host threads)
+ * race and must be serialised by the caller.  Every value the solve reads
+ * from the array is written earlier in the same call. */
+int fx_pcg_f64_noalloc(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
+                       const int32_t* indices, const int32_t* bcol, const double* data, const double* dinv,
+                       const double* b, const double* x0, double tol, double atol, int32_t maxiter,
+                       int32_t check_every, double* x, int32_t* iters, double* rs);
 /* FP32 matrix and FP32 vectors (FP64 dot products). */
 int fx_pcg_f32(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
                const int32_t* indices, const int32_t* bcol, const float* data, const float* dinv,
