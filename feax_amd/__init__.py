@@ -13,7 +13,7 @@ from .assembler import (TracedStructure, create_energy_fn, create_J_bc_csr_funct
                         get_jacobian, get_jacobian_info, get_res)
 from .csr import CSRMatrix
 from .DCboundary import DirichletBC, DirichletBCConfig, DirichletBCSpec, dirichlet_bc_config
-from .mesh import Mesh, StructuredGrid
+from .mesh import Mesh, StructuredGrid, hex20_mesh
 from .problem import MatrixView, Problem
 from .solution import Solution
 from .solver import create_solver

@@ -68,6 +68,21 @@ SIGNATURES = {
     "fx_tet4_residual_workspace_bytes": (_SZ, [_I32]),
     "fx_tet4_residual_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P,
                                     _P, _P, _I32, _P, _P, _I32, _P, _P, _SZ]),
+    "fx_hex20_pattern": (_I32, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "fx_hex20_face_incidence": (_I32, [_I32, _I32, _P, _I32, _P, _P, _P]),
+    "fx_hex20_assemble_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                    _I32, _I32, _F64, _P, _P, _P, _SZ]),
+    "fx_hex20_assemble_f32": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                    _I32, _I32, _F64, _P, _P, _P, _SZ]),
+    "fx_hex20_assemble_workspace_bytes": (_SZ, [_I32, _I32, _I32]),
+    "fx_hex20_residual_workspace_bytes": (_SZ, [_I32]),
+    "fx_hex20_residual_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P,
+                                    _P, _P, _I32, _P, _P, _I32, _P, _P, _SZ]),
+    "fx_hex20_param_vjp_workspace_bytes": (_SZ, [_I32, _I32]),
+    "fx_hex20_param_vjp_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P,
+                                     _P, _SZ]),
+    "fx_hex20_simp_compliance_sens_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "fx_hex20_traction_vjp_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _P]),
     "fx_csr_is_block3": (_I32, [_I32, _P, _P, _P]),
     "fx_csr_spmv_f64": (_I32, [_P, _I32, _I64, _P, _P, _P, _P, _P]),
     "fx_csr_spmv_f32": (_I32, [_P, _I32, _I64, _P, _P, _P, _P, _P]),

@@ -1,4 +1,4 @@
-// Shared by the HEX8 and TET4 element kernels: the per-node gather of
+// Shared by the HEX8, TET4, QUAD4 and HEX20 element kernels: the per-node gather of
 // element matrices into the node's three CSR rows (deterministic: the
 // node's elements in ascending order, no atomics), the row finish with the
 // Dirichlet elimination, and the Dirichlet rows of the residual.
@@ -17,7 +17,7 @@ constexpr int kMaxDeg = 64;        // LDS accumulator: 9 * kMaxDeg values per wa
 // The node's three CSR rows from its LDS accumulator (acc: row i, block kb,
 // column j at i * len + 3 kb + j): Jacobi dinv, the difference form, and the
 // store with the Dirichlet elimination of feax/DCboundary.py:147-199.
-template <typename T, typename C, int NPE = 8, int V = 3>
+template <typename T, typename C, int NPE = 8, int V = 3, int SHIFT = 3>
 __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, int row0, int nval,
                                             const int* __restrict__ inc_ptr, const int* __restrict__ inc,
                                             const uint8_t* __restrict__ rank, const int* __restrict__ indices,
@@ -28,8 +28,8 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
   int kself = 0;
   {
     const int pk = inc[inc_ptr[nd]];
-    const int a0 = pk & 7;
-    kself = rank[NPE * NPE * static_cast<long>(pk >> 3) + (NPE + 1) * a0];
+    const int a0 = pk & ((1 << SHIFT) - 1);
+    kself = rank[NPE * NPE * static_cast<long>(pk >> SHIFT) + (NPE + 1) * a0];
   }
   if (dinv && lane < V) {
     // Jacobi: 1 / (K_bc[r, r] + shift) (feax/solvers/common.py:86-94); BC rows have diagonal 1.
@@ -73,16 +73,17 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
   }
 }
 
-// NPE nodes per element (8: HEX8, 4: TET4 / QUAD4), V DOFs per node (3, or
-// 2 in the plane): element matrices of V^2 NPE^2 values, blocks (a, b) at
-// V^2 (NPE a + b), ranks at NPE (NPE e + a) + b.
-template <typename T, typename C, int NPE = 8, int V = 3>
+// NPE nodes per element (8: HEX8, 4: TET4 / QUAD4, 20: HEX20), V DOFs per
+// node (3, or 2 in the plane): element matrices of V^2 NPE^2 values, blocks
+// (a, b) at V^2 (NPE a + b), ranks at NPE (NPE e + a) + b.  An incidence
+// entry is (e << SHIFT) | a; nodes have at most MAXDEG neighbours.
+template <typename T, typename C, int NPE = 8, int V = 3, int SHIFT = 3, int MAXDEG = kMaxDeg>
 __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
     int num_nodes, const int* __restrict__ inc_ptr, const int* __restrict__ inc, const uint8_t* __restrict__ rank,
     const int* __restrict__ indptr, const int* __restrict__ indices, const C* __restrict__ ke,
     const uint8_t* __restrict__ bc_mask, int symmetric, int diff_form, double jacobi_shift,
     double* __restrict__ dinv, T* __restrict__ data) {
-  __shared__ C acc_all[kNodesPerBlock][9 * kMaxDeg];
+  __shared__ C acc_all[kNodesPerBlock][9 * MAXDEG];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nd = blockIdx.x * kNodesPerBlock + wid;
   if (nd >= num_nodes) return;
@@ -90,12 +91,12 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
   const int row0 = indptr[V * nd];
   const int len = indptr[V * nd + 1] - row0;  // V * deg
   const int nval = V * len;
-  if (len > V * kMaxDeg) return;  // host guarantees deg <= kMaxDeg
+  if (len > V * MAXDEG) return;  // host guarantees deg <= MAXDEG
   for (int i = lane; i < nval; i += 64) acc[i] = C(0);
   __builtin_amdgcn_wave_barrier();
   for (int k = inc_ptr[nd]; k < inc_ptr[nd + 1]; ++k) {
     const int pk = inc[k];
-    const int e = pk >> 3, a = pk & 7;
+    const int e = pk >> SHIFT, a = pk & ((1 << SHIFT) - 1);
     const C* src = ke + V * V * NPE * NPE * static_cast<long>(e) + V * V * NPE * a;  // blocks (a, b), b < NPE
     const uint8_t* rk = rank + NPE * NPE * static_cast<long>(e) + NPE * a;
     // V^2 NPE values: lane idx -> block b = idx / V^2, entry t = idx % V^2;
@@ -107,8 +108,8 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
     }
     __builtin_amdgcn_wave_barrier();
   }
-  finish_rows<T, C, NPE, V>(nd, lane, acc, len, row0, nval, inc_ptr, inc, rank, indices, bc_mask, symmetric,
-                         diff_form, jacobi_shift, dinv, data);
+  finish_rows<T, C, NPE, V, SHIFT>(nd, lane, acc, len, row0, nval, inc_ptr, inc, rank, indices, bc_mask,
+                                  symmetric, diff_form, jacobi_shift, dinv, data);
 }
 
 __global__ void bc_res_kernel(int nbc, const int* __restrict__ rows, const double* __restrict__ vals,

@@ -27,13 +27,15 @@ const int kFaceVerts[6][4] = {{0, 1, 3, 2}, {0, 1, 4, 5}, {0, 3, 4, 7},
 const int kTetFaceVerts[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
 
 // The CSR pattern, element ranks and node incidence of a mesh of NPE-node
-// cells (the incidence entry of node a of element e is e * 8 + a).
+// cells (the incidence entry of node a of element e is (e << shift) | a:
+// shift 3 for up to 8 nodes per cell, 5 for HEX20).
 int cell_pattern(const char* name, int npe, int vec, int32_t num_nodes, int32_t num_cells, const int32_t* cells,
                  int64_t* nse_out, int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
-                 int32_t* inc_ptr, int32_t* inc) {
+                 int32_t* inc_ptr, int32_t* inc, int shift = 3) {
   if (num_nodes <= 0 || num_cells < 0 || !cells || !nse_out || !indptr)
     return fx::fail(FX_EINVAL, "%s: bad arguments", name);
-  if (static_cast<int64_t>(num_nodes) * vec >= INT32_MAX || static_cast<int64_t>(num_cells) * 8 >= INT32_MAX)
+  if (npe > (1 << shift)) return fx::fail(FX_EINVAL, "%s: %d nodes per cell need a wider incidence", name, npe);
+  if (static_cast<int64_t>(num_nodes) * vec >= INT32_MAX || (static_cast<int64_t>(num_cells) << shift) >= INT32_MAX)
     return fx::fail(FX_ELIMIT, "%s: mesh too large for int32 indices", name);
   const int64_t E = num_cells, N = num_nodes;
   // node -> (element, local) incidence, ascending element id per node.
@@ -48,7 +50,7 @@ int cell_pattern(const char* name, int npe, int vec, int32_t num_nodes, int32_t 
   {
     std::vector<int32_t> pos(iptr.begin(), iptr.end() - 1);
     for (int64_t e = 0; e < E; ++e)
-      for (int a = 0; a < npe; ++a) ilist[pos[cells[e * npe + a]]++] = static_cast<int32_t>(e * 8 + a);
+      for (int a = 0; a < npe; ++a) ilist[pos[cells[e * npe + a]]++] = static_cast<int32_t>((e << shift) | a);
   }
   // neighbour lists
   std::vector<int32_t> nptr(N + 1, 0);
@@ -59,7 +61,7 @@ int cell_pattern(const char* name, int npe, int vec, int32_t num_nodes, int32_t 
   for (int64_t nd = 0; nd < N; ++nd) {
     tmp.clear();
     for (int32_t k = iptr[nd]; k < iptr[nd + 1]; ++k) {
-      int32_t e = ilist[k] >> 3;
+      int32_t e = ilist[k] >> shift;
       for (int b = 0; b < npe; ++b) tmp.push_back(cells[static_cast<int64_t>(e) * npe + b]);
     }
     std::sort(tmp.begin(), tmp.end());
@@ -209,6 +211,47 @@ extern "C" int fx_quad4_face_incidence(int32_t num_nodes, int32_t num_cells, con
     for (int v = 0; v < 2; ++v) {
       int a = kQuadEdges[lf][v];
       finc[pos[cells[static_cast<int64_t>(e) * 4 + a]]++] = f * 8 + a;
+    }
+  }
+  return FX_OK;
+}
+
+// HEX20 (feax/basis.py:71-77): 20 nodes, meshio order; incidence entries
+// (e << 5) | a.
+extern "C" int fx_hex20_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse_out,
+                                int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                                int32_t* inc_ptr, int32_t* inc) {
+  return cell_pattern("fx_hex20_pattern", 20, 3, num_nodes, num_cells, cells, nse_out, max_degree, indptr, indices,
+                      elem_rank, inc_ptr, inc, 5);
+}
+
+// The 8 nodes of each HEX20 facet: its 4 corners (the HEX8 facet, in the
+// same order) and the 4 edge nodes between them.
+extern "C" int fx_hex20_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells,
+                                       int32_t num_faces, const int32_t* faces, int32_t* finc_ptr,
+                                       int32_t* finc) {
+  static const int kH20FaceNodes[6][8] = {{0, 1, 3, 2, 8, 9, 10, 11},   {0, 1, 4, 5, 8, 12, 16, 17},
+                                          {0, 3, 4, 7, 11, 15, 16, 19}, {1, 2, 5, 6, 9, 13, 17, 18},
+                                          {3, 2, 7, 6, 10, 14, 18, 19}, {4, 5, 7, 6, 12, 13, 14, 15}};
+  if (num_nodes <= 0 || num_faces < 0 || !cells || !finc_ptr || (num_faces > 0 && (!faces || !finc)))
+    return fx::fail(FX_EINVAL, "fx_hex20_face_incidence: bad arguments");
+  if ((static_cast<int64_t>(num_faces) << 5) >= INT32_MAX)
+    return fx::fail(FX_ELIMIT, "fx_hex20_face_incidence: too many faces");
+  std::vector<int32_t> cnt(static_cast<size_t>(num_nodes) + 1, 0);
+  for (int32_t f = 0; f < num_faces; ++f) {
+    int32_t e = faces[2 * f], lf = faces[2 * f + 1];
+    if (e < 0 || e >= num_cells || lf < 0 || lf > 5)
+      return fx::fail(FX_EINVAL, "fx_hex20_face_incidence: bad face (%d, %d)", e, lf);
+    for (int v = 0; v < 8; ++v) cnt[cells[static_cast<int64_t>(e) * 20 + kH20FaceNodes[lf][v]] + 1]++;
+  }
+  for (int32_t i = 0; i < num_nodes; ++i) cnt[i + 1] += cnt[i];
+  std::copy(cnt.begin(), cnt.end(), finc_ptr);
+  std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
+  for (int32_t f = 0; f < num_faces; ++f) {
+    int32_t e = faces[2 * f], lf = faces[2 * f + 1];
+    for (int v = 0; v < 8; ++v) {
+      int a = kH20FaceNodes[lf][v];
+      finc[pos[cells[static_cast<int64_t>(e) * 20 + a]]++] = (f << 5) | a;
     }
   }
   return FX_OK;

@@ -1,4 +1,4 @@
-"""HEX8 / TET4 / QUAD4 reference-element facts needed on the host.
+"""HEX8 / HEX20 / TET4 / QUAD4 reference-element facts needed on the host.
 
 Geometry itself (shape gradients, JxW, Nanson scale) is never stored: the
 HIP kernels recompute it per Gauss point from the node coordinates.  The
@@ -21,7 +21,13 @@ TET4_FACE_INDS = np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]], dtype=np
 # QUAD4 (feax/basis.py:90-95): basix quadrilateral edges through the
 # re-order [0, 1, 3, 2]; 2x2 Gauss points, 2 per edge; dim = vec = 2.
 QUAD4_FACE_INDS = np.array([[0, 1], [0, 3], [1, 2], [3, 2]], dtype=np.int64)
+# HEX20 (feax/basis.py:71-77): basix serendipity degree 2 in meshio order
+# (8 corners, then the edge nodes of these corner pairs); boundary faces are
+# selected by their 4 corners, as for HEX8; 2x2x2 Gauss points.
+HEX20_EDGES = np.array([(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4),
+                        (0, 4), (1, 5), (2, 6), (3, 7)], dtype=np.int64)
 ELEMENTS = {"HEX8": dict(npe=8, faces=HEX8_FACE_INDS, num_quads=NUM_QUADS, num_face_quads=NUM_FACE_QUADS, dim=3),
+            "HEX20": dict(npe=20, faces=HEX8_FACE_INDS, num_quads=NUM_QUADS, num_face_quads=NUM_FACE_QUADS, dim=3),
             "TET4": dict(npe=4, faces=TET4_FACE_INDS, num_quads=1, num_face_quads=1, dim=3),
             "QUAD4": dict(npe=4, faces=QUAD4_FACE_INDS, num_quads=4, num_face_quads=2, dim=2)}
 
@@ -40,11 +46,31 @@ def _hex8_face_ref(lf, s, t):
     return [(s, t, 0.0), (s, 0.0, t), (0.0, s, t), (1.0, s, t), (s, 1.0, t), (s, t, 1.0)][lf]
 
 
+def _hex20_values(xi):
+    """HEX20 basis values (M, 20): the 20-node nodal serendipity functions
+    psi re-combined to basix's DOFs (corner values, edge means):
+    phi_edge = 3/2 psi_edge, phi_corner = psi_corner - 1/4 sum of its edges' psi."""
+    V = _REF_NODES["HEX8"]
+    s = 2.0 * np.atleast_2d(xi) - 1.0
+    sg = 2.0 * V - 1.0
+    corner = np.prod(1.0 + sg[None] * s[:, None], axis=2) * ((sg[None] * s[:, None]).sum(axis=2) - 2.0) / 8.0
+    edge = np.empty((len(s), 12))
+    for k, (p, q) in enumerate(HEX20_EDGES):
+        ax = int(np.flatnonzero(V[p] != V[q])[0])
+        i, j = [d for d in range(3) if d != ax]
+        edge[:, k] = 0.25 * (1.0 - s[:, ax] ** 2) * (1.0 + sg[p, i] * s[:, i]) * (1.0 + sg[p, j] * s[:, j])
+    phi = np.concatenate([corner, 1.5 * edge], axis=1)
+    for v in range(8):
+        at = [k for k, (p, q) in enumerate(HEX20_EDGES) if v in (p, q)]
+        phi[:, v] -= 0.25 * edge[:, at].sum(axis=1)
+    return phi
+
+
 def reference_quad_points(ele_type):
     """Volume quadrature points (nq, dim) on the reference cell in the order
     the kernels use for quad variables (basix's: the first coordinate
     slowest; feax/basis.py:155-197)."""
-    if ele_type == "HEX8":
+    if ele_type in ("HEX8", "HEX20"):
         return np.array([(a, b, c) for a in _G2 for b in _G2 for c in _G2])
     if ele_type == "TET4":
         return np.array([(0.25, 0.25, 0.25)])
@@ -57,7 +83,7 @@ def reference_face_quad_points(ele_type):
     """Face quadrature points (num_faces, nfq, dim) on the reference cell
     (feax/basis.py:244-325): 2x2 Gauss points on each HEX8 facet, the
     centroid of each TET4 facet, 2 Gauss points along each QUAD4 edge."""
-    if ele_type == "HEX8":
+    if ele_type in ("HEX8", "HEX20"):
         return np.array([[_hex8_face_ref(lf, s, t) for s in _G2 for t in _G2] for lf in range(6)])
     V = _REF_NODES[ele_type]
     if ele_type == "TET4":
@@ -70,6 +96,8 @@ def reference_face_quad_points(ele_type):
 def shape_values(ele_type, xi):
     """Shape function values (M, npe) at reference points xi (M, dim)."""
     xi = np.atleast_2d(np.asarray(xi, float))
+    if ele_type == "HEX20":
+        return _hex20_values(xi)
     if ele_type == "TET4":
         return np.concatenate([1.0 - xi.sum(axis=1, keepdims=True), xi], axis=1)
     V = _REF_NODES[ele_type]

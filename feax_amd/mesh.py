@@ -110,3 +110,30 @@ def rectangle_mesh(Nx, Ny, domain_x=1.0, domain_y=1.0, origin=(0.0, 0.0), ele_ty
     n0 = i * (Ny + 1) + j
     cells = np.stack([n0, n0 + (Ny + 1), n0 + (Ny + 1) + 1, n0 + 1], axis=1).astype(np.int32)
     return Mesh(points, cells, ele_type="QUAD4")
+
+
+# HEX20 edge nodes (local 8..19): the midpoints of these corner pairs (the
+# meshio / Abaqus hexahedron20 order, feax/basis.py:71-77).
+HEX20_EDGES = np.array([(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4),
+                        (0, 4), (1, 5), (2, 6), (3, 7)], dtype=np.int64)
+
+
+def hex20_mesh(dims, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0)):
+    """A structured HEX20 box (feax reads HEX20 meshes from files; this
+    builds one): the corners of StructuredGrid(dims) with their node ids and
+    HEX8 corner order, plus one node at the midpoint of every grid edge,
+    numbered after the corners in the order the cells first list them."""
+    grid = StructuredGrid(dims, spacing, origin)
+    m8 = grid.to_mesh()
+    N8 = len(m8.points)
+    pairs = np.sort(m8.cells[:, HEX20_EDGES], axis=2).reshape(-1, 2)       # (12 E, 2)
+    key = pairs[:, 0] * N8 + pairs[:, 1]
+    uniq, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")
+    new_id = np.empty(len(uniq), np.int64)
+    new_id[order] = np.arange(len(uniq))
+    edge_nodes = (N8 + new_id[inv]).reshape(len(m8.cells), 12)
+    firsts = pairs[first[order]]
+    mids = 0.5 * (m8.points[firsts[:, 0]] + m8.points[firsts[:, 1]])
+    return Mesh(np.concatenate([m8.points, mids]), np.concatenate([m8.cells, edge_nodes], axis=1),
+                ele_type="HEX20")

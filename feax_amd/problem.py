@@ -79,14 +79,14 @@ class Problem:
             self.mesh, self.vec = self.mesh[0], self.vec[0]
             self.ele_type = self.ele_type[0] if isinstance(self.ele_type, list) else self.ele_type
             self.gauss_order = self.gauss_order[0] if isinstance(self.gauss_order, list) else self.gauss_order
-        ok = ((self.ele_type in ("HEX8", "TET4") and self.vec == 3 and self.dim == 3)
+        ok = ((self.ele_type in ("HEX8", "HEX20", "TET4") and self.vec == 3 and self.dim == 3)
               or (self.ele_type == "QUAD4" and self.vec == 2 and self.dim == 2)
               or (self.ele_type == "HEX8" and self.vec == 1 and self.dim == 3)
               or (self.ele_type == "QUAD4" and self.vec == 1 and self.dim == 2)
               or (self.ele_type == "TET4" and self.vec == 1 and self.dim == 3))
         if not ok:
             raise NotImplementedError(
-                f"the HIP path covers HEX8 and TET4 with vec=3, dim=3, QUAD4 with vec=2, dim=2 and scalar HEX8 "
+                f"the HIP path covers HEX8, HEX20 and TET4 with vec=3, dim=3, QUAD4 with vec=2, dim=2 and scalar HEX8 "
                 f"(vec=1, dim=3) (got {self.ele_type}, vec={self.vec}, dim={self.dim})")
         default_order = 0 if self.ele_type == "TET4" else 2
         if self.gauss_order not in (None, default_order):
@@ -163,13 +163,18 @@ class Problem:
         allf = (np.concatenate(faces) if faces else np.zeros((0, 2), np.int64)).astype(np.int32)
         self.all_faces = np.ascontiguousarray(allf)
         finc_ptr = np.zeros(N + 1, dtype=np.int32)
-        finc = np.zeros(max(len(element_info(self.ele_type)["faces"][0]) * len(allf), 1), dtype=np.int32)
+        # nodes per face in the face incidence: the facet's vertices (HEX20:
+        # its 4 corners and 4 edge nodes)
+        fnodes = 8 if self.ele_type == "HEX20" else len(element_info(self.ele_type)["faces"][0])
+        finc = np.zeros(max(fnodes * len(allf), 1), dtype=np.int32)
         nat.check(getattr(L, f"fx_{tag}_face_incidence")(N, E, nat.ptr(self.cells), len(allf),
                                                          nat.ptr(self.all_faces), nat.ptr(finc_ptr), nat.ptr(finc)),
                   f"fx_{tag}_face_incidence")
         self.face_inc_ptr, self.face_inc = finc_ptr, finc
-        if self.csr_max_degree > 64:
-            raise NotImplementedError(f"node degree {self.csr_max_degree} > 64 not supported by the assembly kernel")
+        max_deg = 96 if self.ele_type == "HEX20" else 64
+        if self.csr_max_degree > max_deg:
+            raise NotImplementedError(f"node degree {self.csr_max_degree} > {max_deg} not supported by the "
+                                      "assembly kernel")
 
     @property
     def csr_row_of_slot(self):

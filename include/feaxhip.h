@@ -323,6 +323,60 @@ int fx_tet4_simp_compliance_sens_f64(void* stream, int32_t num_cells, const doub
 int fx_tet4_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, const double* y, const double* z,
                              const int32_t* cells, const int32_t* faces, const double* fdir, const double* lam,
                              const uint8_t* bc_mask, double scale, double* grad_svar);
+/* ------------------------------------------------------------------ */
+/* HEX20 (feax/basis.py:71-77: basix serendipity degree 2 -- DOFs the corner
+ * values and the edge means --, meshio node order: 8 corners then the edge
+ * nodes of (0,1) (1,2) (2,3) (3,0) (4,5) (5,6) (6,7) (7,4) (0,4) (1,5)
+ * (2,6) (3,7); 2x2x2 Gauss points, 2x2 per facet; faces selected by their 4
+ * corners): the structure builders and two-pass kernels of the TET4 entry
+ * points, with cells (E, 20), elem_rank E * 400 (rank of node b in node
+ * a's neighbour list at 400 e + 20 a + b), node incidence entries
+ * (e << 5) | a, face incidence entries (f << 5) | a over each facet's 8
+ * nodes, and 4 surface values per face (svar F * 4, fdir F * 3).  The
+ * workspace is required (3600 doubles per element for the matrix, 60 for
+ * the residual); FP64 element arithmetic (FX_ASM_DIFF3 honoured,
+ * FX_ASM_COMPUTE32 ignored); nodes may have up to 96 neighbours.  Materials:
+ * linear elastic, SIMP, Neo-Hookean; no thermal strain. */
+int fx_hex20_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse,
+                     int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                     int32_t* inc_ptr, int32_t* inc);
+int fx_hex20_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int32_t num_faces,
+                            const int32_t* faces, int32_t* finc_ptr, int32_t* finc);
+int fx_hex20_assemble_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                          const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                          int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, double* data,
+                          void* workspace, size_t ws_bytes);
+int fx_hex20_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                          const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                          int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, float* data,
+                          void* workspace, size_t ws_bytes);
+size_t fx_hex20_assemble_workspace_bytes(int32_t num_cells, int32_t value_bytes, int32_t flags);
+int fx_hex20_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                          const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                          const fx_material* mat, const double* param, const double* u, int32_t num_faces,
+                          const int32_t* faces, const int32_t* finc_ptr, const int32_t* finc, const double* svar,
+                          const double* fdir, int32_t num_bc, const int32_t* bc_rows, const double* bc_vals,
+                          int32_t negate, double* res, void* workspace, size_t ws_bytes);
+size_t fx_hex20_residual_workspace_bytes(int32_t num_cells);
+/* TET4 pullbacks, as fx_hex8_param_vjp_f64 / fx_simp_compliance_sens_f64 /
+ * fx_hex8_traction_vjp_f64 at the single Gauss point (node variables need
+ * fx_hex20_param_vjp_workspace_bytes: one double per element); the traction
+ * gradient has one value per face. */
+size_t fx_hex20_param_vjp_workspace_bytes(int32_t num_cells, int32_t param_loc);
+int fx_hex20_param_vjp_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                           const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                           const fx_material* mat, const double* param, const double* u, const double* lam,
+                           const uint8_t* bc_mask, double scale, double* grad, void* workspace, size_t ws_bytes);
+int fx_hex20_simp_compliance_sens_f64(void* stream, int32_t num_cells, const double* x, const double* y,
+                                      const double* z, const int32_t* cells, const fx_material* mat,
+                                      const double* param, const double* u, double* dc);
+int fx_hex20_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, const double* y, const double* z,
+                              const int32_t* cells, const int32_t* faces, const double* fdir, const double* lam,
+                              const uint8_t* bc_mask, double scale, double* grad_svar);
 /* TET4 Neo-Hookean strain energy, as fx_hex8_energy_f64 at the single
  * Gauss point (workspace fx_tet4_energy_workspace_bytes). */
 size_t fx_tet4_energy_workspace_bytes(int32_t num_cells);

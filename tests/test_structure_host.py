@@ -99,8 +99,10 @@ def test_unsupported_problems_raise():
     assert fe.Problem(tets, vec=1, dim=3, ele_type="TET4").kernel_tag == "tet4s"
     with pytest.raises(NotImplementedError):
         fe.Problem(tets, vec=2, dim=3, ele_type="TET4")
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(ValueError, match=r"\(E, 20\)"):
         fe.Problem(mesh, vec=3, dim=3, ele_type="HEX20")
+    with pytest.raises(NotImplementedError):
+        fe.Problem(mesh, vec=3, dim=3, ele_type="TET10")
     assert fe.Problem(mesh, vec=1, dim=3).kernel_tag == "hex8s"
     with pytest.raises(ValueError, match=r"\(E, 4\)"):
         fe.Problem(mesh, vec=3, dim=3, ele_type="TET4")
