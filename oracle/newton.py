@@ -28,9 +28,12 @@ def residual_bc(model, u, param, mat, bc_rows, bc_vals, tractions=()):
 
 
 def newton(model, mat, param, bc_rows, bc_vals, u0, tractions=(), tol=1e-6, rel_tol=1e-8, max_iter=100,
-           c1=1e-4, rho=0.5, max_backtracks=30):
-    """Returns (u, iterations, residual norms)."""
+           c1=1e-4, rho=0.5, max_backtracks=30, linear="direct", linear_tol=1e-13):
+    """Returns (u, iterations, residual norms).  linear: "direct" (scipy
+    spsolve) or "pcg" (the oracle Jacobi-PCG to relative linear_tol, for an
+    SPD tangent; faster at 100k+ DOF)."""
     import scipy.sparse.linalg as spla
+    from .krylov import jacobi_dinv, pcg
     mask = np.zeros(model.n, bool)
     mask[bc_rows] = True
     u = np.array(u0, float)
@@ -40,7 +43,10 @@ def newton(model, mat, param, bc_rows, bc_vals, u0, tractions=(), tol=1e-6, rel_
     it = 0
     while it < max_iter and rn > tol and rn / max(r0, 1e-30) > rel_tol:
         K = model.scipy_csr(apply_bc_csr(model.csr_values(u, param, mat), model.csr, mask, bc_rows))
-        du = spla.spsolve(K.tocsc(), -r)
+        if linear == "pcg":
+            du, _, _ = pcg(K, -r, None, linear_tol, 0.0, 200000, jacobi_dinv(K.diagonal()))
+        else:
+            du = spla.spsolve(K.tocsc(), -r)
         alpha, ok = 1.0, False
         for _ in range(max_backtracks):
             trial = u + alpha * du
@@ -56,3 +62,22 @@ def newton(model, mat, param, bc_rows, bc_vals, u0, tractions=(), tol=1e-6, rel_
         it += 1
         hist.append(rn)
     return u, it, hist
+
+
+def linearized_predictor(model, mat, param, bc_rows, bc_vals, u_prev, tractions=(), linear_tol=1e-13):
+    """u_prev + d with K(u_prev) d = -R(u_prev) on the free DOFs and d = bc_vals -
+    u_prev on the Dirichlet DOFs (a first-order continuation predictor; the
+    Newton solve that follows is unchanged)."""
+    from .krylov import jacobi_dinv, pcg
+    mask = np.zeros(model.n, bool)
+    mask[bc_rows] = True
+    K = model.scipy_csr(model.csr_values(u_prev, param, mat)).tocsr()
+    r = model.residual(u_prev, param, mat, tractions)
+    d = np.zeros(model.n)
+    d[bc_rows] = bc_vals - u_prev[bc_rows]
+    free = np.flatnonzero(~mask)
+    Kff = K[free][:, free]
+    rhs = -r[free] - K[free] @ d
+    df, _, _ = pcg(Kff.tocsr(), rhs, None, linear_tol, 0.0, 200000, jacobi_dinv(Kff.diagonal()))
+    d[free] = df
+    return u_prev + d

@@ -18,6 +18,16 @@ Fixtures (npz, data only):
   simp_120x40x20_oracle.npz       120x40x20 SIMP cantilever (rho ~ U(0.3,1) seeded,
                       unit patch load): oracle PCG to rel 1e-12, compliance,
                       closed-form dC/drho at 4096 seeded cells, max|dC/drho|
+  batch64_oracle.npz  the seeded set of 64 uneven SIMP cantilevers (nx ~ U{20..80},
+                      ny, nz ~ U{8..24}, rho ~ U(0.3,1), traction 10^U(-4,-2)):
+                      dims of all 64, and for 5-6 of them (the smallest, the
+                      largest, the median, the first three) the oracle Jacobi-PCG solution to
+                      rel 1e-12 at 1024 seeded DOFs, max|u|
+  neohookean_100x20x20_oracle.npz   Neo-Hookean block 100x20x20 (mu ~ U(0.5,1.5) per
+                      cell, kappa 10, bottom clamped, top u_z = -0.4 s, s = 1..5):
+                      oracle Newton (Jacobi-PCG solves to rel 1e-13, started from
+                      the linearised predictor, to |R| <= 1e-11) per step; final
+                      state at 4096 seeded DOFs, max|u|, Newton iterations
 """
 
 import argparse
@@ -132,13 +142,82 @@ def simp_120x40x20_oracle():
              compliance=adjoint.compliance(m, u, [faces], tr), iters=info["iters"])
 
 
+def batch64_instances(seed=SEED, nseg=64):
+    """The 64-instance set in the draw order of the batched benchmark: per
+    instance nx, ny, nz, then rho (cell order), then the traction."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nseg):
+        nx, ny, nz = int(rng.integers(20, 81)), int(rng.integers(8, 25)), int(rng.integers(8, 25))
+        rho = rng.uniform(0.3, 1.0, nx * ny * nz)
+        t = 10.0 ** rng.uniform(-4, -2)
+        out.append(((nx, ny, nz), rho, t))
+    return out
+
+
+def batch64_oracle():
+    inst = batch64_instances()
+    dofs = [3 * (d[0] + 1) * (d[1] + 1) * (d[2] + 1) for d, _, _ in inst]
+    pick = sorted({int(np.argmin(dofs)), int(np.argmax(dofs)), int(np.argsort(dofs)[len(dofs) // 2]), 0, 1, 2})
+    mat = hex8.Material("simp", nu=0.3, E0=1.0, Emin=1e-9, penal=3.0)
+    res = dict(dims=np.array([d for d, _, _ in inst]), tractions=np.array([t for _, _, t in inst]),
+               picked=np.array(pick))
+    for k in pick:
+        t0 = time.time()
+        dims, rho, t = inst[k]
+        pts, cells = structure.structured_grid(dims)
+        L = float(dims[0])
+        faces = structure.boundary_faces(pts, cells, lambda p: np.isclose(p[0], L))
+        m = hex8.Model(pts, cells, [faces])
+        rows, vals = structure.dirichlet_rows(pts, [(lambda p: np.isclose(p[0], 0.0), "all", 0.0)])
+        u, info = krylov.linear_solve(m, mat, rho, [hex8.constant_traction(faces, t)], rows, vals, tol=1e-12,
+                                      atol=0.0, maxiter=200000)
+        sel = np.sort(np.random.default_rng(SEED + k).choice(m.n, 1024, replace=False))
+        res[f"dofs_{k}"] = sel
+        res[f"u_{k}"] = u[sel]
+        res[f"umax_{k}"] = np.abs(u).max()
+        print(f"instance {k} {dims}: {m.n} DOF, {info['iters']} iterations, {time.time() - t0:.0f} s", flush=True)
+    np.savez(os.path.join(HERE, "batch64_oracle.npz"), **res)
+
+
+def neohookean_100x20x20_oracle():
+    from oracle import newton
+    dims = (100, 20, 20)
+    nz = dims[2]
+    pts, cells = structure.structured_grid(dims)
+    m = hex8.Model(pts, cells)
+    bottom, _ = structure.dirichlet_rows(pts, [(lambda p: np.isclose(p[2], 0.0), "all", 0.0)])
+    rows, _ = structure.dirichlet_rows(pts, [(lambda p: np.isclose(p[2], 0.0), "all", 0.0),
+                                             (lambda p: np.isclose(p[2], nz), 2, 0.0)])
+    is_top = ~np.isin(rows, bottom)
+    mu = np.random.default_rng(SEED).uniform(0.5, 1.5, m.E)
+    mat = hex8.Material("neo_hookean", kappa=10.0)
+    u = np.zeros(m.n)
+    iters = []
+    for s in range(1, 6):
+        t0 = time.time()
+        vals = np.where(is_top, -0.4 * s, 0.0)
+        # The converged state does not depend on the initial guess; the
+        # linearised predictor keeps every Newton tangent SPD (Jacobi-PCG
+        # solves to rel 1e-13 instead of sparse LU at 134k DOF).
+        u0 = newton.linearized_predictor(m, mat, mu, rows, vals, u)
+        u, it, hist = newton.newton(m, mat, mu, rows, vals, u0, tol=1e-11, rel_tol=1e-12, linear="pcg")
+        iters.append(it)
+        print(f"step {s}: {it} Newton iterations, |R| {hist[-1]:.3e} (rel {hist[-1] / hist[0]:.2e}), "
+              f"{time.time() - t0:.0f} s", flush=True)
+    sel = np.sort(np.random.default_rng(SEED).choice(m.n, 4096, replace=False))
+    np.savez(os.path.join(HERE, "neohookean_100x20x20_oracle.npz"), dofs=sel, u=u[sel], u_max=np.abs(u).max(),
+             u_norm=np.linalg.norm(u), newton_iters=np.array(iters))
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--large", action="store_true")
     ap.add_argument("--only", default=None)
     a = ap.parse_args()
     jobs = [fixture_2x2x2, cantilever_50x5x5_samples, make_ke_3d, simp_4x2x2] + (
-        [cantilever_200x40x40_oracle, simp_120x40x20_oracle] if a.large else [])
+        [cantilever_200x40x40_oracle, simp_120x40x20_oracle, batch64_oracle, neohookean_100x20x20_oracle]
+        if a.large else [])
     for job in jobs:
         if a.only and job.__name__ != a.only:
             continue

@@ -1,0 +1,238 @@
+"""The benchmark configurations at their full sizes.
+
+* 64 uneven SIMP cantilevers in one batch (2.8 M DOF; nx ~ U{20..80},
+  ny, nz ~ U{8..24}, the seeded set of the batched benchmark): every
+  instance's FP64 residual of the returned state, recomputed here instance
+  by instance, is <= 1e-8 of its right-hand side, for the FP64 and the
+  mixed form; five or six instances match the oracle's Jacobi-PCG to rel
+  1e-12 at seeded DOFs (tests/golden/batch64_oracle.npz).
+* Neo-Hookean block 100x20x20 (133,623 DOF), top compressed by 10 % in 5
+  load steps: each step converges (free residual <= 1e-8 of the first);
+  the final state matches the oracle's Newton with direct solves at seeded
+  DOFs (tests/golden/neohookean_100x20x20_oracle.npz); dPi/dmu by autograd
+  equals the equilibrium partial derivative.  Run with feax's initial guess
+  (previous state with the new Dirichlet values) and DirectSolverOptions,
+  and with the linearised predictor and inner CG of the benchmark.
+* 200x40x40 cantilever with the headline form (Jacobi-PCG, mixed) and
+  120x40x20 SIMP with the FP32 form and the autograd adjoint, against the
+  oracle fixtures.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from conftest import cantilever
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SEED = 20261203
+
+
+def _load(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+def _plane_x(value):
+    return lambda p: np.isclose(p[0], value)
+
+
+def _batch64():
+    """The 64 instances in the draw order of the batched benchmark and of
+    tests/golden/make_golden.py (nx, ny, nz, rho, traction per instance)."""
+    import feax_amd as fe
+
+    class Simp(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.SIMPLinearElastic(nu=0.3, E0=1.0, Emin=1e-9, penal=3.0)
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.0, 0.0, 1.0))]
+
+    rng = np.random.default_rng(SEED)
+    inst = []
+    for _ in range(64):
+        nx, ny, nz = int(rng.integers(20, 81)), int(rng.integers(8, 25)), int(rng.integers(8, 25))
+        prob = Simp(fe.StructuredGrid((nx, ny, nz)).to_mesh(), vec=3, dim=3, location_fns=[_plane_x(nx)])
+        bc = fe.DirichletBCConfig([fe.DirichletBCSpec(_plane_x(0.0), "all", 0.0)]).create_bc(prob)
+        rho = rng.uniform(0.3, 1.0, prob.num_cells)
+        t = 10.0 ** rng.uniform(-4, -2)
+        tp = fe.TracedParams(volume_vars=(rho,),
+                             surface_vars=[(fe.TracedParams.create_uniform_surface_var(prob, t),)])
+        inst.append((prob, bc, tp, (nx, ny, nz), t))
+    return inst
+
+
+@pytest.fixture(scope="module")
+def batch64():
+    import torch
+    import feax_amd as fe
+    from feax_amd.batch import BatchedProblem, batch_bc, batch_params
+    inst = _batch64()
+    f = _load("batch64_oracle.npz")
+    assert np.array_equal(f["dims"], np.array([i[3] for i in inst]))
+    assert np.allclose(f["tractions"], [i[4] for i in inst], rtol=1e-15, atol=0)
+    bp = BatchedProblem([i[0] for i in inst])
+    bcb = batch_bc(bp, [i[1] for i in inst])
+    tpb = batch_params(bp, [i[2] for i in inst])
+    ts = fe.TracedStructure.from_problem(bp)
+    tpd = fe.TracedParams(volume_vars=(torch.as_tensor(tpb.volume_vars[0], device="cuda"),),
+                          surface_vars=[(torch.as_tensor(tpb.surface_vars[0][0], device="cuda"),)])
+    return dict(inst=inst, bp=bp, bc=bcb, ts=ts, tp=tpd, f=f)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch64_every_instance_converges_and_matches_the_oracle(batch64, precision):
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import residual
+    from feax_amd.batch import create_batched_solver
+    from feax_amd.solvers.common import device_norm
+    bp, f = batch64["bp"], batch64["f"]
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=50000, use_jacobi_preconditioner=True,
+                                  precision=precision)
+    solver = create_batched_solver(bp, batch64["bc"], opts, traced_structure=batch64["ts"])
+    print("case: 64 uneven SIMP instances", precision, bp.num_total_dofs_all_vars, "DOF", flush=True)
+    u = solver(batch64["tp"])
+    torch.cuda.synchronize()
+    worst = 0.0
+    for k, ((prob, bc, tp, dims, t), uk) in enumerate(zip(batch64["inst"], bp.split(u))):
+        ts = fe.TracedStructure.from_problem(prob)
+        tpk = fe.TracedParams(volume_vars=(torch.as_tensor(tp.volume_vars[0], device="cuda"),),
+                              surface_vars=[(torch.as_tensor(tp.surface_vars[0][0], device="cuda"),)])
+        uk = uk.contiguous()
+        r = residual(prob, ts, tpk, uk, bc, negate=True)
+        r0 = residual(prob, ts, tpk, torch.zeros_like(uk), bc, negate=True)
+        rel = device_norm(r) / device_norm(r0)
+        worst = max(worst, rel)
+        assert rel <= 1e-8, (k, dims, rel)
+        if k in f["picked"]:
+            sel = f[f"dofs_{k}"]
+            err = np.abs(uk.cpu().numpy()[sel] - f[f"u_{k}"]).max() / float(f[f"umax_{k}"])
+            print(f"instance {k} {dims}: relres {rel:.2e}, sampled oracle err {err:.2e}")
+            assert err <= 1e-6
+    print(precision, "max relres over 64 instances", worst, solver.stats)
+
+
+def _neohookean():
+    import feax_amd as fe
+    dims = (100, 20, 20)
+
+    class NH(fe.Problem):
+        def get_energy_density(self):
+            return fe.materials.CompressibleNeoHookean(kappa=10.0)
+
+    prob = NH(fe.StructuredGrid(dims).to_mesh(), vec=3, dim=3)
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[2], 0.0), "all", 0.0),
+                               fe.DirichletBCSpec(lambda p: np.isclose(p[2], 20.0), "z", 0.0)]).create_bc(prob)
+    node, comp = bc.bc_rows // 3, bc.bc_rows % 3
+    is_top = np.isclose(prob.points[node, 2], 20.0) & (comp == 2)
+    mu = np.random.default_rng(SEED).uniform(0.5, 1.5, prob.num_cells)
+    return prob, bc, is_top, mu
+
+
+@pytest.mark.parametrize("route", ["feax-guess-direct", "predictor-cg"])
+def test_neohookean_100x20x20_five_steps(route):
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import create_energy_fn, residual, strain_energy
+    from feax_amd.solvers.common import device_norm
+    from feax_amd.solvers.newton import linearized_guess
+    f = _load("neohookean_100x20x20_oracle.npz")
+    prob, bc, is_top, mu0 = _neohookean()
+    ts = fe.TracedStructure.from_problem(prob)
+    if route == "feax-guess-direct":
+        lin = fe.DirectSolverOptions(solver="cudss")
+    else:
+        lin = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=1e-10, maxiter=20000,
+                                     use_jacobi_preconditioner=True)
+    solver = fe.create_solver(prob, bc, lin, newton_options=fe.NewtonOptions(tol=1e-10, rel_tol=1e-10),
+                              linear=False, return_solution=False, traced_structure=ts)
+    mu = torch.tensor(mu0, device="cuda", requires_grad=True)
+    tp = fe.TracedParams(volume_vars=(mu,))
+    tpd = fe.TracedParams(volume_vars=(mu.detach(),))
+    u = torch.zeros(ts.n, dtype=torch.float64, device="cuda")
+    for s in range(1, 6):
+        print("case: Neo-Hookean 100x20x20", route, "step", s, flush=True)
+        bcs = bc.replace_vals(np.where(is_top, -0.4 * s, 0.0))
+        if route == "feax-guess-direct":
+            guess = u.detach().clone()            # previous state with the new Dirichlet values
+            guess[torch.as_tensor(bcs.bc_rows.astype(np.int64), device="cuda")] = torch.as_tensor(
+                bcs.bc_vals, device="cuda")
+        else:
+            guess = linearized_guess(prob, bcs, tpd, u.detach(), ts)
+        u = solver(tp if s == 5 else tpd, guess, bcs)
+        st = solver.stats
+        r = residual(prob, ts, tpd, u.detach(), bcs)
+        r_first = st.res_history[0]
+        rel = device_norm(r) / r_first
+        print(f"step {s}: newton {st.newton_iters}, |R| {device_norm(r):.3e}, rel {rel:.2e}")
+        assert rel <= 1e-8
+    uh = u.detach().cpu().numpy()
+    err = np.abs(uh[f["dofs"]] - f["u"]).max() / float(f["u_max"])
+    print(route, "final state vs oracle Newton: max sampled err / max|u|", err)
+    assert err <= 1e-8
+    # dPi/dmu through the Newton adjoint equals the equilibrium partial
+    energy = create_energy_fn(prob, ts)
+    Pi = energy(u, tp)
+    Pi.backward()
+    _, partial = strain_energy(prob, ts, tpd, u.detach(), with_partial=True)
+    gerr = float((mu.grad - partial).abs().max() / partial.abs().max())
+    print(route, "dPi/dmu autograd vs partial", gerr)
+    assert gerr <= 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_c2_jacobi_pcg_headline_forms_match_oracle(precision):
+    """200x40x40 (1,013,643 DOF) with the headline Jacobi-PCG at rel 1e-8,
+    FP64 and mixed, vs the oracle solve (cantilever_200x40x40_oracle.npz):
+    the error the benchmark reports, bounded."""
+    import feax_amd as fe
+    f = _load("cantilever_200x40x40_oracle.npz")
+    p, bc, tp = cantilever((200, 40, 40), (1.0, 1.0, 1.0), 200.0)
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=50000, use_jacobi_preconditioner=True,
+                                  precision=precision)
+    solver = fe.create_solver(p, bc, opts, linear=True, return_solution=False)
+    print("case: 200x40x40 Jacobi-PCG", precision, flush=True)
+    u = solver(tp).cpu().numpy()
+    err = np.abs(u[f["dofs"]] - f["u"]).max() / float(f["u_max"])
+    print(precision, solver.stats, "max sampled err / max|u|", err)
+    assert err <= 1e-6
+
+
+def test_c3_fp32_form_and_autograd_adjoint_match_oracle():
+    """120x40x20 SIMP: the mixed (FP32 matrix) forward solve and the
+    autograd gradient of the compliance (adjoint by recycled directions)
+    vs the oracle's closed-form dC/drho (simp_120x40x20_oracle.npz)."""
+    import torch
+    import feax_amd as fe
+    from feax_amd.gene import create_compliance_fn
+    f = _load("simp_120x40x20_oracle.npz")
+
+    class Simp(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.SIMPLinearElastic(nu=0.3, E0=1.0, Emin=1e-9, penal=3.0)
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.0, 0.0, 1.0))]
+
+    patch = [lambda x: np.isclose(x[0], 120) & (np.abs(x[1] - 20) <= 1 + 1e-9) & (np.abs(x[2] - 10) <= 1 + 1e-9)]
+    prob = Simp(fe.StructuredGrid((120, 40, 20)).to_mesh(), vec=3, dim=3, location_fns=patch)
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda x: np.isclose(x[0], 0.0), "all", 0.0)]).create_bc(prob)
+    rho = torch.tensor(np.random.default_rng(SEED).uniform(0.3, 1.0, prob.num_cells), device="cuda",
+                       requires_grad=True)
+    t = fe.TracedParams.create_uniform_surface_var(prob, 0.25)
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=0.0, maxiter=50000, use_jacobi_preconditioner=True,
+                                  precision="mixed")
+    solver = fe.create_solver(prob, bc, opts, linear=True, return_solution=False)
+    print("case: 120x40x20 SIMP mixed + autograd", flush=True)
+    u = solver(fe.TracedParams(volume_vars=(rho,), surface_vars=[(t,)]))
+    C = create_compliance_fn(prob, 0.25)(u)
+    C.backward()
+    dc = rho.grad.cpu().numpy()
+    err = np.abs(dc[f["cells"]] - f["dc"]).max() / float(f["dc_max"])
+    print("mixed autograd dC/drho err vs oracle", err, "C", float(C), float(f["compliance"]))
+    assert err <= 1e-6
+    assert abs(float(C.detach()) - float(f["compliance"])) <= 1e-8 * abs(float(f["compliance"]))
