@@ -94,27 +94,31 @@ class DirichletBCConfig:
         n = problem.num_total_dofs_all_vars
         if not self.specs:
             return DirichletBC(np.zeros(0, np.int32), np.zeros(n, bool), np.zeros(0, np.float64), n)
-        pts = problem.points
         rows, vals = [], []
-        vec = problem.vec
-        for spec in self.specs:
-            if spec.variable_index not in (None, 0):
-                continue
-            comps = list(range(vec)) if spec.component == "all" else (
-                [spec.component] if spec.component < vec else [])
-            # The location is evaluated once per spec (feax evaluates it per
-            # component, with the same result).
-            flag = location_flags(spec.location, pts, "got") if comps else None
-            nodes = np.nonzero(flag)[0] if comps else np.zeros(0, np.int64)
-            if len(nodes) == 0:
-                continue
-            v = np.asarray(eval_points(spec.value, pts[nodes]), np.float64).reshape(-1)
-            if v.shape != (len(nodes),):
-                raise ValueError(f"a Dirichlet value function must return one value per point "
-                                 f"(got {v.size} values for {len(nodes)} points)")
-            for c in comps:
-                rows.append(nodes * vec + c)
-                vals.append(v)
+        # per variable and spec, as feax/DCboundary.py:427-480: the spec's
+        # component(s) of the variables it names (variable_index None: all)
+        for ind, fe in enumerate(problem.fes):
+            vec, off = fe.vec, problem.offset[ind]
+            pts = np.asarray(fe.points, np.float64)
+            for spec in self.specs:
+                if spec.variable_index is not None and spec.variable_index != ind:
+                    continue
+                comps = list(range(vec)) if spec.component == "all" else (
+                    [spec.component] if spec.component < vec else [])
+                if not comps:
+                    continue
+                # The location is evaluated once per spec (feax evaluates it per
+                # component, with the same result).
+                nodes = np.nonzero(location_flags(spec.location, pts, "got"))[0]
+                if len(nodes) == 0:
+                    continue
+                v = np.asarray(eval_points(spec.value, pts[nodes]), np.float64).reshape(-1)
+                if v.shape != (len(nodes),):
+                    raise ValueError(f"a Dirichlet value function must return one value per point "
+                                     f"(got {v.size} values for {len(nodes)} points)")
+                for c in comps:
+                    rows.append(nodes * vec + c + off)
+                    vals.append(v)
         if rows:
             r = np.concatenate(rows)
             v = np.concatenate(vals)

@@ -17,7 +17,7 @@ LIB = os.path.join(OUT_DIR, "libfeaxhip.so")
 
 SOURCES = ["pattern.cpp", "krylov.hip", "hex8.hip", "adjoint.hip", "batch.hip", "gmg.hip", "bicgstab.hip", "gmres.hip",
            "direct.hip", "tet4.hip", "quad4.hip", "hex8s.hip",
-           "hex20.hip"]
+           "hex20.hip", "hex8ut.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

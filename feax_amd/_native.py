@@ -68,6 +68,14 @@ SIGNATURES = {
     "fx_tet4_residual_workspace_bytes": (_SZ, [_I32]),
     "fx_tet4_residual_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P,
                                     _P, _P, _I32, _P, _P, _I32, _P, _P, _SZ]),
+    "fx_hex8ut_pattern": (_I32, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "fx_hex8ut_face_incidence": (_I32, [_I32, _I32, _P, _I32, _P, _P, _P]),
+    "fx_hex8ut_assemble_workspace_bytes": (_SZ, [_I32, _I32, _I32]),
+    "fx_hex8ut_assemble_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                      _I32, _I32, _F64, _P, _P, _P, _SZ]),
+    "fx_hex8ut_residual_workspace_bytes": (_SZ, [_I32]),
+    "fx_hex8ut_residual_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P,
+                                      _P, _P, _I32, _P, _P, _I32, _P, _P, _SZ]),
     "fx_hex20_pattern": (_I32, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "fx_hex20_face_incidence": (_I32, [_I32, _I32, _P, _I32, _P, _P, _P]),
     "fx_hex20_assemble_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,

@@ -31,7 +31,9 @@ class TracedStructure:
         self.device = dev
         self.ele_type = problem.ele_type
         self.tag = problem.kernel_tag
-        self.vec = problem.vec
+        self.coupled = getattr(problem, "coupled", False)
+        # components per node of the surface direction table: (u, T) -> 3 + 1
+        self.vec = 4 if self.coupled else problem.vec
         self.num_face_quads = problem.fes[0].num_face_quads
         self.num_nodes = problem.num_total_nodes
         self.num_cells = problem.num_cells
@@ -61,6 +63,17 @@ class TracedStructure:
                     raise NotImplementedError(
                         "surface maps on the HIP path are feax_amd.materials.SurfaceTraction (dead loads)")
                 d = np.asarray(m.direction, float)
+                if self.coupled:
+                    # a SurfaceFlux loads the temperature, a SurfaceTraction the displacement
+                    from .materials import SurfaceFlux
+                    full = np.zeros(4)
+                    if isinstance(m, SurfaceFlux):
+                        full[3] = d[0]
+                    elif d.shape == (3,):
+                        full[:3] = d
+                    else:
+                        raise ValueError(f"surface map direction {d} must have 3 components here")
+                    d = full
                 if d.shape != (self.vec,):
                     raise ValueError(f"surface map direction {d} must have {self.vec} components here")
                 fdir[self.face_offsets[i]:self.face_offsets[i + 1]] = d
@@ -102,6 +115,11 @@ def resolve_material(problem, traced_params, ts):
     tm = problem.get_tensor_map()
     if tm is None:
         tm = problem.get_energy_density()
+    if getattr(problem, "coupled", False):
+        tm = problem.get_weak_form()
+        if not getattr(tm, "coupled", False):
+            raise NotImplementedError("a coupled (u, T) problem's get_weak_form() returns a "
+                                      "feax_amd.materials.ThermoElasticCoupled on the HIP path")
     if not isinstance(tm, TensorMap):
         raise NotImplementedError(
             "the HIP element kernels need get_tensor_map()/get_energy_density() to return a "
@@ -127,7 +145,7 @@ def resolve_material(problem, traced_params, ts):
         p, loc = ts._cache[key], nat.FX_PARAM_CELL
     mat = tm.material(loc)
     alpha = getattr(tm, "alpha", None)
-    if alpha is not None:
+    if alpha is not None and not getattr(tm, "coupled", False):
         # thermo-elastic law: the temperature is volume_vars[1], a node var
         if ts.tag != "hex8":
             raise NotImplementedError("thermal strain is built for HEX8 meshes (vec = 3)")
@@ -324,6 +342,20 @@ def param_vjp(problem, ts, traced_params, sol, lam, bc=None, scale=1.0):
     lam = _as_dev(lam, ts)
     mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
     grad = torch.empty_like(p, dtype=torch.float64)
+    if ts.tag == "hex8ut":
+        # E enters only the displacement residual, sigma(grad u, T): the HEX8
+        # thermo-elastic pullback on the u part, T = the state's temperature.
+        n3 = 3 * ts.num_nodes
+        T = u[n3:].contiguous()
+        mat.temperature = T.data_ptr()
+        u, lam = u[:n3].contiguous(), lam[:n3].contiguous()
+        mask = mask[:n3].contiguous() if mask is not None else None
+        wsb = int(nat.lib().fx_hex8_param_vjp_workspace_bytes(ts.num_cells, mat.param_loc))
+        ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=ts.device)
+        nat.call("fx_hex8_param_vjp_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
+                 nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), ctypes.addressof(mat),
+                 nat.ptr(p), nat.ptr(u), nat.ptr(lam), nat.ptr(mask), float(scale), nat.ptr(grad), nat.ptr(ws), wsb)
+        return grad
     wsb = int(getattr(nat.lib(), f"fx_{ts.tag}_param_vjp_workspace_bytes")(ts.num_cells, mat.param_loc))
     ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=ts.device)
     nat.call(f"fx_{ts.tag}_param_vjp_f64", nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y),
@@ -343,6 +375,21 @@ def traction_vjp(problem, ts, lam, bc=None, scale=1.0):
     mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
     q = ts.num_face_quads
     g = torch.empty(ts.num_faces * q, dtype=torch.float64, device=ts.device)
+    if ts.tag == "hex8ut":
+        # traction on u (HEX8 pullback) plus flux on T (scalar HEX8 pullback)
+        n3 = 3 * ts.num_nodes
+        fd = ts.fdir.reshape(-1, 4)
+        gT = torch.empty_like(g)
+        parts = ((fd[:, :3].contiguous(), lam[:n3].contiguous(), None if mask is None else mask[:n3].contiguous(),
+                  "fx_hex8_traction_vjp_f64", g),
+                 (fd[:, 3].contiguous(), lam[n3:].contiguous(), None if mask is None else mask[n3:].contiguous(),
+                  "fx_hex8s_traction_vjp_f64", gT))
+        for fdir, lm, mk, name, out in parts:
+            nat.call(name, nat.stream(), ts.num_faces, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z), nat.ptr(ts.cells),
+                     nat.ptr(ts.faces), nat.ptr(fdir), nat.ptr(lm), nat.ptr(mk), float(scale), nat.ptr(out))
+        g = (g + gT).reshape(-1, q)
+        off = ts.face_offsets
+        return [g[off[i]:off[i + 1]] for i in range(len(off) - 1)]
     nat.call(f"fx_{ts.tag}_traction_vjp_f64", nat.stream(), ts.num_faces, nat.ptr(ts.x), nat.ptr(ts.y),
              nat.ptr(ts.z), nat.ptr(ts.cells), nat.ptr(ts.faces), nat.ptr(ts.fdir), nat.ptr(lam), nat.ptr(mask),
              float(scale), nat.ptr(g))

@@ -257,6 +257,52 @@ extern "C" int fx_hex20_face_incidence(int32_t num_nodes, int32_t num_cells, con
   return FX_OK;
 }
 
+// The monolithic (u, T) HEX8 problem (vec = [3, 1], DOFs u: 3 n + i,
+// T: 3 N + n): every row of node n holds the u columns 3 m + j of its
+// neighbours m, then their T columns 3 N + m.  elem_rank and the node
+// incidence are those of the HEX8 pattern.
+extern "C" int fx_hex8ut_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse_out,
+                                 int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                                 int32_t* inc_ptr, int32_t* inc) {
+  if (num_nodes <= 0 || !nse_out || !indptr) return fx::fail(FX_EINVAL, "fx_hex8ut_pattern: bad arguments");
+  if (static_cast<int64_t>(num_nodes) * 4 >= INT32_MAX) return fx::fail(FX_ELIMIT, "fx_hex8ut_pattern: too many nodes");
+  const int64_t N = num_nodes;
+  std::vector<int32_t> nptr(N + 1);
+  int64_t nnb = 0;
+  int rc = cell_pattern("fx_hex8ut_pattern", 8, 1, num_nodes, num_cells, cells, &nnb, max_degree, nptr.data(),
+                        nullptr, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  // rows 3 n + i (u) then 3 N + n (T), 4 deg entries each
+  int64_t nse = 0;
+  indptr[0] = 0;
+  for (int64_t r = 0; r < 4 * N; ++r) {
+    const int64_t nd = r < 3 * N ? r / 3 : r - 3 * N;
+    nse += 4 * static_cast<int64_t>(nptr[nd + 1] - nptr[nd]);
+    if (nse >= INT32_MAX) return fx::fail(FX_ELIMIT, "fx_hex8ut_pattern: nse exceeds int32");
+    indptr[r + 1] = static_cast<int32_t>(nse);
+  }
+  *nse_out = nse;
+  if (!indices) return FX_OK;
+  std::vector<int32_t> nbr(nnb);
+  rc = cell_pattern("fx_hex8ut_pattern", 8, 1, num_nodes, num_cells, cells, &nnb, nullptr, nptr.data(), nbr.data(),
+                    elem_rank, inc_ptr, inc);
+  if (rc) return rc;
+  for (int64_t r = 0; r < 4 * N; ++r) {
+    const int64_t nd = r < 3 * N ? r / 3 : r - 3 * N;
+    int64_t s = indptr[r];
+    for (int32_t k = nptr[nd]; k < nptr[nd + 1]; ++k)
+      for (int j = 0; j < 3; ++j) indices[s++] = 3 * nbr[k] + j;
+    for (int32_t k = nptr[nd]; k < nptr[nd + 1]; ++k) indices[s++] = static_cast<int32_t>(3 * N + nbr[k]);
+  }
+  return FX_OK;
+}
+
+extern "C" int fx_hex8ut_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells,
+                                        int32_t num_faces, const int32_t* faces, int32_t* finc_ptr,
+                                        int32_t* finc) {
+  return fx_hex8_face_incidence(num_nodes, num_cells, cells, num_faces, faces, finc_ptr, finc);
+}
+
 extern "C" int fx_csr_is_block3(int32_t n, const int32_t* indptr, const int32_t* indices,
                                 int32_t* is_block3) {
   if (n < 0 || !indptr || !is_block3) return fx::fail(FX_EINVAL, "fx_csr_is_block3: bad arguments");

@@ -183,3 +183,46 @@ class SIMPThermoElastic(SIMPLinearElastic):
     def __call__(self, u_grad, rho, T=0.0):
         E = self.modulus(rho)
         return ThermoElastic(self.nu, self.alpha, self.t_ref)(u_grad, E, T)
+
+
+class ThermoElasticCoupled(TensorMap):
+    """The weak form of a monolithic two-variable (u, T) problem
+    (Problem(mesh=[m, m], vec=[3, 1]); feax's get_weak_form,
+    feax/problem.py:697-740, feax/assembler.py:591-647), evaluated by the
+    fx_hex8ut kernels:
+        grad terms [sigma, k grad T], mass terms [0, 0],
+        sigma = E (lam0 tr eps I + 2 mu0 eps) - E alpha (T - t_ref) / (1 - 2 nu) I,
+    E = volume_vars[0] (node or cell variable, or fixed with E=...),
+    optionally through the SIMP law (simp=(E0, Emin, penal)); nu, alpha,
+    t_ref and the conductivity k are constants.  Return it from a Problem's
+    get_weak_form()."""
+
+    kind = nat.FX_MAT_LINEAR_ELASTIC
+    coupled = True
+
+    def __init__(self, nu=0.3, alpha=1e-5, t_ref=0.0, k=1.0, E=None, simp=None):
+        self.nu, self.alpha, self.t_ref, self.k = float(nu), float(alpha), float(t_ref), float(k)
+        self.E = E
+        self.simp = simp
+
+    def weak_form(self, vals, grads, x, E):
+        """(mass_terms, grad_terms) at one point: the function feax would vmap."""
+        T = float(np.asarray(vals[1]).reshape(-1)[0])
+        ug = np.asarray(grads[0], float)
+        if self.simp is not None:
+            E0, Emin, penal = self.simp
+            E = Emin + E ** penal * (E0 - Emin)
+        eps = 0.5 * (ug + ug.T)
+        lam0 = self.nu / ((1 + self.nu) * (1 - 2 * self.nu))
+        mu0 = 1.0 / (2 * (1 + self.nu))
+        sig = E * (lam0 * np.trace(eps) * np.eye(3) + 2 * mu0 * eps)
+        sig -= E * self.alpha * (T - self.t_ref) / (1 - 2 * self.nu) * np.eye(3)
+        return [np.zeros(3), np.zeros(1)], [sig, self.k * np.asarray(grads[1], float).reshape(1, 3)]
+
+    def material(self, param_loc):
+        if self.simp is not None:
+            E0, Emin, penal = self.simp
+            return nat.material_struct(nat.FX_MAT_SIMP, param_loc, nu=self.nu, E0=E0, Emin=Emin, penal=penal,
+                                       kappa=self.k, alpha=self.alpha, t_ref=self.t_ref)
+        return nat.material_struct(self.kind, param_loc, nu=self.nu, kappa=self.k, alpha=self.alpha,
+                                   t_ref=self.t_ref)

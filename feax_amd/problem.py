@@ -1,5 +1,6 @@
-"""Problem: FE structure for HEX8 / TET4 (vec = 3), QUAD4 (vec = 2) and
-scalar HEX8 (vec = 1) problems (the feax.Problem API).
+"""Problem: FE structure for HEX8 / HEX20 / TET4 (vec = 3), QUAD4
+(vec = 2), scalar HEX8 / TET4 / QUAD4 (vec = 1) and the coupled (u, T)
+HEX8 problem (the feax.Problem API).
 
 Mirrors feax.Problem (feax/problem.py:37-846): constructor arguments,
 ``location_fns`` / ``boundary_inds_list``, ``matrix_view``, the physics
@@ -56,8 +57,10 @@ class FiniteElement:
 
 @dataclass
 class Problem:
-    """Vector problem on an explicit mesh: HEX8 / TET4 (vec = dim = 3) or
-    QUAD4 (vec = dim = 2)."""
+    """Vector problem on an explicit mesh: HEX8 / HEX20 / TET4 (vec = dim =
+    3), QUAD4 (vec = dim = 2), the scalar (vec = 1) problems, or the coupled
+    two-variable HEX8 problem (mesh = [m, m], vec = [3, 1])."""
+    coupled = False           # not a dataclass field: set by __post_init__
     mesh: Union[Mesh, List[Mesh]]
     vec: Union[int, List[int]]
     dim: int
@@ -73,9 +76,14 @@ class Problem:
                 self.matrix_view = MatrixView[self.matrix_view.upper()]
             except KeyError:
                 raise ValueError(f"Invalid matrix_view: {self.matrix_view}. Must be 'FULL', 'UPPER', or 'LOWER'")
+        self.coupled = False
         if isinstance(self.mesh, list):
+            if len(self.mesh) == 2:
+                self._init_coupled()
+                return
             if len(self.mesh) != 1:
-                raise NotImplementedError("multi-variable problems are not on the HIP path")
+                raise NotImplementedError("multi-variable problems on the HIP path: two variables, (u, T) "
+                                          "with vec=[3, 1] on one HEX8 mesh")
             self.mesh, self.vec = self.mesh[0], self.vec[0]
             self.ele_type = self.ele_type[0] if isinstance(self.ele_type, list) else self.ele_type
             self.gauss_order = self.gauss_order[0] if isinstance(self.gauss_order, list) else self.gauss_order
@@ -107,12 +115,59 @@ class Problem:
         self._build_csr_assembly_structure()
         self.custom_init(*self.additional_info)
 
+    def _init_coupled(self):
+        """Problem(mesh=[m, m], vec=[3, 1], dim=3, ele_type="HEX8"): a
+        displacement (variable 0) and a temperature (variable 1) on one HEX8
+        mesh in one system, DOFs u 3 n + i then T 3 N + n (feax offsets,
+        feax/problem.py:155-212); the physics is get_weak_form() (a
+        feax_amd.materials.ThermoElasticCoupled)."""
+        m0, m1 = self.mesh
+        vec = list(self.vec) if isinstance(self.vec, (list, tuple)) else None
+        ets = self.ele_type if isinstance(self.ele_type, list) else [self.ele_type] * 2
+        same = (m0 is m1) or (np.array_equal(np.asarray(m0.points), np.asarray(m1.points))
+                              and np.array_equal(np.asarray(m0.cells), np.asarray(m1.cells)))
+        if vec != [3, 1] or ets != ["HEX8", "HEX8"] or self.dim != 3 or not same:
+            raise NotImplementedError("multi-variable problems on the HIP path: (u, T) with vec=[3, 1], dim=3, "
+                                      f"HEX8 on one mesh (got vec={self.vec}, ele_type={self.ele_type})")
+        if self.gauss_order is not None and any(g not in (None, 2) for g in np.atleast_1d(self.gauss_order)):
+            raise NotImplementedError("HEX8 uses feax's default rule (gauss_order=2)")
+        if self.matrix_view != MatrixView.FULL:
+            raise NotImplementedError("the coupled (u, T) problem assembles the FULL matrix")
+        self.coupled = True
+        self.mesh_list = [m0, m1]
+        self.mesh = m0
+        self.vec = [3, 1]
+        self.ele_type = "HEX8"
+        self.npe = 8
+        self.num_vars = 2
+        self.fes = [FiniteElement(m0, 3, 3, "HEX8"), FiniteElement(m1, 1, 3, "HEX8")]
+        self.points = np.ascontiguousarray(m0.points, dtype=np.float64)
+        self.cells = np.ascontiguousarray(m0.cells, dtype=np.int32)
+        if self.cells.ndim != 2 or self.cells.shape[1] != 8:
+            raise ValueError(f"HEX8 cells must be (E, 8), got {self.cells.shape}")
+        self.cells_list = [self.cells, self.cells]
+        self.num_cells = len(self.cells)
+        self.num_total_nodes = len(self.points)
+        self.num_total_dofs_all_vars = 4 * self.num_total_nodes
+        self.offset = [0, 3 * self.num_total_nodes]
+        self.boundary_inds_list = boundary_conditions_inds(self.points, self.cells, self.location_fns, "HEX8")
+        self._build_csr_assembly_structure()
+        self.custom_init(*self.additional_info)
+
     @property
     def kernel_tag(self):
-        """Name of the native kernel family: hex8, tet4, quad4, or hex8s /
-        tet4s / quad4s (the scalar, vec = 1, problems)."""
+        """Name of the native kernel family: hex8, hex20, tet4, quad4, hex8s /
+        tet4s / quad4s (the scalar, vec = 1, problems), or hex8ut (the
+        coupled (u, T) problem)."""
+        if self.coupled:
+            return "hex8ut"
         tag = self.ele_type.lower()
         return tag + "s" if self.vec == 1 else tag
+
+    @property
+    def solution_layout(self):
+        """(num_nodes, vec) per variable (feax/solution.py:42-90)."""
+        return tuple((fe.num_total_nodes, fe.vec) for fe in self.fes)
 
     @property
     def physical_quad_points(self):
@@ -132,7 +187,7 @@ class Problem:
         """CSR pattern + assembly maps (feax/problem.py:316-484) via fx_hex8_pattern / fx_tet4_pattern /
         fx_quad4_pattern."""
         N, E, npe = self.num_total_nodes, self.num_cells, self.npe
-        n = self.vec * N
+        n = self.num_total_dofs_all_vars
         L = nat.lib()
         tag = self.kernel_tag
         pattern = getattr(L, f"fx_{tag}_pattern")
@@ -230,6 +285,20 @@ class Problem:
         """(E, (vec npe)^2) int32: CSR slot of each element Jacobian entry
         jac[a*vec+i, b*vec+j] (feax/problem.py:458-463), from the rank table:
         indptr[vec n_a + i] + vec rank[e, a, b] + j."""
+        if getattr(self, "_vol_slots", None) is None and self.coupled:
+            # row DOF (a, i) / (a, T), column (b, j) / (b, T): u columns at
+            # 3 rank + j, T columns after the 3 deg u columns
+            c = self.cells.astype(np.int64)
+            inds = self._element_inds(self.cells).astype(np.int64)                       # (E, 32)
+            rank = self.elem_rank.reshape(-1, 8, 8).astype(np.int64)
+            deg = (np.diff(self.csr_indptr)[3 * c] // 4).astype(np.int64)                 # (E, a)
+            rowp = self.csr_indptr.astype(np.int64)[inds]                                 # (E, 32)
+            node_of = np.concatenate([np.repeat(np.arange(8), 3), np.arange(8)])          # local node of a DOF
+            comp_of = np.concatenate([np.tile(np.arange(3), 8), np.full(8, 3)])
+            ra = rank[:, node_of[:, None], node_of[None, :]]                              # (E, 32, 32)
+            colpos = np.where(comp_of[None, None, :] < 3, 3 * ra + comp_of[None, None, :],
+                              3 * deg[:, node_of][:, :, None] + ra)
+            self._vol_slots = (rowp[:, :, None] + colpos).reshape(len(c), 1024).astype(np.int32)
         if getattr(self, "_vol_slots", None) is None:
             c = self.cells.astype(np.int64)
             v = self.vec
@@ -241,14 +310,16 @@ class Problem:
         return self._vol_slots
 
     def _element_inds(self, cells):
+        """Per-element DOFs, variable by variable (feax/problem.py:161-181)."""
         c = np.asarray(cells, np.int64)
-        v = self.vec
-        return (v * c[:, :, None] + np.arange(v)[None, None, :]).reshape(len(c), -1).astype(np.int32)
+        parts = [(fe.vec * c[:, :, None] + np.arange(fe.vec)[None, None, :] + off).reshape(len(c), -1)
+                 for fe, off in zip(self.fes, self.offset)]
+        return np.concatenate(parts, axis=1).astype(np.int32)
 
     def _coo(self):
         inds = [self._element_inds(self.cells)] + [self._element_inds(self.cells[b[:, 0]])
                                                    for b in self.boundary_inds_list]
-        nd = self.vec * self.npe
+        nd = inds[0].shape[1]
         I = np.concatenate([np.repeat(i[:, :, None], nd, axis=2).reshape(-1) for i in inds])
         J = np.concatenate([np.repeat(i[:, None, :], nd, axis=1).reshape(-1) for i in inds])
         return I, J
@@ -304,7 +375,11 @@ class Problem:
     # ------------------------------------------------------------------ layout
     def unflatten_fn_sol_list(self, flat):
         flat = getattr(flat, "dofs", flat)
-        return [flat.reshape(self.num_total_nodes, self.vec)]
+        out, start = [], 0
+        for nn, v in self.solution_layout:
+            out.append(flat[start:start + nn * v].reshape(nn, v))
+            start += nn * v
+        return out
 
     # ------------------------------------------------------------------ hooks
     def custom_init(self, *args):

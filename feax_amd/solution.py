@@ -10,7 +10,7 @@ class Solution:
 
     @classmethod
     def from_problem(cls, problem, dofs):
-        return cls(dofs, ((problem.num_total_nodes, problem.vec),))
+        return cls(dofs, problem.solution_layout)
 
     def __array__(self, dtype=None, copy=None):
         d = self.dofs

@@ -77,7 +77,7 @@ def create_solver(problem, bc, solver_options=None, adjoint_solver_options=None,
     def _wrap(fn):
         if not return_solution:
             return fn
-        layout = ((problem.num_total_nodes, problem.vec),)
+        layout = problem.solution_layout
 
         def wrapped(*args, **kwargs):
             args = tuple(a.dofs if isinstance(a, Solution) else a for a in args)

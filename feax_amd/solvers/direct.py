@@ -73,6 +73,10 @@ class BandCholesky:
 
     def __init__(self, problem, device):
         import torch
+        if getattr(problem, "coupled", False):
+            raise NotImplementedError("the banded direct solver factors symmetric J_bc; the coupled (u, T) "
+                                      "Jacobian is not symmetric (use KrylovSolverOptions(solver='gmres' or "
+                                      "'bicgstab'))")
         perm, inv, bw = band_ordering(problem)
         self.n = len(np.asarray(problem.csr_indptr)) - 1
         self.bw = bw

@@ -71,7 +71,9 @@ def solve_fp64(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None, keep=N
     x0 = x0_fn(u0)
     du, info = krylov(J, b, x0, opts, dinv, _b3(opts))
     if keep is not None:
-        keep.update(form="fp64", A=J, dinv=dinv, symmetric=symmetric)
+        # the adjoint operator J_bc^T is J_bc itself only for a symmetric
+        # elimination of a symmetric weak form (not the coupled (u, T) one)
+        keep.update(form="fp64", A=J, dinv=dinv, symmetric=symmetric and not getattr(problem, "coupled", False))
     if opts.check_convergence:
         du = check_convergence(J, du, b, opts)
     if stats is not None:

@@ -200,7 +200,14 @@ def newton_adjoint(problem, ts, tp, bc, v, keep, opts, symmetric=True):
         from .direct import BandCholesky
         keep["adjoint_iters"] = [0]
         return BandCholesky(problem, ts.device).factor(J).solve(rhs)
-    lam, info = pcg(J, rhs, None, opts.tol, opts.atol, opts.maxiter, dinv, None, opts.check_every)
+    if opts.solver in ("bicgstab", "gmres"):
+        # J_bc^T: J_bc itself only for the symmetric elimination of a
+        # symmetric tangent (not the coupled (u, T) one)
+        from .common import krylov
+        At = J if (symmetric and not getattr(problem, "coupled", False)) else J.transpose()
+        lam, info = krylov(At, rhs, None, opts, dinv)
+    else:
+        lam, info = pcg(J, rhs, None, opts.tol, opts.atol, opts.maxiter, dinv, None, opts.check_every)
     keep["adjoint_iters"] = [info.iters]
     return lam
 

@@ -377,6 +377,42 @@ int fx_hex20_simp_compliance_sens_f64(void* stream, int32_t num_cells, const dou
 int fx_hex20_traction_vjp_f64(void* stream, int32_t num_faces, const double* x, const double* y, const double* z,
                               const int32_t* cells, const int32_t* faces, const double* fdir, const double* lam,
                               const uint8_t* bc_mask, double scale, double* grad_svar);
+/* ------------------------------------------------------------------ */
+/* A monolithic two-variable HEX8 problem (feax's Problem(mesh=[m, m],
+ * vec=[3, 1]), feax/problem.py:155-212, feax/assembler.py:591-647):
+ * displacement u at DOF 3 n + i and temperature T at DOF 3 N + n, weak form
+ * sigma = E (lam0 tr eps I + 2 mu0 eps) - E alpha (T - t_ref) / (1 - 2 nu) I
+ * (variable 0) and k grad T (variable 1); E the volume variable (kind
+ * FX_MAT_LINEAR_ELASTIC or FX_MAT_SIMP, node or cell), nu, alpha, t_ref from
+ * fx_material and the conductivity k in fx_material.kappa (temperature
+ * NULL: T is part of the state).  The pattern holds in every row of node n
+ * the u columns of its neighbours, then their T columns (4 deg entries;
+ * indptr has 4N + 1 entries); elem_rank and the node incidence are the
+ * HEX8 ones, faces as HEX8.  Assembly: FP64 only, flags 0, the Jacobian
+ * does not depend on u (u may be NULL), workspace 1024 doubles per
+ * element.  Residual: per face, fdir[4 f + i] (i < 3) scales the surface
+ * value on u and fdir[4 f + 3] on T (4 surface values per face, svar F * 4),
+ * workspace 32 doubles per element. */
+int fx_hex8ut_pattern(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int64_t* nse,
+                      int32_t* max_degree, int32_t* indptr, int32_t* indices, uint8_t* elem_rank,
+                      int32_t* inc_ptr, int32_t* inc);
+int fx_hex8ut_face_incidence(int32_t num_nodes, int32_t num_cells, const int32_t* cells, int32_t num_faces,
+                             const int32_t* faces, int32_t* finc_ptr, int32_t* finc);
+size_t fx_hex8ut_assemble_workspace_bytes(int32_t num_cells, int32_t value_bytes, int32_t flags);
+int fx_hex8ut_assemble_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                           const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                           const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
+                           const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
+                           int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, double* data,
+                           void* workspace, size_t ws_bytes);
+size_t fx_hex8ut_residual_workspace_bytes(int32_t num_cells);
+int fx_hex8ut_residual_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
+                           const double* z, const int32_t* cells, const int32_t* inc_ptr, const int32_t* inc,
+                           const fx_material* mat, const double* param, const double* u, int32_t num_faces,
+                           const int32_t* faces, const int32_t* finc_ptr, const int32_t* finc, const double* svar,
+                           const double* fdir, int32_t num_bc, const int32_t* bc_rows, const double* bc_vals,
+                           int32_t negate, double* res, void* workspace, size_t ws_bytes);
+
 /* TET4 Neo-Hookean strain energy, as fx_hex8_energy_f64 at the single
  * Gauss point (workspace fx_tet4_energy_workspace_bytes). */
 size_t fx_tet4_energy_workspace_bytes(int32_t num_cells);
