@@ -19,11 +19,14 @@
 // reads at the measured streaming rate).
 //
 // PCG follows jax.scipy.sparse.linalg.cg (feax/solvers/common.py:229-239)
-// with scalars kept on the device.  Per iteration: the operator (CSR SpMV,
-// or feax's matrix-free J_bc v from hex8.hip) fused with the p.q block
-// partials, a 1-block finalize (alpha), the vector update fused with
-// r.z / r.r partials, a 1-block finalize (beta, stop test), the direction
-// update.  FP64 partial sums, fixed order: bitwise repeatable.
+// with scalars kept on the device.  Per iteration on a CSR matrix, three
+// launches: the SpMV fused with the p.q block partials; the vector update
+// (each block sums the p.q partials itself for alpha) with the r.z / r.r
+// partials; the direction update (each block sums those for beta; block 0
+// advances the iteration and the stop test).  With feax's matrix-free
+// J_bc v (hex8.hip) as the operator: operator, 1-block finalize (alpha),
+// update, 1-block finalize (beta, stop test), direction.  FP64 partial
+// sums, fixed order: bitwise repeatable.
 #include <algorithm>
 #include <cstdint>
 #include <mutex>
@@ -260,6 +263,99 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, T* __restrict
   store_partials<2>(block_sum<2, kBlock>(acc), lb, partials);
 }
 
+// CSR iterations: three launches, the two 1-block finalizes folded into
+// the vector kernels.  Each of these kernels runs upd_grid(n) blocks of
+// kFinBlock threads, and every block sums all partials of the previous
+// kernel itself (same fixed order in every block, so all blocks hold the
+// same scalar).  gamma alternates between st->g[0] and st->g[1] by the
+// parity of the iteration: the direction kernel of iteration k writes
+// g[(k + 1) % 2] while its other blocks still read g[k % 2].
+constexpr int kUpdBlocks = 256;
+inline int upd_grid(int n) { return std::max(1, std::min(kUpdBlocks, (n + kFinBlock - 1) / kFinBlock)); }
+
+__device__ __forceinline__ Range fin_block_range(int n, int& lblock) {
+  lblock = xcd_block(blockIdx.x, gridDim.x);
+  const int chunk = (n + gridDim.x - 1) / gridDim.x;
+  Range r;
+  r.begin = min(n, lblock * chunk);
+  r.end = min(n, r.begin + chunk);
+  return r;
+}
+
+// alpha = g[par] / (p.q) from the operator partials; x += alpha p;
+// r -= alpha q; z = M r; partials of r.z and r.r.
+template <typename T>
+__global__ __launch_bounds__(kFinBlock) void pcg_update_alpha_kernel(
+    int n, int par, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
+    const T* __restrict__ q, const T* __restrict__ dinv, PcgState* __restrict__ st,
+    const double* __restrict__ op_partials, int op_nparts, double* partials, float* __restrict__ rec_p,
+    double* __restrict__ rec_pq, int rec_max) {
+  if (st->done) return;
+  __shared__ double s_alpha;
+  {
+    const DVec<1> t = sum_partials<1>(op_partials, op_nparts);
+    if (threadIdx.x == 0) {
+      s_alpha = st->g[par] / t.v[0];
+      if (blockIdx.x == 0) {
+        st->alpha = s_alpha;
+        if (rec_pq && st->k < rec_max) rec_pq[st->k] = t.v[0];
+      }
+    }
+  }
+  __syncthreads();
+  const T alpha = static_cast<T>(s_alpha);
+  int lb;
+  const Range rg = fin_block_range(n, lb);
+  float* rec = (rec_p && st->k < rec_max) ? rec_p + static_cast<size_t>(st->k) * n : nullptr;
+  DVec<2> acc;
+  acc.v[0] = 0.0;
+  acc.v[1] = 0.0;
+  for (int i = rg.begin + threadIdx.x; i < rg.end; i += kFinBlock) {
+    const T pi = p[i];
+    if (rec) rec[i] = static_cast<float>(pi);
+    x[i] += alpha * pi;
+    const T ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const T zi = dinv ? dinv[i] * ri : ri;
+    z[i] = zi;
+    acc.v[0] += static_cast<double>(ri) * static_cast<double>(zi);
+    acc.v[1] += static_cast<double>(ri) * static_cast<double>(ri);
+  }
+  store_partials<2>(block_sum<2, kFinBlock>(acc), lb, partials);
+}
+
+// gamma' = r.z, beta = gamma' / g[par], p = z + beta p; block 0 advances
+// the iteration (g[1 - par], k, rs, stop flag).  A block that starts after
+// block 0 raised the stop flag skips the (then unused) direction update.
+template <typename T>
+__global__ __launch_bounds__(kFinBlock) void pcg_dir_beta_kernel(int n, int par, T* __restrict__ p,
+                                                                 const T* __restrict__ z, PcgState* __restrict__ st,
+                                                                 const double* __restrict__ partials, int nparts) {
+  if (st->done) return;
+  __shared__ double s_beta;
+  {
+    const DVec<2> t = sum_partials<2>(partials, nparts);
+    if (threadIdx.x == 0) {
+      const double g_new = t.v[0];
+      s_beta = g_new / st->g[par];
+      if (blockIdx.x == 0) {
+        const int k = st->k + 1;
+        st->g[1 - par] = g_new;
+        st->gamma = g_new;
+        st->beta = s_beta;
+        st->rs = t.v[1];
+        st->k = k;
+        st->done = !(t.v[1] > st->atol2 && k < st->maxiter);
+      }
+    }
+  }
+  __syncthreads();
+  const T beta = static_cast<T>(s_beta);
+  int lb;
+  const Range rg = fin_block_range(n, lb);
+  for (int i = rg.begin + threadIdx.x; i < rg.end; i += kFinBlock) p[i] = z[i] + beta * p[i];
+}
+
 __global__ __launch_bounds__(kFinBlock) void pcg_fin_beta(PcgState* st, const double* __restrict__ partials,
                                                           int nparts) {
   if (st->done) return;
@@ -311,6 +407,8 @@ __global__ __launch_bounds__(kFinBlock) void pcg_fin_init(PcgState* st, const do
   if (threadIdx.x == 0) {
     st->alpha = st->beta = 0.0;
     st->gamma = t.v[0];
+    st->g[0] = t.v[0];
+    st->g[1] = 0.0;
     st->rs = t.v[1];
     st->bb = t.v[2];
     st->atol2 = fmax(tol * tol * t.v[2], atol * atol);
@@ -348,10 +446,12 @@ struct CsrOp {
     hipLaunchKernelGGL((pcg_spmv_kernel<V, T, L, B3, DIFF>), dim3(grid()), dim3(kBlock), 0, s, n, indptr, idx, data,
                        p, q, st, partials);
   }
+  static constexpr bool kThreeLaunch = true;
 };
 
 // Matrix-free J_bc v of the HEX8 mesh (feax's Krylov operator).
 struct MatfreeOp {
+  static constexpr bool kThreeLaunch = false;
   Hex8Op op;
   int grid() const { return hex8_op_blocks(op.num_nodes); }
   void apply(hipStream_t s, const double* xv, double* yv) const { launch_hex8_op(s, op, xv, yv, nullptr, nullptr); }
@@ -360,19 +460,30 @@ struct MatfreeOp {
   }
 };
 
+// One iteration (it = its index from 0, for the gamma parity).  CSR
+// operators, three launches: q = A p with the p.q partials; alpha and the
+// x, r, z updates; beta and the direction.  Matrix-free operator, five:
+// q = A p; finalize alpha; updates; finalize beta; direction.
 template <typename T, typename Op>
-void launch_pcg_iter(hipStream_t s, const PcgArgs& a, const Op& op, int grid_v, const T* dinv, T* x, T* r, T* z,
-                     T* p, T* q, PcgState* st, double* partials, double* op_partials, hipEvent_t ev0,
+void launch_pcg_iter(hipStream_t s, const PcgArgs& a, const Op& op, int it, int grid_v, const T* dinv, T* x, T* r,
+                     T* z, T* p, T* q, PcgState* st, double* partials, double* op_partials, hipEvent_t ev0,
                      hipEvent_t ev1) {
   if (ev0) (void)hipEventRecord(ev0, s);
   op.iter(s, p, q, st, op_partials);
   if (ev1) (void)hipEventRecord(ev1, s);
-  hipLaunchKernelGGL(pcg_fin_alpha, dim3(1), dim3(kFinBlock), 0, s, st, op_partials, op.grid(), a.rec_pq,
-                     a.rec_max);
-  hipLaunchKernelGGL((pcg_update_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, a.n, x, r, z, p, q, dinv, st,
-                     partials, a.rec_p, a.rec_max);
-  hipLaunchKernelGGL(pcg_fin_beta, dim3(1), dim3(kFinBlock), 0, s, st, partials, grid_v);
-  hipLaunchKernelGGL((pcg_dir_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, a.n, p, z, st);
+  if constexpr (Op::kThreeLaunch) {
+    const int gu = upd_grid(a.n), par = it & 1;
+    hipLaunchKernelGGL((pcg_update_alpha_kernel<T>), dim3(gu), dim3(kFinBlock), 0, s, a.n, par, x, r, z, p, q, dinv,
+                       st, op_partials, op.grid(), partials, a.rec_p, a.rec_pq, a.rec_max);
+    hipLaunchKernelGGL((pcg_dir_beta_kernel<T>), dim3(gu), dim3(kFinBlock), 0, s, a.n, par, p, z, st, partials, gu);
+  } else {
+    hipLaunchKernelGGL(pcg_fin_alpha, dim3(1), dim3(kFinBlock), 0, s, st, op_partials, op.grid(), a.rec_pq,
+                       a.rec_max);
+    hipLaunchKernelGGL((pcg_update_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, a.n, x, r, z, p, q, dinv, st,
+                       partials, a.rec_p, a.rec_max);
+    hipLaunchKernelGGL(pcg_fin_beta, dim3(1), dim3(kFinBlock), 0, s, st, partials, grid_v);
+    hipLaunchKernelGGL((pcg_dir_kernel<T>), dim3(grid_v), dim3(kBlock), 0, s, a.n, p, z, st);
+  }
 }
 
 template <typename T, typename Op>
@@ -404,6 +515,9 @@ int pcg_loop(hipStream_t s, const PcgArgs& a, const Op& op, const T* dinv, const
                      atol, maxiter);
   FX_LAUNCHED("pcg_init");
   if (check_every <= 0) check_every = 32;
+  // A captured chunk starts at an even iteration (the gamma parity of the
+  // three-launch CSR iteration repeats with every replay).
+  if (Op::kThreeLaunch && (check_every & 1)) ++check_every;
   struct {
     int k, done;
   } h = {0, 0};
@@ -432,7 +546,7 @@ int pcg_loop(hipStream_t s, const PcgArgs& a, const Op& op, const T* dinv, const
     if (e == hipSuccess) e = hipStreamBeginCapture(gg.stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
       for (int c = 0; c < check_every; ++c)
-        launch_pcg_iter<T, Op>(gg.stream, a, op, grid_v, dinv, x, r, z, p, q, st, partials, op_partials, nullptr,
+        launch_pcg_iter<T, Op>(gg.stream, a, op, c, grid_v, dinv, x, r, z, p, q, st, partials, op_partials, nullptr,
                                nullptr);
       e = hipStreamEndCapture(gg.stream, &gg.graph);
     }
@@ -452,7 +566,8 @@ int pcg_loop(hipStream_t s, const PcgArgs& a, const Op& op, const T* dinv, const
     } else {
       for (int c = 0; c < chunk; ++c) {
         hipEvent_t e0 = timing ? ev[2 * c] : nullptr, e1 = timing ? ev[2 * c + 1] : nullptr;
-        launch_pcg_iter<T, Op>(s, a, op, grid_v, dinv, x, r, z, p, q, st, partials, op_partials, e0, e1);
+        launch_pcg_iter<T, Op>(s, a, op, launched + c, grid_v, dinv, x, r, z, p, q, st, partials, op_partials, e0,
+                               e1);
       }
     }
     launched += chunk;

@@ -12,6 +12,7 @@ namespace fx {
 struct PcgState {
   double alpha, beta, gamma, rs, atol2, bb;
   int k, done, maxiter, nparts;
+  double g[2];  // gamma of even / odd iterations (CSR iterations read g[k % 2])
 };
 
 // J_bc(u) v of the HEX8 mesh (fx_hex8_jvp_f64) as a PCG operator.

@@ -22,8 +22,11 @@ struct DVec {
   double v[NV];
 };
 
-// Sum NV values over a block of BS threads; result valid in thread 0.
-// Fixed order: butterflies inside each wave, then a serial sum over waves.
+// Sum NV values over a block of BS threads; result valid in thread 0 (the
+// other threads get zeros: a defined value, so the compiler cannot treat
+// their use of the result as undefined and specialise the caller for
+// thread 0).  Fixed order: butterflies inside each wave, then a serial sum
+// over waves.
 // Call it once per kernel with all quantities (NV > 1): a second call would
 // reuse the LDS slots while thread 0 may still read the first call's.
 template <int NV, int BS>
@@ -43,6 +46,8 @@ __device__ __forceinline__ DVec<NV> block_sum(DVec<NV> a) {
   }
   __syncthreads();
   DVec<NV> out;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) out.v[k] = 0.0;
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
