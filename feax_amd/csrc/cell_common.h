@@ -16,20 +16,57 @@ constexpr int kMaxDeg = 64;        // LDS accumulator: 9 * kMaxDeg values per wa
 
 // The node's three CSR rows from its LDS accumulator (acc: row i, block kb,
 // column j at i * len + 3 kb + j): Jacobi dinv, the difference form, and the
-// store with the Dirichlet elimination of feax/DCboundary.py:147-199.
-template <typename T, typename C, int NPE = 8, int V = 3, int SHIFT = 3>
+// store with the Dirichlet elimination of feax/DCboundary.py:147-199.  The
+// constrained columns are found once per neighbour block (one lane per
+// block reads the block's column node, a ballot per component collects the
+// Dirichlet flags), not once per value.
+// First column of neighbour block kb = 64 w + lane of the node's rows
+// (V m for column node m), -1 past the node's degree: loaded before the
+// element loop so that the load overlaps it.
+template <int V, int MAXDEG>
+__device__ __forceinline__ void block_first_cols(int lane, int len, int row0, const int* __restrict__ indices,
+                                                 int (&col)[(MAXDEG + 63) / 64]) {
+#pragma unroll
+  for (int w = 0; w < (MAXDEG + 63) / 64; ++w) {
+    const int kb = 64 * w + lane;
+    col[w] = kb < len / V ? indices[row0 + V * kb] : -V;
+  }
+}
+
+template <typename T, typename C, int NPE = 8, int V = 3, int SHIFT = 3, int MAXDEG = kMaxDeg>
 __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, int row0, int nval,
                                             const int* __restrict__ inc_ptr, const int* __restrict__ inc,
                                             const uint8_t* __restrict__ rank, const int* __restrict__ indices,
                                             const uint8_t* __restrict__ bc_mask, int symmetric, int diff_form,
-                                            double jacobi_shift, double* __restrict__ dinv, T* __restrict__ data) {
+                                            double jacobi_shift, double* __restrict__ dinv, T* __restrict__ data,
+                                            const int* col0 = nullptr) {
+  constexpr int kW = (MAXDEG + 63) / 64;
   __builtin_amdgcn_wave_barrier();
-  // Position of node nd in its own neighbour list (its diagonal 3x3 block).
+  const int deg = len / V;
+  int col[kW];
+  if (col0) {
+#pragma unroll
+    for (int w = 0; w < kW; ++w) col[w] = col0[w];
+  } else {
+    block_first_cols<V, MAXDEG>(lane, len, row0, indices, col);
+  }
+  // Per neighbour block kb (one lane each): its column node m.  kself = the
+  // block of node nd itself (its diagonal block); cb[d][w] bit t: column DOF
+  // (m, d) of block kb = 64 w + t is constrained.
   int kself = 0;
-  {
-    const int pk = inc[inc_ptr[nd]];
-    const int a0 = pk & ((1 << SHIFT) - 1);
-    kself = rank[NPE * NPE * static_cast<long>(pk >> SHIFT) + (NPE + 1) * a0];
+  unsigned long long cb[V][kW];
+  bool rbc[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) rbc[i] = bc_mask && bc_mask[V * nd + i];
+#pragma unroll
+  for (int w = 0; w < kW; ++w) {
+    const int kb = 64 * w + lane;
+    const bool in = kb < deg;
+    const int m = col[w] / V;
+    const unsigned long long self = __ballot(in && m == nd);
+    if (self) kself = 64 * w + __builtin_ctzll(self);
+#pragma unroll
+    for (int d = 0; d < V; ++d) cb[d][w] = bc_mask ? __ballot(in && bc_mask[V * m + d] != 0) : 0ull;
   }
   if (dinv && lane < V) {
     // Jacobi: 1 / (K_bc[r, r] + shift) (feax/solvers/common.py:86-94); BC rows have diagonal 1.
@@ -41,16 +78,29 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
     // Difference form: the diagonal block slot holds S = block row sums of
     // the BC-eliminated matrix.  Unconstrained element rows sum to zero
     // (sum_b grad N_b = 0), so for a free row S[c][d] = -(sum of its entries
-    // in eliminated columns (m, d)); a BC row has S[c][d] = delta_cd.
+    // in eliminated columns (m, d)), added in ascending block order; a BC
+    // row has S[c][d] = delta_cd.
     C S = C(0);
     const int c = lane / V, d = lane - V * (lane / V);
     if (lane < V * V) {
-      if (bc_mask && bc_mask[V * nd + c]) {
+      bool rowbc = false;
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+        if (i == c) rowbc = rbc[i];
+      if (rowbc) {
         S = (c == d) ? C(1) : C(0);
-      } else if (bc_mask) {
-        for (int kb = 0; kb < len / V; ++kb) {
-          const int m = indices[row0 + V * kb] / V;
-          if (bc_mask[V * m + d]) S -= acc[c * len + V * kb + d];
+      } else {
+#pragma unroll
+        for (int w = 0; w < kW; ++w) {
+          unsigned long long bits = 0;
+#pragma unroll
+          for (int dd = 0; dd < V; ++dd)
+            if (dd == d) bits = cb[dd][w];
+          while (bits) {
+            const int t = __builtin_ctzll(bits);
+            bits &= bits - 1;
+            S -= acc[c * len + V * (64 * w + t) + d];
+          }
         }
       }
     }
@@ -60,16 +110,23 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
   }
   for (int idx = lane; idx < nval; idx += 64) {
     C v = acc[idx];
-    const int slot = row0 + idx;
-    const bool own = ((idx % len) / V) == kself;
+    const int i = idx / len, p = idx - len * i;
+    const int kb = p / V, d = p - V * kb;
+    const bool own = kb == kself;
     if (bc_mask && !(diff_form && own)) {
-      const int r = V * nd + idx / len;
-      const int c = indices[slot];
-      const bool zr = bc_mask[r] != 0;
-      if (zr || (symmetric && bc_mask[c] != 0)) v = C(0);
-      if (zr && r == c) v = C(1);
+      bool zr = false, zc = false;
+#pragma unroll
+      for (int ii = 0; ii < V; ++ii)
+        if (ii == i) zr = rbc[ii];
+#pragma unroll
+      for (int w = 0; w < kW; ++w)
+#pragma unroll
+        for (int dd = 0; dd < V; ++dd)
+          if (w == (kb >> 6) && dd == d) zc = (cb[dd][w] >> (kb & 63)) & 1ull;
+      if (zr || (symmetric && zc)) v = C(0);
+      if (zr && own && i == d) v = C(1);
     }
-    data[slot] = static_cast<T>(v);
+    data[row0 + idx] = static_cast<T>(v);
   }
 }
 
@@ -92,24 +149,46 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
   const int len = indptr[V * nd + 1] - row0;  // V * deg
   const int nval = V * len;
   if (len > V * MAXDEG) return;  // host guarantees deg <= MAXDEG
+  int col[(MAXDEG + 63) / 64];
+  block_first_cols<V, MAXDEG>(lane, len, row0, indices, col);
   for (int i = lane; i < nval; i += 64) acc[i] = C(0);
   __builtin_amdgcn_wave_barrier();
-  for (int k = inc_ptr[nd]; k < inc_ptr[nd + 1]; ++k) {
-    const int pk = inc[k];
-    const int e = pk >> SHIFT, a = pk & ((1 << SHIFT) - 1);
-    const C* src = ke + V * V * NPE * NPE * static_cast<long>(e) + V * V * NPE * a;  // blocks (a, b), b < NPE
-    const uint8_t* rk = rank + NPE * NPE * static_cast<long>(e) + NPE * a;
-    // V^2 NPE values: lane idx -> block b = idx / V^2, entry t = idx % V^2;
-    // distinct lanes touch distinct accumulator entries.
-    for (int idx = lane; idx < V * V * NPE; idx += 64) {
-      const int b = idx / (V * V), t = idx - V * V * b;
-      const int i = t / V, j = t - V * i;
-      acc[i * len + V * rk[b] + j] += src[idx];
+  // V^2 NPE values per element: lane idx -> block b = idx / V^2, entry
+  // t = idx % V^2; distinct lanes touch distinct accumulator entries.  The
+  // values and ranks of kPf elements are loaded before any is added (their
+  // loads overlap), then added in ascending element order.
+  constexpr int kVal = V * V * NPE, kPer = (kVal + 63) / 64, kPf = 4;
+  const int kend = inc_ptr[nd + 1];
+  for (int k0 = inc_ptr[nd]; k0 < kend; k0 += kPf) {
+    C val[kPf][kPer];
+    int pos[kPf][kPer];
+#pragma unroll
+    for (int u = 0; u < kPf; ++u) {
+      const int k = k0 + u;
+      const int pk = k < kend ? inc[k] : 0;
+      const int e = pk >> SHIFT, a = pk & ((1 << SHIFT) - 1);
+      const C* src = ke + kVal * NPE * static_cast<long>(e) + kVal * a;  // blocks (a, b), b < NPE
+      const uint8_t* rk = rank + NPE * NPE * static_cast<long>(e) + NPE * a;
+#pragma unroll
+      for (int p = 0; p < kPer; ++p) {
+        const int idx = lane + 64 * p;
+        const bool live = k < kend && idx < kVal;
+        const int b = idx / (V * V), t = idx - V * V * b;
+        const int i = t / V, j = t - V * i;
+        val[u][p] = live ? src[idx] : C(0);
+        pos[u][p] = live ? i * len + V * rk[b] + j : -1;
+      }
     }
-    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < kPf; ++u) {
+#pragma unroll
+      for (int p = 0; p < kPer; ++p)
+        if (pos[u][p] >= 0) acc[pos[u][p]] += val[u][p];
+      __builtin_amdgcn_wave_barrier();
+    }
   }
-  finish_rows<T, C, NPE, V, SHIFT>(nd, lane, acc, len, row0, nval, inc_ptr, inc, rank, indices, bc_mask,
-                                  symmetric, diff_form, jacobi_shift, dinv, data);
+  finish_rows<T, C, NPE, V, SHIFT, MAXDEG>(nd, lane, acc, len, row0, nval, inc_ptr, inc, rank, indices, bc_mask,
+                                          symmetric, diff_form, jacobi_shift, dinv, data, col);
 }
 
 __global__ void bc_res_kernel(int nbc, const int* __restrict__ rows, const double* __restrict__ vals,

@@ -287,6 +287,97 @@ __global__ __launch_bounds__(64 * kElemPerBlock) void elem_matrix_kernel(
   }
 }
 
+// Linear laws (linear elastic, SIMP): each wave forms the matrices of 8
+// elements.  Geometry: lane (el, q) = (lane >> 3, lane & 7) builds the
+// Jacobian of element el at Gauss point q once and writes grad N_c for all
+// 8 nodes c and the weights JxW E lam0 / JxW E mu0 to LDS (one lane per
+// point instead of eight).  Blocks: lane (a, b) accumulates, element by
+// element, K_ab = sum_q wl ga gb^T + wm gb ga^T + wm (ga . gb) I with fused
+// multiply-adds in an order under which K_ba = K_ab^T holds bitwise (CG
+// iteration counts react to a non-symmetric rounding of the matrix), and
+// stores its 9 values.
+constexpr int kLinWaves = 2;  // waves per block
+constexpr int kLinElems = 8;  // elements per wave
+
+template <typename C>
+__global__ __launch_bounds__(64 * kLinWaves) void elem_matrix_lin_kernel(
+    int num_cells, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const int* __restrict__ cells, MatParams mat, const double* __restrict__ param, C* __restrict__ ke) {
+  __shared__ C G_all[kLinWaves][kLinElems][8][8][3];  // grad N_c of element el at Gauss point q
+  __shared__ C W_all[kLinWaves][kLinElems][8][2];     // JxW E lam0, JxW E mu0
+  __shared__ C K_all[kLinWaves][576];                 // one element matrix, staged for the store
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ebase = (blockIdx.x * kLinWaves + wid) * kLinElems;
+  if (ebase >= num_cells) return;
+  C(*G)[8][8][3] = G_all[wid];
+  C(*W)[8][2] = W_all[wid];
+  C* KS = K_all[wid];
+  {
+    const int el = lane >> 3, q = lane & 7;
+    const int e = min(ebase + el, num_cells - 1);  // a lane past the end repeats the last element
+    const int* cn = cells + 8 * static_cast<long>(e);
+    const C x0 = static_cast<C>(gauss_pt(q >> 2)), x1 = static_cast<C>(gauss_pt((q >> 1) & 1)),
+            x2 = static_cast<C>(gauss_pt(q & 1));
+    C X[8][3];
+    load_coords<C>(x, y, z, cn, X);
+    C Ji[3][3];
+    const C det = jac_inv<C>(X, x0, x1, x2, Ji);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      C N, d0, d1, d2, g[3];
+      shape<C>(c, x0, x1, x2, N, d0, d1, d2);
+      phys_grad<C>(Ji, d0, d1, d2, g);
+      G[el][q][c][0] = g[0];
+      G[el][q][c][1] = g[1];
+      G[el][q][c][2] = g[2];
+    }
+    const C nu = static_cast<C>(mat.nu);
+    const C lam0 = nu / ((C(1) + nu) * (C(1) - C(2) * nu)), mu0 = C(1) / (C(2) * (C(1) + nu));
+    const C wE = det * C(0.125) * modulus_at<C>(mat, param, e, cn, x0, x1, x2);  // JxW E
+    W[el][q][0] = wE * lam0;
+    W[el][q][1] = wE * mu0;
+  }
+  __builtin_amdgcn_wave_barrier();
+  const int a = lane >> 3, b = lane & 7;
+  for (int el = 0; el < kLinElems; ++el) {
+    const int e = ebase + el;
+    if (e >= num_cells) break;
+    C K[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) K[t] = C(0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const C wl = W[el][q][0], wm = W[el][q][1];
+      C P[3][3];  // P_ij = ga_i gb_j
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) P[i][j] = G[el][q][a][i] * G[el][q][b][j];
+      const C gg = P[0][0] + P[1][1] + P[2][2];
+      // Entry (i, j) of block (a, b) and entry (j, i) of block (b, a) take
+      // the same operations on the same operands (P_ij of one is P_ji of the
+      // other), so the element matrix is exactly symmetric.
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          C k = fma_c(wm, P[j][i], K[3 * i + j]);
+          k = fma_c(wl, P[i][j], k);
+          K[3 * i + j] = i == j ? fma_c(wm, gg, k) : k;
+        }
+    }
+    // Stage the 576 values in LDS and store them coalesced: block (a, b)
+    // at 9 (8 a + b).
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int t = 0; t < 9; ++t) KS[9 * lane + t] = K[t];
+    __builtin_amdgcn_wave_barrier();
+    C* dst = ke + 576 * static_cast<long>(e);
+#pragma unroll
+    for (int i = lane; i < 576; i += 64) dst[i] = KS[i];
+  }
+}
+
 // ------------------------------------------------------------------ residual
 __global__ __launch_bounds__(64 * kNodesPerBlock) void residual_kernel(
     int num_nodes, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
@@ -649,12 +740,17 @@ int assemble(void* stream, int num_nodes, int num_cells, const double* x, const 
     if (ws_bytes < need) return fail(FX_EWORKSPACE, "fx_hex8_assemble: workspace %zu < %zu", ws_bytes, need);
   }
   const int egrid = (num_cells + kElemPerBlock - 1) / kElemPerBlock;
+  const int lgrid = (num_cells + kLinWaves * kLinElems - 1) / (kLinWaves * kLinElems);
 #define FX_ASM_LAUNCH(C, NHV)                                                                                      \
   do {                                                                                                             \
     if (workspace) {                                                                                               \
       C* ke = static_cast<C*>(workspace);                                                                          \
-      hipLaunchKernelGGL((elem_matrix_kernel<C, NHV>), dim3(egrid), dim3(64 * kElemPerBlock), 0, s, num_cells, x,  \
-                         y, z, cells, mp, param, u, ke);                                                           \
+      if (NHV)                                                                                                     \
+        hipLaunchKernelGGL((elem_matrix_kernel<C, NHV>), dim3(egrid), dim3(64 * kElemPerBlock), 0, s, num_cells,   \
+                           x, y, z, cells, mp, param, u, ke);                                                      \
+      else                                                                                                         \
+        hipLaunchKernelGGL((elem_matrix_lin_kernel<C>), dim3(lgrid), dim3(64 * kLinWaves), 0, s, num_cells, x, y,  \
+                           z, cells, mp, param, ke);                                                               \
       hipLaunchKernelGGL((gather_rows_kernel<T, C>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s, num_nodes,       \
                          inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, diff, jacobi_shift, dinv,    \
                          data);                                                                                    \

@@ -157,6 +157,10 @@ __device__ __forceinline__ T modulus_at(const MatParams& m, const double* __rest
   return modulus_of<T>(m, param_at<T>(m, param, e, cn, x0, x1, x2));
 }
 
+// Fused multiply-add in the element's arithmetic type.
+__device__ __forceinline__ double fma_c(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_c(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 template <typename T>
 __device__ __forceinline__ T xor_sum(T v, int mask) {
   return v + __shfl_xor(v, mask, 64);

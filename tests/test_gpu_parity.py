@@ -148,10 +148,19 @@ def test_fp64_solve_matches_direct(small_cantilever, spmv):
     solver = fe.create_solver(p, bc, opts, linear=True, traced_params=tp)
     u = np.asarray(solver(tp, fe.zero_like_initial_guess(p, bc)))
     err = np.linalg.norm(u - small_cantilever["u_dir"]) / np.linalg.norm(small_cantilever["u_dir"])
+    import torch
+    from feax_amd.assembler import assemble_csr
     from oracle import krylov
-    K, b = small_cantilever["info"]["K"], small_cantilever["info"]["b"]
-    _, it_ref, _ = krylov.pcg(K, b, small_cantilever["info"]["x0"], 1e-10, 0.0, 10000, krylov.jacobi_dinv(K.diagonal()))
-    print("fp64", spmv, solver.stats, "err", err, "oracle iters", it_ref)
+    b, x0 = small_cantilever["info"]["b"], small_cantilever["info"]["x0"]
+    # The oracle's PCG on the matrix this solve used: on this slender
+    # cantilever the iteration count moves by several percent under
+    # last-bit differences of the assembled values (the GPU and oracle
+    # matrices differ by ~3e-15 relative and scipy's PCG needs 221 vs 212
+    # iterations on them), so the count is compared on identical input.
+    ts = fe.TracedStructure.from_problem(p)
+    Kg = assemble_csr(p, ts, tp, None, bc, True, torch.float64).to_scipy()
+    _, it_ref, _ = krylov.pcg(Kg, b, x0, 1e-10, 0.0, 10000, krylov.jacobi_dinv(Kg.diagonal()))
+    print("fp64", spmv, solver.stats, "err", err, "oracle PCG iters on the same matrix", it_ref)
     assert err < 1e-8
     # same algorithm and stopping rule; FP rounding may move the count slightly
     assert abs(solver.stats.iters - it_ref) <= max(3, 0.02 * it_ref)
