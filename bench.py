@@ -178,8 +178,11 @@ def torch_cpu_spmv(A64, reps=5):
     old = torch.get_num_threads()
     torch.set_num_threads(threads)
     try:
-        K = torch.sparse_csr_tensor(A64.indptr.cpu().long(), A64.indices.cpu().long(), A64.data.cpu(),
-                                    size=tuple(A64.shape))
+        import warnings
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", UserWarning)      # "sparse CSR support is in beta"
+            K = torch.sparse_csr_tensor(A64.indptr.cpu().long(), A64.indices.cpu().long(), A64.data.cpu(),
+                                        size=tuple(A64.shape))
         x = torch.randn(A64.shape[1], dtype=torch.float64)
         (K @ x)
         t0 = time.perf_counter()

@@ -175,29 +175,38 @@ __global__ __launch_bounds__(kBlock) void block3_index_kernel(int nnodes, const 
 }
 
 // ---------------------------------------------------------------- Jacobi
+// 16 lanes per row scan the row's slots for the (i, i) entries; the row
+// holds one (the CSR is deduplicated), so the lane sum is that value.
+constexpr int kDiagLanes = 16;
+inline int diag_grid(int n) { return (n + kBlock / kDiagLanes - 1) / (kBlock / kDiagLanes); }
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void jacobi_kernel(int n, const int* __restrict__ indptr,
                                                         const int* __restrict__ indices, const T* __restrict__ data,
                                                         double shift, T* __restrict__ dinv) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = blockIdx.x * (kBlock / kDiagLanes) + threadIdx.x / kDiagLanes;
+  const int gl = threadIdx.x % kDiagLanes;
   if (i >= n) return;
   // feax CSRMatrix.diagonal (csr.py:115-121): sum of the row's (i, i) slots.
   double d = 0.0;
-  for (int k = indptr[i]; k < indptr[i + 1]; ++k)
+  for (int k = indptr[i] + gl; k < indptr[i + 1]; k += kDiagLanes)
     if (indices[k] == i) d += static_cast<double>(data[k]);
-  dinv[i] = static_cast<T>(1.0 / (d + shift));
+  d = group_sum<double, kDiagLanes>(d);
+  if (gl == 0) dinv[i] = static_cast<T>(1.0 / (d + shift));
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void diag_kernel(int n, const int* __restrict__ indptr,
                                                       const int* __restrict__ indices, const T* __restrict__ data,
                                                       T* __restrict__ diag) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = blockIdx.x * (kBlock / kDiagLanes) + threadIdx.x / kDiagLanes;
+  const int gl = threadIdx.x % kDiagLanes;
   if (i >= n) return;
   T d = T(0);
-  for (int k = indptr[i]; k < indptr[i + 1]; ++k)
+  for (int k = indptr[i] + gl; k < indptr[i + 1]; k += kDiagLanes)
     if (indices[k] == i) d += data[k];
-  diag[i] = d;
+  d = group_sum<T, kDiagLanes>(d);
+  if (gl == 0) diag[i] = d;
 }
 
 // ---------------------------------------------------------------- PCG
@@ -812,7 +821,7 @@ int fx_csr_spmv_diff3_f32f64(void* stream, int32_t n, int64_t nse, const int32_t
 int fx_csr_jacobi_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, const double* data,
                       double shift, double* dinv) {
   if (n <= 0) return FX_OK;
-  hipLaunchKernelGGL((jacobi_kernel<double>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, as_stream(stream),
+  hipLaunchKernelGGL((jacobi_kernel<double>), dim3(diag_grid(n)), dim3(kBlock), 0, as_stream(stream),
                      n, indptr, indices, data, shift, dinv);
   FX_LAUNCHED("jacobi_kernel");
   return FX_OK;
@@ -821,7 +830,7 @@ int fx_csr_jacobi_f64(void* stream, int32_t n, const int32_t* indptr, const int3
 int fx_csr_jacobi_f32(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, const float* data,
                       double shift, float* dinv) {
   if (n <= 0) return FX_OK;
-  hipLaunchKernelGGL((jacobi_kernel<float>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, as_stream(stream),
+  hipLaunchKernelGGL((jacobi_kernel<float>), dim3(diag_grid(n)), dim3(kBlock), 0, as_stream(stream),
                      n, indptr, indices, data, shift, dinv);
   FX_LAUNCHED("jacobi_kernel");
   return FX_OK;
@@ -830,7 +839,7 @@ int fx_csr_jacobi_f32(void* stream, int32_t n, const int32_t* indptr, const int3
 int fx_csr_diagonal_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, const double* data,
                         double* diag) {
   if (n <= 0) return FX_OK;
-  hipLaunchKernelGGL((diag_kernel<double>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, as_stream(stream), n,
+  hipLaunchKernelGGL((diag_kernel<double>), dim3(diag_grid(n)), dim3(kBlock), 0, as_stream(stream), n,
                      indptr, indices, data, diag);
   FX_LAUNCHED("diag_kernel");
   return FX_OK;
@@ -839,7 +848,7 @@ int fx_csr_diagonal_f64(void* stream, int32_t n, const int32_t* indptr, const in
 int fx_csr_diagonal_f32(void* stream, int32_t n, const int32_t* indptr, const int32_t* indices, const float* data,
                         float* diag) {
   if (n <= 0) return FX_OK;
-  hipLaunchKernelGGL((diag_kernel<float>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, as_stream(stream), n,
+  hipLaunchKernelGGL((diag_kernel<float>), dim3(diag_grid(n)), dim3(kBlock), 0, as_stream(stream), n,
                      indptr, indices, data, diag);
   FX_LAUNCHED("diag_kernel");
   return FX_OK;

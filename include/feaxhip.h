@@ -7,13 +7,24 @@
  *     of the FX_E* codes below; fx_last_error() returns the message of the
  *     last failure on the calling thread.
  *   - Device entry points are stream-ordered on the hipStream_t passed as
- *     `stream` (void* here so the header needs no HIP include), own no
- *     buffers and allocate nothing: scratch comes in through `workspace`
- *     whose size is given by the matching *_workspace_bytes() query.
- *   - Only the PCG drivers synchronise the stream (to read the convergence
- *     flag every `check_every` iterations).  Everything else is async.
+ *     `stream` (void* here so the header needs no HIP include) and allocate
+ *     no device memory: scratch comes in through `workspace`, whose size is
+ *     given by the matching *_workspace_bytes() query.
+ *   - The kernels-only entry points (pattern queries aside) are
+ *     asynchronous.  The iterative drivers (fx_pcg_*, fx_pcg_matfree_f64,
+ *     fx_pcg_gmg_f64, fx_bicgstab_f64, fx_gmres_f64, fx_pcg_batched_*)
+ *     synchronise the stream to read their convergence flags every
+ *     `check_every` iterations, and each call creates a private stream (of
+ *     default priority) and instantiates a HIP graph of one chunk of
+ *     iterations, released before it returns (fx_pcg_*: not with
+ *     FX_PCG_NO_GRAPH or FX_PCG_TIME).  The batched drivers and
+ *     fx_seg_project_f32 also synchronise after uploading their host-side
+ *     instance tables.  The banded direct solver calls rocBLAS / rocSOLVER
+ *     (one rocBLAS handle per device, created on first use and kept) and
+ *     synchronises at the end of a factorisation.  fx_pcg_f64_noalloc
+ *     creates no stream, graph or event.
  *   - Index arrays are int32 (feax keeps all structure in int32,
- *     feax/problem.py:236-244, 441-451).  Meshes are HEX8 with vec = 3.
+ *     feax/problem.py:236-244, 441-451).
  *   - (d) marks device pointers, (h) host pointers.
  */
 #ifndef FEAXHIP_H

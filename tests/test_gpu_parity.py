@@ -305,10 +305,10 @@ def test_matfree_gradient_matches_csr_gradient():
 @pytest.mark.parametrize("kind,loc", [("linear_elastic", "node"), ("simp", "cell"), ("neo_hookean", "cell")])
 def test_two_pass_and_single_pass_element_kernels_agree(kind, loc, monkeypatch):
     """Element-workspace (two-pass) assembly and residual vs the per-node
-    single-pass kernels: the same arithmetic in the same order, so CSR values
-    (FP64, FP32, difference form), dinv and the residual agree to the
-    rounding of an operation (the compiler may contract a multiply-add
-    differently in the two kernels)."""
+    single-pass kernels: the same sums (the linear-law two-pass kernel adds
+    the Gauss points in another order, with fused multiply-adds), so CSR
+    values (FP64, FP32, difference form), dinv and the residual agree to a
+    few roundings of the row maximum."""
     import torch
     import feax_amd as fe
     import feax_amd.assembler as asm
