@@ -46,6 +46,22 @@ class FxGmgLevel(ctypes.Structure):
                 ("r", ctypes.c_void_p), ("d", ctypes.c_void_p)]
 
 
+class FxB3Map(ctypes.Structure):
+    """fx_b3map: block-column map of a block-3 CSR (device pointers)."""
+    _fields_ = [("lidx", ctypes.c_void_p), ("uptr", ctypes.c_void_p), ("ucol", ctypes.c_void_p),
+                ("nodes_per_block", ctypes.c_int32), ("max_unique", ctypes.c_int32)]
+
+
+class B3Map:
+    """An FxB3Map and the device tensors (lidx, uptr, ucol) it points into."""
+
+    def __init__(self, tensors, nodes_per_block, max_unique):
+        self.tensors = tensors
+        lidx, uptr, ucol = tensors
+        self.struct = FxB3Map(lidx.data_ptr(), uptr.data_ptr(), ucol.data_ptr(), int(nodes_per_block),
+                              int(max_unique))
+
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -111,6 +127,11 @@ SIGNATURES = {
                           _P, _P, _P, _I32, _P, _SZ]),
     "fx_pcg_f32f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P,
                              _P, _P, _P, _P, _I32, _P, _SZ]),
+    "fx_csr_block3_lanes": (_I32, [_I32, _I64, _I32]),
+    "fx_pcg_b3map_f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P, _P,
+                                _P, _P, _I32, _P, _SZ]),
+    "fx_pcg_b3map_f32f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P,
+                                   _P, _P, _P, _I32, _P, _SZ]),
     "fx_krylov_project_workspace_bytes": (_SZ, [_I32]),
     "fx_krylov_project_f32": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _SZ]),
     "fx_dot_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _SZ]),

@@ -81,6 +81,41 @@ class TracedStructure:
         self._cache = {}
         self._bcol = None
 
+    def b3map(self, value_bytes):
+        """Block-column map (nat.B3Map) of the block-3 pattern for the SpMV of
+        the PCG with `value_bytes`-wide matrix values, or None when a
+        workgroup's unique column nodes would not fit its LDS staging.  One
+        workgroup runs 256 / lanes consecutive nodes (lanes from
+        fx_csr_block3_lanes); its sorted unique column nodes are
+        ucol[uptr[b]:uptr[b + 1]] and lidx gives each 3x3 block the position
+        of its column node in that list.  Built once per lane count."""
+        import numpy as np
+        import torch
+        lanes = int(nat.lib().fx_csr_block3_lanes(self.n, self.nse, int(value_bytes)))
+        G = 256 // lanes
+        cache = self.__dict__.setdefault("_b3maps", {})
+        if G in cache:
+            return cache[G]
+        N = self.n // 3
+        indptr = self.indptr.cpu().numpy().astype(np.int64)
+        bcol = self.bcol().cpu().numpy().astype(np.int64)
+        deg = (indptr[3 * np.arange(N) + 1] - indptr[3 * np.arange(N)]) // 3
+        blk = np.repeat(np.arange(N, dtype=np.int64) // G, deg)
+        uniq, inv = np.unique(blk * N + bcol, return_inverse=True)
+        nblk = (N + G - 1) // G
+        uptr = np.zeros(nblk + 1, np.int64)
+        np.add.at(uptr, uniq // N + 1, 1)
+        uptr = np.cumsum(uptr)
+        nu = np.diff(uptr)
+        m = None
+        if 3 * 8 * int(nu.max()) <= 48 * 1024:
+            lidx = (inv.reshape(-1) - uptr[blk]).astype(np.int16)
+            keep = (torch.as_tensor(lidx, device=self.device), torch.as_tensor(uptr.astype(np.int32), device=self.device),
+                    torch.as_tensor((uniq % N).astype(np.int32), device=self.device))
+            m = nat.B3Map(keep, G, int(nu.max()))
+        cache[G] = m
+        return m
+
     def bcol(self):
         """Column node of each 3x3 block of the CSR pattern (nse/9 int32)."""
         import torch
@@ -253,6 +288,8 @@ def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, 
     b3 = ts.vec == 3
     A = CSRMatrix(data, ts.indptr, ts.indices, (ts.n, ts.n), block3=b3, form="diff3" if diff_form else "csr",
                   bcol=ts.bcol() if b3 else None)
+    if b3:
+        A.b3map = ts.b3map
     return A if jacobi_shift is None else (A, dinv)
 
 

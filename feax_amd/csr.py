@@ -31,6 +31,9 @@ class CSRMatrix:
         self.block3 = bool(block3)
         self.form = form
         self._bcol = bcol
+        # callable(value_bytes) -> block-column map of the pattern (or None),
+        # set by the assembly for block-3 matrices; used by the PCG SpMV
+        self.b3map = None
 
     @property
     def nse(self):

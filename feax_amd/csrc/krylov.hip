@@ -272,6 +272,86 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, T* __restrict
   store_partials<2>(block_sum<2, kBlock>(acc), lb, partials);
 }
 
+// The PCG SpMV of a block-3 matrix with its block-column map (fx_b3map):
+// the workgroup's kBlock / L nodes share the x entries of their neighbour
+// nodes, so the workgroup copies x of its unique column nodes into LDS once
+// (ucol sorted: the copy reads nearby addresses) and every 3x3 block reads
+// its three x values from LDS at lidx.  own[i] = the list position of the
+// workgroup's i-th node (its diagonal block; the difference form subtracts
+// x_n from the other blocks' x).  Products and sums as in b3_rows.
+// PCG: skip when st->done and write the block partials of p . q; else a
+// plain q = A p.
+template <typename V, typename T, int L, bool DIFF, bool PCG>
+__global__ __launch_bounds__(kBlock) void pcg_spmv_map_kernel(int n, const int* __restrict__ indptr,
+                                                              const uint16_t* __restrict__ lidx,
+                                                              const int* __restrict__ uptr,
+                                                              const int* __restrict__ ucol,
+                                                              const V* __restrict__ data, const T* __restrict__ p,
+                                                              T* __restrict__ q, const PcgState* __restrict__ st,
+                                                              double* partials) {
+  if constexpr (PCG) {
+    if (st->done) return;
+  }
+  constexpr int G = kBlock / L;
+  extern __shared__ unsigned char smem[];
+  __shared__ int own[G];
+  T* xs = reinterpret_cast<T*>(smem);
+  const int ngroups = n / 3;
+  const int first = blockIdx.x * G;
+  {
+    const int u0 = uptr[blockIdx.x], nu = uptr[blockIdx.x + 1] - u0;
+    for (int u = threadIdx.x; u < nu; u += kBlock) {
+      const int c = ucol[u0 + u];
+      xs[3 * u] = p[3 * c];
+      xs[3 * u + 1] = p[3 * c + 1];
+      xs[3 * u + 2] = p[3 * c + 2];
+      if (c >= first && c < first + G) own[c - first] = u;
+    }
+  }
+  __syncthreads();
+  const int gl = threadIdx.x % L;
+  const int g = first + threadIdx.x / L;
+  DVec<1> acc;
+  acc.v[0] = 0.0;
+  if (g < ngroups) {
+    const int r0 = indptr[3 * g];
+    const int len = indptr[3 * g + 1] - r0;  // 3 * deg
+    const int nb = len / 3;
+    const uint16_t* li = lidx + r0 / 9;
+    const V* d0 = data + r0;
+    const V* d1 = d0 + len;
+    const V* d2 = d1 + len;
+    const int lo = own[g - first];
+    const T xs0 = xs[3 * lo], xs1 = xs[3 * lo + 1], xs2 = xs[3 * lo + 2];
+    T a0 = T(0), a1 = T(0), a2 = T(0);
+    for (int kb = gl; kb < nb; kb += L) {
+      const int l = li[kb];
+      T x0 = xs[3 * l], x1 = xs[3 * l + 1], x2 = xs[3 * l + 2];
+      if constexpr (DIFF) {
+        if (l != lo) {
+          x0 -= xs0;
+          x1 -= xs1;
+          x2 -= xs2;
+        }
+      }
+      const int o = 3 * kb;
+      a0 += static_cast<T>(d0[o]) * x0 + static_cast<T>(d0[o + 1]) * x1 + static_cast<T>(d0[o + 2]) * x2;
+      a1 += static_cast<T>(d1[o]) * x0 + static_cast<T>(d1[o + 1]) * x1 + static_cast<T>(d1[o + 2]) * x2;
+      a2 += static_cast<T>(d2[o]) * x0 + static_cast<T>(d2[o + 1]) * x1 + static_cast<T>(d2[o + 2]) * x2;
+    }
+    const T y0 = group_sum<T, L>(a0), y1 = group_sum<T, L>(a1), y2 = group_sum<T, L>(a2);
+    if (gl == 0) {
+      q[3 * g] = y0;
+      q[3 * g + 1] = y1;
+      q[3 * g + 2] = y2;
+      acc.v[0] = static_cast<double>(xs0) * static_cast<double>(y0) +
+                 static_cast<double>(xs1) * static_cast<double>(y1) +
+                 static_cast<double>(xs2) * static_cast<double>(y2);
+    }
+  }
+  if constexpr (PCG) store_partials<1>(block_sum<1, kBlock>(acc), blockIdx.x, partials);
+}
+
 // CSR iterations: three launches, the two 1-block finalizes folded into
 // the vector kernels.  Each of these kernels runs upd_grid(n) blocks of
 // kFinBlock threads, and every block sums all partials of the previous
@@ -454,6 +534,26 @@ struct CsrOp {
   void iter(hipStream_t s, const T* p, T* q, const PcgState* st, double* partials) const {
     hipLaunchKernelGGL((pcg_spmv_kernel<V, T, L, B3, DIFF>), dim3(grid()), dim3(kBlock), 0, s, n, indptr, idx, data,
                        p, q, st, partials);
+  }
+  static constexpr bool kThreeLaunch = true;
+};
+
+// The block-3 PCG operator with a block-column map (LDS-staged x).
+template <typename V, typename T, int L, bool DIFF>
+struct CsrMapOp {
+  int n;
+  const int* indptr;
+  fx_b3map map;
+  const V* data;
+  int grid() const { return grid_for(n / 3, L); }
+  size_t shm() const { return 3 * sizeof(T) * static_cast<size_t>(map.max_unique); }
+  void apply(hipStream_t s, const T* xv, T* yv) const {
+    hipLaunchKernelGGL((pcg_spmv_map_kernel<V, T, L, DIFF, false>), dim3(grid()), dim3(kBlock), shm(), s, n,
+                       indptr, map.lidx, map.uptr, map.ucol, data, xv, yv, nullptr, nullptr);
+  }
+  void iter(hipStream_t s, const T* p, T* q, const PcgState* st, double* partials) const {
+    hipLaunchKernelGGL((pcg_spmv_map_kernel<V, T, L, DIFF, true>), dim3(grid()), dim3(kBlock), shm(), s, n, indptr,
+                       map.lidx, map.uptr, map.ucol, data, p, q, st, partials);
   }
   static constexpr bool kThreeLaunch = true;
 };
@@ -643,6 +743,46 @@ int run_pcg(void* stream, int n, long long nse, int flags, const int* indptr, co
   } else {
     FX_LANE_SWITCH(lanes, return pcg_loop<T>(s, a, CsrOp<V, T, LL, false, false>{n, indptr, a.idx, data}, dinv, b,
                                              x0, tol, atol, maxiter, check_every, x, iters, rs, workspace));
+  }
+  return FX_OK;
+}
+
+// run_pcg on a block-3 matrix through its block-column map.
+template <typename V, typename T>
+int run_pcg_map(void* stream, int n, long long nse, int flags, const int* indptr, const fx_b3map* map,
+                const V* data, const T* dinv, const T* b, const T* x0, double tol, double atol, int maxiter,
+                int check_every, T* x, int* iters, double* rs, float* rec_p, double* rec_pq, int rec_max,
+                void* workspace, size_t ws_bytes) {
+  hipStream_t s = as_stream(stream);
+  if ((rec_p == nullptr) != (rec_pq == nullptr) || (rec_p && rec_max < 0))
+    return fail(FX_EINVAL, "pcg: recording needs rec_p, rec_pq and rec_max >= 0");
+  if (n < 0 || !indptr || !map || !data || !b || !x || !iters || !rs) return fail(FX_EINVAL, "pcg: bad arguments");
+  if (maxiter < 0) return fail(FX_EINVAL, "pcg: maxiter < 0");
+  if (!(flags & (FX_PCG_BLOCK3 | FX_PCG_DIFF3)) || n % 3)
+    return fail(FX_EINVAL, "pcg_b3map: a block-3 matrix (FX_PCG_BLOCK3 or FX_PCG_DIFF3, n %% 3 == 0)");
+  if (!map->lidx || !map->uptr || !map->ucol || map->max_unique < 0)
+    return fail(FX_EINVAL, "pcg_b3map: incomplete map");
+  const size_t need = pcg_ws_bytes(n, sizeof(T));
+  if (!workspace || ws_bytes < need) return fail(FX_EWORKSPACE, "pcg: workspace %zu < %zu bytes", ws_bytes, need);
+  const int lanes = lanes_for(n, nse, true, sizeof(V) == 8);
+  if (map->nodes_per_block != kBlock / lanes)
+    return fail(FX_EINVAL, "pcg_b3map: map built for %d nodes per block, the SpMV runs %d", map->nodes_per_block,
+                kBlock / lanes);
+  if (3 * sizeof(T) * static_cast<size_t>(map->max_unique) > 64 * 1024)
+    return fail(FX_ELIMIT, "pcg_b3map: %d unique columns per block exceed the LDS staging", map->max_unique);
+  if (n == 0) {
+    *iters = 0;
+    rs[0] = 0.0;
+    if (flags & FX_PCG_TIME) rs[1] = rs[2] = 0.0;
+    return FX_OK;
+  }
+  const PcgArgs a{n, nse, flags, indptr, nullptr, rec_p, rec_pq, rec_p ? rec_max : 0};
+  if (flags & FX_PCG_DIFF3) {
+    FX_LANE_SWITCH(lanes, return pcg_loop<T>(s, a, CsrMapOp<V, T, LL, true>{n, indptr, *map, data}, dinv, b, x0,
+                                             tol, atol, maxiter, check_every, x, iters, rs, workspace));
+  } else {
+    FX_LANE_SWITCH(lanes, return pcg_loop<T>(s, a, CsrMapOp<V, T, LL, false>{n, indptr, *map, data}, dinv, b, x0,
+                                             tol, atol, maxiter, check_every, x, iters, rs, workspace));
   }
   return FX_OK;
 }
@@ -902,6 +1042,28 @@ int fx_pcg_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int
                   double* rs, float* rec_p, double* rec_pq, int32_t rec_max, void* workspace, size_t ws_bytes) {
   return run_pcg<float, double>(stream, n, nse, flags, indptr, indices, bcol, data, dinv, b, x0, tol, atol, maxiter,
                                 check_every, x, iters, rs, rec_p, rec_pq, rec_max, workspace, ws_bytes);
+}
+
+int fx_csr_block3_lanes(int32_t n, int64_t nse, int32_t value_bytes) {
+  return lanes_for(n, nse, true, value_bytes == 8);
+}
+
+int fx_pcg_b3map_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr, const fx_b3map* map,
+                     const double* data, const double* dinv, const double* b, const double* x0, double tol,
+                     double atol, int32_t maxiter, int32_t check_every, double* x, int32_t* iters, double* rs,
+                     float* rec_p, double* rec_pq, int32_t rec_max, void* workspace, size_t ws_bytes) {
+  if (flags & FX_PCG_DIFF3) return fail(FX_EINVAL, "fx_pcg_b3map_f64: FX_PCG_DIFF3 is for FP32 matrices");
+  return run_pcg_map<double, double>(stream, n, nse, flags, indptr, map, data, dinv, b, x0, tol, atol, maxiter,
+                                     check_every, x, iters, rs, rec_p, rec_pq, rec_max, workspace, ws_bytes);
+}
+
+int fx_pcg_b3map_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
+                        const fx_b3map* map, const float* data, const double* dinv, const double* b, const double* x0,
+                        double tol, double atol, int32_t maxiter, int32_t check_every, double* x, int32_t* iters,
+                        double* rs, float* rec_p, double* rec_pq, int32_t rec_max, void* workspace,
+                        size_t ws_bytes) {
+  return run_pcg_map<float, double>(stream, n, nse, flags, indptr, map, data, dinv, b, x0, tol, atol, maxiter,
+                                    check_every, x, iters, rs, rec_p, rec_pq, rec_max, workspace, ws_bytes);
 }
 
 int fx_pcg_matfree_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,

@@ -191,6 +191,38 @@ int fx_pcg_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int
                   int32_t check_every, double* x, int32_t* iters, double* rs, float* rec_p,
                   double* rec_pq, int32_t rec_max, void* workspace, size_t ws_bytes);
 
+/* Block-column map of a block-3 CSR for the SpMV of the PCG: the SpMV runs
+ * nodes_per_block consecutive nodes per workgroup (256 / the lane count of
+ * fx_csr_block3_lanes); the workgroup first copies x of the sorted unique
+ * column nodes of its nodes (ucol[uptr[b] .. uptr[b+1])) into LDS, then each
+ * 3x3 block reads x from LDS at lidx (its column node's position in that
+ * list; one entry per block, in bcol order) instead of gathering it through
+ * bcol.  Same products in the same order: results equal the bcol path's
+ * bitwise.  All pointers (d); max_unique = max_b (uptr[b+1] - uptr[b]). */
+typedef struct fx_b3map {
+  const uint16_t* lidx;
+  const int32_t* uptr;
+  const int32_t* ucol;
+  int32_t nodes_per_block;
+  int32_t max_unique;
+} fx_b3map;
+/* Lanes per node the block-3 SpMV kernels use for this matrix
+ * (value_bytes 8: FP64 values, 4: FP32). */
+int fx_csr_block3_lanes(int32_t n, int64_t nse, int32_t value_bytes);
+/* fx_pcg_f64 / fx_pcg_f32f64 on a block-3 matrix (flags FX_PCG_BLOCK3 or
+ * FX_PCG_DIFF3) with its block-column map in place of bcol.  A map built
+ * for another lane count is refused (FX_EINVAL). */
+int fx_pcg_b3map_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
+                     const fx_b3map* map, const double* data, const double* dinv, const double* b,
+                     const double* x0, double tol, double atol, int32_t maxiter, int32_t check_every, double* x,
+                     int32_t* iters, double* rs, float* rec_p, double* rec_pq, int32_t rec_max, void* workspace,
+                     size_t ws_bytes);
+int fx_pcg_b3map_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
+                        const fx_b3map* map, const float* data, const double* dinv, const double* b,
+                        const double* x0, double tol, double atol, int32_t maxiter, int32_t check_every, double* x,
+                        int32_t* iters, double* rs, float* rec_p, double* rec_pq, int32_t rec_max,
+                        void* workspace, size_t ws_bytes);
+
 /* Krylov recycling: Galerkin projection of r onto nvec recorded A-conjugate
  * directions, x0 = sum_k p_k (p_k . r) / (p_k . A p_k) (init-CG, Erhel and
  * Guyomarc'h 2000).  P: (d) nvec*n FP32, pq: (d) nvec, r, x0: (d) n FP64.
