@@ -31,11 +31,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 
-# HBM bytes per pcg_spmv_kernel dispatch on the 200x40x40 workload, from
+# HBM bytes per PCG SpMV dispatch on the 200x40x40 workload
+# (pcg_spmv_kernel FP64 block-3, pcg_spmv_map_kernel FP32 difference form), from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of `bench.py --kernels-only`
 # (profiles/pcg_spmv_cantilever_200x40x40_pmc.md), each scaled by the same counter's
 # reading for a 1 GiB copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
-TRAFFIC = {"f64/block3": 761_544_739, "f32f64/diff3": 439_444_525}
+TRAFFIC = {"f64/block3": 761_557_198, "f32f64/diff3": 435_929_335}
 TRAFFIC_SOURCE = "profiles/pcg_spmv_cantilever_200x40x40_pmc.md"
 
 CONFIGS = {
@@ -710,8 +711,8 @@ def main():
     t_asm64 = event_time(lambda: assemble_csr(problem, ts, tp, None, bc, True, torch.float64), 5)
 
     headline = "mixed"
-    dom = dict(kernel="pcg_spmv_kernel, FP32 difference form, FP64 vectors (f32f64/diff3)", label="f32f64/diff3",
-               B=Bd)
+    dom = dict(kernel="pcg_spmv_map_kernel, FP32 difference form, FP64 vectors, x staged in LDS (f32f64/diff3)",
+               label="f32f64/diff3", B=Bd)
     if results["fp64"]["s"] < results["mixed"]["s"]:
         headline = "fp64"
         dom = dict(kernel="pcg_spmv_kernel, FP64 block-3 (f64/block3)", label="f64/block3", B=B64)
