@@ -81,18 +81,18 @@ class TracedStructure:
         self._cache = {}
         self._bcol = None
 
-    def b3map(self, value_bytes):
+    def b3map(self, value_bytes, threads=256):
         """Block-column map (nat.B3Map) of the block-3 pattern for the SpMV of
         the PCG with `value_bytes`-wide matrix values, or None when a
         workgroup's unique column nodes would not fit its LDS staging.  One
-        workgroup runs 256 / lanes consecutive nodes (lanes from
-        fx_csr_block3_lanes); its sorted unique column nodes are
-        ucol[uptr[b]:uptr[b + 1]] and lidx gives each 3x3 block the position
-        of its column node in that list.  Built once per lane count."""
+        workgroup of `threads` (256 or 512) runs threads / lanes consecutive
+        nodes (lanes from fx_csr_block3_lanes); its sorted unique column
+        nodes are ucol[uptr[b]:uptr[b + 1]] and lidx gives each 3x3 block the
+        position of its column node in that list.  Built once per size."""
         import numpy as np
         import torch
         lanes = int(nat.lib().fx_csr_block3_lanes(self.n, self.nse, int(value_bytes)))
-        G = 256 // lanes
+        G = int(threads) // lanes
         cache = self.__dict__.setdefault("_b3maps", {})
         if G in cache:
             return cache[G]

@@ -281,8 +281,8 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, T* __restrict
 // x_n from the other blocks' x).  Products and sums as in b3_rows.
 // PCG: skip when st->done and write the block partials of p . q; else a
 // plain q = A p.
-template <typename V, typename T, int L, bool DIFF, bool PCG>
-__global__ __launch_bounds__(kBlock) void pcg_spmv_map_kernel(int n, const int* __restrict__ indptr,
+template <typename V, typename T, int L, bool DIFF, bool PCG, int BS>
+__global__ __launch_bounds__(BS) void pcg_spmv_map_kernel(int n, const int* __restrict__ indptr,
                                                               const uint16_t* __restrict__ lidx,
                                                               const int* __restrict__ uptr,
                                                               const int* __restrict__ ucol,
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void pcg_spmv_map_kernel(int n, const int* 
   if constexpr (PCG) {
     if (st->done) return;
   }
-  constexpr int G = kBlock / L;
+  constexpr int G = BS / L;
   extern __shared__ unsigned char smem[];
   __shared__ int own[G];
   T* xs = reinterpret_cast<T*>(smem);
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void pcg_spmv_map_kernel(int n, const int* 
   const int first = blockIdx.x * G;
   {
     const int u0 = uptr[blockIdx.x], nu = uptr[blockIdx.x + 1] - u0;
-    for (int u = threadIdx.x; u < nu; u += kBlock) {
+    for (int u = threadIdx.x; u < nu; u += BS) {
       const int c = ucol[u0 + u];
       xs[3 * u] = p[3 * c];
       xs[3 * u + 1] = p[3 * c + 1];
@@ -349,7 +349,7 @@ __global__ __launch_bounds__(kBlock) void pcg_spmv_map_kernel(int n, const int* 
                  static_cast<double>(xs2) * static_cast<double>(y2);
     }
   }
-  if constexpr (PCG) store_partials<1>(block_sum<1, kBlock>(acc), blockIdx.x, partials);
+  if constexpr (PCG) store_partials<1>(block_sum<1, BS>(acc), blockIdx.x, partials);
 }
 
 // CSR iterations: three launches, the two 1-block finalizes folded into
@@ -538,22 +538,23 @@ struct CsrOp {
   static constexpr bool kThreeLaunch = true;
 };
 
-// The block-3 PCG operator with a block-column map (LDS-staged x).
-template <typename V, typename T, int L, bool DIFF>
+// The block-3 PCG operator with a block-column map (LDS-staged x); BS
+// threads per workgroup = the map's nodes per block x L.
+template <typename V, typename T, int L, bool DIFF, int BS>
 struct CsrMapOp {
   int n;
   const int* indptr;
   fx_b3map map;
   const V* data;
-  int grid() const { return grid_for(n / 3, L); }
+  int grid() const { return std::max(1, (n / 3 + BS / L - 1) / (BS / L)); }
   size_t shm() const { return 3 * sizeof(T) * static_cast<size_t>(map.max_unique); }
   void apply(hipStream_t s, const T* xv, T* yv) const {
-    hipLaunchKernelGGL((pcg_spmv_map_kernel<V, T, L, DIFF, false>), dim3(grid()), dim3(kBlock), shm(), s, n,
+    hipLaunchKernelGGL((pcg_spmv_map_kernel<V, T, L, DIFF, false, BS>), dim3(grid()), dim3(BS), shm(), s, n,
                        indptr, map.lidx, map.uptr, map.ucol, data, xv, yv, nullptr, nullptr);
   }
   void iter(hipStream_t s, const T* p, T* q, const PcgState* st, double* partials) const {
-    hipLaunchKernelGGL((pcg_spmv_map_kernel<V, T, L, DIFF, true>), dim3(grid()), dim3(kBlock), shm(), s, n, indptr,
-                       map.lidx, map.uptr, map.ucol, data, p, q, st, partials);
+    hipLaunchKernelGGL((pcg_spmv_map_kernel<V, T, L, DIFF, true, BS>), dim3(grid()), dim3(BS), shm(), s, n,
+                       indptr, map.lidx, map.uptr, map.ucol, data, p, q, st, partials);
   }
   static constexpr bool kThreeLaunch = true;
 };
@@ -765,9 +766,10 @@ int run_pcg_map(void* stream, int n, long long nse, int flags, const int* indptr
   const size_t need = pcg_ws_bytes(n, sizeof(T));
   if (!workspace || ws_bytes < need) return fail(FX_EWORKSPACE, "pcg: workspace %zu < %zu bytes", ws_bytes, need);
   const int lanes = lanes_for(n, nse, true, sizeof(V) == 8);
-  if (map->nodes_per_block != kBlock / lanes)
-    return fail(FX_EINVAL, "pcg_b3map: map built for %d nodes per block, the SpMV runs %d", map->nodes_per_block,
-                kBlock / lanes);
+  const int bs = map->nodes_per_block * lanes;
+  if (bs != 256 && bs != 512)
+    return fail(FX_EINVAL, "pcg_b3map: %d nodes per block x %d lanes is not a 256- or 512-thread workgroup",
+                map->nodes_per_block, lanes);
   if (3 * sizeof(T) * static_cast<size_t>(map->max_unique) > 64 * 1024)
     return fail(FX_ELIMIT, "pcg_b3map: %d unique columns per block exceed the LDS staging", map->max_unique);
   if (n == 0) {
@@ -777,13 +779,20 @@ int run_pcg_map(void* stream, int n, long long nse, int flags, const int* indptr
     return FX_OK;
   }
   const PcgArgs a{n, nse, flags, indptr, nullptr, rec_p, rec_pq, rec_p ? rec_max : 0};
-  if (flags & FX_PCG_DIFF3) {
-    FX_LANE_SWITCH(lanes, return pcg_loop<T>(s, a, CsrMapOp<V, T, LL, true>{n, indptr, *map, data}, dinv, b, x0,
-                                             tol, atol, maxiter, check_every, x, iters, rs, workspace));
+#define FX_MAP_LOOP(D, B)                                                                                     \
+  FX_LANE_SWITCH(lanes, return pcg_loop<T>(s, a, CsrMapOp<V, T, LL, D, B>{n, indptr, *map, data}, dinv, b, x0, tol, \
+                                           atol, maxiter, check_every, x, iters, rs, workspace))
+  const bool diff = (flags & FX_PCG_DIFF3) != 0;
+  if (diff && bs == 512) {
+    FX_MAP_LOOP(true, 512);
+  } else if (diff) {
+    FX_MAP_LOOP(true, 256);
+  } else if (bs == 512) {
+    FX_MAP_LOOP(false, 512);
   } else {
-    FX_LANE_SWITCH(lanes, return pcg_loop<T>(s, a, CsrMapOp<V, T, LL, false>{n, indptr, *map, data}, dinv, b, x0,
-                                             tol, atol, maxiter, check_every, x, iters, rs, workspace));
+    FX_MAP_LOOP(false, 256);
   }
+#undef FX_MAP_LOOP
   return FX_OK;
 }
 

@@ -118,8 +118,9 @@ def pcg(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=Non
     # 1M DOF; with FP64 values (32 lanes per node, 8 nodes per workgroup)
     # the staging measured slower, so those keep the bcol path
     want = (name == "fx_pcg_f32f64") if use_map is None else (bool(use_map) and name != "fx_pcg_f32")
-    bmap = A.b3map(A.data.element_size()) if (want and flags & (nat.FX_PCG_BLOCK3 | nat.FX_PCG_DIFF3)
-                                              and A.b3map is not None) else None
+    # 256-thread workgroups: 512 (half the SpMV partials to sum) measured slower
+    bmap = A.b3map(A.data.element_size(), 256) if (
+        want and flags & (nat.FX_PCG_BLOCK3 | nat.FX_PCG_DIFF3) and A.b3map is not None) else None
     if bmap is not None:
         nat.call(name.replace("fx_pcg_", "fx_pcg_b3map_"), nat.stream(), n, A.nse, flags, nat.ptr(A.indptr),
                  ctypes.addressof(bmap.struct), nat.ptr(A.data), nat.ptr(dinv), nat.ptr(b), nat.ptr(x0), float(tol),

@@ -192,8 +192,9 @@ int fx_pcg_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int
                   double* rec_pq, int32_t rec_max, void* workspace, size_t ws_bytes);
 
 /* Block-column map of a block-3 CSR for the SpMV of the PCG: the SpMV runs
- * nodes_per_block consecutive nodes per workgroup (256 / the lane count of
- * fx_csr_block3_lanes); the workgroup first copies x of the sorted unique
+ * nodes_per_block consecutive nodes per workgroup (256 or 512 threads
+ * divided by the lane count of fx_csr_block3_lanes: the map's size picks
+ * the workgroup); the workgroup first copies x of the sorted unique
  * column nodes of its nodes (ucol[uptr[b] .. uptr[b+1])) into LDS, then each
  * 3x3 block reads x from LDS at lidx (its column node's position in that
  * list; one entry per block, in bcol order) instead of gathering it through
@@ -210,8 +211,8 @@ typedef struct fx_b3map {
  * (value_bytes 8: FP64 values, 4: FP32). */
 int fx_csr_block3_lanes(int32_t n, int64_t nse, int32_t value_bytes);
 /* fx_pcg_f64 / fx_pcg_f32f64 on a block-3 matrix (flags FX_PCG_BLOCK3 or
- * FX_PCG_DIFF3) with its block-column map in place of bcol.  A map built
- * for another lane count is refused (FX_EINVAL). */
+ * FX_PCG_DIFF3) with its block-column map in place of bcol.  A map whose
+ * nodes_per_block x lanes is not 256 or 512 is refused (FX_EINVAL). */
 int fx_pcg_b3map_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
                      const fx_b3map* map, const double* data, const double* dinv, const double* b,
                      const double* x0, double tol, double atol, int32_t maxiter, int32_t check_every, double* x,
