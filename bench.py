@@ -495,7 +495,7 @@ def batch_bench(steps, warmup, nseg=64, seed=20261203):
         opts = fe.KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=50000, use_jacobi_preconditioner=True,
                                       precision=form)
         solver = create_batched_solver(bp, bcb, opts, traced_structure=ts)
-        for _ in range(max(warmup, 1)):      # the first call autotunes the SpMV lanes
+        for _ in range(max(warmup, 1)):      # the first call autotunes the SpMV configuration
             solver(tpd)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -505,9 +505,9 @@ def batch_bench(steps, warmup, nseg=64, seed=20261203):
         wall = (time.perf_counter() - t1) / steps
         st = solver.stats
         sols[form] = u
-        out[form] = {"s": round(wall, 5), "outer": st.outer, "lanes": st.lanes, "iters_min": min(st.iters),
+        out[form] = {"s": round(wall, 5), "outer": st.outer, "spmv": st.spmv, "iters_min": min(st.iters),
                      "iters_max": max(st.iters), "true_relres_max": max(st.relres)}
-        log(f"batch {form}: {wall:.4f} s, lanes {st.lanes}, iters {min(st.iters)}..{max(st.iters)}, "
+        log(f"batch {form}: {wall:.4f} s, spmv {st.spmv}, iters {min(st.iters)}..{max(st.iters)}, "
             f"outer {st.outer}, max true relres {max(st.relres):.2e}")
     # the same instances one after another
     opts = fe.KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=50000, use_jacobi_preconditioner=True)

@@ -24,6 +24,7 @@ FX_PCG_DIFF3 = 2
 FX_PCG_TIME = 4
 FX_PCG_AUTOTUNE = 8
 FX_PCG_NO_GRAPH = 16
+FX_PCG_SEG_FORCE = 64
 FX_GMG_F32 = 32
 FX_BAND_LDLT = 1
 FX_ASM_DIFF3 = 1
@@ -150,7 +151,7 @@ SIGNATURES = {
     "fx_hex8_jvp_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "fx_pcg_matfree_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P, _F64,
                                   _F64, _I32, _I32, _P, _P, _P, _P, _SZ]),
-    "fx_pcg_batched_workspace_bytes": (_SZ, [_I32, _P, _I32]),
+    "fx_pcg_batched_workspace_bytes": (_SZ, [_I32, _P, _I64, _I32]),
     "fx_pcg_batched_f64": (_I32, [_P, _I32, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P,
                                   _P, _P, _P, _P, _SZ]),
     "fx_pcg_batched_f32f64": (_I32, [_P, _I32, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32,

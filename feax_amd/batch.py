@@ -25,8 +25,6 @@ The same machinery batches parameter sets of one mesh (pass the same
 problem several times), the vmap use of feax.
 """
 
-import ctypes
-
 import numpy as np
 
 from . import _native as nat
@@ -166,10 +164,11 @@ class BatchedStats:
         self.iters = []            # per instance, summed over outer steps
         self.outer = 0
         self.relres = []           # per instance, FP64 ||b - K u|| / ||b||
-        self.lanes = None
+        self.lanes = None          # SpMV lanes per node of the last solve
+        self.spmv = None           # its SpMV configuration: lanes, threads, map
 
     def __repr__(self):
-        return (f"BatchedStats(outer={self.outer}, lanes={self.lanes}, iters=[{min(self.iters)}..{max(self.iters)}], "
+        return (f"BatchedStats(outer={self.outer}, spmv={self.spmv}, iters=[{min(self.iters)}..{max(self.iters)}], "
                 f"relres max {max(self.relres):.2e})")
 
 
@@ -234,11 +233,14 @@ def _seg_record_for(bp, n, capacity, device):
 
 
 def pcg_batched(bp, A, b, x0, dinv, tol, atol, maxiter, check_every=32, out=None, autotune=True, graph=True,
-                record=None):
+                record=None, spmv=None):
     """Segmented Jacobi-PCG over the instances (fx_pcg_batched_f64 /
     _f32f64; with record, an FP32-matrix solve that keeps each instance's
     search directions in it, fx_pcg_batched_record_f32f64).  Returns (x,
-    iters[nseg], rs[nseg], lanes)."""
+    iters[nseg], rs[nseg], spmv) with spmv the SpMV configuration that ran:
+    {"lanes", "threads" (per workgroup), "map" (x staged in LDS through the
+    block-column map)}.  spmv given: run that configuration (FX_PCG_SEG_FORCE)
+    instead of the default or the autotuner's."""
     import torch
     n = A.shape[0]
     dev = A.data.device
@@ -247,12 +249,16 @@ def pcg_batched(bp, A, b, x0, dinv, tol, atol, maxiter, check_every=32, out=None
     x0 = None if x0 is None else x0.to(torch.float64).contiguous()
     dinv = None if dinv is None else dinv.to(torch.float64).contiguous()
     snp = np.ascontiguousarray(bp.seg_node_ptr, dtype=np.int32)
-    need = int(nat.lib().fx_pcg_batched_workspace_bytes(bp.nseg, nat.ptr(snp), 8))
+    need = int(nat.lib().fx_pcg_batched_workspace_bytes(bp.nseg, nat.ptr(snp), int(A.nse), 8))
     ws = _WS.get(str(dev))
     if ws is None or ws.numel() < need:
         ws = torch.empty(need, dtype=torch.uint8, device=dev)
         _WS[str(dev)] = ws
     flags = (nat.FX_PCG_AUTOTUNE if autotune else 0) | (0 if graph else nat.FX_PCG_NO_GRAPH)
+    tuned = np.zeros(3, np.int32)
+    if spmv is not None:
+        flags |= nat.FX_PCG_SEG_FORCE
+        tuned[:] = (int(spmv["lanes"]), int(spmv["threads"]), int(bool(spmv["map"])))
     if A.data.dtype == torch.float64:
         name = "fx_pcg_batched_f64"
     else:
@@ -261,10 +267,9 @@ def pcg_batched(bp, A, b, x0, dinv, tol, atol, maxiter, check_every=32, out=None
             flags |= nat.FX_PCG_DIFF3
     iters = np.zeros(bp.nseg, np.int32)
     rs = np.zeros(bp.nseg, np.float64)
-    lanes = ctypes.c_int32(0)
     common = (nat.stream(), bp.nseg, nat.ptr(snp), A.nse, flags, nat.ptr(A.indptr), nat.ptr(A.bcol),
               nat.ptr(A.data), nat.ptr(dinv), nat.ptr(b), nat.ptr(x0), float(tol), float(atol), int(maxiter),
-              int(check_every), nat.ptr(x), nat.ptr(iters), nat.ptr(rs), ctypes.addressof(lanes))
+              int(check_every), nat.ptr(x), nat.ptr(iters), nat.ptr(rs), nat.ptr(tuned))
     if record is not None:
         if A.data.dtype != torch.float32:
             raise ValueError("recording is for the FP32-matrix batched solve")
@@ -273,16 +278,18 @@ def pcg_batched(bp, A, b, x0, dinv, tol, atol, maxiter, check_every=32, out=None
         record.counts = np.minimum(iters, record.capacity).astype(np.int32)
     else:
         nat.call(name, *common, nat.ptr(ws), ws.numel())
-    return x, iters, rs, lanes.value
+    return x, iters, rs, {"lanes": int(tuned[0]), "threads": int(tuned[1]), "map": bool(tuned[2])}
 
 
-def create_batched_solver(bp, bc, solver_options=None, traced_structure=None, autotune=True, graph=True):
+def create_batched_solver(bp, bc, solver_options=None, traced_structure=None, autotune=True, graph=True,
+                          spmv=None):
     """solver(traced_params, initial_guess=None) -> concatenated u (FP64).
 
     precision "fp64": FP64 assembly of all instances in one launch, one
     segmented Jacobi-PCG.  precision "mixed": FP32 difference-form matrix,
     FP64 vectors, and an FP64 defect correction whose per-instance residual
-    norms decide which instances take another inner solve."""
+    norms decide which instances take another inner solve.  spmv: a fixed
+    SpMV configuration for pcg_batched (default: autotuned when autotune)."""
     import torch
     from .assembler import TracedStructure, assemble_csr, residual
     from .solvers.common import create_x0, inner_tolerance
@@ -300,10 +307,11 @@ def create_batched_solver(bp, bc, solver_options=None, traced_structure=None, au
             out = assemble_csr(bp, ts, traced_params, u0, bc, True, torch.float64, jacobi_shift=shift)
             A, dinv = out if shift is not None else (out, None)
             b = residual(bp, ts, traced_params, u0, bc, negate=True)
-            du, iters, rs, lanes = pcg_batched(bp, A, b, x0, dinv, opts.tol, opts.atol, opts.maxiter,
-                                               opts.check_every, autotune=autotune, graph=graph)
+            du, iters, rs, ran = pcg_batched(bp, A, b, x0, dinv, opts.tol, opts.atol, opts.maxiter,
+                                              opts.check_every, autotune=autotune, graph=graph, spmv=spmv)
             u = u0 + du
-            stats.iters, stats.outer, stats.lanes = [int(i) for i in iters], 1, lanes
+            stats.iters, stats.outer, stats.spmv = [int(i) for i in iters], 1, ran
+            stats.lanes = ran["lanes"]
             bn = _seg_norms(bp, ts, b)
             rn = _seg_norms(bp, ts, residual(bp, ts, traced_params, u, bc, negate=True))
             stats.relres = [float(r / bb) if bb > 0 else 0.0 for r, bb in zip(rn, bn)]
@@ -329,15 +337,15 @@ def create_batched_solver(bp, bc, solver_options=None, traced_structure=None, au
             # one tolerance for all instances: the strictest need among the active ones
             tin = min(inner_tolerance(opts, t_, r_) for t_, r_ in zip(target[act], rn[act]))
             x0i = record.project(rhs) if (record is not None and outer > 0) else None
-            _, iters, _, lanes = pcg_batched(bp, A, rhs, x0i, dinv, tin, 0.0, opts.maxiter, opts.check_every,
-                                             out=d, autotune=autotune, graph=graph,
-                                             record=record if outer == 0 else None)
+            _, iters, _, ran = pcg_batched(bp, A, rhs, x0i, dinv, tin, 0.0, opts.maxiter, opts.check_every,
+                                           out=d, autotune=autotune, graph=graph, spmv=spmv,
+                                           record=record if outer == 0 else None)
             u += d
             residual(bp, ts, traced_params, u, bc, negate=True, out=r)
             rn = _seg_norms(bp, ts, r)
             total += iters
             outer += 1
-            stats.lanes = lanes
+            stats.spmv, stats.lanes = ran, ran["lanes"]
         stats.iters, stats.outer = [int(i) for i in total], outer
         stats.relres = [float(a / bb) if bb > 0 else 0.0 for a, bb in zip(rn, bn)]
         return u

@@ -9,27 +9,41 @@
 // layout, block-3 form) and every CG scalar is per instance:
 //
 //  * every block of every kernel belongs to exactly one instance (block
-//    tables built on the host), so its partial sums belong to that instance
-//    and the finalize kernels run one block per instance;
-//  * each instance has its own alpha, beta, gamma, stopping test
-//    (jax cg semantics) and done flag; converged instances' blocks exit at
-//    their first instruction while the others iterate;
-//  * a chunk of check_every iterations is captured once as a HIP graph and
-//    replayed on the caller's stream until all instances are done (one host
-//    read of the per-instance flags per chunk);
-//  * FX_PCG_AUTOTUNE times the SpMV lane counts 8 / 16 / 32 on the
-//    instance set and keeps the fastest per problem shape;
+//    tables built on the host), so its partial sums belong to that instance;
+//  * an iteration is three launches: q = A p with the p.q partials; alpha
+//    and the x, r, z updates; beta and the new direction.  Every block of
+//    the two vector kernels sums its own instance's partials of the
+//    previous kernel (same fixed order in every block, so all blocks of an
+//    instance hold the same scalar); the instance's first vector block
+//    advances its state.  gamma alternates between g[0] and g[1] by the
+//    parity of the iteration, as in the single-system driver;
+//  * each instance has its own alpha, beta, gamma, stopping test (jax cg
+//    semantics) and done flag; converged instances' blocks exit at their
+//    first instruction while the others iterate;
+//  * a chunk of check_every (even) iterations is captured once as a HIP
+//    graph and replayed on the caller's stream until all instances are done
+//    (one host read of the per-instance flags per chunk);
+//  * the SpMV runs L lanes per node and BS threads per workgroup (BS / L
+//    consecutive nodes of one instance per workgroup), reading x either
+//    through bcol or, with a block-column map built on the device for that
+//    partition, from an LDS copy of the workgroup's unique column nodes;
+//    FX_PCG_AUTOTUNE times lanes 8 / 16 / 32 x workgroups of 256 / 512
+//    threads x map on / off on the instance set and keeps the fastest per
+//    problem shape;
 //  * optionally each instance's search directions p_k (FP32) and p_k.A p_k
 //    are recorded at its own iteration count k, and fx_seg_project_f32
 //    forms the per-instance Galerkin projection onto them (Krylov recycling
 //    for the later inner solves of a mixed-precision defect correction).
 // FP64 partial sums in a fixed order: bitwise repeatable, with or without
-// the graph.
+// the graph, and independent of the SpMV configuration except through the
+// order of the p.q partials (one per SpMV workgroup).
 #include <algorithm>
+#include <climits>
 #include <cstdint>
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -40,34 +54,39 @@
 namespace fx {
 namespace {
 
-constexpr int kSegBlock = 256;  // threads per block
+constexpr int kSegBlock = 256;  // threads per vector-kernel block
 constexpr int kVecRows = 4096;  // rows per vector-kernel block
+constexpr int kMapWords = 2048; // column bitmap of the map build: 65536 nodes
+constexpr int kMapLds = 62 * 1024;  // LDS staging limit of the map SpMV (bytes; 64 KiB with the static LDS)
 
 struct BlockRange {
   int seg, begin, end;  // instance, [begin, end) of nodes (SpMV) or rows (vector kernels)
 };
 
-// Sums of cnt partials starting at p[0..NV) (one finalize block, one
-// block reduction for all NV quantities); valid in thread 0.
-template <int NV>
+// Fixed-order sums of cnt partials starting at p[k] (one block of BS
+// threads, one block reduction for all NV quantities); valid in thread 0.
+template <int NV, int BS>
 __device__ __forceinline__ DVec<NV> seg_sums(const double* const (&p)[NV], int cnt) {
   DVec<NV> acc;
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc.v[k] = 0.0;
-  for (int i = threadIdx.x; i < cnt; i += kFinBlock) {
+  for (int i = threadIdx.x; i < cnt; i += BS) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc.v[k] += p[k][i];
   }
-  return block_sum<NV, kFinBlock>(acc);
+  return block_sum<NV, BS>(acc);
 }
 
-template <typename V, typename T, int L, bool DIFF, bool DOT>
-__global__ __launch_bounds__(kSegBlock) void seg_spmv_kernel(const BlockRange* __restrict__ tab,
-                                                             const int* __restrict__ indptr,
-                                                             const int* __restrict__ bcol, const V* __restrict__ data,
-                                                             const T* __restrict__ x, T* __restrict__ y,
-                                                             const PcgState* __restrict__ st, double* partials) {
-  const BlockRange br = tab[blockIdx.x];
+// q = A p over the nodes of one table block (BS / L nodes, L lanes each).
+// DOT: skip a converged instance and write the block partial of p . q.
+template <typename V, typename T, int L, bool DIFF, bool DOT, int BS>
+__global__ __launch_bounds__(BS) void seg_spmv_kernel(const BlockRange* __restrict__ tab,
+                                                      const int* __restrict__ indptr, const int* __restrict__ bcol,
+                                                      const V* __restrict__ data, const T* __restrict__ x,
+                                                      T* __restrict__ y, const PcgState* __restrict__ st,
+                                                      double* partials) {
+  const int lb = blockIdx.x;
+  const BlockRange br = tab[lb];
   if constexpr (DOT) {
     if (st[br.seg].done) return;  // uniform over the block
   }
@@ -89,8 +108,150 @@ __global__ __launch_bounds__(kSegBlock) void seg_spmv_kernel(const BlockRange* _
     }
   }
   if constexpr (DOT) {
-    const DVec<1> t = block_sum<1, kSegBlock>(acc);
-    if (threadIdx.x == 0) partials[blockIdx.x] = t.v[0];
+    const DVec<1> t = block_sum<1, BS>(acc);
+    if (threadIdx.x == 0) partials[lb] = t.v[0];
+  }
+}
+
+// The same through the block-column map of the table: block b's unique
+// column nodes are ucol[e0 .. e0 + ucnt[b]) with e0 = its first 3x3 block
+// (indptr[3 begin] / 9; a block has at least as many 3x3 blocks as unique
+// columns, so the lists of the blocks never overlap).
+template <typename V, typename T, int L, bool DIFF, bool DOT, int BS>
+__global__ __launch_bounds__(BS) void seg_spmv_map_kernel(const BlockRange* __restrict__ tab,
+                                                          const int* __restrict__ indptr,
+                                                          const uint16_t* __restrict__ lidx,
+                                                          const int* __restrict__ ucnt, const int* __restrict__ ucol,
+                                                          const V* __restrict__ data, const T* __restrict__ x,
+                                                          T* __restrict__ y, const PcgState* __restrict__ st,
+                                                          double* partials) {
+  const int lb = blockIdx.x;
+  const BlockRange br = tab[lb];
+  if constexpr (DOT) {
+    if (st[br.seg].done) return;
+  }
+  extern __shared__ unsigned char smem[];
+  __shared__ int own[BS / L];
+  const int e0 = indptr[3 * br.begin] / 9;
+  DVec<1> acc;
+  acc.v[0] = b3map_rows<V, T, L, DIFF, BS>(br.begin, br.end, indptr, lidx, ucol + e0, ucnt[lb], data, x, y,
+                                           reinterpret_cast<T*>(smem), own);
+  if constexpr (DOT) {
+    const DVec<1> t = block_sum<1, BS>(acc);
+    if (threadIdx.x == 0) partials[lb] = t.v[0];
+  }
+}
+
+// Exclusive prefix sum over the kSegBlock threads of a block; *total = sum.
+__device__ __forceinline__ int block_excl_scan(int v, int* total) {
+  __shared__ int wsum[kSegBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  int base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kSegBlock / 64; ++w) {
+    base += w < wid ? wsum[w] : 0;
+    tot += wsum[w];
+  }
+  *total = tot;
+  return base + x - v;
+}
+
+// Block-column map of an SpMV table, one workgroup per table block: the
+// block's column nodes are marked in an LDS bitmap over [cmin, cmax]; the
+// rank of a column among the set bits is its position in the sorted unique
+// list.  Writes lidx for every 3x3 block of the block's nodes, the list to
+// ucol[e0 ..] and its length to ucnt[b]; info[0] = max list length,
+// info[1] = 1 when some block's column span exceeds the bitmap (the map is
+// then unusable).
+__global__ __launch_bounds__(kSegBlock) void seg_map_build_kernel(const BlockRange* __restrict__ tab,
+                                                                  const int* __restrict__ indptr,
+                                                                  const int* __restrict__ bcol,
+                                                                  uint16_t* __restrict__ lidx,
+                                                                  int* __restrict__ ucnt, int* __restrict__ ucol,
+                                                                  int* info) {
+  __shared__ unsigned bits[kMapWords];
+  __shared__ int wpre[kMapWords];
+  __shared__ int rmin[kSegBlock / 64], rmax[kSegBlock / 64];
+  const BlockRange br = tab[blockIdx.x];
+  const int e0 = indptr[3 * br.begin] / 9, e1 = indptr[3 * br.end] / 9;
+  int cmin = INT_MAX, cmax = INT_MIN;
+  for (int e = e0 + threadIdx.x; e < e1; e += kSegBlock) {
+    const int c = bcol[e];
+    cmin = min(cmin, c);
+    cmax = max(cmax, c);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    cmin = min(cmin, __shfl_xor(cmin, off, 64));
+    cmax = max(cmax, __shfl_xor(cmax, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    rmin[threadIdx.x >> 6] = cmin;
+    rmax[threadIdx.x >> 6] = cmax;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < kSegBlock / 64; ++w) {
+    cmin = min(cmin, rmin[w]);
+    cmax = max(cmax, rmax[w]);
+  }
+  if (e1 <= e0) {
+    if (threadIdx.x == 0) ucnt[blockIdx.x] = 0;
+    return;
+  }
+  const long span = static_cast<long>(cmax) - cmin + 1;
+  if (span > 32L * kMapWords) {
+    if (threadIdx.x == 0) {
+      ucnt[blockIdx.x] = 0;
+      atomicOr(info + 1, 1);
+    }
+    return;
+  }
+  const int nw = static_cast<int>((span + 31) / 32);
+  for (int w = threadIdx.x; w < nw; w += kSegBlock) bits[w] = 0u;
+  __syncthreads();
+  for (int e = e0 + threadIdx.x; e < e1; e += kSegBlock) {
+    const int d = bcol[e] - cmin;
+    atomicOr(&bits[d >> 5], 1u << (d & 31));
+  }
+  __syncthreads();
+  // each thread owns a run of consecutive words
+  const int per = (nw + kSegBlock - 1) / kSegBlock;
+  const int w0 = min(nw, threadIdx.x * per), w1 = min(nw, w0 + per);
+  int mine = 0;
+  for (int w = w0; w < w1; ++w) mine += __popc(bits[w]);
+  int total;
+  int base = block_excl_scan(mine, &total);
+  for (int w = w0; w < w1; ++w) {
+    wpre[w] = base;
+    const unsigned word = bits[w];
+    unsigned rest = word;
+    int pos = base;
+    while (rest) {
+      const int bit = __ffs(rest) - 1;
+      ucol[e0 + pos++] = cmin + 32 * w + bit;
+      rest &= rest - 1u;
+    }
+    base += __popc(word);
+  }
+  __syncthreads();
+  for (int e = e0 + threadIdx.x; e < e1; e += kSegBlock) {
+    const int d = bcol[e] - cmin;
+    const int w = d >> 5;
+    const unsigned below = (1u << (d & 31)) - 1u;
+    lidx[e] = static_cast<uint16_t>(wpre[w] + __popc(bits[w] & below));
+  }
+  if (threadIdx.x == 0) {
+    ucnt[blockIdx.x] = total;
+    atomicMax(info, total);
   }
 }
 
@@ -102,36 +263,40 @@ struct SegRecord {
   int rec_max, n, nseg;
 };
 
-__global__ __launch_bounds__(kFinBlock) void seg_fin_alpha(PcgState* st, const int* __restrict__ mv_ptr,
-                                                           const double* __restrict__ partials, SegRecord rec) {
-  const int s = blockIdx.x;
-  if (st[s].done) return;
-  const double* src[1] = {partials + mv_ptr[s]};
-  const DVec<1> pq = seg_sums<1>(src, mv_ptr[s + 1] - mv_ptr[s]);
-  if (threadIdx.x == 0) {
-    st[s].alpha = st[s].gamma / pq.v[0];
-    if (rec.pq && st[s].k < rec.rec_max) rec.pq[static_cast<size_t>(st[s].k) * rec.nseg + s] = pq.v[0];
-  }
-}
-
-// x += alpha p; r -= alpha q; z = M r; partials of r.z and r.r.
+// alpha = g[par] / (p.q) of the block's instance (its SpMV partials
+// mv_ptr[s] .. mv_ptr[s + 1]); x += alpha p; r -= alpha q; z = M r;
+// partials of r.z and r.r.
 template <typename T>
-__global__ __launch_bounds__(kSegBlock) void seg_update_kernel(const BlockRange* __restrict__ vtab, T* __restrict__ x,
-                                                               T* __restrict__ r, T* __restrict__ z,
-                                                               const T* __restrict__ p, const T* __restrict__ q,
-                                                               const T* __restrict__ dinv,
-                                                               const PcgState* __restrict__ st, double* prz,
-                                                               double* prr, SegRecord rec) {
+__global__ __launch_bounds__(kSegBlock) void seg_update_alpha_kernel(
+    const BlockRange* __restrict__ vtab, int par, const int* __restrict__ mv_ptr, const int* __restrict__ v_ptr,
+    const double* __restrict__ pmv, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
+    const T* __restrict__ q, const T* __restrict__ dinv, PcgState* __restrict__ st, double* prz, double* prr,
+    SegRecord rec) {
   const BlockRange br = vtab[blockIdx.x];
-  if (st[br.seg].done) return;
-  const T alpha = static_cast<T>(st[br.seg].alpha);
-  const int k = st[br.seg].k;
+  const int s = br.seg;
+  if (st[s].done) return;
+  __shared__ double s_alpha;
+  {
+    const double* src[1] = {pmv + mv_ptr[s]};
+    const DVec<1> pq = seg_sums<1, kSegBlock>(src, mv_ptr[s + 1] - mv_ptr[s]);
+    if (threadIdx.x == 0) {
+      s_alpha = st[s].g[par] / pq.v[0];
+      if (static_cast<int>(blockIdx.x) == v_ptr[s]) {
+        st[s].alpha = s_alpha;
+        if (rec.pq && st[s].k < rec.rec_max) rec.pq[static_cast<size_t>(st[s].k) * rec.nseg + s] = pq.v[0];
+      }
+    }
+  }
+  __syncthreads();
+  const T alpha = static_cast<T>(s_alpha);
+  const int k = st[s].k;
   float* rp = (rec.P && k < rec.rec_max) ? rec.P + static_cast<size_t>(k) * rec.n : nullptr;
   DVec<2> acc;
   acc.v[0] = acc.v[1] = 0.0;
   for (int i = br.begin + threadIdx.x; i < br.end; i += kSegBlock) {
-    if (rp) rp[i] = static_cast<float>(p[i]);
-    x[i] += alpha * p[i];
+    const T pi = p[i];
+    if (rp) rp[i] = static_cast<float>(pi);
+    x[i] += alpha * pi;
     const T ri = r[i] - alpha * q[i];
     r[i] = ri;
     const T zi = dinv ? dinv[i] * ri : ri;
@@ -146,32 +311,41 @@ __global__ __launch_bounds__(kSegBlock) void seg_update_kernel(const BlockRange*
   }
 }
 
-__global__ __launch_bounds__(kFinBlock) void seg_fin_beta(PcgState* st, const int* __restrict__ v_ptr,
-                                                          const double* __restrict__ prz,
-                                                          const double* __restrict__ prr) {
-  const int s = blockIdx.x;
-  if (st[s].done) return;
-  const int b0 = v_ptr[s], cnt = v_ptr[s + 1] - v_ptr[s];
-  const double* src[2] = {prz + b0, prr + b0};
-  const DVec<2> t = seg_sums<2>(src, cnt);
-  if (threadIdx.x == 0) {
-    PcgState& S = st[s];
-    const double g_new = t.v[0];
-    S.beta = g_new / S.gamma;
-    S.gamma = g_new;
-    S.rs = t.v[1];
-    S.k += 1;
-    S.done = !(S.rs > S.atol2 && S.k < S.maxiter);
-  }
-}
-
+// gamma' = r.z of the block's instance, beta = gamma' / g[par], p = z +
+// beta p; the instance's first block advances its state (g[1 - par], k,
+// rs, stop flag).  A block that starts after that one raised the stop flag
+// skips the (then unused) direction update.
 template <typename T>
-__global__ __launch_bounds__(kSegBlock) void seg_dir_kernel(const BlockRange* __restrict__ vtab, T* __restrict__ p,
-                                                            const T* __restrict__ z,
-                                                            const PcgState* __restrict__ st) {
+__global__ __launch_bounds__(kSegBlock) void seg_dir_beta_kernel(const BlockRange* __restrict__ vtab, int par,
+                                                                 const int* __restrict__ v_ptr,
+                                                                 const double* __restrict__ prz,
+                                                                 const double* __restrict__ prr, T* __restrict__ p,
+                                                                 const T* __restrict__ z, PcgState* __restrict__ st) {
   const BlockRange br = vtab[blockIdx.x];
-  if (st[br.seg].done) return;
-  const T beta = static_cast<T>(st[br.seg].beta);
+  const int s = br.seg;
+  if (st[s].done) return;
+  __shared__ double s_beta;
+  {
+    const int b0 = v_ptr[s], cnt = v_ptr[s + 1] - v_ptr[s];
+    const double* src[2] = {prz + b0, prr + b0};
+    const DVec<2> t = seg_sums<2, kSegBlock>(src, cnt);
+    if (threadIdx.x == 0) {
+      const double g_new = t.v[0];
+      s_beta = g_new / st[s].g[par];
+      if (static_cast<int>(blockIdx.x) == b0) {
+        PcgState& S = st[s];
+        const int k = S.k + 1;
+        S.g[1 - par] = g_new;
+        S.gamma = g_new;
+        S.beta = s_beta;
+        S.rs = t.v[1];
+        S.k = k;
+        S.done = !(t.v[1] > S.atol2 && k < S.maxiter);
+      }
+    }
+  }
+  __syncthreads();
+  const T beta = static_cast<T>(s_beta);
   for (int i = br.begin + threadIdx.x; i < br.end; i += kSegBlock) p[i] = z[i] + beta * p[i];
 }
 
@@ -212,11 +386,13 @@ __global__ __launch_bounds__(kFinBlock) void seg_fin_init(PcgState* st, const in
   const int s = blockIdx.x;
   const int b0 = v_ptr[s], cnt = v_ptr[s + 1] - v_ptr[s];
   const double* src[3] = {prz + b0, prr + b0, pbb + b0};
-  const DVec<3> t = seg_sums<3>(src, cnt);
+  const DVec<3> t = seg_sums<3, kFinBlock>(src, cnt);
   if (threadIdx.x == 0) {
     PcgState& S = st[s];
     S.alpha = S.beta = 0.0;
     S.gamma = t.v[0];
+    S.g[0] = t.v[0];
+    S.g[1] = 0.0;
     S.rs = t.v[1];
     S.bb = t.v[2];
     S.atol2 = fmax(tol * tol * S.bb, atol * atol);
@@ -309,11 +485,13 @@ inline size_t al256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
 struct SegLayout {
   int nseg = 0, n = 0, nmv = 0, nv = 0;  // instances, rows, SpMV blocks (worst case), vector blocks
+  long long nb3 = 0;                     // 3x3 blocks of the matrix (nse / 9)
   size_t off_st = 0, off_mvtab = 0, off_mvptr = 0, off_vtab = 0, off_vptr = 0, off_rowptr = 0, off_pmv = 0,
-         off_pv = 0, off_vec = 0, vec_bytes = 0, total = 0;
+         off_pv = 0, off_lidx = 0, off_ucol = 0, off_ucnt = 0, off_info = 0, off_vec = 0, vec_bytes = 0, total = 0;
 };
 
-// Worst-case SpMV block count over the lane counts used (L = 32: 8 nodes per block).
+// Worst-case SpMV block count over the configurations used (at least 8
+// nodes per block: 32 lanes x 256 threads).
 inline int mv_blocks_cap(const int* seg_node_ptr, int nseg) {
   long total = 0;
   for (int s = 0; s < nseg; ++s) total += (seg_node_ptr[s + 1] - seg_node_ptr[s] + 7) / 8;
@@ -326,12 +504,13 @@ inline int v_blocks(const int* seg_node_ptr, int nseg) {
   return static_cast<int>(std::max<long>(total, 1));
 }
 
-inline SegLayout seg_layout(int nseg, const int* seg_node_ptr, size_t elem) {
+inline SegLayout seg_layout(int nseg, const int* seg_node_ptr, long long nse, size_t elem) {
   SegLayout L;
   L.nseg = nseg;
   L.n = 3 * seg_node_ptr[nseg];
   L.nmv = mv_blocks_cap(seg_node_ptr, nseg);
   L.nv = v_blocks(seg_node_ptr, nseg);
+  L.nb3 = std::max<long long>(nse / 9, 1);
   size_t o = 0;
   L.off_st = o;
   o += al256(static_cast<size_t>(nseg) * sizeof(PcgState));
@@ -349,6 +528,14 @@ inline SegLayout seg_layout(int nseg, const int* seg_node_ptr, size_t elem) {
   o += al256(static_cast<size_t>(L.nmv) * sizeof(double));
   L.off_pv = o;
   o += al256(3 * static_cast<size_t>(L.nv) * sizeof(double));
+  L.off_lidx = o;
+  o += al256(static_cast<size_t>(L.nb3) * sizeof(uint16_t));
+  L.off_ucol = o;
+  o += al256(static_cast<size_t>(L.nb3) * sizeof(int));
+  L.off_ucnt = o;
+  o += al256(static_cast<size_t>(L.nmv) * sizeof(int));
+  L.off_info = o;
+  o += al256(2 * sizeof(int));
   L.off_vec = o;
   L.vec_bytes = al256(static_cast<size_t>(L.n) * elem);
   o += 4 * L.vec_bytes;
@@ -356,10 +543,8 @@ inline SegLayout seg_layout(int nseg, const int* seg_node_ptr, size_t elem) {
   return L;
 }
 
-// SpMV block table for L lanes per node (kSegBlock / L nodes per block).
-inline void mv_table(int nseg, const int* seg_node_ptr, int lanes, std::vector<BlockRange>& tab,
-                     std::vector<int>& ptr) {
-  const int gpb = kSegBlock / lanes;
+// SpMV block table: gpb consecutive nodes of one instance per block.
+inline void mv_table(int nseg, const int* seg_node_ptr, int gpb, std::vector<BlockRange>& tab, std::vector<int>& ptr) {
   tab.clear();
   ptr.assign(nseg + 1, 0);
   for (int s = 0; s < nseg; ++s) {
@@ -385,21 +570,82 @@ inline void v_table(int nseg, const int* seg_node_ptr, std::vector<BlockRange>& 
   rowptr[nseg] = 3 * seg_node_ptr[nseg];
 }
 
-template <typename V, typename T, int L, bool DIFF>
+// SpMV configuration: lanes per node, threads per workgroup, map on / off.
+struct SegCfg {
+  int lanes, bs;
+  bool map;
+};
+
+// Block-column map storage (in the workspace) and the built map.
+struct SegMap {
+  uint16_t* lidx;
+  int *ucol, *ucnt, *info;
+  int max_unique;  // longest unique-column list; -1: unusable
+};
+
+// Builds the map of the SpMV table dtab (nb blocks, already on the device)
+// and reads back whether it is usable: every block's column span within
+// the bitmap and its LDS copy (3 x max_unique values of T) within kMapLds.
+template <typename T>
+int build_seg_map(hipStream_t s, const BlockRange* dtab, int nb, const int* indptr, const int* bcol, SegMap& m) {
+  FX_HIP(hipMemsetAsync(m.info, 0, 2 * sizeof(int), s));
+  hipLaunchKernelGGL(seg_map_build_kernel, dim3(nb), dim3(kSegBlock), 0, s, dtab, indptr, bcol, m.lidx, m.ucnt,
+                     m.ucol, m.info);
+  FX_LAUNCHED("seg_map_build_kernel");
+  int h[2] = {0, 0};
+  FX_HIP(hipMemcpyAsync(h, m.info, sizeof h, hipMemcpyDeviceToHost, s));
+  FX_HIP(hipStreamSynchronize(s));
+  m.max_unique = (h[1] || 3 * sizeof(T) * static_cast<size_t>(h[0]) > static_cast<size_t>(kMapLds)) ? -1 : h[0];
+  return FX_OK;
+}
+
+template <typename V, typename T, int L, bool DIFF, int BS, bool MAP>
 struct SegOp {
   const BlockRange* tab;
   int nblocks;
   const int *indptr, *bcol;
   const V* data;
-  void apply(hipStream_t s, const T* x, T* y) const {
-    hipLaunchKernelGGL((seg_spmv_kernel<V, T, L, DIFF, false>), dim3(nblocks), dim3(kSegBlock), 0, s, tab, indptr,
-                       bcol, data, x, y, nullptr, nullptr);
+  SegMap map;
+  size_t shm() const { return MAP ? 3 * sizeof(T) * static_cast<size_t>(map.max_unique) : 0; }
+  template <bool DOT>
+  void launch(hipStream_t s, const T* x, T* y, const PcgState* st, double* partials) const {
+    if constexpr (MAP)
+      hipLaunchKernelGGL((seg_spmv_map_kernel<V, T, L, DIFF, DOT, BS>), dim3(nblocks), dim3(BS), shm(), s, tab,
+                         indptr, map.lidx, map.ucnt, map.ucol, data, x, y, st, partials);
+    else
+      hipLaunchKernelGGL((seg_spmv_kernel<V, T, L, DIFF, DOT, BS>), dim3(nblocks), dim3(BS), 0, s, tab, indptr, bcol,
+                         data, x, y, st, partials);
   }
+  void apply(hipStream_t s, const T* x, T* y) const { launch<false>(s, x, y, nullptr, nullptr); }
   void iter(hipStream_t s, const T* p, T* q, const PcgState* st, double* partials) const {
-    hipLaunchKernelGGL((seg_spmv_kernel<V, T, L, DIFF, true>), dim3(nblocks), dim3(kSegBlock), 0, s, tab, indptr,
-                       bcol, data, p, q, st, partials);
+    launch<true>(s, p, q, st, partials);
   }
 };
+
+template <int K>
+using IntC = std::integral_constant<int, K>;
+template <bool K>
+using BoolC = std::integral_constant<bool, K>;
+
+// f(IntC<L>, IntC<BS>, BoolC<MAP>) for the configuration c.
+template <typename F>
+int with_cfg(const SegCfg& c, F&& f) {
+  auto by_bs = [&](auto lc) -> int {
+    if (c.bs == 512) return c.map ? f(lc, IntC<512>{}, BoolC<true>{}) : f(lc, IntC<512>{}, BoolC<false>{});
+    if (c.bs == 256) return c.map ? f(lc, IntC<256>{}, BoolC<true>{}) : f(lc, IntC<256>{}, BoolC<false>{});
+    return fail(FX_EINVAL, "pcg_batched: %d threads per workgroup", c.bs);
+  };
+  switch (c.lanes) {
+    case 8:
+      return by_bs(IntC<8>{});
+    case 16:
+      return by_bs(IntC<16>{});
+    case 32:
+      return by_bs(IntC<32>{});
+    default:
+      return fail(FX_EINVAL, "pcg_batched: lane count %d", c.lanes);
+  }
+}
 
 struct SegBuffers {
   PcgState* st;
@@ -410,20 +656,22 @@ struct SegBuffers {
   SegRecord rec;
 };
 
+// Iteration `it` (from 0; its parity selects the gamma slot).
 template <typename T, typename Op>
-void seg_iteration(hipStream_t s, const Op& op, const SegBuffers& B, const T* dinv, T* x, T* r, T* z, T* p, T* q) {
+void seg_iteration(hipStream_t s, const Op& op, const SegBuffers& B, int it, const T* dinv, T* x, T* r, T* z, T* p,
+                   T* q) {
+  const int par = it & 1;
   op.iter(s, p, q, B.st, B.pmv);
-  hipLaunchKernelGGL(seg_fin_alpha, dim3(B.nseg), dim3(kFinBlock), 0, s, B.st, B.mvptr, B.pmv, B.rec);
-  hipLaunchKernelGGL((seg_update_kernel<T>), dim3(B.nv), dim3(kSegBlock), 0, s, B.vtab, x, r, z, p, q, dinv, B.st,
-                     B.prz, B.prr, B.rec);
-  hipLaunchKernelGGL(seg_fin_beta, dim3(B.nseg), dim3(kFinBlock), 0, s, B.st, B.vptr, B.prz, B.prr);
-  hipLaunchKernelGGL((seg_dir_kernel<T>), dim3(B.nv), dim3(kSegBlock), 0, s, B.vtab, p, z, B.st);
+  hipLaunchKernelGGL((seg_update_alpha_kernel<T>), dim3(B.nv), dim3(kSegBlock), 0, s, B.vtab, par, B.mvptr, B.vptr,
+                     B.pmv, x, r, z, p, q, dinv, B.st, B.prz, B.prr, B.rec);
+  hipLaunchKernelGGL((seg_dir_beta_kernel<T>), dim3(B.nv), dim3(kSegBlock), 0, s, B.vtab, par, B.vptr, B.prz, B.prr,
+                     p, z, B.st);
 }
 
 template <typename T, typename Op>
 int seg_pcg_loop(hipStream_t s, const Op& op, const SegBuffers& B, const T* dinv, const T* b, const T* x0, T* x,
                  T* r, T* z, T* p, T* q, int n, double tol, double atol, int maxiter, int check_every, bool graph,
-                 int* iters, double* rs, int* chunks_out) {
+                 int* iters, double* rs) {
   if (x0) {
     if (x0 != x) FX_HIP(hipMemcpyAsync(x, x0, static_cast<size_t>(n) * sizeof(T), hipMemcpyDeviceToDevice, s));
   } else {
@@ -436,6 +684,7 @@ int seg_pcg_loop(hipStream_t s, const Op& op, const SegBuffers& B, const T* dinv
                      maxiter);
   FX_LAUNCHED("seg_pcg_init");
   if (check_every <= 0) check_every = 32;
+  if (check_every & 1) ++check_every;  // every replay starts at an even iteration
   std::vector<PcgState> h(B.nseg);
   auto all_done = [&]() {
     for (const auto& S : h)
@@ -452,14 +701,14 @@ int seg_pcg_loop(hipStream_t s, const Op& op, const SegBuffers& B, const T* dinv
     hipError_t e = hipStreamCreateWithFlags(&gg.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamBeginCapture(gg.stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      for (int c = 0; c < check_every; ++c) seg_iteration<T, Op>(gg.stream, op, B, dinv, x, r, z, p, q);
+      for (int c = 0; c < check_every; ++c) seg_iteration<T, Op>(gg.stream, op, B, c, dinv, x, r, z, p, q);
       e = hipStreamEndCapture(gg.stream, &gg.graph);
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0);
     if (e != hipSuccess) rc = hip_fail(e, "seg_pcg graph capture");
   }
   const hipGraphExec_t ge = gg.exec;
-  int launched = 0, chunks = 0;
+  int launched = 0;
   while (rc == FX_OK && !all_done() && launched < maxiter) {
     if (ge) {
       hipError_t e = hipGraphLaunch(ge, s);
@@ -468,7 +717,7 @@ int seg_pcg_loop(hipStream_t s, const Op& op, const SegBuffers& B, const T* dinv
         break;
       }
     } else {
-      for (int c = 0; c < check_every; ++c) seg_iteration<T, Op>(s, op, B, dinv, x, r, z, p, q);
+      for (int c = 0; c < check_every; ++c) seg_iteration<T, Op>(s, op, B, launched + c, dinv, x, r, z, p, q);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) {
         rc = hip_fail(e, "seg_pcg_iter");
@@ -476,7 +725,6 @@ int seg_pcg_loop(hipStream_t s, const Op& op, const SegBuffers& B, const T* dinv
       }
     }
     launched += check_every;
-    ++chunks;
     hipError_t e = hipMemcpyAsync(h.data(), B.st, B.nseg * sizeof(PcgState), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) rc = hip_fail(e, "seg_pcg flags");
@@ -486,23 +734,26 @@ int seg_pcg_loop(hipStream_t s, const Op& op, const SegBuffers& B, const T* dinv
     iters[i] = h[i].k;
     rs[i] = h[i].rs;
   }
-  if (chunks_out) *chunks_out = chunks;
   return FX_OK;
 }
 
-// Autotune cache: problem shape -> lanes.
+// Autotune cache: problem shape -> SpMV configuration.
 std::mutex g_tune_mu;
-std::map<std::tuple<int, long long, int, int, int>, int> g_tune;
+std::map<std::tuple<int, long long, int, int, int>, SegCfg> g_tune;
 
+// Times q = A x (5 launches after a warm-up) for lanes 8 / 16 / 32 x
+// workgroups of 256 / 512 threads x map off / on (a map is built for each
+// partition; candidates whose map is unusable are skipped) and returns the
+// fastest.  Uses the table and map storage of the workspace.
 template <typename V, typename T, bool DIFF>
-int tune_lanes(hipStream_t s, int nseg, const int* seg_node_ptr, long long nse, const int* indptr, const int* bcol,
-               const V* data, const T* xin, T* yout, BlockRange* dtab, int* lanes_out) {
+int tune_cfg(hipStream_t s, int nseg, const int* seg_node_ptr, long long nse, const int* indptr, const int* bcol,
+             const V* data, const T* xin, T* yout, BlockRange* dtab, SegMap map, SegCfg* out) {
   const auto key = std::make_tuple(seg_node_ptr[nseg], nse, nseg, static_cast<int>(sizeof(V)), DIFF ? 1 : 0);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
-      *lanes_out = it->second;
+      *out = it->second;
       return FX_OK;
     }
   }
@@ -511,57 +762,60 @@ int tune_lanes(hipStream_t s, int nseg, const int* seg_node_ptr, long long nse, 
   FX_HIP(hipEventCreate(&evg.ev[0]));
   FX_HIP(hipEventCreate(&evg.ev[1]));
   hipEvent_t e0 = evg.ev[0], e1 = evg.ev[1];
-  int best = 0;
+  SegCfg best{0, 0, false};
   float best_ms = 1e30f;
   std::vector<BlockRange> tab;
   std::vector<int> ptr;
   for (int lanes : {8, 16, 32}) {
-    mv_table(nseg, seg_node_ptr, lanes, tab, ptr);
-    FX_HIP(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(BlockRange), hipMemcpyHostToDevice, s));
-    const int nb = static_cast<int>(tab.size());
-    auto launch = [&]() {
-      switch (lanes) {
-        case 8:
-          hipLaunchKernelGGL((seg_spmv_kernel<V, T, 8, DIFF, false>), dim3(nb), dim3(kSegBlock), 0, s, dtab, indptr,
-                             bcol, data, xin, yout, nullptr, nullptr);
-          break;
-        case 16:
-          hipLaunchKernelGGL((seg_spmv_kernel<V, T, 16, DIFF, false>), dim3(nb), dim3(kSegBlock), 0, s, dtab, indptr,
-                             bcol, data, xin, yout, nullptr, nullptr);
-          break;
-        default:
-          hipLaunchKernelGGL((seg_spmv_kernel<V, T, 32, DIFF, false>), dim3(nb), dim3(kSegBlock), 0, s, dtab, indptr,
-                             bcol, data, xin, yout, nullptr, nullptr);
+    for (int bs : {256, 512}) {
+      mv_table(nseg, seg_node_ptr, bs / lanes, tab, ptr);
+      FX_HIP(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(BlockRange), hipMemcpyHostToDevice, s));
+      const int nb = static_cast<int>(tab.size());
+      for (bool use_map : {false, true}) {
+        if (use_map) {
+          int rc = build_seg_map<T>(s, dtab, nb, indptr, bcol, map);
+          if (rc) return rc;
+          if (map.max_unique < 0) continue;
+        }
+        const SegCfg c{lanes, bs, use_map};
+        auto run = [&](auto lc, auto bc, auto mc) -> int {
+          const SegOp<V, T, decltype(lc)::value, DIFF, decltype(bc)::value, decltype(mc)::value> op{
+              dtab, nb, indptr, bcol, data, map};
+          op.apply(s, xin, yout);
+          FX_HIP(hipEventRecord(e0, s));
+          for (int rep = 0; rep < 5; ++rep) op.apply(s, xin, yout);
+          FX_HIP(hipEventRecord(e1, s));
+          FX_HIP(hipEventSynchronize(e1));
+          FX_LAUNCHED("seg_spmv_tune");
+          return FX_OK;
+        };
+        int rc = with_cfg(c, run);
+        if (rc) return rc;
+        float ms = 0.f;
+        FX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = c;
+        }
       }
-    };
-    launch();
-    FX_HIP(hipEventRecord(e0, s));
-    for (int rep = 0; rep < 5; ++rep) launch();
-    FX_HIP(hipEventRecord(e1, s));
-    FX_HIP(hipEventSynchronize(e1));
-    FX_LAUNCHED("seg_spmv_tune");
-    float ms = 0.f;
-    FX_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = lanes;
+      FX_HIP(hipStreamSynchronize(s));  // `tab` is refilled next
     }
   }
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[key] = best;
   }
-  *lanes_out = best;
+  *out = best;
   return FX_OK;
 }
 
 template <typename V, typename T, bool DIFF>
 int run_batched(void* stream, int nseg, const int* seg_node_ptr, long long nse, int flags, const int* indptr,
                 const int* bcol, const V* data, const T* dinv, const T* b, const T* x0, double tol, double atol,
-                int maxiter, int check_every, T* x, int* iters, double* rs, int* lanes_used, void* workspace,
+                int maxiter, int check_every, T* x, int* iters, double* rs, int* tuned, void* workspace,
                 size_t ws_bytes, SegRecord rec = SegRecord{nullptr, nullptr, 0, 0, 0}) {
   hipStream_t s = as_stream(stream);
-  if (nseg <= 0 || !seg_node_ptr || !indptr || !bcol || !data || !b || !x || !iters || !rs)
+  if (nseg <= 0 || !seg_node_ptr || !indptr || !bcol || !data || !b || !x || !iters || !rs || nse < 0)
     return fail(FX_EINVAL, "pcg_batched: bad arguments");
   if ((rec.P != nullptr) != (rec.pq != nullptr) || (rec.P && rec.rec_max <= 0))
     return fail(FX_EINVAL, "pcg_batched: a record needs P, pq and rec_max > 0");
@@ -569,9 +823,16 @@ int run_batched(void* stream, int nseg, const int* seg_node_ptr, long long nse, 
   for (int i = 0; i < nseg; ++i)
     if (seg_node_ptr[i + 1] < seg_node_ptr[i] || seg_node_ptr[0] != 0)
       return fail(FX_EINVAL, "pcg_batched: seg_node_ptr must start at 0 and be non-decreasing");
-  const SegLayout Lay = seg_layout(nseg, seg_node_ptr, sizeof(T));
+  const SegLayout Lay = seg_layout(nseg, seg_node_ptr, nse, sizeof(T));
   if (!workspace || ws_bytes < Lay.total)
     return fail(FX_EWORKSPACE, "pcg_batched: workspace %zu < %zu bytes", ws_bytes, Lay.total);
+  if (Lay.n == 0) {  // no rows: nothing to launch
+    for (int i = 0; i < nseg; ++i) {
+      iters[i] = 0;
+      rs[i] = 0.0;
+    }
+    return FX_OK;
+  }
   char* w = static_cast<char*>(workspace);
   SegBuffers B;
   B.st = reinterpret_cast<PcgState*>(w + Lay.off_st);
@@ -587,45 +848,57 @@ int run_batched(void* stream, int nseg, const int* seg_node_ptr, long long nse, 
   B.rec = rec;
   B.rec.n = 3 * seg_node_ptr[nseg];
   B.rec.nseg = nseg;
+  SegMap map{reinterpret_cast<uint16_t*>(w + Lay.off_lidx), reinterpret_cast<int*>(w + Lay.off_ucol),
+             reinterpret_cast<int*>(w + Lay.off_ucnt), reinterpret_cast<int*>(w + Lay.off_info), -1};
   T* r = reinterpret_cast<T*>(w + Lay.off_vec);
   T* z = reinterpret_cast<T*>(w + Lay.off_vec + Lay.vec_bytes);
   T* p = reinterpret_cast<T*>(w + Lay.off_vec + 2 * Lay.vec_bytes);
   T* q = reinterpret_cast<T*>(w + Lay.off_vec + 3 * Lay.vec_bytes);
 
-  int lanes = sizeof(V) == 8 ? 32 : 16;
-  if (flags & FX_PCG_AUTOTUNE) {
-    int rc = tune_lanes<V, T, DIFF>(s, nseg, seg_node_ptr, nse, indptr, bcol, data, b, q, B.mvtab, &lanes);
+  // Default: FP64 values 32 lanes through bcol; FP32 values 16 lanes with
+  // the map (falls back to bcol when the map is unusable).
+  SegCfg cfg{sizeof(V) == 8 ? 32 : 16, 256, sizeof(V) == 4};
+  if (flags & FX_PCG_SEG_FORCE) {
+    if (!tuned) return fail(FX_EINVAL, "pcg_batched: FX_PCG_SEG_FORCE needs the configuration in tuned");
+    cfg = SegCfg{tuned[0], tuned[1], tuned[2] != 0};
+    if ((cfg.lanes != 8 && cfg.lanes != 16 && cfg.lanes != 32) || (cfg.bs != 256 && cfg.bs != 512))
+      return fail(FX_EINVAL, "pcg_batched: configuration %d lanes x %d threads", cfg.lanes, cfg.bs);
+  } else if (flags & FX_PCG_AUTOTUNE) {
+    int rc = tune_cfg<V, T, DIFF>(s, nseg, seg_node_ptr, nse, indptr, bcol, data, b, q, B.mvtab, map, &cfg);
     if (rc) return rc;
   }
   std::vector<BlockRange> mvtab, vtab;
   std::vector<int> mvptr, vptr, rowptr;
-  mv_table(nseg, seg_node_ptr, lanes, mvtab, mvptr);
+  mv_table(nseg, seg_node_ptr, cfg.bs / cfg.lanes, mvtab, mvptr);
   v_table(nseg, seg_node_ptr, vtab, vptr, rowptr);
   if (static_cast<int>(mvtab.size()) > Lay.nmv || static_cast<int>(vtab.size()) > Lay.nv)
     return fail(FX_ELIMIT, "pcg_batched: block tables exceed the workspace layout");
   B.nv = static_cast<int>(vtab.size());
+  const int nb = static_cast<int>(mvtab.size());
   FX_HIP(hipMemcpyAsync(B.mvtab, mvtab.data(), mvtab.size() * sizeof(BlockRange), hipMemcpyHostToDevice, s));
   FX_HIP(hipMemcpyAsync(B.mvptr, mvptr.data(), mvptr.size() * sizeof(int), hipMemcpyHostToDevice, s));
   FX_HIP(hipMemcpyAsync(B.vtab, vtab.data(), vtab.size() * sizeof(BlockRange), hipMemcpyHostToDevice, s));
   FX_HIP(hipMemcpyAsync(B.vptr, vptr.data(), vptr.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  if (cfg.map) {
+    int rc = build_seg_map<T>(s, B.mvtab, nb, indptr, bcol, map);  // synchronises the stream
+    if (rc) return rc;
+    if (map.max_unique < 0) cfg.map = false;
+  }
   FX_HIP(hipStreamSynchronize(s));  // the host vectors above go out of scope
-  if (lanes_used) *lanes_used = lanes;
-  const int nb = static_cast<int>(mvtab.size());
+  if (tuned) {
+    tuned[0] = cfg.lanes;
+    tuned[1] = cfg.bs;
+    tuned[2] = cfg.map ? 1 : 0;
+  }
   const bool graph = !(flags & FX_PCG_NO_GRAPH);
   const int n = Lay.n;
-  switch (lanes) {
-    case 8:
-      return seg_pcg_loop<T>(s, SegOp<V, T, 8, DIFF>{B.mvtab, nb, indptr, bcol, data}, B, dinv, b, x0, x, r, z, p, q,
-                             n, tol, atol, maxiter, check_every, graph, iters, rs, nullptr);
-    case 16:
-      return seg_pcg_loop<T>(s, SegOp<V, T, 16, DIFF>{B.mvtab, nb, indptr, bcol, data}, B, dinv, b, x0, x, r, z, p,
-                             q, n, tol, atol, maxiter, check_every, graph, iters, rs, nullptr);
-    case 32:
-      return seg_pcg_loop<T>(s, SegOp<V, T, 32, DIFF>{B.mvtab, nb, indptr, bcol, data}, B, dinv, b, x0, x, r, z, p,
-                             q, n, tol, atol, maxiter, check_every, graph, iters, rs, nullptr);
-    default:
-      return fail(FX_EINVAL, "pcg_batched: lane count %d", lanes);
-  }
+  auto go = [&](auto lc, auto bc, auto mc) -> int {
+    const SegOp<V, T, decltype(lc)::value, DIFF, decltype(bc)::value, decltype(mc)::value> op{B.mvtab, nb, indptr,
+                                                                                             bcol, data, map};
+    return seg_pcg_loop<T>(s, op, B, dinv, b, x0, x, r, z, p, q, n, tol, atol, maxiter, check_every, graph, iters,
+                           rs);
+  };
+  return with_cfg(cfg, go);
 }
 
 }  // namespace
@@ -635,33 +908,34 @@ using namespace fx;
 
 extern "C" {
 
-size_t fx_pcg_batched_workspace_bytes(int32_t nseg, const int32_t* seg_node_ptr, int32_t vec_elem_bytes) {
-  if (nseg <= 0 || !seg_node_ptr || (vec_elem_bytes != 4 && vec_elem_bytes != 8)) return 0;
-  return seg_layout(nseg, seg_node_ptr, static_cast<size_t>(vec_elem_bytes)).total;
+size_t fx_pcg_batched_workspace_bytes(int32_t nseg, const int32_t* seg_node_ptr, int64_t nse,
+                                      int32_t vec_elem_bytes) {
+  if (nseg <= 0 || !seg_node_ptr || nse < 0 || (vec_elem_bytes != 4 && vec_elem_bytes != 8)) return 0;
+  return seg_layout(nseg, seg_node_ptr, nse, static_cast<size_t>(vec_elem_bytes)).total;
 }
 
 int fx_pcg_batched_f64(void* stream, int32_t nseg, const int32_t* seg_node_ptr, int64_t nse, int32_t flags,
                        const int32_t* indptr, const int32_t* bcol, const double* data, const double* dinv,
                        const double* b, const double* x0, double tol, double atol, int32_t maxiter,
-                       int32_t check_every, double* x, int32_t* iters, double* rs, int32_t* lanes_used,
+                       int32_t check_every, double* x, int32_t* iters, double* rs, int32_t* tuned,
                        void* workspace, size_t ws_bytes) {
   if (flags & FX_PCG_DIFF3) return fail(FX_EINVAL, "fx_pcg_batched_f64: FX_PCG_DIFF3 is for FP32 matrices");
   return run_batched<double, double, false>(stream, nseg, seg_node_ptr, nse, flags, indptr, bcol, data, dinv, b, x0,
-                                            tol, atol, maxiter, check_every, x, iters, rs, lanes_used, workspace,
+                                            tol, atol, maxiter, check_every, x, iters, rs, tuned, workspace,
                                             ws_bytes);
 }
 
 int fx_pcg_batched_f32f64(void* stream, int32_t nseg, const int32_t* seg_node_ptr, int64_t nse, int32_t flags,
                           const int32_t* indptr, const int32_t* bcol, const float* data, const double* dinv,
                           const double* b, const double* x0, double tol, double atol, int32_t maxiter,
-                          int32_t check_every, double* x, int32_t* iters, double* rs, int32_t* lanes_used,
+                          int32_t check_every, double* x, int32_t* iters, double* rs, int32_t* tuned,
                           void* workspace, size_t ws_bytes) {
   if (flags & FX_PCG_DIFF3)
     return run_batched<float, double, true>(stream, nseg, seg_node_ptr, nse, flags, indptr, bcol, data, dinv, b, x0,
-                                            tol, atol, maxiter, check_every, x, iters, rs, lanes_used, workspace,
+                                            tol, atol, maxiter, check_every, x, iters, rs, tuned, workspace,
                                             ws_bytes);
   return run_batched<float, double, false>(stream, nseg, seg_node_ptr, nse, flags, indptr, bcol, data, dinv, b, x0,
-                                           tol, atol, maxiter, check_every, x, iters, rs, lanes_used, workspace,
+                                           tol, atol, maxiter, check_every, x, iters, rs, tuned, workspace,
                                            ws_bytes);
 }
 
@@ -669,16 +943,16 @@ int fx_pcg_batched_record_f32f64(void* stream, int32_t nseg, const int32_t* seg_
                                  int32_t flags, const int32_t* indptr, const int32_t* bcol, const float* data,
                                  const double* dinv, const double* b, const double* x0, double tol, double atol,
                                  int32_t maxiter, int32_t check_every, double* x, int32_t* iters, double* rs,
-                                 int32_t* lanes_used, float* rec_p, double* rec_pq, int32_t rec_max,
+                                 int32_t* tuned, float* rec_p, double* rec_pq, int32_t rec_max,
                                  void* workspace, size_t ws_bytes) {
   if (!rec_p || !rec_pq || rec_max <= 0) return fail(FX_EINVAL, "fx_pcg_batched_record_f32f64: bad record");
   const SegRecord rec{rec_p, rec_pq, rec_max, 0, 0};
   if (flags & FX_PCG_DIFF3)
     return run_batched<float, double, true>(stream, nseg, seg_node_ptr, nse, flags, indptr, bcol, data, dinv, b, x0,
-                                            tol, atol, maxiter, check_every, x, iters, rs, lanes_used, workspace,
+                                            tol, atol, maxiter, check_every, x, iters, rs, tuned, workspace,
                                             ws_bytes, rec);
   return run_batched<float, double, false>(stream, nseg, seg_node_ptr, nse, flags, indptr, bcol, data, dinv, b, x0,
-                                           tol, atol, maxiter, check_every, x, iters, rs, lanes_used, workspace,
+                                           tol, atol, maxiter, check_every, x, iters, rs, tuned, workspace,
                                            ws_bytes, rec);
 }
 

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 namespace fx {
 
 template <typename T, int L>
@@ -65,6 +67,70 @@ __device__ __forceinline__ void b3_rows(const int* __restrict__ indptr, const in
   y0 = group_sum<T, L>(a0);
   y1 = group_sum<T, L>(a1);
   y2 = group_sum<T, L>(a2);
+}
+
+// One workgroup of the block-3 SpMV through a block-column map: the rows of
+// nodes first .. end - 1 (end - first <= BS / L), q = A p.  The workgroup's
+// nu unique column nodes ucol[0 .. nu) (sorted, so the copy reads nearby
+// addresses) are copied from p into LDS (xs: 3 nu values) once, and every
+// 3x3 block reads its three x values from xs at lidx (the position of its
+// column node in that list).  own[i] = the list position of node first + i
+// (its diagonal block; the difference form subtracts x_n from the other
+// blocks' x).  Products and sums in the order of b3_rows.  Returns this
+// thread's share of p . q (nonzero in the first lane of a node only).
+// Contains a __syncthreads: every thread of the workgroup calls it.
+template <typename V, typename T, int L, bool DIFF, int BS>
+__device__ __forceinline__ double b3map_rows(int first, int end, const int* __restrict__ indptr,
+                                             const uint16_t* __restrict__ lidx, const int* __restrict__ ucol,
+                                             int nu, const V* __restrict__ data, const T* __restrict__ p,
+                                             T* __restrict__ q, T* xs, int* own) {
+  for (int u = threadIdx.x; u < nu; u += BS) {
+    const int c = ucol[u];
+    xs[3 * u] = p[3 * c];
+    xs[3 * u + 1] = p[3 * c + 1];
+    xs[3 * u + 2] = p[3 * c + 2];
+    if (c >= first && c < end) own[c - first] = u;
+  }
+  __syncthreads();
+  const int gl = threadIdx.x % L;
+  const int g = first + threadIdx.x / L;
+  double dot = 0.0;
+  if (g < end) {
+    const int r0 = indptr[3 * g];
+    const int len = indptr[3 * g + 1] - r0;  // 3 * deg
+    const int nb = len / 3;
+    const uint16_t* li = lidx + r0 / 9;
+    const V* d0 = data + r0;
+    const V* d1 = d0 + len;
+    const V* d2 = d1 + len;
+    const int lo = own[g - first];
+    const T xs0 = xs[3 * lo], xs1 = xs[3 * lo + 1], xs2 = xs[3 * lo + 2];
+    T a0 = T(0), a1 = T(0), a2 = T(0);
+    for (int kb = gl; kb < nb; kb += L) {
+      const int l = li[kb];
+      T x0 = xs[3 * l], x1 = xs[3 * l + 1], x2 = xs[3 * l + 2];
+      if constexpr (DIFF) {
+        if (l != lo) {
+          x0 -= xs0;
+          x1 -= xs1;
+          x2 -= xs2;
+        }
+      }
+      const int o = 3 * kb;
+      a0 += static_cast<T>(d0[o]) * x0 + static_cast<T>(d0[o + 1]) * x1 + static_cast<T>(d0[o + 2]) * x2;
+      a1 += static_cast<T>(d1[o]) * x0 + static_cast<T>(d1[o + 1]) * x1 + static_cast<T>(d1[o + 2]) * x2;
+      a2 += static_cast<T>(d2[o]) * x0 + static_cast<T>(d2[o + 1]) * x1 + static_cast<T>(d2[o + 2]) * x2;
+    }
+    const T y0 = group_sum<T, L>(a0), y1 = group_sum<T, L>(a1), y2 = group_sum<T, L>(a2);
+    if (gl == 0) {
+      q[3 * g] = y0;
+      q[3 * g + 1] = y1;
+      q[3 * g + 2] = y2;
+      dot = static_cast<double>(xs0) * static_cast<double>(y0) + static_cast<double>(xs1) * static_cast<double>(y1) +
+            static_cast<double>(xs2) * static_cast<double>(y2);
+    }
+  }
+  return dot;
 }
 
 }  // namespace fx

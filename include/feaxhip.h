@@ -549,27 +549,42 @@ int fx_pcg_matfree_f64(void* stream, int32_t num_nodes, int32_t num_cells, const
  * and the CSR (block-3 form, bcol from fx_csr_block3_index) is the
  * concatenation of the instances' CSR.  Every CG scalar, the stopping test
  * (jax cg semantics, tol/atol/maxiter shared) and the convergence flag are
- * per instance; iters and rs are HOST arrays of nseg entries.  A chunk of
- * check_every iterations is captured once as a HIP graph and replayed
- * (FX_PCG_NO_GRAPH: launch kernels directly; results are identical).
- * FX_PCG_AUTOTUNE: time the SpMV lane counts 8/16/32 on this instance set
- * (cached per shape); *lanes_used (nullable) reports the choice.
+ * per instance; iters and rs are HOST arrays of nseg entries.  An
+ * iteration is three launches (SpMV; alpha and the x, r, z updates; beta
+ * and the direction), each block summing its own instance's partials of
+ * the previous launch.  A chunk of check_every iterations (rounded up to
+ * even) is captured once as a HIP graph and replayed (FX_PCG_NO_GRAPH:
+ * launch kernels directly; results are identical).  The SpMV runs L lanes
+ * per node in workgroups of 256 or 512 threads, each over consecutive nodes
+ * of one instance, reading x through bcol or through a block-column map
+ * that the call builds on the device for that partition (x of the
+ * workgroup's unique column nodes staged in LDS).  Default: FP64 values 32
+ * lanes / 256 threads / bcol, FP32 values 16 / 256 / map.
+ * FX_PCG_AUTOTUNE: time lanes 8/16/32 x 256/512 threads x map off/on on
+ * this instance set (cached per shape); tuned (nullable, HOST, 3 entries)
+ * receives lanes, threads per workgroup and map (0/1) of the run.
+ * FX_PCG_SEG_FORCE: run the configuration given in tuned[0..2] instead
+ * (lanes 8/16/32, threads 256/512, map 0/1; a map that cannot be used --
+ * a workgroup's column nodes spanning more than 65536 node numbers, or
+ * its LDS copy above 64 KiB -- falls back to bcol, and tuned[2] says so).
  * fx_pcg_batched_f32f64: FP32 matrix (FX_PCG_DIFF3: difference form), FP64
  * vectors.  Workspace: fx_pcg_batched_workspace_bytes(nseg, seg_node_ptr,
- * 8).
+ * nse, 8) (map storage included: 6 bytes per 3x3 block).
  * ------------------------------------------------------------------------ */
 #define FX_PCG_AUTOTUNE 8
 #define FX_PCG_NO_GRAPH 16
-size_t fx_pcg_batched_workspace_bytes(int32_t nseg, const int32_t* seg_node_ptr, int32_t vec_elem_bytes);
+#define FX_PCG_SEG_FORCE 64
+size_t fx_pcg_batched_workspace_bytes(int32_t nseg, const int32_t* seg_node_ptr, int64_t nse,
+                                      int32_t vec_elem_bytes);
 int fx_pcg_batched_f64(void* stream, int32_t nseg, const int32_t* seg_node_ptr, int64_t nse, int32_t flags,
                        const int32_t* indptr, const int32_t* bcol, const double* data, const double* dinv,
                        const double* b, const double* x0, double tol, double atol, int32_t maxiter,
-                       int32_t check_every, double* x, int32_t* iters, double* rs, int32_t* lanes_used,
+                       int32_t check_every, double* x, int32_t* iters, double* rs, int32_t* tuned,
                        void* workspace, size_t ws_bytes);
 int fx_pcg_batched_f32f64(void* stream, int32_t nseg, const int32_t* seg_node_ptr, int64_t nse, int32_t flags,
                           const int32_t* indptr, const int32_t* bcol, const float* data, const double* dinv,
                           const double* b, const double* x0, double tol, double atol, int32_t maxiter,
-                          int32_t check_every, double* x, int32_t* iters, double* rs, int32_t* lanes_used,
+                          int32_t check_every, double* x, int32_t* iters, double* rs, int32_t* tuned,
                           void* workspace, size_t ws_bytes);
 /* fx_pcg_batched_record_f32f64: fx_pcg_batched_f32f64 that also records
  * each instance's search directions at its own iteration count k <
@@ -583,7 +598,7 @@ int fx_pcg_batched_record_f32f64(void* stream, int32_t nseg, const int32_t* seg_
                                  int32_t flags, const int32_t* indptr, const int32_t* bcol, const float* data,
                                  const double* dinv, const double* b, const double* x0, double tol, double atol,
                                  int32_t maxiter, int32_t check_every, double* x, int32_t* iters, double* rs,
-                                 int32_t* lanes_used, float* rec_p, double* rec_pq, int32_t rec_max,
+                                 int32_t* tuned, float* rec_p, double* rec_pq, int32_t rec_max,
                                  void* workspace, size_t ws_bytes);
 /* x0 = per-instance Galerkin projection of r onto the recorded directions:
  * x0[i] = sum_{k < counts[s]} (p_k . r)_s / rec_pq[k nseg + s] p_k[i] for

@@ -70,6 +70,72 @@ def test_batched_solve_matches_instance_oracles(batch, precision):
     assert max(errs) < 1e-8
     assert max(solver.stats.relres) <= 2e-10
     assert solver.stats.lanes in (8, 16, 32)
+    assert solver.stats.spmv["threads"] in (256, 512)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_every_spmv_configuration_matches_oracles_and_map_is_bitwise(batch, precision):
+    """Every SpMV configuration the autotuner chooses from (lanes 8/16/32 x
+    256/512 threads per workgroup x block-column map off/on) solves every
+    instance to the oracle; for one (lanes, threads) the map run is bitwise
+    the bcol run (the same products and sums in the same order, x only read
+    from the LDS copy), so a wrong map entry cannot hide."""
+    import feax_amd as fe
+    from feax_amd.batch import create_batched_solver
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-10, atol=0.0, maxiter=20000, use_jacobi_preconditioner=True,
+                                  precision=precision)
+    for lanes in (8, 16, 32):
+        for threads in (256, 512):
+            out = {}
+            for use_map in (False, True):
+                cfg = {"lanes": lanes, "threads": threads, "map": use_map}
+                print("case:", precision, cfg, flush=True)
+                solver = create_batched_solver(batch["bp"], batch["bc"], opts, autotune=False, spmv=cfg)
+                u = solver(batch["tp"])
+                assert solver.stats.spmv == cfg
+                errs = _errors(batch["bp"], u, batch["ref"])
+                assert max(errs) < 1e-8, (cfg, errs)
+                out[use_map] = (u.cpu().numpy().tobytes(), list(solver.stats.iters))
+            assert out[True] == out[False], (lanes, threads)
+
+
+def test_map_falls_back_to_bcol_on_wide_column_spans():
+    """A scrambled node numbering (75,645 nodes in random order) spreads
+    each workgroup's column nodes over more than the map's 65536-node
+    bitmap: a run asked for the map reports it off and is bitwise the bcol
+    run, and it solves the problem (FP64 residual from the assembly)."""
+    import feax_amd as fe
+    from feax_amd.batch import BatchedProblem, batch_bc, batch_params, create_batched_solver
+    rng = np.random.default_rng(5)
+    base, _, _, _, _ = _simp_instance((44, 40, 40), rng)
+    perm = rng.permutation(base.num_total_nodes)
+    pts = np.empty_like(base.points)
+    pts[perm] = base.points
+
+    class Simp(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.SIMPLinearElastic(nu=0.3, E0=1.0, Emin=1e-9, penal=3.0)
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.0, 0.0, 1.0))]
+
+    prob = Simp(fe.Mesh(pts, perm[base.cells]), vec=3, dim=3, location_fns=[lambda p: np.isclose(p[0], 44.0)])
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: np.isclose(p[0], 0.0), "all", 0.0)]).create_bc(prob)
+    tp = fe.TracedParams(volume_vars=(rng.uniform(0.3, 1.0, prob.num_cells),),
+                         surface_vars=[(fe.TracedParams.create_uniform_surface_var(prob, 1e-3),)])
+    bp = BatchedProblem([prob])
+    bcb, tpb = batch_bc(bp, [bc]), batch_params(bp, [tp])
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=20000, use_jacobi_preconditioner=True)
+    out = {}
+    for use_map in (True, False):
+        print("case: scrambled numbering, map asked", use_map, flush=True)
+        solver = create_batched_solver(bp, bcb, opts, autotune=False,
+                                       spmv={"lanes": 16, "threads": 256, "map": use_map})
+        u = solver(tpb)
+        assert solver.stats.spmv["map"] is False
+        assert max(solver.stats.relres) <= 1.5e-8
+        out[use_map] = u.cpu().numpy().tobytes()
+    assert out[True] == out[False]
 
 
 def test_batched_iterations_match_single_instance_solves(batch):
