@@ -54,7 +54,8 @@
 namespace fx {
 namespace {
 
-constexpr int kSegBlock = 256;  // threads per vector-kernel block
+constexpr int kSegBlock = 256;  // threads per block: init, projection and map-build kernels
+constexpr int kSegVec = 512;    // threads per block of the iteration's vector kernels (256 / 1024 slower)
 constexpr int kVecRows = 4096;  // rows per vector-kernel block
 constexpr int kMapWords = 2048; // column bitmap of the map build: 65536 nodes
 constexpr int kMapLds = 62 * 1024;  // LDS staging limit of the map SpMV (bytes; 64 KiB with the static LDS)
@@ -267,7 +268,7 @@ struct SegRecord {
 // mv_ptr[s] .. mv_ptr[s + 1]); x += alpha p; r -= alpha q; z = M r;
 // partials of r.z and r.r.
 template <typename T>
-__global__ __launch_bounds__(kSegBlock) void seg_update_alpha_kernel(
+__global__ __launch_bounds__(kSegVec) void seg_update_alpha_kernel(
     const BlockRange* __restrict__ vtab, int par, const int* __restrict__ mv_ptr, const int* __restrict__ v_ptr,
     const double* __restrict__ pmv, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
     const T* __restrict__ q, const T* __restrict__ dinv, PcgState* __restrict__ st, double* prz, double* prr,
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_update_alpha_kernel(
   __shared__ double s_alpha;
   {
     const double* src[1] = {pmv + mv_ptr[s]};
-    const DVec<1> pq = seg_sums<1, kSegBlock>(src, mv_ptr[s + 1] - mv_ptr[s]);
+    const DVec<1> pq = seg_sums<1, kSegVec>(src, mv_ptr[s + 1] - mv_ptr[s]);
     if (threadIdx.x == 0) {
       s_alpha = st[s].g[par] / pq.v[0];
       if (static_cast<int>(blockIdx.x) == v_ptr[s]) {
@@ -293,7 +294,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_update_alpha_kernel(
   float* rp = (rec.P && k < rec.rec_max) ? rec.P + static_cast<size_t>(k) * rec.n : nullptr;
   DVec<2> acc;
   acc.v[0] = acc.v[1] = 0.0;
-  for (int i = br.begin + threadIdx.x; i < br.end; i += kSegBlock) {
+  for (int i = br.begin + threadIdx.x; i < br.end; i += kSegVec) {
     const T pi = p[i];
     if (rp) rp[i] = static_cast<float>(pi);
     x[i] += alpha * pi;
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_update_alpha_kernel(
     acc.v[0] += static_cast<double>(ri) * static_cast<double>(zi);
     acc.v[1] += static_cast<double>(ri) * static_cast<double>(ri);
   }
-  const DVec<2> t = block_sum<2, kSegBlock>(acc);
+  const DVec<2> t = block_sum<2, kSegVec>(acc);
   if (threadIdx.x == 0) {
     prz[blockIdx.x] = t.v[0];
     prr[blockIdx.x] = t.v[1];
@@ -316,11 +317,11 @@ __global__ __launch_bounds__(kSegBlock) void seg_update_alpha_kernel(
 // rs, stop flag).  A block that starts after that one raised the stop flag
 // skips the (then unused) direction update.
 template <typename T>
-__global__ __launch_bounds__(kSegBlock) void seg_dir_beta_kernel(const BlockRange* __restrict__ vtab, int par,
-                                                                 const int* __restrict__ v_ptr,
-                                                                 const double* __restrict__ prz,
-                                                                 const double* __restrict__ prr, T* __restrict__ p,
-                                                                 const T* __restrict__ z, PcgState* __restrict__ st) {
+__global__ __launch_bounds__(kSegVec) void seg_dir_beta_kernel(const BlockRange* __restrict__ vtab, int par,
+                                                               const int* __restrict__ v_ptr,
+                                                               const double* __restrict__ prz,
+                                                               const double* __restrict__ prr, T* __restrict__ p,
+                                                               const T* __restrict__ z, PcgState* __restrict__ st) {
   const BlockRange br = vtab[blockIdx.x];
   const int s = br.seg;
   if (st[s].done) return;
@@ -328,7 +329,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_dir_beta_kernel(const BlockRang
   {
     const int b0 = v_ptr[s], cnt = v_ptr[s + 1] - v_ptr[s];
     const double* src[2] = {prz + b0, prr + b0};
-    const DVec<2> t = seg_sums<2, kSegBlock>(src, cnt);
+    const DVec<2> t = seg_sums<2, kSegVec>(src, cnt);
     if (threadIdx.x == 0) {
       const double g_new = t.v[0];
       s_beta = g_new / st[s].g[par];
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_dir_beta_kernel(const BlockRang
   }
   __syncthreads();
   const T beta = static_cast<T>(s_beta);
-  for (int i = br.begin + threadIdx.x; i < br.end; i += kSegBlock) p[i] = z[i] + beta * p[i];
+  for (int i = br.begin + threadIdx.x; i < br.end; i += kSegVec) p[i] = z[i] + beta * p[i];
 }
 
 // r = b - q (q = A x0), z = M r, p = z; partials of r.z, r.r, b.b.
@@ -662,9 +663,9 @@ void seg_iteration(hipStream_t s, const Op& op, const SegBuffers& B, int it, con
                    T* q) {
   const int par = it & 1;
   op.iter(s, p, q, B.st, B.pmv);
-  hipLaunchKernelGGL((seg_update_alpha_kernel<T>), dim3(B.nv), dim3(kSegBlock), 0, s, B.vtab, par, B.mvptr, B.vptr,
+  hipLaunchKernelGGL((seg_update_alpha_kernel<T>), dim3(B.nv), dim3(kSegVec), 0, s, B.vtab, par, B.mvptr, B.vptr,
                      B.pmv, x, r, z, p, q, dinv, B.st, B.prz, B.prr, B.rec);
-  hipLaunchKernelGGL((seg_dir_beta_kernel<T>), dim3(B.nv), dim3(kSegBlock), 0, s, B.vtab, par, B.vptr, B.prz, B.prr,
+  hipLaunchKernelGGL((seg_dir_beta_kernel<T>), dim3(B.nv), dim3(kSegVec), 0, s, B.vtab, par, B.vptr, B.prz, B.prr,
                      p, z, B.st);
 }
 

@@ -130,11 +130,24 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
   }
 }
 
+// Position of block (a, b), a <= b, in the packed upper block triangle of
+// an NPE-node element matrix (row a of blocks starts after the
+// NPE + (NPE - 1) + ... + (NPE - a + 1) blocks of the rows above).
+template <int NPE>
+__host__ __device__ __forceinline__ int packed_block(int a, int b) {
+  return a * (2 * NPE - a + 1) / 2 + (b - a);
+}
+
 // NPE nodes per element (8: HEX8, 4: TET4 / QUAD4, 20: HEX20), V DOFs per
 // node (3, or 2 in the plane): element matrices of V^2 NPE^2 values, blocks
-// (a, b) at V^2 (NPE a + b), ranks at NPE (NPE e + a) + b.  An incidence
-// entry is (e << SHIFT) | a; nodes have at most MAXDEG neighbours.
-template <typename T, typename C, int NPE = 8, int V = 3, int SHIFT = 3, int MAXDEG = kMaxDeg>
+// (a, b) at V^2 (NPE a + b), ranks at NPE (NPE e + a) + b.  PACK: the
+// element matrices are exactly symmetric and only their upper block
+// triangle is stored (V^2 NPE (NPE + 1) / 2 values per element, block
+// (a, b), a <= b, at V^2 packed_block(a, b)); block (a, b) with a > b is
+// read as the transpose of block (b, a), which equals it bit for bit.  An
+// incidence entry is (e << SHIFT) | a; nodes have at most MAXDEG
+// neighbours.
+template <typename T, typename C, int NPE = 8, int V = 3, int SHIFT = 3, int MAXDEG = kMaxDeg, bool PACK = false>
 __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
     int num_nodes, const int* __restrict__ inc_ptr, const int* __restrict__ inc, const uint8_t* __restrict__ rank,
     const int* __restrict__ indptr, const int* __restrict__ indices, const C* __restrict__ ke,
@@ -157,7 +170,8 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
   // t = idx % V^2; distinct lanes touch distinct accumulator entries.  The
   // values and ranks of kPf elements are loaded before any is added (their
   // loads overlap), then added in ascending element order.
-  constexpr int kVal = V * V * NPE, kPer = (kVal + 63) / 64, kPf = 4;
+  constexpr int kVal = V * V * NPE, kPer = (kVal + 63) / 64, kPf = 4;  // 8: 35 % slower (occupancy 8 -> 5)
+  constexpr long kElem = PACK ? V * V * NPE * (NPE + 1) / 2 : V * V * NPE * NPE;  // workspace values per element
   const int kend = inc_ptr[nd + 1];
   for (int k0 = inc_ptr[nd]; k0 < kend; k0 += kPf) {
     C val[kPf][kPer];
@@ -167,7 +181,7 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
       const int k = k0 + u;
       const int pk = k < kend ? inc[k] : 0;
       const int e = pk >> SHIFT, a = pk & ((1 << SHIFT) - 1);
-      const C* src = ke + kVal * NPE * static_cast<long>(e) + kVal * a;  // blocks (a, b), b < NPE
+      const C* src = ke + kElem * e;
       const uint8_t* rk = rank + NPE * NPE * static_cast<long>(e) + NPE * a;
 #pragma unroll
       for (int p = 0; p < kPer; ++p) {
@@ -175,7 +189,12 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
         const bool live = k < kend && idx < kVal;
         const int b = idx / (V * V), t = idx - V * V * b;
         const int i = t / V, j = t - V * i;
-        val[u][p] = live ? src[idx] : C(0);
+        int off;
+        if constexpr (PACK)
+          off = a <= b ? V * V * packed_block<NPE>(a, b) + t : V * V * packed_block<NPE>(b, a) + V * j + i;
+        else
+          off = kVal * a + idx;  // blocks (a, b), b < NPE
+        val[u][p] = live ? src[off] : C(0);
         pos[u][p] = live ? i * len + V * rk[b] + j : -1;
       }
     }

@@ -21,10 +21,11 @@
 //   elimination of feax/DCboundary.py:147-199 fused into the store.
 // No float atomics; bitwise repeatable; CSR values written exactly once.
 //
-// Given a workspace (576 values per element), the same sums run in two
-// passes instead: one wave per element forms the element matrix once, then
-// the per-node waves gather it (elem_matrix_kernel, gather_rows_kernel;
-// 1.6 ms instead of 4.9 ms for the 320,000-element cantilever).  The
+// Given a workspace (576 values per element; 324 for the linear laws,
+// whose exactly symmetric element matrices are stored as their upper block
+// triangle), the same sums run in two passes instead: one wave per element
+// forms the element matrix once, then the per-node waves gather it
+// (elem_matrix_lin_kernel / elem_matrix_kernel, gather_rows_kernel).  The
 // residual has the same two forms.
 #include <cstdint>
 
@@ -294,10 +295,12 @@ __global__ __launch_bounds__(64 * kElemPerBlock) void elem_matrix_kernel(
 // point instead of eight).  Blocks: lane (a, b) accumulates, element by
 // element, K_ab = sum_q wl ga gb^T + wm gb ga^T + wm (ga . gb) I with fused
 // multiply-adds in an order under which K_ba = K_ab^T holds bitwise (CG
-// iteration counts react to a non-symmetric rounding of the matrix), and
-// stores its 9 values.
+// iteration counts react to a non-symmetric rounding of the matrix); the
+// lanes with a <= b store their 9 values (the upper block triangle, 324
+// values per element; the gather reads block (b, a) as its transpose).
 constexpr int kLinWaves = 2;  // waves per block
 constexpr int kLinElems = 8;  // elements per wave
+constexpr int kPacked = 324;  // values of the upper block triangle of a HEX8 element matrix (36 blocks)
 
 template <typename C>
 __global__ __launch_bounds__(64 * kLinWaves) void elem_matrix_lin_kernel(
@@ -305,7 +308,7 @@ __global__ __launch_bounds__(64 * kLinWaves) void elem_matrix_lin_kernel(
     const int* __restrict__ cells, MatParams mat, const double* __restrict__ param, C* __restrict__ ke) {
   __shared__ C G_all[kLinWaves][kLinElems][8][8][3];  // grad N_c of element el at Gauss point q
   __shared__ C W_all[kLinWaves][kLinElems][8][2];     // JxW E lam0, JxW E mu0
-  __shared__ C K_all[kLinWaves][576];                 // one element matrix, staged for the store
+  __shared__ C K_all[kLinWaves][kPacked];             // one element matrix (upper triangle), staged for the store
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ebase = (blockIdx.x * kLinWaves + wid) * kLinElems;
   if (ebase >= num_cells) return;
@@ -366,15 +369,19 @@ __global__ __launch_bounds__(64 * kLinWaves) void elem_matrix_lin_kernel(
           K[3 * i + j] = i == j ? fma_c(wm, gg, k) : k;
         }
     }
-    // Stage the 576 values in LDS and store them coalesced: block (a, b)
-    // at 9 (8 a + b).
+    // Stage the upper block triangle (the matrix is exactly symmetric) in
+    // LDS and store its 324 values coalesced: block (a, b), a <= b, at
+    // 9 packed_block(a, b).
     __builtin_amdgcn_wave_barrier();
+    if (a <= b) {
+      const int pb = 9 * packed_block<8>(a, b);
 #pragma unroll
-    for (int t = 0; t < 9; ++t) KS[9 * lane + t] = K[t];
+      for (int t = 0; t < 9; ++t) KS[pb + t] = K[t];
+    }
     __builtin_amdgcn_wave_barrier();
-    C* dst = ke + 576 * static_cast<long>(e);
+    C* dst = ke + kPacked * static_cast<long>(e);
 #pragma unroll
-    for (int i = lane; i < 576; i += 64) dst[i] = KS[i];
+    for (int i = lane; i < kPacked; i += 64) dst[i] = KS[i];
   }
 }
 
@@ -751,7 +758,8 @@ int assemble(void* stream, int num_nodes, int num_cells, const double* x, const 
       else                                                                                                         \
         hipLaunchKernelGGL((elem_matrix_lin_kernel<C>), dim3(lgrid), dim3(64 * kLinWaves), 0, s, num_cells, x, y,  \
                            z, cells, mp, param, ke);                                                               \
-      hipLaunchKernelGGL((gather_rows_kernel<T, C>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s, num_nodes,       \
+      hipLaunchKernelGGL((gather_rows_kernel<T, C, 8, 3, 3, kMaxDeg, !NHV>), dim3(grid), dim3(64 * kNodesPerBlock), \
+                         0, s, num_nodes,                                                                          \
                          inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, diff, jacobi_shift, dinv,    \
                          data);                                                                                    \
     } else {                                                                                                       \

@@ -276,7 +276,8 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, T* __restrict
 // workgroup b runs the rows of nodes b G .. b G + G - 1 (G = BS / L) with
 // x of its unique column nodes ucol[uptr[b] .. uptr[b + 1]) staged in LDS
 // (b3map_rows).  (Dealing the workgroups to the XCDs in contiguous ranges,
-// so that neighbouring blocks share an L2, measured 5 % slower.)  PCG: skip when st->done and write the block partials of
+// so that neighbouring blocks share an L2, measured 5 % slower; loading the
+// node's 3x3 blocks into registers ahead of the x copy, 4 % slower.)  PCG: skip when st->done and write the block partials of
 // p . q; else a plain q = A p.
 template <typename V, typename T, int L, bool DIFF, bool PCG, int BS>
 __global__ __launch_bounds__(BS) void pcg_spmv_map_kernel(int n, const int* __restrict__ indptr,
@@ -307,7 +308,8 @@ __global__ __launch_bounds__(BS) void pcg_spmv_map_kernel(int n, const int* __re
 // same scalar).  gamma alternates between st->g[0] and st->g[1] by the
 // parity of the iteration: the direction kernel of iteration k writes
 // g[(k + 1) % 2] while its other blocks still read g[k % 2].
-constexpr int kUpdBlocks = 256;
+constexpr int kUpdBlocks = 256;  // one block per CU (384 / 512 / 1024 measured 1.4-2x slower)
+constexpr int kPre = 4;  // rows per thread loaded ahead of the partial sums (1M rows: 3.9 per thread)
 inline int upd_grid(int n) { return std::max(1, std::min(kUpdBlocks, (n + kFinBlock - 1) / kFinBlock)); }
 
 __device__ __forceinline__ Range fin_block_range(int n, int& lblock) {
@@ -328,6 +330,22 @@ __global__ __launch_bounds__(kFinBlock) void pcg_update_alpha_kernel(
     const double* __restrict__ op_partials, int op_nparts, double* partials, float* __restrict__ rec_p,
     double* __restrict__ rec_pq, int rec_max) {
   if (st->done) return;
+  int lb;
+  const Range rg = fin_block_range(n, lb);
+  // The thread's first kPre rows are loaded before the partial sums, so
+  // their HBM latency overlaps the (latency-bound) sum of the SpMV partials.
+  T pv[kPre] = {}, qv[kPre] = {}, xv[kPre] = {}, rv[kPre] = {}, dv[kPre] = {};
+#pragma unroll
+  for (int j = 0; j < kPre; ++j) {
+    const int i = rg.begin + threadIdx.x + j * kFinBlock;
+    if (i < rg.end) {
+      pv[j] = p[i];
+      qv[j] = q[i];
+      xv[j] = x[i];
+      rv[j] = r[i];
+      if (dinv) dv[j] = dinv[i];
+    }
+  }
   __shared__ double s_alpha;
   {
     const DVec<1> t = sum_partials<1>(op_partials, op_nparts);
@@ -341,13 +359,25 @@ __global__ __launch_bounds__(kFinBlock) void pcg_update_alpha_kernel(
   }
   __syncthreads();
   const T alpha = static_cast<T>(s_alpha);
-  int lb;
-  const Range rg = fin_block_range(n, lb);
   float* rec = (rec_p && st->k < rec_max) ? rec_p + static_cast<size_t>(st->k) * n : nullptr;
   DVec<2> acc;
   acc.v[0] = 0.0;
   acc.v[1] = 0.0;
-  for (int i = rg.begin + threadIdx.x; i < rg.end; i += kFinBlock) {
+#pragma unroll
+  for (int j = 0; j < kPre; ++j) {
+    const int i = rg.begin + threadIdx.x + j * kFinBlock;
+    if (i < rg.end) {
+      if (rec) rec[i] = static_cast<float>(pv[j]);
+      x[i] = xv[j] + alpha * pv[j];
+      const T ri = rv[j] - alpha * qv[j];
+      r[i] = ri;
+      const T zi = dinv ? dv[j] * ri : ri;
+      z[i] = zi;
+      acc.v[0] += static_cast<double>(ri) * static_cast<double>(zi);
+      acc.v[1] += static_cast<double>(ri) * static_cast<double>(ri);
+    }
+  }
+  for (int i = rg.begin + threadIdx.x + kPre * kFinBlock; i < rg.end; i += kFinBlock) {
     const T pi = p[i];
     if (rec) rec[i] = static_cast<float>(pi);
     x[i] += alpha * pi;
@@ -369,6 +399,17 @@ __global__ __launch_bounds__(kFinBlock) void pcg_dir_beta_kernel(int n, int par,
                                                                  const T* __restrict__ z, PcgState* __restrict__ st,
                                                                  const double* __restrict__ partials, int nparts) {
   if (st->done) return;
+  int lb;
+  const Range rg = fin_block_range(n, lb);
+  T zv[kPre] = {}, pv[kPre] = {};
+#pragma unroll
+  for (int j = 0; j < kPre; ++j) {
+    const int i = rg.begin + threadIdx.x + j * kFinBlock;
+    if (i < rg.end) {
+      zv[j] = z[i];
+      pv[j] = p[i];
+    }
+  }
   __shared__ double s_beta;
   {
     const DVec<2> t = sum_partials<2>(partials, nparts);
@@ -388,9 +429,12 @@ __global__ __launch_bounds__(kFinBlock) void pcg_dir_beta_kernel(int n, int par,
   }
   __syncthreads();
   const T beta = static_cast<T>(s_beta);
-  int lb;
-  const Range rg = fin_block_range(n, lb);
-  for (int i = rg.begin + threadIdx.x; i < rg.end; i += kFinBlock) p[i] = z[i] + beta * p[i];
+#pragma unroll
+  for (int j = 0; j < kPre; ++j) {
+    const int i = rg.begin + threadIdx.x + j * kFinBlock;
+    if (i < rg.end) p[i] = zv[j] + beta * pv[j];
+  }
+  for (int i = rg.begin + threadIdx.x + kPre * kFinBlock; i < rg.end; i += kFinBlock) p[i] = z[i] + beta * p[i];
 }
 
 __global__ __launch_bounds__(kFinBlock) void pcg_fin_beta(PcgState* st, const double* __restrict__ partials,

@@ -294,8 +294,10 @@ int fx_hex8_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, con
                          const int32_t* indices, const fx_material* mat, const double* param,
                          const double* u, const uint8_t* bc_mask, int32_t symmetric, int32_t flags,
                          double jacobi_shift, double* dinv, float* data, void* workspace, size_t ws_bytes);
-/* 576 element-matrix values per element: value_bytes 8 (f64) or 4 (f32;
- * FP32 workspace only with FX_ASM_COMPUTE32). */
+/* 576 element-matrix values per element (the linear laws use 324 of them:
+ * their element matrices are exactly symmetric and only the upper 3x3
+ * block triangle is stored): value_bytes 8 (f64) or 4 (f32; FP32 workspace
+ * only with FX_ASM_COMPUTE32). */
 size_t fx_hex8_assemble_workspace_bytes(int32_t num_cells, int32_t value_bytes, int32_t flags);
 
 /* Global residual R(u) = volume + traction (FP64): feax get_res
