@@ -12,8 +12,11 @@ Two extras for HEX8 vec = 3 patterns:
   on the device by fx_csr_block3_index).
 * ``form == "diff3"``: FP32 values in the difference form written by the
   assembler for the mixed-precision solver (the diagonal 3x3 block slot
-  holds the block row sums S; off-diagonal blocks act on x_m - x_n).  Its
-  ``@`` applies that operator to FP64 vectors; the array is not a plain
+  holds the block row sums S; off-diagonal blocks act on x_m - x_n), stored
+  block by block: a node's values start where its three CSR rows start
+  (indptr[3 m]) and hold its 3x3 blocks one after another, nine values each,
+  row-major (one contiguous run per block for the SpMV lane that reads it).
+  Its ``@`` applies that operator to FP64 vectors; the array is not a plain
   CSR value array.
 """
 

@@ -16,7 +16,8 @@ constexpr int kMaxDeg = 64;        // LDS accumulator: 9 * kMaxDeg values per wa
 
 // The node's three CSR rows from its LDS accumulator (acc: row i, block kb,
 // column j at i * len + 3 kb + j): Jacobi dinv, the difference form, and the
-// store with the Dirichlet elimination of feax/DCboundary.py:147-199.  The
+// store with the Dirichlet elimination of feax/DCboundary.py:147-199 (the
+// difference form is stored block by block: its 3x3 blocks contiguous).  The
 // constrained columns are found once per neighbour block (one lane per
 // block reads the block's column node, a ballot per component collects the
 // Dirichlet flags), not once per value.
@@ -108,10 +109,23 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
     if (lane < V * V) acc[c * len + V * kself + d] = S;
     __builtin_amdgcn_wave_barrier();
   }
-  for (int idx = lane; idx < nval; idx += 64) {
-    C v = acc[idx];
-    const int i = idx / len, p = idx - len * i;
-    const int kb = p / V, d = p - V * kb;
+  // Output position o of the node's V rows: CSR order (row i, block kb,
+  // column d at i len + V kb + d), or, for the difference form, block
+  // order (the V x V block kb contiguous, row-major, at V^2 kb + V i + d).
+  for (int o = lane; o < nval; o += 64) {
+    int i, kb, d;
+    if (diff_form) {
+      kb = o / (V * V);
+      const int rem = o - V * V * kb;
+      i = rem / V;
+      d = rem - V * i;
+    } else {
+      i = o / len;
+      const int p = o - len * i;
+      kb = p / V;
+      d = p - V * kb;
+    }
+    C v = acc[i * len + V * kb + d];
     const bool own = kb == kself;
     if (bc_mask && !(diff_form && own)) {
       bool zr = false, zc = false;
@@ -126,7 +140,7 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
       if (zr || (symmetric && zc)) v = C(0);
       if (zr && own && i == d) v = C(1);
     }
-    data[row0 + idx] = static_cast<T>(v);
+    data[row0 + o] = static_cast<T>(v);
   }
 }
 

@@ -26,10 +26,30 @@ __device__ __forceinline__ T csr_row(const int* __restrict__ indptr, const int* 
   return group_sum<T, L>(acc);
 }
 
+// Values of 3x3 block kb of the node whose rows start at r0 (len = 3 deg
+// values per row): rows of the CSR layout (three runs of three values), or
+// the block layout of the difference form (nine consecutive values,
+// row-major).  v[3 i + j] = entry (i, j).
+template <typename V, bool BLK>
+__device__ __forceinline__ void b3_block(const V* __restrict__ data, int r0, int len, int kb, V (&v)[9]) {
+  if constexpr (BLK) {
+    const V* b = data + r0 + 9 * kb;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) v[t] = b[t];
+  } else {
+    const V* b = data + r0 + 3 * kb;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) v[3 * i + j] = b[i * len + j];
+  }
+}
+
 // Rows 3m..3m+2 of a block-3 pattern; one lane per 3x3 block (L lanes per
-// node): one column index, three consecutive x entries and three runs of
-// three consecutive values.  DIFF: difference form -- the node's own block
-// slot holds S (block row sums), the others act on x_m - x_n:
+// node): one column index, three consecutive x entries and the block's
+// nine values.  DIFF: difference form (values in the block layout) -- the
+// node's own block slot holds S (block row sums), the others act on
+// x_m - x_n:
 //   y = sum_{m != n} K_nm (x_m - x_n) + S x_n.
 template <typename V, typename T, int L, bool DIFF>
 __device__ __forceinline__ void b3_rows(const int* __restrict__ indptr, const int* __restrict__ bcol,
@@ -39,9 +59,6 @@ __device__ __forceinline__ void b3_rows(const int* __restrict__ indptr, const in
   const int len = indptr[3 * m + 1] - r0;  // 3 * deg
   const int nb = len / 3;
   const int* bc = bcol + r0 / 9;
-  const V* d0 = data + r0;
-  const V* d1 = d0 + len;
-  const V* d2 = d1 + len;
   T a0 = T(0), a1 = T(0), a2 = T(0);
   T xs0 = T(0), xs1 = T(0), xs2 = T(0);
   if constexpr (DIFF) {
@@ -59,10 +76,11 @@ __device__ __forceinline__ void b3_rows(const int* __restrict__ indptr, const in
         x2 -= xs2;
       }
     }
-    const int o = 3 * kb;
-    a0 += static_cast<T>(d0[o]) * x0 + static_cast<T>(d0[o + 1]) * x1 + static_cast<T>(d0[o + 2]) * x2;
-    a1 += static_cast<T>(d1[o]) * x0 + static_cast<T>(d1[o + 1]) * x1 + static_cast<T>(d1[o + 2]) * x2;
-    a2 += static_cast<T>(d2[o]) * x0 + static_cast<T>(d2[o + 1]) * x1 + static_cast<T>(d2[o + 2]) * x2;
+    V v[9];
+    b3_block<V, DIFF>(data, r0, len, kb, v);
+    a0 += static_cast<T>(v[0]) * x0 + static_cast<T>(v[1]) * x1 + static_cast<T>(v[2]) * x2;
+    a1 += static_cast<T>(v[3]) * x0 + static_cast<T>(v[4]) * x1 + static_cast<T>(v[5]) * x2;
+    a2 += static_cast<T>(v[6]) * x0 + static_cast<T>(v[7]) * x1 + static_cast<T>(v[8]) * x2;
   }
   y0 = group_sum<T, L>(a0);
   y1 = group_sum<T, L>(a1);
@@ -100,9 +118,6 @@ __device__ __forceinline__ double b3map_rows(int first, int end, const int* __re
     const int len = indptr[3 * g + 1] - r0;  // 3 * deg
     const int nb = len / 3;
     const uint16_t* li = lidx + r0 / 9;
-    const V* d0 = data + r0;
-    const V* d1 = d0 + len;
-    const V* d2 = d1 + len;
     const int lo = own[g - first];
     const T xs0 = xs[3 * lo], xs1 = xs[3 * lo + 1], xs2 = xs[3 * lo + 2];
     T a0 = T(0), a1 = T(0), a2 = T(0);
@@ -116,10 +131,11 @@ __device__ __forceinline__ double b3map_rows(int first, int end, const int* __re
           x2 -= xs2;
         }
       }
-      const int o = 3 * kb;
-      a0 += static_cast<T>(d0[o]) * x0 + static_cast<T>(d0[o + 1]) * x1 + static_cast<T>(d0[o + 2]) * x2;
-      a1 += static_cast<T>(d1[o]) * x0 + static_cast<T>(d1[o + 1]) * x1 + static_cast<T>(d1[o + 2]) * x2;
-      a2 += static_cast<T>(d2[o]) * x0 + static_cast<T>(d2[o + 1]) * x1 + static_cast<T>(d2[o + 2]) * x2;
+      V v[9];
+      b3_block<V, DIFF>(data, r0, len, kb, v);
+      a0 += static_cast<T>(v[0]) * x0 + static_cast<T>(v[1]) * x1 + static_cast<T>(v[2]) * x2;
+      a1 += static_cast<T>(v[3]) * x0 + static_cast<T>(v[4]) * x1 + static_cast<T>(v[5]) * x2;
+      a2 += static_cast<T>(v[6]) * x0 + static_cast<T>(v[7]) * x1 + static_cast<T>(v[8]) * x2;
     }
     const T y0 = group_sum<T, L>(a0), y1 = group_sum<T, L>(a1), y2 = group_sum<T, L>(a2);
     if (gl == 0) {

@@ -121,7 +121,10 @@ int fx_csr_spmv_b3_f32(void* stream, int32_t n, int64_t nse, const int32_t* indp
                        const float* data, const float* x, float* y);
 /* Difference-form operator (FX_ASM_DIFF3 values, FP32) on FP64 vectors:
  * y = sum_{m != n} K_nm (x_m - x_n) + S_n x_n, S_n stored in the diagonal
- * block slot.  Equals K x; FP32 rounding acts on differences only. */
+ * block slot.  Equals K x; FP32 rounding acts on differences only.  The
+ * difference form is stored block by block: node m's values start at
+ * indptr[3 m] and hold its 3x3 blocks in bcol order, nine values each,
+ * row-major (the CSR layout's footprint, reordered within the node). */
 int fx_csr_spmv_diff3_f32f64(void* stream, int32_t n, int64_t nse, const int32_t* indptr,
                              const int32_t* bcol, const float* data, const double* x, double* y);
 
