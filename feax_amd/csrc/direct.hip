@@ -11,19 +11,21 @@
 // blocked right-looking factorisation stays inside that range:
 //
 //   for each block column [c0, c0 + w), w <= nb:
-//     L11 = potrf(A11)                    (rocSOLVER, w x w)
+//     L11 = potrf(A11)                    (chol_block_kernel, w x w in LDS)
 //     L21 = A21 L11^-T                    (rocBLAS trsm, h x w, h <= bw)
 //     A22 -= L21 L21^T  (lower triangle)  (rocBLAS syrk, h x h)
 //
 // The entries of L21 past the bandwidth are stored zeros and stay zero
 // (Cholesky fills only inside the envelope).  The triangular solves are
-// hand-written: one launch per 256-row block, whose diagonal triangle is
-// solved in 64-row sub-blocks (one wave from LDS, then all threads update
-// the rows below / above), while the other workgroups apply the block's
-// coupling to the next bandwidth rows (forward) or the previous ones
-// (backward) as coalesced multiply-adds; they read only entries within the
-// bandwidth.  Rows are in the caller's ordering `perm` (new -> old), chosen
-// to reduce the bandwidth.
+// hand-written: per 256-row block, a wide launch forms the block's coupling
+// to the block solved just before it (16 workgroups forward, one wave per
+// 4 columns backward), then one launch whose workgroup 0 subtracts it and
+// solves the diagonal triangle in 64-row sub-blocks (one wave from LDS,
+// then all threads update the rows below / above), while its other
+// workgroups apply the previous block to the rest of the bandwidth rows
+// (forward) or columns (backward) as coalesced multiply-adds; only entries
+// within the bandwidth are read.  Rows are in the caller's ordering `perm`
+// (new -> old), chosen to reduce the bandwidth.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -155,27 +157,68 @@ __device__ __forceinline__ void col_dots(const double* __restrict__ ab, int64_t 
   }
 }
 
-// Forward step (L y = b, in place).  On entry y[cK, cK + wK) is final (cK
-// < 0: nothing yet).  Block 0: the next block [c1, c1 + w1) takes its
-// coupling to block K, then solves its triangle in kSub sub-blocks (wave 0
-// solves a sub-block from LDS, all threads update the rows below it).
-// Other blocks: rows [c1 + w1, min(n, cK + wK + bw)) take the coupling to
-// block K.  Only entries within the bandwidth are read.
+// Coupling of the next block to the block just solved, spread over many
+// workgroups so that it leaves the serial path of the single workgroup
+// that solves the next block's triangle.  Forward: rows [c, c + w) of
+// block k against the kSolveNb columns from cK (block k - 1, final y):
+// P[q kSolveNb + r] = sum_{u < kChunk} L(c + r, cK + q kChunk + u) y, one
+// workgroup per column chunk q, one thread per row (for one column the
+// threads read consecutive addresses).  Entries past the bandwidth count 0.
+constexpr int kChunk = 16;
+constexpr int kChunks = kSolveNb / kChunk;
+
+// Workspace of the triangular solves: y (n values, padded to 32) and the
+// coupling partials (kChunks x kSolveNb), 256-byte aligned.
+inline size_t solve_region_bytes(int n) {
+  return ((((static_cast<size_t>(n) + 31) & ~static_cast<size_t>(31)) + kChunks * kSolveNb) * 8 + 255) &
+         ~static_cast<size_t>(255);
+}
+__global__ __launch_bounds__(kThreads) void band_fwd_couple_kernel(int bw, int64_t ldab, const double* __restrict__ ab,
+                                                                   const double* __restrict__ y, int cK, int c, int w,
+                                                                   double* __restrict__ P) {
+  __shared__ double yk[kChunk];
+  const int q = blockIdx.x, t = threadIdx.x;
+  const int j0 = cK + q * kChunk;
+  if (t < kChunk) yk[t] = y[j0 + t];
+  __syncthreads();
+  double acc = 0.0;
+  if (t < w) {
+    const int r = c + t;
+    double l[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) l[u] = (r - (j0 + u) <= bw) ? ab[boff(r, j0 + u, ldab)] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) acc += l[u] * yk[u];
+  }
+  P[q * kSolveNb + t] = acc;
+}
+
+// Forward step (L y = b, in place) for block [c1, c1 + w1).  On entry
+// y[cK, cK + wK) is final (wK = 0: the first block) and, when wK > 0,
+// band_fwd_couple_kernel has left the block's coupling to it in P.
+// Block 0: subtracts that coupling (chunks in a fixed order), then solves
+// the triangle in kSub sub-blocks (wave 0 solves a sub-block from LDS, all
+// threads update the rows below it).  Other blocks: rows [c1 + w1,
+// min(n, cK + wK + bw)) take the coupling to block K.  Only entries within
+// the bandwidth are read.
 __global__ __launch_bounds__(kThreads) void band_fwd_kernel(int n, int bw, int64_t ldab,
                                                             const double* __restrict__ ab, double* __restrict__ y,
-                                                            int cK, int wK, int unit) {
+                                                            int cK, int wK, int c1, int unit,
+                                                            const double* __restrict__ P) {
   __shared__ double yk[kSolveNb];
   __shared__ double yb[kSolveNb];
   __shared__ double Ld[kSub][kSub + 1];
   const int t = threadIdx.x;
-  const int c1 = cK < 0 ? 0 : cK + wK;
   const int w1 = min(kSolveNb, n - c1);
-  if (t < wK) yk[t] = y[cK + t];
-  __syncthreads();
   if (blockIdx.x == 0) {
     if (t < w1) {
       double v = y[c1 + t];
-      if (wK > 0) v -= row_dot(ab, ldab, c1 + t, cK, max(0, c1 + t - bw - cK), wK, yk);
+      if (wK > 0) {
+        double cpl = 0.0;
+#pragma unroll
+        for (int q = 0; q < kChunks; ++q) cpl += P[q * kSolveNb + t];
+        v -= cpl;
+      }
       yb[t] = v;
     }
     for (int s0 = 0; s0 < w1; s0 += kSub) {
@@ -201,10 +244,34 @@ __global__ __launch_bounds__(kThreads) void band_fwd_kernel(int n, int bw, int64
     if (t < w1) y[c1 + t] = yb[t];
     return;
   }
+  if (t < wK) yk[t] = y[cK + t];
+  __syncthreads();
   const int r = c1 + w1 + (blockIdx.x - 1) * kThreads + t;
   const int rend = min(n, cK + wK + bw);
   if (r >= rend) return;
   y[r] -= row_dot(ab, ldab, r, cK, max(0, r - bw - cK), wK, yk);
+}
+
+// Backward coupling: C[j] = sum over the wK rows from cK (block k + 1,
+// final x) of L(r, c1 + j) x_r for the w1 columns of block k.  One wave per
+// kJc columns, lanes over the rows (coalesced), wave reductions.
+constexpr int kJc = 4;
+__global__ __launch_bounds__(kThreads) void band_bwd_couple_kernel(int bw, int64_t ldab, const double* __restrict__ ab,
+                                                                   const double* __restrict__ y, int cK, int wK,
+                                                                   int c1, int w1, double* __restrict__ C) {
+  __shared__ double xk[kSolveNb];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  if (t < wK) xk[t] = y[cK + t];
+  __syncthreads();
+  const int jj = (blockIdx.x * (kThreads / 64) + wid) * kJc;
+  if (jj >= w1) return;  // uniform over the wave
+  double sv[kJc];
+  col_dots<kJc>(ab, ldab, bw, cK, wK, c1 + jj, 1, c1 + w1, xk, sv);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < kJc; ++q)
+      if (jj + q < w1) C[jj + q] = sv[q];
+  }
 }
 
 // Backward step (L^T x = y, in place).  On entry y[cK, cK + wK) holds final
@@ -214,30 +281,17 @@ __global__ __launch_bounds__(kThreads) void band_fwd_kernel(int n, int bw, int64
 // triangle in kSub sub-blocks from the bottom.
 __global__ __launch_bounds__(kThreads) void band_bwd_kernel(int n, int bw, int64_t ldab,
                                                             const double* __restrict__ ab, double* __restrict__ y,
-                                                            int cK, int wK, int c1, int w1, int unit) {
+                                                            int cK, int wK, int c1, int w1, int unit,
+                                                            const double* __restrict__ C) {
   __shared__ double xk[kSolveNb];
   __shared__ double xb[kSolveNb];
   __shared__ double Ld[kSub][kSub + 1];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   constexpr int kWaves = kThreads / 64;
-  if (t < wK) xk[t] = y[cK + t];
-  __syncthreads();
   if (blockIdx.x == 0) {
     constexpr int kJ = 8;
-    for (int jj = wid; jj < w1; jj += kWaves * kJ) {
-      double s[kJ];
-      if (wK > 0) {
-        col_dots<kJ>(ab, ldab, bw, cK, wK, c1 + jj, kWaves, c1 + w1, xk, s);
-      } else {
-#pragma unroll
-        for (int q = 0; q < kJ; ++q) s[q] = 0.0;
-      }
-      if (lane == 0) {
-#pragma unroll
-        for (int q = 0; q < kJ; ++q)
-          if (jj + q * kWaves < w1) xb[jj + q * kWaves] = y[c1 + jj + q * kWaves] - s[q];
-      }
-    }
+    if (t < w1) xb[t] = wK > 0 ? y[c1 + t] - C[t] : y[c1 + t];
+    __syncthreads();
     for (int s0 = ((w1 - 1) / kSub) * kSub; s0 >= 0; s0 -= kSub) {
       const int sw = min(kSub, w1 - s0);
       load_diag(ab, ldab, bw, c1 + s0, sw, unit, Ld);
@@ -269,6 +323,8 @@ __global__ __launch_bounds__(kThreads) void band_bwd_kernel(int n, int bw, int64
     return;
   }
   if (wK == 0) return;
+  if (t < wK) xk[t] = y[cK + t];
+  __syncthreads();
   const int jlo = max(0, cK - bw);
   const int j = c1 - 1 - ((blockIdx.x - 1) * kWaves + wid);
   if (j < jlo) return;
@@ -310,6 +366,108 @@ __global__ __launch_bounds__(kThreads) void ldlt_panel_kernel(int64_t ldab, doub
   ab[o] = v / ab[boff(c0 + j, c0 + j, ldab)];
 }
 
+// Cholesky of the w x w (w <= kFactorNb) diagonal block at (c0, c0) of the
+// band's dense view (leading dimension ld), in place, lower triangle: one
+// workgroup, the block in LDS.  Right-looking in panels of kPanel columns:
+// wave 0 factors a panel from registers (lane l holds rows p0 + l and
+// p0 + 64 + l of the panel's columns; pivots and the column entries of the
+// panel rows travel by lane shuffles, no barrier per column), then all
+// waves apply the panel to the trailing triangle (each entry one
+// kPanel-term dot from LDS; 4 x 4 register tiles measured slower).  The
+// block is loaded with all of a thread's loads in flight at once.  A pivot
+// that is not positive (or NaN) stops
+// the factorisation and sets *info to its column + 1, as LAPACK's potrf
+// does; else *info = 0.
+constexpr int kPanel = 16;
+constexpr int kCholThreads = 1024;
+static_assert(kFactorNb <= 128, "two panel rows per lane");
+__global__ __launch_bounds__(kCholThreads) void chol_block_kernel(double* __restrict__ ab, int64_t ld, int c0, int w,
+                                                                  int* __restrict__ info) {
+  __shared__ double A[kFactorNb][kFactorNb + 1];
+  __shared__ int s_failed;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  double* blk = ab + c0 + static_cast<int64_t>(c0) * ld;
+  {
+    constexpr int kLd = kFactorNb * kFactorNb / kCholThreads;
+    double v[kLd];
+#pragma unroll
+    for (int q = 0; q < kLd; ++q) {
+      const int e = t + q * kCholThreads;
+      const int i = e % kFactorNb, j = e / kFactorNb;
+      v[q] = (i >= j && i < w) ? blk[i + static_cast<int64_t>(j) * ld] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kLd; ++q) {
+      const int e = t + q * kCholThreads;
+      A[e % kFactorNb][e / kFactorNb] = v[q];
+    }
+  }
+  if (t == 0) s_failed = 0;
+  __syncthreads();
+  for (int p0 = 0; p0 < w; p0 += kPanel) {
+    const int pw = min(kPanel, w - p0), pe = p0 + pw;
+    if (wid == 0) {
+      double a[2][kPanel];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int i = p0 + lane + 64 * r;
+#pragma unroll
+        for (int c = 0; c < kPanel; ++c) a[r][c] = (i < w && c < pw && i >= p0 + c) ? A[i][p0 + c] : 0.0;
+      }
+      int failed = 0;
+#pragma unroll
+      for (int c = 0; c < kPanel; ++c) {
+        if (c < pw && !failed) {  // uniform
+          const double d = __shfl(a[0][c], c, 64);
+          if (!(d > 0.0)) {
+            failed = p0 + c + 1;
+          } else {
+            const double sq = sqrt(d), inv = 1.0 / sq;
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+              const int i = p0 + lane + 64 * r;
+              a[r][c] = i == p0 + c ? sq : a[r][c] * inv;
+            }
+#pragma unroll
+            for (int k = c + 1; k < kPanel; ++k) {
+              const double lkc = __shfl(a[0][c], k, 64);  // L(p0 + k, p0 + c)
+#pragma unroll
+              for (int r = 0; r < 2; ++r) a[r][k] -= a[r][c] * lkc;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int i = p0 + lane + 64 * r;
+#pragma unroll
+        for (int c = 0; c < kPanel; ++c)
+          if (i < w && c < pw && i >= p0 + c) A[i][p0 + c] = a[r][c];
+      }
+      if (lane == 0 && failed) s_failed = failed;
+    }
+    __syncthreads();
+    if (s_failed) break;  // uniform
+    const int m = w - pe;  // trailing triangle rows / columns [pe, w)
+    for (int e = t; e < m * m; e += kCholThreads) {
+      const int i = pe + e % m, k = pe + e / m;
+      if (i >= k) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < kPanel; ++j)
+          if (j < pw) acc += A[i][p0 + j] * A[k][p0 + j];
+        A[i][k] -= acc;
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < kFactorNb * kFactorNb; e += kCholThreads) {
+    const int i = e % kFactorNb, j = e / kFactorNb;
+    if (i >= j && i < w) blk[i + static_cast<int64_t>(j) * ld] = A[i][j];
+  }
+  if (t == 0) *info = s_failed;
+}
+
 // One rocBLAS handle per device for the process (rocSOLVER runs on it).
 rocblas_handle blas_handle(int& err) {
   static std::mutex mu;
@@ -344,8 +502,8 @@ int64_t fx_band_ldab(int32_t bw) { return bw < 0 ? -1 : static_cast<int64_t>(bw)
 size_t fx_band_workspace_bytes(int32_t n, int32_t bw) {
   if (n < 0 || bw < 0) return 0;
   const size_t nb = (static_cast<size_t>(n) + kFactorNb - 1) / kFactorNb;
-  return 256 + ((nb * sizeof(int) + 255) & ~static_cast<size_t>(255)) +
-         ((static_cast<size_t>(n) * 8 + 255) & ~static_cast<size_t>(255)) +
+  // info flags, the solves' region, the LDL^T panel copy (bw x kFactorNb)
+  return 256 + ((nb * sizeof(int) + 255) & ~static_cast<size_t>(255)) + solve_region_bytes(n) +
          static_cast<size_t>(bw) * kFactorNb * 8;
 }
 
@@ -394,7 +552,7 @@ int band_factor(const char* name, bool ldlt, hipStream_t s, int n, int bw, int64
   const int nblocks = (n + kFactorNb - 1) / kFactorNb;
   int* dinfo = reinterpret_cast<int*>(wsp + 256);
   double* wt = reinterpret_cast<double*>(wsp + 256 + ((nblocks * sizeof(int) + 255) & ~static_cast<size_t>(255)) +
-                                         ((static_cast<size_t>(n) * 8 + 255) & ~static_cast<size_t>(255)));
+                                         solve_region_bytes(n));
   const int ld = static_cast<int>(ldab - 1);
   auto A = [&](int64_t i, int64_t j) { return ab + i + j * static_cast<int64_t>(ld); };
   const double one = 1.0, minus_one = -1.0;
@@ -409,8 +567,9 @@ int band_factor(const char* name, bool ldlt, hipStream_t s, int n, int bw, int64
       st = rocsolver_dgetrf_npvt(h, w, w, A(c0, c0), ld, dinfo + k);
       if (st != rocblas_status_success) return blas_fail(st, "rocsolver_dgetrf_npvt");
     } else {
-      st = rocsolver_dpotrf(h, rocblas_fill_lower, w, A(c0, c0), ld, dinfo + k);
-      if (st != rocblas_status_success) return blas_fail(st, "rocsolver_dpotrf");
+      hipLaunchKernelGGL(chol_block_kernel, dim3(1), dim3(kCholThreads), 0, s, ab, static_cast<int64_t>(ld), c0, w,
+                         dinfo + k);
+      FX_LAUNCHED("chol_block_kernel");
     }
     if (hh <= 0) continue;
     st = rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
@@ -482,16 +641,19 @@ int fx_band_solve_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, int32_t
   const int unit = (flags & FX_BAND_LDLT) ? 1 : 0;
   hipLaunchKernelGGL(permute_kernel, dim3(gv), dim3(kThreads), 0, s, n, perm, b, y, 0);
   // L y = P b
+  double* cp = y + ((static_cast<size_t>(n) + 31) & ~static_cast<size_t>(31));  // coupling partials
   const int nsb = (n + kSolveNb - 1) / kSolveNb;
-  for (int k = -1; k < nsb - 1; ++k) {
-    const int cK = k < 0 ? -1 : k * kSolveNb;
-    const int wK = k < 0 ? 0 : kSolveNb;
-    const int c1 = k < 0 ? 0 : cK + wK;
+  for (int k = 0; k < nsb; ++k) {
+    const int c1 = k * kSolveNb;
     const int w1 = std::min(kSolveNb, n - c1);
-    const int rend = k < 0 ? 0 : std::min(n, cK + wK + bw);
+    const int cK = k > 0 ? c1 - kSolveNb : 0;
+    const int wK = k > 0 ? kSolveNb : 0;
+    if (k > 0)
+      hipLaunchKernelGGL(band_fwd_couple_kernel, dim3(kChunks), dim3(kThreads), 0, s, bw, ldab, ab, y, cK, c1, w1, cp);
+    const int rend = k > 0 ? std::min(n, cK + wK + bw) : 0;
     const int others = std::max(0, rend - (c1 + w1));
     hipLaunchKernelGGL(band_fwd_kernel, dim3(1 + (others + kThreads - 1) / kThreads), dim3(kThreads), 0, s, n, bw,
-                       ldab, ab, y, cK, wK, unit);
+                       ldab, ab, y, cK, wK, c1, unit, cp);
   }
   FX_LAUNCHED("fx_band_solve_f64 forward");
   if (unit) hipLaunchKernelGGL(diag_scale_kernel, dim3(gv), dim3(kThreads), 0, s, n, ldab, ab, y);
@@ -501,11 +663,16 @@ int fx_band_solve_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, int32_t
     const int wK = k < nsb ? std::min(kSolveNb, n - cK) : 0;
     const int c1 = (k - 1) * kSolveNb;
     const int w1 = std::min(kSolveNb, n - c1);
+    if (wK > 0) {
+      const int waves = (w1 + kJc - 1) / kJc;
+      hipLaunchKernelGGL(band_bwd_couple_kernel, dim3((waves + kThreads / 64 - 1) / (kThreads / 64)), dim3(kThreads),
+                         0, s, bw, ldab, ab, y, cK, wK, c1, w1, cp);
+    }
     const int jlo = std::max(0, cK - bw);
     const int others = wK > 0 ? std::max(0, c1 - jlo) : 0;
     const int wpb = kThreads / 64;
     hipLaunchKernelGGL(band_bwd_kernel, dim3(1 + (others + wpb - 1) / wpb), dim3(kThreads), 0, s, n, bw, ldab, ab, y,
-                       cK, wK, c1, w1, unit);
+                       cK, wK, c1, w1, unit, cp);
   }
   FX_LAUNCHED("fx_band_solve_f64 backward");
   hipLaunchKernelGGL(permute_kernel, dim3(gv), dim3(kThreads), 0, s, n, perm, y, x, 1);

@@ -732,7 +732,7 @@ int fx_gmres_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int3
  * fx_band_from_csr_f64: zero ab and scatter the lower triangle of the
  *   (symmetric) CSR; FX_EINVAL if an entry lies past bw.
  * fx_band_cholesky_f64: in-place L (A = L L^T), blocked right-looking
- *   (rocSOLVER potrf on 128-column diagonal blocks, rocBLAS
+ *   (a one-workgroup Cholesky of each 128-column diagonal block, rocBLAS
  *   trsm / syrk on the band below); *info = 0, or the 1-based column of the
  *   first non-positive pivot (the matrix is not positive definite).  Uses
  *   one rocBLAS handle per device, set to `stream` at each call.
