@@ -701,6 +701,14 @@ def main():
     y64 = torch.empty_like(x64)
     t_spmv64 = event_time(lambda: A64.matvec(x64, out=y64), 50)
     t_diff = event_time(lambda: Ad.matvec(x64, out=y64), 50)
+    t_sym = None
+    if Ad.upper_storage() is not None:
+        import ctypes
+        from feax_amd import _native as nat
+        sym_s, vals_s = Ad.upper_storage()
+        t_sym = event_time(lambda: nat.call("fx_csr_spmv_b3sym_f32f64", nat.stream(), n, nat.ptr(Ad.indptr),
+                                            ctypes.addressof(sym_s.struct), nat.ptr(vals_s), nat.ptr(x64),
+                                            nat.ptr(y64)), 50)
     # Algorithmic bytes per launch: nse (s_v + 4) + 4 (n + 1) + 2 n s_x.
     B64 = nse * (8 + 4) + 4 * (n + 1) + 2 * n * 8
     Bd = nse * (4 + 4) + 4 * (n + 1) + 2 * n * 8
@@ -710,9 +718,24 @@ def main():
     Bmind = nse * 4 + (nse // 9) * 4 + 4 * (n // 3 + 1) + 2 * n * 8
     t_asm64 = event_time(lambda: assemble_csr(problem, ts, tp, None, bc, True, torch.float64), 5)
 
+    # The mixed form's SpMV on upper-block storage (fx_pcg_b3sym_f32f64):
+    # its algorithmic bytes are what that kernel must move -- the stored
+    # upper blocks (9 FP32 each), one 16-bit code per block of the pattern,
+    # the row pointers and upper-list starts per node, the workgroups'
+    # column lists, x and y.
+    Ad_up = Ad.upper_storage()
+    if Ad_up is not None:
+        sym_t, _ = Ad_up
+        nu_cols = int(sym_t.tensors[2].numel())
+        Bsym = (sym_t.num_upper * 36 + (nse // 9) * 2 + 4 * (n // 3 + 1) * 2 + 4 * (nu_cols + len(sym_t.tensors[1]))
+                + 2 * n * 8)
     headline = "mixed"
-    dom = dict(kernel="pcg_spmv_map_kernel, FP32 difference form, FP64 vectors, x staged in LDS (f32f64/diff3)",
-               label="f32f64/diff3", B=Bd)
+    if "f32f64/sym3" in spmv_live and Ad_up is not None:
+        dom = dict(kernel="spmv_sym_kernel, FP32 difference form on upper-block storage, FP64 vectors "
+                          "(f32f64/sym3)", label="f32f64/sym3", B=Bsym)
+    else:
+        dom = dict(kernel="pcg_spmv_map_kernel, FP32 difference form, FP64 vectors, x staged in LDS "
+                          "(f32f64/diff3)", label="f32f64/diff3", B=Bd)
     if results["fp64"]["s"] < results["mixed"]["s"]:
         headline = "fp64"
         dom = dict(kernel="pcg_spmv_kernel, FP64 block-3 (f64/block3)", label="f64/block3", B=B64)
@@ -744,8 +767,15 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(achieved / 1e9, 1),
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                 "traffic": traffic, "kernel_us": round(dom["t"] * 1e6, 2), "alg_bytes": dom["B"],
-                "achieved_def": "alg_bytes (B_spmv = nse (s_v + 4) + 4 (n + 1) + 2 n s_x) / kernel_us (HIP events "
-                                "around every SpMV launch of one in-solve run)"}
+                "achieved_def": ("alg_bytes (upper blocks 36 num_upper + 2 B code per block + per-node row and "
+                                 "upper-list starts + workgroup column lists + x and y) / kernel_us"
+                                 if dom["label"] == "f32f64/sym3" else
+                                 "alg_bytes (B_spmv = nse (s_v + 4) + 4 (n + 1) + 2 n s_x) / kernel_us") +
+                                " (HIP events around every SpMV launch of one in-solve run)"}
+    if dom["label"] == "f32f64/sym3":
+        # the same operator as the full-storage CSR SpMV: feax's CSR formula
+        # bytes over this kernel's time, for comparison with that kernel
+        roofline["csr_formula_equiv_GBps"] = round(Bd / dom["t"] / 1e9, 1)
     if traffic is not None:
         # The HBM bytes the kernel moves (rocprof FETCH_SIZE + WRITE_SIZE):
         # the block-3 kernel reads one column index per 3x3 block, so it
@@ -779,7 +809,9 @@ def main():
                  "diff3_us": round(t_diff * 1e6, 2), "diff3_alg_GBps": round(Bd / t_diff / 1e9, 1),
                  "in_solve_us": {k: round(v[0], 2) for k, v in spmv_live.items()},
                  "in_solve_launches": {k: v[1] for k, v in spmv_live.items()},
-                 "B_alg_fp64": B64, "B_alg_diff3": Bd, "B_min_fp64": Bmin64, "B_min_diff3": Bmind},
+                 "B_alg_fp64": B64, "B_alg_diff3": Bd, "B_min_fp64": Bmin64, "B_min_diff3": Bmind,
+                 "sym3_us": None if t_sym is None else round(t_sym * 1e6, 2),
+                 "B_alg_sym3": None if Ad_up is None else Bsym},
         "assemble_fp64_ms": round(t_asm64 * 1e3, 3),
         "direct": direct,
         "grad_solve_ratio": None if grad is None else grad["ratio_adjoint"],

@@ -53,6 +53,24 @@ class FxB3Map(ctypes.Structure):
                 ("nodes_per_block", ctypes.c_int32), ("max_unique", ctypes.c_int32)]
 
 
+class FxB3Sym(ctypes.Structure):
+    """fx_b3sym: upper-block storage map of a symmetric block-3 pattern (device pointers)."""
+    _fields_ = [("code", ctypes.c_void_p), ("uptr", ctypes.c_void_p), ("ucol", ctypes.c_void_p),
+                ("up_ptr", ctypes.c_void_p), ("nodes_per_block", ctypes.c_int32), ("max_unique", ctypes.c_int32),
+                ("max_degree", ctypes.c_int32), ("num_upper", ctypes.c_int32)]
+
+
+class B3Sym:
+    """An FxB3Sym and the device tensors (code, uptr, ucol, up_ptr) it points into."""
+
+    def __init__(self, tensors, nodes_per_block, max_unique, max_degree, num_upper):
+        self.tensors = tensors
+        code, uptr, ucol, up_ptr = tensors
+        self.num_upper = int(num_upper)
+        self.struct = FxB3Sym(code.data_ptr(), uptr.data_ptr(), ucol.data_ptr(), up_ptr.data_ptr(),
+                              int(nodes_per_block), int(max_unique), int(max_degree), int(num_upper))
+
+
 class B3Map:
     """An FxB3Map and the device tensors (lidx, uptr, ucol) it points into."""
 
@@ -115,6 +133,10 @@ SIGNATURES = {
     "fx_csr_spmv_b3_f64": (_I32, [_P, _I32, _I64, _P, _P, _P, _P, _P]),
     "fx_csr_spmv_b3_f32": (_I32, [_P, _I32, _I64, _P, _P, _P, _P, _P]),
     "fx_csr_spmv_diff3_f32f64": (_I32, [_P, _I32, _I64, _P, _P, _P, _P, _P]),
+    "fx_b3sym_compact_f32": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P]),
+    "fx_csr_spmv_b3sym_f32f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P]),
+    "fx_pcg_b3sym_f32f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P, _P,
+                                   _P, _P, _I32, _P, _SZ]),
     "fx_csr_jacobi_f64": (_I32, [_P, _I32, _P, _P, _P, _F64, _P]),
     "fx_csr_jacobi_f32": (_I32, [_P, _I32, _P, _P, _P, _F64, _P]),
     "fx_csr_diagonal_f64": (_I32, [_P, _I32, _P, _P, _P, _P]),

@@ -116,6 +116,26 @@ class TracedStructure:
         cache[G] = m
         return m
 
+    def b3sym(self):
+        """Upper-block storage map (nat.B3Sym) of the block-3 pattern for the
+        symmetric difference-form SpMV (fx_pcg_b3sym_f32f64), or None when the
+        pattern does not fit its kernels (upper_block_tables).  Built once."""
+        import torch
+        if "_b3sym" in self.__dict__:
+            return self._b3sym
+        self._b3sym = None
+        if self.vec != 3 or self.n % 3 or self.n == 0:
+            return None
+        tab = upper_block_tables(self.indptr.cpu().numpy(), self.bcol().cpu().numpy())
+        if tab is not None:
+            dev = self.device
+            t = (torch.as_tensor(tab["code"].view(np.int16), device=dev),
+                 torch.as_tensor(tab["uptr"], device=dev), torch.as_tensor(tab["ucol"], device=dev),
+                 torch.as_tensor(tab["up_ptr"], device=dev))
+            self._b3sym = nat.B3Sym(t, tab["nodes_per_block"], tab["max_unique"], tab["max_degree"],
+                                    tab["num_upper"])
+        return self._b3sym
+
     def bcol(self):
         """Column node of each 3x3 block of the CSR pattern (nse/9 int32)."""
         import torch
@@ -129,6 +149,61 @@ class TracedStructure:
     @classmethod
     def from_problem(cls, problem, device="cuda", free_scratch=False):
         return cls(problem, device)
+
+
+def upper_block_tables(indptr, bcol, nodes_per_block=64):
+    """Host tables of the upper-block storage (fx_b3sym) of a block-3
+    pattern: the symmetric difference-form SpMV reads node n's blocks with
+    column c >= n from its own contiguous list (from block up_ptr[n]) and a
+    lower block (n, c) as the transpose of the stored block (c, n).
+
+    Returns None when the pattern does not fit the kernels: not symmetric, a
+    node without its diagonal block, a node degree above 32, or more than
+    1023 unique column nodes per workgroup of `nodes_per_block` nodes.
+    Otherwise a dict: code (uint16 per block, bcol order:
+    (l << 6) | (lower << 5) | r, l = the column node's position in the
+    workgroup's sorted unique column list, r = the stored block's position in
+    its owner's upper list -- the column node's for a lower block, else the
+    row node's), uptr / ucol (the workgroups' unique column lists), up_ptr
+    (N + 1), num_upper, max_unique, max_degree, nodes_per_block."""
+    G = int(nodes_per_block)
+    ip = np.asarray(indptr, np.int64)
+    bc = np.asarray(bcol, np.int64)
+    N = (len(ip) - 1) // 3
+    if N == 0:
+        return None
+    bstart = ip[3 * np.arange(N)] // 9
+    deg = (ip[3 * np.arange(N) + 1] - ip[3 * np.arange(N)]) // 3
+    if deg.max() > 32 or len(bc) != int(deg.sum()):
+        return None
+    node = np.repeat(np.arange(N, dtype=np.int64), deg)
+    lower = bc < node
+    if np.count_nonzero(bc == node) != N:
+        return None                                   # a node without its diagonal block
+    nlow = np.bincount(node[lower], minlength=N)
+    up_ptr = np.zeros(N + 1, np.int64)
+    up_ptr[1:] = np.cumsum(deg - nlow)
+    keys = node * N + bc                              # ascending: nodes, then sorted columns
+    want = bc * N + node
+    mirror = np.searchsorted(keys, want)
+    if np.any(mirror >= len(keys)) or np.any(keys[np.minimum(mirror, len(keys) - 1)] != want):
+        return None                                   # pattern not symmetric
+    kb = np.arange(len(bc)) - bstart[node]
+    r = np.where(lower, mirror - bstart[bc] - nlow[bc], kb - nlow[node])
+    blk = node // G
+    uniq, inv = np.unique(blk * N + bc, return_inverse=True)
+    nblk = (N + G - 1) // G
+    uptr = np.zeros(nblk + 1, np.int64)
+    np.add.at(uptr, uniq // N + 1, 1)
+    uptr = np.cumsum(uptr)
+    lidx = inv.reshape(-1) - uptr[blk]
+    maxu = int(np.diff(uptr).max())
+    if maxu > 1023 or r.max() > 31:
+        return None
+    code = ((lidx << 6) | (lower.astype(np.int64) << 5) | r).astype(np.uint16)
+    return dict(code=code, uptr=uptr.astype(np.int32), ucol=(uniq % N).astype(np.int32),
+                up_ptr=up_ptr.astype(np.int32), num_upper=int(up_ptr[-1]), max_unique=maxu,
+                max_degree=int(deg.max()), nodes_per_block=G)
 
 
 def _as_dev(a, ts, dtype=None):
@@ -290,6 +365,7 @@ def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, 
                   bcol=ts.bcol() if b3 else None)
     if b3:
         A.b3map = ts.b3map
+        A.b3sym = ts.b3sym
     return A if jacobi_shift is None else (A, dinv)
 
 

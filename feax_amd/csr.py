@@ -20,6 +20,8 @@ Two extras for HEX8 vec = 3 patterns:
   CSR value array.
 """
 
+import ctypes
+
 import numpy as np
 
 from . import _native as nat
@@ -37,6 +39,10 @@ class CSRMatrix:
         # callable(value_bytes) -> block-column map of the pattern (or None),
         # set by the assembly for block-3 matrices; used by the PCG SpMV
         self.b3map = None
+        # callable() -> upper-block storage map of the pattern (or None), set
+        # by the assembly for block-3 matrices (see upper_storage)
+        self.b3sym = None
+        self._upper = None
 
     @property
     def nse(self):
@@ -51,8 +57,10 @@ class CSRMatrix:
         return 2
 
     def with_data(self, data, form=None):
-        return CSRMatrix(data, self.indptr, self.indices, self.shape, self.block3,
-                         self.form if form is None else form, self._bcol)
+        A = CSRMatrix(data, self.indptr, self.indices, self.shape, self.block3,
+                      self.form if form is None else form, self._bcol)
+        A.b3map, A.b3sym = self.b3map, self.b3sym
+        return A
 
     @property
     def bcol(self):
@@ -66,6 +74,33 @@ class CSRMatrix:
                      nat.ptr(self.indices), nat.ptr(b))
             self._bcol = b
         return self._bcol
+
+    def upper_storage(self):
+        """(nat.B3Sym, upper values) of a difference-form matrix that is
+        bitwise symmetric, else None.  The upper 3x3 blocks are copied once
+        into the compact storage of fx_b3sym_compact_f32 (about half the
+        values), which also checks every lower block against the transpose of
+        its mirror; the PCG then runs its SpMV on the compact copy
+        (fx_pcg_b3sym_f32f64): the same operator from about half the bytes.
+        A matrix that is not exactly symmetric (row-only Dirichlet
+        elimination, a non-symmetric tangent) or a pattern the kernels do not
+        take keeps the full storage.  Built once per matrix (one host sync)."""
+        import torch
+        if self.form != "diff3" or self.b3sym is None:
+            return None
+        if self._upper is None:
+            self._upper = False
+            sym = self.b3sym()
+            if sym is not None:
+                dev = self.data.device
+                upper = torch.empty(9 * sym.num_upper, dtype=torch.float32, device=dev)
+                bad = torch.empty(2, dtype=torch.int32, device=dev)
+                nat.call("fx_b3sym_compact_f32", nat.stream(), self.shape[0], nat.ptr(self.indptr),
+                         nat.ptr(self.bcol), ctypes.addressof(sym.struct), nat.ptr(self.data), nat.ptr(upper),
+                         nat.ptr(bad))
+                if int(bad.abs().sum().item()) == 0:
+                    self._upper = (sym, upper)
+        return self._upper or None
 
     def _sfx(self):
         import torch

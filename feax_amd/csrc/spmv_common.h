@@ -149,4 +149,110 @@ __device__ __forceinline__ double b3map_rows(int first, int end, const int* __re
   return dot;
 }
 
+// Upper-block storage of a bitwise-symmetric block-3 difference-form
+// matrix (fx_b3sym): node n's blocks with column node c >= n (its own S
+// block first, then its upper neighbours in column order) lie contiguously,
+// nine FP32 values each (row-major), from block up_ptr[n]; a lower block
+// (n, c), c < n, is read as the transpose of the stored block (c, n).
+// code[k] for the k-th block of the CSR pattern (bcol order):
+// (l << 6) | (lower << 5) | r, with l the position of the column node in
+// the workgroup's unique column list and r the position of the stored block
+// in its owner's upper list (owner = c for a lower block, else n).
+//
+// One workgroup of BS threads runs G = BS / L consecutive nodes (workgroup
+// index b; the caller deals b to XCDs in contiguous ranges, so that the
+// upper blocks a lower block reads were fetched into the same L2 by the
+// workgroups just before).  Lane gl of a node takes blocks gl + j L,
+// j < U (degree <= U L, checked by the entry point): codes and upper
+// blocks are loaded before the LDS staging of x, lower blocks after it
+// (their owner's first block comes from LDS).  Returns this thread's share
+// of p . q (first lane of a node).  Contains a __syncthreads.
+template <int L, int BS, int U>
+__device__ __forceinline__ double b3sym_rows(int b, int nnodes, const int* __restrict__ indptr,
+                                             const uint16_t* __restrict__ code, const int* __restrict__ uptr,
+                                             const int* __restrict__ ucol, const int* __restrict__ up_ptr,
+                                             const float* __restrict__ vals, const double* __restrict__ p,
+                                             double* __restrict__ q, double* xs, int* sup, int* own) {
+  constexpr int G = BS / L;
+  const int first = b * G;
+  const int end = min(first + G, nnodes);
+  const int u0 = uptr[b];
+  const int nu = uptr[b + 1] - u0;
+  const int gl = threadIdx.x % L;
+  const int g = first + threadIdx.x / L;
+  const bool act = g < end;
+  int r0 = 0, nb = 0, base = 0;
+  if (act) {
+    r0 = indptr[3 * g];
+    nb = (indptr[3 * g + 1] - r0) / 3;
+    base = up_ptr[g];
+  }
+  int w[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    const int kb = gl + j * L;
+    w[j] = kb < nb ? static_cast<int>(code[r0 / 9 + kb]) : -1;
+  }
+  float v[U][9];
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    if (w[j] >= 0 && !(w[j] & 32)) {
+      const float* bp = vals + 9 * static_cast<size_t>(base + (w[j] & 31));
+#pragma unroll
+      for (int t = 0; t < 9; ++t) v[j][t] = bp[t];
+    }
+  }
+  for (int u = threadIdx.x; u < nu; u += BS) {
+    const int c = ucol[u0 + u];
+    xs[3 * u] = p[3 * c];
+    xs[3 * u + 1] = p[3 * c + 1];
+    xs[3 * u + 2] = p[3 * c + 2];
+    sup[u] = up_ptr[c];
+    if (c >= first && c < end) own[c - first] = u;
+  }
+  __syncthreads();
+  double dot = 0.0;
+  if (act) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (w[j] >= 0 && (w[j] & 32)) {
+        const float* bp = vals + 9 * static_cast<size_t>(sup[w[j] >> 6] + (w[j] & 31));
+#pragma unroll
+        for (int t = 0; t < 9; ++t) v[j][t] = bp[t];
+      }
+    }
+    const int lo = own[g - first];
+    const double xs0 = xs[3 * lo], xs1 = xs[3 * lo + 1], xs2 = xs[3 * lo + 2];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (w[j] >= 0) {
+        const int l = w[j] >> 6;
+        double x0 = xs[3 * l], x1 = xs[3 * l + 1], x2 = xs[3 * l + 2];
+        if (l != lo) {
+          x0 -= xs0;
+          x1 -= xs1;
+          x2 -= xs2;
+        }
+        const bool low = (w[j] & 32) != 0;
+        const float* e = v[j];
+        const float e01 = low ? e[3] : e[1], e10 = low ? e[1] : e[3];
+        const float e02 = low ? e[6] : e[2], e20 = low ? e[2] : e[6];
+        const float e12 = low ? e[7] : e[5], e21 = low ? e[5] : e[7];
+        a0 += static_cast<double>(e[0]) * x0 + static_cast<double>(e01) * x1 + static_cast<double>(e02) * x2;
+        a1 += static_cast<double>(e10) * x0 + static_cast<double>(e[4]) * x1 + static_cast<double>(e12) * x2;
+        a2 += static_cast<double>(e20) * x0 + static_cast<double>(e21) * x1 + static_cast<double>(e[8]) * x2;
+      }
+    }
+    const double y0 = group_sum<double, L>(a0), y1 = group_sum<double, L>(a1), y2 = group_sum<double, L>(a2);
+    if (gl == 0) {
+      q[3 * g] = y0;
+      q[3 * g + 1] = y1;
+      q[3 * g + 2] = y2;
+      dot = xs0 * y0 + xs1 * y1 + xs2 * y2;
+    }
+  }
+  return dot;
+}
+
 }  // namespace fx

@@ -1,0 +1,146 @@
+"""Upper-block storage of the FP32 difference form (fx_b3sym_compact_f32,
+fx_csr_spmv_b3sym_f32f64, fx_pcg_b3sym_f32f64): the mixed form's PCG runs its
+SpMV on the upper 3x3 blocks only, reading each lower block as the transpose
+of its mirror.  Checked against the full difference-form kernel (same
+operator, different summation grouping: FP64 rounding only), against the
+block-column-map PCG, and for the refusal of a non-symmetric matrix."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import cantilever
+
+pytestmark = pytest.mark.gpu
+
+EPS64 = np.finfo(np.float64).eps
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _distorted_problem(dims=(9, 5, 4), seed=8):
+    """A distorted, renumbered box (general upper / lower block structure)."""
+    import torch
+    import feax_amd as fe
+    rng = np.random.default_rng(seed)
+    mesh = fe.StructuredGrid(dims, spacing=(1.0, 1.0, 1.0)).to_mesh()
+    pts = mesh.points + rng.uniform(-0.1, 0.1, mesh.points.shape)
+    perm = rng.permutation(len(pts))
+    cells = np.argsort(perm)[mesh.cells][rng.permutation(len(mesh.cells))]
+    pts = pts[perm]
+    L = float(dims[0])
+
+    class P(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.IsotropicLinearElastic(nu=0.3)
+
+        def get_surface_maps(self):
+            return [fe.materials.SurfaceTraction((0.0, 0.0, 1.0))]
+
+    prob = P(fe.Mesh(pts, cells), vec=3, dim=3, location_fns=[lambda p: np.isclose(p[0], L, atol=0.2)])
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: p[0] < 0.2, "all", 0.0)]).create_bc(prob)
+    ts = fe.TracedStructure.from_problem(prob)
+    E = torch.as_tensor(rng.uniform(1.0, 2.0, prob.num_cells), device="cuda")
+    tp = fe.TracedParams(volume_vars=(E,), surface_vars=[(fe.TracedParams.create_uniform_surface_var(prob, 0.1),)])
+    return prob, bc, ts, tp
+
+
+def test_sym_spmv_equals_the_difference_form_kernel():
+    import torch
+    from feax_amd import _native as nat
+    from feax_amd.assembler import assemble_csr
+    prob, bc, ts, tp = _distorted_problem()
+    print("case: b3sym spmv, distorted 9x5x4", flush=True)
+    A, _ = assemble_csr(prob, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12)
+    A64 = assemble_csr(prob, ts, tp, None, bc, True, torch.float64)
+    up = A.upper_storage()
+    assert up is not None
+    sym, vals = up
+    nbl = A.nse // 9
+    assert sym.num_upper * 2 > nbl and sym.num_upper < nbl          # about half the blocks
+    rng = np.random.default_rng(3)
+    x = torch.as_tensor(1.0 + 1e-2 * rng.standard_normal(ts.n), device="cuda")
+    y_full = A.matvec(x)
+    y_sym = torch.empty_like(y_full)
+    nat.call("fx_csr_spmv_b3sym_f32f64", nat.stream(), ts.n, nat.ptr(A.indptr), ctypes.addressof(sym.struct),
+             nat.ptr(vals), nat.ptr(x), nat.ptr(y_sym))
+    import scipy.sparse as sp
+    K = sp.csr_matrix((_np(A64.data), _np(A64.indices), _np(A64.indptr)), shape=(ts.n, ts.n))
+    scale = abs(K) @ np.abs(_np(x))
+    err = np.abs(_np(y_sym) - _np(y_full))
+    print("sym vs diff3 max err / (|K||x| eps)", (err / (scale * EPS64)).max())
+    assert np.all(err <= 64 * EPS64 * scale)
+    # the compact copy is exactly the upper blocks of the full storage
+    tab_up = _np(sym.tensors[3])
+    blocks = _np(A.data).reshape(-1, 9)
+    node = np.repeat(np.arange(ts.n // 3), np.diff(_np(A.indptr)[0::3]) // 9)   # 9 deg values per node
+    upper_mask = _np(A.bcol) >= node
+    assert np.array_equal(_np(vals).reshape(-1, 9), blocks[upper_mask])
+    assert tab_up[-1] == sym.num_upper
+
+
+def test_sym_storage_refused_for_row_only_elimination():
+    import torch
+    from feax_amd import _native as nat
+    from feax_amd.assembler import assemble_csr
+    prob, bc, ts, tp = _distorted_problem()
+    print("case: b3sym refusal, row-only elimination", flush=True)
+    A, _ = assemble_csr(prob, ts, tp, None, bc, False, torch.float32, diff_form=True, jacobi_shift=1e-12)
+    sym = ts.b3sym()
+    assert sym is not None
+    upper = torch.empty(9 * sym.num_upper, dtype=torch.float32, device="cuda")
+    bad = torch.empty(2, dtype=torch.int32, device="cuda")
+    nat.call("fx_b3sym_compact_f32", nat.stream(), ts.n, nat.ptr(A.indptr), nat.ptr(A.bcol),
+             ctypes.addressof(sym.struct), nat.ptr(A.data), nat.ptr(upper), nat.ptr(bad))
+    b = _np(bad)
+    print("asymmetric blocks", b)
+    assert b[0] > 0 and b[1] == 0
+    assert A.upper_storage() is None
+
+
+def test_sym_pcg_matches_the_map_pcg_and_repeats_bitwise():
+    import torch
+    from feax_amd.assembler import assemble_csr, residual
+    from feax_amd.solvers.common import pcg
+    prob, bc, ts, tp = _distorted_problem()
+    print("case: b3sym pcg vs map pcg", flush=True)
+    A, dinv = assemble_csr(prob, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12)
+    b = residual(prob, ts, tp, torch.zeros(ts.n, dtype=torch.float64, device="cuda"), bc, negate=True)
+    x1, i1 = pcg(A, b, None, 1e-10, 0.0, 5000, dinv)                 # upper-block storage
+    x1b, i1b = pcg(A, b, None, 1e-10, 0.0, 5000, dinv)
+    x2, i2 = pcg(A, b, None, 1e-10, 0.0, 5000, dinv, use_map=True)   # full storage, block-column map
+    print("iterations sym", i1.iters, "map", i2.iters)
+    assert A.upper_storage() is not None
+    assert abs(i1.iters - i2.iters) <= max(2, 0.02 * i2.iters)
+    assert i1.iters == i1b.iters and _np(x1).tobytes() == _np(x1b).tobytes()
+    d = np.abs(_np(x1) - _np(x2)).max() / np.abs(_np(x2)).max()
+    print("rel diff", d)
+    assert d < 1e-8
+
+
+def test_mixed_solver_runs_the_sym_kernel_and_matches_direct():
+    import scipy.sparse.linalg as spl
+    import feax_amd as fe
+    from oracle import hex8, krylov
+    from conftest import E_MOD, NU, TRACTION, oracle_model
+    from feax_amd.solvers.common import SpmvTimer
+    problem, bc, tp = cantilever((50, 5, 5), (2.0, 2.0, 2.0), 100.0)
+    print("case: mixed solver on b3sym, cantilever 50x5x5", flush=True)
+    opts = fe.KrylovSolverOptions(solver="cg", tol=1e-8, atol=0.0, maxiter=20000, use_jacobi_preconditioner=True,
+                                  precision="mixed")
+    solver = fe.create_solver(problem, bc, opts, linear=True, traced_params=tp)
+    with SpmvTimer() as t:
+        u = np.asarray(solver(tp, fe.zero_like_initial_guess(problem, bc)))
+    assert "f32f64/sym3" in t.kernels and t.kernels["f32f64/sym3"][1] > 0
+    # the oracle's FP64 system, solved directly
+    m = oracle_model(problem)
+    mat = hex8.Material("linear_elastic", nu=NU)
+    tr = [hex8.constant_traction(problem.boundary_inds_list[0], TRACTION)]
+    _, info = krylov.linear_solve(m, mat, np.full(m.N, E_MOD), tr, bc.bc_rows, bc.bc_vals, tol=1e-6, atol=0.0)
+    ud = spl.spsolve(info["K"].tocsc(), info["b"])
+    err = np.linalg.norm(u - ud) / np.linalg.norm(ud)
+    print("mixed (sym) vs direct", err, solver.stats)
+    assert solver.stats.relres <= 1e-8 and err < 1e-6
