@@ -310,8 +310,9 @@ __global__ __launch_bounds__(BS) void pcg_spmv_map_kernel(int n, const int* __re
 constexpr int kSymLanes = 8, kSymThreads = 512, kSymPasses = 4;
 constexpr int kSymNodes = kSymThreads / kSymLanes;
 
+// Seven or more waves per SIMD (at most 72 VGPRs): the kernel is latency-bound.
 template <bool PCG>
-__global__ __launch_bounds__(kSymThreads) void spmv_sym_kernel(int n, int max_unique, const int* __restrict__ indptr,
+__global__ __launch_bounds__(kSymThreads) __attribute__((amdgpu_waves_per_eu(7, 8))) void spmv_sym_kernel(int n, int max_unique, const int* __restrict__ indptr,
                                                                fx_b3sym sym, const float* __restrict__ vals,
                                                                const double* __restrict__ p, double* __restrict__ q,
                                                                const PcgState* __restrict__ st, double* partials) {
@@ -393,7 +394,12 @@ __global__ __launch_bounds__(kSymThreads) void sym_compact_kernel(int nnodes, co
 // g[(k + 1) % 2] while its other blocks still read g[k % 2].
 constexpr int kUpdBlocks = 256;  // one block per CU (384 / 512 / 1024 measured 1.4-2x slower)
 constexpr int kPre = 4;  // rows per thread loaded ahead of the partial sums (1M rows: 3.9 per thread)
-inline int upd_grid(int n) { return std::max(1, std::min(kUpdBlocks, (n + kFinBlock - 1) / kFinBlock)); }
+inline int upd_grid(int n) {
+  // Tuning override (FEAXHIP_UPD_BLOCKS, 64..2048), read once.
+  static const int ov = getenv("FEAXHIP_UPD_BLOCKS") ? atoi(getenv("FEAXHIP_UPD_BLOCKS")) : 0;
+  const int nb = (ov >= 64 && ov <= kMaxGridBlocks) ? ov : kUpdBlocks;
+  return std::max(1, std::min(nb, (n + kFinBlock - 1) / kFinBlock));
+}
 
 __device__ __forceinline__ Range fin_block_range(int n, int& lblock) {
   lblock = xcd_block(blockIdx.x, gridDim.x);

@@ -163,10 +163,13 @@ __device__ __forceinline__ double b3map_rows(int first, int end, const int* __re
 // index b; the caller deals b to XCDs in contiguous ranges, so that the
 // upper blocks a lower block reads were fetched into the same L2 by the
 // workgroups just before).  Lane gl of a node takes blocks gl + j L,
-// j < U (degree <= U L, checked by the entry point): codes and upper
-// blocks are loaded before the LDS staging of x, lower blocks after it
-// (their owner's first block comes from LDS).  Returns this thread's share
-// of p . q (first lane of a node).  Contains a __syncthreads.
+// j < U (degree <= U L, checked by the entry point).  Load order (the
+// kernel is latency-bound, so every independent chain starts first): the
+// column list entries, the staged x and upper-list starts of the first BS
+// columns into registers, the codes, the upper blocks; then the LDS writes
+// and the barrier; then the lower blocks (their owner's first block comes
+// from LDS).  Returns this thread's share of p . q (first lane of a node).
+// Contains a __syncthreads.
 template <int L, int BS, int U>
 __device__ __forceinline__ double b3sym_rows(int b, int nnodes, const int* __restrict__ indptr,
                                              const uint16_t* __restrict__ code, const int* __restrict__ uptr,
@@ -178,6 +181,9 @@ __device__ __forceinline__ double b3sym_rows(int b, int nnodes, const int* __res
   const int end = min(first + G, nnodes);
   const int u0 = uptr[b];
   const int nu = uptr[b + 1] - u0;
+  const bool st0 = static_cast<int>(threadIdx.x) < nu;
+  int cu = 0;
+  if (st0) cu = ucol[u0 + threadIdx.x];
   const int gl = threadIdx.x % L;
   const int g = first + threadIdx.x / L;
   const bool act = g < end;
@@ -186,6 +192,14 @@ __device__ __forceinline__ double b3sym_rows(int b, int nnodes, const int* __res
     r0 = indptr[3 * g];
     nb = (indptr[3 * g + 1] - r0) / 3;
     base = up_ptr[g];
+  }
+  double sx0 = 0.0, sx1 = 0.0, sx2 = 0.0;
+  int sb = 0;
+  if (st0) {
+    sx0 = p[3 * cu];
+    sx1 = p[3 * cu + 1];
+    sx2 = p[3 * cu + 2];
+    sb = up_ptr[cu];
   }
   int w[U];
 #pragma unroll
@@ -202,7 +216,14 @@ __device__ __forceinline__ double b3sym_rows(int b, int nnodes, const int* __res
       for (int t = 0; t < 9; ++t) v[j][t] = bp[t];
     }
   }
-  for (int u = threadIdx.x; u < nu; u += BS) {
+  if (st0) {
+    xs[3 * threadIdx.x] = sx0;
+    xs[3 * threadIdx.x + 1] = sx1;
+    xs[3 * threadIdx.x + 2] = sx2;
+    sup[threadIdx.x] = sb;
+    if (cu >= first && cu < end) own[cu - first] = threadIdx.x;
+  }
+  for (int u = threadIdx.x + BS; u < nu; u += BS) {
     const int c = ucol[u0 + u];
     xs[3 * u] = p[3 * c];
     xs[3 * u + 1] = p[3 * c + 1];
