@@ -29,6 +29,7 @@ FX_GMG_F32 = 32
 FX_BAND_LDLT = 1
 FX_ASM_DIFF3 = 1
 FX_ASM_COMPUTE32 = 2
+FX_ASM_UPPER = 4
 
 
 class FxMaterial(ctypes.Structure):
@@ -163,9 +164,9 @@ SIGNATURES = {
     "fx_axpby_f64": (_I32, [_P, _I32, _F64, _P, _F64, _P, _P]),
     "fx_bc_lift_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P]),
     "fx_hex8_assemble_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                                    _I32, _I32, _F64, _P, _P, _P, _SZ]),
+                                    _I32, _I32, _F64, _P, _P, _P, _SZ, _P, _P]),
     "fx_hex8_assemble_f32": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                                    _I32, _I32, _F64, _P, _P, _P, _SZ]),
+                                    _I32, _I32, _F64, _P, _P, _P, _SZ, _P, _P]),
     "fx_hex8_assemble_workspace_bytes": (_SZ, [_I32, _I32, _I32]),
     "fx_hex8_residual_workspace_bytes": (_SZ, [_I32]),
     "fx_hex8_residual_f64": (_I32, [_P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P,

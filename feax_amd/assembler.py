@@ -78,6 +78,12 @@ class TracedStructure:
                     raise ValueError(f"surface map direction {d} must have {self.vec} components here")
                 fdir[self.face_offsets[i]:self.face_offsets[i + 1]] = d
         self.fdir = torch.as_tensor(fdir.reshape(-1), **f64)
+        # An optional order for the HEX8 per-node gather (morton_order: each
+        # XCD then gathers a compact region and an element's workspace stays
+        # in its L2 for all 8 readers; it halves the gather's HBM reads at 1M
+        # DOF but measured 6 % slower, the kernel being bound by its index
+        # arithmetic).  Results do not depend on it; None = natural order.
+        self.node_order = None
         self._cache = {}
         self._bcol = None
 
@@ -149,6 +155,24 @@ class TracedStructure:
     @classmethod
     def from_problem(cls, problem, device="cuda", free_scratch=False):
         return cls(problem, device)
+
+
+def morton_order(points):
+    """Node indices sorted along a Z-order (Morton) curve of the coordinates
+    (one scale for all axes, 21 bits per axis in 3-D, 31 in 2-D)."""
+    P = np.asarray(points, np.float64)
+    if len(P) == 0:
+        return np.zeros(0, np.int32)
+    d = P.shape[1]
+    bits = 21 if d == 3 else 31
+    lo = P.min(axis=0)
+    span = max(float((P.max(axis=0) - lo).max()), 1e-300)
+    q = np.minimum(((P - lo) / span * (2 ** bits - 1)).astype(np.int64), 2 ** bits - 1)
+    key = np.zeros(len(P), np.int64)
+    for b in range(bits):
+        for k in range(d):
+            key |= ((q[:, k] >> b) & 1) << (d * b + (d - 1 - k))
+    return np.argsort(key, kind="stable").astype(np.int32)
 
 
 def upper_block_tables(indptr, bcol, nodes_per_block=64):
@@ -336,32 +360,55 @@ def elem_workspace(ts, nbytes):
     return ws.data_ptr(), ws.numel()
 
 
+def upper_assembly_ok(ts, mat, symmetric, ws_bytes):
+    """Whether the FP32 difference form can be assembled straight into the
+    upper-block storage (FX_ASM_UPPER): a HEX8 vec = 3 mesh whose pattern has
+    the upper-block tables, a linear law (its element matrices are exactly
+    symmetric, so the eliminated matrix is bitwise symmetric), symmetric
+    elimination and an element workspace."""
+    return (ts.tag == "hex8" and ts.vec == 3 and bool(symmetric) and ws_bytes > 0
+            and mat.kind in (nat.FX_MAT_LINEAR_ELASTIC, nat.FX_MAT_SIMP) and ts.b3sym() is not None)
+
+
 def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, dtype=None, diff_form=False,
-                 jacobi_shift=None):
+                 jacobi_shift=None, upper=False):
     """BC-eliminated CSR Jacobian on the device (fx_hex8_assemble_* / fx_tet4_assemble_* /
     fx_quad4_assemble_*).
 
-    diff_form: FP32 difference form for the mixed solver.  jacobi_shift:
-    also return dinv = 1 / (diag + shift) (FP64) computed in the same pass.
-    Returns the CSRMatrix, or (CSRMatrix, dinv) when jacobi_shift is given.
+    diff_form: FP32 difference form for the mixed solver.  upper: with
+    diff_form, store it as the upper-block storage of a symmetric matrix
+    (form "sym3": the operator of the difference form from about half the
+    values; used when upper_assembly_ok, else the full difference form is
+    returned).  jacobi_shift: also return dinv = 1 / (diag + shift) (FP64)
+    computed in the same pass.  Returns the CSRMatrix, or (CSRMatrix, dinv)
+    when jacobi_shift is given.
     """
     import torch
     dtype = torch.float64 if dtype is None else dtype
     mat, p = resolve_material(problem, traced_params, ts)
-    data = torch.empty(ts.nse, dtype=dtype, device=ts.device)
     mask = bc.device(ts.device)[1] if (bc is not None and len(bc.bc_rows)) else None
     u = _as_dev(sol, ts) if sol is not None else None
     dinv = torch.empty(ts.n, dtype=torch.float64, device=ts.device) if jacobi_shift is not None else None
     name = f"fx_{ts.tag}_assemble_f64" if dtype == torch.float64 else f"fx_{ts.tag}_assemble_f32"
     flags = nat.FX_ASM_DIFF3 if diff_form else 0
-    wsb = getattr(nat.lib(), f"fx_{ts.tag}_assemble_workspace_bytes")(ts.num_cells, data.element_size(), flags)
+    esize = torch.empty(0, dtype=dtype).element_size()
+    wsb = getattr(nat.lib(), f"fx_{ts.tag}_assemble_workspace_bytes")(ts.num_cells, esize, flags)
     wp, wn = elem_workspace(ts, wsb)
+    sym = None
+    if upper and diff_form and dtype == torch.float32 and upper_assembly_ok(ts, mat, symmetric, wn):
+        sym = ts.b3sym()
+        flags |= nat.FX_ASM_UPPER
+    data = torch.empty(9 * sym.num_upper if sym is not None else ts.nse, dtype=dtype, device=ts.device)
+    extra = ()
+    if ts.tag == "hex8":
+        extra = (nat.ptr(ts.node_order), nat.ptr(sym.tensors[3]) if sym is not None else None)
     nat.call(name, nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
              nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), nat.ptr(ts.rank), nat.ptr(ts.indptr),
              nat.ptr(ts.indices), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u), nat.ptr(mask), int(bool(symmetric)),
-             flags, float(jacobi_shift or 0.0), nat.ptr(dinv), nat.ptr(data), wp, wn)
+             flags, float(jacobi_shift or 0.0), nat.ptr(dinv), nat.ptr(data), wp, wn, *extra)
     b3 = ts.vec == 3
-    A = CSRMatrix(data, ts.indptr, ts.indices, (ts.n, ts.n), block3=b3, form="diff3" if diff_form else "csr",
+    form = "sym3" if sym is not None else ("diff3" if diff_form else "csr")
+    A = CSRMatrix(data, ts.indptr, ts.indices, (ts.n, ts.n), block3=b3, form=form,
                   bcol=ts.bcol() if b3 else None)
     if b3:
         A.b3map = ts.b3map

@@ -18,6 +18,12 @@ Two extras for HEX8 vec = 3 patterns:
   row-major (one contiguous run per block for the SpMV lane that reads it).
   Its ``@`` applies that operator to FP64 vectors; the array is not a plain
   CSR value array.
+* ``form == "sym3"``: the difference form of a bitwise-symmetric matrix in
+  upper-block storage (``data`` holds node n's blocks with column >= n, nine
+  FP32 values each, from block ``up_ptr[n]`` of the pattern's fx_b3sym
+  tables; a lower block is the transpose of its mirror): the same operator
+  as "diff3" from about half the values, assembled directly by the HEX8
+  FP32 assembly (FX_ASM_UPPER).
 """
 
 import ctypes
@@ -86,6 +92,8 @@ class CSRMatrix:
         elimination, a non-symmetric tangent) or a pattern the kernels do not
         take keeps the full storage.  Built once per matrix (one host sync)."""
         import torch
+        if self.form == "sym3":
+            return self.b3sym(), self.data
         if self.form != "diff3" or self.b3sym is None:
             return None
         if self._upper is None:
@@ -116,6 +124,13 @@ class CSRMatrix:
         n = self.shape[0]
         if v.shape[0] != self.shape[1]:
             raise ValueError(f"matvec: vector length {v.shape[0]} != {self.shape[1]}")
+        if self.form == "sym3":
+            v = v.to(torch.float64).contiguous()
+            y = torch.empty(n, dtype=torch.float64, device=self.data.device) if out is None else out
+            sym = self.b3sym()
+            nat.call("fx_csr_spmv_b3sym_f32f64", nat.stream(), n, nat.ptr(self.indptr), ctypes.addressof(sym.struct),
+                     nat.ptr(self.data), nat.ptr(v), nat.ptr(y))
+            return y
         if self.form == "diff3":
             v = v.to(torch.float64).contiguous()
             y = torch.empty(n, dtype=torch.float64, device=self.data.device) if out is None else out

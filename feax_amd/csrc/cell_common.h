@@ -8,6 +8,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "common.h"
+
 namespace fx {
 namespace {
 
@@ -40,7 +42,7 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
                                             const uint8_t* __restrict__ rank, const int* __restrict__ indices,
                                             const uint8_t* __restrict__ bc_mask, int symmetric, int diff_form,
                                             double jacobi_shift, double* __restrict__ dinv, T* __restrict__ data,
-                                            const int* col0 = nullptr) {
+                                            const int* col0 = nullptr, const int* __restrict__ up_ptr = nullptr) {
   constexpr int kW = (MAXDEG + 63) / 64;
   __builtin_amdgcn_wave_barrier();
   const int deg = len / V;
@@ -112,7 +114,14 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
   // Output position o of the node's V rows: CSR order (row i, block kb,
   // column d at i len + V kb + d), or, for the difference form, block
   // order (the V x V block kb contiguous, row-major, at V^2 kb + V i + d).
-  for (int o = lane; o < nval; o += 64) {
+  // up_ptr (difference form of a symmetric matrix): only the blocks from
+  // the node's own block on (columns >= nd: the lower blocks are the
+  // transposes of blocks other nodes store) go to the upper-block storage,
+  // from block up_ptr[nd] on.
+  const bool up = diff_form && up_ptr;
+  const int o_first = up ? V * V * kself : 0;
+  const long out0 = up ? static_cast<long>(V * V) * up_ptr[nd] : static_cast<long>(row0);
+  for (int o = o_first + lane; o < nval; o += 64) {
     int i, kb, d;
     if (diff_form) {
       kb = o / (V * V);
@@ -140,7 +149,7 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
       if (zr || (symmetric && zc)) v = C(0);
       if (zr && own && i == d) v = C(1);
     }
-    data[row0 + o] = static_cast<T>(v);
+    data[out0 + (o - o_first)] = static_cast<T>(v);
   }
 }
 
@@ -161,16 +170,26 @@ __host__ __device__ __forceinline__ int packed_block(int a, int b) {
 // read as the transpose of block (b, a), which equals it bit for bit.  An
 // incidence entry is (e << SHIFT) | a; nodes have at most MAXDEG
 // neighbours.
+//
+// order (nullable): the nodes in a space-filling-curve order; block b then
+// runs the nodes order[4 b' ..] of its XCD-contiguous index b', so each XCD
+// gathers one compact region of the mesh and an element's workspace, read
+// by its NPE nodes, is an L2 hit after its first reader (HEX8 at 1M DOF:
+// FETCH 2.64 -> 1.19 GB, but 811 -> 864 us: the kernel is bound by its
+// per-value index arithmetic, VALU busy about 50 %, not by HBM, so the
+// assembly passes no order by default).  up_ptr (nullable): upper-block
+// store of a symmetric difference form (finish_rows).
 template <typename T, typename C, int NPE = 8, int V = 3, int SHIFT = 3, int MAXDEG = kMaxDeg, bool PACK = false>
 __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
-    int num_nodes, const int* __restrict__ inc_ptr, const int* __restrict__ inc, const uint8_t* __restrict__ rank,
-    const int* __restrict__ indptr, const int* __restrict__ indices, const C* __restrict__ ke,
-    const uint8_t* __restrict__ bc_mask, int symmetric, int diff_form, double jacobi_shift,
-    double* __restrict__ dinv, T* __restrict__ data) {
+    int num_nodes, const int* __restrict__ order, const int* __restrict__ inc_ptr, const int* __restrict__ inc,
+    const uint8_t* __restrict__ rank, const int* __restrict__ indptr, const int* __restrict__ indices,
+    const C* __restrict__ ke, const uint8_t* __restrict__ bc_mask, int symmetric, int diff_form,
+    double jacobi_shift, double* __restrict__ dinv, T* __restrict__ data, const int* __restrict__ up_ptr) {
   __shared__ C acc_all[kNodesPerBlock][9 * MAXDEG];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int nd = blockIdx.x * kNodesPerBlock + wid;
-  if (nd >= num_nodes) return;
+  const int slot = (order ? xcd_block(blockIdx.x, gridDim.x) : static_cast<int>(blockIdx.x)) * kNodesPerBlock + wid;
+  if (slot >= num_nodes) return;
+  const int nd = order ? order[slot] : slot;
   C* acc = acc_all[wid];
   const int row0 = indptr[V * nd];
   const int len = indptr[V * nd + 1] - row0;  // V * deg
@@ -221,7 +240,7 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
     }
   }
   finish_rows<T, C, NPE, V, SHIFT, MAXDEG>(nd, lane, acc, len, row0, nval, inc_ptr, inc, rank, indices, bc_mask,
-                                          symmetric, diff_form, jacobi_shift, dinv, data, col);
+                                          symmetric, diff_form, jacobi_shift, dinv, data, col, up_ptr);
 }
 
 __global__ void bc_res_kernel(int nbc, const int* __restrict__ rows, const double* __restrict__ vals,

@@ -517,8 +517,8 @@ int h20_assemble(void* stream, int num_nodes, int num_cells, const double* x, co
   FX_LAUNCHED("h20_elem_matrix_kernel");
   hipLaunchKernelGGL((gather_rows_kernel<T, double, 20, 3, kH20Shift, kH20MaxDeg>),
                      dim3((num_nodes + kNodesPerBlock - 1) / kNodesPerBlock), dim3(64 * kNodesPerBlock), 0, s,
-                     num_nodes, inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, diff, jacobi_shift, dinv,
-                     data);
+                     num_nodes, nullptr, inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, diff,
+                     jacobi_shift, dinv, data, nullptr);
   FX_LAUNCHED("gather_rows_kernel (HEX20)");
   return FX_OK;
 }

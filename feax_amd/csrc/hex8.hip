@@ -727,10 +727,15 @@ int assemble(void* stream, int num_nodes, int num_cells, const double* x, const 
              const int* cells, const int* inc_ptr, const int* inc, const uint8_t* rank, const int* indptr,
              const int* indices, const fx_material* mat, const double* param, const double* u,
              const uint8_t* bc_mask, int symmetric, int flags, double jacobi_shift, double* dinv, T* data,
-             void* workspace, size_t ws_bytes) {
+             void* workspace, size_t ws_bytes, const int* node_order, const int* up_ptr) {
   if (num_nodes <= 0 || num_cells <= 0 || !x || !y || !z || !cells || !inc_ptr || !inc || !rank || !indptr ||
       !indices || !mat || !param || !data)
     return fail(FX_EINVAL, "fx_hex8_assemble: bad arguments");
+  const bool upper = (flags & FX_ASM_UPPER) != 0;
+  if (upper && (sizeof(T) != 4 || !(flags & FX_ASM_DIFF3) || !symmetric || !workspace || !up_ptr ||
+                mat->kind == FX_MAT_NEO_HOOKEAN))
+    return fail(FX_EINVAL, "fx_hex8_assemble: FX_ASM_UPPER stores the FP32 difference form of a linear law with "
+                           "symmetric elimination; it needs the workspace and up_ptr");
   if (mat->kind != FX_MAT_LINEAR_ELASTIC && mat->kind != FX_MAT_SIMP && mat->kind != FX_MAT_NEO_HOOKEAN)
     return fail(FX_EINVAL, "fx_hex8_assemble: unknown material %d", mat->kind);
   const bool nh = mat->kind == FX_MAT_NEO_HOOKEAN;
@@ -759,9 +764,8 @@ int assemble(void* stream, int num_nodes, int num_cells, const double* x, const 
         hipLaunchKernelGGL((elem_matrix_lin_kernel<C>), dim3(lgrid), dim3(64 * kLinWaves), 0, s, num_cells, x, y,  \
                            z, cells, mp, param, ke);                                                               \
       hipLaunchKernelGGL((gather_rows_kernel<T, C, 8, 3, 3, kMaxDeg, !NHV>), dim3(grid), dim3(64 * kNodesPerBlock), \
-                         0, s, num_nodes,                                                                          \
-                         inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, diff, jacobi_shift, dinv,    \
-                         data);                                                                                    \
+                         0, s, num_nodes, node_order, inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, \
+                         diff, jacobi_shift, dinv, data, upper ? up_ptr : nullptr);                                \
     } else {                                                                                                       \
       hipLaunchKernelGGL((assemble_kernel<T, C, NHV>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s, num_nodes, x,  \
                          y, z, cells, inc_ptr, inc, rank, indptr, indices, mp, param, u, bc_mask, symmetric, diff, \
@@ -791,9 +795,10 @@ int fx_hex8_assemble_f64(void* stream, int32_t num_nodes, int32_t num_cells, con
                          const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
                          const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
                          int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, double* data,
-                         void* workspace, size_t ws_bytes) {
+                         void* workspace, size_t ws_bytes, const int32_t* node_order, const int32_t* up_ptr) {
   return assemble<double>(stream, num_nodes, num_cells, x, y, z, cells, inc_ptr, inc, elem_rank, indptr, indices,
-                          mat, param, u, bc_mask, symmetric, flags, jacobi_shift, dinv, data, workspace, ws_bytes);
+                          mat, param, u, bc_mask, symmetric, flags, jacobi_shift, dinv, data, workspace, ws_bytes,
+                          node_order, up_ptr);
 }
 
 int fx_hex8_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,
@@ -801,9 +806,10 @@ int fx_hex8_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, con
                          const uint8_t* elem_rank, const int32_t* indptr, const int32_t* indices,
                          const fx_material* mat, const double* param, const double* u, const uint8_t* bc_mask,
                          int32_t symmetric, int32_t flags, double jacobi_shift, double* dinv, float* data,
-                         void* workspace, size_t ws_bytes) {
+                         void* workspace, size_t ws_bytes, const int32_t* node_order, const int32_t* up_ptr) {
   return assemble<float>(stream, num_nodes, num_cells, x, y, z, cells, inc_ptr, inc, elem_rank, indptr, indices,
-                         mat, param, u, bc_mask, symmetric, flags, jacobi_shift, dinv, data, workspace, ws_bytes);
+                         mat, param, u, bc_mask, symmetric, flags, jacobi_shift, dinv, data, workspace, ws_bytes,
+                         node_order, up_ptr);
 }
 
 size_t fx_hex8_assemble_workspace_bytes(int32_t num_cells, int32_t value_bytes, int32_t flags) {

@@ -281,7 +281,8 @@ int s_assemble(void* stream, int num_nodes, int num_cells, const double* x, cons
                      to_params(mat), param, ke);
   const int grid = (num_nodes + kNodesPerBlock - 1) / kNodesPerBlock;
   hipLaunchKernelGGL((gather_rows_kernel<T, double, 8, 1>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s, num_nodes,
-                     inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, 0, jacobi_shift, dinv, data);
+                     nullptr, inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, 0, jacobi_shift, dinv, data,
+                     nullptr);
   FX_LAUNCHED("fx_hex8s_assemble");
   return FX_OK;
 }

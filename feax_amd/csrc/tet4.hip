@@ -427,8 +427,8 @@ int tet_assemble(void* stream, int num_nodes, int num_cells, const double* x, co
     hipLaunchKernelGGL((tet_elem_matrix_kernel<C, NHV>), dim3(egrid), dim3(kTetThreads), 0, s, num_cells, x, y, z, \
                        cells, mp, param, u, ke);                                                                   \
     hipLaunchKernelGGL((gather_rows_kernel<T, C, 4>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s, num_nodes,      \
-                       inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, diff, jacobi_shift, dinv,      \
-                       data);                                                                                      \
+                       nullptr, inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, diff, jacobi_shift,   \
+                       dinv, data, nullptr);                                                                                      \
   } while (0)
   if (nh) {
     if (c32) FX_TET_LAUNCH(float, true);
@@ -620,8 +620,8 @@ int ts_assemble(void* stream, int num_nodes, int num_cells, const double* x, con
   hipLaunchKernelGGL(ts_elem_matrix_kernel, dim3(static_cast<int>((16L * num_cells + kTetThreads - 1) / kTetThreads)),
                      dim3(kTetThreads), 0, s, num_cells, x, y, z, cells, to_params(mat), param, ke);
   hipLaunchKernelGGL((gather_rows_kernel<T, double, 4, 1>), dim3((num_nodes + kNodesPerBlock - 1) / kNodesPerBlock),
-                     dim3(64 * kNodesPerBlock), 0, s, num_nodes, inc_ptr, inc, rank, indptr, indices, ke, bc_mask,
-                     symmetric, 0, jacobi_shift, dinv, data);
+                     dim3(64 * kNodesPerBlock), 0, s, num_nodes, nullptr, inc_ptr, inc, rank, indptr, indices, ke,
+                     bc_mask, symmetric, 0, jacobi_shift, dinv, data, nullptr);
   FX_LAUNCHED("fx_tet4s_assemble");
   return FX_OK;
 }

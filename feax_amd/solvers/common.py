@@ -86,14 +86,14 @@ def pcg(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=Non
     import torch
     n = A.shape[0]
     mdt = A.data.dtype
-    if A.form == "diff3":
+    if A.form in ("diff3", "sym3"):
         vec_dtype = torch.float64
     vdt = mdt if vec_dtype is None else vec_dtype
     b = b.to(vdt).contiguous()
     x = torch.empty(n, dtype=vdt, device=A.data.device) if out is None else out
     x0 = None if x0 is None else x0.to(vdt).contiguous()
     dinv = None if dinv is None else dinv.to(vdt).contiguous()
-    if A.form == "diff3":
+    if A.form in ("diff3", "sym3"):
         flags = nat.FX_PCG_DIFF3
     else:
         use_b3 = A.block3 if block3 is None else (bool(block3) and A.block3)
@@ -116,7 +116,8 @@ def pcg(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=Non
     # A bitwise-symmetric difference form runs its SpMV on upper-block
     # storage (half the matrix bytes; lower blocks are read transposed from
     # blocks the same XCD fetched just before): 78 -> 50 us at 1M DOF.
-    upper = A.upper_storage() if (name == "fx_pcg_f32f64" and use_map is None and flags & nat.FX_PCG_DIFF3) else None
+    upper = A.upper_storage() if (A.form == "sym3" or (
+        name == "fx_pcg_f32f64" and use_map is None and flags & nat.FX_PCG_DIFF3)) else None
     if upper is not None:
         sym, vals = upper
         nat.call("fx_pcg_b3sym_f32f64", nat.stream(), n, A.nse, flags, nat.ptr(A.indptr), ctypes.addressof(sym.struct),

@@ -134,8 +134,9 @@ def _solve_matfree(problem, ts, tp, bc, u0, opts, symmetric, stats, keep):
 def solve_mixed(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None, keep=None):
     """FP32 matrix inside an FP64 defect correction.  Returns u (FP64).
 
-    precision "mixed": FP32 difference-form CSR (FX_ASM_DIFF3) applied to
-    FP64 CG vectors.  precision "mixed-fp32vec": FP32 CSR and FP32 CG
+    precision "mixed": FP32 difference-form CSR (FX_ASM_DIFF3; in upper-block
+    storage, FX_ASM_UPPER, when the law and elimination make it symmetric)
+    applied to FP64 CG vectors.  precision "mixed-fp32vec": FP32 CSR and FP32 CG
     vectors (FP64 dot products) on the residual scaled to unit norm.
     The outer residual b - K u is always the FP64 matrix-free kernel.
     """
@@ -143,7 +144,11 @@ def solve_mixed(problem, ts, tp, bc, u0, opts, symmetric=True, stats=None, keep=
     n = ts.n
     fp32vec = opts.precision == "mixed-fp32vec"
     shift = opts.jacobi_shift if opts.use_jacobi_preconditioner else None
-    out = assemble_csr(problem, ts, tp, u0, bc, symmetric, torch.float32, diff_form=not fp32vec, jacobi_shift=shift)
+    # The difference form of a linear law with symmetric elimination goes
+    # straight into upper-block storage (half the store; its PCG SpMV reads
+    # about half the bytes); otherwise the full difference form.
+    out = assemble_csr(problem, ts, tp, u0, bc, symmetric, torch.float32, diff_form=not fp32vec, jacobi_shift=shift,
+                       upper=not fp32vec)
     A, dinv = out if shift is not None else (out, None)
     if fp32vec and dinv is not None:
         dinv = dinv.to(torch.float32)

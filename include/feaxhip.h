@@ -326,18 +326,29 @@ int fx_bc_lift_f64(void* stream, int32_t n, int32_t nbc, const int32_t* bc_rows,
 /* fx_hex8_assemble_f32: element matrices are formed in FP64 and rounded to
  * FP32 once at the store; FX_ASM_COMPUTE32 forms them in FP32 arithmetic. */
 #define FX_ASM_COMPUTE32 2
+/* FX_ASM_UPPER (fx_hex8_assemble_f32 with FX_ASM_DIFF3, a linear law,
+ * symmetric elimination and the workspace): store the difference form as
+ * the upper-block storage of fx_b3sym (data: 9 num_upper FP32, node n's
+ * blocks with column >= n from block up_ptr[n]); bitwise the array
+ * fx_b3sym_compact_f32 makes from the full difference form.
+ * node_order (d, N, nullable): the order in which the per-node gather runs
+ * the nodes (a space-filling-curve order of the mesh; the result does not
+ * depend on it).  up_ptr (d, N + 1): with FX_ASM_UPPER only. */
+#define FX_ASM_UPPER 4
 int fx_hex8_assemble_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x,
                          const double* y, const double* z, const int32_t* cells, const int32_t* inc_ptr,
                          const int32_t* inc, const uint8_t* elem_rank, const int32_t* indptr,
                          const int32_t* indices, const fx_material* mat, const double* param,
                          const double* u, const uint8_t* bc_mask, int32_t symmetric, int32_t flags,
-                         double jacobi_shift, double* dinv, double* data, void* workspace, size_t ws_bytes);
+                         double jacobi_shift, double* dinv, double* data, void* workspace, size_t ws_bytes,
+                         const int32_t* node_order, const int32_t* up_ptr);
 int fx_hex8_assemble_f32(void* stream, int32_t num_nodes, int32_t num_cells, const double* x,
                          const double* y, const double* z, const int32_t* cells, const int32_t* inc_ptr,
                          const int32_t* inc, const uint8_t* elem_rank, const int32_t* indptr,
                          const int32_t* indices, const fx_material* mat, const double* param,
                          const double* u, const uint8_t* bc_mask, int32_t symmetric, int32_t flags,
-                         double jacobi_shift, double* dinv, float* data, void* workspace, size_t ws_bytes);
+                         double jacobi_shift, double* dinv, float* data, void* workspace, size_t ws_bytes,
+                         const int32_t* node_order, const int32_t* up_ptr);
 /* 576 element-matrix values per element (the linear laws use 324 of them:
  * their element matrices are exactly symmetric and only the upper 3x3
  * block triangle is stored): value_bytes 8 (f64) or 4 (f32; FP32 workspace

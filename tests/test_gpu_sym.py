@@ -144,3 +144,53 @@ def test_mixed_solver_runs_the_sym_kernel_and_matches_direct():
     err = np.linalg.norm(u - ud) / np.linalg.norm(ud)
     print("mixed (sym) vs direct", err, solver.stats)
     assert solver.stats.relres <= 1e-8 and err < 1e-6
+
+
+@pytest.mark.parametrize("loc", ["cell", "node"])
+def test_upper_assembly_equals_the_compacted_full_difference_form(loc):
+    """FX_ASM_UPPER writes bitwise the array that the compaction of the full
+    difference form makes; the Jacobi diagonal is unchanged."""
+    import torch
+    import feax_amd as fe
+    from feax_amd.assembler import assemble_csr
+    prob, bc, ts, tp = _distorted_problem()
+    if loc == "node":
+        rng = np.random.default_rng(4)
+        tp = fe.TracedParams(volume_vars=(torch.as_tensor(rng.uniform(1.0, 2.0, prob.num_total_nodes), device="cuda"),),
+                             surface_vars=tp.surface_vars)
+    print("case: upper assembly,", loc, flush=True)
+    Au, du = assemble_csr(prob, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12, upper=True)
+    Af, df = assemble_csr(prob, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12)
+    assert Au.form == "sym3" and Af.form == "diff3"
+    sym, vals = Af.upper_storage()
+    assert Au.data.numel() == vals.numel() == 9 * sym.num_upper
+    assert torch.equal(Au.data.view(torch.int32), vals.view(torch.int32))
+    assert torch.equal(du, df)
+    x = torch.as_tensor(np.random.default_rng(2).standard_normal(ts.n), device="cuda")
+    assert torch.equal(Au.matvec(x), Au.matvec(x))
+    # row-only elimination cannot use it: the full difference form comes back
+    Ar, _ = assemble_csr(prob, ts, tp, None, bc, False, torch.float32, diff_form=True, jacobi_shift=1e-12, upper=True)
+    assert Ar.form == "diff3"
+
+
+def test_gather_node_order_does_not_change_the_matrix():
+    """The per-node gather's processing order (Morton, XCD-contiguous) is a
+    schedule only: the assembled values are bitwise those of the natural
+    order, FP64 and FP32 difference form."""
+    import torch
+    from feax_amd.assembler import assemble_csr, morton_order
+    prob, bc, ts, tp = _distorted_problem()
+    print("case: gather node order", flush=True)
+    assert ts.node_order is None
+    A1, d1 = assemble_csr(prob, ts, tp, None, bc, True, torch.float64, jacobi_shift=1e-12)
+    B1, _ = assemble_csr(prob, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12)
+    order = morton_order(prob.points)
+    assert sorted(order.tolist()) == list(range(ts.num_nodes))
+    ts.node_order = torch.as_tensor(order, device="cuda")
+    try:
+        A2, d2 = assemble_csr(prob, ts, tp, None, bc, True, torch.float64, jacobi_shift=1e-12)
+        B2, _ = assemble_csr(prob, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12)
+    finally:
+        ts.node_order = None
+    assert torch.equal(A1.data, A2.data) and torch.equal(d1, d2)
+    assert torch.equal(B1.data.view(torch.int32), B2.data.view(torch.int32))

@@ -129,3 +129,15 @@ def test_upper_block_tables_refuse_what_the_kernels_cannot_take():
     ipb, _, bcolb, _ = _block_pattern(big)
     assert upper_block_tables(ipb, bcolb, nodes_per_block=1) is not None
     assert upper_block_tables(ipb, bcolb, nodes_per_block=big.num_total_nodes) is None
+
+
+def test_morton_order_is_a_permutation_along_the_z_curve():
+    from feax_amd.assembler import morton_order
+    import feax_amd as fe
+    pts = fe.StructuredGrid((3, 3, 3), spacing=(1.0, 1.0, 1.0)).to_mesh().points
+    o = morton_order(pts)
+    assert sorted(o.tolist()) == list(range(len(pts)))
+    # the first 8 nodes on the curve are the unit cube at the origin
+    assert {tuple(p) for p in pts[o[:8]]} == {(x, y, z) for x in (0., 1.) for y in (0., 1.) for z in (0., 1.)}
+    q = np.array([[0.0, 0.0], [1.0, 1.0], [1.0, 0.0], [0.0, 1.0]])
+    assert morton_order(q).tolist() == [0, 3, 2, 1]
