@@ -121,22 +121,17 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
   const bool up = diff_form && up_ptr;
   const int o_first = up ? V * V * kself : 0;
   const long out0 = up ? static_cast<long>(V * V) * up_ptr[nd] : static_cast<long>(row0);
-  for (int o = o_first + lane; o < nval; o += 64) {
-    int i, kb, d;
-    if (diff_form) {
-      kb = o / (V * V);
-      const int rem = o - V * V * kb;
-      i = rem / V;
-      d = rem - V * i;
-    } else {
-      i = o / len;
-      const int p = o - len * i;
-      kb = p / V;
-      d = p - V * kb;
-    }
-    C v = acc[i * len + V * kb + d];
+  // Dirichlet elimination of value (row i, block kb, column d).
+  bool any_bc = false;
+#pragma unroll
+  for (int i = 0; i < V; ++i) any_bc = any_bc || rbc[i];
+#pragma unroll
+  for (int w = 0; w < kW; ++w)
+#pragma unroll
+    for (int d = 0; d < V; ++d) any_bc = any_bc || cb[d][w] != 0ull;
+  auto eliminate = [&](C v, int i, int kb, int d) {
     const bool own = kb == kself;
-    if (bc_mask && !(diff_form && own)) {
+    if (!(diff_form && own)) {
       bool zr = false, zc = false;
 #pragma unroll
       for (int ii = 0; ii < V; ++ii)
@@ -149,7 +144,30 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
       if (zr || (symmetric && zc)) v = C(0);
       if (zr && own && i == d) v = C(1);
     }
-    data[out0 + (o - o_first)] = static_cast<T>(v);
+    return v;
+  };
+  if (diff_form) {
+    for (int o = o_first + lane; o < nval; o += 64) {
+      const int kb = o / (V * V);
+      const int rem = o - V * V * kb;
+      const int i = rem / V, d = rem - V * i;
+      C v = acc[i * len + V * kb + d];
+      if (any_bc) v = eliminate(v, i, kb, d);
+      data[out0 + (o - o_first)] = static_cast<T>(v);
+    }
+  } else {
+    // row by row: no division by the (run-time) row length
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      for (int p = lane; p < len; p += 64) {
+        C v = acc[i * len + p];
+        if (any_bc) {
+          const int kb = p / V, d = p - V * kb;
+          v = eliminate(v, i, kb, d);
+        }
+        data[out0 + i * len + p] = static_cast<T>(v);
+      }
+    }
   }
 }
 
