@@ -138,6 +138,10 @@ SIGNATURES = {
     "fx_csr_spmv_b3sym_f32f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P]),
     "fx_pcg_b3sym_f32f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P, _P,
                                    _P, _P, _I32, _P, _SZ]),
+    "fx_b3sym_compact_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P]),
+    "fx_csr_spmv_b3sym_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _P]),
+    "fx_pcg_b3sym_f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _P, _P, _P,
+                                _P, _P, _I32, _P, _SZ]),
     "fx_csr_jacobi_f64": (_I32, [_P, _I32, _P, _P, _P, _F64, _P]),
     "fx_csr_jacobi_f32": (_I32, [_P, _I32, _P, _P, _P, _F64, _P]),
     "fx_csr_diagonal_f64": (_I32, [_P, _I32, _P, _P, _P, _P]),

@@ -122,25 +122,28 @@ class TracedStructure:
         cache[G] = m
         return m
 
-    def b3sym(self):
+    def b3sym(self, nodes_per_block=64):
         """Upper-block storage map (nat.B3Sym) of the block-3 pattern for the
-        symmetric difference-form SpMV (fx_pcg_b3sym_f32f64), or None when the
-        pattern does not fit its kernels (upper_block_tables).  Built once."""
+        symmetric SpMV kernels (fx_pcg_b3sym_*: 64 nodes per workgroup for
+        the FP32 difference form, 32 for FP64 values), or None when the
+        pattern does not fit them (upper_block_tables).  Built once per
+        workgroup size."""
         import torch
-        if "_b3sym" in self.__dict__:
-            return self._b3sym
-        self._b3sym = None
+        cache = self.__dict__.setdefault("_b3syms", {})
+        if nodes_per_block in cache:
+            return cache[nodes_per_block]
+        cache[nodes_per_block] = None
         if self.vec != 3 or self.n % 3 or self.n == 0:
             return None
-        tab = upper_block_tables(self.indptr.cpu().numpy(), self.bcol().cpu().numpy())
+        tab = upper_block_tables(self.indptr.cpu().numpy(), self.bcol().cpu().numpy(), nodes_per_block)
         if tab is not None:
             dev = self.device
             t = (torch.as_tensor(tab["code"].view(np.int16), device=dev),
                  torch.as_tensor(tab["uptr"], device=dev), torch.as_tensor(tab["ucol"], device=dev),
                  torch.as_tensor(tab["up_ptr"], device=dev))
-            self._b3sym = nat.B3Sym(t, tab["nodes_per_block"], tab["max_unique"], tab["max_degree"],
-                                    tab["num_upper"])
-        return self._b3sym
+            cache[nodes_per_block] = nat.B3Sym(t, tab["nodes_per_block"], tab["max_unique"], tab["max_degree"],
+                                               tab["num_upper"])
+        return cache[nodes_per_block]
 
     def bcol(self):
         """Column node of each 3x3 block of the CSR pattern (nse/9 int32)."""
@@ -413,6 +416,9 @@ def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, 
     if b3:
         A.b3map = ts.b3map
         A.b3sym = ts.b3sym
+        # a linear law with symmetric elimination assembles a bitwise-symmetric
+        # matrix: the solvers may run it from upper-block storage
+        A.sym_hint = bool(symmetric) and mat.kind in (nat.FX_MAT_LINEAR_ELASTIC, nat.FX_MAT_SIMP)
     return A if jacobi_shift is None else (A, dinv)
 
 

@@ -49,6 +49,9 @@ class CSRMatrix:
         # by the assembly for block-3 matrices (see upper_storage)
         self.b3sym = None
         self._upper = None
+        # set by the assembly when the matrix is symmetric bit for bit by
+        # construction (upper_storage is then worth its compaction pass)
+        self.sym_hint = False
 
     @property
     def nse(self):
@@ -66,6 +69,7 @@ class CSRMatrix:
         A = CSRMatrix(data, self.indptr, self.indices, self.shape, self.block3,
                       self.form if form is None else form, self._bcol)
         A.b3map, A.b3sym = self.b3map, self.b3sym
+        A.sym_hint = self.sym_hint and form in (None, self.form)
         return A
 
     @property
@@ -82,30 +86,33 @@ class CSRMatrix:
         return self._bcol
 
     def upper_storage(self):
-        """(nat.B3Sym, upper values) of a difference-form matrix that is
-        bitwise symmetric, else None.  The upper 3x3 blocks are copied once
-        into the compact storage of fx_b3sym_compact_f32 (about half the
+        """(nat.B3Sym, upper values) of a bitwise-symmetric block-3 matrix,
+        else None: the FP32 difference form (form "diff3"; "sym3" is already
+        stored so) or the FP64 block-3 CSR.  The upper 3x3 blocks are copied
+        once into the compact storage of fx_b3sym_compact_* (about half the
         values), which also checks every lower block against the transpose of
         its mirror; the PCG then runs its SpMV on the compact copy
-        (fx_pcg_b3sym_f32f64): the same operator from about half the bytes.
-        A matrix that is not exactly symmetric (row-only Dirichlet
-        elimination, a non-symmetric tangent) or a pattern the kernels do not
-        take keeps the full storage.  Built once per matrix (one host sync)."""
+        (fx_pcg_b3sym_*): the same operator from about half the bytes.  Only
+        tried for matrices the assembly marks symmetric by construction
+        (sym_hint: a linear law with symmetric elimination); one that is not
+        exactly symmetric keeps the full storage.  Built once per matrix (one
+        host sync)."""
         import torch
         if self.form == "sym3":
-            return self.b3sym(), self.data
-        if self.form != "diff3" or self.b3sym is None:
+            return self.b3sym(64), self.data
+        fp64 = self.form == "csr" and self.block3 and self.data.dtype == torch.float64
+        if not (self.form == "diff3" or fp64) or self.b3sym is None or not self.sym_hint:
             return None
         if self._upper is None:
             self._upper = False
-            sym = self.b3sym()
+            sym = self.b3sym(32 if fp64 else 64)
             if sym is not None:
                 dev = self.data.device
-                upper = torch.empty(9 * sym.num_upper, dtype=torch.float32, device=dev)
+                upper = torch.empty(9 * sym.num_upper, dtype=self.data.dtype, device=dev)
                 bad = torch.empty(2, dtype=torch.int32, device=dev)
-                nat.call("fx_b3sym_compact_f32", nat.stream(), self.shape[0], nat.ptr(self.indptr),
-                         nat.ptr(self.bcol), ctypes.addressof(sym.struct), nat.ptr(self.data), nat.ptr(upper),
-                         nat.ptr(bad))
+                nat.call("fx_b3sym_compact_f64" if fp64 else "fx_b3sym_compact_f32", nat.stream(), self.shape[0],
+                         nat.ptr(self.indptr), nat.ptr(self.bcol), ctypes.addressof(sym.struct), nat.ptr(self.data),
+                         nat.ptr(upper), nat.ptr(bad))
                 if int(bad.abs().sum().item()) == 0:
                     self._upper = (sym, upper)
         return self._upper or None
@@ -127,7 +134,7 @@ class CSRMatrix:
         if self.form == "sym3":
             v = v.to(torch.float64).contiguous()
             y = torch.empty(n, dtype=torch.float64, device=self.data.device) if out is None else out
-            sym = self.b3sym()
+            sym = self.b3sym(64)
             nat.call("fx_csr_spmv_b3sym_f32f64", nat.stream(), n, nat.ptr(self.indptr), ctypes.addressof(sym.struct),
                      nat.ptr(self.data), nat.ptr(v), nat.ptr(y))
             return y

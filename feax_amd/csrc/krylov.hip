@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -301,82 +302,113 @@ __global__ __launch_bounds__(BS) void pcg_spmv_map_kernel(int n, const int* __re
   if constexpr (PCG) store_partials<1>(block_sum<1, BS>(acc), blockIdx.x, partials);
 }
 
-// Upper-block (symmetric) storage of the FP32 difference form (fx_b3sym):
-// 8 lanes per node, 64 nodes per 512-thread workgroup, four blocks per lane
-// (degree <= 32).  Workgroups are dealt to the XCDs in contiguous node
-// ranges (xcd_block), so the stored block a lower block reads was brought
-// into this XCD's L2 by a workgroup shortly before: HBM sees the upper half
-// of the matrix once.
-constexpr int kSymLanes = 8, kSymThreads = 512, kSymPasses = 4;
-constexpr int kSymNodes = kSymThreads / kSymLanes;
+// Upper-block (symmetric) storage (fx_b3sym): FP32 difference form with
+// 8 lanes per node, 64 nodes per 512-thread workgroup, four blocks per lane;
+// FP64 plain block-3 values with 16 lanes per node, 32 nodes per workgroup,
+// two blocks per lane (the nine FP64 values of four blocks would take the
+// registers of two waves).  Degree <= 32 either way.  Workgroups are dealt
+// to the XCDs in contiguous node ranges (xcd_block), so the stored block a
+// lower block reads was brought into this XCD's L2 by a workgroup shortly
+// before: HBM sees the upper half of the matrix once.
+template <typename V>
+struct SymCfg;
+template <>
+struct SymCfg<float> {
+  static constexpr int L = 8, U = 4, BS = 512;
+  static constexpr bool DIFF = true;
+};
+template <>
+struct SymCfg<double> {
+  static constexpr int L = 16, U = 2, BS = 512;
+  static constexpr bool DIFF = false;
+};
+template <typename V>
+constexpr int sym_nodes() {
+  return SymCfg<V>::BS / SymCfg<V>::L;
+}
+constexpr int kSymThreads = 512, kSymMaxDegree = 32;
+static_assert(SymCfg<float>::L * SymCfg<float>::U == kSymMaxDegree && SymCfg<double>::L * SymCfg<double>::U ==
+              kSymMaxDegree, "degree bound");
 
 // Seven or more waves per SIMD (at most 72 VGPRs): the kernel is latency-bound.
-template <bool PCG>
-__global__ __launch_bounds__(kSymThreads) __attribute__((amdgpu_waves_per_eu(7, 8))) void spmv_sym_kernel(int n, int max_unique, const int* __restrict__ indptr,
-                                                               fx_b3sym sym, const float* __restrict__ vals,
-                                                               const double* __restrict__ p, double* __restrict__ q,
-                                                               const PcgState* __restrict__ st, double* partials) {
+template <typename V, bool PCG>
+__global__ __launch_bounds__(kSymThreads) __attribute__((amdgpu_waves_per_eu(7, 8))) void spmv_sym_kernel(
+    int n, int max_unique, const int* __restrict__ indptr, fx_b3sym sym, const V* __restrict__ vals,
+    const double* __restrict__ p, double* __restrict__ q, const PcgState* __restrict__ st, double* partials) {
+  using C = SymCfg<V>;
   if constexpr (PCG) {
     if (st->done) return;
   }
   extern __shared__ unsigned char smem[];
-  __shared__ int own[kSymNodes];
+  __shared__ int own[sym_nodes<V>()];
   double* xs = reinterpret_cast<double*>(smem);
   int* sup = reinterpret_cast<int*>(xs + 3 * static_cast<size_t>(max_unique));
   DVec<1> acc;
-  acc.v[0] = b3sym_rows<kSymLanes, kSymThreads, kSymPasses>(xcd_block(blockIdx.x, gridDim.x), n / 3, indptr,
-                                                             sym.code, sym.uptr, sym.ucol, sym.up_ptr, vals, p, q, xs,
-                                                             sup, own);
+  acc.v[0] = b3sym_rows<V, C::DIFF, C::L, C::BS, C::U>(xcd_block(blockIdx.x, gridDim.x), n / 3, indptr, sym.code,
+                                                         sym.uptr, sym.ucol, sym.up_ptr, vals, p, q, xs, sup, own);
   if constexpr (PCG) store_partials<1>(block_sum<1, kSymThreads>(acc), blockIdx.x, partials);
 }
 
-// Upper blocks of a full difference-form matrix (block layout) into the
-// compact storage, and a bitwise symmetry check: bad[0] counts lower blocks
-// (n, c) that differ from the transpose of (c, n), bad[1] blocks whose code
-// does not match the pattern (counters zeroed by the caller).
+// Upper blocks of a full block-3 matrix into the compact storage, and a
+// bitwise symmetry check: bad[0] counts lower blocks (n, c) that differ from
+// the transpose of (c, n), bad[1] blocks whose code does not match the
+// pattern (counters zeroed by the caller).  V = float: the difference form
+// (block layout); V = double: feax's CSR rows.  8 lanes per node.
+constexpr int kCompactLanes = 8, kCompactNodes = kSymThreads / kCompactLanes;
+template <typename V>
 __global__ __launch_bounds__(kSymThreads) void sym_compact_kernel(int nnodes, const int* __restrict__ indptr,
                                                                   const int* __restrict__ bcol, fx_b3sym sym,
-                                                                  const float* __restrict__ data,
-                                                                  float* __restrict__ upper, int* __restrict__ bad) {
-  const int g = blockIdx.x * kSymNodes + threadIdx.x / kSymLanes;
-  const int gl = threadIdx.x % kSymLanes;
+                                                                  const V* __restrict__ data, V* __restrict__ upper,
+                                                                  int* __restrict__ bad) {
+  constexpr bool BLK = SymCfg<V>::DIFF;
+  using U = typename std::conditional<sizeof(V) == 8, uint64_t, uint32_t>::type;
+  const int g = blockIdx.x * kCompactNodes + threadIdx.x / kCompactLanes;
+  const int gl = threadIdx.x % kCompactLanes;
   if (g >= nnodes) return;
   const int r0 = indptr[3 * g];
+  const int len = indptr[3 * g + 1] - r0;
   const int k0 = r0 / 9;
-  const int nb = (indptr[3 * g + 1] - r0) / 3;
+  const int nb = len / 3;
   const int base = sym.up_ptr[g];
   const int nlow = nb - (sym.up_ptr[g + 1] - base);
   int nbad = 0, nsym = 0;
-  for (int kb = gl; kb < nb; kb += kSymLanes) {
+  for (int kb = gl; kb < nb; kb += kCompactLanes) {
     const int c = bcol[k0 + kb];
     const int w = sym.code[k0 + kb];
     const int r = w & 31;
     const bool low = (w & 32) != 0;
-    const float* blk = data + 9 * static_cast<size_t>(k0 + kb);
+    V blk[9];
+    b3_block<V, BLK>(data, r0, len, kb, blk);
     if (low != (c < g) || low != (kb < nlow)) {
       ++nbad;
     } else if (!low) {
       if (r != kb - nlow) {
         ++nbad;
       } else {
-        float* dst = upper + 9 * static_cast<size_t>(base + r);
+        V* dst = upper + 9 * static_cast<size_t>(base + r);
 #pragma unroll
         for (int t = 0; t < 9; ++t) dst[t] = blk[t];
       }
     } else {
       const int c0 = indptr[3 * c];
-      const int nlow_c = (indptr[3 * c + 1] - c0) / 3 - (sym.up_ptr[c + 1] - sym.up_ptr[c]);
-      const int km = c0 / 9 + nlow_c + r;
-      if (bcol[km] != g) {
+      const int lenc = indptr[3 * c + 1] - c0;
+      const int nlow_c = lenc / 3 - (sym.up_ptr[c + 1] - sym.up_ptr[c]);
+      const int km = nlow_c + r;
+      if (bcol[c0 / 9 + km] != g) {
         ++nbad;
       } else {
-        const uint32_t* a = reinterpret_cast<const uint32_t*>(blk);
-        const uint32_t* m = reinterpret_cast<const uint32_t*>(data + 9 * static_cast<size_t>(km));
+        V mb[9];
+        b3_block<V, BLK>(data, c0, lenc, km, mb);
         bool eq = true;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-          for (int j = 0; j < 3; ++j) eq = eq && (a[3 * i + j] == m[3 * j + i]);
+          for (int j = 0; j < 3; ++j) {
+            U a, m;
+            __builtin_memcpy(&a, &blk[3 * i + j], sizeof(V));
+            __builtin_memcpy(&m, &mb[3 * j + i], sizeof(V));
+            eq = eq && a == m;
+          }
         nsym += eq ? 0 : 1;
       }
     }
@@ -640,20 +672,22 @@ struct CsrMapOp {
   static constexpr bool kThreeLaunch = true;
 };
 
-// The difference-form PCG operator on upper-block storage (fx_b3sym).
+// The PCG operator on upper-block storage (fx_b3sym): the FP32 difference
+// form (V = float) or the FP64 block-3 matrix (V = double).
+template <typename V>
 struct CsrSymOp {
   int n;
   const int* indptr;
   fx_b3sym sym;
-  const float* vals;
-  int grid() const { return std::max(1, (n / 3 + kSymNodes - 1) / kSymNodes); }
+  const V* vals;
+  int grid() const { return std::max(1, (n / 3 + sym_nodes<V>() - 1) / sym_nodes<V>()); }
   size_t shm() const { return (3 * sizeof(double) + sizeof(int)) * static_cast<size_t>(sym.max_unique); }
   void apply(hipStream_t s, const double* xv, double* yv) const {
-    hipLaunchKernelGGL((spmv_sym_kernel<false>), dim3(grid()), dim3(kSymThreads), shm(), s, n, sym.max_unique,
+    hipLaunchKernelGGL((spmv_sym_kernel<V, false>), dim3(grid()), dim3(kSymThreads), shm(), s, n, sym.max_unique,
                        indptr, sym, vals, xv, yv, nullptr, nullptr);
   }
   void iter(hipStream_t s, const double* p, double* q, const PcgState* st, double* partials) const {
-    hipLaunchKernelGGL((spmv_sym_kernel<true>), dim3(grid()), dim3(kSymThreads), shm(), s, n, sym.max_unique,
+    hipLaunchKernelGGL((spmv_sym_kernel<V, true>), dim3(grid()), dim3(kSymThreads), shm(), s, n, sym.max_unique,
                        indptr, sym, vals, p, q, st, partials);
   }
   static constexpr bool kThreeLaunch = true;
@@ -897,14 +931,15 @@ int run_pcg_map(void* stream, int n, long long nse, int flags, const int* indptr
 }
 
 // Host-side checks of an fx_b3sym against the kernels' fixed shape.
-inline int check_b3sym(const char* who, int n, const fx_b3sym* sym) {
+template <typename V>
+int check_b3sym(const char* who, int n, const fx_b3sym* sym) {
   if (!sym || !sym->code || !sym->uptr || !sym->ucol || !sym->up_ptr) return fail(FX_EINVAL, "%s: incomplete map", who);
   if (n % 3) return fail(FX_EINVAL, "%s: n %% 3 != 0", who);
-  if (sym->nodes_per_block != kSymNodes)
+  if (sym->nodes_per_block != sym_nodes<V>())
     return fail(FX_EINVAL, "%s: the map has %d nodes per block, the kernels run %d", who, sym->nodes_per_block,
-                kSymNodes);
-  if (sym->max_degree > kSymLanes * kSymPasses || sym->max_degree < 0)
-    return fail(FX_ELIMIT, "%s: node degree %d exceeds %d blocks", who, sym->max_degree, kSymLanes * kSymPasses);
+                sym_nodes<V>());
+  if (sym->max_degree > kSymMaxDegree || sym->max_degree < 0)
+    return fail(FX_ELIMIT, "%s: node degree %d exceeds %d blocks", who, sym->max_degree, kSymMaxDegree);
   if (sym->max_unique < 0 || sym->max_unique > 1023 ||
       (3 * sizeof(double) + sizeof(int)) * static_cast<size_t>(sym->max_unique) > 64 * 1024)
     return fail(FX_ELIMIT, "%s: %d unique columns per block exceed the code or the LDS staging", who,
@@ -912,8 +947,9 @@ inline int check_b3sym(const char* who, int n, const fx_b3sym* sym) {
   return FX_OK;
 }
 
+template <typename V>
 int run_pcg_sym(void* stream, int n, long long nse, int flags, const int* indptr, const fx_b3sym* sym,
-                const float* vals, const double* dinv, const double* b, const double* x0, double tol, double atol,
+                const V* vals, const double* dinv, const double* b, const double* x0, double tol, double atol,
                 int maxiter, int check_every, double* x, int* iters, double* rs, float* rec_p, double* rec_pq,
                 int rec_max, void* workspace, size_t ws_bytes) {
   hipStream_t s = as_stream(stream);
@@ -921,8 +957,13 @@ int run_pcg_sym(void* stream, int n, long long nse, int flags, const int* indptr
     return fail(FX_EINVAL, "pcg: recording needs rec_p, rec_pq and rec_max >= 0");
   if (n < 0 || !indptr || !vals || !b || !x || !iters || !rs) return fail(FX_EINVAL, "pcg: bad arguments");
   if (maxiter < 0) return fail(FX_EINVAL, "pcg: maxiter < 0");
-  if (!(flags & FX_PCG_DIFF3)) return fail(FX_EINVAL, "pcg_b3sym: the upper-block storage is of the difference form");
-  if (int rc = check_b3sym("pcg_b3sym", n, sym)) return rc;
+  const int want = SymCfg<V>::DIFF ? FX_PCG_DIFF3 : FX_PCG_BLOCK3;
+  if (!(flags & want) || (flags & (FX_PCG_DIFF3 | FX_PCG_BLOCK3)) != want)
+    return fail(FX_EINVAL, "pcg_b3sym: %s", SymCfg<V>::DIFF ? "FP32 upper-block storage is of the difference form "
+                                                              "(FX_PCG_DIFF3)"
+                                                            : "FP64 upper-block storage is of the block-3 matrix "
+                                                              "(FX_PCG_BLOCK3)");
+  if (int rc = check_b3sym<V>("pcg_b3sym", n, sym)) return rc;
   const size_t need = pcg_ws_bytes(n, sizeof(double));
   if (!workspace || ws_bytes < need) return fail(FX_EWORKSPACE, "pcg: workspace %zu < %zu bytes", ws_bytes, need);
   if (n == 0) {
@@ -932,8 +973,34 @@ int run_pcg_sym(void* stream, int n, long long nse, int flags, const int* indptr
     return FX_OK;
   }
   const PcgArgs a{n, nse, flags, indptr, nullptr, rec_p, rec_pq, rec_p ? rec_max : 0};
-  return pcg_loop<double>(s, a, CsrSymOp{n, indptr, *sym, vals}, dinv, b, x0, tol, atol, maxiter, check_every, x,
-                          iters, rs, workspace);
+  return pcg_loop<double>(s, a, CsrSymOp<V>{n, indptr, *sym, vals}, dinv, b, x0, tol, atol, maxiter, check_every,
+                          x, iters, rs, workspace);
+}
+
+template <typename V>
+int b3sym_compact(void* stream, int32_t n, const int32_t* indptr, const int32_t* bcol, const fx_b3sym* sym,
+                  const V* data, V* upper, int32_t* bad) {
+  if (int rc = check_b3sym<V>("fx_b3sym_compact", n, sym)) return rc;
+  if (!indptr || !bcol || !data || !upper || !bad) return fail(FX_EINVAL, "fx_b3sym_compact: bad arguments");
+  hipStream_t s = as_stream(stream);
+  FX_HIP(hipMemsetAsync(bad, 0, 2 * sizeof(int32_t), s));
+  const int nn = n / 3;
+  if (nn == 0) return FX_OK;
+  hipLaunchKernelGGL((sym_compact_kernel<V>), dim3((nn + kCompactNodes - 1) / kCompactNodes), dim3(kSymThreads), 0, s,
+                     nn, indptr, bcol, *sym, data, upper, bad);
+  FX_LAUNCHED("sym_compact_kernel");
+  return FX_OK;
+}
+
+template <typename V>
+int spmv_b3sym(void* stream, int32_t n, const int32_t* indptr, const fx_b3sym* sym, const V* upper, const double* x,
+               double* y) {
+  if (int rc = check_b3sym<V>("fx_csr_spmv_b3sym", n, sym)) return rc;
+  if (!indptr || !upper || !x || !y) return fail(FX_EINVAL, "fx_csr_spmv_b3sym: bad arguments");
+  if (n == 0) return FX_OK;
+  CsrSymOp<V>{n, indptr, *sym, upper}.apply(as_stream(stream), x, y);
+  FX_LAUNCHED("spmv_sym_kernel");
+  return FX_OK;
 }
 
 // ---------------------------------------------------------------- vectors
@@ -1217,26 +1284,22 @@ int fx_pcg_b3map_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, con
 
 int fx_b3sym_compact_f32(void* stream, int32_t n, const int32_t* indptr, const int32_t* bcol, const fx_b3sym* sym,
                          const float* data, float* upper, int32_t* bad) {
-  if (int rc = check_b3sym("fx_b3sym_compact_f32", n, sym)) return rc;
-  if (!indptr || !bcol || !data || !upper || !bad) return fail(FX_EINVAL, "fx_b3sym_compact_f32: bad arguments");
-  hipStream_t s = as_stream(stream);
-  FX_HIP(hipMemsetAsync(bad, 0, 2 * sizeof(int32_t), s));
-  const int nn = n / 3;
-  if (nn == 0) return FX_OK;
-  hipLaunchKernelGGL(sym_compact_kernel, dim3((nn + kSymNodes - 1) / kSymNodes), dim3(kSymThreads), 0, s, nn, indptr,
-                     bcol, *sym, data, upper, bad);
-  FX_LAUNCHED("sym_compact_kernel");
-  return FX_OK;
+  return b3sym_compact<float>(stream, n, indptr, bcol, sym, data, upper, bad);
+}
+
+int fx_b3sym_compact_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* bcol, const fx_b3sym* sym,
+                         const double* data, double* upper, int32_t* bad) {
+  return b3sym_compact<double>(stream, n, indptr, bcol, sym, data, upper, bad);
 }
 
 int fx_csr_spmv_b3sym_f32f64(void* stream, int32_t n, const int32_t* indptr, const fx_b3sym* sym, const float* upper,
                              const double* x, double* y) {
-  if (int rc = check_b3sym("fx_csr_spmv_b3sym_f32f64", n, sym)) return rc;
-  if (!indptr || !upper || !x || !y) return fail(FX_EINVAL, "fx_csr_spmv_b3sym_f32f64: bad arguments");
-  if (n == 0) return FX_OK;
-  CsrSymOp{n, indptr, *sym, upper}.apply(as_stream(stream), x, y);
-  FX_LAUNCHED("spmv_sym_kernel");
-  return FX_OK;
+  return spmv_b3sym<float>(stream, n, indptr, sym, upper, x, y);
+}
+
+int fx_csr_spmv_b3sym_f64(void* stream, int32_t n, const int32_t* indptr, const fx_b3sym* sym, const double* upper,
+                          const double* x, double* y) {
+  return spmv_b3sym<double>(stream, n, indptr, sym, upper, x, y);
 }
 
 int fx_pcg_b3sym_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
@@ -1244,8 +1307,16 @@ int fx_pcg_b3sym_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, con
                         const double* x0, double tol, double atol, int32_t maxiter, int32_t check_every, double* x,
                         int32_t* iters, double* rs, float* rec_p, double* rec_pq, int32_t rec_max, void* workspace,
                         size_t ws_bytes) {
-  return run_pcg_sym(stream, n, nse, flags, indptr, sym, upper, dinv, b, x0, tol, atol, maxiter, check_every, x,
-                     iters, rs, rec_p, rec_pq, rec_max, workspace, ws_bytes);
+  return run_pcg_sym<float>(stream, n, nse, flags, indptr, sym, upper, dinv, b, x0, tol, atol, maxiter, check_every,
+                            x, iters, rs, rec_p, rec_pq, rec_max, workspace, ws_bytes);
+}
+
+int fx_pcg_b3sym_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr, const fx_b3sym* sym,
+                     const double* upper, const double* dinv, const double* b, const double* x0, double tol,
+                     double atol, int32_t maxiter, int32_t check_every, double* x, int32_t* iters, double* rs,
+                     float* rec_p, double* rec_pq, int32_t rec_max, void* workspace, size_t ws_bytes) {
+  return run_pcg_sym<double>(stream, n, nse, flags, indptr, sym, upper, dinv, b, x0, tol, atol, maxiter, check_every,
+                             x, iters, rs, rec_p, rec_pq, rec_max, workspace, ws_bytes);
 }
 
 int fx_pcg_matfree_f64(void* stream, int32_t num_nodes, int32_t num_cells, const double* x, const double* y,

@@ -170,11 +170,14 @@ __device__ __forceinline__ double b3map_rows(int first, int end, const int* __re
 // and the barrier; then the lower blocks (their owner's first block comes
 // from LDS).  Returns this thread's share of p . q (first lane of a node).
 // Contains a __syncthreads.
-template <int L, int BS, int U>
+// V = float with DIFF: the FP32 difference form (the diagonal slot holds
+// the block row sums S, the other blocks act on x_m - x_n); V = double
+// without DIFF: the plain block-3 matrix y = sum_m K_nm x_m.
+template <typename V, bool DIFF, int L, int BS, int U>
 __device__ __forceinline__ double b3sym_rows(int b, int nnodes, const int* __restrict__ indptr,
                                              const uint16_t* __restrict__ code, const int* __restrict__ uptr,
                                              const int* __restrict__ ucol, const int* __restrict__ up_ptr,
-                                             const float* __restrict__ vals, const double* __restrict__ p,
+                                             const V* __restrict__ vals, const double* __restrict__ p,
                                              double* __restrict__ q, double* xs, int* sup, int* own) {
   constexpr int G = BS / L;
   const int first = b * G;
@@ -207,11 +210,11 @@ __device__ __forceinline__ double b3sym_rows(int b, int nnodes, const int* __res
     const int kb = gl + j * L;
     w[j] = kb < nb ? static_cast<int>(code[r0 / 9 + kb]) : -1;
   }
-  float v[U][9];
+  V v[U][9];
 #pragma unroll
   for (int j = 0; j < U; ++j) {
     if (w[j] >= 0 && !(w[j] & 32)) {
-      const float* bp = vals + 9 * static_cast<size_t>(base + (w[j] & 31));
+      const V* bp = vals + 9 * static_cast<size_t>(base + (w[j] & 31));
 #pragma unroll
       for (int t = 0; t < 9; ++t) v[j][t] = bp[t];
     }
@@ -237,7 +240,7 @@ __device__ __forceinline__ double b3sym_rows(int b, int nnodes, const int* __res
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       if (w[j] >= 0 && (w[j] & 32)) {
-        const float* bp = vals + 9 * static_cast<size_t>(sup[w[j] >> 6] + (w[j] & 31));
+        const V* bp = vals + 9 * static_cast<size_t>(sup[w[j] >> 6] + (w[j] & 31));
 #pragma unroll
         for (int t = 0; t < 9; ++t) v[j][t] = bp[t];
       }
@@ -250,16 +253,18 @@ __device__ __forceinline__ double b3sym_rows(int b, int nnodes, const int* __res
       if (w[j] >= 0) {
         const int l = w[j] >> 6;
         double x0 = xs[3 * l], x1 = xs[3 * l + 1], x2 = xs[3 * l + 2];
-        if (l != lo) {
-          x0 -= xs0;
-          x1 -= xs1;
-          x2 -= xs2;
+        if constexpr (DIFF) {
+          if (l != lo) {
+            x0 -= xs0;
+            x1 -= xs1;
+            x2 -= xs2;
+          }
         }
         const bool low = (w[j] & 32) != 0;
-        const float* e = v[j];
-        const float e01 = low ? e[3] : e[1], e10 = low ? e[1] : e[3];
-        const float e02 = low ? e[6] : e[2], e20 = low ? e[2] : e[6];
-        const float e12 = low ? e[7] : e[5], e21 = low ? e[5] : e[7];
+        const V* e = v[j];
+        const V e01 = low ? e[3] : e[1], e10 = low ? e[1] : e[3];
+        const V e02 = low ? e[6] : e[2], e20 = low ? e[2] : e[6];
+        const V e12 = low ? e[7] : e[5], e21 = low ? e[5] : e[7];
         a0 += static_cast<double>(e[0]) * x0 + static_cast<double>(e01) * x1 + static_cast<double>(e02) * x2;
         a1 += static_cast<double>(e10) * x0 + static_cast<double>(e[4]) * x1 + static_cast<double>(e12) * x2;
         a2 += static_cast<double>(e20) * x0 + static_cast<double>(e21) * x1 + static_cast<double>(e[8]) * x2;

@@ -116,18 +116,19 @@ def pcg(A: CSRMatrix, b, x0=None, tol=1e-10, atol=1e-10, maxiter=10000, dinv=Non
     # A bitwise-symmetric difference form runs its SpMV on upper-block
     # storage (half the matrix bytes; lower blocks are read transposed from
     # blocks the same XCD fetched just before): 78 -> 50 us at 1M DOF.
-    upper = A.upper_storage() if (A.form == "sym3" or (
-        name == "fx_pcg_f32f64" and use_map is None and flags & nat.FX_PCG_DIFF3)) else None
+    upper = A.upper_storage() if (A.form == "sym3" or (use_map is None and (
+        (name == "fx_pcg_f32f64" and flags & nat.FX_PCG_DIFF3) or (name == "fx_pcg_f64" and flags & nat.FX_PCG_BLOCK3)))
+    ) else None
     if upper is not None:
         sym, vals = upper
-        nat.call("fx_pcg_b3sym_f32f64", nat.stream(), n, A.nse, flags, nat.ptr(A.indptr), ctypes.addressof(sym.struct),
+        nat.call("fx_pcg_b3sym_f64" if name == "fx_pcg_f64" else "fx_pcg_b3sym_f32f64", nat.stream(), n, A.nse, flags, nat.ptr(A.indptr), ctypes.addressof(sym.struct),
                  nat.ptr(vals), nat.ptr(dinv), nat.ptr(b), nat.ptr(x0), float(tol), float(atol), int(maxiter),
                  int(check_every), nat.ptr(x), ctypes.addressof(iters), ctypes.addressof(rs), *rec, nat.ptr(ws),
                  ws.numel())
         if record is not None:
             record.count = min(iters.value, record.capacity)
         if timer is not None:
-            timer.add("f32f64/sym3", rs[1], rs[2])
+            timer.add("f64/sym3" if name == "fx_pcg_f64" else "f32f64/sym3", rs[1], rs[2])
         return x, PCGInfo(iters.value, rs[0])
     # FP32 block-3 values with FP64 vectors: the SpMV with the block-column
     # map (x staged in LDS), same results as the bcol path and 7 % faster at

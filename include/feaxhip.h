@@ -238,9 +238,10 @@ int fx_pcg_b3map_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, con
  * of the column node in the workgroup's sorted unique column list
  * ucol[uptr[b] .. uptr[b+1]) and r the position of the stored block inside
  * its owner's upper list (owner = c for a lower block, else n).  The
- * kernels run nodes_per_block = 64 nodes per workgroup, node degree <= 32,
- * max_unique <= 1023 (FX_EINVAL / FX_ELIMIT otherwise).  up_ptr has
- * n/3 + 1 entries; num_upper = up_ptr[n/3].  All pointers (d). */
+ * kernels run nodes_per_block = 64 nodes per workgroup for FP32 values and
+ * 32 for FP64 values, node degree <= 32, max_unique <= 1023 (FX_EINVAL /
+ * FX_ELIMIT otherwise).  up_ptr has n/3 + 1 entries; num_upper =
+ * up_ptr[n/3].  All pointers (d). */
 typedef struct fx_b3sym {
   const uint16_t* code;
   const int32_t* uptr;
@@ -251,22 +252,33 @@ typedef struct fx_b3sym {
   int32_t max_degree;
   int32_t num_upper;
 } fx_b3sym;
-/* Upper blocks of a full difference-form matrix (FX_ASM_DIFF3 values, block
- * layout) into upper (9 num_upper FP32), with a bitwise symmetry check:
- * bad[0] (d) = lower blocks differing from the transpose of their mirror,
- * bad[1] = blocks whose code disagrees with the pattern (both zero: the
- * storage is exact).  bad is zeroed on the stream first. */
+/* Upper blocks of a full block-3 matrix into upper (9 num_upper values),
+ * with a bitwise symmetry check: bad[0] (d) = lower blocks differing from
+ * the transpose of their mirror, bad[1] = blocks whose code disagrees with
+ * the pattern (both zero: the storage is exact).  bad is zeroed on the
+ * stream first.  _f32: a difference-form matrix (FX_ASM_DIFF3 values, block
+ * layout); _f64: feax's CSR values. */
 int fx_b3sym_compact_f32(void* stream, int32_t n, const int32_t* indptr, const int32_t* bcol, const fx_b3sym* sym,
                          const float* data, float* upper, int32_t* bad);
-/* y = A x on upper-block storage (the operator of fx_csr_spmv_diff3_f32f64). */
+int fx_b3sym_compact_f64(void* stream, int32_t n, const int32_t* indptr, const int32_t* bcol, const fx_b3sym* sym,
+                         const double* data, double* upper, int32_t* bad);
+/* y = A x on upper-block storage: the operator of fx_csr_spmv_diff3_f32f64
+ * (_f32f64) or of fx_csr_spmv_b3_f64 (_f64). */
 int fx_csr_spmv_b3sym_f32f64(void* stream, int32_t n, const int32_t* indptr, const fx_b3sym* sym, const float* upper,
                              const double* x, double* y);
-/* fx_pcg_f32f64 (flags must hold FX_PCG_DIFF3) on upper-block storage. */
+int fx_csr_spmv_b3sym_f64(void* stream, int32_t n, const int32_t* indptr, const fx_b3sym* sym, const double* upper,
+                          const double* x, double* y);
+/* fx_pcg_f32f64 (flags must hold FX_PCG_DIFF3) / fx_pcg_f64 (flags must hold
+ * FX_PCG_BLOCK3) on upper-block storage. */
 int fx_pcg_b3sym_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr,
                         const fx_b3sym* sym, const float* upper, const double* dinv, const double* b,
                         const double* x0, double tol, double atol, int32_t maxiter, int32_t check_every, double* x,
                         int32_t* iters, double* rs, float* rec_p, double* rec_pq, int32_t rec_max, void* workspace,
                         size_t ws_bytes);
+int fx_pcg_b3sym_f64(void* stream, int32_t n, int64_t nse, int32_t flags, const int32_t* indptr, const fx_b3sym* sym,
+                     const double* upper, const double* dinv, const double* b, const double* x0, double tol,
+                     double atol, int32_t maxiter, int32_t check_every, double* x, int32_t* iters, double* rs,
+                     float* rec_p, double* rec_pq, int32_t rec_max, void* workspace, size_t ws_bytes);
 
 /* Krylov recycling: Galerkin projection of r onto nvec recorded A-conjugate
  * directions, x0 = sum_k p_k (p_k . r) / (p_k . A p_k) (init-CG, Erhel and

@@ -194,3 +194,49 @@ def test_gather_node_order_does_not_change_the_matrix():
         ts.node_order = None
     assert torch.equal(A1.data, A2.data) and torch.equal(d1, d2)
     assert torch.equal(B1.data.view(torch.int32), B2.data.view(torch.int32))
+
+
+def test_fp64_sym_spmv_and_pcg_match_the_block3_path():
+    """FP64 block-3 matrix on upper-block storage (32-node workgroups): the
+    SpMV equals the block-3 kernel to FP64 rounding, the PCG matches the
+    full-storage PCG, and a row-only eliminated matrix is refused."""
+    import torch
+    import scipy.sparse as sp
+    from feax_amd import _native as nat
+    from feax_amd.assembler import assemble_csr, residual
+    from feax_amd.solvers.common import pcg, SpmvTimer
+    prob, bc, ts, tp = _distorted_problem()
+    print("case: fp64 b3sym", flush=True)
+    A, dinv = assemble_csr(prob, ts, tp, None, bc, True, torch.float64, jacobi_shift=1e-12)
+    assert A.sym_hint
+    up = A.upper_storage()
+    assert up is not None
+    sym, vals = up
+    assert sym.struct.nodes_per_block == 32 and vals.dtype == torch.float64
+    x = torch.as_tensor(np.random.default_rng(6).standard_normal(ts.n), device="cuda")
+    y_b3 = A.matvec(x)
+    y_sym = torch.empty_like(y_b3)
+    nat.call("fx_csr_spmv_b3sym_f64", nat.stream(), ts.n, nat.ptr(A.indptr), ctypes.addressof(sym.struct),
+             nat.ptr(vals), nat.ptr(x), nat.ptr(y_sym))
+    K = sp.csr_matrix((_np(A.data), _np(A.indices), _np(A.indptr)), shape=(ts.n, ts.n))
+    scale = abs(K) @ np.abs(_np(x))
+    err = np.abs(_np(y_sym) - _np(y_b3))
+    print("fp64 sym vs block3 max err / (|K||x| eps)", (err / (scale * EPS64 + 1e-300)).max())
+    assert np.all(err <= 64 * EPS64 * scale)
+    b = residual(prob, ts, tp, torch.zeros(ts.n, dtype=torch.float64, device="cuda"), bc, negate=True)
+    with SpmvTimer() as t:
+        x1, i1 = pcg(A, b, None, 1e-10, 0.0, 5000, dinv, True)
+    assert "f64/sym3" in t.kernels
+    x2, i2 = pcg(A, b, None, 1e-10, 0.0, 5000, dinv, True, use_map=False)
+    print("iterations sym", i1.iters, "block3", i2.iters)
+    assert abs(i1.iters - i2.iters) <= max(2, 0.02 * i2.iters)
+    assert np.abs(_np(x1) - _np(x2)).max() <= 1e-8 * np.abs(_np(x2)).max()
+    # row-only elimination: not symmetric, refused by the check
+    Ar = assemble_csr(prob, ts, tp, None, bc, False, torch.float64)
+    assert not Ar.sym_hint and Ar.upper_storage() is None
+    upper = torch.empty(9 * sym.num_upper, dtype=torch.float64, device="cuda")
+    bad = torch.empty(2, dtype=torch.int32, device="cuda")
+    nat.call("fx_b3sym_compact_f64", nat.stream(), ts.n, nat.ptr(Ar.indptr), nat.ptr(Ar.bcol),
+             ctypes.addressof(sym.struct), nat.ptr(Ar.data), nat.ptr(upper), nat.ptr(bad))
+    bb = _np(bad)
+    assert bb[0] > 0 and bb[1] == 0
