@@ -461,10 +461,10 @@ __global__ __launch_bounds__(kFinBlock) void pcg_update_alpha_kernel(
     const int i = rg.begin + threadIdx.x + j * kFinBlock;
     if (i < rg.end) {
       pv[j] = p[i];
-      qv[j] = q[i];
-      xv[j] = x[i];
-      rv[j] = r[i];
-      if (dinv) dv[j] = dinv[i];
+      qv[j] = ldnt(q + i);
+      xv[j] = ldnt(x + i);
+      rv[j] = ldnt(r + i);
+      if (dinv) dv[j] = ldnt(dinv + i);
     }
   }
   __shared__ double s_alpha;
@@ -489,9 +489,9 @@ __global__ __launch_bounds__(kFinBlock) void pcg_update_alpha_kernel(
     const int i = rg.begin + threadIdx.x + j * kFinBlock;
     if (i < rg.end) {
       if (rec) rec[i] = static_cast<float>(pv[j]);
-      x[i] = xv[j] + alpha * pv[j];
+      __builtin_nontemporal_store(xv[j] + alpha * pv[j], x + i);
       const T ri = rv[j] - alpha * qv[j];
-      r[i] = ri;
+      __builtin_nontemporal_store(ri, r + i);
       const T zi = dinv ? dv[j] * ri : ri;
       z[i] = zi;
       acc.v[0] += static_cast<double>(ri) * static_cast<double>(zi);
