@@ -272,6 +272,11 @@ SIGNATURES = {
     "fx_band_from_csr_f64": (_I32, [_P, _I32, _P, _P, _P, _P, _I32, _I64, _P, _P, _SZ]),
     "fx_band_cholesky_f64": (_I32, [_P, _I32, _I32, _I64, _P, _P, _P, _SZ]),
     "fx_band_ldlt_f64": (_I32, [_P, _I32, _I32, _I64, _P, _P, _P, _SZ]),
+    "fx_band_inv_block": (_I32, [_I32]),
+    "fx_band_inv_bytes": (_SZ, [_I32, _I32]),
+    "fx_band_solve_inv_workspace_bytes": (_SZ, [_I32, _I32]),
+    "fx_band_invert_diag_f64": (_I32, [_P, _I32, _I32, _I64, _I32, _P, _P]),
+    "fx_band_solve_inv_f64": (_I32, [_P, _I32, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _SZ]),
     "fx_band_solve_f64": (_I32, [_P, _I32, _I32, _I64, _I32, _P, _P, _P, _P, _P, _SZ]),
     "fx_gmres_workspace_bytes": (_SZ, [_I32, _I32, _I32]),
     "fx_gmres_f64": (_I32, [_P, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _F64, _F64, _I32, _I32, _I32, _P, _P,

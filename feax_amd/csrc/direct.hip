@@ -31,7 +31,10 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <map>
 #include <mutex>
+#include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -468,6 +471,221 @@ __global__ __launch_bounds__(kCholThreads) void chol_block_kernel(double* __rest
   if (t == 0) *info = s_failed;
 }
 
+// ------------------------------------------------------------------
+// Triangular solves with the inverses of the diagonal blocks.
+//
+// The solve of block k (NBS rows) is two launches per direction instead of
+// a chain of serial sub-block solves: (1) a wide launch forms the block's
+// coupling to the rest of the band as partial sums (forward: the rows of
+// block k against the <= bw solved columns before it, left-looking;
+// backward: the columns of block k against the <= bw solved rows after it),
+// 64 columns (rows) per workgroup; (2) a launch of NBS / 64 workgroups
+// subtracts the partials (fixed order) from the right-hand side and
+// multiplies by the precomputed inverse of the diagonal block (a dense
+// lower-triangular GEMV), so no step waits on a serial triangle.  The
+// inverses (rocBLAS trtri, strided batched) are formed once per
+// factorisation.  Only entries within the bandwidth are read.
+constexpr int kPart = 64;        // columns (forward) / rows (backward) per partial workgroup
+constexpr int kDiagThreads = 1024;
+
+// Largest solve block below ldab (the diagonal block must be a plain
+// column-major matrix inside the band storage), at most 512.
+inline int inv_block(int bw) {
+  const int64_t ldab = static_cast<int64_t>(bw) + 2 * kFactorNb;
+  int nbs = 512;
+  while (nbs > 64 && nbs >= ldab) nbs /= 2;
+  return nbs;
+}
+inline int part_chunks(int bw) { return (bw + kPart - 1) / kPart + 1; }
+
+// Forward partials: P[q NBS + r] = sum_{j in chunk q} L(c1 + r, j) y_j over
+// the columns [jlo + 128 q, min(c1, jlo + 128 q + 128)), jlo = max(0,
+// c1 - bw).  Workgroup (q, row tile): 64 rows, its 4 waves take 32 columns
+// each (lanes over the rows: consecutive rows read consecutive addresses,
+// 32 loads in flight per lane), then a fixed-order sum over the waves.  A
+// block's coupling is only ~21 MB at 1M DOF, so the work is spread thin
+// over many short-lived waves.
+constexpr int kFwdCols = 128;
+template <int NBS>
+__global__ __launch_bounds__(256) void inv_fwd_partial_kernel(int bw, int64_t ldab, const double* __restrict__ ab,
+                                                              const double* __restrict__ y, int c1, int w1, int jlo,
+                                                              double* __restrict__ P) {
+  __shared__ double ys[kFwdCols];
+  __shared__ double red[4][64];
+  const int q = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int j0 = jlo + q * kFwdCols;
+  const int jn = min(kFwdCols, c1 - j0);
+  if (t < kFwdCols) ys[t] = t < jn ? y[j0 + t] : 0.0;
+  __syncthreads();
+  const int rr = blockIdx.y * 64 + lane;
+  double acc = 0.0;
+  if (rr < w1) {
+    const int r = c1 + rr;
+    double l[32];
+#pragma unroll
+    for (int u = 0; u < 32; ++u) {
+      const int jj = wv * 32 + u, j = j0 + jj;
+      l[u] = (jj < jn && r - j <= bw) ? ab[boff(r, j, ldab)] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 32; ++u) acc += l[u] * ys[wv * 32 + u];
+  }
+  red[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0 && blockIdx.y * 64 + lane < NBS)
+    P[q * NBS + blockIdx.y * 64 + lane] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
+// Backward partials: P[q NBS + j] = sum_{i in chunk q} L(i, c1 + j) x_i over
+// the rows [i0, i0 + 64), i0 = c1 + w1 + 64 q, below the block (i < ihi).
+// Workgroup (q, tile): a 64 x 64 tile of L staged in LDS (coalesced: rows
+// consecutive for one column), then column sums.
+template <int NBS>
+__global__ __launch_bounds__(256) void inv_bwd_partial_kernel(int bw, int64_t ldab, const double* __restrict__ ab,
+                                                              const double* __restrict__ x, int c1, int w1, int ihi,
+                                                              double* __restrict__ P) {
+  __shared__ double T[kPart][kPart + 1];
+  __shared__ double xs[kPart];
+  __shared__ double red[4][kPart];
+  const int q = blockIdx.x, tile = blockIdx.y, t = threadIdx.x;
+  const int i0 = c1 + w1 + q * kPart;
+  const int cj = c1 + tile * kPart;
+  if (t < kPart) xs[t] = (i0 + t < ihi) ? x[i0 + t] : 0.0;
+#pragma unroll
+  for (int e0 = 0; e0 < kPart * kPart; e0 += 256) {
+    const int e = e0 + t;
+    const int row = e % kPart, col = e / kPart;
+    const int i = i0 + row, j = cj + col;
+    T[row][col] = (i < ihi && j < c1 + w1 && i - j <= bw) ? ab[boff(i, j, ldab)] : 0.0;
+  }
+  __syncthreads();
+  const int col = t % kPart, part = t / kPart;
+  double s = 0.0;
+#pragma unroll
+  for (int rr = 0; rr < kPart / 4; ++rr) {
+    const int row = part * (kPart / 4) + rr;
+    s += T[row][col] * xs[row];
+  }
+  red[part][col] = s;
+  __syncthreads();
+  if (t < kPart && tile * kPart + t < NBS) P[q * NBS + tile * kPart + t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+}
+
+// v_r = rhs[c1 + r] - sum_q P[q NBS + r] for r < w1, into LDS (every
+// workgroup of the diagonal launch forms all of v).  Fixed order: each of
+// the kDiagThreads / NBS thread groups sums the chunks q = h, h + H, ...
+// (32 loads in flight, added in q order); the groups' sums are added in
+// group order.
+template <int NBS>
+__device__ __forceinline__ void inv_rhs(const double* __restrict__ rhs, int c1, int w1, int nq,
+                                        const double* __restrict__ P, double* v, double (*grp)[NBS]) {
+  constexpr int H = kDiagThreads / NBS;
+  constexpr int kB = 32;  // partials loaded before any is added
+  const int t = threadIdx.x;
+  const int r = t % NBS, h = t / NBS;
+  double a = 0.0;
+  if (r < w1) {
+    for (int q0 = h; q0 < nq; q0 += kB * H) {
+      double pv[kB];
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        const int q = q0 + u * H;
+        pv[u] = q < nq ? P[q * NBS + r] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kB; ++u) a += pv[u];
+    }
+  }
+  grp[h][r] = a;
+  __syncthreads();
+  if (h == 0) {
+    // v past the block (the last block of the matrix) is 0, so that a zero
+    // of the inverse never multiplies stale LDS
+    double sum = 0.0;
+#pragma unroll
+    for (int g = 0; g < H; ++g) sum += grp[g][r];
+    v[r] = r < w1 ? rhs[c1 + r] - sum : 0.0;
+  }
+  __syncthreads();
+}
+
+// Forward diagonal step: y[c1 + r] = sum_{j <= r} Linv(r, j) v_j for the 64
+// rows of this workgroup (Linv column-major, leading dimension NBS); 16
+// waves take NBS / 16 columns each, lanes the rows (coalesced), then a
+// fixed-order sum over the waves.
+template <int NBS>
+__global__ __launch_bounds__(kDiagThreads) void inv_fwd_diag_kernel(const double* __restrict__ rhs,
+                                                                    double* __restrict__ y, int c1, int w1, int nq,
+                                                                    const double* __restrict__ P,
+                                                                    const double* __restrict__ Linv) {
+  constexpr int H = kDiagThreads / NBS;
+  constexpr int kW = kDiagThreads / 64, kCols = NBS / kW;
+  __shared__ double v[NBS];
+  __shared__ double grp[H][NBS];
+  __shared__ double red[kW][64];
+  inv_rhs<NBS>(rhs, c1, w1, nq, P, v, grp);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
+  double acc = 0.0;
+  if (r < w1) {
+    double l[kCols];
+#pragma unroll
+    for (int u = 0; u < kCols; ++u) {
+      const int j = wv * kCols + u;
+      l[u] = (j <= r) ? Linv[r + static_cast<size_t>(NBS) * j] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kCols; ++u) acc += l[u] * v[wv * kCols + u];
+  }
+  red[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0 && r < w1) {
+    double sum = 0.0;
+#pragma unroll
+    for (int w = 0; w < kW; ++w) sum += red[w][lane];
+    y[c1 + r] = sum;
+  }
+}
+
+// Backward diagonal step: x[c1 + r] = sum_{j >= r} Linv(j, r) v_j (column r
+// of Linv, contiguous): 16 waves, each takes 4 of the workgroup's 64 rows,
+// lanes over j, wave sums.
+template <int NBS>
+__global__ __launch_bounds__(kDiagThreads) void inv_bwd_diag_kernel(const double* __restrict__ rhs,
+                                                                    double* __restrict__ x, int c1, int w1, int nq,
+                                                                    const double* __restrict__ P,
+                                                                    const double* __restrict__ Linv) {
+  constexpr int H = kDiagThreads / NBS;
+  constexpr int kW = kDiagThreads / 64, kPer = NBS / 64;
+  __shared__ double v[NBS];
+  __shared__ double grp[H][NBS];
+  inv_rhs<NBS>(rhs, c1, w1, nq, P, v, grp);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double l[4][kPer];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int r = blockIdx.x * 64 + wv * 4 + k;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int j = lane + 64 * u;
+      l[k][u] = (r < w1 && j >= r && j < w1) ? Linv[j + static_cast<size_t>(NBS) * r] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int r = blockIdx.x * 64 + wv * 4 + k;
+    double a = 0.0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int j = lane + 64 * u;
+      a += l[k][u] * (j < w1 ? v[j] : 0.0);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (lane == 0 && r < w1) x[c1 + r] = a;
+  }
+}
+
 // One rocBLAS handle per device for the process (rocSOLVER runs on it).
 rocblas_handle blas_handle(int& err) {
   static std::mutex mu;
@@ -624,6 +842,146 @@ int fx_band_ldlt_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, double* 
     return fail(FX_EWORKSPACE, "fx_band_ldlt_f64: workspace %zu < %zu", ws_bytes, fx_band_workspace_bytes(n, bw));
   return band_factor("fx_band_ldlt_f64", true, as_stream(stream), n, bw, ldab, ab, info,
                      static_cast<char*>(workspace));
+}
+
+int32_t fx_band_inv_block(int32_t bw) { return bw < 0 ? -1 : inv_block(bw); }
+
+size_t fx_band_inv_bytes(int32_t n, int32_t bw) {
+  if (n < 0 || bw < 0) return 0;
+  const size_t nbs = static_cast<size_t>(inv_block(bw));
+  return ((static_cast<size_t>(n) + nbs - 1) / nbs) * nbs * nbs * sizeof(double);
+}
+
+size_t fx_band_solve_inv_workspace_bytes(int32_t n, int32_t bw) {
+  if (n < 0 || bw < 0) return 0;
+  const size_t vec = ((static_cast<size_t>(n) * 8 + 255) & ~static_cast<size_t>(255));
+  const size_t parts = static_cast<size_t>(part_chunks(bw)) * inv_block(bw) * 8;
+  return 2 * vec + ((parts + 255) & ~static_cast<size_t>(255));
+}
+
+int fx_band_invert_diag_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, int32_t flags, const double* ab,
+                            double* inv) {
+  if (n < 0 || bw < 0 || ldab < fx_band_ldab(bw) || (flags & ~FX_BAND_LDLT) || (n > 0 && (!ab || !inv)))
+    return fail(FX_EINVAL, "fx_band_invert_diag_f64: bad arguments");
+  if (n == 0) return FX_OK;
+  if (ldab - 1 > INT32_MAX) return fail(FX_ELIMIT, "fx_band_invert_diag_f64: ldab %lld too large", (long long)ldab);
+  int err = 0;
+  rocblas_handle h = blas_handle(err);
+  if (err) return fail(FX_ELIB, "fx_band_invert_diag_f64: rocblas_create_handle failed");
+  hipStream_t s = as_stream(stream);
+  rocblas_status st = rocblas_set_stream(h, s);
+  if (st != rocblas_status_success) return blas_fail(st, "rocblas_set_stream");
+  const int nbs = inv_block(bw);
+  const int nfull = n / nbs, rest = n - nfull * nbs;
+  const rocblas_diagonal dg = (flags & FX_BAND_LDLT) ? rocblas_diagonal_unit : rocblas_diagonal_non_unit;
+  const int ld = static_cast<int>(ldab - 1);
+  if (nfull > 0) {
+    st = rocblas_dtrtri_strided_batched(h, rocblas_fill_lower, dg, nbs, ab, ld, static_cast<rocblas_stride>(nbs) * ldab,
+                                        inv, nbs, static_cast<rocblas_stride>(nbs) * nbs, nfull);
+    if (st != rocblas_status_success) return blas_fail(st, "rocblas_dtrtri_strided_batched");
+  }
+  if (rest > 0) {
+    st = rocblas_dtrtri(h, rocblas_fill_lower, dg, rest, ab + static_cast<size_t>(nfull) * nbs * ldab, ld,
+                        inv + static_cast<size_t>(nfull) * nbs * nbs, nbs);
+    if (st != rocblas_status_success) return blas_fail(st, "rocblas_dtrtri");
+  }
+  return FX_OK;
+}
+
+int fx_band_solve_inv_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, int32_t flags, const double* ab,
+                          const double* inv, const int32_t* perm, const double* b, double* x, void* workspace,
+                          size_t ws_bytes) {
+  if (n < 0 || bw < 0 || ldab < fx_band_ldab(bw) || (flags & ~FX_BAND_LDLT) || (n > 0 && (!ab || !inv || !b || !x)))
+    return fail(FX_EINVAL, "fx_band_solve_inv_f64: bad arguments");
+  if (!workspace || ws_bytes < fx_band_solve_inv_workspace_bytes(n, bw))
+    return fail(FX_EWORKSPACE, "fx_band_solve_inv_f64: workspace %zu < %zu", ws_bytes,
+                fx_band_solve_inv_workspace_bytes(n, bw));
+  if (n == 0) return FX_OK;
+  hipStream_t s = as_stream(stream);
+  const size_t vec = ((static_cast<size_t>(n) * 8 + 255) & ~static_cast<size_t>(255));
+  double* yb = static_cast<double*>(workspace);                                  // P b, then x (permuted)
+  double* y = reinterpret_cast<double*>(static_cast<char*>(workspace) + vec);   // L^-1 P b
+  double* P = reinterpret_cast<double*>(static_cast<char*>(workspace) + 2 * vec);
+  const int gv = std::max(1, std::min((n + kThreads - 1) / kThreads, 2048));
+  const int unit = (flags & FX_BAND_LDLT) ? 1 : 0;
+  const int nbs = inv_block(bw);
+  const int nsb = (n + nbs - 1) / nbs;
+  hipLaunchKernelGGL(permute_kernel, dim3(gv), dim3(kThreads), 0, s, n, perm, b, yb, 0);
+  // The 2 x 2 x nsb launches of the two directions touch only ab, inv and
+  // the workspace: they are captured once as a HIP graph per (device, ab,
+  // inv, workspace, n, bw, ldab, flags) and replayed (kept until the process
+  // ends; the graph reads the current contents of those buffers, so a new
+  // factorisation in the same storage reuses it).
+  static std::mutex gmu;
+  static std::map<std::tuple<int, const void*, const void*, const void*, int, int, int64_t, int>, hipGraphExec_t>
+      graphs;
+  int dev = 0;
+  FX_HIP(hipGetDevice(&dev));
+  const auto key = std::make_tuple(dev, static_cast<const void*>(ab), static_cast<const void*>(inv),
+                                   static_cast<const void*>(workspace), static_cast<int>(n), static_cast<int>(bw),
+                                   static_cast<int64_t>(ldab), static_cast<int>(flags));
+  auto launch = [&](hipStream_t s, auto nbc) -> int {
+    constexpr int NBS = decltype(nbc)::value;
+    // L y = P b, block by block from the first
+    for (int k = 0; k < nsb; ++k) {
+      const int c1 = k * NBS, w1 = std::min(NBS, n - c1);
+      const int jlo = std::max(0, c1 - bw);
+      const int nq = (c1 - jlo + kFwdCols - 1) / kFwdCols;
+      if (nq > 0)
+        hipLaunchKernelGGL((inv_fwd_partial_kernel<NBS>), dim3(nq, (w1 + 63) / 64), dim3(256), 0, s, bw, ldab, ab, y,
+                           c1, w1, jlo, P);
+      hipLaunchKernelGGL((inv_fwd_diag_kernel<NBS>), dim3((w1 + 63) / 64), dim3(kDiagThreads), 0, s, yb, y, c1, w1, nq,
+                         P, inv + static_cast<size_t>(k) * NBS * NBS);
+    }
+    FX_LAUNCHED("fx_band_solve_inv_f64 forward");
+    if (unit) hipLaunchKernelGGL(diag_scale_kernel, dim3(gv), dim3(kThreads), 0, s, n, ldab, ab, y);
+    // L^T x = y, block by block from the last (x into yb)
+    for (int k = nsb - 1; k >= 0; --k) {
+      const int c1 = k * NBS, w1 = std::min(NBS, n - c1);
+      const int ihi = std::min(n, c1 + w1 + bw);
+      const int nq = (ihi - (c1 + w1) + kPart - 1) / kPart;
+      if (nq > 0)
+        hipLaunchKernelGGL((inv_bwd_partial_kernel<NBS>), dim3(nq, (w1 + kPart - 1) / kPart), dim3(256), 0, s, bw,
+                           ldab, ab, yb, c1, w1, ihi, P);
+      hipLaunchKernelGGL((inv_bwd_diag_kernel<NBS>), dim3((w1 + 63) / 64), dim3(kDiagThreads), 0, s, y, yb, c1, w1,
+                         std::max(nq, 0), P, inv + static_cast<size_t>(k) * NBS * NBS);
+    }
+    FX_LAUNCHED("fx_band_solve_inv_f64 backward");
+    return FX_OK;
+  };
+  auto launch_all = [&](hipStream_t cs) -> int {
+    switch (nbs) {
+      case 512: return launch(cs, std::integral_constant<int, 512>{});
+      case 256: return launch(cs, std::integral_constant<int, 256>{});
+      case 128: return launch(cs, std::integral_constant<int, 128>{});
+      default: return launch(cs, std::integral_constant<int, 64>{});
+    }
+  };
+  hipGraphExec_t exec = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(gmu);
+    auto it = graphs.find(key);
+    if (it != graphs.end()) exec = it->second;
+  }
+  if (!exec) {
+    GraphGuard gg;
+    hipError_t e = hipStreamCreateWithFlags(&gg.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(gg.stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "fx_band_solve_inv_f64 capture");
+    const int rc = launch_all(gg.stream);
+    e = hipStreamEndCapture(gg.stream, &gg.graph);
+    if (rc) return rc;
+    if (e == hipSuccess) e = hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return hip_fail(e, "fx_band_solve_inv_f64 graph");
+    exec = gg.exec;
+    gg.exec = nullptr;  // kept in the cache
+    std::lock_guard<std::mutex> lock(gmu);
+    graphs[key] = exec;
+  }
+  FX_HIP(hipGraphLaunch(exec, s));
+  hipLaunchKernelGGL(permute_kernel, dim3(gv), dim3(kThreads), 0, s, n, perm, yb, x, 1);
+  FX_LAUNCHED("fx_band_solve_inv_f64 permute");
+  return FX_OK;
 }
 
 int fx_band_solve_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, int32_t flags, const double* ab,

@@ -91,6 +91,23 @@ class BandCholesky:
             ab = torch.empty(self.n * self.ldab, dtype=torch.float64, device=device)
             BandCholesky._buffers[key] = ab
         self.ab = ab
+        # inverses of the diagonal blocks (fx_band_invert_diag_f64), formed
+        # after each factorisation, and the workspace of the solves that use
+        # them (fx_band_solve_inv_f64)
+        ikey = (str(device), "inv", int(nat.lib().fx_band_inv_bytes(self.n, bw)))
+        inv_blocks = BandCholesky._buffers.get(ikey)
+        if inv_blocks is None:
+            inv_blocks = torch.empty(ikey[2] // 8, dtype=torch.float64, device=device)
+            BandCholesky._buffers[ikey] = inv_blocks
+        self.inv_blocks = inv_blocks
+        # kept per size like the band storage: the library caches the graph of
+        # the blocked solves per (band, inverse blocks, workspace) addresses
+        wkey = (str(device), "solve_ws", int(nat.lib().fx_band_solve_inv_workspace_bytes(self.n, bw)))
+        ws_solve = BandCholesky._buffers.get(wkey)
+        if ws_solve is None:
+            ws_solve = torch.empty(wkey[2], dtype=torch.uint8, device=device)
+            BandCholesky._buffers[wkey] = ws_solve
+        self.ws_solve = ws_solve
         self.token = None
         self.ldlt = False
 
@@ -119,8 +136,11 @@ class BandCholesky:
             if info.value:
                 raise RuntimeError(f"direct solver: zero pivot {info.value} of {self.n} in the L D L^T factorisation "
                                    "(J_bc is singular)")
+        nat.call("fx_band_invert_diag_f64", nat.stream(), self.n, self.bw, self.ldab,
+                 nat.FX_BAND_LDLT if self.ldlt else 0, nat.ptr(self.ab), nat.ptr(self.inv_blocks))
         self.token = token if token is not None else object()
         self.ab.owner = self.token
+        self.inv_blocks.owner = self.token
         return self
 
     @classmethod
@@ -129,16 +149,27 @@ class BandCholesky:
         cls._buffers.clear()
 
     def valid(self):
-        return self.token is not None and getattr(self.ab, "owner", None) is self.token
+        return (self.token is not None and getattr(self.ab, "owner", None) is self.token
+                and getattr(self.inv_blocks, "owner", None) is self.token)
 
-    def solve(self, b, out=None):
+    def solve(self, b, out=None, blocked_inverse=True):
+        """x = J^-1 b from the factor.  blocked_inverse: the triangular solves
+        multiply by the inverses of the diagonal blocks (two launches per
+        block, fx_band_solve_inv_f64); False: the sub-block solves of
+        fx_band_solve_f64 (same result to rounding)."""
         import torch
         if not self.valid():
             raise RuntimeError("direct solver: the factorisation was overwritten by a later one")
         b = b.to(torch.float64).contiguous()
         x = torch.empty_like(b) if out is None else out
-        nat.call("fx_band_solve_f64", nat.stream(), self.n, self.bw, self.ldab, nat.FX_BAND_LDLT if self.ldlt else 0,
-                 nat.ptr(self.ab), nat.ptr(self.perm), nat.ptr(b), nat.ptr(x), nat.ptr(self.ws), self.ws.numel())
+        flags = nat.FX_BAND_LDLT if self.ldlt else 0
+        if blocked_inverse:
+            nat.call("fx_band_solve_inv_f64", nat.stream(), self.n, self.bw, self.ldab, flags, nat.ptr(self.ab),
+                     nat.ptr(self.inv_blocks), nat.ptr(self.perm), nat.ptr(b), nat.ptr(x), nat.ptr(self.ws_solve),
+                     self.ws_solve.numel())
+        else:
+            nat.call("fx_band_solve_f64", nat.stream(), self.n, self.bw, self.ldab, flags, nat.ptr(self.ab),
+                     nat.ptr(self.perm), nat.ptr(b), nat.ptr(x), nat.ptr(self.ws), self.ws.numel())
         return x
 
 

@@ -819,6 +819,24 @@ int fx_band_ldlt_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, double* 
                      size_t ws_bytes);
 int fx_band_solve_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, int32_t flags, const double* ab,
                       const int32_t* perm, const double* b, double* x, void* workspace, size_t ws_bytes);
+/* Triangular solves through the inverses of the diagonal blocks (the
+ * default solve of the Python layer): nbs = fx_band_inv_block(bw) rows per
+ * block (512, or the largest power of two below ldab), inv holds
+ * ceil(n / nbs) nbs x nbs FP64 blocks, column-major (fx_band_inv_bytes),
+ * formed after a factorisation by fx_band_invert_diag_f64 (rocBLAS trtri,
+ * strided batched; flags FX_BAND_LDLT: unit L).  fx_band_solve_inv_f64: the
+ * x of fx_band_solve_f64, two launches per block and direction (the
+ * block's coupling to the band as partial sums over 64 columns / rows, then
+ * the partials subtracted and the inverse block applied), workspace >=
+ * fx_band_solve_inv_workspace_bytes(n, bw). */
+int32_t fx_band_inv_block(int32_t bw);
+size_t fx_band_inv_bytes(int32_t n, int32_t bw);
+size_t fx_band_solve_inv_workspace_bytes(int32_t n, int32_t bw);
+int fx_band_invert_diag_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, int32_t flags, const double* ab,
+                            double* inv);
+int fx_band_solve_inv_f64(void* stream, int32_t n, int32_t bw, int64_t ldab, int32_t flags, const double* ab,
+                          const double* inv, const int32_t* perm, const double* b, double* x, void* workspace,
+                          size_t ws_bytes);
 
 /* ------------------------------------------------------------------ */
 /* Geometric multigrid on a structured HEX8 box (StructuredGrid numbering,

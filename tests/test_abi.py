@@ -10,7 +10,7 @@ from conftest import ROOT
 
 def header_symbols():
     h = open(os.path.join(ROOT, "include", "feaxhip.h")).read()
-    return set(re.findall(r"^(?:int|size_t|int64_t) (fx_\w+)\(", h, re.M))
+    return set(re.findall(r"^(?:int|size_t|int64_t|int32_t) (fx_\w+)\(", h, re.M))
 
 
 def test_library_exports_every_declared_symbol():

@@ -186,3 +186,53 @@ def test_indefinite_matrices_take_ldlt_and_singular_ones_are_reported():
     assert not F.ldlt
     x = F.solve(b)
     assert np.linalg.norm(K @ _np(x) - _np(b)) <= 1e-10 * np.linalg.norm(_np(b))
+
+
+@pytest.mark.parametrize("dims", [(30, 10, 10), (9, 4, 3)])
+def test_blocked_inverse_solve_matches_the_sub_block_solve_and_scipy(dims):
+    """fx_band_solve_inv_f64 (inverse diagonal blocks, two launches per block)
+    against fx_band_solve_f64 and scipy, on a box whose band spans many
+    512-row solve blocks and on one small enough for a single block; the
+    L D L^T form (unit L) on an indefinite shift of the same matrix."""
+    import torch
+    import scipy.sparse.linalg as spl
+    import feax_amd as fe
+    from feax_amd import _native as nat
+    from feax_amd.assembler import assemble_csr, residual
+    from feax_amd.solvers.direct import BandCholesky
+    prob, bc, rho = _simp(dims)
+    ts = fe.TracedStructure.from_problem(prob)
+    tp, _ = _tp(prob, rho)
+    print("case: blocked-inverse solve", dims, flush=True)
+    J = assemble_csr(prob, ts, tp, None, bc, True, torch.float64)
+    F = BandCholesky(prob, ts.device).factor(J)
+    nbs = int(nat.lib().fx_band_inv_block(F.bw))
+    print("n", F.n, "bw", F.bw, "nbs", nbs, "blocks", (F.n + nbs - 1) // nbs)
+    b = residual(prob, ts, tp, torch.zeros(ts.n, dtype=torch.float64, device="cuda"), bc, negate=True)
+    x1 = F.solve(b)
+    x2 = F.solve(b, blocked_inverse=False)
+    import scipy.sparse as sp
+    K = sp.csr_matrix((_np(J.data), _np(J.indices), _np(J.indptr)), shape=(ts.n, ts.n))
+    xs = spl.spsolve(K.tocsc(), _np(b))
+    scale = np.abs(xs).max()
+    print("inv vs sub-block", np.abs(_np(x1) - _np(x2)).max() / scale, "inv vs scipy",
+          np.abs(_np(x1) - xs).max() / scale)
+    assert np.abs(_np(x1) - _np(x2)).max() <= 1e-10 * scale
+    assert np.abs(_np(x1) - xs).max() <= 1e-9 * scale
+    # an indefinite matrix (J - sigma I on the free rows) takes L D L^T
+    sigma = 0.5 * float(spl.eigsh(K, k=1, which="LA", return_eigenvectors=False)[0])
+    d = _np(J.data).copy()
+    rows = np.repeat(np.arange(ts.n), np.diff(_np(J.indptr)))
+    diag = (_np(J.indices) == rows) & ~np.isin(rows, bc.bc_rows)
+    d[diag] -= sigma
+    A = J.with_data(torch.as_tensor(d, device="cuda"))
+    G = BandCholesky(prob, ts.device).factor(A)
+    assert G.ldlt
+    y1 = G.solve(b)
+    y2 = G.solve(b, blocked_inverse=False)
+    Ks = sp.csr_matrix((d, _np(J.indices), _np(J.indptr)), shape=(ts.n, ts.n))
+    ys = spl.spsolve(Ks.tocsc(), _np(b))
+    sc = np.abs(ys).max()
+    print("ldlt inv vs sub-block", np.abs(_np(y1) - _np(y2)).max() / sc, "vs scipy", np.abs(_np(y1) - ys).max() / sc)
+    assert np.abs(_np(y1) - _np(y2)).max() <= 1e-8 * sc
+    assert np.abs(_np(y1) - ys).max() <= 1e-7 * sc
