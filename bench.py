@@ -577,10 +577,12 @@ def direct_bench(problem, bc, tp, ts, u0, bnorm, oracle_fix):
 
 
 def kernels_only(cfg, iters=8, copy_bytes=1 << 30):
-    """A short run for hardware-counter passes: `iters` PCG iterations in
-    each matrix form (so pcg_spmv_kernel is dispatched `iters` times per
-    form) and three plain copies of `copy_bytes` (torch's copy kernel), whose
-    FETCH_SIZE / WRITE_SIZE calibrate the counters against known bytes."""
+    """A short run for hardware-counter passes: the FP64, FP32 difference-form
+    and FP32 upper-block assemblies, `iters` PCG iterations in each matrix
+    form (FP64 and FP32 on full storage, FP64 and FP32 on upper-block
+    storage: every SpMV kernel is dispatched `iters` times) and three plain
+    copies of `copy_bytes` (torch's copy kernel), whose FETCH_SIZE /
+    WRITE_SIZE calibrate the counters against known bytes."""
     import torch
     from feax_amd.assembler import assemble_csr
     from feax_amd.solvers.common import create_jacobi_preconditioner, pcg
@@ -589,12 +591,19 @@ def kernels_only(cfg, iters=8, copy_bytes=1 << 30):
     A64 = assemble_csr(problem, ts, tp, None, bc, True, torch.float64)
     d64 = create_jacobi_preconditioner(A64)
     Ad, dd = assemble_csr(problem, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12)
+    Au, du = assemble_csr(problem, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12,
+                          upper=True)
     g = torch.Generator(device="cuda").manual_seed(20261203)
     b = torch.randn(ts.n, dtype=torch.float64, device="cuda", generator=g)
     torch.cuda.synchronize()
     log(f"kernels-only: n={ts.n} nse={ts.nse}; {iters} PCG iterations per form")
+    # full storage: FP64 block-3 (pcg_spmv_kernel), FP32 difference form
+    # through the block-column map (pcg_spmv_map_kernel)
+    pcg(A64, b, None, 0.0, 0.0, iters, d64, True, check_every=iters, use_map=False)
+    pcg(Ad, b, None, 0.0, 0.0, iters, dd, True, check_every=iters, use_map=True)
+    # upper-block storage (spmv_sym_kernel): FP64 (compacted), FP32 (assembled so)
     pcg(A64, b, None, 0.0, 0.0, iters, d64, True, check_every=iters)
-    pcg(Ad, b, None, 0.0, 0.0, iters, dd, True, check_every=iters)
+    pcg(Au, b, None, 0.0, 0.0, iters, du, True, check_every=iters)
     src = torch.empty(copy_bytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
     for _ in range(3):
