@@ -31,12 +31,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 
-# HBM bytes per PCG SpMV dispatch on the 200x40x40 workload
-# (pcg_spmv_kernel FP64 block-3, pcg_spmv_map_kernel FP32 difference form), from
+# HBM bytes per PCG SpMV dispatch on the 200x40x40 workload, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of `bench.py --kernels-only`
-# (profiles/pcg_spmv_cantilever_200x40x40_pmc.md), each scaled by the same counter's
-# reading for a 1 GiB copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
-TRAFFIC = {"f64/block3": 761_557_198, "f32f64/diff3": 435_929_335}
+# (profiles/pcg_spmv_cantilever_200x40x40_pmc.md), each scaled by the same
+# counter's reading for a 1 GiB copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0):
+# spmv_sym_kernel on upper-block storage (FP32 difference form / FP64),
+# pcg_spmv_kernel (FP64 full storage) and pcg_spmv_map_kernel (FP32
+# difference form, full storage).
+TRAFFIC = {"f32f64/sym3": 220_246_513, "f64/sym3": 509_965_897, "f64/block3": 761_586_020,
+           "f32f64/diff3": 435_903_568}
 TRAFFIC_SOURCE = "profiles/pcg_spmv_cantilever_200x40x40_pmc.md"
 
 CONFIGS = {
@@ -705,7 +708,7 @@ def main():
     # Dominant kernel: the SpMV inside the PCG iterations.  Standalone launches
     # of the same kernels, timed with HIP events.
     A64 = assemble_csr(problem, ts, tp, None, bc, True, torch.float64)
-    Ad, _ = assemble_csr(problem, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12)
+    Ad, dd = assemble_csr(problem, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12)
     x64 = torch.randn(n, dtype=torch.float64, device="cuda")
     y64 = torch.empty_like(x64)
     t_spmv64 = event_time(lambda: A64.matvec(x64, out=y64), 50)
@@ -726,6 +729,22 @@ def main():
     Bmin64 = nse * 8 + (nse // 9) * 4 + 4 * (n // 3 + 1) + 2 * n * 8
     Bmind = nse * 4 + (nse // 9) * 4 + 4 * (n // 3 + 1) + 2 * n * 8
     t_asm64 = event_time(lambda: assemble_csr(problem, ts, tp, None, bc, True, torch.float64), 5)
+    # The full-storage SpMV kernels (feax's CSR layout: the FP64 block-3
+    # kernel and the FP32 difference form through the block-column map),
+    # timed inside 200 PCG iterations each: the solves above run the
+    # upper-block storage instead.
+    from feax_amd.solvers.common import create_jacobi_preconditioner, pcg
+    b_full = torch.randn(n, dtype=torch.float64, device="cuda")
+    with SpmvTimer() as t_full:
+        pcg(A64, b_full, None, 0.0, 0.0, 200, create_jacobi_preconditioner(A64), True, use_map=False)
+        pcg(Ad, b_full, None, 0.0, 0.0, 200, dd, True, use_map=True)
+    full_storage = {}
+    for lab, (tot, cnt) in t_full.kernels.items():
+        if cnt:
+            us = tot / cnt
+            tr = TRAFFIC.get(lab) if args.config == "cantilever_200x40x40" else None
+            full_storage[lab] = {"kernel_us": round(us, 2), "traffic": tr,
+                                 "frac_counter": None if tr is None else round(tr / (us * 1e-6) / HBM_PEAK, 4)}
 
     # The mixed form's SpMV on upper-block storage (fx_pcg_b3sym_f32f64):
     # its algorithmic bytes are what that kernel must move -- the stored
@@ -785,6 +804,7 @@ def main():
         # the same operator as the full-storage CSR SpMV: feax's CSR formula
         # bytes over this kernel's time, for comparison with that kernel
         roofline["csr_formula_equiv_GBps"] = round(Bd / dom["t"] / 1e9, 1)
+    roofline["full_storage_kernels"] = full_storage
     if traffic is not None:
         # The HBM bytes the kernel moves (rocprof FETCH_SIZE + WRITE_SIZE):
         # the block-3 kernel reads one column index per 3x3 block, so it

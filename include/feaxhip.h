@@ -20,9 +20,12 @@
  *     FX_PCG_NO_GRAPH or FX_PCG_TIME).  The batched drivers and
  *     fx_seg_project_f32 also synchronise after uploading their host-side
  *     instance tables.  The banded direct solver calls rocBLAS / rocSOLVER
- *     (one rocBLAS handle per device, created on first use and kept) and
- *     synchronises at the end of a factorisation.  fx_pcg_f64_noalloc
- *     creates no stream, graph or event.
+ *     (one rocBLAS handle per device, created on first use and kept; rocBLAS
+ *     manages its own workspace) and synchronises at the end of a
+ *     factorisation; fx_band_solve_inv_f64 captures its launches once as a
+ *     HIP graph per (band, inverse blocks, workspace) address set and keeps
+ *     it until the process ends.  fx_pcg_f64_noalloc creates no stream,
+ *     graph or event.
  *   - Index arrays are int32 (feax keeps all structure in int32,
  *     feax/problem.py:236-244, 441-451).
  *   - (d) marks device pointers, (h) host pointers.
