@@ -1,4 +1,4 @@
-// Shared by the HEX8, TET4, QUAD4 and HEX20 element kernels: the per-node gather of
+// Shared by the HEX8, TET4, QUAD4 and HEX20 element kernels: the per-node gathers of
 // element matrices into the node's three CSR rows (deterministic: the
 // node's elements in ascending order, no atomics), the row finish with the
 // Dirichlet elimination, and the Dirichlet rows of the residual.
@@ -36,13 +36,49 @@ __device__ __forceinline__ void block_first_cols(int lane, int len, int row0, co
   }
 }
 
+// Dirichlet flags of the node's V rows and of the V columns of each of its
+// neighbour blocks (block kb = 64 w + lane), as stored in bc_mask.  All the
+// loads are issued together, unconditionally (a lane past the node's degree
+// reads the node's own flags and is ignored), so that a caller can issue
+// them early and let their latency overlap other loads.
+template <int V, int KW>
+struct DirichletFlags {
+  uint8_t row[V];
+  uint8_t col[V][KW];
+};
+
+template <int V, int MAXDEG>
+__device__ __forceinline__ DirichletFlags<V, (MAXDEG + 63) / 64> load_dirichlet_flags(
+    int nd, const int (&col)[(MAXDEG + 63) / 64], const uint8_t* __restrict__ bc_mask) {
+  DirichletFlags<V, (MAXDEG + 63) / 64> f;
+  if (bc_mask) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) f.row[i] = bc_mask[V * nd + i];
+#pragma unroll
+    for (int w = 0; w < (MAXDEG + 63) / 64; ++w) {
+      const int m = col[w] >= 0 ? col[w] / V : nd;
+#pragma unroll
+      for (int d = 0; d < V; ++d) f.col[d][w] = bc_mask[V * m + d];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < V; ++i) f.row[i] = 0;
+#pragma unroll
+    for (int w = 0; w < (MAXDEG + 63) / 64; ++w)
+#pragma unroll
+      for (int d = 0; d < V; ++d) f.col[d][w] = 0;
+  }
+  return f;
+}
+
 template <typename T, typename C, int NPE = 8, int V = 3, int SHIFT = 3, int MAXDEG = kMaxDeg>
 __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, int row0, int nval,
                                             const int* __restrict__ inc_ptr, const int* __restrict__ inc,
                                             const uint8_t* __restrict__ rank, const int* __restrict__ indices,
                                             const uint8_t* __restrict__ bc_mask, int symmetric, int diff_form,
                                             double jacobi_shift, double* __restrict__ dinv, T* __restrict__ data,
-                                            const int* col0 = nullptr, const int* __restrict__ up_ptr = nullptr) {
+                                            const int* col0 = nullptr, const int* __restrict__ up_ptr = nullptr,
+                                            const DirichletFlags<V, (MAXDEG + 63) / 64>* pre = nullptr) {
   constexpr int kW = (MAXDEG + 63) / 64;
   __builtin_amdgcn_wave_barrier();
   const int deg = len / V;
@@ -59,8 +95,9 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
   int kself = 0;
   unsigned long long cb[V][kW];
   bool rbc[V];
+  const DirichletFlags<V, kW> f = pre ? *pre : load_dirichlet_flags<V, MAXDEG>(nd, col, bc_mask);
 #pragma unroll
-  for (int i = 0; i < V; ++i) rbc[i] = bc_mask && bc_mask[V * nd + i];
+  for (int i = 0; i < V; ++i) rbc[i] = f.row[i] != 0;
 #pragma unroll
   for (int w = 0; w < kW; ++w) {
     const int kb = 64 * w + lane;
@@ -69,7 +106,7 @@ __device__ __forceinline__ void finish_rows(int nd, int lane, C* acc, int len, i
     const unsigned long long self = __ballot(in && m == nd);
     if (self) kself = 64 * w + __builtin_ctzll(self);
 #pragma unroll
-    for (int d = 0; d < V; ++d) cb[d][w] = bc_mask ? __ballot(in && bc_mask[V * m + d] != 0) : 0ull;
+    for (int d = 0; d < V; ++d) cb[d][w] = bc_mask ? __ballot(in && f.col[d][w] != 0) : 0ull;
   }
   if (dinv && lane < V) {
     // Jacobi: 1 / (K_bc[r, r] + shift) (feax/solvers/common.py:86-94); BC rows have diagonal 1.
@@ -192,11 +229,13 @@ __host__ __device__ __forceinline__ int packed_block(int a, int b) {
 // order (nullable): the nodes in a space-filling-curve order; block b then
 // runs the nodes order[4 b' ..] of its XCD-contiguous index b', so each XCD
 // gathers one compact region of the mesh and an element's workspace, read
-// by its NPE nodes, is an L2 hit after its first reader (HEX8 at 1M DOF:
-// FETCH 2.64 -> 1.19 GB, but 811 -> 864 us: the kernel is bound by its
-// per-value index arithmetic, VALU busy about 50 %, not by HBM, so the
-// assembly passes no order by default).  up_ptr (nullable): upper-block
-// store of a symmetric difference form (finish_rows).
+// by its NPE nodes, is an L2 hit after its first reader (HEX8 at 1M DOF,
+// gather_wide_rows_kernel: FETCH 2.52 -> 1.16 GB with the same time, about
+// 0.48 ms: the gather waits on its chain of dependent loads, not on HBM, so
+// the assembly passes no order by default).  up_ptr (nullable):
+// upper-block store of a symmetric difference form (finish_rows).
+// HEX20 runs this kernel; elements of 4 or 8 nodes run
+// gather_wide_rows_kernel below.
 template <typename T, typename C, int NPE = 8, int V = 3, int SHIFT = 3, int MAXDEG = kMaxDeg, bool PACK = false>
 __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
     int num_nodes, const int* __restrict__ order, const int* __restrict__ inc_ptr, const int* __restrict__ inc,
@@ -259,6 +298,90 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void gather_rows_kernel(
   }
   finish_rows<T, C, NPE, V, SHIFT, MAXDEG>(nd, lane, acc, len, row0, nval, inc_ptr, inc, rank, indices, bc_mask,
                                           symmetric, diff_form, jacobi_shift, dinv, data, col, up_ptr);
+}
+
+// The same gather for elements of NPE = 4 or 8 nodes (64 / NPE elements per
+// pass): lane (k, b) of a pass holds block (a, b) of the node's element k
+// (a = the node's local index in it), so one round of loads brings every
+// value of up to 64 / NPE elements, and each lane forms its addresses once
+// per block instead of once per value.  The blocks are then added into the
+// LDS rows element by element (the lanes of element k add their NPE blocks
+// while the others wait; the NPE blocks of one element go to distinct
+// neighbour slots), in ascending element order: the sums are those of
+// gather_rows_kernel bit for bit.  The node's Dirichlet flags are loaded
+// with the first pass, so the wave waits on three rounds of dependent loads
+// (row range and incidences; columns, element blocks and flags; stores)
+// instead of one per element pass and one per flag.  HEX8 at 1M DOF: about
+// 250 VALU instructions per node instead of 690, 0.48 ms instead of 0.78.
+template <typename T, typename C, int NPE, int V = 3, int SHIFT = 3, int MAXDEG = kMaxDeg, bool PACK = false>
+__global__ __launch_bounds__(64 * kNodesPerBlock) void gather_wide_rows_kernel(
+    int num_nodes, const int* __restrict__ order, const int* __restrict__ inc_ptr, const int* __restrict__ inc,
+    const uint8_t* __restrict__ rank, const int* __restrict__ indptr, const int* __restrict__ indices,
+    const C* __restrict__ ke, const uint8_t* __restrict__ bc_mask, int symmetric, int diff_form,
+    double jacobi_shift, double* __restrict__ dinv, T* __restrict__ data, const int* __restrict__ up_ptr) {
+  static_assert(NPE == 4 || NPE == 8, "lanes (k, b): NPE must divide the wave");
+  constexpr int kEpp = 64 / NPE;  // elements per pass
+  constexpr long kElem = PACK ? V * V * NPE * (NPE + 1) / 2 : V * V * NPE * NPE;  // workspace values per element
+  __shared__ C acc_all[kNodesPerBlock][9 * MAXDEG];
+  // the wave index is uniform: the node, its row range and its element
+  // range come through the scalar unit
+  const int wid = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int slot = (order ? xcd_block(blockIdx.x, gridDim.x) : static_cast<int>(blockIdx.x)) * kNodesPerBlock + wid;
+  if (slot >= num_nodes) return;
+  const int nd = order ? order[slot] : slot;
+  C* acc = acc_all[wid];
+  const int row0 = indptr[V * nd];
+  const int len = indptr[V * nd + 1] - row0;  // V * deg
+  const int nval = V * len;
+  if (len > V * MAXDEG) return;  // host guarantees deg <= MAXDEG
+  const int kbeg = inc_ptr[nd], kend = inc_ptr[nd + 1];
+  const int kl = lane / NPE, b = lane - NPE * kl;
+  // lanes past the node's last element load that element again and add nothing
+  auto incidence = [&](int k0) { return k0 < kend ? inc[k0 + kl < kend ? k0 + kl : kend - 1] : 0; };
+  int col[(MAXDEG + 63) / 64];
+  block_first_cols<V, MAXDEG>(lane, len, row0, indices, col);
+  int pk = incidence(kbeg);
+  for (int i = lane; i < nval; i += 64) acc[i] = C(0);
+  // The Dirichlet flags are loaded with the first element pass's values (or
+  // alone, for a node without elements): finish_rows reads them.
+  DirichletFlags<V, (MAXDEG + 63) / 64> flags;
+  if (kbeg >= kend) flags = load_dirichlet_flags<V, MAXDEG>(nd, col, bc_mask);
+  for (int k0 = kbeg; k0 < kend; k0 += kEpp) {
+    const int e = pk >> SHIFT, a = pk & ((1 << SHIFT) - 1);
+    const int q = rank[NPE * NPE * static_cast<long>(e) + NPE * a + b];
+    const bool tr = PACK && a > b;  // block (a, b) read as the transpose of block (b, a)
+    const C* src = ke + kElem * e + V * V * (PACK ? packed_block<NPE>(tr ? b : a, tr ? a : b) : NPE * a + b);
+    C v[V * V];
+#pragma unroll
+    for (int t = 0; t < V * V; ++t) v[t] = src[t];
+    if (k0 == kbeg) flags = load_dirichlet_flags<V, MAXDEG>(nd, col, bc_mask);
+    if constexpr (PACK) {
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+#pragma unroll
+        for (int j = i + 1; j < V; ++j) {
+          const C lo = v[V * i + j], hi = v[V * j + i];
+          v[V * i + j] = tr ? hi : lo;
+          v[V * j + i] = tr ? lo : hi;
+        }
+    }
+    C* dst = acc + V * q;
+    const int nk = kend - k0 < kEpp ? kend - k0 : kEpp;
+    pk = incidence(k0 + kEpp);
+    __builtin_amdgcn_wave_barrier();
+    for (int r = 0; r < nk; ++r) {
+      if (kl == r) {
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+#pragma unroll
+          for (int j = 0; j < V; ++j) dst[i * len + j] += v[V * i + j];
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  finish_rows<T, C, NPE, V, SHIFT, MAXDEG>(nd, lane, acc, len, row0, nval, inc_ptr, inc, rank, indices, bc_mask,
+                                          symmetric, diff_form, jacobi_shift, dinv, data, col, up_ptr, &flags);
 }
 
 __global__ void bc_res_kernel(int nbc, const int* __restrict__ rows, const double* __restrict__ vals,

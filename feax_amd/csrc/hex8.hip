@@ -154,9 +154,9 @@ __global__ __launch_bounds__(64 * kNodesPerBlock) void assemble_kernel(
 //      Jacobian inverse and grad N_c at Gauss point q once (LDS), lanes
 //      (a, b) then sum the 3x3 block K_ab over q = 0..7; the 576 values
 //      are staged in LDS and stored coalesced to ke[e][a][b][i][j];
-//   2. gather_rows_kernel, one wave per node (as assemble_kernel): the
-//      blocks of the node's elements, ascending element order, are added
-//      into the LDS rows at the slots of elem_rank and stored by
+//   2. gather_wide_rows_kernel, one wave per node (as assemble_kernel):
+//      the blocks of the node's elements, ascending element order, are
+//      added into the LDS rows at the slots of elem_rank and stored by
 //      finish_rows.
 // Each element's geometry is computed once instead of once per node, and
 // no wave waits on a chain of dependent loads per element.  The arithmetic
@@ -763,9 +763,10 @@ int assemble(void* stream, int num_nodes, int num_cells, const double* x, const 
       else                                                                                                         \
         hipLaunchKernelGGL((elem_matrix_lin_kernel<C>), dim3(lgrid), dim3(64 * kLinWaves), 0, s, num_cells, x, y,  \
                            z, cells, mp, param, ke);                                                               \
-      hipLaunchKernelGGL((gather_rows_kernel<T, C, 8, 3, 3, kMaxDeg, !NHV>), dim3(grid), dim3(64 * kNodesPerBlock), \
-                         0, s, num_nodes, node_order, inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, \
-                         diff, jacobi_shift, dinv, data, upper ? up_ptr : nullptr);                                \
+      hipLaunchKernelGGL((gather_wide_rows_kernel<T, C, 8, 3, 3, kMaxDeg, !NHV>), dim3(grid),                       \
+                         dim3(64 * kNodesPerBlock), 0, s, num_nodes, node_order, inc_ptr, inc, rank, indptr,       \
+                         indices, ke, bc_mask, symmetric, diff, jacobi_shift, dinv, data,                          \
+                         upper ? up_ptr : nullptr);                                                                \
     } else {                                                                                                       \
       hipLaunchKernelGGL((assemble_kernel<T, C, NHV>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s, num_nodes, x,  \
                          y, z, cells, inc_ptr, inc, rank, indptr, indices, mp, param, u, bc_mask, symmetric, diff, \

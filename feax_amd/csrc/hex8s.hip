@@ -280,9 +280,9 @@ int s_assemble(void* stream, int num_nodes, int num_cells, const double* x, cons
   hipLaunchKernelGGL((s_elem_matrix_kernel<double>), dim3(egrid), dim3(kSThreads), 0, s, num_cells, x, y, z, cells,
                      to_params(mat), param, ke);
   const int grid = (num_nodes + kNodesPerBlock - 1) / kNodesPerBlock;
-  hipLaunchKernelGGL((gather_rows_kernel<T, double, 8, 1>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s, num_nodes,
-                     nullptr, inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, 0, jacobi_shift, dinv, data,
-                     nullptr);
+  hipLaunchKernelGGL((gather_wide_rows_kernel<T, double, 8, 1>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s,
+                     num_nodes, nullptr, inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, 0, jacobi_shift,
+                     dinv, data, nullptr);
   FX_LAUNCHED("fx_hex8s_assemble");
   return FX_OK;
 }

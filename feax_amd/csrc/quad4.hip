@@ -608,9 +608,9 @@ int quad_assemble(void* stream, int num_nodes, int num_cells, const double* x, c
   else
     hipLaunchKernelGGL((quad_elem_matrix_kernel<double, false>), dim3(egrid), dim3(kQuadThreads), 0, s, num_cells,
                        x, y, cells, mp, param, u, ke);
-  hipLaunchKernelGGL((gather_rows_kernel<T, double, 4, 2>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s, num_nodes,
-                     nullptr, inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, 0, jacobi_shift, dinv, data,
-                     nullptr);
+  hipLaunchKernelGGL((gather_wide_rows_kernel<T, double, 4, 2>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s,
+                     num_nodes, nullptr, inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, 0, jacobi_shift,
+                     dinv, data, nullptr);
   FX_LAUNCHED("fx_quad4_assemble");
   return FX_OK;
 }
@@ -789,7 +789,8 @@ int qs_assemble(void* stream, int num_nodes, int num_cells, const double* x, con
   double* ke = static_cast<double*>(workspace);
   hipLaunchKernelGGL(qs_elem_matrix_kernel, dim3(static_cast<int>((16L * num_cells + kQuadThreads - 1) / kQuadThreads)),
                      dim3(kQuadThreads), 0, s, num_cells, x, y, cells, to_params(mat), param, ke);
-  hipLaunchKernelGGL((gather_rows_kernel<T, double, 4, 1>), dim3((num_nodes + kNodesPerBlock - 1) / kNodesPerBlock),
+  hipLaunchKernelGGL((gather_wide_rows_kernel<T, double, 4, 1>),
+                     dim3((num_nodes + kNodesPerBlock - 1) / kNodesPerBlock),
                      dim3(64 * kNodesPerBlock), 0, s, num_nodes, nullptr, inc_ptr, inc, rank, indptr, indices, ke,
                      bc_mask, symmetric, 0, jacobi_shift, dinv, data, nullptr);
   FX_LAUNCHED("fx_quad4s_assemble");

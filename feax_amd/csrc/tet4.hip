@@ -426,9 +426,9 @@ int tet_assemble(void* stream, int num_nodes, int num_cells, const double* x, co
     C* ke = static_cast<C*>(workspace);                                                                            \
     hipLaunchKernelGGL((tet_elem_matrix_kernel<C, NHV>), dim3(egrid), dim3(kTetThreads), 0, s, num_cells, x, y, z, \
                        cells, mp, param, u, ke);                                                                   \
-    hipLaunchKernelGGL((gather_rows_kernel<T, C, 4>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s, num_nodes,      \
+    hipLaunchKernelGGL((gather_wide_rows_kernel<T, C, 4>), dim3(grid), dim3(64 * kNodesPerBlock), 0, s, num_nodes, \
                        nullptr, inc_ptr, inc, rank, indptr, indices, ke, bc_mask, symmetric, diff, jacobi_shift,   \
-                       dinv, data, nullptr);                                                                                      \
+                       dinv, data, nullptr);                                                                       \
   } while (0)
   if (nh) {
     if (c32) FX_TET_LAUNCH(float, true);
@@ -619,7 +619,8 @@ int ts_assemble(void* stream, int num_nodes, int num_cells, const double* x, con
   double* ke = static_cast<double*>(workspace);
   hipLaunchKernelGGL(ts_elem_matrix_kernel, dim3(static_cast<int>((16L * num_cells + kTetThreads - 1) / kTetThreads)),
                      dim3(kTetThreads), 0, s, num_cells, x, y, z, cells, to_params(mat), param, ke);
-  hipLaunchKernelGGL((gather_rows_kernel<T, double, 4, 1>), dim3((num_nodes + kNodesPerBlock - 1) / kNodesPerBlock),
+  hipLaunchKernelGGL((gather_wide_rows_kernel<T, double, 4, 1>),
+                     dim3((num_nodes + kNodesPerBlock - 1) / kNodesPerBlock),
                      dim3(64 * kNodesPerBlock), 0, s, num_nodes, nullptr, inc_ptr, inc, rank, indptr, indices, ke,
                      bc_mask, symmetric, 0, jacobi_shift, dinv, data, nullptr);
   FX_LAUNCHED("fx_tet4s_assemble");
