@@ -292,12 +292,19 @@ __global__ __launch_bounds__(64 * kElemPerBlock) void elem_matrix_kernel(
 // elements.  Geometry: lane (el, q) = (lane >> 3, lane & 7) builds the
 // Jacobian of element el at Gauss point q once and writes grad N_c for all
 // 8 nodes c and the weights JxW E lam0 / JxW E mu0 to LDS (one lane per
-// point instead of eight).  Blocks: lane (a, b) accumulates, element by
-// element, K_ab = sum_q wl ga gb^T + wm gb ga^T + wm (ga . gb) I with fused
-// multiply-adds in an order under which K_ba = K_ab^T holds bitwise (CG
-// iteration counts react to a non-symmetric rounding of the matrix); the
-// lanes with a <= b store their 9 values (the upper block triangle, 324
-// values per element; the gather reads block (b, a) as its transpose).
+// point instead of eight).  Blocks: only the upper block triangle is formed
+// and stored (36 blocks, 324 values per element; the gather reads block
+// (b, a) as the transpose of block (a, b), so K_ba = K_ab^T holds bitwise).
+// The wave's 8 x 36 blocks are dealt to its lanes in storage order (block g
+// = 36 el + packed_block(a, b) in 4.5 passes, instead of 8 passes of all 64
+// blocks (a, b)), and block g's 9 values go to ke[324 ebase + 9 g ..].
+// Per Gauss point a lane adds wl ga gb^T and wm ga gb^T to two 3x3 sums
+// (18 fused multiply-adds); the block is then
+//   K_ab = sum_q wl ga gb^T + (sum_q wm ga gb^T)^T + trace(sum_q wm ga gb^T) I,
+// which is sum_q wl ga gb^T + wm gb ga^T + wm (ga . gb) I.  A diagonal
+// block takes its upper triangle for its lower one, so that it is exactly
+// symmetric too (CG iteration counts react to a non-symmetric rounding of
+// the matrix).
 constexpr int kLinWaves = 2;  // waves per block
 constexpr int kLinElems = 8;  // elements per wave
 constexpr int kPacked = 324;  // values of the upper block triangle of a HEX8 element matrix (36 blocks)
@@ -308,13 +315,14 @@ __global__ __launch_bounds__(64 * kLinWaves) void elem_matrix_lin_kernel(
     const int* __restrict__ cells, MatParams mat, const double* __restrict__ param, C* __restrict__ ke) {
   __shared__ C G_all[kLinWaves][kLinElems][8][8][3];  // grad N_c of element el at Gauss point q
   __shared__ C W_all[kLinWaves][kLinElems][8][2];     // JxW E lam0, JxW E mu0
-  __shared__ C K_all[kLinWaves][kPacked];             // one element matrix (upper triangle), staged for the store
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __shared__ C S_all[kLinWaves][9 * 32];               // 32 blocks, staged for the store
+  const int wid = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
   const int ebase = (blockIdx.x * kLinWaves + wid) * kLinElems;
   if (ebase >= num_cells) return;
   C(*G)[8][8][3] = G_all[wid];
   C(*W)[8][2] = W_all[wid];
-  C* KS = K_all[wid];
+  C* S = S_all[wid];
   {
     const int el = lane >> 3, q = lane & 7;
     const int e = min(ebase + el, num_cells - 1);  // a lane past the end repeats the last element
@@ -341,47 +349,66 @@ __global__ __launch_bounds__(64 * kLinWaves) void elem_matrix_lin_kernel(
     W[el][q][1] = wE * mu0;
   }
   __builtin_amdgcn_wave_barrier();
-  const int a = lane >> 3, b = lane & 7;
-  for (int el = 0; el < kLinElems; ++el) {
-    const int e = ebase + el;
-    if (e >= num_cells) break;
+  const int nblk = 36 * min(kLinElems, num_cells - ebase);
+  C* dst = ke + kPacked * static_cast<long>(ebase);
+  for (int g0 = 0; g0 < nblk; g0 += 64) {
+    const int g = g0 + lane;
     C K[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) K[t] = C(0);
+    if (g < nblk) {
+      const int el = g / 36, pb = g - 36 * el;
+      // block row a of the packed triangle starts at a (17 - a) / 2
+      const int a = (pb >= 8) + (pb >= 15) + (pb >= 21) + (pb >= 26) + (pb >= 30) + (pb >= 33) + (pb >= 35);
+      const int b = pb - a * (17 - a) / 2 + a;
+      C K1[9], K2[9];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const C wl = W[el][q][0], wm = W[el][q][1];
-      C P[3][3];  // P_ij = ga_i gb_j
+      for (int t = 0; t < 9; ++t) K1[t] = K2[t] = C(0);
+#pragma unroll 2
+      for (int q = 0; q < 8; ++q) {
+        const C wl = W[el][q][0], wm = W[el][q][1];
+        const C gb0 = G[el][q][b][0], gb1 = G[el][q][b][1], gb2 = G[el][q][b][2];
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) P[i][j] = G[el][q][a][i] * G[el][q][b][j];
-      const C gg = P[0][0] + P[1][1] + P[2][2];
-      // Entry (i, j) of block (a, b) and entry (j, i) of block (b, a) take
-      // the same operations on the same operands (P_ij of one is P_ji of the
-      // other), so the element matrix is exactly symmetric.
+        for (int i = 0; i < 3; ++i) {
+          const C ga = G[el][q][a][i];
+          const C al = wl * ga, am = wm * ga;
+          K1[3 * i + 0] = fma_c(al, gb0, K1[3 * i + 0]);
+          K1[3 * i + 1] = fma_c(al, gb1, K1[3 * i + 1]);
+          K1[3 * i + 2] = fma_c(al, gb2, K1[3 * i + 2]);
+          K2[3 * i + 0] = fma_c(am, gb0, K2[3 * i + 0]);
+          K2[3 * i + 1] = fma_c(am, gb1, K2[3 * i + 1]);
+          K2[3 * i + 2] = fma_c(am, gb2, K2[3 * i + 2]);
+        }
+      }
+      const C tr = K2[0] + K2[4] + K2[8];
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-          C k = fma_c(wm, P[j][i], K[3 * i + j]);
-          k = fma_c(wl, P[i][j], k);
-          K[3 * i + j] = i == j ? fma_c(wm, gg, k) : k;
+          const C v = K1[3 * i + j] + K2[3 * j + i];
+          K[3 * i + j] = i == j ? v + tr : v;
         }
+      if (a == b) {
+        K[3] = K[1];
+        K[6] = K[2];
+        K[7] = K[5];
+      }
     }
-    // Stage the upper block triangle (the matrix is exactly symmetric) in
-    // LDS and store its 324 values coalesced: block (a, b), a <= b, at
-    // 9 packed_block(a, b).
-    __builtin_amdgcn_wave_barrier();
-    if (a <= b) {
-      const int pb = 9 * packed_block<8>(a, b);
+    // Stores through LDS, 32 blocks (288 contiguous values) at a time, so
+    // that each store instruction writes consecutive addresses.
 #pragma unroll
-      for (int t = 0; t < 9; ++t) KS[pb + t] = K[t];
+    for (int h = 0; h < 2; ++h) {
+      const int gh = g0 + 32 * h;
+      if (gh >= nblk) break;
+      __builtin_amdgcn_wave_barrier();
+      if ((lane >> 5) == h) {
+#pragma unroll
+        for (int t = 0; t < 9; ++t) S[9 * (lane & 31) + t] = K[t];
+      }
+      __builtin_amdgcn_wave_barrier();
+      const int nv = 9 * min(32, nblk - gh);
+      for (int i = lane; i < nv; i += 64) dst[9 * gh + i] = S[i];
     }
-    __builtin_amdgcn_wave_barrier();
-    C* dst = ke + kPacked * static_cast<long>(e);
-#pragma unroll
-    for (int i = lane; i < kPacked; i += 64) dst[i] = KS[i];
   }
 }
 
