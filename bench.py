@@ -749,13 +749,13 @@ def main():
     # The mixed form's SpMV on upper-block storage (fx_pcg_b3sym_f32f64):
     # its algorithmic bytes are what that kernel must move -- the stored
     # upper blocks (9 FP32 each), one 16-bit code per block of the pattern,
-    # the row pointers and upper-list starts per node, the workgroups'
-    # column lists, x and y.
+    # the upper-list start per node, the workgroups' column lists (column
+    # and its upper-list start per entry, list starts), x and y.
     Ad_up = Ad.upper_storage()
     if Ad_up is not None:
         sym_t, _ = Ad_up
-        nu_cols = int(sym_t.tensors[2].numel())
-        Bsym = (sym_t.num_upper * 36 + (nse // 9) * 2 + 4 * (n // 3 + 1) * 2 + 4 * (nu_cols + len(sym_t.tensors[1]))
+        nu_cols = int(sym_t.ucol.numel())
+        Bsym = (sym_t.num_upper * 36 + (nse // 9) * 2 + 4 * (n // 3 + 1) + 4 * (2 * nu_cols + sym_t.uptr.numel())
                 + 2 * n * 8)
     headline = "mixed"
     if "f32f64/sym3" in spmv_live and Ad_up is not None:
@@ -795,8 +795,9 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(achieved / 1e9, 1),
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                 "traffic": traffic, "kernel_us": round(dom["t"] * 1e6, 2), "alg_bytes": dom["B"],
-                "achieved_def": ("alg_bytes (upper blocks 36 num_upper + 2 B code per block + per-node row and "
-                                 "upper-list starts + workgroup column lists + x and y) / kernel_us"
+                "achieved_def": ("alg_bytes (upper blocks 36 num_upper + 2 B code per block + per-node "
+                                 "upper-list start + workgroup column lists with their upper-list starts + x and "
+                                 "y) / kernel_us"
                                  if dom["label"] == "f32f64/sym3" else
                                  "alg_bytes (B_spmv = nse (s_v + 4) + 4 (n + 1) + 2 n s_x) / kernel_us") +
                                 " (HIP events around every SpMV launch of one in-solve run)"}

@@ -57,18 +57,19 @@ class FxB3Map(ctypes.Structure):
 class FxB3Sym(ctypes.Structure):
     """fx_b3sym: upper-block storage map of a symmetric block-3 pattern (device pointers)."""
     _fields_ = [("code", ctypes.c_void_p), ("uptr", ctypes.c_void_p), ("ucol", ctypes.c_void_p),
-                ("up_ptr", ctypes.c_void_p), ("nodes_per_block", ctypes.c_int32), ("max_unique", ctypes.c_int32),
-                ("max_degree", ctypes.c_int32), ("num_upper", ctypes.c_int32)]
+                ("usup", ctypes.c_void_p), ("up_ptr", ctypes.c_void_p), ("nodes_per_block", ctypes.c_int32),
+                ("max_unique", ctypes.c_int32), ("max_degree", ctypes.c_int32), ("num_upper", ctypes.c_int32)]
 
 
 class B3Sym:
-    """An FxB3Sym and the device tensors (code, uptr, ucol, up_ptr) it points into."""
+    """An FxB3Sym and the device tensors (code, uptr, ucol, usup, up_ptr) it points into."""
 
     def __init__(self, tensors, nodes_per_block, max_unique, max_degree, num_upper):
         self.tensors = tensors
-        code, uptr, ucol, up_ptr = tensors
+        code, uptr, ucol, usup, up_ptr = tensors
+        self.code, self.uptr, self.ucol, self.usup, self.up_ptr = tensors
         self.num_upper = int(num_upper)
-        self.struct = FxB3Sym(code.data_ptr(), uptr.data_ptr(), ucol.data_ptr(), up_ptr.data_ptr(),
+        self.struct = FxB3Sym(code.data_ptr(), uptr.data_ptr(), ucol.data_ptr(), usup.data_ptr(), up_ptr.data_ptr(),
                               int(nodes_per_block), int(max_unique), int(max_degree), int(num_upper))
 
 

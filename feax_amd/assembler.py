@@ -140,7 +140,7 @@ class TracedStructure:
             dev = self.device
             t = (torch.as_tensor(tab["code"].view(np.int16), device=dev),
                  torch.as_tensor(tab["uptr"], device=dev), torch.as_tensor(tab["ucol"], device=dev),
-                 torch.as_tensor(tab["up_ptr"], device=dev))
+                 torch.as_tensor(tab["usup"], device=dev), torch.as_tensor(tab["up_ptr"], device=dev))
             cache[nodes_per_block] = nat.B3Sym(t, tab["nodes_per_block"], tab["max_unique"], tab["max_degree"],
                                                tab["num_upper"])
         return cache[nodes_per_block]
@@ -187,12 +187,14 @@ def upper_block_tables(indptr, bcol, nodes_per_block=64):
     Returns None when the pattern does not fit the kernels: not symmetric, a
     node without its diagonal block, a node degree above 32, or more than
     1023 unique column nodes per workgroup of `nodes_per_block` nodes.
-    Otherwise a dict: code (uint16 per block, bcol order:
-    (l << 6) | (lower << 5) | r, l = the column node's position in the
-    workgroup's sorted unique column list, r = the stored block's position in
-    its owner's upper list -- the column node's for a lower block, else the
-    row node's), uptr / ucol (the workgroups' unique column lists), up_ptr
-    (N + 1), num_upper, max_unique, max_degree, nodes_per_block."""
+    Otherwise a dict: code (uint16, 32 per node: entry 32 n + kb for node
+    n's block kb in bcol order, (l << 6) | (lower << 5) | r, l = the column
+    node's position in the workgroup's sorted unique column list, r = the
+    stored block's position in its owner's upper list -- the column node's
+    for a lower block, else the row node's; 0xFFFF past the node's degree),
+    uptr / ucol (the workgroups' unique column lists), usup (up_ptr of each
+    ucol entry), up_ptr (N + 1), num_upper, max_unique, max_degree,
+    nodes_per_block."""
     G = int(nodes_per_block)
     ip = np.asarray(indptr, np.int64)
     bc = np.asarray(bcol, np.int64)
@@ -227,10 +229,12 @@ def upper_block_tables(indptr, bcol, nodes_per_block=64):
     maxu = int(np.diff(uptr).max())
     if maxu > 1023 or r.max() > 31:
         return None
-    code = ((lidx << 6) | (lower.astype(np.int64) << 5) | r).astype(np.uint16)
-    return dict(code=code, uptr=uptr.astype(np.int32), ucol=(uniq % N).astype(np.int32),
-                up_ptr=up_ptr.astype(np.int32), num_upper=int(up_ptr[-1]), max_unique=maxu,
-                max_degree=int(deg.max()), nodes_per_block=G)
+    code = np.full(32 * N, 0xFFFF, np.uint16)
+    code[32 * node + kb] = (lidx << 6) | (lower.astype(np.int64) << 5) | r
+    ucol = uniq % N
+    return dict(code=code, uptr=uptr.astype(np.int32), ucol=ucol.astype(np.int32),
+                usup=up_ptr[ucol].astype(np.int32), up_ptr=up_ptr.astype(np.int32), num_upper=int(up_ptr[-1]),
+                max_unique=maxu, max_degree=int(deg.max()), nodes_per_block=G)
 
 
 def _as_dev(a, ts, dtype=None):
@@ -404,7 +408,7 @@ def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, 
     data = torch.empty(9 * sym.num_upper if sym is not None else ts.nse, dtype=dtype, device=ts.device)
     extra = ()
     if ts.tag == "hex8":
-        extra = (nat.ptr(ts.node_order), nat.ptr(sym.tensors[3]) if sym is not None else None)
+        extra = (nat.ptr(ts.node_order), nat.ptr(sym.up_ptr) if sym is not None else None)
     nat.call(name, nat.stream(), ts.num_nodes, ts.num_cells, nat.ptr(ts.x), nat.ptr(ts.y), nat.ptr(ts.z),
              nat.ptr(ts.cells), nat.ptr(ts.inc_ptr), nat.ptr(ts.inc), nat.ptr(ts.rank), nat.ptr(ts.indptr),
              nat.ptr(ts.indices), ctypes.addressof(mat), nat.ptr(p), nat.ptr(u), nat.ptr(mask), int(bool(symmetric)),

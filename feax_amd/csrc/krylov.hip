@@ -326,7 +326,7 @@ template <typename V>
 constexpr int sym_nodes() {
   return SymCfg<V>::BS / SymCfg<V>::L;
 }
-constexpr int kSymThreads = 512, kSymMaxDegree = 32;
+constexpr int kSymThreads = 512, kSymMaxDegree = kSymSlots;
 static_assert(SymCfg<float>::L * SymCfg<float>::U == kSymMaxDegree && SymCfg<double>::L * SymCfg<double>::U ==
               kSymMaxDegree, "degree bound");
 
@@ -344,8 +344,8 @@ __global__ __launch_bounds__(kSymThreads) __attribute__((amdgpu_waves_per_eu(7, 
   double* xs = reinterpret_cast<double*>(smem);
   int* sup = reinterpret_cast<int*>(xs + 3 * static_cast<size_t>(max_unique));
   DVec<1> acc;
-  acc.v[0] = b3sym_rows<V, C::DIFF, C::L, C::BS, C::U>(xcd_block(blockIdx.x, gridDim.x), n / 3, indptr, sym.code,
-                                                         sym.uptr, sym.ucol, sym.up_ptr, vals, p, q, xs, sup, own);
+  acc.v[0] = b3sym_rows<V, C::DIFF, C::L, C::BS, C::U>(xcd_block(blockIdx.x, gridDim.x), n / 3, sym.code, sym.uptr,
+                                                         sym.ucol, sym.usup, sym.up_ptr, vals, p, q, xs, sup, own);
   if constexpr (PCG) store_partials<1>(block_sum<1, kSymThreads>(acc), blockIdx.x, partials);
 }
 
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(kSymThreads) void sym_compact_kernel(int nnodes, co
   int nbad = 0, nsym = 0;
   for (int kb = gl; kb < nb; kb += kCompactLanes) {
     const int c = bcol[k0 + kb];
-    const int w = sym.code[k0 + kb];
+    const int w = sym.code[kSymSlots * g + kb];
     const int r = w & 31;
     const bool low = (w & 32) != 0;
     V blk[9];
@@ -957,7 +957,8 @@ int run_pcg_map(void* stream, int n, long long nse, int flags, const int* indptr
 // Host-side checks of an fx_b3sym against the kernels' fixed shape.
 template <typename V>
 int check_b3sym(const char* who, int n, const fx_b3sym* sym) {
-  if (!sym || !sym->code || !sym->uptr || !sym->ucol || !sym->up_ptr) return fail(FX_EINVAL, "%s: incomplete map", who);
+  if (!sym || !sym->code || !sym->uptr || !sym->ucol || !sym->usup || !sym->up_ptr)
+    return fail(FX_EINVAL, "%s: incomplete map", who);
   if (n % 3) return fail(FX_EINVAL, "%s: n %% 3 != 0", who);
   if (sym->nodes_per_block != sym_nodes<V>())
     return fail(FX_EINVAL, "%s: the map has %d nodes per block, the kernels run %d", who, sym->nodes_per_block,

@@ -154,66 +154,65 @@ __device__ __forceinline__ double b3map_rows(int first, int end, const int* __re
 // block first, then its upper neighbours in column order) lie contiguously,
 // nine FP32 values each (row-major), from block up_ptr[n]; a lower block
 // (n, c), c < n, is read as the transpose of the stored block (c, n).
-// code[k] for the k-th block of the CSR pattern (bcol order):
-// (l << 6) | (lower << 5) | r, with l the position of the column node in
-// the workgroup's unique column list and r the position of the stored block
-// in its owner's upper list (owner = c for a lower block, else n).
+// code[kSymSlots n + kb] describes node n's block kb (kSymNoBlock past its
+// degree): (l << 6) | (lower << 5) | r, with l the position of the column
+// node in the workgroup's unique column list and r the position of the
+// stored block in its owner's upper list (owner = c for a lower block,
+// else n); usup[u] = up_ptr[ucol[u]], the first stored block of the list's
+// column node u.
 //
 // One workgroup of BS threads runs G = BS / L consecutive nodes (workgroup
 // index b; the caller deals b to XCDs in contiguous ranges, so that the
 // upper blocks a lower block reads were fetched into the same L2 by the
 // workgroups just before).  Lane gl of a node takes blocks gl + j L,
-// j < U (degree <= U L, checked by the entry point).  Load order (the
-// kernel is latency-bound, so every independent chain starts first): the
-// column list entries, the staged x and upper-list starts of the first BS
-// columns into registers, the codes, the upper blocks; then the LDS writes
-// and the barrier; then the lower blocks (their owner's first block comes
-// from LDS).  Returns this thread's share of p . q (first lane of a node).
-// Contains a __syncthreads.
+// j < U (degree <= U L = kSymSlots, checked by the entry point).  Load
+// order (the kernel is latency-bound, so every independent chain starts
+// first): the column list entries and their first stored blocks, the
+// node's first stored block and its codes (none depends on another load);
+// the x values of the first BS columns and the upper blocks; then the LDS
+// writes and the barrier; then the lower blocks (their owner's first block
+// comes from LDS).  Returns this thread's share of p . q (first lane of a
+// node).  Contains a __syncthreads.
 // V = float with DIFF: the FP32 difference form (the diagonal slot holds
 // the block row sums S, the other blocks act on x_m - x_n); V = double
 // without DIFF: the plain block-3 matrix y = sum_m K_nm x_m.
+constexpr int kSymSlots = 32;         // code entries per node: the degree bound
+constexpr int kSymNoBlock = 0xFFFF;  // code past a node's degree (l <= 1022 keeps real codes below it)
 template <typename V, bool DIFF, int L, int BS, int U>
-__device__ __forceinline__ double b3sym_rows(int b, int nnodes, const int* __restrict__ indptr,
-                                             const uint16_t* __restrict__ code, const int* __restrict__ uptr,
-                                             const int* __restrict__ ucol, const int* __restrict__ up_ptr,
+__device__ __forceinline__ double b3sym_rows(int b, int nnodes, const uint16_t* __restrict__ code,
+                                             const int* __restrict__ uptr, const int* __restrict__ ucol,
+                                             const int* __restrict__ usup, const int* __restrict__ up_ptr,
                                              const V* __restrict__ vals, const double* __restrict__ p,
                                              double* __restrict__ q, double* xs, int* sup, int* own) {
+  static_assert(L * U == kSymSlots, "one code slot per lane and block");
   constexpr int G = BS / L;
   const int first = b * G;
   const int end = min(first + G, nnodes);
   const int u0 = uptr[b];
   const int nu = uptr[b + 1] - u0;
   const bool st0 = static_cast<int>(threadIdx.x) < nu;
-  int cu = 0;
-  if (st0) cu = ucol[u0 + threadIdx.x];
+  int cu = 0, su = 0;
+  if (st0) {
+    cu = ucol[u0 + threadIdx.x];
+    su = usup[u0 + threadIdx.x];
+  }
   const int gl = threadIdx.x % L;
   const int g = first + threadIdx.x / L;
   const bool act = g < end;
-  int r0 = 0, nb = 0, base = 0;
-  if (act) {
-    r0 = indptr[3 * g];
-    nb = (indptr[3 * g + 1] - r0) / 3;
-    base = up_ptr[g];
-  }
+  const int base = act ? up_ptr[g] : 0;
+  int w[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) w[j] = act ? static_cast<int>(code[kSymSlots * g + gl + j * L]) : kSymNoBlock;
   double sx0 = 0.0, sx1 = 0.0, sx2 = 0.0;
-  int sb = 0;
   if (st0) {
     sx0 = p[3 * cu];
     sx1 = p[3 * cu + 1];
     sx2 = p[3 * cu + 2];
-    sb = up_ptr[cu];
-  }
-  int w[U];
-#pragma unroll
-  for (int j = 0; j < U; ++j) {
-    const int kb = gl + j * L;
-    w[j] = kb < nb ? static_cast<int>(code[r0 / 9 + kb]) : -1;
   }
   V v[U][9];
 #pragma unroll
   for (int j = 0; j < U; ++j) {
-    if (w[j] >= 0 && !(w[j] & 32)) {
+    if (w[j] != kSymNoBlock && !(w[j] & 32)) {
       const V* bp = vals + 9 * static_cast<size_t>(base + (w[j] & 31));
 #pragma unroll
       for (int t = 0; t < 9; ++t) v[j][t] = bp[t];
@@ -223,7 +222,7 @@ __device__ __forceinline__ double b3sym_rows(int b, int nnodes, const int* __res
     xs[3 * threadIdx.x] = sx0;
     xs[3 * threadIdx.x + 1] = sx1;
     xs[3 * threadIdx.x + 2] = sx2;
-    sup[threadIdx.x] = sb;
+    sup[threadIdx.x] = su;
     if (cu >= first && cu < end) own[cu - first] = threadIdx.x;
   }
   for (int u = threadIdx.x + BS; u < nu; u += BS) {
@@ -231,26 +230,26 @@ __device__ __forceinline__ double b3sym_rows(int b, int nnodes, const int* __res
     xs[3 * u] = p[3 * c];
     xs[3 * u + 1] = p[3 * c + 1];
     xs[3 * u + 2] = p[3 * c + 2];
-    sup[u] = up_ptr[c];
+    sup[u] = usup[u0 + u];
     if (c >= first && c < end) own[c - first] = u;
   }
   __syncthreads();
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    if (w[j] != kSymNoBlock && (w[j] & 32)) {
+      const V* bp = vals + 9 * static_cast<size_t>(sup[w[j] >> 6] + (w[j] & 31));
+#pragma unroll
+      for (int t = 0; t < 9; ++t) v[j][t] = bp[t];
+    }
+  }
   double dot = 0.0;
   if (act) {
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      if (w[j] >= 0 && (w[j] & 32)) {
-        const V* bp = vals + 9 * static_cast<size_t>(sup[w[j] >> 6] + (w[j] & 31));
-#pragma unroll
-        for (int t = 0; t < 9; ++t) v[j][t] = bp[t];
-      }
-    }
     const int lo = own[g - first];
     const double xs0 = xs[3 * lo], xs1 = xs[3 * lo + 1], xs2 = xs[3 * lo + 2];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      if (w[j] >= 0) {
+      if (w[j] != kSymNoBlock) {
         const int l = w[j] >> 6;
         double x0 = xs[3 * l], x1 = xs[3 * l + 1], x2 = xs[3 * l + 2];
         if constexpr (DIFF) {

@@ -236,11 +236,13 @@ int fx_pcg_b3map_f32f64(void* stream, int32_t n, int64_t nse, int32_t flags, con
  * blocks with column node c >= n (its S block, then its upper neighbours in
  * column order) are stored contiguously from block up_ptr[n], nine FP32
  * values each (row-major); the lower block (n, c), c < n, is the transpose
- * of the stored block (c, n).  code has one entry per 3x3 block of the CSR
- * pattern, in bcol order: (l << 6) | (lower << 5) | r, with l the position
- * of the column node in the workgroup's sorted unique column list
- * ucol[uptr[b] .. uptr[b+1]) and r the position of the stored block inside
- * its owner's upper list (owner = c for a lower block, else n).  The
+ * of the stored block (c, n).  code has 32 entries per node: code[32 n + kb]
+ * describes node n's block kb of the CSR pattern (bcol order) as
+ * (l << 6) | (lower << 5) | r, with l the position of the column node in
+ * the workgroup's sorted unique column list ucol[uptr[b] .. uptr[b+1]) and
+ * r the position of the stored block inside its owner's upper list
+ * (owner = c for a lower block, else n); entries past the node's degree
+ * hold 0xFFFF.  usup[u] = up_ptr[ucol[u]] (same length as ucol).  The
  * kernels run nodes_per_block = 64 nodes per workgroup for FP32 values and
  * 32 for FP64 values, node degree <= 32, max_unique <= 1023 (FX_EINVAL /
  * FX_ELIMIT otherwise).  up_ptr has n/3 + 1 entries; num_upper =
@@ -249,6 +251,7 @@ typedef struct fx_b3sym {
   const uint16_t* code;
   const int32_t* uptr;
   const int32_t* ucol;
+  const int32_t* usup;
   const int32_t* up_ptr;
   int32_t nodes_per_block;
   int32_t max_unique;

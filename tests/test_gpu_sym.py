@@ -74,7 +74,7 @@ def test_sym_spmv_equals_the_difference_form_kernel():
     print("sym vs diff3 max err / (|K||x| eps)", (err / (scale * EPS64)).max())
     assert np.all(err <= 64 * EPS64 * scale)
     # the compact copy is exactly the upper blocks of the full storage
-    tab_up = _np(sym.tensors[3])
+    tab_up = _np(sym.up_ptr)
     blocks = _np(A.data).reshape(-1, 9)
     node = np.repeat(np.arange(ts.n // 3), np.diff(_np(A.indptr)[0::3]) // 9)   # 9 deg values per node
     upper_mask = _np(A.bcol) >= node

@@ -42,13 +42,15 @@ def _sym_spmv(tab, indptr, bcol, upper, x, G):
         ucol = tab["ucol"][tab["uptr"][b]:tab["uptr"][b + 1]]
         k0 = indptr[3 * g] // 9
         nb = (indptr[3 * g + 1] - indptr[3 * g]) // 3
+        assert np.all(code[32 * g + nb:32 * (g + 1)] == 0xFFFF)   # past the degree
         for kb in range(nb):
-            w = code[k0 + kb]
+            w = code[32 * g + kb]
             l, low, r = w >> 6, (w >> 5) & 1, w & 31
             c = ucol[l]
             assert c == bcol[k0 + kb]
             assert bool(low) == (c < g)
             owner = c if low else g
+            assert tab["usup"][tab["uptr"][b] + l] == tab["up_ptr"][c]
             blk = upper[tab["up_ptr"][owner] + r].reshape(3, 3)
             K = blk.T if low else blk
             y[3 * g:3 * g + 3] += K @ x[3 * c:3 * c + 3]
@@ -129,6 +131,34 @@ def test_upper_block_tables_refuse_what_the_kernels_cannot_take():
     ipb, _, bcolb, _ = _block_pattern(big)
     assert upper_block_tables(ipb, bcolb, nodes_per_block=1) is not None
     assert upper_block_tables(ipb, bcolb, nodes_per_block=big.num_total_nodes) is None
+
+
+def _star_pattern(N):
+    """Block-3 pattern of N nodes: node N-1 is coupled to every node, the
+    others only to themselves and to it (block columns sorted)."""
+    cols = [[n, N - 1] for n in range(N - 1)] + [list(range(N))]
+    deg = np.array([len(c) for c in cols])
+    ip = np.zeros(3 * N + 1, np.int64)
+    start = np.concatenate([[0], np.cumsum(9 * deg)])
+    for n in range(N):
+        for i in range(3):
+            ip[3 * n + i] = start[n] + 3 * deg[n] * i
+    ip[-1] = start[-1]
+    return ip, np.concatenate([np.array(c) for c in cols])
+
+
+def test_upper_block_tables_code_slots_of_a_star_pattern():
+    """A node with 31 lower neighbours fills its 32 code slots; 33 blocks
+    are refused."""
+    from feax_amd.assembler import upper_block_tables
+    ip, bcol = _star_pattern(32)            # node 31: 31 lower blocks + its own
+    tab = upper_block_tables(ip, bcol)
+    assert tab is not None and tab["max_degree"] == 32
+    assert np.all(tab["code"][32 * 31:32 * 32] != 0xFFFF)
+    assert np.all((tab["code"][32 * 31:32 * 31 + 31].astype(np.int64) >> 5) & 1 == 1)   # lower blocks
+    assert np.all(tab["code"][2:32] == 0xFFFF)                                          # node 0: degree 2
+    ip, bcol = _star_pattern(33)
+    assert upper_block_tables(ip, bcol) is None
 
 
 def test_morton_order_is_a_permutation_along_the_z_curve():
