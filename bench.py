@@ -38,8 +38,8 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # spmv_sym_kernel on upper-block storage (FP32 difference form / FP64),
 # pcg_spmv_kernel (FP64 full storage) and pcg_spmv_map_kernel (FP32
 # difference form, full storage).
-TRAFFIC = {"f32f64/sym3": 220_246_513, "f64/sym3": 509_965_897, "f64/block3": 761_586_020,
-           "f32f64/diff3": 435_903_568}
+TRAFFIC = {"f32f64/sym3": 230_133_256, "f64/sym3": 542_662_012, "f64/block3": 761_587_805,
+           "f32f64/diff3": 435_909_679}
 TRAFFIC_SOURCE = "profiles/pcg_spmv_cantilever_200x40x40_pmc.md"
 
 CONFIGS = {
