@@ -82,6 +82,48 @@ def test_sym_spmv_equals_the_difference_form_kernel():
     assert tab_up[-1] == sym.num_upper
 
 
+def test_sym_spmv_with_more_than_512_columns_per_workgroup():
+    """A 4 x 4 x 100 box in natural numbering (the last axis fastest): a
+    64-node workgroup lies on one line of 101 nodes and reaches 594 unique
+    column nodes, more than its 512 threads, so the SpMV stages the rest in
+    its second loop.  The upper-block product equals the full-storage
+    difference form."""
+    import torch
+    import feax_amd as fe
+    from feax_amd import _native as nat
+    from feax_amd.assembler import assemble_csr
+    print("case: b3sym spmv, 4x4x100 box, > 512 columns per workgroup", flush=True)
+    mesh = fe.StructuredGrid((4, 4, 100), spacing=(1.0, 1.0, 1.0)).to_mesh()
+
+    class P(fe.Problem):
+        def get_tensor_map(self):
+            return fe.materials.IsotropicLinearElastic(nu=0.3)
+
+    prob = P(mesh, vec=3, dim=3)
+    bc = fe.DirichletBCConfig([fe.DirichletBCSpec(lambda p: p[0] < 0.5, "all", 0.0)]).create_bc(prob)
+    ts = fe.TracedStructure.from_problem(prob)
+    E = torch.ones(prob.num_cells, dtype=torch.float64, device="cuda")
+    tp = fe.TracedParams(volume_vars=(E,))
+    A, _ = assemble_csr(prob, ts, tp, None, bc, True, torch.float32, diff_form=True, jacobi_shift=1e-12)
+    A64 = assemble_csr(prob, ts, tp, None, bc, True, torch.float64)
+    up = A.upper_storage()
+    assert up is not None
+    sym, vals = up
+    assert sym.struct.max_unique > 512
+    rng = np.random.default_rng(5)
+    x = torch.as_tensor(1.0 + 1e-2 * rng.standard_normal(ts.n), device="cuda")
+    y_full = A.matvec(x)
+    y_sym = torch.empty_like(y_full)
+    nat.call("fx_csr_spmv_b3sym_f32f64", nat.stream(), ts.n, nat.ptr(A.indptr), ctypes.addressof(sym.struct),
+             nat.ptr(vals), nat.ptr(x), nat.ptr(y_sym))
+    import scipy.sparse as sp
+    K = sp.csr_matrix((_np(A64.data), _np(A64.indices), _np(A64.indptr)), shape=(ts.n, ts.n))
+    scale = abs(K) @ np.abs(_np(x))
+    err = np.abs(_np(y_sym) - _np(y_full))
+    print("max err / (|K||x| eps)", (err / (scale * EPS64)).max(), "max_unique", sym.struct.max_unique)
+    assert np.all(err <= 64 * EPS64 * scale)
+
+
 def test_sym_storage_refused_for_row_only_elimination():
     import torch
     from feax_amd import _native as nat
