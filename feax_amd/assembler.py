@@ -378,7 +378,7 @@ def upper_assembly_ok(ts, mat, symmetric, ws_bytes):
 
 
 def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, dtype=None, diff_form=False,
-                 jacobi_shift=None, upper=False):
+                 jacobi_shift=None, upper=False, compute32=False):
     """BC-eliminated CSR Jacobian on the device (fx_hex8_assemble_* / fx_tet4_assemble_* /
     fx_quad4_assemble_*).
 
@@ -387,8 +387,10 @@ def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, 
     (form "sym3": the operator of the difference form from about half the
     values; used when upper_assembly_ok, else the full difference form is
     returned).  jacobi_shift: also return dinv = 1 / (diag + shift) (FP64)
-    computed in the same pass.  Returns the CSRMatrix, or (CSRMatrix, dinv)
-    when jacobi_shift is given.
+    computed in the same pass.  compute32 (HEX8, FP32 values only,
+    FX_ASM_COMPUTE32): element matrices formed in FP32 arithmetic as well
+    (by default they are formed in FP64 and rounded once at the store).
+    Returns the CSRMatrix, or (CSRMatrix, dinv) when jacobi_shift is given.
     """
     import torch
     dtype = torch.float64 if dtype is None else dtype
@@ -398,6 +400,10 @@ def assemble_csr(problem, ts, traced_params, sol=None, bc=None, symmetric=True, 
     dinv = torch.empty(ts.n, dtype=torch.float64, device=ts.device) if jacobi_shift is not None else None
     name = f"fx_{ts.tag}_assemble_f64" if dtype == torch.float64 else f"fx_{ts.tag}_assemble_f32"
     flags = nat.FX_ASM_DIFF3 if diff_form else 0
+    if compute32:
+        if ts.tag != "hex8" or dtype != torch.float32:
+            raise ValueError("compute32 is the HEX8 FP32 assembly's option")
+        flags |= nat.FX_ASM_COMPUTE32
     esize = torch.empty(0, dtype=dtype).element_size()
     wsb = getattr(nat.lib(), f"fx_{ts.tag}_assemble_workspace_bytes")(ts.num_cells, esize, flags)
     wp, wn = elem_workspace(ts, wsb)

@@ -124,6 +124,34 @@ def test_sym_spmv_with_more_than_512_columns_per_workgroup():
     assert np.all(err <= 64 * EPS64 * scale)
 
 
+def test_fp32_arithmetic_assembly_matches_the_fp64_formed_one():
+    """FX_ASM_COMPUTE32: element matrices formed in FP32 (both passes of the
+    two-pass assembly in their float instantiations) agree with the FP64-formed
+    FP32 matrix to FP32 rounding of the node's largest value, plain CSR and
+    difference form, and are bitwise repeatable."""
+    import torch
+    from feax_amd.assembler import assemble_csr
+    prob, bc, ts, tp = _distorted_problem()
+    print("case: hex8 assembly with FP32 arithmetic", flush=True)
+    for diff in (False, True):
+        A, d = assemble_csr(prob, ts, tp, None, bc, True, torch.float32, diff_form=diff, jacobi_shift=1e-12)
+        B, e = assemble_csr(prob, ts, tp, None, bc, True, torch.float32, diff_form=diff, jacobi_shift=1e-12,
+                            compute32=True)
+        B2, _ = assemble_csr(prob, ts, tp, None, bc, True, torch.float32, diff_form=diff, jacobi_shift=1e-12,
+                             compute32=True)
+        assert torch.equal(B.data.view(torch.int32), B2.data.view(torch.int32))
+        a, b_ = _np(A.data).astype(np.float64), _np(B.data).astype(np.float64)
+        # per node: its three rows' values are contiguous in both layouts
+        seg = _np(A.indptr)[0::3]
+        nodemax = np.repeat(np.maximum.reduceat(np.abs(a), seg[:-1]), np.diff(seg))
+        rel = np.abs(a - b_) / nodemax
+        print("diff form" if diff else "csr", "max |FP32-formed - FP64-formed| / node max", rel.max())
+        assert rel.max() < 2e-5
+        assert np.allclose(_np(e), _np(d), rtol=1e-5, atol=0)
+    with pytest.raises(ValueError):
+        assemble_csr(prob, ts, tp, None, bc, True, torch.float64, compute32=True)
+
+
 def test_sym_storage_refused_for_row_only_elimination():
     import torch
     from feax_amd import _native as nat
